@@ -1,0 +1,182 @@
+"""Writers for Bluebottle input files (flow.config, decomp.config).
+
+The text follows the grammar that src/domain.c:72-916 reads, in its rigid
+order, plus the optional trailing EXTENSIONS block understood by this build
+(fixed_dt, perturb, pp_precision, pp_inner_tol, pp_warmstart).  A file
+without the EXTENSIONS block is a plain reference input file.
+"""
+import math
+import os
+
+FACES = "WESNBT"
+
+
+def _fmt(x):
+    if isinstance(x, float):
+        return repr(x)
+    return str(x)
+
+
+def flow_config(*, n=(32, 32, 32), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                rho_f=1.0, nu=0.01, duration=1.0, CFL=0.5,
+                pp_max_iter=10000, pp_residual=1e-8,
+                lamb_max_iter=50, lamb_residual=1e-2, lamb_relax=1.0,
+                lamb_cut=1e-4, p_bc=None, vel_bc=None,
+                init_cond="QUIESCENT", out_plane="HOMOGENEOUS",
+                gradP=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)),
+                g=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)),
+                p_bc_tdelay=0.0, g_bc_tdelay=0.0, v_bc_tdelay=0.0,
+                osci_f=0.0, extensions=None):
+    """Return flow.config text.
+
+    p_bc:   dict face -> ("PERIODIC",) or ("NEUMANN", 0.0)
+    vel_bc: dict comp ('u','v','w') -> dict face -> ("PERIODIC",) |
+            ("DIRICHLET", Dm, Da) | ("NEUMANN",)
+    extensions: dict of optional keys (fixed_dt, perturb=(seed, amp),
+            pp_precision, pp_inner_tol, pp_warmstart).
+    """
+    p_bc = p_bc or {f: ("NEUMANN", 0.0) for f in FACES}
+    vel_bc = vel_bc or {c: {f: ("DIRICHLET", 0.0, 0.0) for f in FACES}
+                        for c in "uvw"}
+    L = []
+    L.append("GLOBAL DOMAIN")
+    for ax, a in zip("XYZ", range(3)):
+        L.append(f"({ax}s, {ax}e, {ax}n) {_fmt(float(lo[a]))} "
+                 f"{_fmt(float(hi[a]))} {int(n[a])}")
+    L.append("")
+    L.append("MPI/GPU SUBDOMAIN DECOMPOSITION")
+    L.append("(In, Jn, Kn) 1 1 1")
+    L.append("")
+    L.append("PHYSICAL PARAMETERS")
+    L.append(f"rho_f {_fmt(float(rho_f))}")
+    L.append(f"nu {_fmt(float(nu))}")
+    for k in ("s_D", "s_k", "s_beta", "s_ref"):
+        L.append(f"{k} 0")
+    L.append("")
+    L.append("SIMULATION PARAMETERS")
+    L.append(f"duration {_fmt(float(duration))}")
+    L.append(f"CFL {_fmt(float(CFL))}")
+    L.append("SCALAR 0")
+    L.append(f"pp_max_iter {int(pp_max_iter)}")
+    L.append(f"pp_residual {_fmt(float(pp_residual))}")
+    L.append(f"lamb_max_iter {int(lamb_max_iter)}")
+    L.append(f"lamb_residual {_fmt(float(lamb_residual))}")
+    L.append(f"lamb_relax {_fmt(float(lamb_relax))}")
+    L.append(f"lamb_cut {_fmt(float(lamb_cut))}")
+    L.append(f"s_lamb_cut {_fmt(float(lamb_cut))}")
+    L.append("")
+    L.append("BOUNDARY CONDITIONS")
+    L.append(f"v_bc_tdelay {_fmt(float(v_bc_tdelay))}")
+    L.append("PRESSURE")
+    for f in FACES:
+        t = p_bc[f]
+        L.append(f"bc.p{f} " + " ".join(_fmt(x) for x in t))
+    for c, head in zip("uvw", ("X-VELOCITY", "Y-VELOCITY", "Z-VELOCITY")):
+        L.append(head)
+        for f in FACES:
+            t = vel_bc[c][f]
+            L.append(f"bc.{c}{f} " + " ".join(_fmt(x) for x in t))
+    L.append("SCALAR")
+    for f in FACES:
+        if p_bc[f][0] == "PERIODIC":
+            L.append(f"bc_s.s{f} PERIODIC")
+        else:
+            L.append(f"bc_s.s{f} NEUMANN 0")
+    L.append("")
+    L.append("INITIAL CONDITION")
+    L.append(f"init_cond {init_cond}")
+    L.append("init_s 0 0")
+    L.append("")
+    L.append("SOLVABILITY ENFORCEMENT PLANE")
+    L.append(f"out_plane {out_plane}")
+    L.append("")
+    L.append("SIMULATION DRIVING CONDITIONS")
+    L.append(f"p_bc_tdelay {_fmt(float(p_bc_tdelay))}")
+    for ax, (m, a) in zip("xyz", gradP):
+        L.append(f"gradP.{ax} {_fmt(float(m))} {_fmt(float(a))}")
+    L.append(f"g_bc_tdelay {_fmt(float(g_bc_tdelay))}")
+    for ax, (m, a) in zip("xyz", g):
+        L.append(f"g.{ax} {_fmt(float(m))} {_fmt(float(a))}")
+    L.append("")
+    L.append("PID CONTROLLER GAINS")
+    L.append("Kp 0")
+    L.append("Ki 0")
+    L.append("Kd 0")
+    L.append("")
+    L.append("TURBULENT FORCING PARAMETERS")
+    L.append("turbA 0")
+    L.append(f"osci_f {_fmt(float(osci_f))}")
+    if extensions:
+        L.append("")
+        L.append("EXTENSIONS")
+        for k, v in extensions.items():
+            if isinstance(v, (tuple, list)):
+                L.append(f"{k} " + " ".join(_fmt(x) for x in v))
+            else:
+                L.append(f"{k} {_fmt(v)}")
+    return "\n".join(L) + "\n"
+
+
+def decomp_config(n, lo, hi):
+    L = ["(I, J, K) 0 0 0"]
+    for ax, a in zip("XYZ", range(3)):
+        L.append(f"({ax}s, {ax}e, {ax}n) {_fmt(float(lo[a]))} "
+                 f"{_fmt(float(hi[a]))} {int(n[a])}")
+    return "\n".join(L) + "\n"
+
+
+def write_case(dirpath, **kw):
+    """Write input/flow.config and input/decomp.config under dirpath."""
+    os.makedirs(os.path.join(dirpath, "input"), exist_ok=True)
+    n = kw.get("n", (32, 32, 32))
+    lo = kw.get("lo", (0.0, 0.0, 0.0))
+    hi = kw.get("hi", (1.0, 1.0, 1.0))
+    with open(os.path.join(dirpath, "input", "flow.config"), "w") as f:
+        f.write(flow_config(**kw))
+    with open(os.path.join(dirpath, "input", "decomp.config"), "w") as f:
+        f.write(decomp_config(n, lo, hi))
+    return dirpath
+
+
+def walls(lid=None):
+    """All-wall velocity BCs (no slip); lid=(comp, face, value) moves one."""
+    bc = {c: {f: ("DIRICHLET", 0.0, 0.0) for f in FACES} for c in "uvw"}
+    if lid is not None:
+        c, f, val = lid
+        bc[c][f] = ("DIRICHLET", float(val), 0.0)
+    return bc
+
+
+def cavity(n=32, steps=100, dt=1e-3, **over):
+    """Lid-driven cavity: unit cube, no-slip walls, top lid u = 1, Re = 100,
+    fixed dt, Jacobi-PCG to 1e-8, zero initial fields."""
+    kw = dict(n=(n, n, n), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+              rho_f=1.0, nu=0.01, duration=steps * dt - 0.5 * dt,
+              pp_max_iter=100000, pp_residual=1e-8,
+              p_bc={f: ("NEUMANN", 0.0) for f in FACES},
+              vel_bc=walls(lid=("u", "T", 1.0)),
+              init_cond="QUIESCENT", out_plane="HOMOGENEOUS",
+              extensions={"fixed_dt": dt})
+    kw.update(over)
+    return kw
+
+
+def channel(n=(256, 128, 128), seed=20260101, amp=0.1, precision="PLAIN",
+            **over):
+    """Periodic channel at Re_tau = 180: periodic in x and y, no-slip walls
+    in z at -1 and 1, constant mean pressure gradient -1, laminar profile plus
+    a seeded uniform(-amp, amp) perturbation."""
+    per = ("PERIODIC",)
+    vel = {c: {"W": per, "E": per, "S": per, "N": per,
+               "B": ("DIRICHLET", 0.0, 0.0), "T": ("DIRICHLET", 0.0, 0.0)}
+           for c in "uvw"}
+    pb = {"W": per, "E": per, "S": per, "N": per,
+          "B": ("NEUMANN", 0.0), "T": ("NEUMANN", 0.0)}
+    ext = {"perturb": (seed, amp), "pp_precision": precision}
+    kw = dict(n=n, lo=(0.0, 0.0, -1.0), hi=(4 * math.pi, 2 * math.pi, 1.0),
+              rho_f=1.0, nu=1.0 / 180.0, duration=1e9, CFL=0.5,
+              pp_max_iter=100000, pp_residual=1e-8, p_bc=pb, vel_bc=vel,
+              init_cond="CHANNEL", out_plane="HOMOGENEOUS",
+              gradP=((-1.0, 0.0), (0.0, 0.0), (0.0, 0.0)), extensions=ext)
+    kw.update(over)
+    return kw

@@ -1,0 +1,351 @@
+/*
+ * Device selection, allocation and host <-> device transfer.
+ *
+ * Follows src/cuda_bluebottle.cu:44-509: pinned host arrays in the reference
+ * layouts, device arrays x-fastest (see bb_common.h).  cuda_dom_push and
+ * cuda_dom_pull convert between the two layouts on the host.  The device
+ * forcing is a per-component constant (src/cuda_bluebottle.cu:1811-1982
+ * only ever adds constants in this path), so no device f_x/f_y/f_z arrays
+ * exist; the host arrays are filled with the constant on pull.
+ */
+#include <string.h>
+
+#include "bb_common.h"
+
+DevDom ddom;
+hipStream_t bb_stream = 0;
+double *_bb_rows = NULL;
+double *_bb_rows2 = NULL;
+double *_bb_scalars = NULL;
+double *bb_host_scalars = NULL;
+
+/* Forcing constants, set by cuda_compute_forcing. */
+double bb_force[3] = {0., 0., 0.};
+extern int bb_history_rotated;
+
+#define NSCALARS 64
+
+extern "C" int cuda_device_count(void)
+{
+  int n = 0;
+  BB_CHECK(hipGetDeviceCount(&n));
+  return n;
+}
+
+extern "C" void cuda_device_init(int device)
+{
+  BB_CHECK(hipSetDevice(device));
+}
+
+extern "C" void cuda_block(void)
+{
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+}
+
+extern "C" void cuda_check_errors(int line)
+{
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "HIP error at line %d: %s\n", line, hipGetErrorString(e));
+    exit(EXIT_FAILURE);
+  }
+}
+
+extern "C" void cuda_blocks_init(void)
+{
+  /* Launch shapes are computed per call from ddom; nothing to cache. */
+}
+
+extern "C" void cuda_update_bc(void)
+{
+  /* The BC values travel to the kernels as arguments on every launch
+   * (src/cuda_bluebottle.cu:435-440 keeps a device copy for the particle
+   * kernels only). */
+}
+
+static Grid make_grid(const grid_info &G)
+{
+  Grid g;
+  g.ni = G.in; g.nj = G.jn; g.nk = G.kn;
+  g.sy = G.inb;
+  g.sz = G.inb * G.jnb;
+  g.n = (long) G.inb * G.jnb * G.knb;
+  return g;
+}
+
+static void *host_alloc(size_t bytes)
+{
+  void *ptr = NULL;
+  BB_CHECK(hipHostMalloc(&ptr, bytes, hipHostMallocDefault));
+  memset(ptr, 0, bytes);
+  cpumem += bytes;
+  return ptr;
+}
+
+extern "C" void cuda_dom_malloc_host(void)
+{
+  /* src/cuda_bluebottle.cu:76-141 */
+  dom_struct *d = &dom[0];
+  size_t cc = d->Gcc.s3b * sizeof(real);
+  size_t fx = d->Gfx.s3b * sizeof(real);
+  size_t fy = d->Gfy.s3b * sizeof(real);
+  size_t fz = d->Gfz.s3b * sizeof(real);
+  p = (real *) host_alloc(cc); p0 = (real *) host_alloc(cc);
+  phi = (real *) host_alloc(cc);
+  u = (real *) host_alloc(fx); u0 = (real *) host_alloc(fx);
+  v = (real *) host_alloc(fy); v0 = (real *) host_alloc(fy);
+  w = (real *) host_alloc(fz); w0 = (real *) host_alloc(fz);
+  u_star = (real *) host_alloc(fx); v_star = (real *) host_alloc(fy);
+  w_star = (real *) host_alloc(fz);
+  f_x = (real *) host_alloc(fx); f_y = (real *) host_alloc(fy);
+  f_z = (real *) host_alloc(fz);
+  conv_u = (real *) host_alloc(fx); conv_v = (real *) host_alloc(fy);
+  conv_w = (real *) host_alloc(fz);
+  conv0_u = (real *) host_alloc(fx); conv0_v = (real *) host_alloc(fy);
+  conv0_w = (real *) host_alloc(fz);
+  diff_u = (real *) host_alloc(fx); diff_v = (real *) host_alloc(fy);
+  diff_w = (real *) host_alloc(fz);
+  diff0_u = (real *) host_alloc(fx); diff0_v = (real *) host_alloc(fy);
+  diff0_w = (real *) host_alloc(fz);
+  flag_u = (int *) host_alloc(d->Gfx.s3b * sizeof(int));
+  flag_v = (int *) host_alloc(d->Gfy.s3b * sizeof(int));
+  flag_w = (int *) host_alloc(d->Gfz.s3b * sizeof(int));
+  phase = (int *) host_alloc(d->Gcc.s3b * sizeof(int));
+  phase_shell = (int *) host_alloc(d->Gcc.s3b * sizeof(int));
+}
+
+static void *dev_alloc(size_t bytes)
+{
+  void *ptr = NULL;
+  BB_CHECK(hipMalloc(&ptr, bytes));
+  BB_CHECK(hipMemset(ptr, 0, bytes));
+  gpumem += bytes;
+  return ptr;
+}
+
+extern "C" void cuda_dom_malloc_dev(void)
+{
+  /* src/cuda_bluebottle.cu:144-432 */
+  dom_struct *d = &dom[0];
+  bb_history_rotated = 0;
+  bb_force[0] = bb_force[1] = bb_force[2] = 0.;
+  ddom.cc = make_grid(d->Gcc);
+  ddom.fx = make_grid(d->Gfx);
+  ddom.fy = make_grid(d->Gfy);
+  ddom.fz = make_grid(d->Gfz);
+  ddom.nx = d->xn; ddom.ny = d->yn; ddom.nz = d->zn;
+  ddom.dx = d->dx; ddom.dy = d->dy; ddom.dz = d->dz;
+  ddom.perx = (bc.pW == PERIODIC);
+  ddom.pery = (bc.pS == PERIODIC);
+  ddom.perz = (bc.pB == PERIODIC);
+
+  size_t cc = ddom.cc.n * sizeof(real);
+  size_t fx = ddom.fx.n * sizeof(real);
+  size_t fy = ddom.fy.n * sizeof(real);
+  size_t fz = ddom.fz.n * sizeof(real);
+  _p = (real *) dev_alloc(cc); _p0 = (real *) dev_alloc(cc);
+  _phi = (real *) dev_alloc(cc); _invM = (real *) dev_alloc(cc);
+  _rhs_p = (real *) dev_alloc(cc); _r_q = (real *) dev_alloc(cc);
+  _z_q = (real *) dev_alloc(cc); _p_q = (real *) dev_alloc(cc);
+  _pb_q = _p_q;                       /* one ghosted search direction */
+  _Apb_q = (real *) dev_alloc(cc);
+  _u = (real *) dev_alloc(fx); _u0 = (real *) dev_alloc(fx);
+  _v = (real *) dev_alloc(fy); _v0 = (real *) dev_alloc(fy);
+  _w = (real *) dev_alloc(fz); _w0 = (real *) dev_alloc(fz);
+  _u_star = (real *) dev_alloc(fx); _v_star = (real *) dev_alloc(fy);
+  _w_star = (real *) dev_alloc(fz);
+  _conv_u = (real *) dev_alloc(fx); _conv_v = (real *) dev_alloc(fy);
+  _conv_w = (real *) dev_alloc(fz);
+  _conv0_u = (real *) dev_alloc(fx); _conv0_v = (real *) dev_alloc(fy);
+  _conv0_w = (real *) dev_alloc(fz);
+  _diff_u = (real *) dev_alloc(fx); _diff_v = (real *) dev_alloc(fy);
+  _diff_w = (real *) dev_alloc(fz);
+  _diff0_u = (real *) dev_alloc(fx); _diff0_v = (real *) dev_alloc(fy);
+  _diff0_w = (real *) dev_alloc(fz);
+  _flag_u = (int *) dev_alloc(ddom.fx.n * sizeof(int));
+  _flag_v = (int *) dev_alloc(ddom.fy.n * sizeof(int));
+  _flag_w = (int *) dev_alloc(ddom.fz.n * sizeof(int));
+  _phase = (int *) dev_alloc(ddom.cc.n * sizeof(int));
+  _phase_shell = (int *) dev_alloc(ddom.cc.n * sizeof(int));
+
+  long rows = (long) ddom.ny * ddom.nz;
+  long faces = (long) (ddom.nx + 1) * (ddom.ny + 1);
+  long cap = rows > faces ? rows : faces;
+  faces = (long) (ddom.ny + 1) * (ddom.nz + 1);
+  cap = cap > faces ? cap : faces;
+  faces = (long) (ddom.nx + 1) * (ddom.nz + 1);
+  cap = cap > faces ? cap : faces;
+  _bb_rows = (double *) dev_alloc(cap * sizeof(double));
+  _bb_rows2 = (double *) dev_alloc(cap * sizeof(double));
+  _bb_scalars = (double *) dev_alloc(NSCALARS * sizeof(double));
+  BB_CHECK(hipHostMalloc((void **) &bb_host_scalars,
+    NSCALARS * sizeof(double), hipHostMallocDefault));
+  /* All fluid until particle cages are built (src/particle.c:556-598). */
+  bb_phase_fluid();
+  bb_flags_reset();
+}
+
+/* ---- layout conversion between reference host order and device order ---- */
+enum { L_GCC = 0, L_GFX, L_GFY, L_GFZ };
+
+static long host_loc(const grid_info &G, int kind, int i, int j, int k)
+{
+  switch (kind) {
+    case L_GFX: return GFX_LOC(i, j, k, G.s1b, G.s2b);
+    case L_GFY: return GFY_LOC(i, j, k, G.s1b, G.s2b);
+    case L_GFZ: return GFZ_LOC(i, j, k, G.s1b, G.s2b);
+    default:    return GCC_LOC(i, j, k, G.s1b, G.s2b);
+  }
+}
+
+static const grid_info &ginfo(int kind)
+{
+  switch (kind) {
+    case L_GFX: return dom[0].Gfx;
+    case L_GFY: return dom[0].Gfy;
+    case L_GFZ: return dom[0].Gfz;
+    default:    return dom[0].Gcc;
+  }
+}
+
+template <typename T>
+static void push_field(const T *host, T *dev, int kind)
+{
+  const grid_info &G = ginfo(kind);
+  size_t n = (size_t) G.s3b;
+  T *stage = (T *) malloc(n * sizeof(T));
+  long d = 0;
+  for (int k = 0; k < G.knb; k++)
+    for (int j = 0; j < G.jnb; j++)
+      for (int i = 0; i < G.inb; i++)
+        stage[d++] = host[host_loc(G, kind, i, j, k)];
+  BB_CHECK(hipMemcpy(dev, stage, n * sizeof(T), hipMemcpyHostToDevice));
+  free(stage);
+}
+
+template <typename T>
+static void pull_field(T *host, const T *dev, int kind)
+{
+  const grid_info &G = ginfo(kind);
+  size_t n = (size_t) G.s3b;
+  T *stage = (T *) malloc(n * sizeof(T));
+  BB_CHECK(hipMemcpy(stage, dev, n * sizeof(T), hipMemcpyDeviceToHost));
+  long d = 0;
+  for (int k = 0; k < G.knb; k++)
+    for (int j = 0; j < G.jnb; j++)
+      for (int i = 0; i < G.inb; i++)
+        host[host_loc(G, kind, i, j, k)] = stage[d++];
+  free(stage);
+}
+
+extern "C" void cuda_dom_push(void)
+{
+  /* src/cuda_bluebottle.cu:443-490 */
+  BB_CHECK(hipDeviceSynchronize());
+  push_field(p, _p, L_GCC); push_field(p0, _p0, L_GCC);
+  push_field(phi, _phi, L_GCC);
+  push_field(u, _u, L_GFX); push_field(u0, _u0, L_GFX);
+  push_field(v, _v, L_GFY); push_field(v0, _v0, L_GFY);
+  push_field(w, _w, L_GFZ); push_field(w0, _w0, L_GFZ);
+  push_field(u_star, _u_star, L_GFX); push_field(v_star, _v_star, L_GFY);
+  push_field(w_star, _w_star, L_GFZ);
+  push_field(conv_u, _conv_u, L_GFX); push_field(conv_v, _conv_v, L_GFY);
+  push_field(conv_w, _conv_w, L_GFZ);
+  push_field(conv0_u, _conv0_u, L_GFX); push_field(conv0_v, _conv0_v, L_GFY);
+  push_field(conv0_w, _conv0_w, L_GFZ);
+  push_field(diff_u, _diff_u, L_GFX); push_field(diff_v, _diff_v, L_GFY);
+  push_field(diff_w, _diff_w, L_GFZ);
+  push_field(diff0_u, _diff0_u, L_GFX); push_field(diff0_v, _diff0_v, L_GFY);
+  push_field(diff0_w, _diff0_w, L_GFZ);
+}
+
+extern "C" void cuda_dom_pull(void)
+{
+  /* src/cuda_bluebottle.cu:492-509 */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(p, _p, L_GCC); pull_field(p0, _p0, L_GCC);
+  pull_field(phi, _phi, L_GCC);
+  pull_field(u, _u, L_GFX); pull_field(u0, _u0, L_GFX);
+  pull_field(v, _v, L_GFY); pull_field(v0, _v0, L_GFY);
+  pull_field(w, _w, L_GFZ); pull_field(w0, _w0, L_GFZ);
+  pull_field(u_star, _u_star, L_GFX); pull_field(v_star, _v_star, L_GFY);
+  pull_field(w_star, _w_star, L_GFZ);
+  /* After cuda_store_u the conv/diff histories are rotated by pointer: the
+   * reference's conv == conv0 at that point, so both come from conv0. */
+  pull_field(conv_u, bb_history_rotated ? _conv0_u : _conv_u, L_GFX);
+  pull_field(conv_v, bb_history_rotated ? _conv0_v : _conv_v, L_GFY);
+  pull_field(conv_w, bb_history_rotated ? _conv0_w : _conv_w, L_GFZ);
+  pull_field(conv0_u, _conv0_u, L_GFX); pull_field(conv0_v, _conv0_v, L_GFY);
+  pull_field(conv0_w, _conv0_w, L_GFZ);
+  pull_field(diff_u, bb_history_rotated ? _diff0_u : _diff_u, L_GFX);
+  pull_field(diff_v, bb_history_rotated ? _diff0_v : _diff_v, L_GFY);
+  pull_field(diff_w, bb_history_rotated ? _diff0_w : _diff_w, L_GFZ);
+  pull_field(diff0_u, _diff0_u, L_GFX); pull_field(diff0_v, _diff0_v, L_GFY);
+  pull_field(diff0_w, _diff0_w, L_GFZ);
+  pull_field(flag_u, _flag_u, L_GFX); pull_field(flag_v, _flag_v, L_GFY);
+  pull_field(flag_w, _flag_w, L_GFZ);
+  pull_field(phase, _phase, L_GCC);
+  pull_field(phase_shell, _phase_shell, L_GCC);
+  dom_struct *d = &dom[0];
+  for (int i = 0; i < d->Gfx.s3b; i++) f_x[i] = bb_force[0];
+  for (int i = 0; i < d->Gfy.s3b; i++) f_y[i] = bb_force[1];
+  for (int i = 0; i < d->Gfz.s3b; i++) f_z[i] = bb_force[2];
+}
+
+/* Pull one Gcc device array (x-fastest, ghosted) into a host buffer of the
+ * same order; used by tests and the solver statistics. */
+extern "C" void bb_pull_raw(const void *dev, void *host, size_t bytes)
+{
+  BB_CHECK(hipDeviceSynchronize());
+  BB_CHECK(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+}
+
+extern "C" void bb_push_raw(void *dev, const void *host, size_t bytes)
+{
+  BB_CHECK(hipDeviceSynchronize());
+  BB_CHECK(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+}
+
+static void dfree(void *ptr)
+{
+  if (ptr) BB_CHECK(hipFree(ptr));
+}
+
+static void hfree(void *ptr)
+{
+  if (ptr) BB_CHECK(hipHostFree(ptr));
+}
+
+extern "C" void cuda_dom_free(void)
+{
+  /* src/cuda_bluebottle.cu:2619 */
+  BB_CHECK(hipDeviceSynchronize());
+  real *dd[] = {_p, _p0, _phi, _invM, _rhs_p, _r_q, _z_q, _p_q, _Apb_q,
+    _u, _u0, _v, _v0, _w, _w0, _u_star, _v_star, _w_star,
+    _conv_u, _conv_v, _conv_w, _conv0_u, _conv0_v, _conv0_w,
+    _diff_u, _diff_v, _diff_w, _diff0_u, _diff0_v, _diff0_w};
+  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) dfree(dd[i]);
+  dfree(_flag_u); dfree(_flag_v); dfree(_flag_w);
+  dfree(_phase); dfree(_phase_shell);
+  dfree(_bb_rows); dfree(_bb_rows2); dfree(_bb_scalars);
+  hfree(bb_host_scalars);
+  _p = _p0 = _phi = _invM = _rhs_p = _r_q = _z_q = _p_q = _pb_q = _Apb_q = NULL;
+  _u = _u0 = _v = _v0 = _w = _w0 = _u_star = _v_star = _w_star = NULL;
+  _conv_u = _conv_v = _conv_w = _conv0_u = _conv0_v = _conv0_w = NULL;
+  _diff_u = _diff_v = _diff_w = _diff0_u = _diff0_v = _diff0_w = NULL;
+  _flag_u = _flag_v = _flag_w = _phase = _phase_shell = NULL;
+  _bb_rows = _bb_rows2 = _bb_scalars = bb_host_scalars = NULL;
+
+  real *hh[] = {p, p0, phi, u, u0, v, v0, w, w0, u_star, v_star, w_star,
+    f_x, f_y, f_z, conv_u, conv_v, conv_w, conv0_u, conv0_v, conv0_w,
+    diff_u, diff_v, diff_w, diff0_u, diff0_v, diff0_w};
+  for (size_t i = 0; i < sizeof(hh) / sizeof(hh[0]); i++) hfree(hh[i]);
+  hfree(flag_u); hfree(flag_v); hfree(flag_w); hfree(phase);
+  hfree(phase_shell);
+  p = p0 = phi = u = u0 = v = v0 = w = w0 = NULL;
+  u_star = v_star = w_star = f_x = f_y = f_z = NULL;
+  conv_u = conv_v = conv_w = conv0_u = conv0_v = conv0_w = NULL;
+  diff_u = diff_v = diff_w = diff0_u = diff0_v = diff0_w = NULL;
+  flag_u = flag_v = flag_w = phase = phase_shell = NULL;
+}

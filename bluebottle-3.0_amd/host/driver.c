@@ -1,0 +1,204 @@
+/*
+ * The timestep driver: src/bluebottle.c:28-436 restated for one GPU holding
+ * the whole domain.  The MPI halo exchanges become cuda_exchange (periodic
+ * planes on the device); the call order of the reference is kept.
+ *
+ * Lamb iterations: with particles the loop runs until the coefficient change
+ * falls below lamb_residual (src/bluebottle.c:200-276).  Without particles
+ * the reference's loop count is undefined (cuda_lamb_err leaves its error
+ * uninitialised, src/cuda_physalis.cu:864,899-902) and every extra pass
+ * recomputes the same fields, so exactly one pass is made.
+ */
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bluebottle.h"
+
+extern int use_restart;
+extern int recorder_enabled;
+int bb_verbose = 1;
+
+/* Particle hooks, provided by the particle/Physalis module when linked. */
+void parts_read_input(void) __attribute__((weak));
+void bb_parts_setup(void) __attribute__((weak));
+void bb_parts_teardown(void) __attribute__((weak));
+void cuda_part_BC(void) __attribute__((weak));
+void cuda_part_BC_star(void) __attribute__((weak));
+void cuda_part_p_fill(void) __attribute__((weak));
+void cuda_move_parts_sub(void) __attribute__((weak));
+void cuda_lamb(void) __attribute__((weak));
+void cuda_lamb_err(real *error, int *num) __attribute__((weak));
+void cuda_update_part_velocity(void) __attribute__((weak));
+void cuda_update_part_position(void) __attribute__((weak));
+void cuda_parts_internal(void) __attribute__((weak));
+
+/* Learned warm start hooks. */
+void bb_warm_setup(void) __attribute__((weak));
+void bb_warm_teardown(void) __attribute__((weak));
+
+static void exchange_vel(real *a, real *b, real *c)
+{
+  cuda_exchange(a, GFX);
+  cuda_exchange(b, GFY);
+  cuda_exchange(c, GFZ);
+}
+
+static void pressure_solve(void)
+{
+  if (pp_precision == PP_MIXED) cuda_PP_cg_mixed();
+  else if (NPARTS > 0) cuda_PP_cg();
+  else cuda_PP_cg_noparts();
+}
+
+int bb_lamb_iters = 0;
+real bb_lamb_err = 0.;
+
+int bluebottle_setup(void)
+{
+  /* src/bluebottle.c:31-142 */
+  domain_read_input();
+  domain_fill();
+  cuda_dom_malloc_host();
+  cuda_dom_malloc_dev();
+  cuda_blocks_init();
+  domain_init_fields();
+  NPARTS = 0;
+  nparts = 0;
+  if (parts_read_input) parts_read_input();
+  if (!use_restart) recorder_PP_init("solver_expd.rec");
+  if (bb_parts_setup) bb_parts_setup();
+  cuda_dom_push();
+  cuda_exchange(_p, GCC);
+  exchange_vel(_u, _v, _w);
+  cuda_build_cages();
+  cuda_find_dt();
+  cuda_dom_BC();
+  cuda_exchange(_p, GCC);
+  exchange_vel(_u, _v, _w);
+  cuda_PP_init_jacobi_preconditioner();
+  if (bb_warm_setup) bb_warm_setup();
+  return 0;
+}
+
+int bluebottle_step(void)
+{
+  /* One pass of src/bluebottle.c:168-407. */
+  ttime += dt;
+  stepnum++;
+  int iter = 0;
+  real iter_err = FLT_MAX;
+  int iter_num = -1;
+  if (rank == 0 && bb_verbose) {
+    printf("\nEXPD: Time = %e of %e (dt = %e) (stepnum = %d).\n", ttime,
+      duration, dt, stepnum);
+    fflush(stdout);
+  }
+  cuda_compute_forcing();
+  compute_vel_BC();
+
+  while (iter_err > lamb_residual) {
+    cuda_U_star();
+    exchange_vel(_u_star, _v_star, _w_star);
+    if (nparts > 0) cuda_part_BC_star();
+    cuda_dom_BC_star();
+    exchange_vel(_u_star, _v_star, _w_star);
+
+    cuda_solvability();
+    if (nparts > 0) cuda_part_BC_star();
+    cuda_dom_BC_star();
+    exchange_vel(_u_star, _v_star, _w_star);
+
+    pressure_solve();
+    cuda_exchange(_phi, GCC);
+    cuda_dom_BC_p(_phi);
+
+    cuda_project();
+    exchange_vel(_u, _v, _w);
+    if (nparts > 0) cuda_part_BC();
+    cuda_dom_BC();
+    exchange_vel(_u, _v, _w);
+
+    cuda_update_p();
+    if (nparts > 0) {
+      cuda_part_BC();
+      cuda_part_p_fill();
+    }
+    cuda_dom_BC_p(_p);
+    cuda_exchange(_p, GCC);
+
+    if (NPARTS > 0) {
+      cuda_move_parts_sub();
+      cuda_lamb();
+      cuda_lamb_err(&iter_err, &iter_num);
+    } else {
+      iter_err = 0.;
+    }
+    iter++;
+    if (iter == lamb_max_iter) break;
+  }
+  bb_lamb_iters = iter;
+  bb_lamb_err = iter_err;
+  if (rank == 0 && bb_verbose && NPARTS > 0) {
+    if (iter < lamb_max_iter)
+      printf("  The Lamb's coefficients converged in %d iterations\n", iter);
+    else
+      printf("  Reached the max number of Lamb's iterations. Continuing...\n");
+  }
+
+  if (NPARTS > 0) {
+    cuda_move_parts_sub();
+    cuda_update_part_velocity();
+    cuda_parts_internal();
+    cuda_dom_BC();
+    exchange_vel(_u, _v, _w);
+  }
+
+  cuda_store_u();
+  dt0 = dt;
+  cuda_find_dt();
+
+  if (NPARTS > 0) {
+    cuda_update_part_position();
+    cuda_build_cages();
+    cuda_PP_init_jacobi_preconditioner();
+  }
+
+  if (dt < 1.e-10 || dt > 1.e10) {
+    printf("N%d >> The solution time step has diverged. Ending simulation.\n",
+      rank);
+    exit(EXIT_FAILURE);
+  }
+  fflush(stdout);
+  return 0;
+}
+
+void bluebottle_teardown(void)
+{
+  cuda_block();
+  if (bb_warm_teardown) bb_warm_teardown();
+  if (bb_parts_teardown) bb_parts_teardown();
+  cuda_dom_free();
+  domain_free();
+}
+
+int bluebottle_main(int argc, char *argv[])
+{
+  parse_cmdline_args(argc, argv);
+  if (rank == 0) printf("N%d >> Running Bluebottle (MI355X)...\n", rank);
+  bluebottle_setup();
+  if (rank == 0) {
+    printf("\n=====BLUEBOTTLE============================");
+    printf("======================================\n");
+    fflush(stdout);
+  }
+  while (ttime <= duration) bluebottle_step();
+  if (rank == 0) {
+    printf("\nEXPD: The simulation has reached its specified duration\n");
+    printf("========================================");
+    printf("========================================\n\n");
+  }
+  bluebottle_teardown();
+  return 0;
+}

@@ -1,0 +1,300 @@
+/*
+ * bluebottle.h -- C ABI of the MI355X per-timestep flow path.
+ *
+ * This header is the drop-in boundary.  It keeps the reference's
+ * global-variable calling convention: the host driver fills the structs and
+ * scalars below, then calls the `void cuda_*(void)` entry points, which read
+ * and write those globals.  Every entry point is `extern "C"` and is
+ * implemented with hand-written HIP kernels for gfx950 in libbluebottle.so.
+ *
+ * Struct layouts are field-for-field those of the reference so that a host
+ * compiled against the reference headers sees the same bytes:
+ *   grid_info     src/domain.h:52-95
+ *   dom_struct    src/domain.h:168-210
+ *   BC            src/bluebottle.h:662-747
+ *   gradP_struct  src/bluebottle.h:875-885
+ *   g_struct      src/bluebottle.h:910-920
+ *   BC_s          src/scalar.h:68-87
+ *
+ * Host arrays (u, v, w, p, ...) use the reference memory layouts
+ * (GFX_LOC is y-fastest, GFY_LOC is z-fastest, GCC_LOC/GFZ_LOC x-fastest,
+ * src/bluebottle.h:70-73).  Device arrays are x-fastest for every grid; the
+ * push/pull entry points convert between the two.
+ */
+#ifndef BLUEBOTTLE_MI355X_H
+#define BLUEBOTTLE_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double real;                 /* src/bluebottle.h:45-47 (-DDOUBLE) */
+
+/* Location macros, reference host layouts (src/bluebottle.h:70-73). */
+#define GCC_LOC(II, JJ, KK, S1, S2) ((II) + (JJ)*(S1) + (KK)*(S2))
+#define GFX_LOC(II, JJ, KK, S1, S2) ((JJ) + (KK)*(S1) + (II)*(S2))
+#define GFY_LOC(II, JJ, KK, S1, S2) ((KK) + (II)*(S1) + (JJ)*(S2))
+#define GFZ_LOC(II, JJ, KK, S1, S2) ((II) + (JJ)*(S1) + (KK)*(S2))
+
+#define PI 3.14159265358979323846
+#define DOM_BUF 1                    /* src/bluebottle.h:141 */
+#define CHAR_BUF_SIZE 256
+#define FILE_NAME_SIZE 256
+#define ROOT_DIR "."
+#define INPUT_DIR "input"
+#define OUTPUT_DIR "output"
+
+/* Boundary condition types (src/bluebottle.h:218-254). */
+#define PERIODIC 0
+#define DIRICHLET 1
+#define NEUMANN 2
+#define PRECURSOR 3
+
+/* Initial conditions (src/bluebottle.h:268-340). */
+#define QUIESCENT 0
+#define SHEAR 1
+#define CHANNEL 2
+#define TAYLOR_GREEN 3
+#define TAYLOR_GREEN_3 4
+#define TURB_CHANNEL 5
+#define TURBULENT 6
+
+/* Solvability plane (src/bluebottle.h:353-425). */
+#define HOMOGENEOUS 10
+#define WEST 0
+#define EAST 1
+#define SOUTH 2
+#define NORTH 3
+#define BOTTOM 4
+#define TOP 5
+
+/* Grid identifiers (src/bluebottle.h:437-473). */
+#define GCC 0
+#define GFX 1
+#define GFY 2
+#define GFZ 3
+
+/* A neighbour that does not exist (the reference uses MPI_PROC_NULL). */
+#define BB_PROC_NULL (-1)
+
+/* Pressure-solver forms selected by the optional `pp_precision` key. */
+#define PP_PLAIN 0                   /* FP64 Jacobi-PCG, the reference form */
+#define PP_MIXED 1                   /* FP32 PCG inside FP64 defect correction */
+
+/* Warm-start selection for the pressure solve (`pp_warmstart` key). */
+#define PP_WARM_OFF 0
+#define PP_WARM_LEARNED 1
+
+typedef struct grid_info {           /* src/domain.h:52-95 */
+  int is, ie, in, isb, ieb, inb;
+  int js, je, jn, jsb, jeb, jnb;
+  int ks, ke, kn, ksb, keb, knb;
+  int _is, _ie, _isb, _ieb;
+  int _js, _je, _jsb, _jeb;
+  int _ks, _ke, _ksb, _keb;
+  int s1, s1b, s2, s2b, s3, s3b;
+  int s2_i, s2_j, s2_k, s2b_i, s2b_j, s2b_k;
+} grid_info;
+
+typedef struct dom_struct {          /* src/domain.h:168-210 */
+  grid_info Gcc;
+  grid_info Gfx;
+  grid_info Gfy;
+  grid_info Gfz;
+  real xs, xe, xl; int xn; real dx;
+  real ys, ye, yl; int yn; real dy;
+  real zs, ze, zl; int zn; real dz;
+  int rank;
+  int e, w, n, s, t, b;
+  int I, Is, Ie, In;
+  int J, Js, Je, Jn;
+  int K, Ks, Ke, Kn;
+  int S1, S2, S3;
+} dom_struct;
+
+typedef struct BC {                  /* src/bluebottle.h:662-747 */
+  int pW, pE, pS, pN, pB, pT;
+  int uW; real uWDm, uWD, uWDa;
+  int uE; real uEDm, uED, uEDa;
+  int uS; real uSDm, uSD, uSDa;
+  int uN; real uNDm, uND, uNDa;
+  int uB; real uBDm, uBD, uBDa;
+  int uT; real uTDm, uTD, uTDa;
+  int vW; real vWDm, vWD, vWDa;
+  int vE; real vEDm, vED, vEDa;
+  int vS; real vSDm, vSD, vSDa;
+  int vN; real vNDm, vND, vNDa;
+  int vB; real vBDm, vBD, vBDa;
+  int vT; real vTDm, vTD, vTDa;
+  int wW; real wWDm, wWD, wWDa;
+  int wE; real wEDm, wED, wEDa;
+  int wS; real wSDm, wSD, wSDa;
+  int wN; real wNDm, wND, wNDa;
+  int wB; real wBDm, wBD, wBDa;
+  int wT; real wTDm, wTD, wTDa;
+  real dsW, dsE, dsS, dsN, dsB, dsT;
+} BC;
+
+typedef struct gradP_struct {        /* src/bluebottle.h:875-885 */
+  real x, xm, xa;
+  real y, ym, ya;
+  real z, zm, za;
+} gradP_struct;
+
+typedef struct g_struct {            /* src/bluebottle.h:910-920 */
+  real x, xm, xa;
+  real y, ym, ya;
+  real z, zm, za;
+} g_struct;
+
+typedef struct BC_s {                /* src/scalar.h:68-87 */
+  int sW; real sWD, sWN;
+  int sE; real sED, sEN;
+  int sN; real sND, sNN;
+  int sS; real sSD, sSN;
+  int sT; real sTD, sTN;
+  int sB; real sBD, sBN;
+} BC_s;
+
+/* ---- Global state (defined in the library; src/bluebottle.c:440-576) ---- */
+extern int rank;                     /* always 0: one GPU holds the domain */
+extern int nprocs;                   /* always 1 */
+extern dom_struct DOM;
+extern dom_struct *dom;
+extern BC bc;
+extern gradP_struct gradP;
+extern g_struct g;
+extern BC_s bc_s;
+
+extern real dt, dt0, CFL, duration, ttime;
+extern int stepnum;
+extern real rho_f, mu, nu;
+extern real pp_residual;
+extern int pp_max_iter;
+extern int lamb_max_iter;
+extern real lamb_residual, lamb_relax, lamb_cut;
+extern real osci_f;
+extern int init_cond;
+extern int out_plane;
+extern real v_bc_tdelay, p_bc_tdelay, g_bc_tdelay;
+extern real pid_int, pid_back, Kp, Ki, Kd;
+extern real turbA, turbl, turb_k0;
+extern long int cpumem, gpumem;
+
+/* Scalar parameters are parsed (src/domain.c:166-206) although scalar
+ * transport itself is not part of this library. */
+extern int SCALAR;
+extern real s_D, s_k, s_beta, s_ref, lamb_cut_scalar, s_init, s_init_rand;
+
+/* Particle counts; zero in the single-phase flow path. */
+extern int NPARTS;
+extern int nparts;
+
+/* Optional trailing config keys (absent from the reference grammar; a file
+ * without them parses exactly as before). */
+extern real fixed_dt;                /* > 0: use this dt every step */
+extern unsigned long long perturb_seed;
+extern real perturb_amp;             /* > 0: seeded uniform(-a, a) on u, v, w */
+extern int pp_precision;             /* PP_PLAIN or PP_MIXED */
+extern int pp_warmstart;             /* PP_WARM_OFF or PP_WARM_LEARNED */
+extern real pp_inner_tol;            /* relative tolerance of a mixed inner solve */
+
+/* Host arrays in reference layouts (src/bluebottle.c:482-490). */
+extern real *p, *u, *v, *w;
+extern real *p0, *u0, *v0, *w0;
+extern real *phi;
+extern real *u_star, *v_star, *w_star;
+extern real *f_x, *f_y, *f_z;
+extern real *conv_u, *conv_v, *conv_w;
+extern real *conv0_u, *conv0_v, *conv0_w;
+extern real *diff_u, *diff_v, *diff_w;
+extern real *diff0_u, *diff0_v, *diff0_w;
+extern int *flag_u, *flag_v, *flag_w;
+extern int *phase, *phase_shell;
+
+/* Device arrays (x-fastest, ghosted; src/bluebottle.c:492-515). */
+extern real *_p, *_u, *_v, *_w;
+extern real *_p0, *_u0, *_v0, *_w0;
+extern real *_phi;
+extern real *_invM;
+extern real *_u_star, *_v_star, *_w_star;
+extern real *_conv_u, *_conv_v, *_conv_w;
+extern real *_conv0_u, *_conv0_v, *_conv0_w;
+extern real *_diff_u, *_diff_v, *_diff_w;
+extern real *_diff0_u, *_diff0_v, *_diff0_w;
+extern real *_rhs_p, *_r_q, *_z_q, *_p_q, *_pb_q, *_Apb_q;
+extern int *_flag_u, *_flag_v, *_flag_w;
+extern int *_phase, *_phase_shell;
+
+/* Per-solve statistics of the last pressure solve (the values the reference
+ * writes to record/solver_expd.rec, src/cuda_solver.cu:236-240). */
+typedef struct pp_stats {
+  int niter;          /* PCG iterations (inner iterations summed for mixed) */
+  int nouter;         /* defect-correction sweeps (1 for the plain form) */
+  real resid;         /* sqrt((r, M^-1 r) / (b, b)) at exit, FP64 */
+  real true_resid;    /* ||b - A phi||_2 / ||b||_2 at exit, FP64 */
+  real sp_rhs;        /* (b, b) */
+  double etime;       /* seconds */
+} pp_stats;
+extern pp_stats pp_last;
+
+/* ---------------- Host C functions (src/domain.h:269-462) ---------------- */
+void parse_cmdline_args(int argc, char *argv[]);   /* src/domain.c:33-70 */
+void domain_read_input(void);                      /* src/domain.c:72-916 */
+void domain_fill(void);                            /* src/domain.c:918-1487 */
+void compute_vel_BC(void);                         /* src/domain.c:1931-2082 */
+void domain_init_fields(void);                     /* src/domain.c:2083-2142 */
+void domain_free(void);
+void rng_init(unsigned long long seed);            /* src/rng.c:30-41 */
+unsigned long long rng_uint64(void);               /* src/rng.c:44-51 */
+double rng_dbl(void);                              /* src/rng.c:78-81 */
+void recorder_PP_init(const char *name);           /* src/recorder.c:157-188 */
+void recorder_PP(const char *name, int niter, real resid, real etime); /* :190-221 */
+
+/* Driver: src/bluebottle.c:28-436 split into its phases so that a caller can
+ * step the solver one timestep at a time.  bluebottle_main() is the whole
+ * program (setup, `while (ttime <= duration)` loop, teardown). */
+int bluebottle_setup(void);          /* :31-142, returns 0 on success */
+int bluebottle_step(void);           /* one pass of :168-407 */
+void bluebottle_teardown(void);      /* :416-430 */
+int bluebottle_main(int argc, char *argv[]);
+
+/* --------------- Device entry points (libbluebottle.so) ------------------ */
+int cuda_device_count(void);                 /* src/bluebottle.h:2604 */
+void cuda_device_init(int device);           /* :2616 */
+void cuda_block(void);                       /* :2642, device synchronise */
+void cuda_dom_malloc_host(void);             /* src/cuda_bluebottle.cu:76-141 */
+void cuda_dom_malloc_dev(void);              /* src/cuda_bluebottle.cu:144-432 */
+void cuda_dom_push(void);                    /* :443-490, host layouts -> device */
+void cuda_dom_pull(void);                    /* :492-509, device -> host layouts */
+void cuda_blocks_init(void);                 /* :511-1006 (launch shapes) */
+void cuda_update_bc(void);                   /* :435-440 */
+void cuda_dom_BC(void);                      /* :1218-1438 + periodic planes */
+void cuda_dom_BC_star(void);                 /* :2111-2311 + periodic planes */
+void cuda_dom_BC_p(real *array);             /* :2538-2589 + periodic planes */
+void cuda_build_cages(void);                 /* src/cuda_particle.cu:1516-1646 */
+void cuda_find_dt(void);                     /* :1753-1808 */
+void cuda_compute_forcing(void);             /* :1811-1982 (constant gradP) */
+void cuda_U_star(void);                      /* :2095-2108 */
+void cuda_solvability(void);                 /* :2313-2492 */
+void cuda_project(void);                     /* :2495-2503 */
+void cuda_update_p(void);                    /* :2506-2535 */
+void cuda_store_u(void);                     /* :2592-2616 (pointer rotation) */
+void cuda_dom_free(void);                    /* :2619 */
+void cuda_PP_init_jacobi_preconditioner(void);  /* src/cuda_solver.cu:32-36 */
+void cuda_PP_cg(void);                       /* src/cuda_solver.cu:39-300 */
+void cuda_PP_cg_noparts(void);               /* src/cuda_solver.cu:574-761 */
+/* No counterpart in the reference: the FP32-inner / FP64-outer form. */
+void cuda_PP_cg_mixed(void);
+/* Periodic ghost refresh of one field; replaces mpi_cuda_exchange_G*
+ * (src/mpi_comm.c:257-405) on a single GPU.  `grid` is GCC/GFX/GFY/GFZ. */
+void cuda_exchange(real *array, int grid);
+void cuda_check_errors(int line);            /* src/cuda_bluebottle.cu:37-42 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,122 @@
+"""ctypes front end of the FP64 host oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Arrays are returned as [i, j, k]
+numpy views of the oracle's x-fastest ghosted storage.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+GCC, GFX, GFY, GFZ = 0, 1, 2, 3
+
+_GRID = {"p": GCC, "p0": GCC, "phi": GCC, "rhs": GCC, "invM": GCC,
+         "phase": GCC, "phase_shell": GCC}
+for _c, _g in (("u", GFX), ("v", GFY), ("w", GFZ)):
+    for _pre in ("", "0", "_star"):
+        _GRID[_c + _pre] = _g
+    for _pre in ("conv_", "conv0_", "diff_", "diff0_"):
+        _GRID[_pre + _c] = _g
+    _GRID["flag_" + _c] = _g
+_GRID.update({"f_x": GFX, "f_y": GFY, "f_z": GFZ})
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_setup.argtypes = [ctypes.c_char_p]
+        L.orc_setup.restype = ctypes.c_int
+        L.orc_step.restype = ctypes.c_int
+        L.orc_pcg.restype = ctypes.c_int
+        L.orc_field.argtypes = [ctypes.c_char_p]
+        L.orc_field.restype = ctypes.POINTER(ctypes.c_double)
+        L.orc_ifield.argtypes = [ctypes.c_char_p]
+        L.orc_ifield.restype = ctypes.POINTER(ctypes.c_int)
+        L.orc_scalar.argtypes = [ctypes.c_char_p]
+        L.orc_scalar.restype = ctypes.c_double
+        L.orc_set_scalar.argtypes = [ctypes.c_char_p, ctypes.c_double]
+        L.orc_canon_sum.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_canon_sum.restype = ctypes.c_double
+        L.orc_canon_linear.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                       ctypes.c_long]
+        L.orc_canon_linear.restype = ctypes.c_double
+        L.orc_bc_p.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        L.orc_exchange.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.orc_dims.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    def __init__(self, case_dir):
+        self.lib = load()
+        self.dir = os.path.abspath(case_dir)
+
+    def __enter__(self):
+        self.setup()
+        return self
+
+    def __exit__(self, *exc):
+        self.teardown()
+
+    def setup(self):
+        rc = self.lib.orc_setup(self.dir.encode())
+        if rc != 0:
+            raise RuntimeError(f"orc_setup failed ({rc})")
+
+    def step(self, nsteps=1):
+        q = 0
+        for _ in range(nsteps):
+            q = self.lib.orc_step()
+            if q < 0:
+                raise RuntimeError("oracle PCG failed")
+        return q
+
+    def dims(self, grid):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.lib.orc_dims(grid, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def field(self, name):
+        grid = _GRID[name]
+        ni, nj, nk = self.dims(grid)
+        size = (ni + 2) * (nj + 2) * (nk + 2)
+        if name.startswith("flag_") or name.startswith("phase"):
+            ptr = self.lib.orc_ifield(name.encode())
+        else:
+            ptr = self.lib.orc_field(name.encode())
+        flat = np.ctypeslib.as_array(ptr, shape=(size,))
+        return flat.reshape(nk + 2, nj + 2, ni + 2).transpose(2, 1, 0)
+
+    def raw(self, name):
+        return self.lib.orc_field(name.encode())
+
+    def scalar(self, name):
+        return self.lib.orc_scalar(name.encode())
+
+    def set_scalar(self, name, v):
+        self.lib.orc_set_scalar(name.encode(), float(v))
+
+    def call(self, name, *args):
+        return getattr(self.lib, name)(*args)
+
+    def teardown(self):
+        self.lib.orc_teardown()
+
+
+def canon_sum(terms):
+    """Canonical sum of an (nx, ny, nz) array of per-cell terms."""
+    L = load()
+    t = np.ascontiguousarray(np.asarray(terms, dtype=np.float64).transpose(2, 1, 0))
+    nz, ny, nx = t.shape
+    return L.orc_canon_sum(t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           nx, ny, nz)
