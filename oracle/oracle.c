@@ -812,6 +812,11 @@ static void spmv(const double *x, double *Ax)
       }
 }
 
+void orc_apply_A(const double *x, double *Ax)
+{
+  spmv(x, Ax);
+}
+
 static double now_s(void)
 {
   struct timeval t;

@@ -46,6 +46,8 @@ void orc_store_u(void);
 void orc_find_dt(void);
 void orc_jacobi(void);
 void orc_build_flags(void);
+/* (-A) x over interior cells with the face flags (ghosts of x as given). */
+void orc_apply_A(const double *x, double *Ax);
 
 /* Canonical reductions, exposed so tests can pin them. */
 double orc_canon_sum(const double *terms, int nx, int ny, int nz);

@@ -51,6 +51,7 @@ def load():
         L.orc_canon_linear.restype = ctypes.c_double
         L.orc_bc_p.argtypes = [ctypes.POINTER(ctypes.c_double)]
         L.orc_exchange.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.orc_apply_A.argtypes = [ctypes.POINTER(ctypes.c_double)] * 2
         L.orc_dims.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
         _lib = L
     return _lib
