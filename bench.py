@@ -36,12 +36,10 @@ KT_SPMV, KT_UPDATE, KT_SEARCH = 0, 1, 2
 # Algorithmic bytes per interior cell of each timed kernel (FP64, one element
 # per array the kernel semantically reads or writes).
 KERNEL_BYTES = {
-    # update_kernel: reads phi, p, r, Ap, invM; writes phi, r
-    "update_kernel": 7 * 8,
-    # spmv_dot_kernel: reads p; writes Ap
-    "spmv_dot_kernel": 2 * 8,
-    # search_kernel: reads r, invM, p; writes p
-    "search_kernel": 4 * 8,
+    # update_kernel: reads phi, p, r, Ap; writes phi, r
+    "update_kernel": 6 * 8,
+    # spmv_search_kernel: reads r, p_old; writes p_new, Ap
+    "spmv_search_kernel": 4 * 8,
 }
 
 
@@ -56,6 +54,15 @@ def parse():
     ap.add_argument("--precision", default="PLAIN", choices=["PLAIN", "MIXED"])
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
+    ap.add_argument("--spmv-variant", type=int, default=None,
+                    help="0: row SpMV kernel, 1: 2.5-D LDS kernel (default)")
+    ap.add_argument("--kchunk", type=int, default=None,
+                    help="planes per workgroup of the 2.5-D SpMV")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="PCG iterations enqueued per host check")
+    ap.add_argument("--calib-only", action="store_true",
+                    help="run only the stream-copy calibration kernel "
+                         "(2^27 doubles per array) and print its GB/s")
     return ap.parse_args()
 
 
@@ -83,10 +90,21 @@ def main():
         print(f"bench.py: this build runs on one GPU that holds the whole "
               f"domain; --gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    lib = bbhip.load()
+    lib.bb_stream_copy_gbs.restype = ctypes.c_double
+    lib.bb_stream_copy_gbs.argtypes = [ctypes.c_long, ctypes.c_int]
+    copy_gbs = lib.bb_stream_copy_gbs(1 << 27, 10)
+    if a.calib_only:
+        print(json.dumps({"stream_copy_gbs": copy_gbs,
+                          "elements": 1 << 27}))
+        return
+    for name, val in (("bb_spmv_variant", a.spmv_variant),
+                      ("bb_spmv_kchunk", a.kchunk), ("bb_pcg_batch", a.batch)):
+        if val is not None:
+            ctypes.c_int.in_dll(lib, name).value = val
     n = (a.nx, a.ny, a.nz)
     case = tempfile.mkdtemp(prefix="bb_bench_")
     bbconfig.write_case(case, **bbconfig.channel(n=n, precision=a.precision))
-    lib = bbhip.load()
     sim = bbhip.Simulation(case, n)
     sim.setup()
     iters = []
@@ -109,8 +127,7 @@ def main():
     cnt = ctypes.c_long()
     ktimes = {}
     for name, kid in (("update_kernel", KT_UPDATE),
-                      ("spmv_dot_kernel", KT_SPMV),
-                      ("search_kernel", KT_SEARCH)):
+                      ("spmv_search_kernel", KT_SPMV)):
         lib.bb_ktimer_read(kid, ctypes.byref(ms), ctypes.byref(cnt))
         ktimes[name] = (ms.value, cnt.value)
     stats = sim.pp_stats()
@@ -141,6 +158,8 @@ def main():
             "grid": [a.nx, a.ny, a.nz],
             "pp_precision": a.precision,
             "pcg_iters_per_step": sum(timed_iters) / max(len(timed_iters), 1),
+            "spmv_variant": ctypes.c_int.in_dll(lib, "bb_spmv_variant").value,
+            "spmv_kchunk": ctypes.c_int.in_dll(lib, "bb_spmv_kchunk").value,
             "last_true_resid": stats["true_resid"],
         },
         "roofline": {
@@ -150,6 +169,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "frac_of_measured_copy": achieved / copy_gbs,
+            "measured_copy_gbs": copy_gbs,
             "traffic": None,
             "bytes_per_cell": KERNEL_BYTES[dom_name],
             "avg_launch_us": avg_s * 1e6,

@@ -15,10 +15,15 @@
  *      starting from +0.0; then the 64 lane values are combined by the
  *      halving tree  v[l] += v[l + off]  for off = 32, 16, ..., 1; the row
  *      partial is v[0];
- *   2. the row partials, numbered r = (j-1) + (k-1)*ny, are combined by one
- *      256-thread workgroup: thread t adds rows t, t+256, ... in increasing
- *      order from +0.0, then the halving tree over 256 threads (off = 128,
- *      ..., 1) leaves the total in thread 0.
+ *   2. group partial: the rows of one plane k are taken four at a time,
+ *      j = 4g+1 .. 4g+4 (a row past ny counts as +0.0), and combined as
+ *      (r0 + r1) + (r2 + r3); groups are numbered g + (k-1)*ceil(ny/4);
+ *   3. the group partials are combined by one 1024-thread workgroup: thread
+ *      t adds groups t, t+1024, ... in increasing order from +0.0, then the
+ *      halving tree over 1024 threads (off = 512, ..., 1) leaves the total
+ *      in thread 0.  Step 3 is its own small launch: the kernel boundary is
+ *      what makes every workgroup's partial visible across the XCDs (a
+ *      device-scope fence per workgroup would write back L2 each time).
  * All arithmetic is IEEE binary64 with FMA contraction disabled
  * (-ffp-contract=off), so one multiply and one add are rounded separately
  * exactly as a plain C compiler does on the host.
@@ -46,7 +51,7 @@
 
 #define WAVE 64
 #define ROWS_PER_BLOCK 4              /* one wavefront per row, 4 per block */
-#define REDUCE_THREADS 256
+#define REDUCE_THREADS 1024
 
 /* A ghosted x-fastest grid.  ni/nj/nk count the interior points (the
  * reference's in/jn/kn); sy/sz are the strides of the ghosted array. */
@@ -79,12 +84,13 @@ extern double *_bb_rows2;             /* second row-partial buffer */
 extern double *_bb_scalars;           /* small device result slots */
 extern double *bb_host_scalars;       /* pinned host mirror of the slots */
 
-/* Interior-row kernel shape: one wavefront per (j, k) row. */
-inline dim3 rows_grid(int ny, int nz)
+/* Interior-row kernel shape: one workgroup per row group (four rows of one
+ * plane), one wavefront per row. */
+__host__ __device__ inline long n_groups(int ny, int nz)
 {
-  long rows = (long) ny * nz;
-  return dim3((unsigned) ((rows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK));
+  return (long) ((ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK) * nz;
 }
+inline dim3 rows_grid(int ny, int nz) { return dim3((unsigned) n_groups(ny, nz)); }
 inline dim3 rows_block() { return dim3(WAVE * ROWS_PER_BLOCK); }
 
 /* Row partial tree over one wavefront (step 1 of the canonical order).  The
@@ -96,7 +102,63 @@ __device__ inline double wave_tree_sum(double v)
   return v;
 }
 
-/* Host-side launchers implemented in bb_reduce.hip. */
+/* Step 3 over `n` partials by one REDUCE_THREADS workgroup; total in thread
+ * 0.  Loads are issued eight at a time and added in index order. */
+__device__ inline double block_canonical_sum(const double *__restrict__ part,
+  long n)
+{
+  __shared__ double s[REDUCE_THREADS];
+  const int RT = REDUCE_THREADS;
+  int t = threadIdx.x;
+  double acc = 0.0;
+  long q = t;
+  for (; q + 7L * RT < n; q += 8L * RT) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = part[q + (long) u * RT];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  for (; q < n; q += RT) acc += part[q];
+  s[t] = acc;
+  __syncthreads();
+  for (int off = RT / 2; off >= WAVE; off >>= 1) {
+    if (t < off) s[t] += s[t + off];
+    __syncthreads();
+  }
+  double v = 0.;
+  if (t < WAVE) v = wave_tree_sum(s[t]);
+  return v;
+}
+
+/* Steps 1-2 for a 4-wavefront workgroup that owns one row group: thread 0
+ * writes the group partial.  All threads must call it with their lane's row
+ * accumulator (+0.0 for a row past the end). */
+__device__ inline void group_partial(double rowacc, double *__restrict__ gpart)
+{
+  __shared__ double s_rows[ROWS_PER_BLOCK];
+  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  double v = wave_tree_sum(rowacc);
+  if (lane == 0) s_rows[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    gpart[blockIdx.x] = (s_rows[0] + s_rows[1]) + (s_rows[2] + s_rows[3]);
+}
+
+/* Group-kernel prologue: this workgroup's plane k and row group, this
+ * wavefront's row j (rowok false past ny). */
+#define GROUP_PROLOGUE                                                        \
+  const Grid &C = D.cc;                                                       \
+  int lane = threadIdx.x & (WAVE - 1);                                        \
+  int ngy_ = (D.ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;                    \
+  int k = (int) (blockIdx.x / ngy_) + 1;                                      \
+  int j = (int) (blockIdx.x % ngy_) * ROWS_PER_BLOCK + (threadIdx.x >> 6) + 1; \
+  bool rowok = (j <= D.ny);                                                   \
+  int nxr = rowok ? D.nx : 0;                                                 \
+  long base = gidx(C, 0, rowok ? j : D.ny, k);
+
+/* Host-side launchers implemented in bb_reduce.hip.  bb_reduce_rows runs
+ * step 3 alone over `nrows` partials (face sums and group partials). */
 void bb_reduce_rows(const double *rows, long nrows, double *out_slot);
 double bb_fetch_scalar(int slot);
 void bb_fetch_scalars(int n);

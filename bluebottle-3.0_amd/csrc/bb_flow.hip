@@ -434,29 +434,22 @@ extern "C" void cuda_project(void)
 }
 
 /* --------------------------- pressure update ---------------------------- */
-/* p = (phase < 0) (p0 + phi) on interior cells, with the row partials of p
- * for the mean. */
+/* p = (phase < 0) (p0 + phi) on interior cells, with the group partials of
+ * p for the mean. */
 __global__ void __launch_bounds__(256)
 update_p_kernel(DevDom D, const double *__restrict__ p0,
   const double *__restrict__ ph, const int *__restrict__ phase,
-  double *__restrict__ p, double *__restrict__ rows)
+  double *__restrict__ p, double *__restrict__ gpart)
 {
-  const Grid &C = D.cc;
-  int lane = threadIdx.x & (WAVE - 1);
-  long row = (long) blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
-  if (row >= (long) D.ny * D.nz) return;
-  int j = (int) (row % D.ny) + 1;
-  int k = (int) (row / D.ny) + 1;
-  long base = gidx(C, 0, j, k);
+  GROUP_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
     long c = base + i;
     double val = (double) (phase[c] < 0) * (p0[c] + ph[c]);
     p[c] = val;
     acc += val;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
 __global__ void __launch_bounds__(256)
@@ -479,7 +472,7 @@ extern "C" void cuda_update_p(void)
   hipLaunchKernelGGL(update_p_kernel, rows_grid(D.ny, D.nz), rows_block(), 0,
     bb_stream, D, _p0, _phi, _phase, _p, _bb_rows);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, (long) D.ny * D.nz, _bb_scalars + 0);
+  bb_reduce_rows(_bb_rows, n_groups(D.ny, D.nz), _bb_scalars + 0);
   real pmean = bb_fetch_scalar(0);
   pmean /= (real) DOM.Gcc.s3;
   hipLaunchKernelGGL(add_const_kernel, dim3((D.cc.ni + 63) / 64,
@@ -530,8 +523,10 @@ max_rows_kernel(const double *__restrict__ rows, long n, double *out)
 
 static void sqmax(const Grid &G, const double *a, int slot)
 {
-  hipLaunchKernelGGL(sqmax_rows_kernel, rows_grid(G.nj, G.nk), rows_block(),
-    0, bb_stream, G, a, _bb_rows);
+  long nrows = (long) G.nj * G.nk;
+  hipLaunchKernelGGL(sqmax_rows_kernel,
+    dim3((unsigned) ((nrows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK)),
+    rows_block(), 0, bb_stream, G, a, _bb_rows);
   hipLaunchKernelGGL(max_rows_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
     bb_stream, _bb_rows, (long) G.nj * G.nk, _bb_scalars + slot);
   BB_LAUNCH_CHECK();

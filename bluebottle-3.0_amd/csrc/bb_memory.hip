@@ -317,10 +317,13 @@ static void hfree(void *ptr)
   if (ptr) BB_CHECK(hipHostFree(ptr));
 }
 
+void bb_solver_free(void);
+
 extern "C" void cuda_dom_free(void)
 {
   /* src/cuda_bluebottle.cu:2619 */
   BB_CHECK(hipDeviceSynchronize());
+  bb_solver_free();
   real *dd[] = {_p, _p0, _phi, _invM, _rhs_p, _r_q, _z_q, _p_q, _Apb_q,
     _u, _u0, _v, _v0, _w, _w0, _u_star, _v_star, _w_star,
     _conv_u, _conv_v, _conv_w, _conv0_u, _conv0_v, _conv0_w,

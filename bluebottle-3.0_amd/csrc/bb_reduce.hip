@@ -10,21 +10,8 @@ __global__ void __launch_bounds__(REDUCE_THREADS)
 reduce_rows_kernel(const double *__restrict__ rows, long nrows,
   double *__restrict__ out)
 {
-  __shared__ double s[REDUCE_THREADS];
-  int t = threadIdx.x;
-  double acc = 0.0;
-  for (long r = t; r < nrows; r += REDUCE_THREADS) acc += rows[r];
-  s[t] = acc;
-  __syncthreads();
-  for (int off = REDUCE_THREADS / 2; off >= WAVE; off >>= 1) {
-    if (t < off) s[t] += s[t + off];
-    __syncthreads();
-  }
-  if (t < WAVE) {
-    double v = s[t];
-    v = wave_tree_sum(v);
-    if (t == 0) *out = v;
-  }
+  double v = block_canonical_sum(rows, nrows);
+  if (threadIdx.x == 0) *out = v;
 }
 
 void bb_reduce_rows(const double *rows, long nrows, double *out_slot)

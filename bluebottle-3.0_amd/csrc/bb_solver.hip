@@ -3,23 +3,27 @@
  * reference's algorithm (src/cuda_solver.cu:32-36, 574-761,
  * src/solver_kernel.cu:26-927) with its recurrences, stopping test
  * (r, M^-1 r) <= pp_residual^2 (b, b), and true-residual refresh every 50
- * iterations.
+ * iterations.  Every rounded value is the reference's; what changes is where
+ * values live and how many passes over HBM an iteration takes.
  *
- * Kernel structure (one wavefront per (j, k) row, lanes stride along x):
- *   rhs_kernel       rhs = -(rho/dt) div u*            + row partials of (b,b)
- *   cg_init_kernel   r = b, p = M^-1 b, phi = 0        + row partials of (r,z)
- *   spmv_dot_kernel  Ap = (-A) p                       + row partials of (p,Ap)
- *   update_kernel    phi += a p, r -= a Ap, z = M^-1 r + row partials of (r,z)
- *   search_kernel    p = z + b p
- * z = M^-1 r is recomputed where it is needed instead of being stored (same
- * rounded value), and the search direction p carries its own ghost layer so
- * no separate pb copy exists.
+ * Kernels (one wavefront per (j, k) row, lanes stride along x):
+ *   rhs_kernel          rhs = -(rho/dt) div u*          + row partials (b,b)
+ *   cg_init_kernel      r = b, phi = 0                   + row partials (r,z)
+ *   spmv_search_kernel  p = z + beta p_old, Ap = (-A) p  + row partials (p,Ap)
+ *   update_kernel       phi += a p, r -= a Ap            + row partials (r,z)
+ *   reduce_*_kernel     canonical second stage; alpha, beta and the stopping
+ *                       test are evaluated on the device
+ * z = M^-1 r is never stored (it is recomputed, bit-identically, where it is
+ * used); M^-1 comes from a 27-entry table indexed by the number of
+ * non-boundary faces per direction; the search update is folded into the
+ * SpMV, which reads periodic neighbours by wrapped index, so the search
+ * direction needs no ghost refresh.  Per iteration: 80 B per cell of HBM
+ * traffic (the reference's kernel sequence moves 136).
  *
  * The 7-point weights flag^2 are 0 exactly on faces of a non-periodic
  * external boundary and 1 elsewhere (src/particle_kernel.cu:481-576 writes
  * only +-1 or, at external boundaries, 0), so they are computed from the
- * cell index; invM = -1/M with M built from the same weights
- * (src/solver_kernel.cu:26-87).
+ * cell index.
  */
 #include <math.h>
 #include <sys/time.h>
@@ -54,61 +58,78 @@ __device__ inline RowW row_weights(const DevDom &D, int j, int k)
   return W;
 }
 
-/* (-A) x at cell c (src/solver_kernel.cu:715-836). */
-__device__ inline double neg_lap(const double *__restrict__ x, long c, long sy,
-  long sz, double fe2, double fw2, const RowW &W, const Coef &K)
+/* --------------------- device-resident CG scalars ------------------------ */
+/* The host enqueues iterations in batches without waiting; these scalars
+ * live in device memory and every kernel of an iteration returns at once
+ * when `done` is set, so the iterates stop exactly where the reference's
+ * host loop breaks. */
+struct CGState {
+  double sp_rhs, rz, rz1, denom, alpha, beta, tol2, diag;
+  int q, done, nan_flag, pad;
+};
+
+/* Jacobi diagonal by the count of non-boundary faces per direction (0..2):
+ * invM = -1/M, M = -idx2 wx - idy2 wy - idz2 wz (src/solver_kernel.cu:66-77),
+ * evaluated once on the host with the same operand order. */
+struct InvMTab {
+  double v[27];
+};
+
+static InvMTab invm_table(const Coef &K)
 {
-  double xC = x[c];
-  return -K.idx2 * (fe2*(x[c + 1] - xC) - fw2*(xC - x[c - 1]))
-         -K.idy2 * (W.fn2*(x[c + sy] - xC) - W.fs2*(xC - x[c - sy]))
-         -K.idz2 * (W.ft2*(x[c + sz] - xC) - W.fb2*(xC - x[c - sz]));
+  InvMTab T;
+  for (int wx = 0; wx < 3; wx++)
+    for (int wy = 0; wy < 3; wy++)
+      for (int wz = 0; wz < 3; wz++) {
+        double M = -K.idx2*wx - K.idy2*wy - K.idz2*wz;
+        T.v[wx + 3 * wy + 9 * wz] = -1./M;
+      }
+  return T;
 }
 
-#define ROW_PROLOGUE                                                          \
-  const Grid &C = D.cc;                                                       \
-  int lane = threadIdx.x & (WAVE - 1);                                        \
-  long row = (long) blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);         \
-  if (row >= (long) D.ny * D.nz) return;                                      \
-  int j = (int) (row % D.ny) + 1;                                             \
-  int k = (int) (row / D.ny) + 1;                                             \
-  long base = gidx(C, 0, j, k);
+/* Non-boundary face count of cell index `a` in a direction with n cells. */
+__device__ inline int wcount(int a, int n, int per)
+{
+  if (per) return 2;
+  return (a == 1 ? 0 : 1) + (a == n ? 0 : 1);
+}
+
+/* Neighbour index with periodic wrap (ghost index in a wall direction). */
+__device__ inline int wrap_lo(int a, int n, int per) { return (per && a == 1) ? n : a - 1; }
+__device__ inline int wrap_hi(int a, int n, int per) { return (per && a == n) ? 1 : a + 1; }
 
 /* ------------------------------ Jacobi ---------------------------------- */
 __global__ void __launch_bounds__(256)
-jacobi_kernel(DevDom D, Coef K, double *__restrict__ invM)
+jacobi_kernel(DevDom D, InvMTab T, double *__restrict__ invM)
 {
-  ROW_PROLOGUE
-  int fn = (!D.pery && j == D.ny) ? 0 : 1, fs = (!D.pery && j == 1) ? 0 : 1;
-  int ft = (!D.perz && k == D.nz) ? 0 : 1, fb = (!D.perz && k == 1) ? 0 : 1;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
-    int fe = (!D.perx && i == D.nx) ? 0 : 1, fw = (!D.perx && i == 1) ? 0 : 1;
-    double M = -K.idx2*(fe*fe + fw*fw)
-               -K.idy2*(fn*fn + fs*fs)
-               -K.idz2*(ft*ft + fb*fb);
-    invM[base + i] = -1./M;
-  }
+  /* src/solver_kernel.cu:26-87: the stored preconditioner of the C ABI. */
+  GROUP_PROLOGUE
+  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
+  for (int i = lane + 1; i <= nxr; i += WAVE)
+    invM[base + i] = T.v[wcount(i, D.nx, D.perx) + 3 * wy + 9 * wz];
 }
 
 extern "C" void cuda_PP_init_jacobi_preconditioner(void)
 {
   hipLaunchKernelGGL(jacobi_kernel, rows_grid(ddom.ny, ddom.nz), rows_block(),
-    0, bb_stream, ddom, coef(), _invM);
+    0, bb_stream, ddom, invm_table(coef()), _invM);
   BB_LAUNCH_CHECK();
 }
 
 /* ------------------------------- RHS ------------------------------------ */
+/* rhs = -(rho/dt) div u* (src/solver_kernel.cu:89-176); (b, b) -> *out. */
 __global__ void __launch_bounds__(256)
 rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
   const double *__restrict__ us, const double *__restrict__ vs,
   const double *__restrict__ ws, double *__restrict__ rhs,
-  double *__restrict__ rows)
+  double *__restrict__ gpart)
 {
-  ROW_PROLOGUE
-  long bu = gidx(D.fx, 0, j, k), bv = gidx(D.fy, 0, j, k),
-       bw = gidx(D.fz, 0, j, k);
+  GROUP_PROLOGUE
+  long bu = gidx(D.fx, 0, rowok ? j : 1, k), bv = gidx(D.fy, 0, rowok ? j : 1, k),
+       bw = gidx(D.fz, 0, rowok ? j : 1, k);
   long vsy = D.fy.sy, wsz = D.fz.sz;
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
     double s = (us[bu + i + 1] - us[bu + i]) * idx;
     s += (vs[bv + i + vsy] - vs[bv + i]) * idy;
     s += (ws[bw + i + wsz] - ws[bw + i]) * idz;
@@ -117,136 +138,361 @@ rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
     rhs[base + i] = b;
     acc += b * b;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
 /* ------------------------------ CG init --------------------------------- */
+/* r = b, phi = 0 (src/solver_kernel.cu:258-283) + group partials of (r, z).
+ * p = z is formed by the first spmv_search pass. */
 __global__ void __launch_bounds__(256)
-cg_init_kernel(DevDom D, const double *__restrict__ rhs,
-  const double *__restrict__ invM, double *__restrict__ r,
-  double *__restrict__ p, double *__restrict__ phi, double *__restrict__ rows)
+cg_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
+  double *__restrict__ r, double *__restrict__ phi,
+  double *__restrict__ gpart)
 {
-  ROW_PROLOGUE
+  GROUP_PROLOGUE
+  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
     long c = base + i;
     double t = rhs[c];
-    double t2 = t * invM[c];
+    double t2 = t * T.v[wcount(i, D.nx, D.perx) + 3 * wy + 9 * wz];
     r[c] = t;
-    p[c] = t2;
     phi[c] = 0.;
     acc += t * t2;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
-/* ---------------------------- SpMV + dot -------------------------------- */
-__global__ void __launch_bounds__(256)
-spmv_dot_kernel(DevDom D, Coef K, const double *__restrict__ p,
-  double *__restrict__ Ap, double *__restrict__ rows)
+/* p_new = z + beta p_old with z = r invM (src/solver_kernel.cu:906-927). */
+template <bool FIRST>
+__device__ inline double p_new(const double *__restrict__ r,
+  const double *__restrict__ pold, long c, double invm, double beta)
 {
-  ROW_PROLOGUE
-  RowW W = row_weights(D, j, k);
+  double z = r[c]*invm;
+  if (FIRST) return z;
+  return z + beta*pold[c];
+}
+
+/* Iteration kernel 1: the search update fused into the SpMV.  Computes
+ * p_new at the cell and its six neighbours from r and p_old (periodic
+ * neighbours by wrapped index, so no ghost refresh of p is needed), writes
+ * p_new and Ap = (-A) p_new, and the group partials of (p, Ap). */
+template <bool FIRST>
+__global__ void __launch_bounds__(256)
+spmv_search_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
+  const double *__restrict__ r, const double *__restrict__ pold,
+  double *__restrict__ pnew, double *__restrict__ Ap,
+  double *__restrict__ gpart)
+{
+  if (st->done) return;
+  GROUP_PROLOGUE
+  double beta = FIRST ? 0. : st->beta;
+  int jr = rowok ? j : 1;
+  RowW W = row_weights(D, jr, k);
+  int jm = wrap_lo(jr, D.ny, D.pery), jp = wrap_hi(jr, D.ny, D.pery);
+  int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+  long bS = gidx(C, 0, jm, k), bN = gidx(C, 0, jp, k);
+  long bB = gidx(C, 0, jr, km), bT = gidx(C, 0, jr, kp);
+  int wy = wcount(jr, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
+  int wyS = wcount(jm, D.ny, D.pery), wyN = wcount(jp, D.ny, D.pery);
+  int wzB = wcount(km, D.nz, D.perz), wzT = wcount(kp, D.nz, D.perz);
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
-    long c = base + i;
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
+    int im = wrap_lo(i, D.nx, D.perx), ip = wrap_hi(i, D.nx, D.perx);
+    int wx = wcount(i, D.nx, D.perx);
+    int wxW = wcount(im, D.nx, D.perx), wxE = wcount(ip, D.nx, D.perx);
+    double pC = p_new<FIRST>(r, pold, base + i, T.v[wx + 3*wy + 9*wz], beta);
+    double pW = p_new<FIRST>(r, pold, base + im, T.v[wxW + 3*wy + 9*wz], beta);
+    double pE = p_new<FIRST>(r, pold, base + ip, T.v[wxE + 3*wy + 9*wz], beta);
+    double pS = p_new<FIRST>(r, pold, bS + i, T.v[wx + 3*wyS + 9*wz], beta);
+    double pN = p_new<FIRST>(r, pold, bN + i, T.v[wx + 3*wyN + 9*wz], beta);
+    double pB = p_new<FIRST>(r, pold, bB + i, T.v[wx + 3*wy + 9*wzB], beta);
+    double pT = p_new<FIRST>(r, pold, bT + i, T.v[wx + 3*wy + 9*wzT], beta);
     double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
     double fw2 = (!D.perx && i == 1) ? 0. : 1.;
-    double a = neg_lap(p, c, C.sy, C.sz, fe2, fw2, W, K);
-    Ap[c] = a;
-    acc += p[c] * a;
+    double a = -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
+               -K.idy2 * (W.fn2*(pN - pC) - W.fs2*(pC - pS))
+               -K.idz2 * (W.ft2*(pT - pC) - W.fb2*(pC - pB));
+    pnew[base + i] = pC;
+    Ap[base + i] = a;
+    acc += pC * a;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
-/* ------------------------ solution/residual update ---------------------- */
-__global__ void __launch_bounds__(256)
-update_kernel(DevDom D, double alpha, const double *__restrict__ p,
-  const double *__restrict__ Ap, const double *__restrict__ invM,
-  double *__restrict__ phi, double *__restrict__ r, double *__restrict__ rows)
+/* The same pass, 2.5-D: a workgroup of 8 wavefronts owns rows j0..j0+7
+ * (two row groups) and marches a chunk of planes in z.  For every plane it
+ * forms p_new once per cell, halo rows and columns included, into one of two
+ * LDS plane buffers; x and y neighbours come from LDS, z neighbours from
+ * registers carried along the march.  Same arithmetic and the same row,
+ * group and final reduction order as spmv_search_kernel. */
+#define T25_ROWS 8
+#define T25_THREADS (WAVE * T25_ROWS)
+
+template <bool FIRST>
+__device__ inline void fill_plane25(const DevDom &D, const InvMTab &T,
+  const double *__restrict__ r, const double *__restrict__ pold, double beta,
+  double *__restrict__ buf, int j0, int kk)
 {
-  ROW_PROLOGUE
+  const Grid &C = D.cc;
+  const int NXP = D.nx + 2;
+  /* z source plane (0 marks a wall ghost, whose value is 0) */
+  int ks = kk;
+  if (kk < 1) ks = D.perz ? D.nz : 0;
+  else if (kk > D.nz) ks = D.perz ? 1 : 0;
+  int wz = ks ? wcount(ks, D.nz, D.perz) : 0;
+  const int items = (T25_ROWS + 2) * NXP;
+  for (int it = threadIdx.x; it < items; it += T25_THREADS) {
+    int rr = it / NXP, xx = it - rr * NXP;
+    int jj = j0 - 1 + rr;
+    int js = jj;
+    if (jj < 1) js = D.pery ? D.ny : 0;
+    else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
+    int xs = xx;
+    if (xx < 1) xs = D.perx ? D.nx : 0;
+    else if (xx > D.nx) xs = D.perx ? 1 : 0;
+    double v = 0.;
+    if (ks && js && xs) {
+      long c = gidx(C, xs, js, ks);
+      double invm = T.v[wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery)
+                        + 9 * wz];
+      v = p_new<FIRST>(r, pold, c, invm, beta);
+    }
+    buf[it] = v;
+  }
+}
+
+template <bool FIRST, int NC>
+__global__ void __launch_bounds__(T25_THREADS)
+spmv_search25_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
+  const double *__restrict__ r, const double *__restrict__ pold,
+  double *__restrict__ pnew, double *__restrict__ Ap,
+  double *__restrict__ gpart, int kchunk)
+{
+  if (st->done) return;
+  extern __shared__ double lds[];
+  __shared__ double s_rows[T25_ROWS];
+  const Grid &C = D.cc;
+  const int NXP = D.nx + 2;
+  const int PL = (T25_ROWS + 2) * NXP;
+  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  int j0 = blockIdx.x * T25_ROWS + 1;
+  int k0 = blockIdx.y * kchunk + 1;
+  int k1 = k0 + kchunk - 1;
+  if (k1 > D.nz) k1 = D.nz;
+  int j = j0 + w;
+  bool rowok = (j <= D.ny);
+  int ngy = (D.ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+  double beta = FIRST ? 0. : st->beta;
+  RowW W = row_weights(D, rowok ? j : 1, 1);
+  double prev[NC], cur[NC], nxt[NC];
+
+  fill_plane25<FIRST>(D, T, r, pold, beta, lds + ((k0 - 1) & 1) * PL, j0, k0 - 1);
+  fill_plane25<FIRST>(D, T, r, pold, beta, lds + (k0 & 1) * PL, j0, k0);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    int x = lane + 1 + WAVE * c;
+    int xx = x <= D.nx ? x : 0;
+    prev[c] = lds[((k0 - 1) & 1) * PL + (w + 1) * NXP + xx];
+    cur[c] = lds[(k0 & 1) * PL + (w + 1) * NXP + xx];
+  }
+  for (int k = k0; k <= k1; k++) {
+    __syncthreads();
+    fill_plane25<FIRST>(D, T, r, pold, beta, lds + ((k + 1) & 1) * PL, j0, k + 1);
+    __syncthreads();
+    const double *P = lds + (k & 1) * PL;
+    const double *PN = lds + ((k + 1) & 1) * PL;
+    double fb2 = (!D.perz && k == 1) ? 0. : 1.;
+    double ft2 = (!D.perz && k == D.nz) ? 0. : 1.;
+    long base = gidx(C, 0, rowok ? j : 1, k);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      int x = lane + 1 + WAVE * c;
+      int xx = x <= D.nx ? x : 0;
+      nxt[c] = PN[(w + 1) * NXP + xx];
+      if (rowok && x <= D.nx) {
+        double pC = cur[c];
+        double pW = P[(w + 1) * NXP + x - 1];
+        double pE = P[(w + 1) * NXP + x + 1];
+        double pS = P[w * NXP + x];
+        double pN = P[(w + 2) * NXP + x];
+        double pB = prev[c], pT = nxt[c];
+        double fe2 = (!D.perx && x == D.nx) ? 0. : 1.;
+        double fw2 = (!D.perx && x == 1) ? 0. : 1.;
+        double a = -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
+                   -K.idy2 * (W.fn2*(pN - pC) - W.fs2*(pC - pS))
+                   -K.idz2 * (ft2*(pT - pC) - fb2*(pC - pB));
+        pnew[base + x] = pC;
+        Ap[base + x] = a;
+        acc += pC * a;
+      }
+    }
+    double v = wave_tree_sum(acc);
+    if (lane == 0) s_rows[w] = v;
+    __syncthreads();
+    if (threadIdx.x < T25_ROWS / ROWS_PER_BLOCK) {
+      int gl = threadIdx.x;
+      int jg = (j0 - 1) / ROWS_PER_BLOCK + gl;
+      if (jg < ngy) {
+        const double *sr = s_rows + gl * ROWS_PER_BLOCK;
+        gpart[jg + (long) (k - 1) * ngy] = (sr[0] + sr[1]) + (sr[2] + sr[3]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) { prev[c] = cur[c]; cur[c] = nxt[c]; }
+  }
+}
+
+/* Convergence test and beta after a new (r, z) (src/cuda_solver.cu:229-267). */
+__device__ inline void cg_after_rz(CGState *st, double s, int q)
+{
+  st->rz1 = s;
+  st->q = q;
+  if (s <= st->tol2*st->sp_rhs) {
+    st->done = 1;
+  } else if (isnan(s)) {
+    st->nan_flag = 1;
+    st->done = 1;
+  } else {
+    st->beta = s/st->rz;
+    st->rz = s;
+  }
+}
+
+/* Iteration kernel 2 (src/solver_kernel.cu:838-862): phi += a p,
+ * r -= a Ap, z = invM r, group partials of (r, z). */
+__global__ void __launch_bounds__(256)
+update_kernel(DevDom D, InvMTab T, const CGState *st,
+  const double *__restrict__ p, const double *__restrict__ Ap,
+  double *__restrict__ phi, double *__restrict__ r,
+  double *__restrict__ gpart)
+{
+  if (st->done) return;
+  GROUP_PROLOGUE
+  double alpha = st->alpha;
+  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
     long c = base + i;
     phi[c] += alpha*p[c];
     double rr = r[c] - alpha*Ap[c];
     r[c] = rr;
-    double z = rr*invM[c];
+    double z = rr*T.v[wcount(i, D.nx, D.perx) + 3*wy + 9*wz];
     acc += rr * z;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
+/* Every 50th iteration (src/cuda_solver.cu:208-224): phi += a p ... */
 __global__ void __launch_bounds__(256)
-update_solution_kernel(DevDom D, double alpha, const double *__restrict__ p,
-  double *__restrict__ phi)
+update_solution_kernel(DevDom D, const CGState *__restrict__ st,
+  const double *__restrict__ p, double *__restrict__ phi)
 {
-  ROW_PROLOGUE
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
+  if (st->done) return;
+  GROUP_PROLOGUE
+  double alpha = st->alpha;
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
     long c = base + i;
     phi[c] += alpha*p[c];
   }
 }
 
-/* r = b - (-A) phi, the periodic true-residual refresh
- * (src/solver_kernel.cu:864-904), with row partials of (r, z). */
-__global__ void __launch_bounds__(256)
-residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
-  const double *__restrict__ rhs, const double *__restrict__ invM,
-  double *__restrict__ r, double *__restrict__ Ap, double *__restrict__ rows)
+/* (-A) x at cell i of a row with periodic neighbours by wrapped index. */
+__device__ inline double neg_lap_wrapped(const DevDom &D, const Coef &K,
+  const RowW &W, const double *__restrict__ x, long base, long bS, long bN,
+  long bB, long bT, int i)
 {
-  ROW_PROLOGUE
-  RowW W = row_weights(D, j, k);
+  int im = wrap_lo(i, D.nx, D.perx), ip = wrap_hi(i, D.nx, D.perx);
+  double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
+  double fw2 = (!D.perx && i == 1) ? 0. : 1.;
+  double xC = x[base + i];
+  return -K.idx2 * (fe2*(x[base + ip] - xC) - fw2*(xC - x[base + im]))
+         -K.idy2 * (W.fn2*(x[bN + i] - xC) - W.fs2*(xC - x[bS + i]))
+         -K.idz2 * (W.ft2*(x[bT + i] - xC) - W.fb2*(xC - x[bB + i]));
+}
+
+#define WRAPPED_ROWS                                                          \
+  int jr = rowok ? j : 1;                                                     \
+  RowW W = row_weights(D, jr, k);                                             \
+  int jm = wrap_lo(jr, D.ny, D.pery), jp = wrap_hi(jr, D.ny, D.pery);         \
+  int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);           \
+  long bS = gidx(C, 0, jm, k), bN = gidx(C, 0, jp, k);                        \
+  long bB = gidx(C, 0, jr, km), bT = gidx(C, 0, jr, kp);
+
+/* ... then r = b - (-A) phi, z = invM r, group partials of (r, z). */
+__global__ void __launch_bounds__(256)
+residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
+  const double *__restrict__ phi, const double *__restrict__ rhs,
+  double *__restrict__ r, double *__restrict__ gpart)
+{
+  if (st->done) return;
+  GROUP_PROLOGUE
+  WRAPPED_ROWS
+  int wy = wcount(jr, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
-    long c = base + i;
-    double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
-    double fw2 = (!D.perx && i == 1) ? 0. : 1.;
-    double a = neg_lap(phi, c, C.sy, C.sz, fe2, fw2, W, K);
-    Ap[c] = a;
-    double rr = rhs[c] - a;
-    r[c] = rr;
-    double z = rr*invM[c];
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
+    double a = neg_lap_wrapped(D, K, W, phi, base, bS, bN, bB, bT, i);
+    double rr = rhs[base + i] - a;
+    r[base + i] = rr;
+    double z = rr*T.v[wcount(i, D.nx, D.perx) + 3*wy + 9*wz];
     acc += rr * z;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
-/* Row partials of ||b - (-A) phi||^2 (diagnostic true residual). */
+/* Group partials of ||b - (-A) phi||^2 (diagnostic true residual). */
 __global__ void __launch_bounds__(256)
 true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
-  const double *__restrict__ rhs, double *__restrict__ rows)
+  const double *__restrict__ rhs, double *__restrict__ gpart)
 {
-  ROW_PROLOGUE
-  RowW W = row_weights(D, j, k);
+  GROUP_PROLOGUE
+  WRAPPED_ROWS
   double acc = 0.0;
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
-    long c = base + i;
-    double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
-    double fw2 = (!D.perx && i == 1) ? 0. : 1.;
-    double rr = rhs[c] - neg_lap(phi, c, C.sy, C.sz, fe2, fw2, W, K);
+  for (int i = lane + 1; i <= nxr; i += WAVE) {
+    double rr = rhs[base + i] - neg_lap_wrapped(D, K, W, phi, base, bS, bN,
+      bB, bT, i);
     acc += rr * rr;
   }
-  acc = wave_tree_sum(acc);
-  if (lane == 0) rows[row] = acc;
+  group_partial(acc, gpart);
 }
 
-/* ---------------------------- search update ----------------------------- */
-__global__ void __launch_bounds__(256)
-search_kernel(DevDom D, double beta, const double *__restrict__ r,
-  const double *__restrict__ invM, double *__restrict__ p)
+/* ------------------------- step 3 with CG control ------------------------ */
+/* (p, Ap) -> alpha = (r, z) / (p, Ap) */
+__global__ void __launch_bounds__(REDUCE_THREADS)
+reduce_alpha_kernel(const double *__restrict__ gpart, long n, CGState *st)
 {
-  ROW_PROLOGUE
-  for (int i = lane + 1; i <= D.nx; i += WAVE) {
-    long c = base + i;
-    double z = r[c]*invM[c];
-    p[c] = z + beta*p[c];
+  if (st->done) return;
+  double s = block_canonical_sum(gpart, n);
+  if (threadIdx.x == 0) {
+    st->denom = s;
+    st->alpha = st->rz/s;
+  }
+}
+
+/* (r, z) -> stopping test and beta */
+__global__ void __launch_bounds__(REDUCE_THREADS)
+reduce_beta_kernel(const double *__restrict__ gpart, long n, CGState *st,
+  int q)
+{
+  if (st->done) return;
+  double s = block_canonical_sum(gpart, n);
+  if (threadIdx.x == 0) cg_after_rz(st, s, q);
+}
+
+/* (r, z) of the initial residual -> fresh CG state; (b, b) from slot. */
+__global__ void __launch_bounds__(REDUCE_THREADS)
+reduce_init_kernel(const double *__restrict__ gpart, long n, CGState *st,
+  const double *__restrict__ sp_rhs_slot, double tol2)
+{
+  double s = block_canonical_sum(gpart, n);
+  if (threadIdx.x == 0) {
+    st->sp_rhs = *sp_rhs_slot;
+    st->rz = s;
+    st->rz1 = 0.; st->denom = 0.; st->alpha = 0.; st->beta = 0.;
+    st->tol2 = tol2;
+    st->q = 0; st->done = 0; st->nan_flag = 0;
   }
 }
 
@@ -260,52 +506,161 @@ static double now_s(void)
 
 #define ROWS_LAUNCH rows_grid(ddom.ny, ddom.nz), rows_block(), 0, bb_stream
 
+static CGState *_cg_state = NULL;
+static CGState *cg_host = NULL;
+static double *_p_alt = NULL;          /* second search-direction buffer */
+static long p_alt_n = 0;
+int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
+int bb_spmv_variant = 1;               /* 0: row kernel, 1: 2.5-D LDS kernel */
+int bb_spmv_kchunk = 8;                /* planes per 2.5-D workgroup */
+
+static void cg_buffers(void)
+{
+  if (!_cg_state) {
+    BB_CHECK(hipMalloc((void **) &_cg_state, sizeof(CGState)));
+    BB_CHECK(hipMemset(_cg_state, 0, sizeof(CGState)));
+    BB_CHECK(hipHostMalloc((void **) &cg_host, sizeof(CGState),
+      hipHostMallocDefault));
+  }
+  if (p_alt_n != ddom.cc.n) {
+    if (_p_alt) BB_CHECK(hipFree(_p_alt));
+    BB_CHECK(hipMalloc((void **) &_p_alt, ddom.cc.n * sizeof(double)));
+    BB_CHECK(hipMemset(_p_alt, 0, ddom.cc.n * sizeof(double)));
+    p_alt_n = ddom.cc.n;
+  }
+}
+
+void bb_solver_free(void)
+{
+  if (_p_alt) BB_CHECK(hipFree(_p_alt));
+  if (_cg_state) BB_CHECK(hipFree(_cg_state));
+  if (cg_host) BB_CHECK(hipHostFree(cg_host));
+  _cg_state = NULL;
+  cg_host = NULL;
+  _p_alt = NULL;
+  p_alt_n = 0;
+}
+
+static void fetch_state(void)
+{
+  BB_CHECK(hipMemcpyAsync(cg_host, _cg_state, sizeof(CGState),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  if (bb_time_kernels) bb_kt_collect();
+}
+
+/* rhs and (b, b) -> slot 0. */
 void bb_pp_rhs(void)
 {
   hipLaunchKernelGGL(rhs_kernel, ROWS_LAUNCH, ddom, 1. / dom[0].dx,
     1. / dom[0].dy, 1. / dom[0].dz, rho_f / dt, _u_star, _v_star, _w_star,
     _rhs_p, _bb_rows);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, (long) ddom.ny * ddom.nz, _bb_scalars + 0);
+  bb_reduce_rows(_bb_rows, n_groups(ddom.ny, ddom.nz), _bb_scalars + 0);
 }
 
-/* Diagnostic: sqrt(||b + A phi||^2 / (b, b)); phi ghosts must be current. */
+/* Diagnostic: sqrt(||b + A phi||^2 / (b, b)). */
 double bb_true_residual(double sp_rhs)
 {
-  bb_wrap(_phi, GCC);
   hipLaunchKernelGGL(true_residual_kernel, ROWS_LAUNCH, ddom, coef(), _phi,
     _rhs_p, _bb_rows);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, (long) ddom.ny * ddom.nz, _bb_scalars + 8);
+  bb_reduce_rows(_bb_rows, n_groups(ddom.ny, ddom.nz), _bb_scalars + 8);
   double rr = bb_fetch_scalar(8);
   return sp_rhs > 0 ? sqrt(rr) / sqrt(sp_rhs) : 0.;
 }
 
-/* The plain FP64 Jacobi-PCG loop (src/cuda_solver.cu:574-761) on the RHS
- * already in _rhs_p with (b, b) already in slot 0. */
+/* Largest x extent the 2.5-D kernel takes (two LDS planes within 64 KiB). */
+static int spmv25_nc(void)
+{
+  int nc = (ddom.nx + WAVE - 1) / WAVE;
+  size_t lds = 2 * (size_t) (T25_ROWS + 2) * (ddom.nx + 2) * sizeof(double);
+  if (nc > 6 || lds > 60 * 1024) return 0;
+  return nc;
+}
+
+template <bool FIRST>
+static void launch_spmv25(int nc, const Coef &K, const InvMTab &T,
+  const double *pold, double *pnew)
+{
+  int kc = bb_spmv_kchunk < 1 ? 1 : bb_spmv_kchunk;
+  dim3 grid((ddom.ny + T25_ROWS - 1) / T25_ROWS, (ddom.nz + kc - 1) / kc);
+  size_t lds = 2 * (size_t) (T25_ROWS + 2) * (ddom.nx + 2) * sizeof(double);
+#define L25(NC) hipLaunchKernelGGL((spmv_search25_kernel<FIRST, NC>), grid, \
+    dim3(T25_THREADS), lds, bb_stream, ddom, K, T, _cg_state, _r_q, pold,   \
+    pnew, _Apb_q, _bb_rows, kc)
+  switch (nc) {
+    case 1: L25(1); break;
+    case 2: L25(2); break;
+    case 3: L25(3); break;
+    case 4: L25(4); break;
+    case 5: L25(5); break;
+    default: L25(6); break;
+  }
+#undef L25
+}
+
+/* Enqueue iteration q of the plain PCG; p buffers alternate by parity. */
+static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
+{
+  double *pnew = (q & 1) ? _p_q : _p_alt;
+  double *pold = (q & 1) ? _p_alt : _p_q;
+  int nc = bb_spmv_variant == 1 ? spmv25_nc() : 0;
+  bb_kt_begin(KT_SPMV);
+  if (nc > 0) {
+    if (q == 1) launch_spmv25<true>(nc, K, T, pold, pnew);
+    else launch_spmv25<false>(nc, K, T, pold, pnew);
+  } else if (q == 1) {
+    hipLaunchKernelGGL(spmv_search_kernel<true>, ROWS_LAUNCH, ddom, K, T,
+      _cg_state, _r_q, pold, pnew, _Apb_q, _bb_rows);
+  } else {
+    hipLaunchKernelGGL(spmv_search_kernel<false>, ROWS_LAUNCH, ddom, K, T,
+      _cg_state, _r_q, pold, pnew, _Apb_q, _bb_rows);
+  }
+  bb_kt_end(KT_SPMV);
+  const long ng = n_groups(ddom.ny, ddom.nz);
+  hipLaunchKernelGGL(reduce_alpha_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
+    bb_stream, _bb_rows, ng, _cg_state);
+  if (q % 50 == 0) {
+    hipLaunchKernelGGL(update_solution_kernel, ROWS_LAUNCH, ddom, _cg_state,
+      pnew, _phi);
+    hipLaunchKernelGGL(residual_kernel, ROWS_LAUNCH, ddom, K, T, _cg_state,
+      _phi, _rhs_p, _r_q, _bb_rows);
+  } else {
+    bb_kt_begin(KT_UPDATE);
+    hipLaunchKernelGGL(update_kernel, ROWS_LAUNCH, ddom, T, _cg_state, pnew,
+      _Apb_q, _phi, _r_q, _bb_rows);
+    bb_kt_end(KT_UPDATE);
+  }
+  hipLaunchKernelGGL(reduce_beta_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
+    bb_stream, _bb_rows, ng, _cg_state, q);
+  BB_LAUNCH_CHECK();
+}
+
+/* The plain FP64 Jacobi-PCG (src/cuda_solver.cu:574-761) on the RHS already
+ * in _rhs_p with (b, b) in slot 0. */
 static void pcg_plain(double ts)
 {
-  const long nrows = (long) ddom.ny * ddom.nz;
   Coef K = coef();
+  InvMTab T = invm_table(K);
   const char *rname = "solver_expd.rec";
+  cg_buffers();
 
-  hipLaunchKernelGGL(cg_init_kernel, ROWS_LAUNCH, ddom, _rhs_p, _invM, _r_q,
-    _p_q, _phi, _bb_rows);
+  hipLaunchKernelGGL(cg_init_kernel, ROWS_LAUNCH, ddom, T, _rhs_p, _r_q, _phi,
+    _bb_rows);
+  hipLaunchKernelGGL(reduce_init_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
+    bb_stream, _bb_rows, n_groups(ddom.ny, ddom.nz), _cg_state,
+    _bb_scalars + 0, pp_residual*pp_residual);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, nrows, _bb_scalars + 1);
-  bb_wrap(_p_q, GCC);
-  bb_fetch_scalars(2);
-  real sp_rhs = bb_host_scalars[0];
-  real sp_rq = bb_host_scalars[1];
-  real sp_rq1 = 0.;
-  int q = 0;
+  fetch_state();
+  real sp_rhs = cg_host->sp_rhs;
   real RHS_TOL = 1.e-8;
 
   pp_last.sp_rhs = sp_rhs;
   pp_last.nouter = 1;
   if (sp_rhs < RHS_TOL*RHS_TOL) {
     double etime = now_s() - ts;
-    recorder_PP(rname, q, 0., etime);
+    recorder_PP(rname, 0, 0., etime);
     if (rank == 0)
       printf("N%d >> Norm of the rhs is less than %.1e, exiting solver\n",
         rank, RHS_TOL);
@@ -313,64 +668,35 @@ static void pcg_plain(double ts)
     pp_last.etime = etime;
     return;
   }
+  int q = 0;
   while (q <= pp_max_iter) {
-    ++q;
-    bb_kt_begin(KT_SPMV);
-    hipLaunchKernelGGL(spmv_dot_kernel, ROWS_LAUNCH, ddom, K, _p_q, _Apb_q,
-      _bb_rows);
-    bb_kt_end(KT_SPMV);
-    BB_LAUNCH_CHECK();
-    bb_reduce_rows(_bb_rows, nrows, _bb_scalars + 2);
-    real DENOM = bb_fetch_scalar(2);
-    real alpha = sp_rq/DENOM;
-
-    if (q % 50 == 0) {
-      hipLaunchKernelGGL(update_solution_kernel, ROWS_LAUNCH, ddom, alpha,
-        _p_q, _phi);
-      bb_wrap(_phi, GCC);
-      hipLaunchKernelGGL(residual_kernel, ROWS_LAUNCH, ddom, K, _phi, _rhs_p,
-        _invM, _r_q, _Apb_q, _bb_rows);
-    } else {
-      bb_kt_begin(KT_UPDATE);
-      hipLaunchKernelGGL(update_kernel, ROWS_LAUNCH, ddom, alpha, _p_q, _Apb_q,
-        _invM, _phi, _r_q, _bb_rows);
-      bb_kt_end(KT_UPDATE);
-    }
-    BB_LAUNCH_CHECK();
-    bb_reduce_rows(_bb_rows, nrows, _bb_scalars + 3);
-    sp_rq1 = bb_fetch_scalar(3);
-
-    if (sp_rq1 <= pp_residual*pp_residual*sp_rhs) {
-      double etime = now_s() - ts;
-      recorder_PP(rname, q, sqrt(sp_rq1)/sqrt(sp_rhs), etime);
-      pp_last.etime = etime;
-      break;
-    } else if (isnan(sp_rq1)) {
-      if (rank == 0) {
-        printf("N%d >> The PP equation did not converge.\n", rank);
-        printf("N%d >> The residual at iteration %d is nan (%lf).\n", rank, q,
-          sqrt(sp_rq1));
-      }
-      exit(EXIT_FAILURE);
-    } else {
-      real beta = sp_rq1/sp_rq;
-      bb_kt_begin(KT_SEARCH);
-      hipLaunchKernelGGL(search_kernel, ROWS_LAUNCH, ddom, beta, _r_q, _invM,
-        _p_q);
-      bb_kt_end(KT_SEARCH);
-      BB_LAUNCH_CHECK();
-      bb_wrap(_p_q, GCC);
-      sp_rq = sp_rq1;
-    }
+    int qend = q + bb_pcg_batch;
+    if (qend > pp_max_iter + 1) qend = pp_max_iter + 1;
+    for (int qq = q + 1; qq <= qend; qq++) enqueue_iteration(qq, K, T);
+    q = qend;
+    fetch_state();
+    if (cg_host->done) break;
   }
-  if (q > pp_max_iter) {
+  real sp_rq1 = cg_host->rz1;
+  if (cg_host->nan_flag) {
+    if (rank == 0) {
+      printf("N%d >> The PP equation did not converge.\n", rank);
+      printf("N%d >> The residual at iteration %d is nan (%lf).\n", rank,
+        cg_host->q, sqrt(sp_rq1));
+    }
+    exit(EXIT_FAILURE);
+  }
+  if (!cg_host->done) {
     printf("N%d >> The pressure-Poisson equation did not converge.\n", rank);
     printf("N%d >> (rhs, rhs) is %e\n", rank, sp_rhs);
     printf("N%d >> Residual at iteration %d is %lf\n", rank, q,
       sqrt(sp_rq1)/sqrt(sp_rhs));
     exit(EXIT_FAILURE);
   }
-  pp_last.niter = q;
+  double etime = now_s() - ts;
+  recorder_PP(rname, cg_host->q, sqrt(sp_rq1)/sqrt(sp_rhs), etime);
+  pp_last.etime = etime;
+  pp_last.niter = cg_host->q;
   pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
 }
 

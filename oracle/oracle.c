@@ -10,9 +10,10 @@
  * checked rather than shared.
  *
  * Reductions follow the canonical order documented in the library (row
- * partials over 64 interleaved lanes + halving tree, then 256 interleaved
- * accumulators + halving tree) because the reference's own order is that of
- * thrust, which is not specified; that order is the definition here.
+ * partials over 64 interleaved lanes + halving tree, groups of four rows of
+ * a plane as (r0 + r1) + (r2 + r3), then 1024 interleaved accumulators +
+ * halving tree) because the reference's own order is that of thrust, which
+ * is not specified; that order is the definition here.
  * Compile with -ffp-contract=off.
  *
  * Pinning: see tests/test_oracle.py (closed forms of src/cuda_testing.cu,
@@ -367,23 +368,36 @@ static double canon_row(const double *t, int n)
 
 double orc_canon_linear(const double *rows, long n)
 {
-  double acc[256];
-  for (int t = 0; t < 256; t++) acc[t] = 0.0;
-  for (long r = 0; r < n; r++) acc[r % 256] += rows[r];
-  for (int off = 128; off >= 1; off >>= 1)
+  double acc[1024];
+  for (int t = 0; t < 1024; t++) acc[t] = 0.0;
+  for (long r = 0; r < n; r++) acc[r % 1024] += rows[r];
+  for (int off = 512; off >= 1; off >>= 1)
     for (int t = 0; t < off; t++) acc[t] += acc[t + off];
   return acc[0];
 }
 
-/* terms: nx*ny*nz values, x-fastest, interior cells only. */
+/* terms: nx*ny*nz values, x-fastest, interior cells only.  Rows are
+ * reduced over 64 interleaved lanes, then taken four at a time within a
+ * plane as (r0 + r1) + (r2 + r3) (a row past ny is +0.0), then the group
+ * partials g = jg + k*ceil(ny/4) over 1024 interleaved accumulators. */
 double orc_canon_sum(const double *terms, int nx, int ny, int nz)
 {
-  long nrows = (long) ny * nz;
-  double *rows = dal(nrows);
+  int ngy = (ny + 3) / 4;
+  long ng = (long) ngy * nz;
+  double *grp = dal(ng);
   #pragma omp parallel for schedule(static)
-  for (long r = 0; r < nrows; r++) rows[r] = canon_row(terms + r * nx, nx);
-  double s = orc_canon_linear(rows, nrows);
-  free(rows);
+  for (long g = 0; g < ng; g++) {
+    int jg = (int) (g % ngy), k = (int) (g / ngy);
+    double r[4];
+    for (int w = 0; w < 4; w++) {
+      int j = 4 * jg + w;
+      r[w] = (j < ny) ? canon_row(terms + ((long) j + (long) k * ny) * nx, nx)
+                      : 0.0;
+    }
+    grp[g] = (r[0] + r[1]) + (r[2] + r[3]);
+  }
+  double s = orc_canon_linear(grp, ng);
+  free(grp);
   return s;
 }
 

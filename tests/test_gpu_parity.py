@@ -4,6 +4,8 @@ Bar: relative L2 <= 1e-10 on u, v, w, p (the flow path is FP64 with the same
 operation order and the same canonical reduction order on both sides, so the
 fields are expected to be bit-identical; that stronger property is asserted
 separately)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -18,8 +20,16 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def run_pair(case_dir, kw, nsteps):
+def set_knobs(variant=1, kchunk=8, batch=16):
+    lib = bbhip.load()
+    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
+    ctypes.c_int.in_dll(lib, "bb_spmv_kchunk").value = kchunk
+    ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = batch
+
+
+def run_pair(case_dir, kw, nsteps, variant=1, kchunk=8, batch=16):
     n = kw["n"]
+    set_knobs(variant, kchunk, batch)
     bbconfig.write_case(case_dir, **kw)
     out = {"gpu": {}, "orc": {}, "iters_gpu": [], "iters_orc": []}
     with orc.Oracle(case_dir) as o:
@@ -62,8 +72,9 @@ def test_cavity_32_full_100_steps(case_dir):
     assert out["iters_gpu"] == out["iters_orc"]
 
 
-def test_cavity_bitwise(case_dir):
-    out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5)
+@pytest.mark.parametrize("variant", [0, 1])
+def test_cavity_bitwise(case_dir, variant):
+    out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
@@ -77,19 +88,57 @@ def test_channel_small(case_dir):
     assert out["gpu_dt"] == out["orc_dt"]
 
 
-def test_channel_bitwise(case_dir):
-    out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3)
+@pytest.mark.parametrize("variant", [0, 1])
+def test_channel_bitwise(case_dir, variant):
+    out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-def test_ragged_sizes(case_dir):
-    """x not a multiple of the 64-lane wavefront, odd sizes, shear walls."""
+@pytest.mark.parametrize("variant,kchunk,batch", [(0, 8, 16), (1, 8, 16),
+                                                  (1, 3, 5), (1, 30, 1)])
+def test_ragged_sizes(case_dir, variant, kchunk, batch):
+    """x not a multiple of the 64-lane wavefront, y not a multiple of the
+    row group, z not a multiple of the plane chunk, shear walls."""
     kw = bbconfig.cavity(n=16, steps=3)
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     kw["vel_bc"] = bbconfig.walls(lid=("v", "E", 0.5))
-    out = run_pair(case_dir, kw, 3)
+    out = run_pair(case_dir, kw, 3, variant, kchunk, batch)
     check(out)
     assert out["iters_gpu"] == out["iters_orc"]
+    for f in FIELDS:
+        assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
+            describe(f, out["gpu"][f], out["orc"][f])
+
+
+def test_periodic_all_and_wide_x(case_dir):
+    """Triply periodic box (wrapped neighbours in every direction) with an x
+    extent past the 2.5-D kernel's limit (row-kernel fallback)."""
+    per = ("PERIODIC",)
+    vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
+    kw = dict(n=(450, 6, 5), lo=(0.0, 0.0, 0.0), hi=(4.5, 0.6, 0.5),
+              rho_f=1.0, nu=0.01, duration=1.0, pp_max_iter=100000,
+              pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
+              init_cond="QUIESCENT",
+              extensions={"fixed_dt": 1e-3, "perturb": (5, 0.2)})
+    out = run_pair(case_dir, kw, 3, variant=1)
+    check(out)
+    for f in FIELDS:
+        assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
+            describe(f, out["gpu"][f], out["orc"][f])
+
+
+def test_periodic_all_25d(case_dir):
+    per = ("PERIODIC",)
+    vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
+    kw = dict(n=(40, 12, 10), lo=(0.0, 0.0, 0.0), hi=(4.0, 1.2, 1.0),
+              rho_f=1.0, nu=0.01, duration=1.0, pp_max_iter=100000,
+              pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
+              init_cond="QUIESCENT",
+              extensions={"fixed_dt": 1e-3, "perturb": (6, 0.2)})
+    out = run_pair(case_dir, kw, 3, variant=1, kchunk=4)
+    for f in FIELDS:
+        assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
+            describe(f, out["gpu"][f], out["orc"][f])

@@ -409,24 +409,30 @@ def test_oracle_deterministic(tmp_path):
 # ------------------------------------------------------ canonical reduction
 def canonical_python(t):
     """The documented order: rows (j,k) over 64 interleaved lanes + halving
-    tree, then 256 interleaved accumulators + halving tree."""
+    tree, groups of four rows of a plane as (r0 + r1) + (r2 + r3), then 1024
+    interleaved accumulators + halving tree."""
     nx, ny, nz = t.shape
-    rows = []
+
+    def row(j, k):
+        lane = [0.0] * 64
+        for i in range(nx):
+            lane[i % 64] = lane[i % 64] + float(t[i, j, k])
+        off = 32
+        while off >= 1:
+            for l in range(off):
+                lane[l] = lane[l] + lane[l + off]
+            off //= 2
+        return lane[0]
+    groups = []
     for k in range(nz):
-        for j in range(ny):
-            lane = [0.0] * 64
-            for i in range(nx):
-                lane[i % 64] = lane[i % 64] + float(t[i, j, k])
-            off = 32
-            while off >= 1:
-                for l in range(off):
-                    lane[l] = lane[l] + lane[l + off]
-                off //= 2
-            rows.append(lane[0])
-    acc = [0.0] * 256
-    for r, v in enumerate(rows):
-        acc[r % 256] = acc[r % 256] + v
-    off = 128
+        for jg in range((ny + 3) // 4):
+            r = [row(j, k) if j < ny else 0.0
+                 for j in range(4 * jg, 4 * jg + 4)]
+            groups.append((r[0] + r[1]) + (r[2] + r[3]))
+    acc = [0.0] * 1024
+    for r, v in enumerate(groups):
+        acc[r % 1024] = acc[r % 1024] + v
+    off = 512
     while off >= 1:
         for l in range(off):
             acc[l] = acc[l] + acc[l + off]
@@ -437,6 +443,8 @@ def canonical_python(t):
 def test_canonical_sum_order():
     rng = np.random.default_rng(11)
     t = rng.standard_normal((70, 9, 5)) * 10.0 ** rng.integers(-8, 8, (70, 9, 5))
+    big = rng.standard_normal((65, 6, 900))
+    assert orc.canon_sum(big) == canonical_python(big)
     assert orc.canon_sum(t) == canonical_python(t)
     assert abs(orc.canon_sum(t) - math.fsum(t.ravel())) <= \
         1e-13 * np.abs(t).sum()
