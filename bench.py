@@ -31,15 +31,20 @@ import bbhip  # noqa: E402
 METRIC = ("timesteps/sec @256^3+500 spheres (plain FP64 vs mixed+learned); "
           "HBM GB/s")
 HBM_PEAK_GBS = 8000.0
-KT_SPMV, KT_UPDATE, KT_SEARCH = 0, 1, 2
+# Kernel-timer ids (the KT_* enum of the library).
+KT_IDS = {"spmv_search_kernel": 0, "update_kernel": 1,
+          "spmv_search_kernel_f32": 5, "update_kernel_f32": 6}
 
-# Algorithmic bytes per interior cell of each timed kernel (FP64, one element
-# per array the kernel semantically reads or writes).
+# Algorithmic bytes per interior cell of each timed kernel (one element per
+# array the kernel semantically reads or writes; FP64 = 8 B, FP32 = 4 B).
 KERNEL_BYTES = {
-    # update_kernel: reads phi, p, r, Ap; writes phi, r
+    # update: reads phi, p, r, Ap; writes phi, r
     "update_kernel": 6 * 8,
-    # spmv_search_kernel: reads r, p_old; writes p_new, Ap
+    # search + SpMV: reads r, p_old; writes p_new, Ap
     "spmv_search_kernel": 4 * 8,
+    # the mixed form's inner sweep, FP32 vectors (phi is the correction e)
+    "update_kernel_f32": 6 * 4,
+    "spmv_search_kernel_f32": 4 * 4,
 }
 
 
@@ -55,11 +60,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
-                    help="0: row SpMV kernel, 1: 2.5-D LDS kernel (default)")
-    ap.add_argument("--kchunk", type=int, default=None,
-                    help="planes per workgroup of the 2.5-D SpMV")
+                    help="0: SpMV with direct neighbour loads, 1: LDS-staged "
+                         "planes (default)")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
+    ap.add_argument("--time-every", type=int, default=8,
+                    help="HIP-event timing on one launch in this many of the "
+                         "timed kernels (0: no event timing)")
     ap.add_argument("--calib-only", action="store_true",
                     help="run only the stream-copy calibration kernel "
                          "(2^27 doubles per array) and print its GB/s")
@@ -99,7 +106,7 @@ def main():
                           "elements": 1 << 27}))
         return
     for name, val in (("bb_spmv_variant", a.spmv_variant),
-                      ("bb_spmv_kchunk", a.kchunk), ("bb_pcg_batch", a.batch)):
+                      ("bb_pcg_batch", a.batch)):
         if val is not None:
             ctypes.c_int.in_dll(lib, name).value = val
     n = (a.nx, a.ny, a.nz)
@@ -113,7 +120,8 @@ def main():
         iters.append(sim.pp_stats()["niter"])
     sim.sync()
     lib.bb_ktimer_reset()
-    lib.bb_ktimer_enable(1)
+    ctypes.c_int.in_dll(lib, "bb_time_every").value = max(a.time_every, 1)
+    lib.bb_ktimer_enable(1 if a.time_every > 0 else 0)
     sim.sync()
     t0 = time.perf_counter()
     timed_iters = []
@@ -126,19 +134,35 @@ def main():
     ms = ctypes.c_double()
     cnt = ctypes.c_long()
     ktimes = {}
-    for name, kid in (("update_kernel", KT_UPDATE),
-                      ("spmv_search_kernel", KT_SPMV)):
+    for name, kid in KT_IDS.items():
         lib.bb_ktimer_read(kid, ctypes.byref(ms), ctypes.byref(cnt))
-        ktimes[name] = (ms.value, cnt.value)
+        if cnt.value > 0:
+            ktimes[name] = (ms.value, cnt.value)
     stats = sim.pp_stats()
     sim.teardown()
 
     elapsed = t1 - t0
     cells = a.nx * a.ny * a.nz
-    dom_name = max(ktimes, key=lambda k: ktimes[k][0])
-    kms, kn = ktimes[dom_name]
-    avg_s = (kms / 1e3) / max(kn, 1)
-    achieved = KERNEL_BYTES[dom_name] * cells / avg_s / 1e9
+    roofline = None
+    if ktimes:
+        dom_name = max(ktimes, key=lambda k: ktimes[k][0])
+        kms, kn = ktimes[dom_name]
+        avg_s = (kms / 1e3) / kn
+        achieved = KERNEL_BYTES[dom_name] * cells / avg_s / 1e9
+        roofline = {
+            "kernel": dom_name,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "frac_of_measured_copy": achieved / copy_gbs,
+            "measured_copy_gbs": copy_gbs,
+            "traffic": None,
+            "bytes_per_cell": KERNEL_BYTES[dom_name],
+            "avg_launch_us": avg_s * 1e6,
+            "launches_timed": kn,
+        }
     out = {
         "metric": METRIC,
         "value": a.steps / elapsed,
@@ -159,24 +183,12 @@ def main():
             "pp_precision": a.precision,
             "pcg_iters_per_step": sum(timed_iters) / max(len(timed_iters), 1),
             "spmv_variant": ctypes.c_int.in_dll(lib, "bb_spmv_variant").value,
-            "spmv_kchunk": ctypes.c_int.in_dll(lib, "bb_spmv_kchunk").value,
+            "last_resid": stats["resid"],
             "last_true_resid": stats["true_resid"],
+            "last_corrections": stats["nouter"],
         },
-        "roofline": {
-            "kernel": dom_name,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "frac_of_measured_copy": achieved / copy_gbs,
-            "measured_copy_gbs": copy_gbs,
-            "traffic": None,
-            "bytes_per_cell": KERNEL_BYTES[dom_name],
-            "avg_launch_us": avg_s * 1e6,
-            "launches": kn,
-        },
-        "kernels_ms": {k: v[0] for k, v in ktimes.items()},
+        "roofline": roofline,
+        "kernels_avg_us": {k: 1e3 * v[0] / v[1] for k, v in ktimes.items()},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case)

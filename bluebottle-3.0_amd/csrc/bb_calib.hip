@@ -9,10 +9,8 @@
 __global__ void __launch_bounds__(256)
 stream_copy_kernel(const double *__restrict__ a, double *__restrict__ b, long n)
 {
-  long stride = (long) gridDim.x * blockDim.x;
-  for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += stride)
-    b[i] = a[i];
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 /* Returns the best-of-`reps` copy bandwidth in GB/s (1e9 bytes/s). */
@@ -26,11 +24,11 @@ extern "C" double bb_stream_copy_gbs(long n, int reps)
   hipEvent_t e0, e1;
   BB_CHECK(hipEventCreate(&e0));
   BB_CHECK(hipEventCreate(&e1));
-  int blocks = 256 * 8 * 4;
+  long blocks = (n + 255) / 256;
   double best = 0.;
   for (int r = 0; r < reps + 1; r++) {
     BB_CHECK(hipEventRecord(e0, bb_stream));
-    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0,
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned) blocks), dim3(256), 0,
       bb_stream, a, b, n);
     BB_CHECK(hipEventRecord(e1, bb_stream));
     BB_CHECK(hipEventSynchronize(e1));

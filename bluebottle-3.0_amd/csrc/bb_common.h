@@ -9,21 +9,24 @@
  *
  * Canonical reduction order.  Every sum over interior cells (the PCG dot
  * products, the pressure mean) is formed the same way so that the result is
- * bit-reproducible and can be mirrored exactly on the host:
- *   1. row partial: for each row (j, k) lane l of a 64-lane wavefront adds
- *      the terms at x-offsets l, l+64, l+128, ... in increasing order,
- *      starting from +0.0; then the 64 lane values are combined by the
- *      halving tree  v[l] += v[l + off]  for off = 32, 16, ..., 1; the row
- *      partial is v[0];
- *   2. group partial: the rows of one plane k are taken four at a time,
- *      j = 4g+1 .. 4g+4 (a row past ny counts as +0.0), and combined as
- *      (r0 + r1) + (r2 + r3); groups are numbered g + (k-1)*ceil(ny/4);
- *   3. the group partials are combined by one 1024-thread workgroup: thread
- *      t adds groups t, t+1024, ... in increasing order from +0.0, then the
- *      halving tree over 1024 threads (off = 512, ..., 1) leaves the total
- *      in thread 0.  Step 3 is its own small launch: the kernel boundary is
- *      what makes every workgroup's partial visible across the XCDs (a
- *      device-scope fence per workgroup would write back L2 each time).
+ * bit-reproducible and can be mirrored exactly on the host.  The domain is
+ * cut into tiles of 64 cells in x by 8 rows in y by 8 planes in z; tile
+ * (tc, tj, tk) is numbered tc + ntx*(tj + nty*tk) with ntx = ceil(nx/64),
+ * nty = ceil(ny/8):
+ *   1. for each (x, y) column of a tile, one lane adds the terms of the
+ *      tile's planes in increasing z, starting from +0.0;
+ *   2. the 64 lane values of a row are combined by the halving tree
+ *      v[l] += v[l + off], off = 32, 16, ..., 1 (a cell outside the domain
+ *      counts as +0.0);
+ *   3. the 8 row values of the tile are combined as
+ *      ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7));
+ *   4. the tile partials are combined by 1024 accumulators: accumulator a
+ *      adds tiles a, a+1024, ... in increasing order from +0.0, then the
+ *      halving tree over the 1024 accumulators (off = 512, ..., 1).
+ * Steps 1-3 are the epilogue of the producing kernel (one 512-thread
+ * workgroup per tile, one wavefront per row); step 4 runs either as its own
+ * launch or redundantly at the start of every workgroup of the consuming
+ * kernel, which then all hold the same bits.
  * All arithmetic is IEEE binary64 with FMA contraction disabled
  * (-ffp-contract=off), so one multiply and one add are rounded separately
  * exactly as a plain C compiler does on the host.
@@ -84,13 +87,8 @@ extern double *_bb_rows2;             /* second row-partial buffer */
 extern double *_bb_scalars;           /* small device result slots */
 extern double *bb_host_scalars;       /* pinned host mirror of the slots */
 
-/* Interior-row kernel shape: one workgroup per row group (four rows of one
- * plane), one wavefront per row. */
-__host__ __device__ inline long n_groups(int ny, int nz)
-{
-  return (long) ((ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK) * nz;
-}
-inline dim3 rows_grid(int ny, int nz) { return dim3((unsigned) n_groups(ny, nz)); }
+/* Row-kernel shape (order-free work such as maxima): four rows per
+ * workgroup, one wavefront per row. */
 inline dim3 rows_block() { return dim3(WAVE * ROWS_PER_BLOCK); }
 
 /* Row partial tree over one wavefront (step 1 of the canonical order).  The
@@ -131,34 +129,94 @@ __device__ inline double block_canonical_sum(const double *__restrict__ part,
   return v;
 }
 
-/* Steps 1-2 for a 4-wavefront workgroup that owns one row group: thread 0
- * writes the group partial.  All threads must call it with their lane's row
- * accumulator (+0.0 for a row past the end). */
-__device__ inline void group_partial(double rowacc, double *__restrict__ gpart)
-{
-  __shared__ double s_rows[ROWS_PER_BLOCK];
-  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
-  double v = wave_tree_sum(rowacc);
-  if (lane == 0) s_rows[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    gpart[blockIdx.x] = (s_rows[0] + s_rows[1]) + (s_rows[2] + s_rows[3]);
-}
+/* ----------------------------- tiles ------------------------------------ */
+#define TILE_X 64
+#define TILE_Y 8
+#define TILE_Z 8
+#define TILE_THREADS (TILE_X * TILE_Y)
 
-/* Group-kernel prologue: this workgroup's plane k and row group, this
- * wavefront's row j (rowok false past ny). */
-#define GROUP_PROLOGUE                                                        \
+struct Tiles {
+  int ntx, nty, ntz;
+};
+
+inline Tiles tiles_of(const DevDom &D)
+{
+  Tiles T;
+  T.ntx = (D.nx + TILE_X - 1) / TILE_X;
+  T.nty = (D.ny + TILE_Y - 1) / TILE_Y;
+  T.ntz = (D.nz + TILE_Z - 1) / TILE_Z;
+  return T;
+}
+inline long n_tiles(const DevDom &D)
+{
+  Tiles T = tiles_of(D);
+  return (long) T.ntx * T.nty * T.ntz;
+}
+inline dim3 tiles_grid(const DevDom &D)
+{
+  Tiles T = tiles_of(D);
+  return dim3(T.ntx, T.nty, T.ntz);
+}
+inline dim3 tiles_block() { return dim3(TILE_THREADS); }
+
+/* Tile-kernel prologue: this thread's cell column (x, j), the plane range
+ * of the tile, and the tile number. */
+#define TILE_PROLOGUE                                                         \
   const Grid &C = D.cc;                                                       \
   int lane = threadIdx.x & (WAVE - 1);                                        \
-  int ngy_ = (D.ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;                    \
-  int k = (int) (blockIdx.x / ngy_) + 1;                                      \
-  int j = (int) (blockIdx.x % ngy_) * ROWS_PER_BLOCK + (threadIdx.x >> 6) + 1; \
-  bool rowok = (j <= D.ny);                                                   \
-  int nxr = rowok ? D.nx : 0;                                                 \
-  long base = gidx(C, 0, rowok ? j : D.ny, k);
+  int wv = threadIdx.x >> 6;                                                  \
+  int x = blockIdx.x * TILE_X + lane + 1;                                     \
+  int j = blockIdx.y * TILE_Y + wv + 1;                                       \
+  int k0 = blockIdx.z * TILE_Z + 1;                                           \
+  int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;                   \
+  bool colok = (x <= D.nx) && (j <= D.ny);                                    \
+  long tile = blockIdx.x + (long) gridDim.x *                                 \
+              (blockIdx.y + (long) gridDim.y * blockIdx.z);                   \
+  (void) tile; (void) colok;
+
+/* Steps 2-3: thread 0 writes the tile partial.  Every thread calls it. */
+__device__ inline void tile_partial(double acc, double *__restrict__ gpart,
+  long tile)
+{
+  __shared__ double s_w[TILE_Y];
+  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  double v = wave_tree_sum(acc);
+  if (lane == 0) s_w[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    gpart[tile] = ((s_w[0] + s_w[1]) + (s_w[2] + s_w[3])) +
+                  ((s_w[4] + s_w[5]) + (s_w[6] + s_w[7]));
+}
+
+/* Step 4 inside a TILE_THREADS (512) workgroup: thread t plays accumulators
+ * t and t + 512.  Returns the total to every thread. */
+__device__ inline double tile_block_sum(const double *__restrict__ part, long n)
+{
+  __shared__ double s[TILE_THREADS];
+  __shared__ double s_tot;
+  const long RT = REDUCE_THREADS;
+  int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+#pragma unroll 4
+  for (long q = t; q < n; q += RT) a += part[q];
+#pragma unroll 4
+  for (long q = t + TILE_THREADS; q < n; q += RT) b += part[q];
+  s[t] = a + b;
+  __syncthreads();
+  for (int off = TILE_THREADS / 2; off >= WAVE; off >>= 1) {
+    if (t < off) s[t] += s[t + off];
+    __syncthreads();
+  }
+  if (t < WAVE) {
+    double v = wave_tree_sum(s[t]);
+    if (t == 0) s_tot = v;
+  }
+  __syncthreads();
+  return s_tot;
+}
 
 /* Host-side launchers implemented in bb_reduce.hip.  bb_reduce_rows runs
- * step 3 alone over `nrows` partials (face sums and group partials). */
+ * the final stage alone over `nrows` partials (face sums, tile partials). */
 void bb_reduce_rows(const double *rows, long nrows, double *out_slot);
 double bb_fetch_scalar(int slot);
 void bb_fetch_scalars(int n);
@@ -172,7 +230,7 @@ void bb_kt_begin(int id);
 void bb_kt_end(int id);
 void bb_kt_collect(void);     /* call after a stream synchronisation */
 
-/* Ghost refresh (periodic planes) and BCs, bb_bc.hip. */
+/* Ghost refresh (periodic planes), BCs and flags, bb_bc.hip. */
 void bb_wrap(double *a, int grid);
 void bb_bc_p(double *a);
 void bb_phase_fluid(void);

@@ -434,22 +434,24 @@ extern "C" void cuda_project(void)
 }
 
 /* --------------------------- pressure update ---------------------------- */
-/* p = (phase < 0) (p0 + phi) on interior cells, with the group partials of
+/* p = (phase < 0) (p0 + phi) on interior cells, with the tile partials of
  * p for the mean. */
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TILE_THREADS)
 update_p_kernel(DevDom D, const double *__restrict__ p0,
   const double *__restrict__ ph, const int *__restrict__ phase,
   double *__restrict__ p, double *__restrict__ gpart)
 {
-  GROUP_PROLOGUE
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    long c = base + i;
-    double val = (double) (phase[c] < 0) * (p0[c] + ph[c]);
-    p[c] = val;
-    acc += val;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double val = (double) (phase[c] < 0) * (p0[c] + ph[c]);
+      p[c] = val;
+      acc += val;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
 __global__ void __launch_bounds__(256)
@@ -469,10 +471,10 @@ extern "C" void cuda_update_p(void)
   /* src/cuda_bluebottle.cu:2506-2535 (the unused Laplacian pass of
    * update_p_laplacian is not evaluated). */
   const DevDom &D = ddom;
-  hipLaunchKernelGGL(update_p_kernel, rows_grid(D.ny, D.nz), rows_block(), 0,
+  hipLaunchKernelGGL(update_p_kernel, tiles_grid(D), tiles_block(), 0,
     bb_stream, D, _p0, _phi, _phase, _p, _bb_rows);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, n_groups(D.ny, D.nz), _bb_scalars + 0);
+  bb_reduce_rows(_bb_rows, n_tiles(D), _bb_scalars + 0);
   real pmean = bb_fetch_scalar(0);
   pmean /= (real) DOM.Gcc.s3;
   hipLaunchKernelGGL(add_const_kernel, dim3((D.cc.ni + 63) / 64,

@@ -169,7 +169,7 @@ extern "C" void cuda_dom_malloc_dev(void)
   _phase = (int *) dev_alloc(ddom.cc.n * sizeof(int));
   _phase_shell = (int *) dev_alloc(ddom.cc.n * sizeof(int));
 
-  long rows = (long) ddom.ny * ddom.nz;
+  long rows = (long) ddom.ny * ddom.nz + n_tiles(ddom);
   long faces = (long) (ddom.nx + 1) * (ddom.ny + 1);
   long cap = rows > faces ? rows : faces;
   faces = (long) (ddom.ny + 1) * (ddom.nz + 1);

@@ -40,14 +40,19 @@ double bb_fetch_scalar(int slot)
 
 /* ------------------------------ kernel timing ---------------------------- */
 int bb_time_kernels = 0;
+int bb_time_every = 8;                 /* time one launch in this many */
 static hipEvent_t kt_a[KT_COUNT], kt_b[KT_COUNT];
-static int kt_pending[KT_COUNT], kt_init = 0;
+static int kt_pending[KT_COUNT], kt_init = 0, kt_active[KT_COUNT];
+static long kt_seen[KT_COUNT];
 static double kt_ms[KT_COUNT];
 static long kt_n[KT_COUNT];
 
 void bb_kt_begin(int id)
 {
+  kt_active[id] = 0;
   if (!bb_time_kernels) return;
+  if (bb_time_every > 1 && (kt_seen[id]++ % bb_time_every) != 0) return;
+  kt_active[id] = 1;
   if (!kt_init) {
     for (int i = 0; i < KT_COUNT; i++) {
       BB_CHECK(hipEventCreate(&kt_a[i]));
@@ -61,7 +66,7 @@ void bb_kt_begin(int id)
 
 void bb_kt_end(int id)
 {
-  if (!bb_time_kernels) return;
+  if (!bb_time_kernels || !kt_active[id]) return;
   BB_CHECK(hipEventRecord(kt_b[id], bb_stream));
   kt_pending[id] = 1;
 }
@@ -91,7 +96,7 @@ extern "C" void bb_ktimer_read(int id, double *ms, long *n)
 extern "C" void bb_ktimer_reset(void)
 {
   bb_kt_collect();
-  for (int i = 0; i < KT_COUNT; i++) { kt_ms[i] = 0.; kt_n[i] = 0; }
+  for (int i = 0; i < KT_COUNT; i++) { kt_ms[i] = 0.; kt_n[i] = 0; kt_seen[i] = 0; }
 }
 
 extern "C" void bb_ktimer_enable(int on)

@@ -6,33 +6,40 @@
  * iterations.  Every rounded value is the reference's; what changes is where
  * values live and how many passes over HBM an iteration takes.
  *
- * Kernels (one wavefront per (j, k) row, lanes stride along x):
- *   rhs_kernel          rhs = -(rho/dt) div u*          + row partials (b,b)
- *   cg_init_kernel      r = b, phi = 0                   + row partials (r,z)
- *   spmv_search_kernel  p = z + beta p_old, Ap = (-A) p  + row partials (p,Ap)
- *   update_kernel       phi += a p, r -= a Ap            + row partials (r,z)
- *   reduce_*_kernel     canonical second stage; alpha, beta and the stopping
- *                       test are evaluated on the device
- * z = M^-1 r is never stored (it is recomputed, bit-identically, where it is
- * used); M^-1 comes from a 27-entry table indexed by the number of
- * non-boundary faces per direction; the search update is folded into the
- * SpMV, which reads periodic neighbours by wrapped index, so the search
- * direction needs no ghost refresh.  Per iteration: 80 B per cell of HBM
- * traffic (the reference's kernel sequence moves 136).
+ * One iteration q is two launches over 64 x 8 x 8 cell tiles (one 512-thread
+ * workgroup per tile, one wavefront per row, each lane marching z):
+ *   search_spmv   [start: (r, z) of the last update -> stopping test, beta]
+ *                 p = z + beta p_old, Ap = (-A) p   + tile partials (p, Ap)
+ *   update        [start: (p, Ap) -> alpha]
+ *                 phi += alpha p, r -= alpha Ap     + tile partials (r, z)
+ * The final reduction stage runs redundantly at the start of every
+ * workgroup of the consuming launch (all get the same bits), so alpha, beta
+ * and the stopping test cost no launch of their own.  z = M^-1 r is never
+ * stored; M^-1 comes from a 27-entry table indexed by the number of
+ * non-boundary faces per direction; search_spmv forms p_new for the tile,
+ * its halo rows, columns and planes in LDS (periodic neighbours by wrapped
+ * index), so the search direction needs no ghost refresh.  HBM traffic per
+ * iteration: 80 bytes per cell (the reference's kernel sequence moves 136).
  *
  * The 7-point weights flag^2 are 0 exactly on faces of a non-periodic
  * external boundary and 1 elsewhere (src/particle_kernel.cu:481-576 writes
  * only +-1 or, at external boundaries, 0), so they are computed from the
  * cell index.
+ *
+ * The host enqueues iterations in batches and reads the device-resident CG
+ * state once per batch; every launch returns at once when the state says
+ * done, so the iterates stop exactly where the reference's host loop breaks.
  */
 #include <math.h>
 #include <sys/time.h>
 
 #include "bb_common.h"
 
-struct Coef {
-  double idx2, idy2, idz2;
+/* Stencil weights 1/dx^2, ... in the arithmetic type V of the vectors. */
+template <typename V> struct CoefT {
+  V idx2, idy2, idz2;
 };
+typedef CoefT<double> Coef;
 
 static Coef coef(void)
 {
@@ -43,37 +50,22 @@ static Coef coef(void)
   return K;
 }
 
-/* Row-constant wall weights for row (j, k). */
-struct RowW {
-  double fn2, fs2, ft2, fb2;
-};
-
-__device__ inline RowW row_weights(const DevDom &D, int j, int k)
+static CoefT<float> coef32(const Coef &K)
 {
-  RowW W;
-  W.fs2 = (!D.pery && j == 1) ? 0. : 1.;
-  W.fn2 = (!D.pery && j == D.ny) ? 0. : 1.;
-  W.fb2 = (!D.perz && k == 1) ? 0. : 1.;
-  W.ft2 = (!D.perz && k == D.nz) ? 0. : 1.;
-  return W;
+  CoefT<float> F;
+  F.idx2 = (float) K.idx2;
+  F.idy2 = (float) K.idy2;
+  F.idz2 = (float) K.idz2;
+  return F;
 }
-
-/* --------------------- device-resident CG scalars ------------------------ */
-/* The host enqueues iterations in batches without waiting; these scalars
- * live in device memory and every kernel of an iteration returns at once
- * when `done` is set, so the iterates stop exactly where the reference's
- * host loop breaks. */
-struct CGState {
-  double sp_rhs, rz, rz1, denom, alpha, beta, tol2, diag;
-  int q, done, nan_flag, pad;
-};
 
 /* Jacobi diagonal by the count of non-boundary faces per direction (0..2):
  * invM = -1/M, M = -idx2 wx - idy2 wy - idz2 wz (src/solver_kernel.cu:66-77),
  * evaluated once on the host with the same operand order. */
-struct InvMTab {
-  double v[27];
+template <typename V> struct InvMT {
+  V v[27];
 };
+typedef InvMT<double> InvMTab;
 
 static InvMTab invm_table(const Coef &K)
 {
@@ -87,6 +79,13 @@ static InvMTab invm_table(const Coef &K)
   return T;
 }
 
+static InvMT<float> invm_table32(const InvMTab &T)
+{
+  InvMT<float> F;
+  for (int i = 0; i < 27; i++) F.v[i] = (float) T.v[i];
+  return F;
+}
+
 /* Non-boundary face count of cell index `a` in a direction with n cells. */
 __device__ inline int wcount(int a, int n, int per)
 {
@@ -94,406 +93,493 @@ __device__ inline int wcount(int a, int n, int per)
   return (a == 1 ? 0 : 1) + (a == n ? 0 : 1);
 }
 
+template <typename V>
+__device__ inline V invm_at(const InvMT<V> &T, const DevDom &D, int x, int j,
+  int k)
+{
+  return T.v[wcount(x, D.nx, D.perx) + 3 * wcount(j, D.ny, D.pery) +
+             9 * wcount(k, D.nz, D.perz)];
+}
+
 /* Neighbour index with periodic wrap (ghost index in a wall direction). */
 __device__ inline int wrap_lo(int a, int n, int per) { return (per && a == 1) ? n : a - 1; }
 __device__ inline int wrap_hi(int a, int n, int per) { return (per && a == n) ? 1 : a + 1; }
 
+/* Face weight flag^2: 0 on a non-periodic boundary face, else 1. */
+__device__ inline double fw_lo(int a, int per) { return (!per && a == 1) ? 0. : 1.; }
+__device__ inline double fw_hi(int a, int n, int per) { return (!per && a == n) ? 0. : 1.; }
+
+/* The device-resident CG state.  rzs[q & 1] holds (r, z) as it stands at
+ * the start of iteration q; launches of iteration q read rzs[(q-1) & 1] and
+ * write rzs[q & 1], never the slot they read. */
+struct CGState {
+  double sp_rhs, tol2, rz1;
+  double rzs[2];
+  double rz_ref;      /* mixed form: (r, z) at the last FP64 correction */
+  double relup;       /* mixed form: inner reduction of (r, z) per sweep */
+  int q, done, nan_flag, inner_done;
+};
+
 /* ------------------------------ Jacobi ---------------------------------- */
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(TILE_THREADS)
 jacobi_kernel(DevDom D, InvMTab T, double *__restrict__ invM)
 {
   /* src/solver_kernel.cu:26-87: the stored preconditioner of the C ABI. */
-  GROUP_PROLOGUE
-  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
-  for (int i = lane + 1; i <= nxr; i += WAVE)
-    invM[base + i] = T.v[wcount(i, D.nx, D.perx) + 3 * wy + 9 * wz];
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++)
+    invM[gidx(C, x, j, k)] = invm_at(T, D, x, j, k);
 }
 
 extern "C" void cuda_PP_init_jacobi_preconditioner(void)
 {
-  hipLaunchKernelGGL(jacobi_kernel, rows_grid(ddom.ny, ddom.nz), rows_block(),
-    0, bb_stream, ddom, invm_table(coef()), _invM);
+  hipLaunchKernelGGL(jacobi_kernel, tiles_grid(ddom), tiles_block(), 0,
+    bb_stream, ddom, invm_table(coef()), _invM);
   BB_LAUNCH_CHECK();
 }
 
 /* ------------------------------- RHS ------------------------------------ */
-/* rhs = -(rho/dt) div u* (src/solver_kernel.cu:89-176); (b, b) -> *out. */
-__global__ void __launch_bounds__(256)
+/* rhs = -(rho/dt) div u* (src/solver_kernel.cu:89-176) + tile partials of
+ * (b, b). */
+__global__ void __launch_bounds__(TILE_THREADS)
 rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
   const double *__restrict__ us, const double *__restrict__ vs,
   const double *__restrict__ ws, double *__restrict__ rhs,
   double *__restrict__ gpart)
 {
-  GROUP_PROLOGUE
-  long bu = gidx(D.fx, 0, rowok ? j : 1, k), bv = gidx(D.fy, 0, rowok ? j : 1, k),
-       bw = gidx(D.fz, 0, rowok ? j : 1, k);
-  long vsy = D.fy.sy, wsz = D.fz.sz;
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    double s = (us[bu + i + 1] - us[bu + i]) * idx;
-    s += (vs[bv + i + vsy] - vs[bv + i]) * idy;
-    s += (ws[bw + i + wsz] - ws[bw + i]) * idz;
-    s *= rho_idt;
-    double b = -s;
-    rhs[base + i] = b;
-    acc += b * b;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long cu = gidx(D.fx, x, j, k), cv = gidx(D.fy, x, j, k),
+           cw = gidx(D.fz, x, j, k);
+      double s = (us[cu + 1] - us[cu]) * idx;
+      s += (vs[cv + D.fy.sy] - vs[cv]) * idy;
+      s += (ws[cw + D.fz.sz] - ws[cw]) * idz;
+      s *= rho_idt;
+      double b = -s;
+      rhs[gidx(C, x, j, k)] = b;
+      acc += b * b;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
 /* ------------------------------ CG init --------------------------------- */
-/* r = b, phi = 0 (src/solver_kernel.cu:258-283) + group partials of (r, z).
- * p = z is formed by the first spmv_search pass. */
-__global__ void __launch_bounds__(256)
+/* r = b, phi = 0 (src/solver_kernel.cu:258-283) + tile partials of (r, z);
+ * p = z is formed by the first search_spmv pass. */
+__global__ void __launch_bounds__(TILE_THREADS)
 cg_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
   double *__restrict__ r, double *__restrict__ phi,
   double *__restrict__ gpart)
 {
-  GROUP_PROLOGUE
-  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    long c = base + i;
-    double t = rhs[c];
-    double t2 = t * T.v[wcount(i, D.nx, D.perx) + 3 * wy + 9 * wz];
-    r[c] = t;
-    phi[c] = 0.;
-    acc += t * t2;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double t = rhs[c];
+      double t2 = t * invm_at(T, D, x, j, k);
+      r[c] = t;
+      phi[c] = 0.;
+      acc += t * t2;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
-/* p_new = z + beta p_old with z = r invM (src/solver_kernel.cu:906-927). */
-template <bool FIRST>
-__device__ inline double p_new(const double *__restrict__ r,
-  const double *__restrict__ pold, long c, double invm, double beta)
+/* (b, b) and (r, z) -> a fresh CG state. */
+__global__ void __launch_bounds__(TILE_THREADS)
+cg_state_init_kernel(const double *__restrict__ bpart,
+  const double *__restrict__ rpart, long n, CGState *st, double tol2,
+  double relup)
 {
-  double z = r[c]*invm;
+  double sb = tile_block_sum(bpart, n);
+  double sr = tile_block_sum(rpart, n);
+  if (threadIdx.x == 0) {
+    st->sp_rhs = sb;
+    st->tol2 = tol2;
+    st->rz1 = sr;
+    st->rzs[0] = 0.;
+    st->rzs[1] = sr;
+    st->rz_ref = sr;
+    st->relup = relup;
+    st->q = 0;
+    st->done = 0;
+    st->nan_flag = 0;
+    st->inner_done = 0;
+  }
+}
+
+/* ------------------------ stopping test and beta ------------------------- */
+/* (src/cuda_solver.cu:229-267) run by every workgroup of a launch on the
+ * same bits; only workgroup 0 writes.  Returns false when the solve is over
+ * (converged or NaN), else sets *beta.  In the mixed form the test ends the
+ * inner FP32 sweep instead (inner_done), either at the target or once (r, z)
+ * has fallen by the factor relup since the last FP64 correction; the host
+ * then forms the FP64 residual and decides. */
+template <bool MIXED>
+__device__ inline bool cg_continue(CGState *st, const double *__restrict__ rpart,
+  long n, int q, double *beta)
+{
+  double s = tile_block_sum(rpart, n);
+  bool first = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && threadIdx.x == 0;
+  bool stop = s <= st->tol2*st->sp_rhs || isnan(s);
+  if (MIXED && !stop) stop = s <= st->relup*st->rz_ref;
+  if (stop) {
+    if (first) {
+      st->rz1 = s;
+      st->q = q - 1;
+      st->nan_flag = isnan(s) ? 1 : 0;
+      if (MIXED) st->inner_done = 1;
+      else st->done = 1;
+    }
+    return false;
+  }
+  *beta = s/st->rzs[(q - 1) & 1];
+  if (first) {
+    st->rzs[q & 1] = s;
+    st->rz1 = s;
+  }
+  return true;
+}
+
+/* ------------------------------ search_spmv ------------------------------ */
+/* The vector kernels are templates on the storage and arithmetic type V of
+ * the CG vectors: double for the plain form, float for the inner sweeps of
+ * the mixed form.  Dot-product terms are always formed and summed in FP64
+ * (for V = double the code below is the plain form's arithmetic). */
+
+/* p_new = z + beta p_old with z = r invM (src/solver_kernel.cu:906-927). */
+template <bool FIRST, typename V>
+__device__ inline V p_new(const V *__restrict__ r, const V *__restrict__ pold,
+  long c, V invm, V beta)
+{
+  V z = r[c]*invm;
   if (FIRST) return z;
   return z + beta*pold[c];
 }
 
-/* Iteration kernel 1: the search update fused into the SpMV.  Computes
- * p_new at the cell and its six neighbours from r and p_old (periodic
- * neighbours by wrapped index, so no ghost refresh of p is needed), writes
- * p_new and Ap = (-A) p_new, and the group partials of (p, Ap). */
-template <bool FIRST>
-__global__ void __launch_bounds__(256)
-spmv_search_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
-  const double *__restrict__ r, const double *__restrict__ pold,
-  double *__restrict__ pnew, double *__restrict__ Ap,
-  double *__restrict__ gpart)
-{
-  if (st->done) return;
-  GROUP_PROLOGUE
-  double beta = FIRST ? 0. : st->beta;
-  int jr = rowok ? j : 1;
-  RowW W = row_weights(D, jr, k);
-  int jm = wrap_lo(jr, D.ny, D.pery), jp = wrap_hi(jr, D.ny, D.pery);
-  int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
-  long bS = gidx(C, 0, jm, k), bN = gidx(C, 0, jp, k);
-  long bB = gidx(C, 0, jr, km), bT = gidx(C, 0, jr, kp);
-  int wy = wcount(jr, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
-  int wyS = wcount(jm, D.ny, D.pery), wyN = wcount(jp, D.ny, D.pery);
-  int wzB = wcount(km, D.nz, D.perz), wzT = wcount(kp, D.nz, D.perz);
-  double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    int im = wrap_lo(i, D.nx, D.perx), ip = wrap_hi(i, D.nx, D.perx);
-    int wx = wcount(i, D.nx, D.perx);
-    int wxW = wcount(im, D.nx, D.perx), wxE = wcount(ip, D.nx, D.perx);
-    double pC = p_new<FIRST>(r, pold, base + i, T.v[wx + 3*wy + 9*wz], beta);
-    double pW = p_new<FIRST>(r, pold, base + im, T.v[wxW + 3*wy + 9*wz], beta);
-    double pE = p_new<FIRST>(r, pold, base + ip, T.v[wxE + 3*wy + 9*wz], beta);
-    double pS = p_new<FIRST>(r, pold, bS + i, T.v[wx + 3*wyS + 9*wz], beta);
-    double pN = p_new<FIRST>(r, pold, bN + i, T.v[wx + 3*wyN + 9*wz], beta);
-    double pB = p_new<FIRST>(r, pold, bB + i, T.v[wx + 3*wy + 9*wzB], beta);
-    double pT = p_new<FIRST>(r, pold, bT + i, T.v[wx + 3*wy + 9*wzT], beta);
-    double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
-    double fw2 = (!D.perx && i == 1) ? 0. : 1.;
-    double a = -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
-               -K.idy2 * (W.fn2*(pN - pC) - W.fs2*(pC - pS))
-               -K.idz2 * (W.ft2*(pT - pC) - W.fb2*(pC - pB));
-    pnew[base + i] = pC;
-    Ap[base + i] = a;
-    acc += pC * a;
-  }
-  group_partial(acc, gpart);
-}
+#define LX (TILE_X + 2)
+#define LY (TILE_Y + 2)
+#define LPLANE (LX * LY)
 
-/* The same pass, 2.5-D: a workgroup of 8 wavefronts owns rows j0..j0+7
- * (two row groups) and marches a chunk of planes in z.  For every plane it
- * forms p_new once per cell, halo rows and columns included, into one of two
- * LDS plane buffers; x and y neighbours come from LDS, z neighbours from
- * registers carried along the march.  Same arithmetic and the same row,
- * group and final reduction order as spmv_search_kernel. */
-#define T25_ROWS 8
-#define T25_THREADS (WAVE * T25_ROWS)
-
-template <bool FIRST>
-__device__ inline void fill_plane25(const DevDom &D, const InvMTab &T,
-  const double *__restrict__ r, const double *__restrict__ pold, double beta,
-  double *__restrict__ buf, int j0, int kk)
+/* p_new over the tile's rows/columns plus a one-cell halo at plane kk into
+ * buf.  A wall ghost holds 0 (the never-written ghost of the search
+ * direction); a periodic ghost holds p_new of the wrapped cell. */
+template <bool FIRST, typename V>
+__device__ inline void fill_plane(const DevDom &D, const InvMT<V> &T,
+  const V *__restrict__ r, const V *__restrict__ pold, V beta,
+  V *__restrict__ buf, int kk)
 {
-  const Grid &C = D.cc;
-  const int NXP = D.nx + 2;
-  /* z source plane (0 marks a wall ghost, whose value is 0) */
   int ks = kk;
   if (kk < 1) ks = D.perz ? D.nz : 0;
   else if (kk > D.nz) ks = D.perz ? 1 : 0;
-  int wz = ks ? wcount(ks, D.nz, D.perz) : 0;
-  const int items = (T25_ROWS + 2) * NXP;
-  for (int it = threadIdx.x; it < items; it += T25_THREADS) {
-    int rr = it / NXP, xx = it - rr * NXP;
-    int jj = j0 - 1 + rr;
-    int js = jj;
+  int xb = blockIdx.x * TILE_X, jb = blockIdx.y * TILE_Y;
+  for (int it = threadIdx.x; it < LPLANE; it += TILE_THREADS) {
+    int rr = it / LX, xx = it - rr * LX;
+    int jj = jb + rr, xg = xb + xx;
+    int js = jj, xs = xg;
     if (jj < 1) js = D.pery ? D.ny : 0;
     else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
-    int xs = xx;
-    if (xx < 1) xs = D.perx ? D.nx : 0;
-    else if (xx > D.nx) xs = D.perx ? 1 : 0;
-    double v = 0.;
-    if (ks && js && xs) {
-      long c = gidx(C, xs, js, ks);
-      double invm = T.v[wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery)
-                        + 9 * wz];
-      v = p_new<FIRST>(r, pold, c, invm, beta);
-    }
+    if (xg < 1) xs = D.perx ? D.nx : 0;
+    else if (xg > D.nx) xs = (D.perx && xg == D.nx + 1) ? 1 : 0;
+    V v = 0;
+    if (ks && js && xs)
+      v = p_new<FIRST, V>(r, pold, gidx(D.cc, xs, js, ks),
+                          invm_at<V>(T, D, xs, js, ks), beta);
     buf[it] = v;
   }
 }
 
-template <bool FIRST, int NC>
-__global__ void __launch_bounds__(T25_THREADS)
-spmv_search25_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
-  const double *__restrict__ r, const double *__restrict__ pold,
-  double *__restrict__ pnew, double *__restrict__ Ap,
-  double *__restrict__ gpart, int kchunk)
+/* (-A) p from the centre value, four in-plane neighbours and the two
+ * z neighbours (src/solver_kernel.cu:715-836). */
+template <typename V>
+__device__ inline V neg_lap7(const CoefT<V> &K, V pC, V pW, V pE, V pS, V pN,
+  V pB, V pT, V fw2, V fe2, V fs2, V fn2, V fb2, V ft2)
 {
-  if (st->done) return;
-  extern __shared__ double lds[];
-  __shared__ double s_rows[T25_ROWS];
-  const Grid &C = D.cc;
-  const int NXP = D.nx + 2;
-  const int PL = (T25_ROWS + 2) * NXP;
-  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
-  int j0 = blockIdx.x * T25_ROWS + 1;
-  int k0 = blockIdx.y * kchunk + 1;
-  int k1 = k0 + kchunk - 1;
-  if (k1 > D.nz) k1 = D.nz;
-  int j = j0 + w;
-  bool rowok = (j <= D.ny);
-  int ngy = (D.ny + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  double beta = FIRST ? 0. : st->beta;
-  RowW W = row_weights(D, rowok ? j : 1, 1);
-  double prev[NC], cur[NC], nxt[NC];
+  return -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
+         -K.idy2 * (fn2*(pN - pC) - fs2*(pC - pS))
+         -K.idz2 * (ft2*(pT - pC) - fb2*(pC - pB));
+}
 
-  fill_plane25<FIRST>(D, T, r, pold, beta, lds + ((k0 - 1) & 1) * PL, j0, k0 - 1);
-  fill_plane25<FIRST>(D, T, r, pold, beta, lds + (k0 & 1) * PL, j0, k0);
+template <bool FIRST, typename V, bool MIXED>
+__global__ void __launch_bounds__(TILE_THREADS)
+search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
+  const double *__restrict__ rpart, long ntiles,
+  const V *__restrict__ r, const V *__restrict__ pold,
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
+{
+  __shared__ V lds[2 * LPLANE];
+  if (st->done | st->inner_done) return;
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
+  V beta = (V) beta64;
+  TILE_PROLOGUE
+  int xl = lane + 1, yl = wv + 1;
+  V fw2 = fw_lo(x, D.perx), fe2 = fw_hi(x, D.nx, D.perx);
+  V fs2 = fw_lo(j, D.pery), fn2 = fw_hi(j, D.ny, D.pery);
+
+  fill_plane<FIRST, V>(D, T, r, pold, beta, lds + ((k0 - 1) & 1) * LPLANE,
+                       k0 - 1);
+  fill_plane<FIRST, V>(D, T, r, pold, beta, lds + (k0 & 1) * LPLANE, k0);
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    int x = lane + 1 + WAVE * c;
-    int xx = x <= D.nx ? x : 0;
-    prev[c] = lds[((k0 - 1) & 1) * PL + (w + 1) * NXP + xx];
-    cur[c] = lds[(k0 & 1) * PL + (w + 1) * NXP + xx];
-  }
+  V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
+  V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
+  double acc = 0.0;
   for (int k = k0; k <= k1; k++) {
     __syncthreads();
-    fill_plane25<FIRST>(D, T, r, pold, beta, lds + ((k + 1) & 1) * PL, j0, k + 1);
+    fill_plane<FIRST, V>(D, T, r, pold, beta, lds + ((k + 1) & 1) * LPLANE,
+                         k + 1);
     __syncthreads();
-    const double *P = lds + (k & 1) * PL;
-    const double *PN = lds + ((k + 1) & 1) * PL;
-    double fb2 = (!D.perz && k == 1) ? 0. : 1.;
-    double ft2 = (!D.perz && k == D.nz) ? 0. : 1.;
-    long base = gidx(C, 0, rowok ? j : 1, k);
-    double acc = 0.0;
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-      int x = lane + 1 + WAVE * c;
-      int xx = x <= D.nx ? x : 0;
-      nxt[c] = PN[(w + 1) * NXP + xx];
-      if (rowok && x <= D.nx) {
-        double pC = cur[c];
-        double pW = P[(w + 1) * NXP + x - 1];
-        double pE = P[(w + 1) * NXP + x + 1];
-        double pS = P[w * NXP + x];
-        double pN = P[(w + 2) * NXP + x];
-        double pB = prev[c], pT = nxt[c];
-        double fe2 = (!D.perx && x == D.nx) ? 0. : 1.;
-        double fw2 = (!D.perx && x == 1) ? 0. : 1.;
-        double a = -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
-                   -K.idy2 * (W.fn2*(pN - pC) - W.fs2*(pC - pS))
-                   -K.idz2 * (ft2*(pT - pC) - fb2*(pC - pB));
-        pnew[base + x] = pC;
-        Ap[base + x] = a;
-        acc += pC * a;
-      }
+    const V *P = lds + (k & 1) * LPLANE;
+    V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
+    if (colok) {
+      V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
+                        P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
+                        nxt, fw2, fe2, fs2, fn2, (V) fw_lo(k, D.perz),
+                        (V) fw_hi(k, D.nz, D.perz));
+      long c = gidx(C, x, j, k);
+      pnew[c] = cur;
+      Ap[c] = a;
+      acc += (double) cur * (double) a;
     }
-    double v = wave_tree_sum(acc);
-    if (lane == 0) s_rows[w] = v;
-    __syncthreads();
-    if (threadIdx.x < T25_ROWS / ROWS_PER_BLOCK) {
-      int gl = threadIdx.x;
-      int jg = (j0 - 1) / ROWS_PER_BLOCK + gl;
-      if (jg < ngy) {
-        const double *sr = s_rows + gl * ROWS_PER_BLOCK;
-        gpart[jg + (long) (k - 1) * ngy] = (sr[0] + sr[1]) + (sr[2] + sr[3]);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NC; c++) { prev[c] = cur[c]; cur[c] = nxt[c]; }
+    prev = cur;
+    cur = nxt;
   }
+  tile_partial(acc, gpart, tile);
 }
 
-/* Convergence test and beta after a new (r, z) (src/cuda_solver.cu:229-267). */
-__device__ inline void cg_after_rz(CGState *st, double s, int q)
-{
-  st->rz1 = s;
-  st->q = q;
-  if (s <= st->tol2*st->sp_rhs) {
-    st->done = 1;
-  } else if (isnan(s)) {
-    st->nan_flag = 1;
-    st->done = 1;
-  } else {
-    st->beta = s/st->rz;
-    st->rz = s;
-  }
-}
-
-/* Iteration kernel 2 (src/solver_kernel.cu:838-862): phi += a p,
- * r -= a Ap, z = invM r, group partials of (r, z). */
-__global__ void __launch_bounds__(256)
-update_kernel(DevDom D, InvMTab T, const CGState *st,
-  const double *__restrict__ p, const double *__restrict__ Ap,
-  double *__restrict__ phi, double *__restrict__ r,
+/* The same pass with every neighbour read from global memory instead of
+ * LDS (a second implementation, used to cross-check the tiled one). */
+template <bool FIRST>
+__global__ void __launch_bounds__(TILE_THREADS)
+search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
+  const double *__restrict__ rpart, long ntiles,
+  const double *__restrict__ r, const double *__restrict__ pold,
+  double *__restrict__ pnew, double *__restrict__ Ap,
   double *__restrict__ gpart)
 {
   if (st->done) return;
-  GROUP_PROLOGUE
-  double alpha = st->alpha;
-  int wy = wcount(j, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
+  double beta = 0.;
+  if (!FIRST && !cg_continue<false>(st, rpart, ntiles, q, &beta)) return;
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    long c = base + i;
-    phi[c] += alpha*p[c];
-    double rr = r[c] - alpha*Ap[c];
-    r[c] = rr;
-    double z = rr*T.v[wcount(i, D.nx, D.perx) + 3*wy + 9*wz];
-    acc += rr * z;
+  if (colok) {
+    int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+    int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+    double fw2 = fw_lo(x, D.perx), fe2 = fw_hi(x, D.nx, D.perx);
+    double fs2 = fw_lo(j, D.pery), fn2 = fw_hi(j, D.ny, D.pery);
+    for (int k = k0; k <= k1; k++) {
+      int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+#define PN(xx, jj, kk) p_new<FIRST, double>(r, pold, gidx(C, xx, jj, kk), \
+        invm_at<double>(T, D, xx, jj, kk), beta)
+      double pC = PN(x, j, k), pW = PN(xm, j, k), pE = PN(xp, j, k);
+      double pS = PN(x, jm, k), pN = PN(x, jp, k);
+      double pB = PN(x, j, km), pT = PN(x, j, kp);
+#undef PN
+      double a = neg_lap7<double>(K, pC, pW, pE, pS, pN, pB, pT, fw2, fe2,
+                                  fs2, fn2, fw_lo(k, D.perz),
+                                  fw_hi(k, D.nz, D.perz));
+      long c = gidx(C, x, j, k);
+      pnew[c] = pC;
+      Ap[c] = a;
+      acc += pC * a;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
-/* Every 50th iteration (src/cuda_solver.cu:208-224): phi += a p ... */
-__global__ void __launch_bounds__(256)
-update_solution_kernel(DevDom D, const CGState *__restrict__ st,
+/* -------------------------------- update --------------------------------- */
+/* (src/solver_kernel.cu:838-862): alpha = (r, z) / (p, Ap); phi += alpha p,
+ * r -= alpha Ap, z = invM r + tile partials of (r, z).  In the mixed form
+ * phi is the FP32 correction accumulated since the last FP64 correction. */
+template <typename V>
+__global__ void __launch_bounds__(TILE_THREADS)
+update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
+  const double *__restrict__ ppart, long ntiles,
+  const V *__restrict__ p, const V *__restrict__ Ap, V *__restrict__ phi,
+  V *__restrict__ r, double *__restrict__ gpart)
+{
+  if (st->done | st->inner_done) return;
+  V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
+  TILE_PROLOGUE
+  double acc = 0.0;
+  if (colok) {
+    long c0 = gidx(C, x, j, k0);
+#pragma unroll
+    for (int kk = 0; kk < TILE_Z; kk++) {
+      if (k0 + kk <= k1) {
+        long c = c0 + (long) kk * C.sz;
+        phi[c] += alpha*p[c];
+        V rr = r[c] - alpha*Ap[c];
+        r[c] = rr;
+        V z = rr*invm_at<V>(T, D, x, j, k0 + kk);
+        acc += (double) rr * (double) z;
+      }
+    }
+  }
+  tile_partial(acc, gpart, tile);
+}
+
+/* Every 50th iteration (src/cuda_solver.cu:208-224): phi += alpha p ... */
+__global__ void __launch_bounds__(TILE_THREADS)
+update_solution_kernel(DevDom D, const CGState *st, int q,
+  const double *__restrict__ ppart, long ntiles,
   const double *__restrict__ p, double *__restrict__ phi)
 {
   if (st->done) return;
-  GROUP_PROLOGUE
-  double alpha = st->alpha;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    long c = base + i;
+  double alpha = st->rzs[q & 1]/tile_block_sum(ppart, ntiles);
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
     phi[c] += alpha*p[c];
   }
 }
 
-/* (-A) x at cell i of a row with periodic neighbours by wrapped index. */
-__device__ inline double neg_lap_wrapped(const DevDom &D, const Coef &K,
-  const RowW &W, const double *__restrict__ x, long base, long bS, long bN,
-  long bB, long bT, int i)
+/* (-A) phi at (x, j, k) with periodic neighbours by wrapped index. */
+__device__ inline double neg_lap_at(const DevDom &D, const Coef &K,
+  const double *__restrict__ f, int x, int j, int k)
 {
-  int im = wrap_lo(i, D.nx, D.perx), ip = wrap_hi(i, D.nx, D.perx);
-  double fe2 = (!D.perx && i == D.nx) ? 0. : 1.;
-  double fw2 = (!D.perx && i == 1) ? 0. : 1.;
-  double xC = x[base + i];
-  return -K.idx2 * (fe2*(x[base + ip] - xC) - fw2*(xC - x[base + im]))
-         -K.idy2 * (W.fn2*(x[bN + i] - xC) - W.fs2*(xC - x[bS + i]))
-         -K.idz2 * (W.ft2*(x[bT + i] - xC) - W.fb2*(xC - x[bB + i]));
+  const Grid &C = D.cc;
+  int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+  int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+  int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+  return neg_lap7(K, f[gidx(C, x, j, k)], f[gidx(C, xm, j, k)],
+                  f[gidx(C, xp, j, k)], f[gidx(C, x, jm, k)],
+                  f[gidx(C, x, jp, k)], f[gidx(C, x, j, km)],
+                  f[gidx(C, x, j, kp)], fw_lo(x, D.perx),
+                  fw_hi(x, D.nx, D.perx), fw_lo(j, D.pery),
+                  fw_hi(j, D.ny, D.pery), fw_lo(k, D.perz),
+                  fw_hi(k, D.nz, D.perz));
 }
 
-#define WRAPPED_ROWS                                                          \
-  int jr = rowok ? j : 1;                                                     \
-  RowW W = row_weights(D, jr, k);                                             \
-  int jm = wrap_lo(jr, D.ny, D.pery), jp = wrap_hi(jr, D.ny, D.pery);         \
-  int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);           \
-  long bS = gidx(C, 0, jm, k), bN = gidx(C, 0, jp, k);                        \
-  long bB = gidx(C, 0, jr, km), bT = gidx(C, 0, jr, kp);
-
-/* ... then r = b - (-A) phi, z = invM r, group partials of (r, z). */
-__global__ void __launch_bounds__(256)
+/* ... then r = b - (-A) phi, z = invM r + tile partials of (r, z). */
+__global__ void __launch_bounds__(TILE_THREADS)
 residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
   const double *__restrict__ phi, const double *__restrict__ rhs,
   double *__restrict__ r, double *__restrict__ gpart)
 {
   if (st->done) return;
-  GROUP_PROLOGUE
-  WRAPPED_ROWS
-  int wy = wcount(jr, D.ny, D.pery), wz = wcount(k, D.nz, D.perz);
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    double a = neg_lap_wrapped(D, K, W, phi, base, bS, bN, bB, bT, i);
-    double rr = rhs[base + i] - a;
-    r[base + i] = rr;
-    double z = rr*T.v[wcount(i, D.nx, D.perx) + 3*wy + 9*wz];
-    acc += rr * z;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
+      r[c] = rr;
+      double z = rr*invm_at(T, D, x, j, k);
+      acc += rr * z;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
-/* Group partials of ||b - (-A) phi||^2 (diagnostic true residual). */
-__global__ void __launch_bounds__(256)
+/* Tile partials of ||b - (-A) phi||^2 (diagnostic true residual). */
+__global__ void __launch_bounds__(TILE_THREADS)
 true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
   const double *__restrict__ rhs, double *__restrict__ gpart)
 {
-  GROUP_PROLOGUE
-  WRAPPED_ROWS
+  TILE_PROLOGUE
   double acc = 0.0;
-  for (int i = lane + 1; i <= nxr; i += WAVE) {
-    double rr = rhs[base + i] - neg_lap_wrapped(D, K, W, phi, base, bS, bN,
-      bB, bT, i);
-    acc += rr * rr;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      double rr = rhs[gidx(C, x, j, k)] - neg_lap_at(D, K, phi, x, j, k);
+      acc += rr * rr;
+    }
   }
-  group_partial(acc, gpart);
+  tile_partial(acc, gpart, tile);
 }
 
-/* ------------------------- step 3 with CG control ------------------------ */
-/* (p, Ap) -> alpha = (r, z) / (p, Ap) */
-__global__ void __launch_bounds__(REDUCE_THREADS)
-reduce_alpha_kernel(const double *__restrict__ gpart, long n, CGState *st)
+/* Stopping test after the last enqueued update of a batch (one workgroup;
+ * the next search_spmv repeats it on the same bits). */
+template <bool MIXED>
+__global__ void __launch_bounds__(TILE_THREADS)
+cg_check_kernel(CGState *st, int q, const double *__restrict__ rpart, long n)
 {
-  if (st->done) return;
-  double s = block_canonical_sum(gpart, n);
-  if (threadIdx.x == 0) {
-    st->denom = s;
-    st->alpha = st->rz/s;
+  if (st->done | st->inner_done) return;
+  double beta;
+  cg_continue<MIXED>(st, rpart, n, q + 1, &beta);
+}
+
+/* --------------------- mixed form: FP64 corrections ---------------------- */
+/* Cold start: phi = 0, e = 0, r = b (in FP32) + tile partials of (b, M^-1 b)
+ * in FP64. */
+__global__ void __launch_bounds__(TILE_THREADS)
+mx_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
+  double *__restrict__ phi, float *__restrict__ e, float *__restrict__ r,
+  double *__restrict__ gpart)
+{
+  TILE_PROLOGUE
+  double acc = 0.0;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double t = rhs[c];
+      phi[c] = 0.;
+      e[c] = 0.f;
+      r[c] = (float) t;
+      acc += t * (t*invm_at(T, D, x, j, k));
+    }
+  }
+  tile_partial(acc, gpart, tile);
+}
+
+/* phi += e (FP64), e = 0. */
+__global__ void __launch_bounds__(TILE_THREADS)
+mx_fold_kernel(DevDom D, double *__restrict__ phi, float *__restrict__ e)
+{
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
+    phi[c] += (double) e[c];
+    e[c] = 0.f;
   }
 }
 
-/* (r, z) -> stopping test and beta */
-__global__ void __launch_bounds__(REDUCE_THREADS)
-reduce_beta_kernel(const double *__restrict__ gpart, long n, CGState *st,
-  int q)
+/* The FP64 defect r = b - (-A) phi, handed to the inner sweep in FP32, and
+ * the tile partials of (r, M^-1 r) in FP64. */
+__global__ void __launch_bounds__(TILE_THREADS)
+mx_residual_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ phi,
+  const double *__restrict__ rhs, float *__restrict__ r,
+  double *__restrict__ gpart)
 {
-  if (st->done) return;
-  double s = block_canonical_sum(gpart, n);
-  if (threadIdx.x == 0) cg_after_rz(st, s, q);
+  TILE_PROLOGUE
+  double acc = 0.0;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
+      r[c] = (float) rr;
+      acc += rr * (rr*invm_at(T, D, x, j, k));
+    }
+  }
+  tile_partial(acc, gpart, tile);
 }
 
-/* (r, z) of the initial residual -> fresh CG state; (b, b) from slot. */
-__global__ void __launch_bounds__(REDUCE_THREADS)
-reduce_init_kernel(const double *__restrict__ gpart, long n, CGState *st,
-  const double *__restrict__ sp_rhs_slot, double tol2)
+/* After a correction: the FP64 (r, M^-1 r) decides convergence; otherwise
+ * it becomes the reference for the next inner sweep.  With `restart` the
+ * next iteration q + 1 starts a fresh Krylov space (classic defect
+ * correction); otherwise the search direction is kept and beta is formed
+ * from the FP64 value by the next search launch (a reliable update). */
+__global__ void __launch_bounds__(TILE_THREADS)
+mx_state_kernel(CGState *st, const double *__restrict__ rpart, long n, int q,
+  int restart)
 {
-  double s = block_canonical_sum(gpart, n);
-  if (threadIdx.x == 0) {
-    st->sp_rhs = *sp_rhs_slot;
-    st->rz = s;
-    st->rz1 = 0.; st->denom = 0.; st->alpha = 0.; st->beta = 0.;
-    st->tol2 = tol2;
-    st->q = 0; st->done = 0; st->nan_flag = 0;
+  double s = tile_block_sum(rpart, n);
+  if (threadIdx.x != 0) return;
+  st->rz1 = s;
+  if (s <= st->tol2*st->sp_rhs || isnan(s)) {
+    st->done = 1;
+    st->nan_flag = isnan(s) ? 1 : 0;
+    st->q = q;
+    return;
   }
+  st->rz_ref = s;
+  st->inner_done = 0;
+  if (restart) st->rzs[(q + 1) & 1] = s;
 }
 
 /* ------------------------------- driver --------------------------------- */
@@ -504,15 +590,35 @@ static double now_s(void)
   return t.tv_sec + t.tv_usec * 1e-6;
 }
 
-#define ROWS_LAUNCH rows_grid(ddom.ny, ddom.nz), rows_block(), 0, bb_stream
+#define TILES_LAUNCH tiles_grid(ddom), tiles_block(), 0, bb_stream
 
 static CGState *_cg_state = NULL;
 static CGState *cg_host = NULL;
 static double *_p_alt = NULL;          /* second search-direction buffer */
-static long p_alt_n = 0;
+static double *_part_p = NULL;         /* tile partials of (p, Ap) */
+static double *_part_r = NULL;         /* tile partials of (r, z) */
+static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
-int bb_spmv_variant = 1;               /* 0: row kernel, 1: 2.5-D LDS kernel */
-int bb_spmv_kchunk = 8;                /* planes per 2.5-D workgroup */
+int bb_spmv_variant = 1;               /* 1: LDS tile kernel, 0: direct loads */
+int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
+int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
+
+/* FP32 vectors of the mixed form's inner sweeps (Gcc layout). */
+static float *_r32 = NULL, *_p32a = NULL, *_p32b = NULL, *_Ap32 = NULL,
+             *_e32 = NULL;
+static long buf32_n = 0;
+
+static void cg_buffers32(void)
+{
+  if (buf32_n == ddom.cc.n) return;
+  float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
+  for (int i = 0; i < 5; i++) {
+    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i]));
+    BB_CHECK(hipMalloc((void **) bufs[i], ddom.cc.n * sizeof(float)));
+    BB_CHECK(hipMemset(*bufs[i], 0, ddom.cc.n * sizeof(float)));
+  }
+  buf32_n = ddom.cc.n;
+}
 
 static void cg_buffers(void)
 {
@@ -522,23 +628,39 @@ static void cg_buffers(void)
     BB_CHECK(hipHostMalloc((void **) &cg_host, sizeof(CGState),
       hipHostMallocDefault));
   }
-  if (p_alt_n != ddom.cc.n) {
+  if (buf_n != ddom.cc.n) {
     if (_p_alt) BB_CHECK(hipFree(_p_alt));
     BB_CHECK(hipMalloc((void **) &_p_alt, ddom.cc.n * sizeof(double)));
     BB_CHECK(hipMemset(_p_alt, 0, ddom.cc.n * sizeof(double)));
-    p_alt_n = ddom.cc.n;
+    buf_n = ddom.cc.n;
+  }
+  long nt = n_tiles(ddom);
+  if (part_n != nt) {
+    if (_part_p) BB_CHECK(hipFree(_part_p));
+    if (_part_r) BB_CHECK(hipFree(_part_r));
+    BB_CHECK(hipMalloc((void **) &_part_p, nt * sizeof(double)));
+    BB_CHECK(hipMalloc((void **) &_part_r, nt * sizeof(double)));
+    part_n = nt;
   }
 }
 
 void bb_solver_free(void)
 {
   if (_p_alt) BB_CHECK(hipFree(_p_alt));
+  if (_part_p) BB_CHECK(hipFree(_part_p));
+  if (_part_r) BB_CHECK(hipFree(_part_r));
   if (_cg_state) BB_CHECK(hipFree(_cg_state));
   if (cg_host) BB_CHECK(hipHostFree(cg_host));
+  float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
+  for (int i = 0; i < 5; i++) {
+    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i]));
+    *bufs[i] = NULL;
+  }
+  buf32_n = 0;
   _cg_state = NULL;
   cg_host = NULL;
-  _p_alt = NULL;
-  p_alt_n = 0;
+  _p_alt = _part_p = _part_r = NULL;
+  buf_n = part_n = 0;
 }
 
 static void fetch_state(void)
@@ -549,108 +671,80 @@ static void fetch_state(void)
   if (bb_time_kernels) bb_kt_collect();
 }
 
-/* rhs and (b, b) -> slot 0. */
+/* rhs and the tile partials of (b, b) in _part_p. */
 void bb_pp_rhs(void)
 {
-  hipLaunchKernelGGL(rhs_kernel, ROWS_LAUNCH, ddom, 1. / dom[0].dx,
+  cg_buffers();
+  hipLaunchKernelGGL(rhs_kernel, TILES_LAUNCH, ddom, 1. / dom[0].dx,
     1. / dom[0].dy, 1. / dom[0].dz, rho_f / dt, _u_star, _v_star, _w_star,
-    _rhs_p, _bb_rows);
+    _rhs_p, _part_p);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, n_groups(ddom.ny, ddom.nz), _bb_scalars + 0);
 }
 
 /* Diagnostic: sqrt(||b + A phi||^2 / (b, b)). */
 double bb_true_residual(double sp_rhs)
 {
-  hipLaunchKernelGGL(true_residual_kernel, ROWS_LAUNCH, ddom, coef(), _phi,
-    _rhs_p, _bb_rows);
+  hipLaunchKernelGGL(true_residual_kernel, TILES_LAUNCH, ddom, coef(), _phi,
+    _rhs_p, _part_r);
   BB_LAUNCH_CHECK();
-  bb_reduce_rows(_bb_rows, n_groups(ddom.ny, ddom.nz), _bb_scalars + 8);
+  bb_reduce_rows(_part_r, n_tiles(ddom), _bb_scalars + 8);
   double rr = bb_fetch_scalar(8);
   return sp_rhs > 0 ? sqrt(rr) / sqrt(sp_rhs) : 0.;
 }
 
-/* Largest x extent the 2.5-D kernel takes (two LDS planes within 64 KiB). */
-static int spmv25_nc(void)
-{
-  int nc = (ddom.nx + WAVE - 1) / WAVE;
-  size_t lds = 2 * (size_t) (T25_ROWS + 2) * (ddom.nx + 2) * sizeof(double);
-  if (nc > 6 || lds > 60 * 1024) return 0;
-  return nc;
-}
-
-template <bool FIRST>
-static void launch_spmv25(int nc, const Coef &K, const InvMTab &T,
-  const double *pold, double *pnew)
-{
-  int kc = bb_spmv_kchunk < 1 ? 1 : bb_spmv_kchunk;
-  dim3 grid((ddom.ny + T25_ROWS - 1) / T25_ROWS, (ddom.nz + kc - 1) / kc);
-  size_t lds = 2 * (size_t) (T25_ROWS + 2) * (ddom.nx + 2) * sizeof(double);
-#define L25(NC) hipLaunchKernelGGL((spmv_search25_kernel<FIRST, NC>), grid, \
-    dim3(T25_THREADS), lds, bb_stream, ddom, K, T, _cg_state, _r_q, pold,   \
-    pnew, _Apb_q, _bb_rows, kc)
-  switch (nc) {
-    case 1: L25(1); break;
-    case 2: L25(2); break;
-    case 3: L25(3); break;
-    case 4: L25(4); break;
-    case 5: L25(5); break;
-    default: L25(6); break;
-  }
-#undef L25
-}
-
-/* Enqueue iteration q of the plain PCG; p buffers alternate by parity. */
+/* Enqueue iteration q; p buffers alternate by parity. */
 static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 {
+  const long nt = n_tiles(ddom);
   double *pnew = (q & 1) ? _p_q : _p_alt;
   double *pold = (q & 1) ? _p_alt : _p_q;
-  int nc = bb_spmv_variant == 1 ? spmv25_nc() : 0;
   bb_kt_begin(KT_SPMV);
-  if (nc > 0) {
-    if (q == 1) launch_spmv25<true>(nc, K, T, pold, pnew);
-    else launch_spmv25<false>(nc, K, T, pold, pnew);
-  } else if (q == 1) {
-    hipLaunchKernelGGL(spmv_search_kernel<true>, ROWS_LAUNCH, ddom, K, T,
-      _cg_state, _r_q, pold, pnew, _Apb_q, _bb_rows);
+  if (bb_spmv_variant == 1) {
+    if (q == 1)
+      hipLaunchKernelGGL((search_spmv_kernel<true, double, false>),
+        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew,
+        _Apb_q, _part_p);
+    else
+      hipLaunchKernelGGL((search_spmv_kernel<false, double, false>),
+        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew,
+        _Apb_q, _part_p);
   } else {
-    hipLaunchKernelGGL(spmv_search_kernel<false>, ROWS_LAUNCH, ddom, K, T,
-      _cg_state, _r_q, pold, pnew, _Apb_q, _bb_rows);
+    if (q == 1)
+      hipLaunchKernelGGL(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew, _Apb_q, _part_p);
+    else
+      hipLaunchKernelGGL(search_spmv_direct_kernel<false>, TILES_LAUNCH, ddom,
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew, _Apb_q, _part_p);
   }
   bb_kt_end(KT_SPMV);
-  const long ng = n_groups(ddom.ny, ddom.nz);
-  hipLaunchKernelGGL(reduce_alpha_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
-    bb_stream, _bb_rows, ng, _cg_state);
   if (q % 50 == 0) {
-    hipLaunchKernelGGL(update_solution_kernel, ROWS_LAUNCH, ddom, _cg_state,
-      pnew, _phi);
-    hipLaunchKernelGGL(residual_kernel, ROWS_LAUNCH, ddom, K, T, _cg_state,
-      _phi, _rhs_p, _r_q, _bb_rows);
+    hipLaunchKernelGGL(update_solution_kernel, TILES_LAUNCH, ddom, _cg_state,
+      q, _part_p, nt, pnew, _phi);
+    hipLaunchKernelGGL(residual_kernel, TILES_LAUNCH, ddom, K, T, _cg_state,
+      _phi, _rhs_p, _r_q, _part_r);
   } else {
     bb_kt_begin(KT_UPDATE);
-    hipLaunchKernelGGL(update_kernel, ROWS_LAUNCH, ddom, T, _cg_state, pnew,
-      _Apb_q, _phi, _r_q, _bb_rows);
+    hipLaunchKernelGGL(update_kernel<double>, TILES_LAUNCH, ddom, T,
+      _cg_state, q, _part_p, nt, pnew, _Apb_q, _phi, _r_q, _part_r);
     bb_kt_end(KT_UPDATE);
   }
-  hipLaunchKernelGGL(reduce_beta_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
-    bb_stream, _bb_rows, ng, _cg_state, q);
   BB_LAUNCH_CHECK();
 }
 
 /* The plain FP64 Jacobi-PCG (src/cuda_solver.cu:574-761) on the RHS already
- * in _rhs_p with (b, b) in slot 0. */
+ * in _rhs_p with the tile partials of (b, b) in _part_p. */
 static void pcg_plain(double ts)
 {
   Coef K = coef();
   InvMTab T = invm_table(K);
   const char *rname = "solver_expd.rec";
+  const long nt = n_tiles(ddom);
   cg_buffers();
 
-  hipLaunchKernelGGL(cg_init_kernel, ROWS_LAUNCH, ddom, T, _rhs_p, _r_q, _phi,
-    _bb_rows);
-  hipLaunchKernelGGL(reduce_init_kernel, dim3(1), dim3(REDUCE_THREADS), 0,
-    bb_stream, _bb_rows, n_groups(ddom.ny, ddom.nz), _cg_state,
-    _bb_scalars + 0, pp_residual*pp_residual);
+  hipLaunchKernelGGL(cg_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p, _r_q,
+    _phi, _part_r);
+  hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
+    bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual, 0.);
   BB_LAUNCH_CHECK();
   fetch_state();
   real sp_rhs = cg_host->sp_rhs;
@@ -673,12 +767,14 @@ static void pcg_plain(double ts)
     int qend = q + bb_pcg_batch;
     if (qend > pp_max_iter + 1) qend = pp_max_iter + 1;
     for (int qq = q + 1; qq <= qend; qq++) enqueue_iteration(qq, K, T);
+    hipLaunchKernelGGL(cg_check_kernel<false>, dim3(1), dim3(TILE_THREADS),
+      0, bb_stream, _cg_state, qend, _part_r, nt);
     q = qend;
     fetch_state();
     if (cg_host->done) break;
   }
   real sp_rq1 = cg_host->rz1;
-  if (cg_host->nan_flag) {
+  if (cg_host->done && cg_host->nan_flag) {
     if (rank == 0) {
       printf("N%d >> The PP equation did not converge.\n", rank);
       printf("N%d >> The residual at iteration %d is nan (%lf).\n", rank,
@@ -718,7 +814,134 @@ extern "C" void cuda_PP_cg(void)
   cuda_PP_cg_noparts();
 }
 
+/* Enqueue inner iteration q of the mixed form (FP32 vectors); `first`
+ * starts a new Krylov space (p = z). */
+static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
+  const InvMT<float> &T)
+{
+  const long nt = n_tiles(ddom);
+  float *pnew = (q & 1) ? _p32a : _p32b;
+  float *pold = (q & 1) ? _p32b : _p32a;
+  bb_kt_begin(KT_SPMV32);
+  if (first)
+    hipLaunchKernelGGL((search_spmv_kernel<true, float, true>), TILES_LAUNCH,
+      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, pnew, _Ap32,
+      _part_p);
+  else
+    hipLaunchKernelGGL((search_spmv_kernel<false, float, true>), TILES_LAUNCH,
+      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, pnew, _Ap32,
+      _part_p);
+  bb_kt_end(KT_SPMV32);
+  bb_kt_begin(KT_UPDATE32);
+  hipLaunchKernelGGL(update_kernel<float>, TILES_LAUNCH, ddom, T, _cg_state,
+    q, _part_p, nt, pnew, _Ap32, _e32, _r32, _part_r);
+  bb_kt_end(KT_UPDATE32);
+  BB_LAUNCH_CHECK();
+}
+
+/* FP32-PCG inside an FP64 defect-correction loop.  The inner sweep is the
+ * plain form's Jacobi-PCG with FP32 vectors (search direction, residual,
+ * A p and the accumulated correction e) and FP64 dot products; it runs
+ * until its (r, z) has fallen by pp_inner_tol^2 since the last correction
+ * or reaches the target.  A correction then folds e into phi (FP64), forms
+ * the FP64 defect b - (-A) phi, and applies the stopping test of the plain
+ * form, (r, M^-1 r) <= pp_residual^2 (b, b), to it in FP64; the solve ends
+ * only there.  The inner sweep continues with the defect as its residual:
+ * by default keeping its search direction (a reliable update, so the
+ * Krylov space survives the correction), or with bb_mixed_restart = 1
+ * restarting from p = z (classic defect correction).  No counterpart in the
+ * reference; it replaces src/cuda_solver.cu:574-761 when selected. */
+static void pcg_mixed(double ts)
+{
+  Coef K = coef();
+  InvMTab T = invm_table(K);
+  CoefT<float> K32 = coef32(K);
+  InvMT<float> T32 = invm_table32(T);
+  const char *rname = "solver_expd.rec";
+  const long nt = n_tiles(ddom);
+  if (!(pp_inner_tol > 0. && pp_inner_tol < 1.)) {
+    fprintf(stderr, "pp_inner_tol must lie in (0, 1), got %g\n", pp_inner_tol);
+    exit(EXIT_FAILURE);
+  }
+  cg_buffers();
+  cg_buffers32();
+
+  hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p, _phi,
+    _e32, _r32, _part_r);
+  hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
+    bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual,
+    pp_inner_tol*pp_inner_tol);
+  BB_LAUNCH_CHECK();
+  fetch_state();
+  real sp_rhs = cg_host->sp_rhs;
+  real RHS_TOL = 1.e-8;
+  pp_last.sp_rhs = sp_rhs;
+  pp_last.nouter = 1;
+  if (sp_rhs < RHS_TOL*RHS_TOL) {
+    double etime = now_s() - ts;
+    recorder_PP(rname, 0, 0., etime);
+    if (rank == 0)
+      printf("N%d >> Norm of the rhs is less than %.1e, exiting solver\n",
+        rank, RHS_TOL);
+    pp_last.niter = 0; pp_last.resid = 0.; pp_last.true_resid = 0.;
+    pp_last.etime = etime;
+    return;
+  }
+  int q = 0, first_q = 1, nouter = 1;
+  int done = cg_host->rz1 <= pp_residual*pp_residual*sp_rhs;
+  while (!done && q <= pp_max_iter && nouter <= pp_max_iter) {
+    int qend = q + bb_pcg_batch;
+    if (qend > pp_max_iter + 1) qend = pp_max_iter + 1;
+    for (int qq = q + 1; qq <= qend; qq++)
+      enqueue_iteration32(qq, qq == first_q, K32, T32);
+    hipLaunchKernelGGL(cg_check_kernel<true>, dim3(1), dim3(TILE_THREADS),
+      0, bb_stream, _cg_state, qend, _part_r, nt);
+    fetch_state();
+    if (!cg_host->inner_done) {
+      q = qend;
+      continue;
+    }
+    q = cg_host->q;
+    hipLaunchKernelGGL(mx_fold_kernel, TILES_LAUNCH, ddom, _phi, _e32);
+    hipLaunchKernelGGL(mx_residual_kernel, TILES_LAUNCH, ddom, K, T, _phi,
+      _rhs_p, _r32, _part_r);
+    hipLaunchKernelGGL(mx_state_kernel, dim3(1), dim3(TILE_THREADS), 0,
+      bb_stream, _cg_state, _part_r, nt, q, bb_mixed_restart);
+    BB_LAUNCH_CHECK();
+    nouter++;
+    fetch_state();
+    done = cg_host->done;
+    if (bb_mixed_restart) first_q = q + 1;
+  }
+  real sp_rq1 = cg_host->rz1;
+  if (done && (cg_host->nan_flag || isnan(sp_rq1))) {
+    if (rank == 0) {
+      printf("N%d >> The PP equation did not converge.\n", rank);
+      printf("N%d >> The residual at iteration %d is nan (%lf).\n", rank, q,
+        sqrt(sp_rq1));
+    }
+    exit(EXIT_FAILURE);
+  }
+  if (!done) {
+    printf("N%d >> The pressure-Poisson equation did not converge.\n", rank);
+    printf("N%d >> (rhs, rhs) is %e\n", rank, sp_rhs);
+    printf("N%d >> Residual at iteration %d is %lf\n", rank, q,
+      sqrt(sp_rq1)/sqrt(sp_rhs));
+    exit(EXIT_FAILURE);
+  }
+  double etime = now_s() - ts;
+  recorder_PP(rname, q, sqrt(sp_rq1)/sqrt(sp_rhs), etime);
+  pp_last.etime = etime;
+  pp_last.niter = q;
+  pp_last.nouter = nouter;
+  pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
+}
+
 extern "C" void cuda_PP_cg_mixed(void)
 {
-  cuda_PP_cg_noparts();
+  double ts = now_s();
+  bb_pp_rhs();
+  pcg_mixed(ts);
+  pp_last.true_resid = (bb_log_true_residual && pp_last.sp_rhs > 0)
+    ? bb_true_residual(pp_last.sp_rhs) : 0.;
 }

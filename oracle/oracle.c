@@ -9,11 +9,12 @@
  * each boundary-condition pass) so that the GPU library's shortcuts are
  * checked rather than shared.
  *
- * Reductions follow the canonical order documented in the library (row
- * partials over 64 interleaved lanes + halving tree, groups of four rows of
- * a plane as (r0 + r1) + (r2 + r3), then 1024 interleaved accumulators +
- * halving tree) because the reference's own order is that of thrust, which
- * is not specified; that order is the definition here.
+ * Reductions follow the canonical order documented in the library (tiles
+ * of 64 x 8 x 8 cells: column sums along z, a halving tree over the 64
+ * columns of a row, a fixed tree over the 8 rows, then 1024 interleaved
+ * accumulators over the tiles + halving tree) because the reference's own
+ * order is that of thrust, which is not specified; that order is the
+ * definition here.
  * Compile with -ffp-contract=off.
  *
  * Pinning: see tests/test_oracle.py (closed forms of src/cuda_testing.cu,
@@ -356,16 +357,6 @@ static void init_fields(void)
 }
 
 /* ------------------------- canonical reductions ------------------------- */
-static double canon_row(const double *t, int n)
-{
-  double lane[64];
-  for (int l = 0; l < 64; l++) lane[l] = 0.0;
-  for (int i = 0; i < n; i++) lane[i % 64] += t[i];
-  for (int off = 32; off >= 1; off >>= 1)
-    for (int l = 0; l < off; l++) lane[l] += lane[l + off];
-  return lane[0];
-}
-
 double orc_canon_linear(const double *rows, long n)
 {
   double acc[1024];
@@ -376,28 +367,41 @@ double orc_canon_linear(const double *rows, long n)
   return acc[0];
 }
 
-/* terms: nx*ny*nz values, x-fastest, interior cells only.  Rows are
- * reduced over 64 interleaved lanes, then taken four at a time within a
- * plane as (r0 + r1) + (r2 + r3) (a row past ny is +0.0), then the group
- * partials g = jg + k*ceil(ny/4) over 1024 interleaved accumulators. */
+/* terms: nx*ny*nz values, x-fastest, interior cells only.  Tiles of 64 x 8
+ * x 8 cells: each (x, y) column of a tile sums its planes in increasing z;
+ * the 64 columns of a row by the halving tree; the 8 rows as
+ * ((w0 + w1) + (w2 + w3)) + ((w4 + w5) + (w6 + w7)); the tiles, numbered
+ * tc + ntx (tj + nty tk), over 1024 interleaved accumulators. */
 double orc_canon_sum(const double *terms, int nx, int ny, int nz)
 {
-  int ngy = (ny + 3) / 4;
-  long ng = (long) ngy * nz;
-  double *grp = dal(ng);
+  int ntx = (nx + 63) / 64, nty = (ny + 7) / 8, ntz = (nz + 7) / 8;
+  long nt = (long) ntx * nty * ntz;
+  double *part = dal(nt);
   #pragma omp parallel for schedule(static)
-  for (long g = 0; g < ng; g++) {
-    int jg = (int) (g % ngy), k = (int) (g / ngy);
-    double r[4];
-    for (int w = 0; w < 4; w++) {
-      int j = 4 * jg + w;
-      r[w] = (j < ny) ? canon_row(terms + ((long) j + (long) k * ny) * nx, nx)
-                      : 0.0;
+  for (long t = 0; t < nt; t++) {
+    int tc = (int) (t % ntx), tj = (int) ((t / ntx) % nty),
+        tk = (int) (t / ((long) ntx * nty));
+    double wv[8];
+    for (int w = 0; w < 8; w++) {
+      int j = tj * 8 + w;
+      double lane[64];
+      for (int l = 0; l < 64; l++) {
+        int x = tc * 64 + l;
+        double acc = 0.0;
+        if (x < nx && j < ny)
+          for (int k = tk * 8; k < tk * 8 + 8 && k < nz; k++)
+            acc += terms[(long) x + (long) j * nx + (long) k * nx * ny];
+        lane[l] = acc;
+      }
+      for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; l++) lane[l] += lane[l + off];
+      wv[w] = lane[0];
     }
-    grp[g] = (r[0] + r[1]) + (r[2] + r[3]);
+    part[t] = ((wv[0] + wv[1]) + (wv[2] + wv[3])) +
+              ((wv[4] + wv[5]) + (wv[6] + wv[7]));
   }
-  double s = orc_canon_linear(grp, ng);
-  free(grp);
+  double s = orc_canon_linear(part, nt);
+  free(part);
   return s;
 }
 

@@ -408,29 +408,37 @@ def test_oracle_deterministic(tmp_path):
 
 # ------------------------------------------------------ canonical reduction
 def canonical_python(t):
-    """The documented order: rows (j,k) over 64 interleaved lanes + halving
-    tree, groups of four rows of a plane as (r0 + r1) + (r2 + r3), then 1024
-    interleaved accumulators + halving tree."""
+    """The documented order: tiles of 64 x 8 x 8 cells; each (x, y) column
+    of a tile summed along z, the 64 columns of a row by the halving tree,
+    the 8 rows as ((w0+w1)+(w2+w3))+((w4+w5)+(w6+w7)), then 1024
+    interleaved accumulators over the tiles + halving tree."""
     nx, ny, nz = t.shape
-
-    def row(j, k):
-        lane = [0.0] * 64
-        for i in range(nx):
-            lane[i % 64] = lane[i % 64] + float(t[i, j, k])
-        off = 32
-        while off >= 1:
-            for l in range(off):
-                lane[l] = lane[l] + lane[l + off]
-            off //= 2
-        return lane[0]
-    groups = []
-    for k in range(nz):
-        for jg in range((ny + 3) // 4):
-            r = [row(j, k) if j < ny else 0.0
-                 for j in range(4 * jg, 4 * jg + 4)]
-            groups.append((r[0] + r[1]) + (r[2] + r[3]))
+    ntx, nty, ntz = -(-nx // 64), -(-ny // 8), -(-nz // 8)
+    tiles = []
+    for tk in range(ntz):
+        for tj in range(nty):
+            for tc in range(ntx):
+                wv = []
+                for w in range(8):
+                    j = tj * 8 + w
+                    lane = []
+                    for l in range(64):
+                        x = tc * 64 + l
+                        acc = 0.0
+                        if x < nx and j < ny:
+                            for k in range(tk * 8, min(tk * 8 + 8, nz)):
+                                acc = acc + float(t[x, j, k])
+                        lane.append(acc)
+                    off = 32
+                    while off >= 1:
+                        for l in range(off):
+                            lane[l] = lane[l] + lane[l + off]
+                        off //= 2
+                    wv.append(lane[0])
+                tiles.append(((wv[0] + wv[1]) + (wv[2] + wv[3])) +
+                             ((wv[4] + wv[5]) + (wv[6] + wv[7])))
     acc = [0.0] * 1024
-    for r, v in enumerate(groups):
+    for r, v in enumerate(tiles):
         acc[r % 1024] = acc[r % 1024] + v
     off = 512
     while off >= 1:
@@ -443,7 +451,7 @@ def canonical_python(t):
 def test_canonical_sum_order():
     rng = np.random.default_rng(11)
     t = rng.standard_normal((70, 9, 5)) * 10.0 ** rng.integers(-8, 8, (70, 9, 5))
-    big = rng.standard_normal((65, 6, 900))
+    big = rng.standard_normal((65, 3, 8200))
     assert orc.canon_sum(big) == canonical_python(big)
     assert orc.canon_sum(t) == canonical_python(t)
     assert abs(orc.canon_sum(t) - math.fsum(t.ravel())) <= \
