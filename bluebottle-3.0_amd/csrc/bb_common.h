@@ -154,25 +154,46 @@ inline long n_tiles(const DevDom &D)
 }
 inline dim3 tiles_grid(const DevDom &D)
 {
-  Tiles T = tiles_of(D);
-  return dim3(T.ntx, T.nty, T.ntz);
+  return dim3((unsigned) n_tiles(D));
 }
 inline dim3 tiles_block() { return dim3(TILE_THREADS); }
 
-/* Tile-kernel prologue: this thread's cell column (x, j), the plane range
- * of the tile, and the tile number. */
+/* XCD-aware tile order.  Workgroups of a launch are handed to the 8 XCDs
+ * round robin (workgroup b to XCD b mod 8); each XCD has its own L2.  Tile
+ * kernels are launched as a 1-D grid of n_tiles workgroups and workgroup b
+ * takes tile tile_of_block(b), so that XCD x owns one contiguous range of
+ * tile numbers (whole rows of x-tiles, neighbouring rows and planes) and the
+ * lines that neighbouring tiles share stay in one L2.  A bijection on
+ * [0, nt) for every nt. */
+#define BB_NXCD 8
+__device__ inline long tile_of_block(long nt)
+{
+  long b = blockIdx.x;
+  long q = nt / BB_NXCD, rem = nt % BB_NXCD;
+  long xcd = b % BB_NXCD, s = b / BB_NXCD;
+  return xcd * q + (xcd < rem ? xcd : rem) + s;
+}
+
+/* Tile-kernel prologue: the tile of this workgroup, this thread's cell
+ * column (x, j), the plane range of the tile and the tile's first x and y
+ * cell minus one (xb, jb). */
 #define TILE_PROLOGUE                                                         \
   const Grid &C = D.cc;                                                       \
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;                              \
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;                              \
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;                              \
+  long tile = tile_of_block((long) ntx_ * nty_ * ntz_);                       \
   int lane = threadIdx.x & (WAVE - 1);                                        \
-  int wv = threadIdx.x >> 6;                                                  \
-  int x = blockIdx.x * TILE_X + lane + 1;                                     \
-  int j = blockIdx.y * TILE_Y + wv + 1;                                       \
-  int k0 = blockIdx.z * TILE_Z + 1;                                           \
+  /* the row index is the same for all lanes of a wavefront */              \
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);                  \
+  int xb = (int) (tile % ntx_) * TILE_X;                                      \
+  int jb = (int) ((tile / ntx_) % nty_) * TILE_Y;                             \
+  int k0 = (int) (tile / ((long) ntx_ * nty_)) * TILE_Z + 1;                  \
+  int x = xb + lane + 1;                                                      \
+  int j = jb + wv + 1;                                                        \
   int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;                   \
   bool colok = (x <= D.nx) && (j <= D.ny);                                    \
-  long tile = blockIdx.x + (long) gridDim.x *                                 \
-              (blockIdx.y + (long) gridDim.y * blockIdx.z);                   \
-  (void) tile; (void) colok;
+  (void) tile; (void) colok; (void) xb; (void) jb;
 
 /* Steps 2-3: thread 0 writes the tile partial.  Every thread calls it. */
 __device__ inline void tile_partial(double acc, double *__restrict__ gpart,

@@ -105,9 +105,15 @@ __device__ inline V invm_at(const InvMT<V> &T, const DevDom &D, int x, int j,
 __device__ inline int wrap_lo(int a, int n, int per) { return (per && a == 1) ? n : a - 1; }
 __device__ inline int wrap_hi(int a, int n, int per) { return (per && a == n) ? 1 : a + 1; }
 
-/* Face weight flag^2: 0 on a non-periodic boundary face, else 1. */
-__device__ inline double fw_lo(int a, int per) { return (!per && a == 1) ? 0. : 1.; }
-__device__ inline double fw_hi(int a, int n, int per) { return (!per && a == n) ? 0. : 1.; }
+/* Face weight flag^2 is 0 on a non-periodic boundary face and 1 elsewhere;
+ * these say whether the low/high face of cell index a is such a wall. */
+__device__ inline bool wall_lo(int a, int per) { return !per && a == 1; }
+__device__ inline bool wall_hi(int a, int n, int per) { return !per && a == n; }
+
+/* w * d for a face weight w in {0, 1}: d itself (1 * d is exact) or 0 * d
+ * (which keeps the sign of zero and NaN of the product). */
+template <typename V>
+__device__ inline V wmul(bool wall, V d) { return wall ? (V) 0 * d : d; }
 
 /* The device-resident CG state.  rzs[q & 1] holds (r, z) as it stands at
  * the start of iteration q; launches of iteration q read rzs[(q-1) & 1] and
@@ -223,7 +229,7 @@ __device__ inline bool cg_continue(CGState *st, const double *__restrict__ rpart
   long n, int q, double *beta)
 {
   double s = tile_block_sum(rpart, n);
-  bool first = (blockIdx.x | blockIdx.y | blockIdx.z) == 0 && threadIdx.x == 0;
+  bool first = blockIdx.x == 0 && threadIdx.x == 0;
   bool stop = s <= st->tol2*st->sp_rhs || isnan(s);
   if (MIXED && !stop) stop = s <= st->relup*st->rz_ref;
   if (stop) {
@@ -264,47 +270,113 @@ __device__ inline V p_new(const V *__restrict__ r, const V *__restrict__ pold,
 #define LY (TILE_Y + 2)
 #define LPLANE (LX * LY)
 
-/* p_new over the tile's rows/columns plus a one-cell halo at plane kk into
- * buf.  A wall ghost holds 0 (the never-written ghost of the search
- * direction); a periodic ghost holds p_new of the wrapped cell. */
+/* The LDS plane of the search kernel holds p_new over the tile's rows and
+ * columns plus a one-cell halo, LX x LY entries; thread t fills entries t
+ * and t + TILE_THREADS.  A wall ghost holds 0 (the never-written ghost of the
+ * search direction); a periodic ghost holds p_new of the wrapped cell. */
+struct PlaneEntry {
+  int cxy;            /* x + y*sy of the source column, -1: the entry is 0
+                         (wall ghost or outside the domain) */
+  int wxy;            /* invM table index, x and y parts */
+};
+
+__device__ inline PlaneEntry plane_entry(const DevDom &D, int xb, int jb,
+  int it)
+{
+  PlaneEntry E;
+  int rr = it / LX, xx = it - rr * LX;
+  int jj = jb + rr, xg = xb + xx;
+  int js = jj, xs = xg;
+  if (jj < 1) js = D.pery ? D.ny : 0;
+  else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
+  if (xg < 1) xs = D.perx ? D.nx : 0;
+  else if (xg > D.nx) xs = (D.perx && xg == D.nx + 1) ? 1 : 0;
+  bool ok = (it < LPLANE) && js && xs;
+  E.cxy = ok ? xs + js * D.cc.sy : -1;
+  E.wxy = ok ? wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery) : 0;
+  return E;
+}
+
+/* Source plane of LDS plane kk (0 = a wall ghost plane). */
+__device__ inline int plane_src(const DevDom &D, int kk)
+{
+  if (kk < 1) return D.perz ? D.nz : 0;
+  if (kk > D.nz) return D.perz ? 1 : 0;
+  return kk;
+}
+
+/* Raw r and p_old of a thread's two entries of one plane. */
+template <typename V> struct PlaneRaw {
+  V r0, p0, r1, p1;
+};
+
+/* Element `i` of a Gcc array by a 32-bit byte offset from the (uniform)
+ * base, so the load takes a scalar base and one vector offset register. */
+template <typename V>
+__device__ inline V ld32(const V *__restrict__ a, unsigned i)
+{
+  return *(const V *) ((const char *) a + i * (unsigned) sizeof(V));
+}
+
 template <bool FIRST, typename V>
-__device__ inline void fill_plane(const DevDom &D, const InvMT<V> &T,
-  const V *__restrict__ r, const V *__restrict__ pold, V beta,
+__device__ inline PlaneRaw<V> plane_load(const DevDom &D,
+  const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
+  const V *__restrict__ pold, int kk)
+{
+  PlaneRaw<V> R;
+  R.r0 = R.p0 = R.r1 = R.p1 = 0;
+  int ks = plane_src(D, kk);
+  unsigned off = (unsigned) ks * (unsigned) D.cc.sz;
+  if (ks && E0.cxy >= 0) {
+    R.r0 = ld32(r, E0.cxy + off);
+    if (!FIRST) R.p0 = ld32(pold, E0.cxy + off);
+  }
+  if (ks && E1.cxy >= 0) {
+    R.r1 = ld32(r, E1.cxy + off);
+    if (!FIRST) R.p1 = ld32(pold, E1.cxy + off);
+  }
+  return R;
+}
+
+template <bool FIRST, typename V>
+__device__ inline void plane_store(const DevDom &D, const InvMT<V> &T,
+  const PlaneEntry &E0, const PlaneEntry &E1, const PlaneRaw<V> &R, V beta,
   V *__restrict__ buf, int kk)
 {
-  int ks = kk;
-  if (kk < 1) ks = D.perz ? D.nz : 0;
-  else if (kk > D.nz) ks = D.perz ? 1 : 0;
-  int xb = blockIdx.x * TILE_X, jb = blockIdx.y * TILE_Y;
-  for (int it = threadIdx.x; it < LPLANE; it += TILE_THREADS) {
-    int rr = it / LX, xx = it - rr * LX;
-    int jj = jb + rr, xg = xb + xx;
-    int js = jj, xs = xg;
-    if (jj < 1) js = D.pery ? D.ny : 0;
-    else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
-    if (xg < 1) xs = D.perx ? D.nx : 0;
-    else if (xg > D.nx) xs = (D.perx && xg == D.nx + 1) ? 1 : 0;
-    V v = 0;
-    if (ks && js && xs)
-      v = p_new<FIRST, V>(r, pold, gidx(D.cc, xs, js, ks),
-                          invm_at<V>(T, D, xs, js, ks), beta);
-    buf[it] = v;
+  int ks = plane_src(D, kk);
+  int wz = 9 * wcount(ks, D.nz, D.perz);
+  V v0 = 0, v1 = 0;
+  if (ks && E0.cxy >= 0) {
+    V z = R.r0*T.v[E0.wxy + wz];
+    v0 = FIRST ? z : z + beta*R.p0;
   }
+  if (ks && E1.cxy >= 0) {
+    V z = R.r1*T.v[E1.wxy + wz];
+    v1 = FIRST ? z : z + beta*R.p1;
+  }
+  buf[threadIdx.x] = v0;
+  if (threadIdx.x + TILE_THREADS < LPLANE) buf[threadIdx.x + TILE_THREADS] = v1;
 }
 
 /* (-A) p from the centre value, four in-plane neighbours and the two
- * z neighbours (src/solver_kernel.cu:715-836). */
+ * z neighbours (src/solver_kernel.cu:715-836); the w* say which faces are
+ * walls (weight 0). */
 template <typename V>
 __device__ inline V neg_lap7(const CoefT<V> &K, V pC, V pW, V pE, V pS, V pN,
-  V pB, V pT, V fw2, V fe2, V fs2, V fn2, V fb2, V ft2)
+  V pB, V pT, bool wW, bool wE, bool wS, bool wN, bool wB, bool wT)
 {
-  return -K.idx2 * (fe2*(pE - pC) - fw2*(pC - pW))
-         -K.idy2 * (fn2*(pN - pC) - fs2*(pC - pS))
-         -K.idz2 * (ft2*(pT - pC) - fb2*(pC - pB));
+  return -K.idx2 * (wmul(wE, pE - pC) - wmul(wW, pC - pW))
+         -K.idy2 * (wmul(wN, pN - pC) - wmul(wS, pC - pS))
+         -K.idz2 * (wmul(wT, pT - pC) - wmul(wB, pC - pB));
 }
 
+/* The tile's planes stream through two LDS buffers; the raw r and p_old of
+ * each plane are loaded one (FP64) or two (FP32) planes ahead of their use,
+ * the first three before the reduction that yields beta, so memory latency
+ * overlaps the work of the planes in between. */
 template <bool FIRST, typename V, bool MIXED>
 __global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles,
   const V *__restrict__ r, const V *__restrict__ pold,
@@ -312,40 +384,57 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
 {
   __shared__ V lds[2 * LPLANE];
   if (st->done | st->inner_done) return;
+  TILE_PROLOGUE
+  PlaneEntry E0 = plane_entry(D, xb, jb, threadIdx.x);
+  PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
+  /* Ring of raw planes in registers: slot (p - k0 + 1) mod RING holds plane
+   * p; plane k + RING - 1 is loaded while plane k is computed. */
+  constexpr int RING = sizeof(V) == 8 ? 3 : 4;
+  PlaneRaw<V> R[RING];
+  R[0] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 - 1);
+  R[1] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0);
+  R[2] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 + 1);
   double beta64 = 0.;
   if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
+  if (RING > 3 && k0 + 2 <= k1 + 1)
+    R[3 % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 + 2);
   V beta = (V) beta64;
-  TILE_PROLOGUE
   int xl = lane + 1, yl = wv + 1;
-  V fw2 = fw_lo(x, D.perx), fe2 = fw_hi(x, D.nx, D.perx);
-  V fs2 = fw_lo(j, D.pery), fn2 = fw_hi(j, D.ny, D.pery);
+  bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
 
-  fill_plane<FIRST, V>(D, T, r, pold, beta, lds + ((k0 - 1) & 1) * LPLANE,
-                       k0 - 1);
-  fill_plane<FIRST, V>(D, T, r, pold, beta, lds + (k0 & 1) * LPLANE, k0);
+  plane_store<FIRST, V>(D, T, E0, E1, R[0], beta,
+                        lds + ((k0 - 1) & 1) * LPLANE, k0 - 1);
+  plane_store<FIRST, V>(D, T, E0, E1, R[1], beta, lds + (k0 & 1) * LPLANE, k0);
   __syncthreads();
   V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
   V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
   double acc = 0.0;
-  for (int k = k0; k <= k1; k++) {
-    __syncthreads();
-    fill_plane<FIRST, V>(D, T, r, pold, beta, lds + ((k + 1) & 1) * LPLANE,
-                         k + 1);
-    __syncthreads();
-    const V *P = lds + (k & 1) * LPLANE;
-    V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
-    if (colok) {
-      V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
-                        P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
-                        nxt, fw2, fe2, fs2, fn2, (V) fw_lo(k, D.perz),
-                        (V) fw_hi(k, D.nz, D.perz));
-      long c = gidx(C, x, j, k);
-      pnew[c] = cur;
-      Ap[c] = a;
-      acc += (double) cur * (double) a;
+#pragma unroll
+  for (int kk = 0; kk < TILE_Z; kk++) {
+    const int k = k0 + kk;
+    if (k <= k1) {                    /* uniform over the workgroup */
+      __syncthreads();
+      plane_store<FIRST, V>(D, T, E0, E1, R[(kk + 2) % RING], beta,
+                            lds + ((k + 1) & 1) * LPLANE, k + 1);
+      if (k + RING - 1 <= k1 + 1)
+        R[kk % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, k + RING - 1);
+      __syncthreads();
+      const V *P = lds + (k & 1) * LPLANE;
+      V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
+      if (colok) {
+        V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
+                          P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
+                          nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
+                          wall_hi(k, D.nz, D.perz));
+        long c = gidx(C, x, j, k);
+        pnew[c] = cur;
+        Ap[c] = a;
+        acc += (double) cur * (double) a;
+      }
+      prev = cur;
+      cur = nxt;
     }
-    prev = cur;
-    cur = nxt;
   }
   tile_partial(acc, gpart, tile);
 }
@@ -368,8 +457,8 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
   if (colok) {
     int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
     int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
-    double fw2 = fw_lo(x, D.perx), fe2 = fw_hi(x, D.nx, D.perx);
-    double fs2 = fw_lo(j, D.pery), fn2 = fw_hi(j, D.ny, D.pery);
+    bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+    bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
     for (int k = k0; k <= k1; k++) {
       int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
 #define PN(xx, jj, kk) p_new<FIRST, double>(r, pold, gidx(C, xx, jj, kk), \
@@ -378,9 +467,9 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
       double pS = PN(x, jm, k), pN = PN(x, jp, k);
       double pB = PN(x, j, km), pT = PN(x, j, kp);
 #undef PN
-      double a = neg_lap7<double>(K, pC, pW, pE, pS, pN, pB, pT, fw2, fe2,
-                                  fs2, fn2, fw_lo(k, D.perz),
-                                  fw_hi(k, D.nz, D.perz));
+      double a = neg_lap7<double>(K, pC, pW, pE, pS, pN, pB, pT, wW, wE, wS,
+                                  wN, wall_lo(k, D.perz),
+                                  wall_hi(k, D.nz, D.perz));
       long c = gidx(C, x, j, k);
       pnew[c] = pC;
       Ap[c] = a;
@@ -402,17 +491,27 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
   V *__restrict__ r, double *__restrict__ gpart)
 {
   if (st->done | st->inner_done) return;
-  V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
   TILE_PROLOGUE
+  /* p and Ap are loaded before the reduction that yields alpha. */
+  long c0 = gidx(C, x, j, k0);
+  V pv[TILE_Z], apv[TILE_Z];
+#pragma unroll
+  for (int kk = 0; kk < TILE_Z; kk++) {
+    pv[kk] = apv[kk] = 0;
+    if (colok && k0 + kk <= k1) {
+      pv[kk] = p[c0 + (long) kk * C.sz];
+      apv[kk] = Ap[c0 + (long) kk * C.sz];
+    }
+  }
+  V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
   double acc = 0.0;
   if (colok) {
-    long c0 = gidx(C, x, j, k0);
 #pragma unroll
     for (int kk = 0; kk < TILE_Z; kk++) {
       if (k0 + kk <= k1) {
         long c = c0 + (long) kk * C.sz;
-        phi[c] += alpha*p[c];
-        V rr = r[c] - alpha*Ap[c];
+        phi[c] += alpha*pv[kk];
+        V rr = r[c] - alpha*apv[kk];
         r[c] = rr;
         V z = rr*invm_at<V>(T, D, x, j, k0 + kk);
         acc += (double) rr * (double) z;
@@ -449,10 +548,10 @@ __device__ inline double neg_lap_at(const DevDom &D, const Coef &K,
   return neg_lap7(K, f[gidx(C, x, j, k)], f[gidx(C, xm, j, k)],
                   f[gidx(C, xp, j, k)], f[gidx(C, x, jm, k)],
                   f[gidx(C, x, jp, k)], f[gidx(C, x, j, km)],
-                  f[gidx(C, x, j, kp)], fw_lo(x, D.perx),
-                  fw_hi(x, D.nx, D.perx), fw_lo(j, D.pery),
-                  fw_hi(j, D.ny, D.pery), fw_lo(k, D.perz),
-                  fw_hi(k, D.nz, D.perz));
+                  f[gidx(C, x, j, kp)], wall_lo(x, D.perx),
+                  wall_hi(x, D.nx, D.perx), wall_lo(j, D.pery),
+                  wall_hi(j, D.ny, D.pery), wall_lo(k, D.perz),
+                  wall_hi(k, D.nz, D.perz));
 }
 
 /* ... then r = b - (-A) phi, z = invM r + tile partials of (r, z). */

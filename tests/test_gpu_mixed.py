@@ -83,18 +83,23 @@ def check_mixed(res, bounds, pp_residual=1e-8):
 def test_mixed_cavity_32_100_steps(case_dir):
     """Cavity 32^3, Re 100, dt 1e-3, 100 steps."""
     res = run_mixed(case_dir, bbconfig.cavity(n=32, steps=100), 100)
-    check_mixed(res, {"u": 1e-7, "v": 1e-7, "w": 1e-7, "p": 1e-6})
+    check_mixed(res, {"u": 1e-10, "v": 1e-9, "w": 1e-10, "p": 1e-9})
 
 
 def test_mixed_channel_64(case_dir):
     """Periodic channel 64x32x32 (singular Neumann-periodic system), 5 steps."""
     res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 5)
-    check_mixed(res, {"u": 1e-8, "v": 1e-6, "w": 1e-6, "p": 1e-5})
+    check_mixed(res, {"u": 1e-9, "v": 1e-7, "w": 1e-6, "p": 1e-6})
 
 
 def test_mixed_restart_variant(case_dir):
+    """Classic defect correction (CG restarted at every FP64 correction).
+    It meets the same residual test but, having discarded its Krylov space
+    at each restart, stops at a different point of the tolerance ball: the
+    low-wavenumber part of the pressure differs by up to ~kappa * tol
+    (measured 1.3e-4 relative in p, 3e-7 in w)."""
     res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, restart=1)
-    check_mixed(res, {"u": 1e-8, "v": 1e-6, "w": 1e-6, "p": 1e-5})
+    check_mixed(res, {"u": 1e-8, "v": 1e-6, "w": 1e-6, "p": 1e-3})
 
 
 @pytest.mark.parametrize("inner_tol", [0.3, 1e-4])
