@@ -6,6 +6,8 @@ order, plus the optional trailing EXTENSIONS block understood by this build
 without the EXTENSIONS block is a plain reference input file.
 """
 import math
+
+import numpy as np
 import os
 
 FACES = "WESNBT"
@@ -180,3 +182,92 @@ def channel(n=(256, 128, 128), seed=20260101, amp=0.1, precision="PLAIN",
               gradP=((-1.0, 0.0), (0.0, 0.0), (0.0, 0.0)), extensions=ext)
     kw.update(over)
     return kw
+
+
+# ------------------------------- particles --------------------------------
+def part_config(parts, l_over_a=1.0):
+    """Return part.config text in the grammar of src/particle.c:66-391.
+
+    parts: list of dicts with keys r, x, y, z and optional u v w, rho, E,
+    sigma, e_dry, coeff_fric, order, rs_r, translating, rotating.
+    """
+    L = [f"n {len(parts)}", f"(l/a) {_fmt(float(l_over_a))}"]
+    for p in parts:
+        g = lambda k, d: float(p.get(k, d))
+        L.append("")
+        L.append(f"r {_fmt(g('r', 1.0))}")
+        L.append(f"(x, y, z) {_fmt(g('x', 0))} {_fmt(g('y', 0))} "
+                 f"{_fmt(g('z', 0))}")
+        L.append(f"(u, v, w) {_fmt(g('u', 0))} {_fmt(g('v', 0))} "
+                 f"{_fmt(g('w', 0))}")
+        L.append("(aFx, aFy, aFz) 0 0 0")
+        L.append("(aLx, aLy, aLz) 0 0 0")
+        L.append(f"rho {_fmt(g('rho', 2.0))}")
+        L.append(f"E {_fmt(g('E', 1e7))}")
+        L.append(f"sigma {_fmt(g('sigma', 0.5))}")
+        L.append(f"e_dry {_fmt(g('e_dry', 0.98))}")
+        L.append(f"coeff_fric {_fmt(g('coeff_fric', 0.5))}")
+        L.append(f"order {int(p.get('order', 4))}")
+        L.append(f"rs/r {_fmt(g('rs_r', 1.2))}")
+        L.append("spring_k 0")
+        L.append("spring (x, y, z) 0 0 0")
+        L.append("spring_l 0")
+        L.append(f"translating {int(p.get('translating', 1))}")
+        L.append(f"rotating {int(p.get('rotating', 1))}")
+        L.append("s 0")
+        L.append("update 0")
+        L.append("cp 0")
+        L.append("rs 0")
+        L.append("s_order 0")
+    return "\n".join(L) + "\n"
+
+
+def write_parts(dirpath, parts, l_over_a=1.0):
+    os.makedirs(os.path.join(dirpath, "input"), exist_ok=True)
+    with open(os.path.join(dirpath, "input", "part.config"), "w") as f:
+        f.write(part_config(parts, l_over_a))
+
+
+def random_spheres(n, box, a, seed, gap=0.5, max_tries=2000000):
+    """Seeded rejection sampler of n non-overlapping spheres of radius a in
+    a periodic box (surface gap >= `gap`, minimum-image distances)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = np.asarray(box, dtype=np.float64)
+    pos = np.zeros((0, 3))
+    tries = 0
+    while len(pos) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"could not place {n} spheres")
+        c = rng.uniform(0.0, 1.0, 3) * L
+        if len(pos):
+            d = pos - c
+            d -= L * np.round(d / L)
+            if np.min(np.sum(d * d, axis=1)) < (2 * a + gap) ** 2:
+                continue
+        pos = np.vstack([pos, c])
+    return pos
+
+
+def sediment(n=(64, 64, 64), nparts=8, a=6.0, seed=20260102, rho_ratio=2.0,
+             order=4, nu=0.1, galileo=100.0, lamb_residual=1e-2,
+             lamb_max_iter=50, **over):
+    """Triply periodic sedimentation box [0, nx) x [0, ny) x [0, nz), dx = 1,
+    spheres of radius a at seeded random positions, gravity in -z set from
+    the Galileo number Ga = sqrt((rho_p/rho_f - 1) g d^3) / nu."""
+    per = ("PERIODIC",)
+    vel = {c: {f: per for f in FACES} for c in "uvw"}
+    d = 2.0 * a
+    gz = (galileo * nu) ** 2 / ((rho_ratio - 1.0) * d ** 3)
+    kw = dict(n=n, lo=(0.0, 0.0, 0.0), hi=tuple(float(x) for x in n),
+              rho_f=1.0, nu=nu, duration=1e9, CFL=0.5, pp_max_iter=100000,
+              pp_residual=1e-8, p_bc={f: per for f in FACES}, vel_bc=vel,
+              init_cond="QUIESCENT", out_plane="HOMOGENEOUS",
+              lamb_max_iter=lamb_max_iter, lamb_residual=lamb_residual,
+              lamb_relax=1.0, lamb_cut=1e-4,
+              g=((0.0, 0.0), (0.0, 0.0), (-gz, 0.0)), extensions={})
+    kw.update(over)
+    pos = random_spheres(nparts, n, a, seed)
+    parts = [dict(r=a, x=p[0], y=p[1], z=p[2], rho=rho_ratio, order=order)
+             for p in pos]
+    return kw, parts

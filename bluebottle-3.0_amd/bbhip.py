@@ -37,6 +37,34 @@ class PPStats(ctypes.Structure):
                 ("sp_rhs", ctypes.c_double), ("etime", ctypes.c_double)]
 
 
+def _part_fields():
+    d, i = ctypes.c_double, ctypes.c_int
+    f = [("N", i), ("r", d)]
+    f += [(n, d) for n in "x y z u v w u0 v0 w0 udot vdot wdot udot0 vdot0 "
+          "wdot0 axx axy axz ayx ayy ayz azx azy azz ox oy oz ox0 oy0 oz0 "
+          "oxdot oydot ozdot oxdot0 oydot0 ozdot0 Fx Fy Fz Lx Ly Lz aFx aFy "
+          "aFz aLx aLy aLz kFx kFy kFz iFx iFy iFz iLx iLy iLz".split()]
+    f += [("nodes", i * 26), ("rho", d), ("E", d), ("sigma", d),
+          ("order", i), ("rs", d), ("ncoeff", i), ("spring_k", d),
+          ("spring_x", d), ("spring_y", d), ("spring_z", d), ("spring_l", d),
+          ("translating", i), ("rotating", i), ("St", d * 12),
+          ("iSt", i * 12), ("e_dry", d), ("coeff_fric", d)]
+    for n in ("pnm_re", "pnm_im", "phinm_re", "phinm_im", "chinm_re",
+              "chinm_im", "pnm_re0", "pnm_im0", "phinm_re0", "phinm_im0",
+              "chinm_re0", "chinm_im0"):
+        f.append((n, d * 15))
+    f += [("ncoll_part", i), ("ncoll_wall", i), ("s", d), ("update", i),
+          ("srs", d), ("q", d), ("cp", d), ("sorder", i), ("sncoeff", i),
+          ("anm_re", d * 25), ("anm_im", d * 25), ("anm_re0", d * 25),
+          ("anm_im0", d * 25)]
+    return f
+
+
+class PartStruct(ctypes.Structure):
+    """The reference's part_struct, member for member."""
+    _fields_ = _part_fields()
+
+
 _lib = None
 
 
@@ -135,6 +163,27 @@ class Simulation:
         ptr = ctypes.POINTER(ct).in_dll(self.lib, name)
         flat = np.ctypeslib.as_array(ptr, shape=(size,))
         return host_to_ijk(flat, grid, self.n)
+
+    def parts(self):
+        """Device particle array copied to the host, as a list of dicts."""
+        n = ctypes.c_int.in_dll(self.lib, "nparts").value
+        if n == 0:
+            return []
+        self.lib.cuda_part_pull()
+        ptr = ctypes.POINTER(PartStruct).in_dll(self.lib, "parts")
+        out = []
+        for k in range(n):
+            P = ptr[k]
+            d = {}
+            for name, ct in PartStruct._fields_:
+                v = getattr(P, name)
+                d[name] = np.array(v[:]) if hasattr(v, "_length_") else v
+            out.append(d)
+        return out
+
+    def lamb_stats(self):
+        return (ctypes.c_int.in_dll(self.lib, "bb_lamb_iters").value,
+                ctypes.c_double.in_dll(self.lib, "bb_lamb_err").value)
 
     def scalar(self, name, ctype=ctypes.c_double):
         return ctype.in_dll(self.lib, name).value

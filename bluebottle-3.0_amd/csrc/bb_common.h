@@ -35,6 +35,7 @@
 #define BB_COMMON_H
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -79,6 +80,7 @@ struct DevDom {
 };
 
 extern DevDom ddom;                   /* host copy, filled at malloc time */
+extern uint8_t *_bb_fluid;            /* 1: fluid cell, 0: particle cell */
 extern hipStream_t bb_stream;         /* the library's single stream */
 
 /* Scratch used by reductions. */

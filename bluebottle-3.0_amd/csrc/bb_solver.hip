@@ -33,6 +33,8 @@
 #include <math.h>
 #include <sys/time.h>
 
+#include <stdint.h>
+
 #include "bb_common.h"
 
 /* Stencil weights 1/dx^2, ... in the arithmetic type V of the vectors. */
@@ -101,6 +103,19 @@ __device__ inline V invm_at(const InvMT<V> &T, const DevDom &D, int x, int j,
              9 * wcount(k, D.nz, D.perz)];
 }
 
+/* Particle cells take no part in the solve: their right-hand side is zero
+ * (src/particle_kernel.cu:1748-1753) and their rows of the particle SpMV
+ * (src/solver_kernel.cu:652-712) give exactly zero while the search
+ * direction is zero there.  With the Jacobi factor set to zero in particle
+ * cells (`fluid` = 0) the search direction stays zero in them, so the
+ * fluid rows see the same neighbours and every dot product the same terms;
+ * the residual in particle cells is never used.  fluid == NULL: no
+ * particles. */
+__device__ inline bool solid(const uint8_t *__restrict__ fluid, long c)
+{
+  return fluid != NULL && fluid[c] == 0;
+}
+
 /* Neighbour index with periodic wrap (ghost index in a wall direction). */
 __device__ inline int wrap_lo(int a, int n, int per) { return (per && a == 1) ? n : a - 1; }
 __device__ inline int wrap_hi(int a, int n, int per) { return (per && a == n) ? 1 : a + 1; }
@@ -150,8 +165,8 @@ extern "C" void cuda_PP_init_jacobi_preconditioner(void)
 __global__ void __launch_bounds__(TILE_THREADS)
 rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
   const double *__restrict__ us, const double *__restrict__ vs,
-  const double *__restrict__ ws, double *__restrict__ rhs,
-  double *__restrict__ gpart)
+  const double *__restrict__ ws, const uint8_t *__restrict__ fluid,
+  double *__restrict__ rhs, double *__restrict__ gpart)
 {
   TILE_PROLOGUE
   double acc = 0.0;
@@ -164,6 +179,7 @@ rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
       s += (ws[cw + D.fz.sz] - ws[cw]) * idz;
       s *= rho_idt;
       double b = -s;
+      if (solid(fluid, gidx(C, x, j, k))) b = 0.;
       rhs[gidx(C, x, j, k)] = b;
       acc += b * b;
     }
@@ -176,8 +192,8 @@ rhs_kernel(DevDom D, double idx, double idy, double idz, double rho_idt,
  * p = z is formed by the first search_spmv pass. */
 __global__ void __launch_bounds__(TILE_THREADS)
 cg_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
-  double *__restrict__ r, double *__restrict__ phi,
-  double *__restrict__ gpart)
+  const uint8_t *__restrict__ fluid, double *__restrict__ r,
+  double *__restrict__ phi, double *__restrict__ gpart)
 {
   TILE_PROLOGUE
   double acc = 0.0;
@@ -185,7 +201,7 @@ cg_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double t = rhs[c];
-      double t2 = t * invm_at(T, D, x, j, k);
+      double t2 = t * (solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
       r[c] = t;
       phi[c] = 0.;
       acc += t * t2;
@@ -321,7 +337,7 @@ __device__ inline V ld32(const V *__restrict__ a, unsigned i)
 template <bool FIRST, typename V>
 __device__ inline PlaneRaw<V> plane_load(const DevDom &D,
   const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
-  const V *__restrict__ pold, int kk)
+  const V *__restrict__ pold, const uint8_t *__restrict__ fluid, int kk)
 {
   PlaneRaw<V> R;
   R.r0 = R.p0 = R.r1 = R.p1 = 0;
@@ -330,10 +346,12 @@ __device__ inline PlaneRaw<V> plane_load(const DevDom &D,
   if (ks && E0.cxy >= 0) {
     R.r0 = ld32(r, E0.cxy + off);
     if (!FIRST) R.p0 = ld32(pold, E0.cxy + off);
+    if (fluid != NULL && fluid[E0.cxy + off] == 0) R.r0 = 0;
   }
   if (ks && E1.cxy >= 0) {
     R.r1 = ld32(r, E1.cxy + off);
     if (!FIRST) R.p1 = ld32(pold, E1.cxy + off);
+    if (fluid != NULL && fluid[E1.cxy + off] == 0) R.r1 = 0;
   }
   return R;
 }
@@ -380,6 +398,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles,
   const V *__restrict__ r, const V *__restrict__ pold,
+  const uint8_t *__restrict__ fluid,
   V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
 {
   __shared__ V lds[2 * LPLANE];
@@ -391,13 +410,13 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
    * p; plane k + RING - 1 is loaded while plane k is computed. */
   constexpr int RING = sizeof(V) == 8 ? 3 : 4;
   PlaneRaw<V> R[RING];
-  R[0] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 - 1);
-  R[1] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0);
-  R[2] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 + 1);
+  R[0] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 - 1);
+  R[1] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0);
+  R[2] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 1);
   double beta64 = 0.;
   if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
   if (RING > 3 && k0 + 2 <= k1 + 1)
-    R[3 % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, k0 + 2);
+    R[3 % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 2);
   V beta = (V) beta64;
   int xl = lane + 1, yl = wv + 1;
   bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
@@ -418,7 +437,7 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       plane_store<FIRST, V>(D, T, E0, E1, R[(kk + 2) % RING], beta,
                             lds + ((k + 1) & 1) * LPLANE, k + 1);
       if (k + RING - 1 <= k1 + 1)
-        R[kk % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, k + RING - 1);
+        R[kk % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + RING - 1);
       __syncthreads();
       const V *P = lds + (k & 1) * LPLANE;
       V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
@@ -446,6 +465,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
 search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles,
   const double *__restrict__ r, const double *__restrict__ pold,
+  const uint8_t *__restrict__ fluid,
   double *__restrict__ pnew, double *__restrict__ Ap,
   double *__restrict__ gpart)
 {
@@ -462,6 +482,7 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
     for (int k = k0; k <= k1; k++) {
       int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
 #define PN(xx, jj, kk) p_new<FIRST, double>(r, pold, gidx(C, xx, jj, kk), \
+        solid(fluid, gidx(C, xx, jj, kk)) ? 0. : \
         invm_at<double>(T, D, xx, jj, kk), beta)
       double pC = PN(x, j, k), pW = PN(xm, j, k), pE = PN(xp, j, k);
       double pS = PN(x, jm, k), pN = PN(x, jp, k);
@@ -487,7 +508,8 @@ template <typename V>
 __global__ void __launch_bounds__(TILE_THREADS)
 update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
   const double *__restrict__ ppart, long ntiles,
-  const V *__restrict__ p, const V *__restrict__ Ap, V *__restrict__ phi,
+  const V *__restrict__ p, const V *__restrict__ Ap,
+  const uint8_t *__restrict__ fluid, V *__restrict__ phi,
   V *__restrict__ r, double *__restrict__ gpart)
 {
   if (st->done | st->inner_done) return;
@@ -513,7 +535,7 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
         phi[c] += alpha*pv[kk];
         V rr = r[c] - alpha*apv[kk];
         r[c] = rr;
-        V z = rr*invm_at<V>(T, D, x, j, k0 + kk);
+        V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
         acc += (double) rr * (double) z;
       }
     }
@@ -558,7 +580,8 @@ __device__ inline double neg_lap_at(const DevDom &D, const Coef &K,
 __global__ void __launch_bounds__(TILE_THREADS)
 residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
   const double *__restrict__ phi, const double *__restrict__ rhs,
-  double *__restrict__ r, double *__restrict__ gpart)
+  const uint8_t *__restrict__ fluid, double *__restrict__ r,
+  double *__restrict__ gpart)
 {
   if (st->done) return;
   TILE_PROLOGUE
@@ -568,7 +591,7 @@ residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
       r[c] = rr;
-      double z = rr*invm_at(T, D, x, j, k);
+      double z = rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
       acc += rr * z;
     }
   }
@@ -578,12 +601,14 @@ residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
 /* Tile partials of ||b - (-A) phi||^2 (diagnostic true residual). */
 __global__ void __launch_bounds__(TILE_THREADS)
 true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
-  const double *__restrict__ rhs, double *__restrict__ gpart)
+  const double *__restrict__ rhs, const uint8_t *__restrict__ fluid,
+  double *__restrict__ gpart)
 {
   TILE_PROLOGUE
   double acc = 0.0;
   if (colok) {
     for (int k = k0; k <= k1; k++) {
+      if (solid(fluid, gidx(C, x, j, k))) continue;
       double rr = rhs[gidx(C, x, j, k)] - neg_lap_at(D, K, phi, x, j, k);
       acc += rr * rr;
     }
@@ -607,8 +632,8 @@ cg_check_kernel(CGState *st, int q, const double *__restrict__ rpart, long n)
  * in FP64. */
 __global__ void __launch_bounds__(TILE_THREADS)
 mx_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
-  double *__restrict__ phi, float *__restrict__ e, float *__restrict__ r,
-  double *__restrict__ gpart)
+  const uint8_t *__restrict__ fluid, double *__restrict__ phi,
+  float *__restrict__ e, float *__restrict__ r, double *__restrict__ gpart)
 {
   TILE_PROLOGUE
   double acc = 0.0;
@@ -619,7 +644,7 @@ mx_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
       phi[c] = 0.;
       e[c] = 0.f;
       r[c] = (float) t;
-      acc += t * (t*invm_at(T, D, x, j, k));
+      acc += t * (t*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k)));
     }
   }
   tile_partial(acc, gpart, tile);
@@ -642,8 +667,8 @@ mx_fold_kernel(DevDom D, double *__restrict__ phi, float *__restrict__ e)
  * the tile partials of (r, M^-1 r) in FP64. */
 __global__ void __launch_bounds__(TILE_THREADS)
 mx_residual_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ phi,
-  const double *__restrict__ rhs, float *__restrict__ r,
-  double *__restrict__ gpart)
+  const double *__restrict__ rhs, const uint8_t *__restrict__ fluid,
+  float *__restrict__ r, double *__restrict__ gpart)
 {
   TILE_PROLOGUE
   double acc = 0.0;
@@ -652,7 +677,7 @@ mx_residual_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ phi,
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
       r[c] = (float) rr;
-      acc += rr * (rr*invm_at(T, D, x, j, k));
+      acc += rr * (rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k)));
     }
   }
   tile_partial(acc, gpart, tile);
@@ -690,6 +715,12 @@ static double now_s(void)
 }
 
 #define TILES_LAUNCH tiles_grid(ddom), tiles_block(), 0, bb_stream
+
+/* The particle mask of the PCG, NULL without particles. */
+static const uint8_t *fluid_mask(void)
+{
+  return NPARTS > 0 ? _bb_fluid : NULL;
+}
 
 static CGState *_cg_state = NULL;
 static CGState *cg_host = NULL;
@@ -776,7 +807,7 @@ void bb_pp_rhs(void)
   cg_buffers();
   hipLaunchKernelGGL(rhs_kernel, TILES_LAUNCH, ddom, 1. / dom[0].dx,
     1. / dom[0].dy, 1. / dom[0].dz, rho_f / dt, _u_star, _v_star, _w_star,
-    _rhs_p, _part_p);
+    fluid_mask(), _rhs_p, _part_p);
   BB_LAUNCH_CHECK();
 }
 
@@ -784,7 +815,7 @@ void bb_pp_rhs(void)
 double bb_true_residual(double sp_rhs)
 {
   hipLaunchKernelGGL(true_residual_kernel, TILES_LAUNCH, ddom, coef(), _phi,
-    _rhs_p, _part_r);
+    _rhs_p, fluid_mask(), _part_r);
   BB_LAUNCH_CHECK();
   bb_reduce_rows(_part_r, n_tiles(ddom), _bb_scalars + 8);
   double rr = bb_fetch_scalar(8);
@@ -801,30 +832,31 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   if (bb_spmv_variant == 1) {
     if (q == 1)
       hipLaunchKernelGGL((search_spmv_kernel<true, double, false>),
-        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew,
+        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
         _Apb_q, _part_p);
     else
       hipLaunchKernelGGL((search_spmv_kernel<false, double, false>),
-        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew,
+        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
         _Apb_q, _part_p);
   } else {
     if (q == 1)
       hipLaunchKernelGGL(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
-        K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew, _Apb_q, _part_p);
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew, _Apb_q, _part_p);
     else
       hipLaunchKernelGGL(search_spmv_direct_kernel<false>, TILES_LAUNCH, ddom,
-        K, T, _cg_state, q, _part_r, nt, _r_q, pold, pnew, _Apb_q, _part_p);
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew, _Apb_q, _part_p);
   }
   bb_kt_end(KT_SPMV);
   if (q % 50 == 0) {
     hipLaunchKernelGGL(update_solution_kernel, TILES_LAUNCH, ddom, _cg_state,
       q, _part_p, nt, pnew, _phi);
     hipLaunchKernelGGL(residual_kernel, TILES_LAUNCH, ddom, K, T, _cg_state,
-      _phi, _rhs_p, _r_q, _part_r);
+      _phi, _rhs_p, fluid_mask(), _r_q, _part_r);
   } else {
     bb_kt_begin(KT_UPDATE);
     hipLaunchKernelGGL(update_kernel<double>, TILES_LAUNCH, ddom, T,
-      _cg_state, q, _part_p, nt, pnew, _Apb_q, _phi, _r_q, _part_r);
+      _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi, _r_q,
+      _part_r);
     bb_kt_end(KT_UPDATE);
   }
   BB_LAUNCH_CHECK();
@@ -840,8 +872,8 @@ static void pcg_plain(double ts)
   const long nt = n_tiles(ddom);
   cg_buffers();
 
-  hipLaunchKernelGGL(cg_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p, _r_q,
-    _phi, _part_r);
+  hipLaunchKernelGGL(cg_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
+    fluid_mask(), _r_q, _phi, _part_r);
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual, 0.);
   BB_LAUNCH_CHECK();
@@ -908,8 +940,8 @@ extern "C" void cuda_PP_cg_noparts(void)
 
 extern "C" void cuda_PP_cg(void)
 {
-  /* The particle form differs only where cells carry a particle phase; with
-   * no particles present it is the same computation. */
+  /* The particle form (src/cuda_solver.cu:39-300): the same iteration with
+   * the particle cells masked out (fluid_mask). */
   cuda_PP_cg_noparts();
 }
 
@@ -924,16 +956,18 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   bb_kt_begin(KT_SPMV32);
   if (first)
     hipLaunchKernelGGL((search_spmv_kernel<true, float, true>), TILES_LAUNCH,
-      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, pnew, _Ap32,
+      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
+      _Ap32,
       _part_p);
   else
     hipLaunchKernelGGL((search_spmv_kernel<false, float, true>), TILES_LAUNCH,
-      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, pnew, _Ap32,
+      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
+      _Ap32,
       _part_p);
   bb_kt_end(KT_SPMV32);
   bb_kt_begin(KT_UPDATE32);
   hipLaunchKernelGGL(update_kernel<float>, TILES_LAUNCH, ddom, T, _cg_state,
-    q, _part_p, nt, pnew, _Ap32, _e32, _r32, _part_r);
+    q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
   bb_kt_end(KT_UPDATE32);
   BB_LAUNCH_CHECK();
 }
@@ -965,8 +999,8 @@ static void pcg_mixed(double ts)
   cg_buffers();
   cg_buffers32();
 
-  hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p, _phi,
-    _e32, _r32, _part_r);
+  hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
+    fluid_mask(), _phi, _e32, _r32, _part_r);
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual,
     pp_inner_tol*pp_inner_tol);
@@ -1003,7 +1037,7 @@ static void pcg_mixed(double ts)
     q = cg_host->q;
     hipLaunchKernelGGL(mx_fold_kernel, TILES_LAUNCH, ddom, _phi, _e32);
     hipLaunchKernelGGL(mx_residual_kernel, TILES_LAUNCH, ddom, K, T, _phi,
-      _rhs_p, _r32, _part_r);
+      _rhs_p, fluid_mask(), _r32, _part_r);
     hipLaunchKernelGGL(mx_state_kernel, dim3(1), dim3(TILE_THREADS), 0,
       bb_stream, _cg_state, _part_r, nt, q, bb_mixed_restart);
     BB_LAUNCH_CHECK();

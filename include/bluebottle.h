@@ -188,9 +188,125 @@ extern long int cpumem, gpumem;
 extern int SCALAR;
 extern real s_D, s_k, s_beta, s_ref, lamb_cut_scalar, s_init, s_init_rand;
 
-/* Particle counts; zero in the single-phase flow path. */
-extern int NPARTS;
-extern int nparts;
+/* ----------------- Particles (src/particle.h, src/physalis.h) ------------- */
+#define NNODES 26                 /* Lebedev nodes, src/particle.h:42 */
+#define MAX_COEFFS 15             /* src/particle.h:55 */
+#define S_MAX_COEFFS 25           /* src/particle.h:70 */
+#define MAX_NEIGHBORS 12          /* src/particle.h:83 */
+#define NCOEFFS 15                /* src/physalis.h:52 */
+#define DIV_ST -1e-10             /* src/physalis.h:40 */
+/* Node codes of quadrature nodes cut by a wall (src/physalis.h:64-196). */
+#define WEST_WALL_D -100
+#define WEST_WALL_N -101
+#define EAST_WALL_D -110
+#define EAST_WALL_N -111
+#define SOUTH_WALL_D -120
+#define SOUTH_WALL_N -121
+#define NORTH_WALL_D -130
+#define NORTH_WALL_N -131
+#define BOTTOM_WALL_D -140
+#define BOTTOM_WALL_N -141
+#define TOP_WALL_D -150
+#define TOP_WALL_N -151
+
+/* One particle, member for member as src/particle.h:343-446 (the host and
+ * device arrays hold this layout; 3128 bytes on x86-64). */
+typedef struct part_struct {
+  int N;
+  real r;
+  real x, y, z;
+  real u, v, w;
+  real u0, v0, w0;
+  real udot, vdot, wdot;
+  real udot0, vdot0, wdot0;
+  real axx, axy, axz, ayx, ayy, ayz, azx, azy, azz;
+  real ox, oy, oz;
+  real ox0, oy0, oz0;
+  real oxdot, oydot, ozdot;
+  real oxdot0, oydot0, ozdot0;
+  real Fx, Fy, Fz;
+  real Lx, Ly, Lz;
+  real aFx, aFy, aFz;
+  real aLx, aLy, aLz;
+  real kFx, kFy, kFz;
+  real iFx, iFy, iFz;
+  real iLx, iLy, iLz;
+  int nodes[NNODES];
+  real rho;
+  real E;
+  real sigma;
+  int order;
+  real rs;
+  int ncoeff;
+  real spring_k;
+  real spring_x, spring_y, spring_z;
+  real spring_l;
+  int translating;
+  int rotating;
+  real St[MAX_NEIGHBORS];
+  int iSt[MAX_NEIGHBORS];
+  real e_dry;
+  real coeff_fric;
+  real pnm_re[MAX_COEFFS];
+  real pnm_im[MAX_COEFFS];
+  real phinm_re[MAX_COEFFS];
+  real phinm_im[MAX_COEFFS];
+  real chinm_re[MAX_COEFFS];
+  real chinm_im[MAX_COEFFS];
+  real pnm_re0[MAX_COEFFS];
+  real pnm_im0[MAX_COEFFS];
+  real phinm_re0[MAX_COEFFS];
+  real phinm_im0[MAX_COEFFS];
+  real chinm_re0[MAX_COEFFS];
+  real chinm_im0[MAX_COEFFS];
+  int ncoll_part;
+  int ncoll_wall;
+  real s;
+  int update;
+  real srs;
+  real q;
+  real cp;
+  int sorder;
+  int sncoeff;
+  real anm_re[S_MAX_COEFFS];
+  real anm_im[S_MAX_COEFFS];
+  real anm_re0[S_MAX_COEFFS];
+  real anm_im0[S_MAX_COEFFS];
+} part_struct;
+
+/* Bin grid of the collision search (src/particle.h:545-650). */
+typedef struct bin_grid_info {
+  int _is, _ie, _isb, _ieb, in, inb;
+  int _js, _je, _jsb, _jeb, jn, jnb;
+  int _ks, _ke, _ksb, _keb, kn, knb;
+  int s1, s2, s3, s1b, s2b, s3b;
+  int s2_i, s2_j, s2_k, s2b_i, s2b_j, s2b_k;
+} bin_grid_info;
+
+typedef struct bin_struct {
+  bin_grid_info Gcc;
+  real xs, xe, xl;
+  int xn;
+  real dx;
+  real ys, ye, yl;
+  int yn;
+  real dy;
+  real zs, ze, zl;
+  int zn;
+  real dz;
+} bin_struct;
+
+/* Particle globals (src/particle.c:27-50). */
+extern int NPARTS;                   /* particles in the domain */
+extern int nparts;                   /* particles held (all of them here) */
+extern part_struct *parts;           /* host copy */
+extern part_struct *_parts;          /* device copy */
+extern bin_struct bins;
+extern real interaction_length_ratio;
+extern real bin_size;
+extern int ncoeffs_max;
+extern int s_ncoeffs_max;
+extern real volume_fraction, total_parts_mass, total_parts_volume, rho_avg;
 
 /* Optional trailing config keys (absent from the reference grammar; a file
  * without them parses exactly as before). */
@@ -288,6 +404,29 @@ void cuda_PP_cg(void);                       /* src/cuda_solver.cu:39-300 */
 void cuda_PP_cg_noparts(void);               /* src/cuda_solver.cu:574-761 */
 /* No counterpart in the reference: the FP32-inner / FP64-outer form. */
 void cuda_PP_cg_mixed(void);
+/* Particles and Physalis.  On one GPU every particle is held once; a
+ * particle whose cage or quadrature nodes cross a periodic boundary is
+ * handled through its periodic image positions x +- L (the same values the
+ * reference's ghost copies carry, src/particle_kernel.cu:1166-1210). */
+void parts_read_input(void);                 /* src/particle.c:66-391 */
+void parts_init(void);                       /* src/particle.c:393-554 */
+void init_bins(void);                        /* src/particle.c:744-834 */
+void parts_free(void);                       /* src/particle.c:876-879 */
+void cuda_part_malloc_dev(void);             /* src/cuda_particle.cu */
+void cuda_part_push(void);                   /* host parts -> device */
+void cuda_part_pull(void);                   /* device parts -> host */
+void cuda_part_free(void);
+void cuda_init_physalis(void);               /* src/cuda_physalis.cu:61-135 */
+void cuda_part_BC(void);                     /* src/cuda_particle.cu:1649-1663 */
+void cuda_part_BC_star(void);                /* :1665-1679 */
+void cuda_part_BC_p(void);                   /* :1681-1686 */
+void cuda_part_p_fill(void);                 /* :1688-1693 */
+void cuda_parts_internal(void);              /* :1695-1706 */
+void cuda_lamb(void);                        /* src/cuda_physalis.cu:137-220 */
+void cuda_lamb_err(real *error, int *number);   /* :859-903 */
+void cuda_move_parts_sub(void);              /* src/cuda_particle.cu:1335-1473 */
+void cuda_update_part_velocity(void);        /* :1477-1496 */
+void cuda_update_part_position(void);        /* :1498-1513 + periodic wrap */
 /* Periodic ghost refresh of one field; replaces mpi_cuda_exchange_G*
  * (src/mpi_comm.c:257-405) on a single GPU.  `grid` is GCC/GFX/GFY/GFZ. */
 void cuda_exchange(real *array, int grid);

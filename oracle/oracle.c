@@ -58,6 +58,8 @@ static struct {
   int stepnum;
   int niter;
   double resid, true_resid, sp_rhs;
+  int lamb_iters;
+  double lamb_err;
 } O;
 
 static double *p_, *p0_, *phi_, *u_, *u0_, *v_, *v0_, *w_, *w0_;
@@ -122,6 +124,10 @@ static int get_bc(const char *key, double *a, double *b)
   exit(1);
 }
 
+static int O_lamb_max_iter;
+static double O_lamb_residual, O_lamb_relax, O_lamb_cut;
+static double O_sbeta, O_sref, O_g[3], O_gm[3], O_ga[3];
+
 static int parse(const char *dir)
 {
   char path[1024];
@@ -178,6 +184,17 @@ static int parse(const char *dir)
     O.gP[c] = 0.;
   }
   O.osci_f = get_num("osci_f");
+  O_lamb_max_iter = find_line("lamb_max_iter", rest) ? atoi(rest) : 1;
+  O_lamb_residual = find_line("lamb_residual", rest) ? atof(rest) : 0.;
+  O_lamb_relax = find_line("lamb_relax", rest) ? atof(rest) : 1.;
+  O_lamb_cut = find_line("lamb_cut", rest) ? atof(rest) : 0.;
+  O_sbeta = find_line("s_beta", rest) ? atof(rest) : 0.;
+  O_sref = find_line("s_ref", rest) ? atof(rest) : 0.;
+  const char *gkk[] = {"g.x", "g.y", "g.z"};
+  for (int c = 0; c < 3; c++) {
+    O_gm[c] = O_ga[c] = O_g[c] = 0.;
+    if (find_line(gkk[c], rest)) sscanf(rest, "%lf %lf", &O_gm[c], &O_ga[c]);
+  }
   O.fixed_dt = find_line("fixed_dt", rest) ? atof(rest) : 0.;
   O.perturb_amp = 0.;
   O.perturb_seed = 0;
@@ -578,6 +595,8 @@ void orc_forcing(void)
   double delta = O.ttime - O.p_bc_tdelay;
   for (int c = 0; c < 3; c++) O.gP[c] = rampv(O.gPm[c], O.gPa[c], delta, O.gP[c]);
   O.gP[2] = O.gP[2] * cos(O.osci_f*O.ttime);
+  double dg = O.ttime - O.g_bc_tdelay;
+  for (int c = 0; c < 3; c++) O_g[c] = rampv(O_gm[c], O_ga[c], dg, O_g[c]);
   double fv[3];
   for (int c = 0; c < 3; c++) fv[c] = -O.gP[c]/O.rho_f;
   for (long q = 0; q < O.fx.n; q++) { fx_[q] = 0.; fx_[q] += fv[0]; }
@@ -794,6 +813,11 @@ void orc_solvability(void)
 }
 
 /* --------------------------------- PCG ---------------------------------- */
+#define FOR_CC for (int k = 1; k <= O.nz; k++) for (int j = 1; j <= O.ny; j++) \
+               for (int i = 1; i <= O.nx; i++)
+
+#include "parts.c"
+
 void orc_jacobi(void)
 {
   /* src/solver_kernel.cu:26-87 (-DJACOBI) */
@@ -830,9 +854,53 @@ static void spmv(const double *x, double *Ax)
       }
 }
 
+/* src/solver_kernel.cu:528-713: (-A) x with the particle modification:
+ * in a particle cell the diagonal factor is -dx^2/6 and the off-diagonals
+ * vanish; in a fluid cell the link to a particle neighbour drops its
+ * off-diagonal. */
+static void spmv_parts(const double *x, double *Ax)
+{
+  double idx2 = 1./(O.dx * O.dx), idy2 = 1./(O.dy * O.dy),
+         idz2 = 1./(O.dz * O.dz);
+  const OG C = O.cc;
+  #pragma omp parallel for schedule(static)
+  for (int k = 1; k <= O.nz; k++)
+    for (int j = 1; j <= O.ny; j++)
+      for (int i = 1; i <= O.nx; i++) {
+        long c = IDX(C, i, j, k);
+        int fe = flu_[IDX(O.fx, i + 1, j, k)], fw = flu_[IDX(O.fx, i, j, k)];
+        int fn = flv_[IDX(O.fy, i, j + 1, k)], fs = flv_[IDX(O.fy, i, j, k)];
+        int ft = flw_[IDX(O.fz, i, j, k + 1)], fb = flw_[IDX(O.fz, i, j, k)];
+        int pc = phase_[c];
+        double pfx = -(O.dx * O.dx)/6. * (pc > -1) + (double) (pc == -1);
+        double pfy = -(O.dy * O.dy)/6. * (pc > -1) + (double) (pc == -1);
+        double pfz = -(O.dz * O.dz)/6. * (pc > -1) + (double) (pc == -1);
+        int fl = pc == -1;
+        int pfe = fl * !(fl && phase_[c + 1] > -1);
+        int pfw = fl * !(fl && phase_[c - 1] > -1);
+        int pfn = fl * !(fl && phase_[c + C.sy] > -1);
+        int pfs = fl * !(fl && phase_[c - C.sy] > -1);
+        int pft = fl * !(fl && phase_[c + C.sz] > -1);
+        int pfb = fl * !(fl && phase_[c - C.sz] > -1);
+        double a = -idx2 * (fe*fe*(x[c + 1]*pfe - pfx*x[c])
+                          - fw*fw*(x[c]*pfx - pfw*x[c - 1]));
+        a += -idy2 * (fn*fn*(x[c + C.sy]*pfn - pfy*x[c])
+                    - fs*fs*(x[c]*pfy - pfs*x[c - C.sy]));
+        a += -idz2 * (ft*ft*(x[c + C.sz]*pft - pfz*x[c])
+                    - fb*fb*(x[c]*pfz - pfb*x[c - C.sz]));
+        Ax[c] = a;
+      }
+}
+
+static void spmv_any(const double *x, double *Ax)
+{
+  if (NP_ > 0) spmv_parts(x, Ax);
+  else spmv(x, Ax);
+}
+
 void orc_apply_A(const double *x, double *Ax)
 {
-  spmv(x, Ax);
+  spmv_any(x, Ax);
 }
 
 static double now_s(void)
@@ -841,9 +909,6 @@ static double now_s(void)
   gettimeofday(&t, 0);
   return t.tv_sec + t.tv_usec * 1e-6;
 }
-
-#define FOR_CC for (int k = 1; k <= O.nz; k++) for (int j = 1; j <= O.ny; j++) \
-               for (int i = 1; i <= O.nx; i++)
 
 int orc_pcg(void)
 {
@@ -864,6 +929,29 @@ int orc_pcg(void)
         s *= rho_idt;
         rhs_[IDX(O.cc, i, j, k)] = -s;
       }
+  if (NP_ > 0) {
+    /* src/particle_kernel.cu:1744-1753: the right-hand side keeps its value
+     * in fluid cells (phase_shell is 1 there) and is multiplied by zero in
+     * particle cells; src/solver_kernel.cu:212-256 then adds -rhs/dx^2 of
+     * particle neighbours, which are zero. */
+    FOR_CC {
+      long c = IDX(O.cc, i, j, k);
+      rhs_[c] = (double) (phase_[c] < 0 && shell_[c]) * rhs_[c];
+    }
+    orc_exchange(rhs_, 0);
+    double idx2 = 1./(O.dx * O.dx), idy2 = 1./(O.dy * O.dy),
+           idz2 = 1./(O.dz * O.dz);
+    FOR_CC {
+      long c = IDX(O.cc, i, j, k);
+      int is_fluid = phase_[c] == -1;
+      rhs_[c] += is_fluid * (phase_[c + 1] > -1) * idx2 * (-rhs_[c + 1]);
+      rhs_[c] += is_fluid * (phase_[c - 1] > -1) * idx2 * (-rhs_[c - 1]);
+      rhs_[c] += is_fluid * (phase_[c + O.cc.sy] > -1) * idy2 * (-rhs_[c + O.cc.sy]);
+      rhs_[c] += is_fluid * (phase_[c - O.cc.sy] > -1) * idy2 * (-rhs_[c - O.cc.sy]);
+      rhs_[c] += is_fluid * (phase_[c + O.cc.sz] > -1) * idz2 * (-rhs_[c + O.cc.sz]);
+      rhs_[c] += is_fluid * (phase_[c - O.cc.sz] > -1) * idz2 * (-rhs_[c - O.cc.sz]);
+    }
+  }
   double sp_rhs = dot_cc(rhs_, rhs_);
   orc_exchange(rhs_, 0);
   FOR_CC {
@@ -882,13 +970,13 @@ int orc_pcg(void)
   }
   while (q <= O.pp_max_iter) {
     ++q;
-    spmv(pbq_, Apq_);
+    spmv_any(pbq_, Apq_);
     double DENOM = dot_cc(pq_, Apq_);
     double alpha = sp_rq/DENOM;
     if (q % 50 == 0) {
       FOR_CC { long c = IDX(O.cc, i, j, k); phi_[c] += alpha*pq_[c]; }
       orc_exchange(phi_, 0);
-      spmv(phi_, Apq_);
+      spmv_any(phi_, Apq_);
       FOR_CC {
         long c = IDX(O.cc, i, j, k);
         rq_[c] = rhs_[c] - Apq_[c];
@@ -938,7 +1026,7 @@ int orc_pcg(void)
   double *tmp = dal(O.cc.n), *ph = dal(O.cc.n);
   memcpy(ph, phi_, O.cc.n * sizeof(double));
   orc_exchange(ph, 0);
-  spmv(ph, tmp);
+  spmv_any(ph, tmp);
   FOR_CC { long c = IDX(O.cc, i, j, k); tmp[c] = rhs_[c] - tmp[c]; }
   O.true_resid = sqrt(dot_cc(tmp, tmp))/sqrt(sp_rhs);
   free(tmp);
@@ -1057,9 +1145,12 @@ int orc_setup(const char *dir)
   tbuf_ = dal((long) O.nx * O.ny * O.nz);
   for (long c = 0; c < O.cc.n; c++) { phase_[c] = -1; shell_[c] = -1; }
   init_fields();
+  rc = parts_parse(dir);
+  if (rc) return rc;
+  o_init_physalis();
   /* src/bluebottle.c:113-135 */
   orc_exchange(p_, 0); exch_vel(u_, v_, w_);
-  orc_build_flags();
+  orc_build_cages();
   orc_find_dt();
   orc_bc_all();
   orc_exchange(p_, 0); exch_vel(u_, v_, w_);
@@ -1069,32 +1160,66 @@ int orc_setup(const char *dir)
 
 int orc_step(void)
 {
-  /* src/bluebottle.c:168-407 without particles: one Lamb pass. */
+  /* src/bluebottle.c:168-407: the Lamb iteration repeats u* ... pressure
+   * until the Lamb coefficients settle (one pass without particles). */
   O.ttime += O.dt;
   O.stepnum++;
   orc_forcing();
   orc_vel_bc_ramp();
-  orc_u_star();
-  exch_vel(us_, vs_, ws_);
-  orc_bc_star();
-  exch_vel(us_, vs_, ws_);
-  orc_solvability();
-  orc_bc_star();
-  exch_vel(us_, vs_, ws_);
-  int q = orc_pcg();
-  if (q < 0) return q;
-  orc_exchange(phi_, 0);
-  orc_bc_p(phi_);
-  orc_project();
-  exch_vel(u_, v_, w_);
-  orc_bc_all();                       /* dom_BC applies p, u, v, w */
-  exch_vel(u_, v_, w_);
-  orc_update_p();
-  orc_bc_p(p_);
-  orc_exchange(p_, 0);
+  int q = 0, iter = 0;
+  double err = DBL_MAX;
+  while (err > O_lamb_residual) {
+    orc_u_star();
+    exch_vel(us_, vs_, ws_);
+    o_part_bc(us_, vs_, ws_);
+    orc_bc_star();
+    exch_vel(us_, vs_, ws_);
+    orc_solvability();
+    o_part_bc(us_, vs_, ws_);
+    orc_bc_star();
+    exch_vel(us_, vs_, ws_);
+    q = orc_pcg();
+    if (q < 0) return q;
+    orc_exchange(phi_, 0);
+    orc_bc_p(phi_);
+    orc_project();
+    exch_vel(u_, v_, w_);
+    o_part_bc(u_, v_, w_);
+    orc_bc_all();                     /* dom_BC applies p, u, v, w */
+    exch_vel(u_, v_, w_);
+    orc_update_p();
+    o_part_bc(u_, v_, w_);
+    o_part_p_fill();
+    orc_bc_p(p_);
+    orc_exchange(p_, 0);
+    if (NP_ > 0) {
+      int num;
+      o_move_sub();
+      o_lamb();
+      err = o_lamb_err(&num);
+    } else {
+      err = 0.;
+    }
+    iter++;
+    if (iter == O_lamb_max_iter) break;
+  }
+  O.lamb_iters = iter;
+  O.lamb_err = err;
+  if (NP_ > 0) {
+    o_move_sub();
+    o_move_a();
+    o_internal();
+    orc_bc_all();
+    exch_vel(u_, v_, w_);
+  }
   orc_store_u();
   O.dt0 = O.dt;
   orc_find_dt();
+  if (NP_ > 0) {
+    o_move_b();
+    orc_build_cages();
+    orc_jacobi();
+  }
   return q;
 }
 
@@ -1105,6 +1230,9 @@ void orc_teardown(void)
     w0_, ws_, cw_, c0w_, dw_, d0w_, fz_, tbuf_};
   for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); i++) free(d[i]);
   free(flu_); free(flv_); free(flw_); free(phase_); free(shell_);
+  free(P_);
+  P_ = NULL;
+  NP_ = 0;
   p_ = NULL;
 }
 
@@ -1149,6 +1277,9 @@ double orc_scalar(const char *n)
   if (strcmp(n, "resid") == 0) return O.resid;
   if (strcmp(n, "true_resid") == 0) return O.true_resid;
   if (strcmp(n, "sp_rhs") == 0) return O.sp_rhs;
+  if (strcmp(n, "lamb_iters") == 0) return O.lamb_iters;
+  if (strcmp(n, "lamb_err") == 0) return O.lamb_err;
+  if (strcmp(n, "nparts") == 0) return NP_;
   return NAN;
 }
 

@@ -26,7 +26,8 @@ int *orc_ifield(const char *name);
 /* Interior extents of a grid (0 = Gcc, 1 = Gfx, 2 = Gfy, 3 = Gfz). */
 void orc_dims(int grid, int *ni, int *nj, int *nk);
 
-/* Scalars: dt dt0 ttime stepnum niter resid true_resid sp_rhs */
+/* Scalars: dt dt0 ttime stepnum niter resid true_resid sp_rhs lamb_iters
+ * lamb_err nparts */
 double orc_scalar(const char *name);
 void orc_set_scalar(const char *name, double v);
 
@@ -48,6 +49,23 @@ void orc_jacobi(void);
 void orc_build_flags(void);
 /* (-A) x over interior cells with the face flags (ghosts of x as given). */
 void orc_apply_A(const double *x, double *Ax);
+
+/* Particles (input/part.config).  orc_build_cages rebuilds phase,
+ * phase_shell and the face flags. */
+void orc_build_cages(void);
+int orc_nparts(void);
+/* 126 values: x y z u v w ox oy oz udot vdot wdot oxdot oydot ozdot
+ * Fx Fy Fz Lx Ly Lz iFx iFy iFz iLx iLy iLz axx..azz, then 15 each of
+ * pnm_re pnm_im phinm_re phinm_im chinm_re chinm_im. */
+int orc_part_state(int n, double *out);
+int orc_part_set_coeffs(int n, const double *c90);
+/* Lamb velocity (U[0..2]) and n = 0 pressure term (U[3]) of particle n at
+ * (x, y, z) relative to its centre. */
+void orc_lamb_eval(int n, double x, double y, double z, double *U);
+/* One Physalis update (quadrature, coefficients, forces) on the current
+ * fields; returns the Lamb error. */
+double orc_lamb_once(void);
+void orc_lebedev(double *theta, double *phi, double *w);
 
 /* Canonical reductions, exposed so tests can pin them. */
 double orc_canon_sum(const double *terms, int nx, int ny, int nz);

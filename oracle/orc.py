@@ -53,6 +53,16 @@ def load():
         L.orc_exchange.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.orc_apply_A.argtypes = [ctypes.POINTER(ctypes.c_double)] * 2
         L.orc_dims.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+        D = ctypes.POINTER(ctypes.c_double)
+        L.orc_nparts.restype = ctypes.c_int
+        L.orc_part_state.argtypes = [ctypes.c_int, D]
+        L.orc_part_state.restype = ctypes.c_int
+        L.orc_part_set_coeffs.argtypes = [ctypes.c_int, D]
+        L.orc_part_set_coeffs.restype = ctypes.c_int
+        L.orc_lamb_eval.argtypes = [ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, D]
+        L.orc_lamb_once.restype = ctypes.c_double
+        L.orc_lebedev.argtypes = [D, D, D]
         _lib = L
     return _lib
 
@@ -112,6 +122,52 @@ class Oracle:
 
     def teardown(self):
         self.lib.orc_teardown()
+
+    # -------------------------------- particles --------------------------
+    def nparts(self):
+        return self.lib.orc_nparts()
+
+    def part_state(self, n):
+        """Dict of one particle's state (names as PART_FIELDS)."""
+        out = np.zeros(126)
+        self.lib.orc_part_state(n, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)))
+        return unpack_state(out)
+
+    def set_coeffs(self, n, coeffs):
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+        assert c.shape == (90,)
+        self.lib.orc_part_set_coeffs(n, c.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)))
+
+    def lamb_eval(self, n, x, y, z):
+        U = np.zeros(4)
+        self.lib.orc_lamb_eval(n, x, y, z,
+                               U.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return U
+
+
+PART_SCALARS = ("x y z u v w ox oy oz udot vdot wdot oxdot oydot ozdot Fx Fy "
+                "Fz Lx Ly Lz iFx iFy iFz iLx iLy iLz axx axy axz ayx ayy ayz "
+                "azx azy azz").split()
+PART_COEFFS = ("pnm_re", "pnm_im", "phinm_re", "phinm_im", "chinm_re",
+               "chinm_im")
+
+
+def unpack_state(v):
+    d = {k: float(v[i]) for i, k in enumerate(PART_SCALARS)}
+    for i, k in enumerate(PART_COEFFS):
+        d[k] = np.array(v[36 + 15 * i:36 + 15 * (i + 1)])
+    return d
+
+
+def lebedev():
+    """Nodes (theta, phi) and weights of the 26-node rule."""
+    L = load()
+    t, p, w = np.zeros(26), np.zeros(26), np.zeros(26)
+    P = ctypes.POINTER(ctypes.c_double)
+    L.orc_lebedev(t.ctypes.data_as(P), p.ctypes.data_as(P), w.ctypes.data_as(P))
+    return t, p, w
 
 
 def canon_sum(terms):
