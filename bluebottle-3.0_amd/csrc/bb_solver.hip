@@ -598,6 +598,56 @@ residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
   tile_partial(acc, gpart, tile);
 }
 
+/* Start from a given phi (the learned guess): r = b - (-A) phi and the tile
+ * partials of (r, z). */
+__global__ void __launch_bounds__(TILE_THREADS)
+cg_init_warm_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ rhs,
+  const uint8_t *__restrict__ fluid, const double *__restrict__ phi,
+  double *__restrict__ r, double *__restrict__ gpart)
+{
+  TILE_PROLOGUE
+  double acc = 0.0;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
+      r[c] = rr;
+      double z = rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
+      acc += rr * z;
+    }
+  }
+  tile_partial(acc, gpart, tile);
+}
+
+/* Tile partials of (b, psi) and (psi, (-A) psi) over fluid cells. */
+__global__ void __launch_bounds__(TILE_THREADS)
+energy_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
+  const double *__restrict__ psi, const double *__restrict__ rhs,
+  double *__restrict__ part_b, double *__restrict__ part_a)
+{
+  TILE_PROLOGUE
+  double ab = 0.0, aa = 0.0;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      if (solid(fluid, c)) continue;
+      double ps = psi[c];
+      ab += rhs[c] * ps;
+      aa += ps * neg_lap_at(D, K, psi, x, j, k);
+    }
+  }
+  tile_partial(ab, part_b, tile);
+  tile_partial(aa, part_a, tile);
+}
+
+__global__ void __launch_bounds__(TILE_THREADS)
+scale_kernel(DevDom D, double a, double *__restrict__ f)
+{
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) f[gidx(C, x, j, k)] *= a;
+}
+
 /* Tile partials of ||b - (-A) phi||^2 (diagnostic true residual). */
 __global__ void __launch_bounds__(TILE_THREADS)
 true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
@@ -648,6 +698,15 @@ mx_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
     }
   }
   tile_partial(acc, gpart, tile);
+}
+
+/* e = 0. */
+__global__ void __launch_bounds__(TILE_THREADS)
+mx_zero_kernel(DevDom D, float *__restrict__ e)
+{
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) e[gidx(C, x, j, k)] = 0.f;
 }
 
 /* phi += e (FP64), e = 0. */
@@ -727,6 +786,7 @@ static CGState *cg_host = NULL;
 static double *_p_alt = NULL;          /* second search-direction buffer */
 static double *_part_p = NULL;         /* tile partials of (p, Ap) */
 static double *_part_r = NULL;         /* tile partials of (r, z) */
+static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
 int bb_spmv_variant = 1;               /* 1: LDS tile kernel, 0: direct loads */
@@ -768,8 +828,12 @@ static void cg_buffers(void)
   if (part_n != nt) {
     if (_part_p) BB_CHECK(hipFree(_part_p));
     if (_part_r) BB_CHECK(hipFree(_part_r));
+    if (_part_w1) BB_CHECK(hipFree(_part_w1));
+    if (_part_w2) BB_CHECK(hipFree(_part_w2));
     BB_CHECK(hipMalloc((void **) &_part_p, nt * sizeof(double)));
     BB_CHECK(hipMalloc((void **) &_part_r, nt * sizeof(double)));
+    BB_CHECK(hipMalloc((void **) &_part_w1, nt * sizeof(double)));
+    BB_CHECK(hipMalloc((void **) &_part_w2, nt * sizeof(double)));
     part_n = nt;
   }
 }
@@ -779,6 +843,9 @@ void bb_solver_free(void)
   if (_p_alt) BB_CHECK(hipFree(_p_alt));
   if (_part_p) BB_CHECK(hipFree(_part_p));
   if (_part_r) BB_CHECK(hipFree(_part_r));
+  if (_part_w1) BB_CHECK(hipFree(_part_w1));
+  if (_part_w2) BB_CHECK(hipFree(_part_w2));
+  _part_w1 = _part_w2 = NULL;
   if (_cg_state) BB_CHECK(hipFree(_cg_state));
   if (cg_host) BB_CHECK(hipHostFree(cg_host));
   float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
@@ -820,6 +887,26 @@ double bb_true_residual(double sp_rhs)
   bb_reduce_rows(_part_r, n_tiles(ddom), _bb_scalars + 8);
   double rr = bb_fetch_scalar(8);
   return sp_rhs > 0 ? sqrt(rr) / sqrt(sp_rhs) : 0.;
+}
+
+/* The learned guess psi in _phi becomes alpha psi with
+ * alpha = (b, psi) / (psi, (-A) psi); returns alpha (phi is left alone when
+ * alpha is not positive and finite). */
+double bb_pp_energy_scale(void)
+{
+  cg_buffers();
+  const long nt = n_tiles(ddom);
+  hipLaunchKernelGGL(energy_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
+    _phi, _rhs_p, _part_w1, _part_w2);
+  bb_reduce_rows(_part_w1, nt, _bb_scalars + 9);
+  bb_reduce_rows(_part_w2, nt, _bb_scalars + 10);
+  bb_fetch_scalars(11);
+  double a = bb_host_scalars[9] / bb_host_scalars[10];
+  if (a > 0. && isfinite(a)) {
+    hipLaunchKernelGGL(scale_kernel, TILES_LAUNCH, ddom, a, _phi);
+    BB_LAUNCH_CHECK();
+  }
+  return a;
 }
 
 /* Enqueue iteration q; p buffers alternate by parity. */
@@ -864,7 +951,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 
 /* The plain FP64 Jacobi-PCG (src/cuda_solver.cu:574-761) on the RHS already
  * in _rhs_p with the tile partials of (b, b) in _part_p. */
-static void pcg_plain(double ts)
+static void pcg_plain(double ts, int warm)
 {
   Coef K = coef();
   InvMTab T = invm_table(K);
@@ -872,8 +959,12 @@ static void pcg_plain(double ts)
   const long nt = n_tiles(ddom);
   cg_buffers();
 
-  hipLaunchKernelGGL(cg_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
-    fluid_mask(), _r_q, _phi, _part_r);
+  if (warm)
+    hipLaunchKernelGGL(cg_init_warm_kernel, TILES_LAUNCH, ddom, K, T, _rhs_p,
+      fluid_mask(), _phi, _r_q, _part_r);
+  else
+    hipLaunchKernelGGL(cg_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
+      fluid_mask(), _r_q, _phi, _part_r);
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual, 0.);
   BB_LAUNCH_CHECK();
@@ -929,11 +1020,16 @@ static void pcg_plain(double ts)
 
 int bb_log_true_residual = 1;
 
+extern "C" int cuda_PP_warmstart_infer(void);
+extern "C" void cuda_PP_warmstart_train(void);
+
 extern "C" void cuda_PP_cg_noparts(void)
 {
   double ts = now_s();
   bb_pp_rhs();
-  pcg_plain(ts);
+  int warm = pp_warmstart == PP_WARM_LEARNED ? cuda_PP_warmstart_infer() : 0;
+  pcg_plain(ts, warm);
+  if (pp_warmstart == PP_WARM_LEARNED) cuda_PP_warmstart_train();
   pp_last.true_resid = (bb_log_true_residual && pp_last.sp_rhs > 0)
     ? bb_true_residual(pp_last.sp_rhs) : 0.;
 }
@@ -984,7 +1080,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
  * Krylov space survives the correction), or with bb_mixed_restart = 1
  * restarting from p = z (classic defect correction).  No counterpart in the
  * reference; it replaces src/cuda_solver.cu:574-761 when selected. */
-static void pcg_mixed(double ts)
+static void pcg_mixed(double ts, int warm)
 {
   Coef K = coef();
   InvMTab T = invm_table(K);
@@ -999,8 +1095,14 @@ static void pcg_mixed(double ts)
   cg_buffers();
   cg_buffers32();
 
-  hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
-    fluid_mask(), _phi, _e32, _r32, _part_r);
+  if (warm) {
+    hipLaunchKernelGGL(mx_zero_kernel, TILES_LAUNCH, ddom, _e32);
+    hipLaunchKernelGGL(mx_residual_kernel, TILES_LAUNCH, ddom, K, T, _phi,
+      _rhs_p, fluid_mask(), _r32, _part_r);
+  } else {
+    hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
+      fluid_mask(), _phi, _e32, _r32, _part_r);
+  }
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual,
     pp_inner_tol*pp_inner_tol);
@@ -1074,7 +1176,9 @@ extern "C" void cuda_PP_cg_mixed(void)
 {
   double ts = now_s();
   bb_pp_rhs();
-  pcg_mixed(ts);
+  int warm = pp_warmstart == PP_WARM_LEARNED ? cuda_PP_warmstart_infer() : 0;
+  pcg_mixed(ts, warm);
+  if (pp_warmstart == PP_WARM_LEARNED) cuda_PP_warmstart_train();
   pp_last.true_resid = (bb_log_true_residual && pp_last.sp_rhs > 0)
     ? bb_true_residual(pp_last.sp_rhs) : 0.;
 }

@@ -1,0 +1,795 @@
+/*
+ * Learned warm start for the pressure solve: a 3-level 3-D convolutional
+ * U-Net (16/32/64 channels, 3x3x3 kernels, zero padding, ReLU, 2x2x2
+ * average pooling, nearest-neighbour upsampling, skip connections by
+ * channel concatenation, a final 1x1x1 convolution to one channel).
+ *
+ * Weights and activations are BF16 on the MFMA units with FP32
+ * accumulation (v_mfma_f32_16x16x16_bf16); master weights, gradients and
+ * the Adam moments are FP32.  Tensors are channels-last: element (v, c) of
+ * a tensor with row length ld sits at v*ld + c, v = x + nx*(y + ny*z), so a
+ * lane's four consecutive channels are one 8-byte load.
+ *
+ * A 3x3x3 convolution is an implicit GEMM: M = voxels, N = output channels,
+ * K = 27 taps x input channels.  One wavefront owns a 16-voxel x 16-channel
+ * output tile and runs one MFMA per (tap, 16-channel input chunk).  The
+ * backward data pass is the same kernel on the output gradient with the
+ * weights flipped and transposed; the weight gradient is a GEMM over voxels
+ * (M = output channel, N = input channel, K = voxels) split into voxel
+ * chunks whose partial sums are added in a fixed order (deterministic).
+ *
+ * No counterpart in the reference (it solves every Poisson problem from
+ * phi = 0, src/solver_kernel.cu:258-283).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "bb_common.h"
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+/* BF16 <-> FP32, round to nearest even. */
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+  union { float f; uint32_t u; } x;
+  x.f = f;
+  uint32_t u = x.u;
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t) (u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t) (u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t b)
+{
+  union { float f; uint32_t u; } x;
+  x.u = ((uint32_t) b) << 16;
+  return x.f;
+}
+
+/* A tensor view: base, row length (channels per voxel in the buffer),
+ * channel offset, volume dims. */
+struct TV {
+  void *p;
+  int ld, c0;
+  int nx, ny, nz;
+};
+
+__device__ inline bool vox_xyz(const TV &T, long v, int *x, int *y, int *z)
+{
+  long n = (long) T.nx * T.ny * T.nz;
+  if (v >= n) return false;
+  *x = (int) (v % T.nx);
+  long r = v / T.nx;
+  *y = (int) (r % T.ny);
+  *z = (int) (r / T.ny);
+  return true;
+}
+
+/* Four consecutive channels (c .. c+3) of voxel (x, y, z) as BF16, zero
+ * outside the volume or past cmax. */
+template <typename IN>
+__device__ inline bf16x4 load4(const TV &T, int x, int y, int z, int c,
+  int cmax)
+{
+  bf16x4 r = {0, 0, 0, 0};
+  if (x < 0 || y < 0 || z < 0 || x >= T.nx || y >= T.ny || z >= T.nz)
+    return r;
+  long base = ((long) x + (long) T.nx * (y + (long) T.ny * z)) * T.ld + T.c0;
+  const IN *p = (const IN *) T.p + base;
+  for (int e = 0; e < 4; e++) {
+    if (c + e < cmax) {
+      if (sizeof(IN) == 2) r[e] = (short) ((const uint16_t *) p)[c + e];
+      else r[e] = (short) f2bf(((const float *) p)[c + e]);
+    }
+  }
+  return r;
+}
+
+/* ------------------------------- convolution ----------------------------- */
+/* Y[v, co] = act(sum_tap sum_ci X[v + off(tap), ci] W[tap][co][ci] + b[co]).
+ * W is BF16 [taps][Cout_pad][Cin_pad]; Cin_pad, Cout_pad multiples of 16.
+ * Output: FP32 (yf) and/or BF16 (yb) views; relu applies max(0, .).
+ * Grid: (ceil(nvox / 64), Cout_pad / 16); 256 threads (4 wavefronts). */
+template <typename IN>
+__global__ void __launch_bounds__(256)
+conv_fwd_kernel(TV X, int cin, int cin_pad, const uint16_t *__restrict__ W,
+  int taps, const float *__restrict__ bias, int cout, int cout_pad,
+  TV Yf, TV Yb, int relu)
+{
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  long v0 = ((long) blockIdx.x * 4 + wv) * 16;
+  int co0 = blockIdx.y * 16;
+  /* A-fragment voxel of this lane and its coordinates */
+  long va = v0 + (lane & 15);
+  int ax, ay, az;
+  bool aok = vox_xyz(X, va, &ax, &ay, &az);
+  int kq = 4 * (lane >> 4);
+  int n = lane & 15;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < taps; tap++) {
+    int dx = 0, dy = 0, dz = 0;
+    if (taps == 27) { dx = tap % 3 - 1; dy = (tap / 3) % 3 - 1; dz = tap / 9 - 1; }
+    const uint16_t *Wt = W + ((long) tap * cout_pad + co0 + n) * cin_pad;
+    for (int ch = 0; ch < cin_pad; ch += 16) {
+      bf16x4 a = {0, 0, 0, 0};
+      if (aok) a = load4<IN>(X, ax + dx, ay + dy, az + dz, ch + kq, cin);
+      bf16x4 b;
+      for (int e = 0; e < 4; e++) b[e] = (short) Wt[ch + kq + e];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+  /* D[m][n]: lane holds m = 4*(lane/16) + e, n = lane % 16 */
+  int co = co0 + n;
+  if (co >= cout) return;
+  float bb = bias ? bias[co] : 0.f;
+  for (int e = 0; e < 4; e++) {
+    long v = v0 + 4 * (lane >> 4) + e;
+    int x, y, z;
+    if (!vox_xyz(Yf.p ? Yf : Yb, v, &x, &y, &z)) continue;
+    float val = acc[e] + bb;
+    if (relu) val = val > 0.f ? val : 0.f;
+    if (Yf.p) ((float *) Yf.p)[v * Yf.ld + Yf.c0 + co] = val;
+    if (Yb.p) ((uint16_t *) Yb.p)[v * Yb.ld + Yb.c0 + co] = f2bf(val);
+  }
+}
+
+/* Weight gradient partials: P[chunk][tap][co][ci] = sum over the chunk's
+ * voxels v of G[v, co] X[v + off(tap), ci].  Grid: (nchunks, taps *
+ * cout_pad/16 * cin_pad/16 / 4); 4 wavefronts, each one (tap, co-tile,
+ * ci-tile).  G is FP32, X is BF16. */
+__global__ void __launch_bounds__(256)
+conv_wgrad_kernel(TV G, int cout, TV X, int cin, int taps, int cout_pad,
+  int cin_pad, long chunk, float *__restrict__ P)
+{
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int nco = cout_pad / 16, nci = cin_pad / 16;
+  int job = blockIdx.y * 4 + wv;
+  if (job >= taps * nco * nci) return;
+  int tap = job / (nco * nci), rem = job % (nco * nci);
+  int co0 = (rem / nci) * 16, ci0 = (rem % nci) * 16;
+  int dx = 0, dy = 0, dz = 0;
+  if (taps == 27) { dx = tap % 3 - 1; dy = (tap / 3) % 3 - 1; dz = tap / 9 - 1; }
+  long nv = (long) G.nx * G.ny * G.nz;
+  long vb = (long) blockIdx.x * chunk, ve = vb + chunk < nv ? vb + chunk : nv;
+  int m = lane & 15, kq = 4 * (lane >> 4);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (long v0 = vb; v0 < ve; v0 += 16) {
+    bf16x4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
+    for (int e = 0; e < 4; e++) {
+      long v = v0 + kq + e;
+      int x, y, z;
+      if (v >= ve || !vox_xyz(G, v, &x, &y, &z)) continue;
+      if (co0 + m < cout)
+        a[e] = (short) f2bf(((const float *) G.p)[v * G.ld + G.c0 + co0 + m]);
+      int xs = x + dx, ys = y + dy, zs = z + dz;
+      if (ci0 + m < cin && xs >= 0 && ys >= 0 && zs >= 0 && xs < X.nx &&
+          ys < X.ny && zs < X.nz) {
+        long vs = (long) xs + (long) X.nx * (ys + (long) X.ny * zs);
+        b[e] = (short) ((const uint16_t *) X.p)[vs * X.ld + X.c0 + ci0 + m];
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  /* D[m = co][n = ci] */
+  int nn = lane & 15;
+  float *Pc = P + (long) blockIdx.x * taps * cout_pad * cin_pad;
+  for (int e = 0; e < 4; e++) {
+    int co = co0 + 4 * (lane >> 4) + e;
+    Pc[((long) tap * cout_pad + co) * cin_pad + ci0 + nn] = acc[e];
+  }
+}
+
+/* dW = sum of the chunk partials in chunk order; db[co] likewise from the
+ * bias partials. */
+__global__ void sum_chunks_kernel(const float *__restrict__ P, long n,
+  int nchunks, float *__restrict__ out)
+{
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; c++) s += P[(long) c * n + i];
+  out[i] = s;
+}
+
+/* Bias-gradient partials: Pb[chunk][co] = sum over the chunk of G[v, co]. */
+__global__ void bias_grad_kernel(TV G, int cout, long chunk, float *Pb)
+{
+  int co = threadIdx.x;
+  if (co >= cout) return;
+  long nv = (long) G.nx * G.ny * G.nz;
+  long vb = (long) blockIdx.x * chunk, ve = vb + chunk < nv ? vb + chunk : nv;
+  float s = 0.f;
+  for (long v = vb; v < ve; v++) s += ((const float *) G.p)[v * G.ld + G.c0 + co];
+  Pb[(long) blockIdx.x * blockDim.x + co] = s;
+}
+
+/* Backward-data weights: Wt[tap][ci][co] = W[26 - tap][co][ci] (flipped,
+ * transposed), BF16. */
+__global__ void flip_weights_kernel(const uint16_t *__restrict__ W, int taps,
+  int cout_pad, int cin_pad, uint16_t *__restrict__ Wt)
+{
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long) taps * cout_pad * cin_pad;
+  if (i >= n) return;
+  int tap = (int) (i / ((long) cin_pad * cout_pad));
+  int rem = (int) (i % ((long) cin_pad * cout_pad));
+  int ci = rem / cout_pad, co = rem % cout_pad;
+  int ft = taps - 1 - tap;
+  Wt[i] = W[((long) ft * cout_pad + co) * cin_pad + ci];
+}
+
+/* ------------------------------ elementwise ------------------------------ */
+/* G[v, c] *= (Y[v, c] > 0) over c < C (ReLU backward). */
+__global__ void relu_bwd_kernel(TV G, TV Y, int C)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) G.nx * G.ny * G.nz;
+  if (t >= nv * C) return;
+  long v = t / C;
+  int c = (int) (t % C);
+  uint16_t y = ((const uint16_t *) Y.p)[v * Y.ld + Y.c0 + c];
+  if (!(bf2f(y) > 0.f)) ((float *) G.p)[v * G.ld + G.c0 + c] = 0.f;
+}
+
+/* 2x2x2 average pool, BF16 in and out (coarse dims from C). */
+__global__ void pool_fwd_kernel(TV F, TV Cr, int C)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) Cr.nx * Cr.ny * Cr.nz;
+  if (t >= nv * C) return;
+  long v = t / C;
+  int c = (int) (t % C), x, y, z;
+  vox_xyz(Cr, v, &x, &y, &z);
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    int xf = 2 * x + (q & 1), yf = 2 * y + ((q >> 1) & 1), zf = 2 * z + (q >> 2);
+    long vf = (long) xf + (long) F.nx * (yf + (long) F.ny * zf);
+    s += bf2f(((const uint16_t *) F.p)[vf * F.ld + F.c0 + c]);
+  }
+  ((uint16_t *) Cr.p)[v * Cr.ld + Cr.c0 + c] = f2bf(0.125f * s);
+}
+
+/* Pool backward: G_fine[v, c] += 0.125 G_coarse[parent(v), c] (FP32). */
+__global__ void pool_bwd_kernel(TV Gf, TV Gc, int C)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) Gf.nx * Gf.ny * Gf.nz;
+  if (t >= nv * C) return;
+  long v = t / C;
+  int c = (int) (t % C), x, y, z;
+  vox_xyz(Gf, v, &x, &y, &z);
+  long vc = (long) (x / 2) + (long) Gc.nx * ((y / 2) + (long) Gc.ny * (z / 2));
+  ((float *) Gf.p)[v * Gf.ld + Gf.c0 + c] +=
+    0.125f * ((const float *) Gc.p)[vc * Gc.ld + Gc.c0 + c];
+}
+
+/* Nearest upsample x2, BF16. */
+__global__ void up_fwd_kernel(TV Cr, TV F, int C)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) F.nx * F.ny * F.nz;
+  if (t >= nv * C) return;
+  long v = t / C;
+  int c = (int) (t % C), x, y, z;
+  vox_xyz(F, v, &x, &y, &z);
+  long vc = (long) (x / 2) + (long) Cr.nx * ((y / 2) + (long) Cr.ny * (z / 2));
+  ((uint16_t *) F.p)[v * F.ld + F.c0 + c] =
+    ((const uint16_t *) Cr.p)[vc * Cr.ld + Cr.c0 + c];
+}
+
+/* Upsample backward: G_coarse[v, c] = sum of the 8 children (FP32). */
+__global__ void up_bwd_kernel(TV Gf, TV Gc, int C)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) Gc.nx * Gc.ny * Gc.nz;
+  if (t >= nv * C) return;
+  long v = t / C;
+  int c = (int) (t % C), x, y, z;
+  vox_xyz(Gc, v, &x, &y, &z);
+  float s = 0.f;
+  for (int q = 0; q < 8; q++) {
+    int xf = 2 * x + (q & 1), yf = 2 * y + ((q >> 1) & 1), zf = 2 * z + (q >> 2);
+    long vf = (long) xf + (long) Gf.nx * (yf + (long) Gf.ny * zf);
+    s += ((const float *) Gf.p)[vf * Gf.ld + Gf.c0 + c];
+  }
+  ((float *) Gc.p)[v * Gc.ld + Gc.c0 + c] = s;
+}
+
+__global__ void fill_f32_kernel(float *a, long n, float v)
+{
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+/* Loss 0.5 * mean (y - t)^2 over the volume: per-block partials, and the
+ * output gradient dy = (y - t) / n. */
+__global__ void __launch_bounds__(256)
+mse_kernel(const float *__restrict__ y, const float *__restrict__ t, long n,
+  float *__restrict__ dy, float *__restrict__ part)
+{
+  __shared__ float s[256];
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  float d = 0.f;
+  if (i < n) {
+    d = y[i] - t[i];
+    dy[i] = d / (float) n;
+  }
+  s[threadIdx.x] = 0.5f * d * d;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+/* Adam on FP32 master weights with BF16 copies (biases: wb == NULL). */
+__global__ void adam_kernel(float *w, const float *__restrict__ g, float *m,
+  float *v, long n, float lr, float b1, float b2, float eps, float c1,
+  float c2, uint16_t *wb)
+{
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void to_bf16_kernel(const float *__restrict__ a, long n,
+  uint16_t *__restrict__ b)
+{
+  long i = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+/* ================================ network ================================ */
+#define NLAYER 11
+/* layer table: cin, cout, taps, level, relu */
+static const int LC[NLAYER][5] = {
+  { 2, 16, 27, 0, 1},   /* 0 enc0a */
+  {16, 16, 27, 0, 1},   /* 1 enc0b -> cat0[32..47] */
+  {16, 32, 27, 1, 1},   /* 2 enc1a */
+  {32, 32, 27, 1, 1},   /* 3 enc1b -> cat1[64..95] */
+  {32, 64, 27, 2, 1},   /* 4 bot_a */
+  {64, 64, 27, 2, 1},   /* 5 bot_b */
+  {96, 32, 27, 1, 1},   /* 6 dec1a (cat1) */
+  {32, 32, 27, 1, 1},   /* 7 dec1b */
+  {48, 16, 27, 0, 1},   /* 8 dec0a (cat0) */
+  {16, 16, 27, 0, 1},   /* 9 dec0b */
+  {16,  1,  1, 0, 0},   /* 10 out 1x1 */
+};
+
+static inline int pad16(int c) { return (c + 15) / 16 * 16; }
+
+struct Layer {
+  int cin, cout, taps, level, relu, cinp, coutp;
+  long nw;                      /* taps * coutp * cinp */
+  float *w, *b, *gw, *gb, *mw, *vw, *mb, *vb;
+  uint16_t *wb, *wt;            /* BF16 weights, flipped/transposed */
+};
+
+struct Acts {
+  int n[3][3];                  /* dims per level */
+  long nv[3];
+  /* BF16 activations */
+  uint16_t *in0, *e0a, *cat0, *p0, *e1a, *cat1, *p1, *b0, *b1, *d1a, *d1,
+           *d0a, *d0;
+  float *y;                     /* output */
+  /* FP32 gradients */
+  float *g_e0a, *g_cat0, *g_p0, *g_e1a, *g_cat1, *g_p1, *g_b0, *g_b1, *g_d1a,
+        *g_d1, *g_d0a, *g_d0, *g_y, *g_in0;
+  float *part, *loss_part;
+  long part_n, loss_n;
+};
+
+/* Two activation sets: the training crop and the full domain. */
+struct UNet {
+  Layer L[NLAYER];
+  Acts acts[2];
+  int have[2];
+  int next;
+  Acts *A;
+  long step;
+};
+
+static UNet *net = NULL;
+
+static unsigned long long urng = 0x9e3779b97f4a7c15ULL;
+static double urand(void)
+{
+  urng ^= urng >> 12; urng ^= urng << 25; urng ^= urng >> 27;
+  return (double) ((urng * 2685821657736338717ULL) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void *dzalloc(size_t bytes)
+{
+  void *p = NULL;
+  BB_CHECK(hipMalloc(&p, bytes > 0 ? bytes : 1));
+  BB_CHECK(hipMemset(p, 0, bytes > 0 ? bytes : 1));
+  return p;
+}
+
+static inline unsigned blocks(long n, int t) { return (unsigned) ((n + t - 1) / t); }
+
+static void upload_weights(Layer &Y)
+{
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks(Y.nw, 256)), dim3(256), 0,
+    bb_stream, Y.w, Y.nw, Y.wb);
+}
+
+/* Create the network with He-initialised weights from `seed`. */
+extern "C" int bb_unet_create(unsigned long long seed)
+{
+  if (net) return 0;
+  net = (UNet *) calloc(1, sizeof(UNet));
+  urng = seed ? seed : 0x9e3779b97f4a7c15ULL;
+  for (int l = 0; l < NLAYER; l++) {
+    Layer &Y = net->L[l];
+    Y.cin = LC[l][0]; Y.cout = LC[l][1]; Y.taps = LC[l][2];
+    Y.level = LC[l][3]; Y.relu = LC[l][4];
+    Y.cinp = pad16(Y.cin); Y.coutp = pad16(Y.cout);
+    Y.nw = (long) Y.taps * Y.coutp * Y.cinp;
+    float *hw = (float *) calloc(Y.nw, sizeof(float));
+    double sd = sqrt(2.0 / (Y.taps * Y.cin));
+    for (int t = 0; t < Y.taps; t++)
+      for (int co = 0; co < Y.cout; co++)
+        for (int ci = 0; ci < Y.cin; ci++) {
+          /* Box-Muller */
+          double u1 = urand() + 1e-300, u2 = urand();
+          double g = sqrt(-2.0 * log(u1)) * cos(2.0 * PI * u2);
+          hw[((long) t * Y.coutp + co) * Y.cinp + ci] = (float) (g * sd);
+        }
+    Y.w = (float *) dzalloc(Y.nw * sizeof(float));
+    BB_CHECK(hipMemcpy(Y.w, hw, Y.nw * sizeof(float), hipMemcpyHostToDevice));
+    free(hw);
+    Y.gw = (float *) dzalloc(Y.nw * sizeof(float));
+    Y.mw = (float *) dzalloc(Y.nw * sizeof(float));
+    Y.vw = (float *) dzalloc(Y.nw * sizeof(float));
+    Y.b = (float *) dzalloc(Y.coutp * sizeof(float));
+    Y.gb = (float *) dzalloc(Y.coutp * sizeof(float));
+    Y.mb = (float *) dzalloc(Y.coutp * sizeof(float));
+    Y.vb = (float *) dzalloc(Y.coutp * sizeof(float));
+    Y.wb = (uint16_t *) dzalloc(Y.nw * sizeof(uint16_t));
+    Y.wt = (uint16_t *) dzalloc(Y.nw * sizeof(uint16_t));
+    upload_weights(Y);
+  }
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  return 0;
+}
+
+static void free_acts(Acts &A)
+{
+  void *p[] = {A.in0, A.e0a, A.cat0, A.p0, A.e1a, A.cat1, A.p1, A.b0, A.b1,
+    A.d1a, A.d1, A.d0a, A.d0, A.y, A.g_e0a, A.g_cat0, A.g_p0, A.g_e1a,
+    A.g_cat1, A.g_p1, A.g_b0, A.g_b1, A.g_d1a, A.g_d1, A.g_d0a, A.g_d0, A.g_y,
+    A.g_in0, A.part, A.loss_part};
+  for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
+    if (p[i]) BB_CHECK(hipFree(p[i]));
+  memset(&A, 0, sizeof(A));
+}
+
+#define CHUNK 4096L
+
+/* Activation buffers for a volume nx x ny x nz (each divisible by 4). */
+extern "C" int bb_unet_volume(int nx, int ny, int nz)
+{
+  if (!net) return -1;
+  if (nx % 4 || ny % 4 || nz % 4 || nx < 4 || ny < 4 || nz < 4) {
+    fprintf(stderr, "bb_unet_volume: dims must be positive multiples of 4 "
+      "(%d %d %d)\n", nx, ny, nz);
+    return -2;
+  }
+  for (int s = 0; s < 2; s++)
+    if (net->have[s] && net->acts[s].n[0][0] == nx &&
+        net->acts[s].n[0][1] == ny && net->acts[s].n[0][2] == nz) {
+      net->A = &net->acts[s];
+      return 0;
+    }
+  int slot = net->next;
+  net->next = 1 - net->next;
+  if (net->have[slot]) free_acts(net->acts[slot]);
+  net->A = &net->acts[slot];
+  Acts &A = *net->A;
+  for (int l = 0; l < 3; l++) {
+    A.n[l][0] = nx >> l; A.n[l][1] = ny >> l; A.n[l][2] = nz >> l;
+    A.nv[l] = (long) A.n[l][0] * A.n[l][1] * A.n[l][2];
+  }
+  long n0 = A.nv[0], n1 = A.nv[1], n2 = A.nv[2];
+  A.in0 = (uint16_t *) dzalloc(n0 * 16 * 2);
+  A.e0a = (uint16_t *) dzalloc(n0 * 16 * 2);
+  A.cat0 = (uint16_t *) dzalloc(n0 * 48 * 2);
+  A.p0 = (uint16_t *) dzalloc(n1 * 16 * 2);
+  A.e1a = (uint16_t *) dzalloc(n1 * 32 * 2);
+  A.cat1 = (uint16_t *) dzalloc(n1 * 96 * 2);
+  A.p1 = (uint16_t *) dzalloc(n2 * 32 * 2);
+  A.b0 = (uint16_t *) dzalloc(n2 * 64 * 2);
+  A.b1 = (uint16_t *) dzalloc(n2 * 64 * 2);
+  A.d1a = (uint16_t *) dzalloc(n1 * 32 * 2);
+  A.d1 = (uint16_t *) dzalloc(n1 * 32 * 2);
+  A.d0a = (uint16_t *) dzalloc(n0 * 16 * 2);
+  A.d0 = (uint16_t *) dzalloc(n0 * 16 * 2);
+  A.y = (float *) dzalloc(n0 * 4);
+  A.g_e0a = (float *) dzalloc(n0 * 16 * 4);
+  A.g_cat0 = (float *) dzalloc(n0 * 48 * 4);
+  A.g_p0 = (float *) dzalloc(n1 * 16 * 4);
+  A.g_e1a = (float *) dzalloc(n1 * 32 * 4);
+  A.g_cat1 = (float *) dzalloc(n1 * 96 * 4);
+  A.g_p1 = (float *) dzalloc(n2 * 32 * 4);
+  A.g_b0 = (float *) dzalloc(n2 * 64 * 4);
+  A.g_b1 = (float *) dzalloc(n2 * 64 * 4);
+  A.g_d1a = (float *) dzalloc(n1 * 32 * 4);
+  A.g_d1 = (float *) dzalloc(n1 * 32 * 4);
+  A.g_d0a = (float *) dzalloc(n0 * 16 * 4);
+  A.g_d0 = (float *) dzalloc(n0 * 16 * 4);
+  A.g_y = (float *) dzalloc(n0 * 4);
+  A.g_in0 = (float *) dzalloc(n0 * 16 * 4);
+  long nchunks = (n0 + CHUNK - 1) / CHUNK;
+  A.part_n = nchunks * 27L * 64 * 96;
+  A.part = (float *) dzalloc(A.part_n * 4);
+  A.loss_n = blocks(n0, 256);
+  A.loss_part = (float *) dzalloc(A.loss_n * 4);
+  net->have[slot] = 1;
+  return 0;
+}
+
+static TV tv(void *p, int ld, int c0, const Acts &A, int level)
+{
+  TV T;
+  T.p = p; T.ld = ld; T.c0 = c0;
+  T.nx = A.n[level][0]; T.ny = A.n[level][1]; T.nz = A.n[level][2];
+  return T;
+}
+
+static TV none(void)
+{
+  TV T;
+  memset(&T, 0, sizeof(T));
+  return T;
+}
+
+static void conv(int l, TV X, bool xf32, TV Yf, TV Yb)
+{
+  Layer &Y = net->L[l];
+  long nv = (long) X.nx * X.ny * X.nz;
+  dim3 grid(blocks(nv, 64), Y.coutp / 16);
+  if (xf32)
+    hipLaunchKernelGGL(conv_fwd_kernel<float>, grid, dim3(256), 0, bb_stream,
+      X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout, Y.coutp, Yf, Yb, Y.relu);
+  else
+    hipLaunchKernelGGL(conv_fwd_kernel<uint16_t>, grid, dim3(256), 0,
+      bb_stream, X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout, Y.coutp, Yf, Yb,
+      Y.relu);
+  BB_LAUNCH_CHECK();
+}
+
+template <typename K, typename... Args>
+static void ew(K kern, long n, Args... args)
+{
+  hipLaunchKernelGGL(kern, dim3(blocks(n, 256)), dim3(256), 0, bb_stream,
+    args...);
+}
+
+/* Forward pass on the BF16 2-channel input already in A.in0 (16-channel
+ * rows, channels 0 and 1); output FP32 in A.y. */
+static void forward(void)
+{
+  Acts &A = *net->A;
+  conv(0, tv(A.in0, 16, 0, A, 0), false, none(), tv(A.e0a, 16, 0, A, 0));
+  conv(1, tv(A.e0a, 16, 0, A, 0), false, none(), tv(A.cat0, 48, 32, A, 0));
+  ew(pool_fwd_kernel, A.nv[1] * 16, tv(A.cat0, 48, 32, A, 0),
+     tv(A.p0, 16, 0, A, 1), 16);
+  conv(2, tv(A.p0, 16, 0, A, 1), false, none(), tv(A.e1a, 32, 0, A, 1));
+  conv(3, tv(A.e1a, 32, 0, A, 1), false, none(), tv(A.cat1, 96, 64, A, 1));
+  ew(pool_fwd_kernel, A.nv[2] * 32, tv(A.cat1, 96, 64, A, 1),
+     tv(A.p1, 32, 0, A, 2), 32);
+  conv(4, tv(A.p1, 32, 0, A, 2), false, none(), tv(A.b0, 64, 0, A, 2));
+  conv(5, tv(A.b0, 64, 0, A, 2), false, none(), tv(A.b1, 64, 0, A, 2));
+  ew(up_fwd_kernel, A.nv[1] * 64, tv(A.b1, 64, 0, A, 2),
+     tv(A.cat1, 96, 0, A, 1), 64);
+  conv(6, tv(A.cat1, 96, 0, A, 1), false, none(), tv(A.d1a, 32, 0, A, 1));
+  conv(7, tv(A.d1a, 32, 0, A, 1), false, none(), tv(A.d1, 32, 0, A, 1));
+  ew(up_fwd_kernel, A.nv[0] * 32, tv(A.d1, 32, 0, A, 1),
+     tv(A.cat0, 48, 0, A, 0), 32);
+  conv(8, tv(A.cat0, 48, 0, A, 0), false, none(), tv(A.d0a, 16, 0, A, 0));
+  conv(9, tv(A.d0a, 16, 0, A, 0), false, none(), tv(A.d0, 16, 0, A, 0));
+  conv(10, tv(A.d0, 16, 0, A, 0), false, tv(A.y, 1, 0, A, 0), none());
+}
+
+/* Backward of layer l: G (FP32, the gradient of the layer's output after
+ * its ReLU mask has been applied), X (its BF16 input); accumulates dW, db
+ * and writes the input gradient into GX (FP32) when GX.p != NULL. */
+static void layer_bwd(int l, TV G, TV X, TV GX)
+{
+  Layer &Y = net->L[l];
+  Acts &A = *net->A;
+  long nv = (long) G.nx * G.ny * G.nz;
+  long nchunks = (nv + CHUNK - 1) / CHUNK;
+  int jobs = Y.taps * (Y.coutp / 16) * (Y.cinp / 16);
+  hipLaunchKernelGGL(conv_wgrad_kernel, dim3((unsigned) nchunks,
+    (unsigned) ((jobs + 3) / 4)), dim3(256), 0, bb_stream, G, Y.cout, X,
+    Y.cin, Y.taps, Y.coutp, Y.cinp, CHUNK, A.part);
+  ew(sum_chunks_kernel, Y.nw, A.part, Y.nw, (int) nchunks, Y.gw);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned) nchunks),
+    dim3(Y.coutp), 0, bb_stream, G, Y.cout, CHUNK, A.part);
+  ew(sum_chunks_kernel, (long) Y.coutp, A.part, (long) Y.coutp,
+     (int) nchunks, Y.gb);
+  if (GX.p) {
+    ew(flip_weights_kernel, Y.nw, Y.wb, Y.taps, Y.coutp, Y.cinp, Y.wt);
+    dim3 grid(blocks(nv, 64), Y.cinp / 16);
+    hipLaunchKernelGGL(conv_fwd_kernel<float>, grid, dim3(256), 0, bb_stream,
+      G, Y.cout, Y.coutp, Y.wt, Y.taps, (const float *) NULL, Y.cin, Y.cinp,
+      GX, none(), 0);
+  }
+  BB_LAUNCH_CHECK();
+}
+
+static void relu_bwd(TV G, TV Y, int C)
+{
+  ew(relu_bwd_kernel, (long) G.nx * G.ny * G.nz * C, G, Y, C);
+}
+
+/* Backward pass from the output gradient in A.g_y. */
+static void backward(void)
+{
+  Acts &A = *net->A;
+  /* out (1x1, no ReLU): d0 <- */
+  layer_bwd(10, tv(A.g_y, 1, 0, A, 0), tv(A.d0, 16, 0, A, 0),
+            tv(A.g_d0, 16, 0, A, 0));
+  relu_bwd(tv(A.g_d0, 16, 0, A, 0), tv(A.d0, 16, 0, A, 0), 16);
+  layer_bwd(9, tv(A.g_d0, 16, 0, A, 0), tv(A.d0a, 16, 0, A, 0),
+            tv(A.g_d0a, 16, 0, A, 0));
+  relu_bwd(tv(A.g_d0a, 16, 0, A, 0), tv(A.d0a, 16, 0, A, 0), 16);
+  layer_bwd(8, tv(A.g_d0a, 16, 0, A, 0), tv(A.cat0, 48, 0, A, 0),
+            tv(A.g_cat0, 48, 0, A, 0));
+  /* cat0 = [up(d1) 0..31 | e0 32..47] */
+  ew(up_bwd_kernel, A.nv[1] * 32, tv(A.g_cat0, 48, 0, A, 0),
+     tv(A.g_d1, 32, 0, A, 1), 32);
+  relu_bwd(tv(A.g_d1, 32, 0, A, 1), tv(A.d1, 32, 0, A, 1), 32);
+  layer_bwd(7, tv(A.g_d1, 32, 0, A, 1), tv(A.d1a, 32, 0, A, 1),
+            tv(A.g_d1a, 32, 0, A, 1));
+  relu_bwd(tv(A.g_d1a, 32, 0, A, 1), tv(A.d1a, 32, 0, A, 1), 32);
+  layer_bwd(6, tv(A.g_d1a, 32, 0, A, 1), tv(A.cat1, 96, 0, A, 1),
+            tv(A.g_cat1, 96, 0, A, 1));
+  /* cat1 = [up(b1) 0..63 | e1 64..95] */
+  ew(up_bwd_kernel, A.nv[2] * 64, tv(A.g_cat1, 96, 0, A, 1),
+     tv(A.g_b1, 64, 0, A, 2), 64);
+  relu_bwd(tv(A.g_b1, 64, 0, A, 2), tv(A.b1, 64, 0, A, 2), 64);
+  layer_bwd(5, tv(A.g_b1, 64, 0, A, 2), tv(A.b0, 64, 0, A, 2),
+            tv(A.g_b0, 64, 0, A, 2));
+  relu_bwd(tv(A.g_b0, 64, 0, A, 2), tv(A.b0, 64, 0, A, 2), 64);
+  layer_bwd(4, tv(A.g_b0, 64, 0, A, 2), tv(A.p1, 32, 0, A, 2),
+            tv(A.g_p1, 32, 0, A, 2));
+  /* e1 gradient: skip path (already in g_cat1[64..95]) + pool path */
+  ew(pool_bwd_kernel, A.nv[1] * 32, tv(A.g_cat1, 96, 64, A, 1),
+     tv(A.g_p1, 32, 0, A, 2), 32);
+  relu_bwd(tv(A.g_cat1, 96, 64, A, 1), tv(A.cat1, 96, 64, A, 1), 32);
+  layer_bwd(3, tv(A.g_cat1, 96, 64, A, 1), tv(A.e1a, 32, 0, A, 1),
+            tv(A.g_e1a, 32, 0, A, 1));
+  relu_bwd(tv(A.g_e1a, 32, 0, A, 1), tv(A.e1a, 32, 0, A, 1), 32);
+  layer_bwd(2, tv(A.g_e1a, 32, 0, A, 1), tv(A.p0, 16, 0, A, 1),
+            tv(A.g_p0, 16, 0, A, 1));
+  ew(pool_bwd_kernel, A.nv[0] * 16, tv(A.g_cat0, 48, 32, A, 0),
+     tv(A.g_p0, 16, 0, A, 1), 16);
+  relu_bwd(tv(A.g_cat0, 48, 32, A, 0), tv(A.cat0, 48, 32, A, 0), 16);
+  layer_bwd(1, tv(A.g_cat0, 48, 32, A, 0), tv(A.e0a, 16, 0, A, 0),
+            tv(A.g_e0a, 16, 0, A, 0));
+  relu_bwd(tv(A.g_e0a, 16, 0, A, 0), tv(A.e0a, 16, 0, A, 0), 16);
+  layer_bwd(0, tv(A.g_e0a, 16, 0, A, 0), tv(A.in0, 16, 0, A, 0), none());
+}
+
+/* ---------------------------- C entry points ----------------------------- */
+/* Input: two FP32 channels-last fields (v*2 + c) on the device, copied to
+ * the network's BF16 input. */
+__global__ void pack_input_kernel(const float *__restrict__ in, long nv,
+  uint16_t *__restrict__ x)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  x[v * 16] = f2bf(in[2 * v]);
+  x[v * 16 + 1] = f2bf(in[2 * v + 1]);
+}
+
+/* Forward on a device input (FP32, 2 channels, channels-last) of the
+ * current volume; the FP32 output (one channel) goes to `out` (device). */
+extern "C" int bb_unet_forward(const float *in, float *out)
+{
+  if (!net || !net->A) return -1;
+  Acts &A = *net->A;
+  ew(pack_input_kernel, A.nv[0], in, A.nv[0], A.in0);
+  forward();
+  BB_CHECK(hipMemcpyAsync(out, A.y, A.nv[0] * sizeof(float),
+    hipMemcpyDeviceToDevice, bb_stream));
+  BB_LAUNCH_CHECK();
+  return 0;
+}
+
+/* Loss 0.5 mean (y - t)^2 of the last forward against `target` (device,
+ * FP32), its backward pass, and the gradients in the layers; returns the
+ * loss (host). */
+extern "C" double bb_unet_backward(const float *target)
+{
+  Acts &A = *net->A;
+  hipLaunchKernelGGL(mse_kernel, dim3((unsigned) A.loss_n), dim3(256), 0,
+    bb_stream, A.y, target, A.nv[0], A.g_y, A.loss_part);
+  backward();
+  float *h = (float *) malloc(A.loss_n * sizeof(float));
+  BB_CHECK(hipMemcpyAsync(h, A.loss_part, A.loss_n * sizeof(float),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  double s = 0.;
+  for (long i = 0; i < A.loss_n; i++) s += h[i];
+  free(h);
+  return s / (double) A.nv[0];
+}
+
+extern "C" void bb_unet_adam(float lr)
+{
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  net->step++;
+  float c1 = 1.f - powf(b1, (float) net->step);
+  float c2 = 1.f - powf(b2, (float) net->step);
+  for (int l = 0; l < NLAYER; l++) {
+    Layer &Y = net->L[l];
+    ew(adam_kernel, Y.nw, Y.w, Y.gw, Y.mw, Y.vw, Y.nw, lr, b1, b2, eps, c1,
+       c2, Y.wb);
+    ew(adam_kernel, (long) Y.coutp, Y.b, Y.gb, Y.mb, Y.vb, (long) Y.coutp,
+       lr, b1, b2, eps, c1, c2, (uint16_t *) NULL);
+  }
+  BB_LAUNCH_CHECK();
+}
+
+/* Parameter access for tests: layer count and sizes, FP32 master weights
+ * [taps][cout_pad][cin_pad], biases [cout_pad], gradients likewise. */
+extern "C" int bb_unet_layer_info(int l, int *info)
+{
+  if (!net || l < 0 || l >= NLAYER) return -1;
+  Layer &Y = net->L[l];
+  info[0] = Y.cin; info[1] = Y.cout; info[2] = Y.taps; info[3] = Y.level;
+  info[4] = Y.relu; info[5] = Y.cinp; info[6] = Y.coutp;
+  return NLAYER;
+}
+
+extern "C" int bb_unet_get(int l, int what, float *host)
+{
+  if (!net || l < 0 || l >= NLAYER) return -1;
+  Layer &Y = net->L[l];
+  const float *src = what == 0 ? Y.w : what == 1 ? Y.b : what == 2 ? Y.gw : Y.gb;
+  long n = (what == 0 || what == 2) ? Y.nw : Y.coutp;
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  BB_CHECK(hipMemcpy(host, src, n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int bb_unet_set(int l, int what, const float *host)
+{
+  if (!net || l < 0 || l >= NLAYER || (what != 0 && what != 1)) return -1;
+  Layer &Y = net->L[l];
+  float *dst = what == 0 ? Y.w : Y.b;
+  long n = what == 0 ? Y.nw : Y.coutp;
+  BB_CHECK(hipMemcpy(dst, host, n * sizeof(float), hipMemcpyHostToDevice));
+  if (what == 0) upload_weights(Y);
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  return 0;
+}
+
+extern "C" void bb_unet_destroy(void)
+{
+  if (!net) return;
+  for (int s = 0; s < 2; s++)
+    if (net->have[s]) free_acts(net->acts[s]);
+  for (int l = 0; l < NLAYER; l++) {
+    Layer &Y = net->L[l];
+    void *p[] = {Y.w, Y.b, Y.gw, Y.gb, Y.mw, Y.vw, Y.mb, Y.vb, Y.wb, Y.wt};
+    for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++)
+      if (p[i]) BB_CHECK(hipFree(p[i]));
+  }
+  free(net);
+  net = NULL;
+}

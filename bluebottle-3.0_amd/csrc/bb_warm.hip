@@ -1,0 +1,281 @@
+/*
+ * Learned initial guess for the pressure solve (pp_warmstart LEARNED).
+ *
+ * Each solve of -A phi = -b can start from phi0 = alpha psi, where
+ * psi = U-Net(b / |b|_rms, (p0 - mean p0) / |p0 - mean|_rms) is the
+ * network's predicted shape of the solution and
+ *   alpha = (b, psi) / (psi, (-A) psi)
+ * is the energy-optimal scale along psi, so the A-norm error of phi0 never
+ * exceeds that of the reference's phi0 = 0; alpha <= 0 (or a non-finite
+ * value) falls back to phi0 = 0.  Every solve still ends at the same test
+ * (r, M^-1 r) <= pp_residual^2 (b, b).
+ *
+ * The network is trained online: the first bb_warm_train_solves solves
+ * each contribute one (b, p0) -> phi pair (target phi / |phi|_rms), on
+ * which bb_warm_steps Adam steps are taken over random cubic crops of
+ * bb_warm_crop cells; afterwards the network only infers.
+ *
+ * No counterpart in the reference, which always starts from phi = 0
+ * (src/solver_kernel.cu:258-283).
+ */
+#include <math.h>
+#include <stdint.h>
+
+#include "bb_common.h"
+
+extern "C" int bb_unet_create(unsigned long long seed);
+extern "C" int bb_unet_volume(int nx, int ny, int nz);
+extern "C" int bb_unet_forward(const float *in, float *out);
+extern "C" double bb_unet_backward(const float *target);
+extern "C" void bb_unet_adam(float lr);
+extern "C" void bb_unet_destroy(void);
+
+int bb_warm_train_solves = 64;         /* solves that feed the training */
+int bb_warm_steps = 4;                 /* Adam steps per training pair */
+int bb_warm_crop = 64;                 /* training crop edge (cells) */
+float bb_warm_lr = 1e-3f;
+unsigned long long bb_warm_seed = 20260104ULL;
+/* Statistics of the last solve. */
+int bb_warm_used = 0;                  /* 1: phi0 came from the network */
+double bb_warm_alpha = 0.;
+double bb_warm_loss = 0.;              /* last training loss */
+int bb_warm_solves = 0;                /* solves seen */
+
+static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
+static float *_cin = NULL, *_ctgt = NULL;
+static double *_wpart = NULL, *wpart_host = NULL;
+static long wnv = 0;
+static int crop_edge = 0;
+
+/* Interior cell v = x + nx*(y + ny*z) of the Gcc grid. */
+__device__ inline long cell_of_v(const DevDom &D, long v)
+{
+  int x = (int) (v % D.nx);
+  long r = v / D.nx;
+  int y = (int) (r % D.ny), z = (int) (r / D.ny);
+  return gidx(D.cc, x + 1, y + 1, z + 1);
+}
+
+/* Per-block partial sums of {b, b^2, p, p^2} over interior cells. */
+__global__ void __launch_bounds__(256)
+moments_kernel(DevDom D, const double *__restrict__ b,
+  const double *__restrict__ p, double *__restrict__ part)
+{
+  __shared__ double s[4][256];
+  long nv = (long) D.nx * D.ny * D.nz;
+  double a[4] = {0., 0., 0., 0.};
+  for (long v = (long) blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (long) gridDim.x * blockDim.x) {
+    long c = cell_of_v(D, v);
+    double x = b[c], y = p[c];
+    a[0] += x; a[1] += x * x; a[2] += y; a[3] += y * y;
+  }
+  for (int q = 0; q < 4; q++) s[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int q = 0; q < 4; q++) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; q++) part[blockIdx.x * 4 + q] = s[q][0];
+}
+
+#define MBLK 512
+
+static void moments(const double *b, const double *p, double m[4])
+{
+  hipLaunchKernelGGL(moments_kernel, dim3(MBLK), dim3(256), 0, bb_stream,
+    ddom, b, p, _wpart);
+  BB_CHECK(hipMemcpyAsync(wpart_host, _wpart, MBLK * 4 * sizeof(double),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  for (int q = 0; q < 4; q++) m[q] = 0.;
+  for (int i = 0; i < MBLK; i++)
+    for (int q = 0; q < 4; q++) m[q] += wpart_host[i * 4 + q];
+}
+
+/* Network input: [b * ib, (p - pm) * ip] per interior cell; particle cells
+ * carry zeros. */
+__global__ void warm_input_kernel(DevDom D, const double *__restrict__ b,
+  const double *__restrict__ p, const uint8_t *__restrict__ fluid, double ib,
+  double pm, double ip, float *__restrict__ in)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) D.nx * D.ny * D.nz;
+  if (v >= nv) return;
+  long c = cell_of_v(D, v);
+  bool f = fluid == NULL || fluid[c];
+  in[2 * v] = f ? (float) (b[c] * ib) : 0.f;
+  in[2 * v + 1] = f ? (float) ((p[c] - pm) * ip) : 0.f;
+}
+
+/* Training target phi * s (interior cells). */
+__global__ void warm_target_kernel(DevDom D, const double *__restrict__ phi,
+  const uint8_t *__restrict__ fluid, double pm, double s,
+  float *__restrict__ t)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) D.nx * D.ny * D.nz;
+  if (v >= nv) return;
+  long c = cell_of_v(D, v);
+  bool f = fluid == NULL || fluid[c];
+  t[v] = f ? (float) ((phi[c] - pm) * s) : 0.f;
+}
+
+/* Random cubic crop (edge e, origin ox, oy, oz) of the input and target. */
+__global__ void crop_kernel(DevDom D, const float *__restrict__ in,
+  const float *__restrict__ t, int e, int ox, int oy, int oz,
+  float *__restrict__ cin, float *__restrict__ ct)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long n = (long) e * e * e;
+  if (v >= n) return;
+  int x = (int) (v % e), y = (int) ((v / e) % e), z = (int) (v / ((long) e * e));
+  long src = (long) (x + ox) + (long) D.nx * ((y + oy) + (long) D.ny * (z + oz));
+  cin[2 * v] = in[2 * src];
+  cin[2 * v + 1] = in[2 * src + 1];
+  ct[v] = t[src];
+}
+
+/* psi (FP32 network output) into the Gcc FP64 array phi, zero in particle
+ * cells and ghosts untouched. */
+__global__ void warm_place_kernel(DevDom D, const float *__restrict__ y,
+  const uint8_t *__restrict__ fluid, double *__restrict__ phi)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) D.nx * D.ny * D.nz;
+  if (v >= nv) return;
+  long c = cell_of_v(D, v);
+  bool f = fluid == NULL || fluid[c];
+  phi[c] = f ? (double) y[v] : 0.;
+}
+
+static inline unsigned nb(long n) { return (unsigned) ((n + 255) / 256); }
+
+static bool dims_ok(void)
+{
+  return ddom.nx % 4 == 0 && ddom.ny % 4 == 0 && ddom.nz % 4 == 0;
+}
+
+void bb_warm_setup(void)
+{
+  if (pp_warmstart != PP_WARM_LEARNED) return;
+  if (!dims_ok()) {
+    fprintf(stderr, "pp_warmstart LEARNED needs grid dimensions divisible by "
+      "4 (got %d %d %d)\n", ddom.nx, ddom.ny, ddom.nz);
+    exit(EXIT_FAILURE);
+  }
+  bb_unet_create(bb_warm_seed);
+  wnv = (long) ddom.nx * ddom.ny * ddom.nz;
+  BB_CHECK(hipMalloc((void **) &_win, 2 * wnv * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_wtgt, wnv * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_wout, wnv * sizeof(float)));
+  int e = bb_warm_crop;
+  int m = ddom.nx < ddom.ny ? ddom.nx : ddom.ny;
+  m = m < ddom.nz ? m : ddom.nz;
+  if (e > m) e = m;
+  e -= e % 4;
+  crop_edge = e;
+  long ne = (long) e * e * e;
+  BB_CHECK(hipMalloc((void **) &_cin, 2 * ne * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_ctgt, ne * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_wpart, MBLK * 4 * sizeof(double)));
+  BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
+    hipHostMallocDefault));
+  bb_warm_solves = 0;
+}
+
+void bb_warm_teardown(void)
+{
+  if (!_win) return;
+  float *f[] = {_win, _wtgt, _wout, _cin, _ctgt};
+  for (int i = 0; i < 5; i++) BB_CHECK(hipFree(f[i]));
+  BB_CHECK(hipFree(_wpart));
+  BB_CHECK(hipHostFree(wpart_host));
+  _win = _wtgt = _wout = _cin = _ctgt = NULL;
+  _wpart = NULL;
+  wpart_host = NULL;
+  bb_unet_destroy();
+}
+
+static double rms_scale(double sum, double sum2, long n, double *mean)
+{
+  *mean = sum / n;
+  double var = sum2 / n - (*mean) * (*mean);
+  return var > 1e-300 ? 1. / sqrt(var) : 0.;
+}
+
+/* Network input from the current right-hand side and p0. */
+static void build_input(const uint8_t *fluid)
+{
+  double m[4];
+  moments(_rhs_p, _p0, m);
+  double bm, pm;
+  double ib = rms_scale(m[0], m[1], wnv, &bm);
+  double ip = rms_scale(m[2], m[3], wnv, &pm);
+  hipLaunchKernelGGL(warm_input_kernel, dim3(nb(wnv)), dim3(256), 0,
+    bb_stream, ddom, _rhs_p, _p0, fluid, ib, pm, ip, _win);
+}
+
+/* The scale alpha and phi0 = alpha psi (psi already in _phi); implemented in
+ * bb_solver.hip with the solver's operator and masks. */
+double bb_pp_energy_scale(void);
+
+/* C ABI: the learned guess for the coming solve into _phi.  Returns 1 if
+ * it was placed, 0 if the solve starts from phi = 0. */
+extern "C" int cuda_PP_warmstart_infer(void)
+{
+  bb_warm_used = 0;
+  if (pp_warmstart != PP_WARM_LEARNED || !_win) return 0;
+  if (bb_warm_solves < bb_warm_train_solves) return 0;
+  const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
+  build_input(fluid);
+  bb_unet_volume(ddom.nx, ddom.ny, ddom.nz);
+  bb_unet_forward(_win, _wout);
+  hipLaunchKernelGGL(warm_place_kernel, dim3(nb(wnv)), dim3(256), 0,
+    bb_stream, ddom, _wout, fluid, _phi);
+  BB_LAUNCH_CHECK();
+  double a = bb_pp_energy_scale();
+  bb_warm_alpha = a;
+  bb_warm_used = (a > 0. && isfinite(a)) ? 1 : 0;
+  return bb_warm_used;
+}
+
+static unsigned long long crng = 0x2545F4914F6CDD1DULL;
+static int crand(int n)
+{
+  crng ^= crng << 13; crng ^= crng >> 7; crng ^= crng << 17;
+  return n > 0 ? (int) (crng % (unsigned long long) n) : 0;
+}
+
+/* C ABI: one training round on the solve just finished (input b, p0;
+ * target phi). */
+extern "C" void cuda_PP_warmstart_train(void)
+{
+  if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
+  int train = bb_warm_solves < bb_warm_train_solves;
+  bb_warm_solves++;
+  if (!train) return;
+  const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
+  build_input(fluid);
+  double m[4];
+  moments(_phi, _phi, m);
+  double phm;
+  double is = rms_scale(m[0], m[1], wnv, &phm);
+  if (is == 0.) return;
+  hipLaunchKernelGGL(warm_target_kernel, dim3(nb(wnv)), dim3(256), 0,
+    bb_stream, ddom, _phi, fluid, phm, is, _wtgt);
+  int e = crop_edge;
+  bb_unet_volume(e, e, e);
+  for (int s = 0; s < bb_warm_steps; s++) {
+    int ox = crand(ddom.nx - e + 1), oy = crand(ddom.ny - e + 1),
+        oz = crand(ddom.nz - e + 1);
+    hipLaunchKernelGGL(crop_kernel, dim3(nb((long) e * e * e)), dim3(256), 0,
+      bb_stream, ddom, _win, _wtgt, e, ox, oy, oz, _cin, _ctgt);
+    bb_unet_forward(_cin, _wout);
+    bb_warm_loss = bb_unet_backward(_ctgt);
+    bb_unet_adam(bb_warm_lr);
+  }
+  BB_LAUNCH_CHECK();
+}

@@ -427,6 +427,13 @@ void cuda_lamb_err(real *error, int *number);   /* :859-903 */
 void cuda_move_parts_sub(void);              /* src/cuda_particle.cu:1335-1473 */
 void cuda_update_part_velocity(void);        /* :1477-1496 */
 void cuda_update_part_position(void);        /* :1498-1513 + periodic wrap */
+/* Learned initial guess of the pressure solve (pp_warmstart LEARNED); no
+ * counterpart in the reference, which starts every solve from phi = 0
+ * (src/solver_kernel.cu:258-283).  infer places alpha * U-Net(b, p0) in
+ * _phi before a solve (returns 1 if it did); train feeds the finished
+ * solve to the network while it is learning. */
+int cuda_PP_warmstart_infer(void);
+void cuda_PP_warmstart_train(void);
 /* Periodic ghost refresh of one field; replaces mpi_cuda_exchange_G*
  * (src/mpi_comm.c:257-405) on a single GPU.  `grid` is GCC/GFX/GFY/GFZ. */
 void cuda_exchange(real *array, int grid);

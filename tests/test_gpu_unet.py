@@ -1,0 +1,159 @@
+"""The learned-warm-start U-Net on the GPU against its CPU reference, and the
+warm start inside the pressure solve.
+
+- forward pass: BF16 MFMA kernels vs the torch FP32 CPU network with BF16
+  rounding at the same points (oracle/unet_ref.py), relative L2 <= 2e-3;
+- gradients of 0.5 mean (y - t)^2 with respect to every weight and bias
+  vs torch autograd on the CPU, relative L2 <= 5e-2 per layer (the device
+  rounds the back-propagated gradients to BF16 as MFMA inputs);
+- Adam steps on one sample reduce its loss;
+- in the solver: every solve with the learned guess still ends at the same
+  FP64 stopping test, and after training the guess lowers the PCG
+  iteration count against the plain start (phi = 0).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import bbconfig
+import bbhip
+
+pytestmark = pytest.mark.gpu
+
+
+def lib():
+    L = bbhip.load()
+    L.bb_unet_create.argtypes = [ctypes.c_ulonglong]
+    L.bb_unet_volume.argtypes = [ctypes.c_int] * 3
+    L.bb_unet_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.bb_unet_backward.argtypes = [ctypes.c_void_p]
+    L.bb_unet_backward.restype = ctypes.c_double
+    L.bb_unet_adam.argtypes = [ctypes.c_float]
+    L.bb_unet_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.bb_unet_layer_info.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    return L
+
+
+def layer_sizes(L):
+    info = (ctypes.c_int * 7)()
+    out = []
+    for l in range(11):
+        L.bb_unet_layer_info(l, info)
+        cin, cout, taps, _, _, cinp, coutp = list(info)
+        out.append((taps * coutp * cinp, coutp))
+    return out
+
+
+def get_params(L, what_w, what_b):
+    ws, bs = [], []
+    for l, (nw, nb) in enumerate(layer_sizes(L)):
+        w = np.zeros(nw, dtype=np.float32)
+        b = np.zeros(nb, dtype=np.float32)
+        L.bb_unet_get(l, what_w, w.ctypes.data)
+        L.bb_unet_get(l, what_b, b.ctypes.data)
+        ws.append(w)
+        bs.append(b)
+    return ws, bs
+
+
+@pytest.fixture(scope="module")
+def net():
+    import torch
+    L = lib()
+    L.cuda_device_init(0)
+    L.bb_unet_create(1234)
+    yield L, torch
+    L.bb_unet_destroy()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_and_gradients(net):
+    L, torch = net
+    import unet_ref
+    dims = (16, 12, 8)
+    nv = dims[0] * dims[1] * dims[2]
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((nv, 2)).astype(np.float32)
+    t = rng.standard_normal(nv).astype(np.float32)
+    assert L.bb_unet_volume(*dims) == 0
+    xd = torch.from_numpy(x).cuda()
+    td = torch.from_numpy(t).cuda()
+    yd = torch.zeros(nv, device="cuda")
+    torch.cuda.synchronize()
+    L.bb_unet_forward(xd.data_ptr(), yd.data_ptr())
+    L.cuda_block()
+    loss = L.bb_unet_backward(td.data_ptr())
+    L.cuda_block()
+    y = yd.cpu().numpy()
+    ws, bs = get_params(L, 0, 1)
+    gws, gbs = get_params(L, 2, 3)
+    rloss, ry, rgw, rgb = unet_ref.grads(ws, bs, x, t, dims)
+    yr = ry.transpose(2, 1, 0).ravel()      # (nz, ny, nx) -> channels-last
+    ef = rel(y, yr)
+    print("forward rel L2", ef, "loss", loss, rloss)
+    assert ef <= 2e-3
+    assert abs(loss - rloss) <= 2e-3 * abs(rloss)
+    for l in range(11):
+        ew = rel(gws[l], rgw[l])
+        eb = rel(gbs[l][:len(rgb[l])], rgb[l])
+        print(f"layer {l}: dW rel {ew:.2e}  db rel {eb:.2e}")
+        assert ew <= 5e-2 and eb <= 5e-2, l
+
+
+def test_adam_reduces_loss(net):
+    L, torch = net
+    dims = (16, 16, 16)
+    nv = 16 ** 3
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((nv, 2)).astype(np.float32)
+    # a smooth target the network can fit
+    g = np.arange(16) / 16.0
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    t = (np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Z)).transpose(2, 1, 0)
+    t = t.ravel().astype(np.float32)
+    assert L.bb_unet_volume(*dims) == 0
+    xd = torch.from_numpy(x).cuda()
+    td = torch.from_numpy(t).cuda()
+    yd = torch.zeros(nv, device="cuda")
+    torch.cuda.synchronize()
+    losses = []
+    for _ in range(60):
+        L.bb_unet_forward(xd.data_ptr(), yd.data_ptr())
+        losses.append(L.bb_unet_backward(td.data_ptr()))
+        L.bb_unet_adam(ctypes.c_float(3e-3))
+    L.cuda_block()
+    print("loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_warm_start_in_solver(case_dir):
+    """Sedimentation box 32^3 with two spheres: 6 training solves, then the
+    learned guess; every solve reaches the FP64 stopping test."""
+    kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=7)
+    kw["extensions"] = {"pp_warmstart": "LEARNED"}
+    bbconfig.write_case(case_dir, **kw)
+    bbconfig.write_parts(case_dir, parts)
+    L = lib()
+    ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 6
+    ctypes.c_int.in_dll(L, "bb_warm_steps").value = 20
+    ctypes.c_int.in_dll(L, "bb_warm_crop").value = 32
+    with bbhip.Simulation(case_dir, kw["n"]) as s:
+        iters, used = [], []
+        for _ in range(4):
+            s.step()
+            st = s.pp_stats()
+            assert st["resid"] <= 1e-8 or st["sp_rhs"] < 1e-16
+            iters.append(st["niter"])
+            used.append(ctypes.c_int.in_dll(L, "bb_warm_used").value)
+        print("iterations", iters, "learned guess used", used,
+              "alpha", ctypes.c_double.in_dll(L, "bb_warm_alpha").value,
+              "loss", ctypes.c_double.in_dll(L, "bb_warm_loss").value)
+    ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 64
+    ctypes.c_int.in_dll(L, "bb_warm_steps").value = 4
+    ctypes.c_int.in_dll(L, "bb_warm_crop").value = 64
