@@ -1,13 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark of the per-timestep flow path on one MI355X.
+"""Benchmark of the per-timestep path on one MI355X.
 
-A "step" is one full Bluebottle timestep (u*, BCs, solvability, Jacobi-PCG
-pressure solve to the reference's 1e-8 stopping test, projection, pressure
-update, dt) on synthetic input of the periodic-channel workload
-(256 x 128 x 128, Re_tau 180, seeded perturbation), all fields resident in
-HBM.  Prints one JSON line (rank 0) with the throughput, a live roofline of
-the dominant kernel (HIP events on the library stream) and a CPU baseline of
-the FP64 host oracle on a bounded sample.
+A "step" is one full Bluebottle timestep: the Lamb iteration loop of u*,
+BCs, solvability, Jacobi-PCG pressure solve to the reference's 1e-8 stopping
+test, projection, pressure update and Physalis Lamb coefficients, then the
+particle motion, all fields resident in HBM.
+
+Default workload: triply periodic sedimentation box 256^3 with 500 spheres
+of radius 6 dx (seeded non-overlapping positions, density ratio 2, gravity
+in -z, Lamb order 4).  It runs twice, once per form:
+  plain          FP64 PCG from phi = 0 (the reference's solver);
+  mixed+learned  FP32 PCG inside FP64 defect correction, initial guess from
+                 the U-Net trained online on the solver's own solves (its
+                 training solves happen in untimed warm-up steps).
+`value` is the mixed+learned form's timesteps/s; the plain form's figures
+sit beside it.  Prints one JSON line with the throughput, a live roofline
+of the plain form's dominant kernel (HIP events on the library stream) and
+a CPU baseline of the FP64 host oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -51,12 +60,24 @@ KERNEL_BYTES = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ny", type=int, default=128)
-    ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--precision", default="PLAIN", choices=["PLAIN", "MIXED"])
+    ap.add_argument("--workload", default="sediment",
+                    choices=["sediment", "channel"],
+                    help="sediment: periodic box with spheres (default); "
+                         "channel: single-phase channel at Re_tau 180")
+    ap.add_argument("--form", default="both",
+                    choices=["both", "plain", "mixed-learned", "mixed"],
+                    help="which forms to time (mixed: FP32-PCG without the "
+                         "learned guess)")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--nparts", type=int, default=500)
+    ap.add_argument("--radius", type=float, default=6.0)
+    ap.add_argument("--seed", type=int, default=20260103)
+    ap.add_argument("--train-solves", type=int, default=None,
+                    help="solves that feed the network's online training")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
@@ -67,28 +88,164 @@ def parse():
     ap.add_argument("--time-every", type=int, default=8,
                     help="HIP-event timing on one launch in this many of the "
                          "timed kernels (0: no event timing)")
+    ap.add_argument("--verbose", action="store_true",
+                    help="per-step progress on stderr")
     ap.add_argument("--calib-only", action="store_true",
                     help="run only the stream-copy calibration kernel "
                          "(2^27 doubles per array) and print its GB/s")
-    return ap.parse_args()
+    a = ap.parse_args()
+    dflt = (256, 256, 256) if a.workload == "sediment" else (256, 128, 128)
+    a.n = tuple(v if v is not None else d
+                for v, d in zip((a.nx, a.ny, a.nz), dflt))
+    return a
 
 
-def cpu_baseline(case_dir):
-    """Time the FP64 host oracle on one timestep of the same workload."""
+def write_case(a, case, precision, warm):
+    ext = {"pp_precision": precision, "pp_warmstart": warm}
+    if a.workload == "channel":
+        kw = bbconfig.channel(n=a.n, precision=precision)
+        kw["extensions"] = dict(kw.get("extensions") or {}, **ext)
+        bbconfig.write_case(case, **kw)
+        return
+    kw, parts = bbconfig.sediment(n=a.n, nparts=a.nparts, a=a.radius,
+                                  seed=a.seed)
+    kw["extensions"] = ext
+    bbconfig.write_case(case, **kw)
+    bbconfig.write_parts(case, parts)
+
+
+def log(a, *msg):
+    if a.verbose:
+        print(*msg, file=sys.stderr, flush=True)
+
+
+def run_form(lib, a, precision, warm):
+    """Warm-up (and, for the learned guess, the network's training solves),
+    then exactly a.steps timed timesteps."""
+    case = tempfile.mkdtemp(prefix="bb_bench_")
+    write_case(a, case, precision, warm)
+    tot = ctypes.c_long.in_dll(lib, "bb_pcg_iters_total")
+    solves = ctypes.c_long.in_dll(lib, "bb_pcg_solves_total")
+    wsolves = ctypes.c_int.in_dll(lib, "bb_warm_solves")
+    train = ctypes.c_int.in_dll(lib, "bb_warm_train_solves").value
+    sim = bbhip.Simulation(case, a.n)
+    tw0 = time.perf_counter()
+    sim.setup()
+    nw = 0
+    while nw < a.warmup or (warm == "LEARNED" and wsolves.value < train):
+        sim.step()
+        nw += 1
+        log(a, f"[{precision}/{warm}] warm-up step {nw}: solves "
+               f"{wsolves.value}/{train} loss "
+               f"{ctypes.c_double.in_dll(lib, 'bb_warm_loss').value:.3e}")
+        if nw > a.warmup + 1000:
+            raise RuntimeError("network training did not finish")
+    sim.sync()
+    tw1 = time.perf_counter()
+    lib.bb_ktimer_reset()
+    ctypes.c_int.in_dll(lib, "bb_time_every").value = max(a.time_every, 1)
+    lib.bb_ktimer_enable(1 if a.time_every > 0 else 0)
+    it0, sv0 = tot.value, solves.value
+    per_step, lamb, resid = [], [], []
+    sim.sync()
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        i0 = tot.value
+        sim.step()
+        per_step.append(tot.value - i0)
+        lamb.append(sim.lamb_stats()[0])
+        resid.append(sim.pp_stats()["resid"])
+        log(a, f"[{precision}/{warm}] step {s}: PCG iterations "
+               f"{per_step[-1]}, Lamb passes {lamb[-1]}")
+    sim.sync()
+    t1 = time.perf_counter()
+    lib.bb_ktimer_enable(0)
+    ms = ctypes.c_double()
+    cnt = ctypes.c_long()
+    ktimes = {}
+    for name, kid in KT_IDS.items():
+        lib.bb_ktimer_read(kid, ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value > 0:
+            ktimes[name] = (ms.value, cnt.value)
+    stats = sim.pp_stats()
+    nsolves = solves.value - sv0
+    sim.teardown()
+    shutil.rmtree(case, ignore_errors=True)
+    elapsed = t1 - t0
+    return {
+        "timesteps_per_s": a.steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "pcg_iters_per_step": sum(per_step) / len(per_step),
+        "pcg_iters_per_solve": (tot.value - it0) / max(nsolves, 1),
+        "lamb_passes_per_step": sum(lamb) / len(lamb),
+        "max_final_resid": max(resid),
+        "last_true_resid": stats["true_resid"],
+        "warmup_steps": nw,
+        "warmup_s": tw1 - tw0,
+        "ktimes": ktimes,
+    }
+
+
+def roofline_of(ktimes, cells, copy_gbs):
+    if not ktimes:
+        return None
+    dom_name = max(ktimes, key=lambda k: ktimes[k][0])
+    kms, kn = ktimes[dom_name]
+    avg_s = (kms / 1e3) / kn
+    achieved = KERNEL_BYTES[dom_name] * cells / avg_s / 1e9
+    return {
+        "kernel": dom_name,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "frac_of_measured_copy": achieved / copy_gbs,
+        "measured_copy_gbs": copy_gbs,
+        "traffic": None,
+        "bytes_per_cell": KERNEL_BYTES[dom_name],
+        "avg_launch_us": avg_s * 1e6,
+        "launches_timed": kn,
+    }
+
+
+def cpu_baseline(a, iters_per_step, lamb_per_step):
+    """Time the FP64 host oracle on a bounded sample of the same workload:
+    two truncated first timesteps (Lamb loop cut to one pass, PCG cut to
+    k1 and k2 iterations).  Their difference gives the time per PCG
+    iteration, the rest the fixed cost of one Lamb pass (u*, BCs, cages,
+    projection, Physalis); the CPU timestep is then priced with the plain
+    GPU form's measured iterations and Lamb passes per step."""
     import orc  # the checker, used here only as the CPU baseline
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = orc.Oracle(case_dir)
-    o.setup()
-    t0 = time.perf_counter()
-    o.step(1)
-    t1 = time.perf_counter()
-    niter = o.scalar("niter")
-    o.teardown()
-    return {"value": 1.0 / (t1 - t0), "unit": "timesteps/s", "cores": cores,
+    case = tempfile.mkdtemp(prefix="bb_cpu_")
+    write_case(a, case, "PLAIN", "OFF")
+    k1, k2 = 2, 12
+    t = {}
+    for k in (k1, k2):
+        o = orc.Oracle(case)
+        o.setup()
+        o.set_scalar("pp_max_iter", k)
+        o.set_scalar("lamb_max_iter", 1)
+        t0 = time.perf_counter()
+        o.step(1)           # stops after k iterations (not converged)
+        t[k] = time.perf_counter() - t0
+        o.teardown()
+    shutil.rmtree(case, ignore_errors=True)
+    t_iter = max(t[k2] - t[k1], 0.0) / (k2 - k1)
+    t_pass = max(t[k1] - k1 * t_iter, 0.0)
+    t_step = iters_per_step * t_iter + lamb_per_step * t_pass
+    return {"value": 1.0 / t_step, "unit": "timesteps/s", "cores": cores,
             "kind": "port",
-            "sample": f"first timestep of the same workload ({int(niter)} PCG "
-                      f"iterations), FP64 C oracle with OpenMP on {cores} "
-                      f"threads"}
+            "sample": f"FP64 C oracle (OpenMP, {cores} threads) on the same "
+                      f"workload: two truncated first timesteps, one Lamb "
+                      f"pass each, PCG stopped at {k1} and {k2} iterations "
+                      f"({t[k1]:.1f} s and {t[k2]:.1f} s); "
+                      f"{t_iter * 1e3:.1f} ms per PCG iteration and "
+                      f"{t_pass:.2f} s per Lamb pass outside the solve, priced "
+                      f"at the plain GPU form's {iters_per_step:.0f} PCG "
+                      f"iterations and {lamb_per_step:.1f} Lamb passes per "
+                      f"timestep"}
 
 
 def main():
@@ -106,93 +263,62 @@ def main():
                           "elements": 1 << 27}))
         return
     for name, val in (("bb_spmv_variant", a.spmv_variant),
-                      ("bb_pcg_batch", a.batch)):
+                      ("bb_pcg_batch", a.batch),
+                      ("bb_warm_train_solves", a.train_solves)):
         if val is not None:
             ctypes.c_int.in_dll(lib, name).value = val
-    n = (a.nx, a.ny, a.nz)
-    case = tempfile.mkdtemp(prefix="bb_bench_")
-    bbconfig.write_case(case, **bbconfig.channel(n=n, precision=a.precision))
-    sim = bbhip.Simulation(case, n)
-    sim.setup()
-    iters = []
-    for _ in range(a.warmup):
-        sim.step()
-        iters.append(sim.pp_stats()["niter"])
-    sim.sync()
-    lib.bb_ktimer_reset()
-    ctypes.c_int.in_dll(lib, "bb_time_every").value = max(a.time_every, 1)
-    lib.bb_ktimer_enable(1 if a.time_every > 0 else 0)
-    sim.sync()
-    t0 = time.perf_counter()
-    timed_iters = []
-    for _ in range(a.steps):
-        sim.step()
-        timed_iters.append(sim.pp_stats()["niter"])
-    sim.sync()
-    t1 = time.perf_counter()
-    lib.bb_ktimer_enable(0)
-    ms = ctypes.c_double()
-    cnt = ctypes.c_long()
-    ktimes = {}
-    for name, kid in KT_IDS.items():
-        lib.bb_ktimer_read(kid, ctypes.byref(ms), ctypes.byref(cnt))
-        if cnt.value > 0:
-            ktimes[name] = (ms.value, cnt.value)
-    stats = sim.pp_stats()
-    sim.teardown()
-
-    elapsed = t1 - t0
-    cells = a.nx * a.ny * a.nz
-    roofline = None
-    if ktimes:
-        dom_name = max(ktimes, key=lambda k: ktimes[k][0])
-        kms, kn = ktimes[dom_name]
-        avg_s = (kms / 1e3) / kn
-        achieved = KERNEL_BYTES[dom_name] * cells / avg_s / 1e9
-        roofline = {
-            "kernel": dom_name,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "frac_of_measured_copy": achieved / copy_gbs,
-            "measured_copy_gbs": copy_gbs,
-            "traffic": None,
-            "bytes_per_cell": KERNEL_BYTES[dom_name],
-            "avg_launch_us": avg_s * 1e6,
-            "launches_timed": kn,
-        }
+    forms = {"both": [("plain", "PLAIN", "OFF"),
+                      ("mixed_learned", "MIXED", "LEARNED")],
+             "plain": [("plain", "PLAIN", "OFF")],
+             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED")],
+             "mixed": [("mixed", "MIXED", "OFF")]}[a.form]
+    res = {}
+    for key, prec, warm in forms:
+        res[key] = run_form(lib, a, prec, warm)
+    cells = a.n[0] * a.n[1] * a.n[2]
+    head = res[forms[-1][0]]
+    plain = res.get("plain")
+    if a.workload == "sediment":
+        wl = (f"sedimentation {a.n[0]}x{a.n[1]}x{a.n[2]} triply periodic, "
+              f"{a.nparts} spheres a={a.radius:g} dx, rho 2, Lamb order 4, "
+              f"PCG to 1e-8; timed form: {forms[-1][0]}")
+    else:
+        wl = (f"periodic channel {a.n[0]}x{a.n[1]}x{a.n[2]}, Re_tau 180, "
+              f"PCG to 1e-8; timed form: {forms[-1][0]}")
+    forms_out = {}
+    for key, r in res.items():
+        forms_out[key] = {k: v for k, v in r.items() if k != "ktimes"}
+        forms_out[key]["kernels_avg_us"] = {
+            k: 1e3 * v[0] / v[1] for k, v in r["ktimes"].items()}
     out = {
         "metric": METRIC,
-        "value": a.steps / elapsed,
+        "value": head["timesteps_per_s"],
         "unit": "timesteps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "PLAIN" else "f32+f64",
+        "dtype": "f64" if forms[-1][1] == "PLAIN" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"periodic channel {a.nx}x{a.ny}x{a.nz}, Re_tau 180, "
-                        f"{a.precision.lower()} Jacobi-PCG to 1e-8",
-            "grid": [a.nx, a.ny, a.nz],
-            "pp_precision": a.precision,
-            "pcg_iters_per_step": sum(timed_iters) / max(len(timed_iters), 1),
+            "workload": wl,
+            "grid": list(a.n),
+            "nparts": a.nparts if a.workload == "sediment" else 0,
+            "forms": forms_out,
+            "speedup_vs_plain": (head["timesteps_per_s"] /
+                                 plain["timesteps_per_s"]
+                                 if plain and head is not plain else None),
             "spmv_variant": ctypes.c_int.in_dll(lib, "bb_spmv_variant").value,
-            "last_resid": stats["resid"],
-            "last_true_resid": stats["true_resid"],
-            "last_corrections": stats["nouter"],
         },
-        "roofline": roofline,
-        "kernels_avg_us": {k: 1e3 * v[0] / v[1] for k, v in ktimes.items()},
+        "roofline": roofline_of((plain or head)["ktimes"], cells, copy_gbs),
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case)
-    shutil.rmtree(case, ignore_errors=True)
+        ref = plain or head
+        out["cpu_baseline"] = cpu_baseline(a, ref["pcg_iters_per_step"],
+                                           ref["lamb_passes_per_step"])
     print(json.dumps(out))
 
 

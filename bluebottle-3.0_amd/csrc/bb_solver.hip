@@ -619,33 +619,47 @@ cg_init_warm_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ rhs,
   tile_partial(acc, gpart, tile);
 }
 
-/* Tile partials of (b, psi) and (psi, (-A) psi) over fluid cells. */
+/* Tile partials of the Galerkin system on span{psi, q}: (b, psi), (b, q),
+ * (psi, -A psi), (psi, -A q), (q, -A q), over fluid cells. */
 __global__ void __launch_bounds__(TILE_THREADS)
-energy_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
-  const double *__restrict__ psi, const double *__restrict__ rhs,
-  double *__restrict__ part_b, double *__restrict__ part_a)
+galerkin_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
+  const double *__restrict__ psi, const double *__restrict__ q,
+  const double *__restrict__ rhs, double *__restrict__ part)
 {
   TILE_PROLOGUE
-  double ab = 0.0, aa = 0.0;
+  double a[5] = {0., 0., 0., 0., 0.};
   if (colok) {
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       if (solid(fluid, c)) continue;
-      double ps = psi[c];
-      ab += rhs[c] * ps;
-      aa += ps * neg_lap_at(D, K, psi, x, j, k);
+      double ps = psi[c], qs = q[c], b = rhs[c];
+      double Aps = neg_lap_at(D, K, psi, x, j, k);
+      double Aq = neg_lap_at(D, K, q, x, j, k);
+      a[0] += b * ps;
+      a[1] += b * qs;
+      a[2] += ps * Aps;
+      a[3] += ps * Aq;
+      a[4] += qs * Aq;
     }
   }
-  tile_partial(ab, part_b, tile);
-  tile_partial(aa, part_a, tile);
+  long nt = (long) ntx_ * nty_ * ntz_;
+  for (int m = 0; m < 5; m++) {
+    tile_partial(a[m], part + m * nt, tile);
+    __syncthreads();                  /* tile_partial's LDS is reused */
+  }
 }
 
+/* phi = alpha psi + beta q on interior cells. */
 __global__ void __launch_bounds__(TILE_THREADS)
-scale_kernel(DevDom D, double a, double *__restrict__ f)
+combine_kernel(DevDom D, double al, const double *__restrict__ psi, double be,
+  const double *__restrict__ q, double *__restrict__ phi)
 {
   TILE_PROLOGUE
   if (!colok) return;
-  for (int k = k0; k <= k1; k++) f[gidx(C, x, j, k)] *= a;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
+    phi[c] = al * psi[c] + be * q[c];
+  }
 }
 
 /* Tile partials of ||b - (-A) phi||^2 (diagnostic true residual). */
@@ -792,6 +806,8 @@ int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
 int bb_spmv_variant = 1;               /* 1: LDS tile kernel, 0: direct loads */
 int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
+long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
+long bb_pcg_solves_total = 0;
 
 /* FP32 vectors of the mixed form's inner sweeps (Gcc layout). */
 static float *_r32 = NULL, *_p32a = NULL, *_p32b = NULL, *_Ap32 = NULL,
@@ -832,7 +848,7 @@ static void cg_buffers(void)
     if (_part_w2) BB_CHECK(hipFree(_part_w2));
     BB_CHECK(hipMalloc((void **) &_part_p, nt * sizeof(double)));
     BB_CHECK(hipMalloc((void **) &_part_r, nt * sizeof(double)));
-    BB_CHECK(hipMalloc((void **) &_part_w1, nt * sizeof(double)));
+    BB_CHECK(hipMalloc((void **) &_part_w1, 5 * nt * sizeof(double)));
     BB_CHECK(hipMalloc((void **) &_part_w2, nt * sizeof(double)));
     part_n = nt;
   }
@@ -889,24 +905,44 @@ double bb_true_residual(double sp_rhs)
   return sp_rhs > 0 ? sqrt(rr) / sqrt(sp_rhs) : 0.;
 }
 
-/* The learned guess psi in _phi becomes alpha psi with
- * alpha = (b, psi) / (psi, (-A) psi); returns alpha (phi is left alone when
- * alpha is not positive and finite). */
-double bb_pp_energy_scale(void)
+/* The initial guess with the least A-norm error in span{psi, q}:
+ * phi = alpha psi + beta q solving the 2 x 2 Galerkin system
+ *   [(psi,A psi) (psi,A q); (q,A psi) (q,A q)] [alpha; beta]
+ *     = [(b, psi); (b, q)]
+ * (A = -Laplacian, symmetric positive semidefinite).  use_psi / use_q
+ * select the directions; a singular or non-positive system drops the
+ * direction at fault.  Returns 1 if phi was set (0: start from phi = 0). */
+int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
+  int use_q, double *alpha, double *beta)
 {
   cg_buffers();
   const long nt = n_tiles(ddom);
-  hipLaunchKernelGGL(energy_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
-    _phi, _rhs_p, _part_w1, _part_w2);
-  bb_reduce_rows(_part_w1, nt, _bb_scalars + 9);
-  bb_reduce_rows(_part_w2, nt, _bb_scalars + 10);
-  bb_fetch_scalars(11);
-  double a = bb_host_scalars[9] / bb_host_scalars[10];
-  if (a > 0. && isfinite(a)) {
-    hipLaunchKernelGGL(scale_kernel, TILES_LAUNCH, ddom, a, _phi);
-    BB_LAUNCH_CHECK();
+  *alpha = *beta = 0.;
+  if (!use_psi && !use_q) return 0;
+  const double *ps = use_psi ? psi : q, *qq = use_q ? q : psi;
+  hipLaunchKernelGGL(galerkin_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
+    ps, qq, _rhs_p, _part_w1);
+  for (int m = 0; m < 5; m++)
+    bb_reduce_rows(_part_w1 + m * nt, nt, _bb_scalars + 16 + m);
+  bb_fetch_scalars(21);
+  double *S = bb_host_scalars + 16;
+  double bp = S[0], bq = S[1], app = S[2], apq = S[3], aqq = S[4];
+  double a = 0., b = 0.;
+  double det = app * aqq - apq * apq;
+  if (use_psi && use_q && det > 1e-12 * app * aqq && app > 0. && aqq > 0.) {
+    a = (bp * aqq - bq * apq) / det;
+    b = (bq * app - bp * apq) / det;
+  } else if (use_psi && app > 0.) {
+    a = bp / app;
+  } else if (use_q && aqq > 0.) {
+    b = bq / aqq;
   }
-  return a;
+  if (!isfinite(a) || !isfinite(b) || (a == 0. && b == 0.)) return 0;
+  hipLaunchKernelGGL(combine_kernel, TILES_LAUNCH, ddom, a, ps, b, qq, _phi);
+  BB_LAUNCH_CHECK();
+  *alpha = a;
+  *beta = b;
+  return 1;
 }
 
 /* Enqueue iteration q; p buffers alternate by parity. */
@@ -1016,6 +1052,8 @@ static void pcg_plain(double ts, int warm)
   pp_last.etime = etime;
   pp_last.niter = cg_host->q;
   pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
+  bb_pcg_iters_total += cg_host->q;
+  bb_pcg_solves_total++;
 }
 
 int bb_log_true_residual = 1;
@@ -1170,6 +1208,8 @@ static void pcg_mixed(double ts, int warm)
   pp_last.niter = q;
   pp_last.nouter = nouter;
   pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
+  bb_pcg_iters_total += q;
+  bb_pcg_solves_total++;
 }
 
 extern "C" void cuda_PP_cg_mixed(void)

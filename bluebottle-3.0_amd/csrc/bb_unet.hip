@@ -193,11 +193,15 @@ __global__ void sum_chunks_kernel(const float *__restrict__ P, long n,
   out[i] = s;
 }
 
-/* Bias-gradient partials: Pb[chunk][co] = sum over the chunk of G[v, co]. */
+/* Bias-gradient partials: Pb[chunk][co] = sum over the chunk of G[v, co];
+ * zero for the padding channels co >= cout. */
 __global__ void bias_grad_kernel(TV G, int cout, long chunk, float *Pb)
 {
   int co = threadIdx.x;
-  if (co >= cout) return;
+  if (co >= cout) {
+    Pb[(long) blockIdx.x * blockDim.x + co] = 0.f;
+    return;
+  }
   long nv = (long) G.nx * G.ny * G.nz;
   long vb = (long) blockIdx.x * chunk, ve = vb + chunk < nv ? vb + chunk : nv;
   float s = 0.f;

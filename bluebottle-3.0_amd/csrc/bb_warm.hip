@@ -1,13 +1,14 @@
 /*
  * Learned initial guess for the pressure solve (pp_warmstart LEARNED).
  *
- * Each solve of -A phi = -b can start from phi0 = alpha psi, where
- * psi = U-Net(b / |b|_rms, (p0 - mean p0) / |p0 - mean|_rms) is the
- * network's predicted shape of the solution and
- *   alpha = (b, psi) / (psi, (-A) psi)
- * is the energy-optimal scale along psi, so the A-norm error of phi0 never
- * exceeds that of the reference's phi0 = 0; alpha <= 0 (or a non-finite
- * value) falls back to phi0 = 0.  Every solve still ends at the same test
+ * Each solve of -A phi = -b can start from the guess with the least A-norm
+ * error in span{psi, phi_last}: psi = U-Net(b / |b|_rms, (p - mean p) /
+ * |p - mean|_rms) is the network's predicted shape of the solution from the
+ * right-hand side and the latest pressure p, and phi_last is the previous
+ * solve's solution (the last pressure increment).  The 2 x 2 Galerkin
+ * system fixes both weights, so the guess is never worse in the A-norm
+ * than phi0 = 0 (the reference's start), than the last pressure alone, or
+ * than the network alone.  Every solve still ends at the same test
  * (r, M^-1 r) <= pp_residual^2 (b, b).
  *
  * The network is trained online: the first bb_warm_train_solves solves
@@ -41,6 +42,11 @@ double bb_warm_alpha = 0.;
 double bb_warm_loss = 0.;              /* last training loss */
 int bb_warm_solves = 0;                /* solves seen */
 
+int bb_warm_use_net = 1;               /* 0: span{phi_last} only */
+int bb_warm_use_last = 1;              /* 0: span{psi} only */
+double bb_warm_beta = 0.;
+static int have_last = 0;
+static double *_psi = NULL, *_phi_last = NULL;
 static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
 static float *_cin = NULL, *_ctgt = NULL;
 static double *_wpart = NULL, *wpart_host = NULL;
@@ -158,7 +164,7 @@ static bool dims_ok(void)
   return ddom.nx % 4 == 0 && ddom.ny % 4 == 0 && ddom.nz % 4 == 0;
 }
 
-void bb_warm_setup(void)
+extern "C" void bb_warm_setup(void)
 {
   if (pp_warmstart != PP_WARM_LEARNED) return;
   if (!dims_ok()) {
@@ -181,18 +187,26 @@ void bb_warm_setup(void)
   BB_CHECK(hipMalloc((void **) &_cin, 2 * ne * sizeof(float)));
   BB_CHECK(hipMalloc((void **) &_ctgt, ne * sizeof(float)));
   BB_CHECK(hipMalloc((void **) &_wpart, MBLK * 4 * sizeof(double)));
+  BB_CHECK(hipMalloc((void **) &_psi, ddom.cc.n * sizeof(double)));
+  BB_CHECK(hipMemset(_psi, 0, ddom.cc.n * sizeof(double)));
+  BB_CHECK(hipMalloc((void **) &_phi_last, ddom.cc.n * sizeof(double)));
+  BB_CHECK(hipMemset(_phi_last, 0, ddom.cc.n * sizeof(double)));
+  have_last = 0;
   BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
     hipHostMallocDefault));
   bb_warm_solves = 0;
 }
 
-void bb_warm_teardown(void)
+extern "C" void bb_warm_teardown(void)
 {
   if (!_win) return;
   float *f[] = {_win, _wtgt, _wout, _cin, _ctgt};
   for (int i = 0; i < 5; i++) BB_CHECK(hipFree(f[i]));
   BB_CHECK(hipFree(_wpart));
   BB_CHECK(hipHostFree(wpart_host));
+  BB_CHECK(hipFree(_psi));
+  BB_CHECK(hipFree(_phi_last));
+  _psi = _phi_last = NULL;
   _win = _wtgt = _wout = _cin = _ctgt = NULL;
   _wpart = NULL;
   wpart_host = NULL;
@@ -210,35 +224,38 @@ static double rms_scale(double sum, double sum2, long n, double *mean)
 static void build_input(const uint8_t *fluid)
 {
   double m[4];
-  moments(_rhs_p, _p0, m);
+  moments(_rhs_p, _p, m);
   double bm, pm;
   double ib = rms_scale(m[0], m[1], wnv, &bm);
   double ip = rms_scale(m[2], m[3], wnv, &pm);
   hipLaunchKernelGGL(warm_input_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _rhs_p, _p0, fluid, ib, pm, ip, _win);
+    bb_stream, ddom, _rhs_p, _p, fluid, ib, pm, ip, _win);
 }
 
-/* The scale alpha and phi0 = alpha psi (psi already in _phi); implemented in
- * bb_solver.hip with the solver's operator and masks. */
-double bb_pp_energy_scale(void);
+/* Galerkin guess on span{psi, q}, bb_solver.hip. */
+int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
+  int use_q, double *alpha, double *beta);
 
-/* C ABI: the learned guess for the coming solve into _phi.  Returns 1 if
- * it was placed, 0 if the solve starts from phi = 0. */
+/* C ABI: the guess for the coming solve into _phi.  Returns 1 if it was
+ * placed, 0 if the solve starts from phi = 0. */
 extern "C" int cuda_PP_warmstart_infer(void)
 {
   bb_warm_used = 0;
+  bb_warm_alpha = bb_warm_beta = 0.;
   if (pp_warmstart != PP_WARM_LEARNED || !_win) return 0;
-  if (bb_warm_solves < bb_warm_train_solves) return 0;
   const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
-  build_input(fluid);
-  bb_unet_volume(ddom.nx, ddom.ny, ddom.nz);
-  bb_unet_forward(_win, _wout);
-  hipLaunchKernelGGL(warm_place_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _wout, fluid, _phi);
-  BB_LAUNCH_CHECK();
-  double a = bb_pp_energy_scale();
-  bb_warm_alpha = a;
-  bb_warm_used = (a > 0. && isfinite(a)) ? 1 : 0;
+  int net_ready = bb_warm_use_net && bb_warm_solves >= bb_warm_train_solves;
+  int last = bb_warm_use_last && have_last;
+  if (net_ready) {
+    build_input(fluid);
+    bb_unet_volume(ddom.nx, ddom.ny, ddom.nz);
+    bb_unet_forward(_win, _wout);
+    hipLaunchKernelGGL(warm_place_kernel, dim3(nb(wnv)), dim3(256), 0,
+      bb_stream, ddom, _wout, fluid, _psi);
+    BB_LAUNCH_CHECK();
+  }
+  bb_warm_used = bb_pp_galerkin_guess(_psi, net_ready, _phi_last, last,
+    &bb_warm_alpha, &bb_warm_beta);
   return bb_warm_used;
 }
 
@@ -254,8 +271,11 @@ static int crand(int n)
 extern "C" void cuda_PP_warmstart_train(void)
 {
   if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
-  int train = bb_warm_solves < bb_warm_train_solves;
+  int train = bb_warm_use_net && bb_warm_solves < bb_warm_train_solves;
   bb_warm_solves++;
+  BB_CHECK(hipMemcpyAsync(_phi_last, _phi, ddom.cc.n * sizeof(double),
+    hipMemcpyDeviceToDevice, bb_stream));
+  have_last = 1;
   if (!train) return;
   const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
   build_input(fluid);

@@ -34,9 +34,10 @@ void cuda_update_part_velocity(void) __attribute__((weak));
 void cuda_update_part_position(void) __attribute__((weak));
 void cuda_parts_internal(void) __attribute__((weak));
 
-/* Learned warm start hooks. */
-void bb_warm_setup(void) __attribute__((weak));
-void bb_warm_teardown(void) __attribute__((weak));
+/* Learned warm start hooks (bb_warm.hip); they return at once unless
+ * pp_warmstart is LEARNED. */
+void bb_warm_setup(void);
+void bb_warm_teardown(void);
 
 static void exchange_vel(real *a, real *b, real *c)
 {
@@ -78,7 +79,7 @@ int bluebottle_setup(void)
   cuda_exchange(_p, GCC);
   exchange_vel(_u, _v, _w);
   cuda_PP_init_jacobi_preconditioner();
-  if (bb_warm_setup) bb_warm_setup();
+  bb_warm_setup();
   return 0;
 }
 
@@ -177,7 +178,7 @@ int bluebottle_step(void)
 void bluebottle_teardown(void)
 {
   cuda_block();
-  if (bb_warm_teardown) bb_warm_teardown();
+  bb_warm_teardown();
   if (bb_parts_teardown) bb_parts_teardown();
   cuda_dom_free();
   domain_free();

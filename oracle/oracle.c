@@ -1288,4 +1288,6 @@ void orc_set_scalar(const char *n, double v)
   if (strcmp(n, "dt") == 0) O.dt = v;
   else if (strcmp(n, "dt0") == 0) O.dt0 = v;
   else if (strcmp(n, "ttime") == 0) O.ttime = v;
+  else if (strcmp(n, "pp_max_iter") == 0) O.pp_max_iter = (int) v;
+  else if (strcmp(n, "lamb_max_iter") == 0) O_lamb_max_iter = (int) v;
 }

@@ -132,28 +132,51 @@ def test_adam_reduces_loss(net):
     assert losses[-1] < 0.5 * losses[0]
 
 
-def test_warm_start_in_solver(case_dir):
-    """Sedimentation box 32^3 with two spheres: 6 training solves, then the
-    learned guess; every solve reaches the FP64 stopping test."""
-    kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=7)
-    kw["extensions"] = {"pp_warmstart": "LEARNED"}
+def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
+    """PCG iterations per timestep (all Lamb passes) with a given start:
+    'zero' (the reference's phi0 = 0), 'last' (span{previous solution}),
+    'learned' (span{network guess, previous solution})."""
+    kw = dict(kw)
+    ext = dict(kw.get("extensions") or {})
+    ext["pp_warmstart"] = "OFF" if mode == "zero" else "LEARNED"
+    kw["extensions"] = ext
     bbconfig.write_case(case_dir, **kw)
     bbconfig.write_parts(case_dir, parts)
+    ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1 if mode == "learned" else 0
+    tot = ctypes.c_long.in_dll(L, "bb_pcg_iters_total")
+    per_step, resid = [], []
+    with bbhip.Simulation(case_dir, kw["n"]) as s:
+        for _ in range(nsteps):
+            t0 = tot.value
+            s.step()
+            per_step.append(tot.value - t0)
+            resid.append(s.pp_stats()["resid"])
+    ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1
+    return per_step, resid
+
+
+def test_warm_start_in_solver(case_dir):
+    """Sedimentation box 32^3 with two spheres: 8 training solves, then the
+    learned guess.  Every solve reaches the FP64 stopping test; the
+    iterations per step of the three starts are reported, and the guess
+    with the network may not need more iterations than without it."""
+    kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=7)
     L = lib()
-    ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 6
+    L.bb_unet_destroy()             # the solver builds its own network
+    ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 8
     ctypes.c_int.in_dll(L, "bb_warm_steps").value = 20
     ctypes.c_int.in_dll(L, "bb_warm_crop").value = 32
-    with bbhip.Simulation(case_dir, kw["n"]) as s:
-        iters, used = [], []
-        for _ in range(4):
-            s.step()
-            st = s.pp_stats()
-            assert st["resid"] <= 1e-8 or st["sp_rhs"] < 1e-16
-            iters.append(st["niter"])
-            used.append(ctypes.c_int.in_dll(L, "bb_warm_used").value)
-        print("iterations", iters, "learned guess used", used,
-              "alpha", ctypes.c_double.in_dll(L, "bb_warm_alpha").value,
-              "loss", ctypes.c_double.in_dll(L, "bb_warm_loss").value)
-    ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 64
-    ctypes.c_int.in_dll(L, "bb_warm_steps").value = 4
-    ctypes.c_int.in_dll(L, "bb_warm_crop").value = 64
+    try:
+        res = {m: solve_iterations(case_dir, kw, parts, 4, m, L)
+               for m in ("zero", "last", "learned")}
+    finally:
+        ctypes.c_int.in_dll(L, "bb_warm_train_solves").value = 64
+        ctypes.c_int.in_dll(L, "bb_warm_steps").value = 4
+        ctypes.c_int.in_dll(L, "bb_warm_crop").value = 64
+    for m, (it, rs) in res.items():
+        print(f"{m:8s} PCG iterations per step {it}  final resid {rs}")
+        assert all(r <= 1e-8 for r in rs)
+    z, last, lrn = (sum(res[m][0][2:]) for m in ("zero", "last", "learned"))
+    print("steps 3-4: zero", z, "last", last, "learned", lrn)
+    assert last < z
+    assert lrn <= 1.1 * last
