@@ -67,9 +67,11 @@ def parse():
                     help="sediment: periodic box with spheres (default); "
                          "channel: single-phase channel at Re_tau 180")
     ap.add_argument("--form", default="both",
-                    choices=["both", "plain", "mixed-learned", "mixed"],
-                    help="which forms to time (mixed: FP32-PCG without the "
-                         "learned guess)")
+                    choices=["both", "plain", "mixed-learned", "mixed",
+                             "mixed-last"],
+                    help="which forms to time (mixed: FP32-PCG from phi = 0; "
+                         "mixed-last: FP32-PCG with the guess from the "
+                         "previous solution only, no network)")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -119,11 +121,12 @@ def log(a, *msg):
         print(*msg, file=sys.stderr, flush=True)
 
 
-def run_form(lib, a, precision, warm):
+def run_form(lib, a, precision, warm, use_net=1):
     """Warm-up (and, for the learned guess, the network's training solves),
     then exactly a.steps timed timesteps."""
     case = tempfile.mkdtemp(prefix="bb_bench_")
     write_case(a, case, precision, warm)
+    ctypes.c_int.in_dll(lib, "bb_warm_use_net").value = use_net
     tot = ctypes.c_long.in_dll(lib, "bb_pcg_iters_total")
     solves = ctypes.c_long.in_dll(lib, "bb_pcg_solves_total")
     wsolves = ctypes.c_int.in_dll(lib, "bb_warm_solves")
@@ -132,7 +135,8 @@ def run_form(lib, a, precision, warm):
     tw0 = time.perf_counter()
     sim.setup()
     nw = 0
-    while nw < a.warmup or (warm == "LEARNED" and wsolves.value < train):
+    while nw < a.warmup or (warm == "LEARNED" and use_net and
+                            wsolves.value < train):
         sim.step()
         nw += 1
         log(a, f"[{precision}/{warm}] warm-up step {nw}: solves "
@@ -267,14 +271,15 @@ def main():
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:
             ctypes.c_int.in_dll(lib, name).value = val
-    forms = {"both": [("plain", "PLAIN", "OFF"),
-                      ("mixed_learned", "MIXED", "LEARNED")],
-             "plain": [("plain", "PLAIN", "OFF")],
-             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED")],
-             "mixed": [("mixed", "MIXED", "OFF")]}[a.form]
+    forms = {"both": [("plain", "PLAIN", "OFF", 1),
+                      ("mixed_learned", "MIXED", "LEARNED", 1)],
+             "plain": [("plain", "PLAIN", "OFF", 1)],
+             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED", 1)],
+             "mixed": [("mixed", "MIXED", "OFF", 1)],
+             "mixed-last": [("mixed_last", "MIXED", "LEARNED", 0)]}[a.form]
     res = {}
-    for key, prec, warm in forms:
-        res[key] = run_form(lib, a, prec, warm)
+    for key, prec, warm, use_net in forms:
+        res[key] = run_form(lib, a, prec, warm, use_net)
     cells = a.n[0] * a.n[1] * a.n[2]
     head = res[forms[-1][0]]
     plain = res.get("plain")

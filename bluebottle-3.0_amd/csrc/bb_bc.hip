@@ -270,11 +270,13 @@ void bb_phase_fluid(void)
 }
 
 /* Particle cages are built in bb_particle.hip when particles exist. */
-void bb_build_particle_cages(void) __attribute__((weak));
+void bb_build_particle_cages(void);
+void bb_particle_face_lists(void);
 
 extern "C" void cuda_build_cages(void)
 {
   bb_flags_reset();
-  if (NPARTS > 0 && bb_build_particle_cages) bb_build_particle_cages();
+  if (NPARTS > 0) bb_build_particle_cages();
   bb_flags_external();
+  if (NPARTS > 0) bb_particle_face_lists();
 }

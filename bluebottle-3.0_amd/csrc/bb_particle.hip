@@ -440,27 +440,58 @@ void bb_build_particle_cages(void)
 /* u, v or w on faces next to a particle with flag < 1 gets the Lamb
  * velocity of the particle plus its rigid motion
  * (src/particle_kernel.cu:1370-1653).  Faces: index 1..n+1 along `dir`,
- * every row (ghosts included) across it, as the reference. */
+ * every row (ghosts included) across it, as the reference.
+ *
+ * Which faces qualify depends only on the cages (phase and flag), so the
+ * cage build lists them once per direction (face_list_kernel) and every BC
+ * call runs one thread per listed face: full wavefronts instead of the few
+ * lanes per wavefront a sweep over all faces would keep busy.  Each face's
+ * value depends on that face alone, so the list order is free. */
 template <int dir>
-__global__ void part_bc_kernel(PDom P, double *vel,
-  const int *__restrict__ phase, const int *__restrict__ flag,
-  const part_struct *__restrict__ parts, double nu)
+__device__ inline bool bc_face(const PDom &P, long t, const int *phase,
+  const int *flag, int *i_, int *j_, int *k_, int *np)
+{
+  const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
+  int i = (int) (t % G.sy), j = (int) ((t / G.sy) % (G.sz / G.sy)),
+      k = (int) (t / G.sz);
+  if (dir == 0 && (i < 1 || i > P.nx + 1 || j > P.ny + 1 || k > P.nz + 1)) return false;
+  if (dir == 1 && (j < 1 || j > P.ny + 1 || i > P.nx + 1 || k > P.nz + 1)) return false;
+  if (dir == 2 && (k < 1 || k > P.nz + 1 || i > P.nx + 1 || j > P.ny + 1)) return false;
+  long c1 = gidx(P.cc, i, j, k);
+  long c0 = c1 - (dir == 0 ? 1 : (dir == 1 ? P.cc.sy : P.cc.sz));
+  int p0 = phase[c0], p1 = phase[c1];
+  if (!(flag[gidx(G, i, j, k)] < 1 && (p0 > -1 || p1 > -1))) return false;
+  *i_ = i; *j_ = j; *k_ = k;
+  *np = p0 > -1 ? p0 : p1;
+  return true;
+}
+
+template <int dir>
+__global__ void face_list_kernel(PDom P, const int *__restrict__ phase,
+  const int *__restrict__ flag, int *__restrict__ count, int *__restrict__ list)
 {
   const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
   long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.n) return;
-  int i = (int) (t % G.sy), j = (int) ((t / G.sy) % (G.sz / G.sy)),
-      k = (int) (t / G.sz);
-  if (dir == 0 && (i < 1 || i > P.nx + 1 || j > P.ny + 1 || k > P.nz + 1)) return;
-  if (dir == 1 && (j < 1 || j > P.ny + 1 || i > P.nx + 1 || k > P.nz + 1)) return;
-  if (dir == 2 && (k < 1 || k > P.nz + 1 || i > P.nx + 1 || j > P.ny + 1)) return;
-  long c1 = gidx(P.cc, i, j, k);
-  long c0 = c1 - (dir == 0 ? 1 : (dir == 1 ? P.cc.sy : P.cc.sz));
-  int p0 = phase[c0], p1 = phase[c1];
-  long C = gidx(G, i, j, k);
-  if (!(flag[C] < 1 && (p0 > -1 || p1 > -1))) return;
-  int np = p0 > -1 ? p0 : p1;
+  int i, j, k, np;
+  if (!bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np)) return;
+  list[atomicAdd(&count[dir], 1)] = (int) t;
+}
+
+template <int dir>
+__global__ void part_bc_kernel(PDom P, double *vel,
+  const int *__restrict__ phase, const int *__restrict__ flag,
+  const part_struct *__restrict__ parts, double nu,
+  const int *__restrict__ list, int nlist)
+{
+  const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
+  long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlist) return;
+  long t = list[e];
+  int i, j, k, np;
+  if (!bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np)) return;
   const part_struct *Q = &parts[np];
+  long C = gidx(G, i, j, k);
   /* Position of the face. */
   double x = dir == 0 ? (i - DOM_BUF) * P.dx + P.xs : (i - 0.5) * P.dx + P.xs;
   double y = dir == 1 ? (j - DOM_BUF) * P.dy + P.ys : (j - 0.5) * P.dy + P.ys;
@@ -496,15 +527,58 @@ __global__ void part_bc_kernel(PDom P, double *vel,
   vel[C] = val;
 }
 
+/* Face lists of the current cages (device), their lengths (host). */
+static int *_bc_list[3] = {NULL, NULL, NULL};
+static int *_bc_count = NULL;
+static int bc_n[3] = {0, 0, 0};
+
+/* Called by cuda_build_cages (bb_bc.hip) once every flag is final. */
+void bb_particle_face_lists(void)
+{
+  PDom P = pdom();
+  const long gn[3] = {ddom.fx.n, ddom.fy.n, ddom.fz.n};
+  if (!_bc_count) {
+    BB_CHECK(hipMalloc((void **) &_bc_count, 3 * sizeof(int)));
+    for (int d = 0; d < 3; d++)
+      BB_CHECK(hipMalloc((void **) &_bc_list[d], gn[d] * sizeof(int)));
+  }
+  BB_CHECK(hipMemsetAsync(_bc_count, 0, 3 * sizeof(int), bb_stream));
+  hipLaunchKernelGGL(face_list_kernel<0>, dim3(nblk(gn[0], 256)), dim3(256),
+    0, bb_stream, P, _phase, _flag_u, _bc_count, _bc_list[0]);
+  hipLaunchKernelGGL(face_list_kernel<1>, dim3(nblk(gn[1], 256)), dim3(256),
+    0, bb_stream, P, _phase, _flag_v, _bc_count, _bc_list[1]);
+  hipLaunchKernelGGL(face_list_kernel<2>, dim3(nblk(gn[2], 256)), dim3(256),
+    0, bb_stream, P, _phase, _flag_w, _bc_count, _bc_list[2]);
+  BB_LAUNCH_CHECK();
+  BB_CHECK(hipMemcpyAsync(bc_n, _bc_count, 3 * sizeof(int),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+}
+
+static void free_face_lists(void)
+{
+  if (!_bc_count) return;
+  for (int d = 0; d < 3; d++) {
+    BB_CHECK(hipFree(_bc_list[d]));
+    _bc_list[d] = NULL;
+    bc_n[d] = 0;
+  }
+  BB_CHECK(hipFree(_bc_count));
+  _bc_count = NULL;
+}
+
 static void part_bc(double *u, double *v, double *w)
 {
   PDom P = pdom();
-  hipLaunchKernelGGL(part_bc_kernel<0>, dim3(nblk(ddom.fx.n, 256)), dim3(256), 0,
-    bb_stream, P, u, _phase, _flag_u, _parts, nu);
-  hipLaunchKernelGGL(part_bc_kernel<1>, dim3(nblk(ddom.fy.n, 256)), dim3(256), 0,
-    bb_stream, P, v, _phase, _flag_v, _parts, nu);
-  hipLaunchKernelGGL(part_bc_kernel<2>, dim3(nblk(ddom.fz.n, 256)), dim3(256), 0,
-    bb_stream, P, w, _phase, _flag_w, _parts, nu);
+  if (bc_n[0] > 0)
+    hipLaunchKernelGGL(part_bc_kernel<0>, dim3(nblk(bc_n[0], 256)), dim3(256),
+      0, bb_stream, P, u, _phase, _flag_u, _parts, nu, _bc_list[0], bc_n[0]);
+  if (bc_n[1] > 0)
+    hipLaunchKernelGGL(part_bc_kernel<1>, dim3(nblk(bc_n[1], 256)), dim3(256),
+      0, bb_stream, P, v, _phase, _flag_v, _parts, nu, _bc_list[1], bc_n[1]);
+  if (bc_n[2] > 0)
+    hipLaunchKernelGGL(part_bc_kernel<2>, dim3(nblk(bc_n[2], 256)), dim3(256),
+      0, bb_stream, P, w, _phase, _flag_w, _parts, nu, _bc_list[2], bc_n[2]);
   BB_LAUNCH_CHECK();
 }
 
@@ -1078,12 +1152,17 @@ __global__ void bin_kernel(int nparts, const part_struct *__restrict__ parts,
  * (z, then y, then x offsets -1..1, as the reference) and, inside a bin,
  * in particle order; a bin across a periodic face holds the neighbours'
  * images (positions x +- L). */
-__global__ void collision_parts_kernel(int nparts, part_struct *parts,
+/* One wavefront per particle i: the lanes test 64 candidates j at a time
+ * for membership of the target bin, and lane 0 applies the matches in
+ * increasing j, so the forces add up in the order of a scan over j. */
+__global__ void __launch_bounds__(WAVE)
+collision_parts_kernel(int nparts, part_struct *parts,
   const int *__restrict__ pbin, bin_struct B, double eps, double mu,
   double rhof, double nu, int interaction_length_ratio, double dt,
   double xl, double yl, double zl, int perx, int pery, int perz)
 {
-  int i = threadIdx.x + blockIdx.x * blockDim.x;
+  int i = blockIdx.x;
+  int lane = threadIdx.x;
   if (i >= nparts) return;
   const bin_grid_info &G = B.Gcc;
   int bin = pbin[i];
@@ -1113,8 +1192,14 @@ __global__ void collision_parts_kernel(int nparts, part_struct *parts,
     }
     if (!ok) continue;
     int tbin = tb[0] + tb[1] * G.s1b + tb[2] * G.s2b;
-    for (int j = 0; j < nparts; j++) {
-      if (pbin[j] != tbin || j == i) continue;
+    for (int j0 = 0; j0 < nparts; j0 += WAVE) {
+      int jl = j0 + lane;
+      unsigned long long hits =
+        __ballot(jl < nparts && pbin[jl] == tbin && jl != i);
+      while (hits) {
+      int j = j0 + __ffsll((unsigned long long) hits) - 1;
+      hits &= hits - 1;
+      if (lane != 0) continue;
       const part_struct *Pj = &parts[j];
       double xj = img(Pj->x, sh[0], xl), yj = img(Pj->y, sh[1], yl),
              zj = img(Pj->z, sh[2], zl);
@@ -1268,6 +1353,7 @@ __global__ void collision_parts_kernel(int nparts, part_struct *parts,
       Pi->iLx += Lox;
       Pi->iLy += Loy;
       Pi->iLz += Loz;
+      }
     }
   }
 }
@@ -1393,7 +1479,7 @@ static void collide(void)
   if (nparts > 1) {
     hipLaunchKernelGGL(bin_kernel, pgrid(), dim3(256), 0, bb_stream, nparts,
       _parts, bins, _pbin);
-    hipLaunchKernelGGL(collision_parts_kernel, pgrid(), dim3(256), 0,
+    hipLaunchKernelGGL(collision_parts_kernel, dim3(nparts), dim3(WAVE), 0,
       bb_stream, nparts, _parts, _pbin, bins, 0.01, mu, rho_f, nu,
       (int) interaction_length_ratio, dt, DOM.xl, DOM.yl, DOM.zl, ddom.perx,
       ddom.pery, ddom.perz);
@@ -1471,6 +1557,7 @@ extern "C" void cuda_part_free(void)
   if (err_host) BB_CHECK(hipHostFree(err_host));
   if (_bb_fluid) BB_CHECK(hipFree(_bb_fluid));
   _bb_fluid = NULL;
+  free_face_lists();
   _parts = NULL; _perr = NULL; _pnum = NULL; _pbin = NULL; _err_out = NULL;
   err_host = NULL;
 }

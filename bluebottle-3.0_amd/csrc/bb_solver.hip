@@ -458,6 +458,99 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   tile_partial(acc, gpart, tile);
 }
 
+/* The same pass over a column of `tzb` tiles stacked in z: one workgroup
+ * streams all their planes through the LDS ring, so the pipeline start
+ * (ring fill, the reduction that yields beta) and the two halo planes are
+ * paid once per column instead of once per tile, and each tile's partial
+ * is written exactly as above (same per-column z order, same tree).  The
+ * ring of raw planes rotates through named registers: Ra holds plane
+ * k + 1 (stored into LDS at the top of step k), Rb plane k + 2. */
+template <bool FIRST, typename V, bool MIXED>
+__global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
+  int q, const double *__restrict__ rpart, long ntiles, int tzb,
+  const V *__restrict__ r, const V *__restrict__ pold,
+  const uint8_t *__restrict__ fluid,
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
+{
+  __shared__ V lds[2 * LPLANE];
+  if (st->done | st->inner_done) return;
+  const Grid &C = D.cc;
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
+  const int ngz = (ntz_ + tzb - 1) / tzb;
+  long g = tile_of_block((long) ntx_ * nty_ * ngz);
+  int lane = threadIdx.x & (WAVE - 1);
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
+  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
+  int xb = tx * TILE_X, jb = ty * TILE_Y;
+  int k0 = tz0 * TILE_Z + 1;
+  int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
+  int x = xb + lane + 1, j = jb + wv + 1;
+  bool colok = (x <= D.nx) && (j <= D.ny);
+  PlaneEntry E0 = plane_entry(D, xb, jb, threadIdx.x);
+  PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
+  constexpr bool TWO = true;                /* two planes in flight */
+  PlaneRaw<V> Rp = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 - 1);
+  PlaneRaw<V> Rc = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0);
+  PlaneRaw<V> Ra = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 1);
+  PlaneRaw<V> Rb;
+  Rb.r0 = Rb.p0 = Rb.r1 = Rb.p1 = 0;
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
+  if (TWO && k0 + 2 <= kend + 1)
+    Rb = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 2);
+  V beta = (V) beta64;
+  int xl = lane + 1, yl = wv + 1;
+  bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+
+  plane_store<FIRST, V>(D, T, E0, E1, Rp, beta,
+                        lds + ((k0 - 1) & 1) * LPLANE, k0 - 1);
+  plane_store<FIRST, V>(D, T, E0, E1, Rc, beta, lds + (k0 & 1) * LPLANE, k0);
+  __syncthreads();
+  V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
+  V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
+  double acc = 0.0;
+  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
+  const long tstride = (long) ntx_ * nty_;
+  for (int k = k0; k <= kend; k++) {
+    __syncthreads();
+    plane_store<FIRST, V>(D, T, E0, E1, Ra, beta,
+                          lds + ((k + 1) & 1) * LPLANE, k + 1);
+    if (TWO) {
+      Ra = Rb;
+      if (k + 3 <= kend + 1)
+        Rb = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + 3);
+    } else if (k + 2 <= kend + 1) {
+      Ra = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + 2);
+    }
+    __syncthreads();
+    const V *P = lds + (k & 1) * LPLANE;
+    V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
+    if (colok) {
+      V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
+                        P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
+                        nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
+                        wall_hi(k, D.nz, D.perz));
+      long c = gidx(C, x, j, k);
+      pnew[c] = cur;
+      Ap[c] = a;
+      acc += (double) cur * (double) a;
+    }
+    prev = cur;
+    cur = nxt;
+    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
+      tile_partial(acc, gpart, tile);   /* uniform over the workgroup */
+      acc = 0.0;
+      tile += tstride;
+    }
+  }
+}
+
 /* The same pass with every neighbour read from global memory instead of
  * LDS (a second implementation, used to cross-check the tiled one). */
 template <bool FIRST>
@@ -541,6 +634,72 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
     }
   }
   tile_partial(acc, gpart, tile);
+}
+
+/* The same update over a column of `tzb` tiles stacked in z (the grouping
+ * of search_spmv_group_kernel): alpha's reduction once per column, each
+ * tile's planes of p and Ap loaded together, each tile's partial written
+ * as above. */
+template <typename V>
+__global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+update_group_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
+  const double *__restrict__ ppart, long ntiles, int tzb,
+  const V *__restrict__ p, const V *__restrict__ Ap,
+  const uint8_t *__restrict__ fluid, V *__restrict__ phi,
+  V *__restrict__ r, double *__restrict__ gpart)
+{
+  if (st->done | st->inner_done) return;
+  const Grid &C = D.cc;
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
+  const int ngz = (ntz_ + tzb - 1) / tzb;
+  long g = tile_of_block((long) ntx_ * nty_ * ngz);
+  int lane = threadIdx.x & (WAVE - 1);
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
+  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
+  int x = tx * TILE_X + lane + 1, j = ty * TILE_Y + wv + 1;
+  bool colok = (x <= D.nx) && (j <= D.ny);
+  int tz1 = tz0 + tzb < ntz_ ? tz0 + tzb : ntz_;
+  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
+  const long tstride = (long) ntx_ * nty_;
+  /* the reduction runs once per column (the single-tile kernel overlaps
+   * it with its loads; here it is amortised over the column instead) */
+  V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
+  for (int tz = tz0; tz < tz1; tz++, tile += tstride) {
+    int k0 = tz * TILE_Z + 1;
+    int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;
+    /* 32-bit element offsets: one vector register per address */
+    unsigned c0 = (unsigned) gidx(C, x, j, k0), sz = (unsigned) C.sz;
+    V pv[TILE_Z], apv[TILE_Z];
+#pragma unroll
+    for (int kk = 0; kk < TILE_Z; kk++) {
+      pv[kk] = apv[kk] = 0;
+      if (colok && k0 + kk <= k1) {
+        pv[kk] = ld32(p, c0 + kk * sz);
+        apv[kk] = ld32(Ap, c0 + kk * sz);
+      }
+    }
+    double acc = 0.0;
+    if (colok) {
+#pragma unroll
+      for (int kk = 0; kk < TILE_Z; kk++) {
+        if (k0 + kk <= k1) {
+          unsigned c = c0 + kk * sz;
+          V *ph = (V *) ((char *) phi + c * (unsigned) sizeof(V));
+          V *rp = (V *) ((char *) r + c * (unsigned) sizeof(V));
+          *ph += alpha*pv[kk];
+          V rr = *rp - alpha*apv[kk];
+          *rp = rr;
+          V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
+          acc += (double) rr * (double) z;
+        }
+      }
+    }
+    tile_partial(acc, gpart, tile);
+  }
 }
 
 /* Every 50th iteration (src/cuda_solver.cu:208-224): phi += alpha p ... */
@@ -680,6 +839,20 @@ true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
   tile_partial(acc, gpart, tile);
 }
 
+/* out = b - (-A) x in fluid cells, 0 in particle cells (interior). */
+__global__ void __launch_bounds__(TILE_THREADS)
+defect_kernel(DevDom D, Coef K, const double *__restrict__ x_in,
+  const double *__restrict__ rhs, const uint8_t *__restrict__ fluid,
+  double *__restrict__ out)
+{
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
+    out[c] = solid(fluid, c) ? 0. : rhs[c] - neg_lap_at(D, K, x_in, x, j, k);
+  }
+}
+
 /* Stopping test after the last enqueued update of a batch (one workgroup;
  * the next search_spmv repeats it on the same bits). */
 template <bool MIXED>
@@ -803,7 +976,10 @@ static double *_part_r = NULL;         /* tile partials of (r, z) */
 static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
-int bb_spmv_variant = 1;               /* 1: LDS tile kernel, 0: direct loads */
+int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
+                                          tiles, 1: one tile per workgroup,
+                                          0: direct loads */
+int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
@@ -945,6 +1121,33 @@ int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
   return 1;
 }
 
+/* Tiles per workgroup of the z-column search kernel: as many as keep at
+ * least four workgroups per CU (1024 on MI355X), at most 4. */
+static int spmv_tzb(void)
+{
+  if (bb_spmv_tzb > 0) return bb_spmv_tzb;
+  long nt = n_tiles(ddom);
+  long t = nt / 1024;
+  return t < 1 ? 1 : (t > 4 ? 4 : (int) t);
+}
+
+static dim3 spmv_group_grid(int tzb)
+{
+  long ntx = (ddom.nx + TILE_X - 1) / TILE_X;
+  long nty = (ddom.ny + TILE_Y - 1) / TILE_Y;
+  long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
+  return dim3((unsigned) (ntx * nty * ((ntz + tzb - 1) / tzb)));
+}
+
+/* The defect b - (-A) x of the current right-hand side (ghosts of x must
+ * be current). */
+void bb_pp_defect(const double *x, double *out)
+{
+  hipLaunchKernelGGL(defect_kernel, TILES_LAUNCH, ddom, coef(), x, _rhs_p,
+    fluid_mask(), out);
+  BB_LAUNCH_CHECK();
+}
+
 /* Enqueue iteration q; p buffers alternate by parity. */
 static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 {
@@ -952,7 +1155,19 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   double *pnew = (q & 1) ? _p_q : _p_alt;
   double *pold = (q & 1) ? _p_alt : _p_q;
   bb_kt_begin(KT_SPMV);
-  if (bb_spmv_variant == 1) {
+  if (bb_spmv_variant == 2) {
+    int tzb = spmv_tzb();
+    if (q == 1)
+      hipLaunchKernelGGL((search_spmv_group_kernel<true, double, false>),
+        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+        _cg_state, q, _part_r, nt, tzb, _r_q, pold, fluid_mask(), pnew,
+        _Apb_q, _part_p);
+    else
+      hipLaunchKernelGGL((search_spmv_group_kernel<false, double, false>),
+        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+        _cg_state, q, _part_r, nt, tzb, _r_q, pold, fluid_mask(), pnew,
+        _Apb_q, _part_p);
+  } else if (bb_spmv_variant == 1) {
     if (q == 1)
       hipLaunchKernelGGL((search_spmv_kernel<true, double, false>),
         TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
@@ -977,9 +1192,16 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
       _phi, _rhs_p, fluid_mask(), _r_q, _part_r);
   } else {
     bb_kt_begin(KT_UPDATE);
-    hipLaunchKernelGGL(update_kernel<double>, TILES_LAUNCH, ddom, T,
-      _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi, _r_q,
-      _part_r);
+    int tzb = spmv_tzb();
+    if (bb_spmv_variant == 2 && tzb > 1) {
+      hipLaunchKernelGGL(update_group_kernel<double>, spmv_group_grid(tzb),
+        tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
+        pnew, _Apb_q, fluid_mask(), _phi, _r_q, _part_r);
+    } else {
+      hipLaunchKernelGGL(update_kernel<double>, TILES_LAUNCH, ddom, T,
+        _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi, _r_q,
+        _part_r);
+    }
     bb_kt_end(KT_UPDATE);
   }
   BB_LAUNCH_CHECK();
@@ -1088,20 +1310,38 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   float *pnew = (q & 1) ? _p32a : _p32b;
   float *pold = (q & 1) ? _p32b : _p32a;
   bb_kt_begin(KT_SPMV32);
-  if (first)
+  if (bb_spmv_variant == 2) {
+    int tzb = spmv_tzb();
+    if (first)
+      hipLaunchKernelGGL((search_spmv_group_kernel<true, float, true>),
+        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+        _cg_state, q, _part_r, nt, tzb, _r32, pold, fluid_mask(), pnew,
+        _Ap32, _part_p);
+    else
+      hipLaunchKernelGGL((search_spmv_group_kernel<false, float, true>),
+        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+        _cg_state, q, _part_r, nt, tzb, _r32, pold, fluid_mask(), pnew,
+        _Ap32, _part_p);
+  } else if (first) {
     hipLaunchKernelGGL((search_spmv_kernel<true, float, true>), TILES_LAUNCH,
       ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
-      _Ap32,
-      _part_p);
-  else
+      _Ap32, _part_p);
+  } else {
     hipLaunchKernelGGL((search_spmv_kernel<false, float, true>), TILES_LAUNCH,
       ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
-      _Ap32,
-      _part_p);
+      _Ap32, _part_p);
+  }
   bb_kt_end(KT_SPMV32);
   bb_kt_begin(KT_UPDATE32);
-  hipLaunchKernelGGL(update_kernel<float>, TILES_LAUNCH, ddom, T, _cg_state,
-    q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
+  int tzb = spmv_tzb();
+  if (bb_spmv_variant == 2 && tzb > 1) {
+    hipLaunchKernelGGL(update_group_kernel<float>, spmv_group_grid(tzb),
+      tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
+      pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
+  } else {
+    hipLaunchKernelGGL(update_kernel<float>, TILES_LAUNCH, ddom, T, _cg_state,
+      q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
+  }
   bb_kt_end(KT_UPDATE32);
   BB_LAUNCH_CHECK();
 }

@@ -90,48 +90,173 @@ __device__ inline bf16x4 load4(const TV &T, int x, int y, int z, int c,
 /* ------------------------------- convolution ----------------------------- */
 /* Y[v, co] = act(sum_tap sum_ci X[v + off(tap), ci] W[tap][co][ci] + b[co]).
  * W is BF16 [taps][Cout_pad][Cin_pad]; Cin_pad, Cout_pad multiples of 16.
- * Output: FP32 (yf) and/or BF16 (yb) views; relu applies max(0, .).
- * Grid: (ceil(nvox / 64), Cout_pad / 16); 256 threads (4 wavefronts). */
-template <typename IN>
+ *
+ * MFMA operands: A = W (m = output channel, k = input channel), B = X
+ * (k = input channel, n = voxel), D = Y (m = output channel, n = voxel).
+ * Fragment layouts of v_mfma_f32_16x16x{16,32}_bf16 (KPL = K/4 elements
+ * per lane): A[m][k] in lane m + 16*(k/KPL), element k%KPL; B[k][n] in lane
+ * n + 16*(k/KPL); D[m][n] in lane n + 16*(m/4), element m%4.  A lane thus
+ * loads KPL consecutive channels of one voxel (one 8- or 16-byte load of
+ * the channels-last row) and KPL consecutive input channels of one weight
+ * row, and holds four consecutive output channels of one voxel.
+ *
+ * One wavefront: NT tiles of 16 consecutive voxels x NCO tiles of 16
+ * output channels (each B fragment feeds NCO MFMAs, each A fragment NT).
+ * Block: 4 wavefronts; grid (ceil(nvox / (64 NT)), Cout_pad / (16 NCO)).
+ * `vec`: rows aligned for vector loads of KPL channels (else per element). */
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int KS> struct Frag;
+template <> struct Frag<16> { typedef bf16x4 T; };
+template <> struct Frag<32> { typedef bf16x8 T; };
+
+template <int KS>
+__device__ inline f32x4 mfma_bf16(typename Frag<KS>::T a,
+  typename Frag<KS>::T b, f32x4 c);
+template <> __device__ inline f32x4 mfma_bf16<16>(bf16x4 a, bf16x4 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+template <> __device__ inline f32x4 mfma_bf16<32>(bf16x8 a, bf16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+template <int KPL> struct U16V;
+template <> struct U16V<4> { typedef u16x4 T; };
+template <> struct U16V<8> { typedef u16x8 T; };
+
+/* KPL channels c .. c+KPL-1 (zero at c >= cmax) of row `row` of T as BF16. */
+template <typename IN, int KS>
+__device__ inline typename Frag<KS>::T load_row(const TV &T, long row, int c,
+  int cmax, bool vec)
+{
+  constexpr int KPL = KS / 4;
+  typedef typename Frag<KS>::T F;
+  typedef typename U16V<KPL>::T U;
+  const IN *p = (const IN *) T.p + row * T.ld + T.c0 + c;
+  U h;
+  if (vec && c + KPL <= cmax) {
+    if (sizeof(IN) == 2) {
+      h = *(const U *) p;
+    } else {
+#pragma unroll
+      for (int e4 = 0; e4 < KPL / 4; e4++) {
+        float4 f = ((const float4 *) p)[e4];
+        h[4 * e4] = f2bf(f.x); h[4 * e4 + 1] = f2bf(f.y);
+        h[4 * e4 + 2] = f2bf(f.z); h[4 * e4 + 3] = f2bf(f.w);
+      }
+    }
+    return __builtin_bit_cast(F, h);
+  }
+#pragma unroll
+  for (int e = 0; e < KPL; e++) {
+    h[e] = 0;
+    if (c + e < cmax)
+      h[e] = sizeof(IN) == 2 ? ((const uint16_t *) p)[e]
+                             : f2bf(((const float *) p)[e]);
+  }
+  return __builtin_bit_cast(F, h);
+}
+
+template <int KS>
+__device__ inline typename Frag<KS>::T load_w(const uint16_t *__restrict__ w)
+{
+  typedef typename U16V<KS / 4>::T U;
+  return __builtin_bit_cast(typename Frag<KS>::T, *(const U *) w);
+}
+
+template <typename IN, int KS, int NCO>
 __global__ void __launch_bounds__(256)
 conv_fwd_kernel(TV X, int cin, int cin_pad, const uint16_t *__restrict__ W,
   int taps, const float *__restrict__ bias, int cout, int cout_pad,
-  TV Yf, TV Yb, int relu)
+  TV Yf, TV Yb, int relu, int vec)
 {
+  constexpr int NT = 2, KPL = KS / 4;
+  typedef typename Frag<KS>::T F;
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  long v0 = ((long) blockIdx.x * 4 + wv) * 16;
-  int co0 = blockIdx.y * 16;
-  /* A-fragment voxel of this lane and its coordinates */
-  long va = v0 + (lane & 15);
-  int ax, ay, az;
-  bool aok = vox_xyz(X, va, &ax, &ay, &az);
-  int kq = 4 * (lane >> 4);
-  int n = lane & 15;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  long vbase = ((long) blockIdx.x * 4 + wv) * (16 * NT);
+  int co0 = blockIdx.y * 16 * NCO;
+  int q = lane >> 4, n = lane & 15;
+  long nvox = (long) X.nx * X.ny * X.nz;
+  /* this lane's voxel of each voxel tile (B operand and D column) */
+  int vx[NT], vy[NT], vz[NT];
+  bool vok[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+    vok[t] = vox_xyz(X, vbase + 16 * t + n, &vx[t], &vy[t], &vz[t]);
+  f32x4 acc[NT][NCO];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int c = 0; c < NCO; c++) acc[t][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
   for (int tap = 0; tap < taps; tap++) {
     int dx = 0, dy = 0, dz = 0;
     if (taps == 27) { dx = tap % 3 - 1; dy = (tap / 3) % 3 - 1; dz = tap / 9 - 1; }
-    const uint16_t *Wt = W + ((long) tap * cout_pad + co0 + n) * cin_pad;
-    for (int ch = 0; ch < cin_pad; ch += 16) {
-      bf16x4 a = {0, 0, 0, 0};
-      if (aok) a = load4<IN>(X, ax + dx, ay + dy, az + dz, ch + kq, cin);
-      bf16x4 b;
-      for (int e = 0; e < 4; e++) b[e] = (short) Wt[ch + kq + e];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    long row[NT];
+    bool rok[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      int xs = vx[t] + dx, ys = vy[t] + dy, zs = vz[t] + dz;
+      rok[t] = vok[t] && xs >= 0 && ys >= 0 && zs >= 0 && xs < X.nx &&
+               ys < X.ny && zs < X.nz;
+      row[t] = rok[t] ? (long) xs + (long) X.nx * (ys + (long) X.ny * zs) : 0;
+    }
+    const uint16_t *Wt = W + ((long) tap * cout_pad + co0 + n) * cin_pad + KPL * q;
+    for (int ch = 0; ch < cin_pad; ch += KS) {
+      F b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        b[t] = __builtin_bit_cast(F, (typename U16V<KPL>::T) 0);
+        if (rok[t]) b[t] = load_row<IN, KS>(X, row[t], ch + KPL * q, cin, vec);
+      }
+#pragma unroll
+      for (int c = 0; c < NCO; c++) {
+        F a = load_w<KS>(Wt + (long) c * 16 * cin_pad + ch);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          acc[t][c] = mfma_bf16<KS>(a, b[t], acc[t][c]);
+      }
     }
   }
-  /* D[m][n]: lane holds m = 4*(lane/16) + e, n = lane % 16 */
-  int co = co0 + n;
-  if (co >= cout) return;
-  float bb = bias ? bias[co] : 0.f;
-  for (int e = 0; e < 4; e++) {
-    long v = v0 + 4 * (lane >> 4) + e;
-    int x, y, z;
-    if (!vox_xyz(Yf.p ? Yf : Yb, v, &x, &y, &z)) continue;
-    float val = acc[e] + bb;
-    if (relu) val = val > 0.f ? val : 0.f;
-    if (Yf.p) ((float *) Yf.p)[v * Yf.ld + Yf.c0 + co] = val;
-    if (Yb.p) ((uint16_t *) Yb.p)[v * Yb.ld + Yb.c0 + co] = f2bf(val);
+  (void) nvox;
+  /* D[m][n]: lane holds output channels co0 + 16 c + 4 q + e of voxel n */
+  const TV &O = Yf.p ? Yf : Yb;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    if (!vok[t]) continue;
+    long v = vbase + 16 * t + n;
+#pragma unroll
+    for (int c = 0; c < NCO; c++) {
+      int cb = co0 + 16 * c + 4 * q;
+      if (cb >= cout) continue;
+      float val[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        float bb = (bias && cb + e < cout) ? bias[cb + e] : 0.f;
+        val[e] = acc[t][c][e] + bb;
+        if (relu) val[e] = val[e] > 0.f ? val[e] : 0.f;
+      }
+      bool full = cb + 4 <= cout && (O.ld & 3) == 0 && (O.c0 & 3) == 0;
+      if (Yf.p) {
+        float *o = (float *) Yf.p + v * Yf.ld + Yf.c0 + cb;
+        if (full) *(float4 *) o = make_float4(val[0], val[1], val[2], val[3]);
+        else for (int e = 0; e < 4; e++) if (cb + e < cout) o[e] = val[e];
+      }
+      if (Yb.p) {
+        uint16_t *o = (uint16_t *) Yb.p + v * Yb.ld + Yb.c0 + cb;
+        if (full) {
+          uint2 u;
+          u.x = (uint32_t) f2bf(val[0]) | ((uint32_t) f2bf(val[1]) << 16);
+          u.y = (uint32_t) f2bf(val[2]) | ((uint32_t) f2bf(val[3]) << 16);
+          *(uint2 *) o = u;
+        } else {
+          for (int e = 0; e < 4; e++) if (cb + e < cout) o[e] = f2bf(val[e]);
+        }
+      }
+    }
   }
 }
 
@@ -558,19 +683,45 @@ static TV none(void)
   return T;
 }
 
+/* Launch of conv_fwd_kernel: K step 32 where the input channels allow,
+ * up to four output-channel tiles per wavefront. */
+template <typename IN>
+static void conv_launch(TV X, int cin, int cinp, const uint16_t *W, int taps,
+  const float *bias, int cout, int coutp, TV Yf, TV Yb, int relu)
+{
+  int ks = cinp % 32 == 0 ? 32 : 16, kpl = ks / 4;
+  int tiles = coutp / 16;
+  int nco = tiles % 4 == 0 ? 4 : (tiles % 3 == 0 ? 3 : (tiles % 2 == 0 ? 2 : 1));
+  int vec = X.ld % kpl == 0 && X.c0 % kpl == 0;
+  long nv = (long) X.nx * X.ny * X.nz;
+  dim3 grid(blocks(nv, 128), tiles / nco);
+#define CONV_L(KS, NCO) hipLaunchKernelGGL((conv_fwd_kernel<IN, KS, NCO>), \
+    grid, dim3(256), 0, bb_stream, X, cin, cinp, W, taps, bias, cout, coutp, \
+    Yf, Yb, relu, vec)
+  if (ks == 32) {
+    if (nco == 4) CONV_L(32, 4);
+    else if (nco == 3) CONV_L(32, 3);
+    else if (nco == 2) CONV_L(32, 2);
+    else CONV_L(32, 1);
+  } else {
+    if (nco == 4) CONV_L(16, 4);
+    else if (nco == 3) CONV_L(16, 3);
+    else if (nco == 2) CONV_L(16, 2);
+    else CONV_L(16, 1);
+  }
+#undef CONV_L
+  BB_LAUNCH_CHECK();
+}
+
 static void conv(int l, TV X, bool xf32, TV Yf, TV Yb)
 {
   Layer &Y = net->L[l];
-  long nv = (long) X.nx * X.ny * X.nz;
-  dim3 grid(blocks(nv, 64), Y.coutp / 16);
   if (xf32)
-    hipLaunchKernelGGL(conv_fwd_kernel<float>, grid, dim3(256), 0, bb_stream,
-      X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout, Y.coutp, Yf, Yb, Y.relu);
+    conv_launch<float>(X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout, Y.coutp,
+                       Yf, Yb, Y.relu);
   else
-    hipLaunchKernelGGL(conv_fwd_kernel<uint16_t>, grid, dim3(256), 0,
-      bb_stream, X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout, Y.coutp, Yf, Yb,
-      Y.relu);
-  BB_LAUNCH_CHECK();
+    conv_launch<uint16_t>(X, Y.cin, Y.cinp, Y.wb, Y.taps, Y.b, Y.cout,
+                          Y.coutp, Yf, Yb, Y.relu);
 }
 
 template <typename K, typename... Args>
@@ -626,10 +777,8 @@ static void layer_bwd(int l, TV G, TV X, TV GX)
      (int) nchunks, Y.gb);
   if (GX.p) {
     ew(flip_weights_kernel, Y.nw, Y.wb, Y.taps, Y.coutp, Y.cinp, Y.wt);
-    dim3 grid(blocks(nv, 64), Y.cinp / 16);
-    hipLaunchKernelGGL(conv_fwd_kernel<float>, grid, dim3(256), 0, bb_stream,
-      G, Y.cout, Y.coutp, Y.wt, Y.taps, (const float *) NULL, Y.cin, Y.cinp,
-      GX, none(), 0);
+    conv_launch<float>(G, Y.cout, Y.coutp, Y.wt, Y.taps, (const float *) NULL,
+                       Y.cin, Y.cinp, GX, none(), 0);
   }
   BB_LAUNCH_CHECK();
 }

@@ -1,20 +1,23 @@
 /*
  * Learned initial guess for the pressure solve (pp_warmstart LEARNED).
  *
- * Each solve of -A phi = -b can start from the guess with the least A-norm
- * error in span{psi, phi_last}: psi = U-Net(b / |b|_rms, (p - mean p) /
- * |p - mean|_rms) is the network's predicted shape of the solution from the
- * right-hand side and the latest pressure p, and phi_last is the previous
- * solve's solution (the last pressure increment).  The 2 x 2 Galerkin
- * system fixes both weights, so the guess is never worse in the A-norm
- * than phi0 = 0 (the reference's start), than the last pressure alone, or
- * than the network alone.  Every solve still ends at the same test
+ * Each solve of -A phi = -b starts from the guess with the least A-norm
+ * error in span{phi_last, psi}: phi_last is the previous solve's solution,
+ * and psi = U-Net(r0 / |r0|_rms, fluid mask) is the network's prediction of
+ * the correction phi - phi_last from the defect r0 = b - (-A) phi_last of
+ * the coming solve (for a linear operator the correction is a fixed
+ * function of the defect, which is what the network learns).  The 2 x 2
+ * Galerkin system fixes both weights, so the guess is never worse in the
+ * A-norm than phi0 = 0 (the reference's start) or than the last solution
+ * alone.  Every solve still ends at the same test
  * (r, M^-1 r) <= pp_residual^2 (b, b).
  *
  * The network is trained online: the first bb_warm_train_solves solves
- * each contribute one (b, p0) -> phi pair (target phi / |phi|_rms), on
- * which bb_warm_steps Adam steps are taken over random cubic crops of
- * bb_warm_crop cells; afterwards the network only infers.
+ * each contribute one r0 -> (phi - phi_last) pair (both scaled to unit
+ * RMS), on which bb_warm_steps Adam steps are taken over random cubic
+ * crops of bb_warm_crop cells; afterwards the network only infers, by
+ * default for the first solve of each timestep (the later Lamb passes of
+ * a step start from span{phi_last}, whose defect is already small).
  *
  * No counterpart in the reference, which always starts from phi = 0
  * (src/solver_kernel.cu:258-283).
@@ -43,10 +46,15 @@ double bb_warm_loss = 0.;              /* last training loss */
 int bb_warm_solves = 0;                /* solves seen */
 
 int bb_warm_use_net = 1;               /* 0: span{phi_last} only */
+static int net_step = -1;              /* timestep of the last inference */
+int bb_warm_net_all_passes = 0;        /* 0: the network runs for the first
+                                          solve of a timestep only; the
+                                          later Lamb passes of the step use
+                                          span{phi_last} */
 int bb_warm_use_last = 1;              /* 0: span{psi} only */
 double bb_warm_beta = 0.;
 static int have_last = 0;
-static double *_psi = NULL, *_phi_last = NULL;
+static double *_psi = NULL, *_phi_last = NULL, *_wres = NULL;
 static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
 static float *_cin = NULL, *_ctgt = NULL;
 static double *_wpart = NULL, *wpart_host = NULL;
@@ -62,10 +70,11 @@ __device__ inline long cell_of_v(const DevDom &D, long v)
   return gidx(D.cc, x + 1, y + 1, z + 1);
 }
 
-/* Per-block partial sums of {b, b^2, p, p^2} over interior cells. */
+/* Per-block partial sums of {d, d^2, p, p^2} with d = b - w p over
+ * interior cells. */
 __global__ void __launch_bounds__(256)
 moments_kernel(DevDom D, const double *__restrict__ b,
-  const double *__restrict__ p, double *__restrict__ part)
+  const double *__restrict__ p, double w, double *__restrict__ part)
 {
   __shared__ double s[4][256];
   long nv = (long) D.nx * D.ny * D.nz;
@@ -73,7 +82,7 @@ moments_kernel(DevDom D, const double *__restrict__ b,
   for (long v = (long) blockIdx.x * blockDim.x + threadIdx.x; v < nv;
        v += (long) gridDim.x * blockDim.x) {
     long c = cell_of_v(D, v);
-    double x = b[c], y = p[c];
+    double y = p[c], x = b[c] - w * y;
     a[0] += x; a[1] += x * x; a[2] += y; a[3] += y * y;
   }
   for (int q = 0; q < 4; q++) s[q][threadIdx.x] = a[q];
@@ -89,10 +98,10 @@ moments_kernel(DevDom D, const double *__restrict__ b,
 
 #define MBLK 512
 
-static void moments(const double *b, const double *p, double m[4])
+static void moments(const double *b, const double *p, double w, double m[4])
 {
   hipLaunchKernelGGL(moments_kernel, dim3(MBLK), dim3(256), 0, bb_stream,
-    ddom, b, p, _wpart);
+    ddom, b, p, w, _wpart);
   BB_CHECK(hipMemcpyAsync(wpart_host, _wpart, MBLK * 4 * sizeof(double),
     hipMemcpyDeviceToHost, bb_stream));
   BB_CHECK(hipStreamSynchronize(bb_stream));
@@ -101,32 +110,32 @@ static void moments(const double *b, const double *p, double m[4])
     for (int q = 0; q < 4; q++) m[q] += wpart_host[i * 4 + q];
 }
 
-/* Network input: [b * ib, (p - pm) * ip] per interior cell; particle cells
- * carry zeros. */
-__global__ void warm_input_kernel(DevDom D, const double *__restrict__ b,
-  const double *__restrict__ p, const uint8_t *__restrict__ fluid, double ib,
-  double pm, double ip, float *__restrict__ in)
+/* Network input: [(r0 - rm) * ir, fluid] per interior cell (fluid = 1 in
+ * fluid cells, 0 in particle cells, where r0 is 0 too). */
+__global__ void warm_input_kernel(DevDom D, const double *__restrict__ r0,
+  const uint8_t *__restrict__ fluid, double rm, double ir,
+  float *__restrict__ in)
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
   long nv = (long) D.nx * D.ny * D.nz;
   if (v >= nv) return;
   long c = cell_of_v(D, v);
   bool f = fluid == NULL || fluid[c];
-  in[2 * v] = f ? (float) (b[c] * ib) : 0.f;
-  in[2 * v + 1] = f ? (float) ((p[c] - pm) * ip) : 0.f;
+  in[2 * v] = f ? (float) ((r0[c] - rm) * ir) : 0.f;
+  in[2 * v + 1] = f ? 1.f : 0.f;
 }
 
-/* Training target phi * s (interior cells). */
+/* Training target ((phi - phi_last) - m) * s (interior cells). */
 __global__ void warm_target_kernel(DevDom D, const double *__restrict__ phi,
-  const uint8_t *__restrict__ fluid, double pm, double s,
-  float *__restrict__ t)
+  const double *__restrict__ last, const uint8_t *__restrict__ fluid,
+  double m, double s, float *__restrict__ t)
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
   long nv = (long) D.nx * D.ny * D.nz;
   if (v >= nv) return;
   long c = cell_of_v(D, v);
   bool f = fluid == NULL || fluid[c];
-  t[v] = f ? (float) ((phi[c] - pm) * s) : 0.f;
+  t[v] = f ? (float) (((phi[c] - last[c]) - m) * s) : 0.f;
 }
 
 /* Random cubic crop (edge e, origin ox, oy, oz) of the input and target. */
@@ -191,7 +200,10 @@ extern "C" void bb_warm_setup(void)
   BB_CHECK(hipMemset(_psi, 0, ddom.cc.n * sizeof(double)));
   BB_CHECK(hipMalloc((void **) &_phi_last, ddom.cc.n * sizeof(double)));
   BB_CHECK(hipMemset(_phi_last, 0, ddom.cc.n * sizeof(double)));
+  BB_CHECK(hipMalloc((void **) &_wres, ddom.cc.n * sizeof(double)));
+  BB_CHECK(hipMemset(_wres, 0, ddom.cc.n * sizeof(double)));
   have_last = 0;
+  net_step = -1;
   BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
     hipHostMallocDefault));
   bb_warm_solves = 0;
@@ -206,7 +218,8 @@ extern "C" void bb_warm_teardown(void)
   BB_CHECK(hipHostFree(wpart_host));
   BB_CHECK(hipFree(_psi));
   BB_CHECK(hipFree(_phi_last));
-  _psi = _phi_last = NULL;
+  BB_CHECK(hipFree(_wres));
+  _psi = _phi_last = _wres = NULL;
   _win = _wtgt = _wout = _cin = _ctgt = NULL;
   _wpart = NULL;
   wpart_host = NULL;
@@ -220,16 +233,20 @@ static double rms_scale(double sum, double sum2, long n, double *mean)
   return var > 1e-300 ? 1. / sqrt(var) : 0.;
 }
 
-/* Network input from the current right-hand side and p0. */
+/* The defect b - (-A) x, bb_solver.hip. */
+void bb_pp_defect(const double *x, double *out);
+
+/* Network input from the defect of phi_last for the current right-hand
+ * side (phi_last = 0 before the first solve: the defect is b). */
 static void build_input(const uint8_t *fluid)
 {
+  bb_pp_defect(_phi_last, _wres);
   double m[4];
-  moments(_rhs_p, _p, m);
-  double bm, pm;
-  double ib = rms_scale(m[0], m[1], wnv, &bm);
-  double ip = rms_scale(m[2], m[3], wnv, &pm);
+  moments(_wres, _wres, 0., m);
+  double rm;
+  double ir = rms_scale(m[0], m[1], wnv, &rm);
   hipLaunchKernelGGL(warm_input_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _rhs_p, _p, fluid, ib, pm, ip, _win);
+    bb_stream, ddom, _wres, fluid, rm, ir, _win);
 }
 
 /* Galerkin guess on span{psi, q}, bb_solver.hip. */
@@ -244,7 +261,9 @@ extern "C" int cuda_PP_warmstart_infer(void)
   bb_warm_alpha = bb_warm_beta = 0.;
   if (pp_warmstart != PP_WARM_LEARNED || !_win) return 0;
   const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
-  int net_ready = bb_warm_use_net && bb_warm_solves >= bb_warm_train_solves;
+  int net_ready = bb_warm_use_net && bb_warm_solves >= bb_warm_train_solves &&
+                  (bb_warm_net_all_passes || stepnum != net_step);
+  if (net_ready) net_step = stepnum;
   int last = bb_warm_use_last && have_last;
   if (net_ready) {
     build_input(fluid);
@@ -266,26 +285,19 @@ static int crand(int n)
   return n > 0 ? (int) (crng % (unsigned long long) n) : 0;
 }
 
-/* C ABI: one training round on the solve just finished (input b, p0;
- * target phi). */
-extern "C" void cuda_PP_warmstart_train(void)
+/* Adam steps on random crops of the pair (defect of phi_last -> phi -
+ * phi_last) of the solve just finished. */
+static void train_pair(void)
 {
-  if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
-  int train = bb_warm_use_net && bb_warm_solves < bb_warm_train_solves;
-  bb_warm_solves++;
-  BB_CHECK(hipMemcpyAsync(_phi_last, _phi, ddom.cc.n * sizeof(double),
-    hipMemcpyDeviceToDevice, bb_stream));
-  have_last = 1;
-  if (!train) return;
   const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
   build_input(fluid);
   double m[4];
-  moments(_phi, _phi, m);
-  double phm;
-  double is = rms_scale(m[0], m[1], wnv, &phm);
+  moments(_phi, _phi_last, 1., m);
+  double em;
+  double is = rms_scale(m[0], m[1], wnv, &em);
   if (is == 0.) return;
   hipLaunchKernelGGL(warm_target_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _phi, fluid, phm, is, _wtgt);
+    bb_stream, ddom, _phi, _phi_last, fluid, em, is, _wtgt);
   int e = crop_edge;
   bb_unet_volume(e, e, e);
   for (int s = 0; s < bb_warm_steps; s++) {
@@ -298,4 +310,17 @@ extern "C" void cuda_PP_warmstart_train(void)
     bb_unet_adam(bb_warm_lr);
   }
   BB_LAUNCH_CHECK();
+}
+
+/* C ABI: one training round on the solve just finished (input: the defect
+ * of phi_last; target: phi - phi_last), then phi_last = phi. */
+extern "C" void cuda_PP_warmstart_train(void)
+{
+  if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
+  int train = bb_warm_use_net && bb_warm_solves < bb_warm_train_solves;
+  bb_warm_solves++;
+  if (train) train_pair();
+  BB_CHECK(hipMemcpyAsync(_phi_last, _phi, ddom.cc.n * sizeof(double),
+    hipMemcpyDeviceToDevice, bb_stream));
+  have_last = 1;
 }

@@ -30,7 +30,7 @@ FIELDS = ("u", "v", "w", "p")
 def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None):
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = restart
-    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 1
+    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 2
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = 16
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})

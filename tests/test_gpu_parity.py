@@ -20,16 +20,18 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def set_knobs(variant=1, kchunk=8, batch=16):
+def set_knobs(variant=2, tzb=0, batch=16):
+    """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
+    1: one tile per workgroup, 0: direct neighbour loads."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
-    ctypes.c_int.in_dll(lib, "bb_spmv_kchunk").value = kchunk
+    ctypes.c_int.in_dll(lib, "bb_spmv_tzb").value = tzb
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = batch
 
 
-def run_pair(case_dir, kw, nsteps, variant=1, kchunk=8, batch=16):
+def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16):
     n = kw["n"]
-    set_knobs(variant, kchunk, batch)
+    set_knobs(variant, tzb, batch)
     bbconfig.write_case(case_dir, **kw)
     out = {"gpu": {}, "orc": {}, "iters_gpu": [], "iters_orc": []}
     with orc.Oracle(case_dir) as o:
@@ -72,9 +74,9 @@ def test_cavity_32_full_100_steps(case_dir):
     assert out["iters_gpu"] == out["iters_orc"]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_cavity_bitwise(case_dir, variant):
-    out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant)
+@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 2)])
+def test_cavity_bitwise(case_dir, variant, tzb):
+    out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant, tzb)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
@@ -88,24 +90,27 @@ def test_channel_small(case_dir):
     assert out["gpu_dt"] == out["orc_dt"]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_channel_bitwise(case_dir, variant):
-    out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant)
+@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 4)])
+def test_channel_bitwise(case_dir, variant, tzb):
+    out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
+                   tzb)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,kchunk,batch", [(0, 8, 16), (1, 8, 16),
-                                                  (1, 3, 5), (1, 30, 1)])
-def test_ragged_sizes(case_dir, variant, kchunk, batch):
+@pytest.mark.parametrize("variant,tzb,batch", [(0, 0, 16), (1, 0, 16),
+                                               (1, 0, 5), (2, 2, 1),
+                                               (2, 3, 16), (2, 4, 5)])
+def test_ragged_sizes(case_dir, variant, tzb, batch):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
-    row group, z not a multiple of the plane chunk, shear walls."""
+    row group, z not a multiple of the tile depth (and the tile count in z,
+    3, not a multiple of the z-column length), shear walls."""
     kw = bbconfig.cavity(n=16, steps=3)
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     kw["vel_bc"] = bbconfig.walls(lid=("v", "E", 0.5))
-    out = run_pair(case_dir, kw, 3, variant, kchunk, batch)
+    out = run_pair(case_dir, kw, 3, variant, tzb, batch)
     check(out)
     assert out["iters_gpu"] == out["iters_orc"]
     for f in FIELDS:
@@ -123,7 +128,7 @@ def test_periodic_all_and_wide_x(case_dir):
               pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
               init_cond="QUIESCENT",
               extensions={"fixed_dt": 1e-3, "perturb": (5, 0.2)})
-    out = run_pair(case_dir, kw, 3, variant=1)
+    out = run_pair(case_dir, kw, 3, variant=2, tzb=1)
     check(out)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
@@ -138,7 +143,7 @@ def test_periodic_all_25d(case_dir):
               pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
               init_cond="QUIESCENT",
               extensions={"fixed_dt": 1e-3, "perturb": (6, 0.2)})
-    out = run_pair(case_dir, kw, 3, variant=1, kchunk=4)
+    out = run_pair(case_dir, kw, 3, variant=2, tzb=2)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
