@@ -85,6 +85,11 @@ def parse():
     ap.add_argument("--spmv-variant", type=int, default=None,
                     help="0: SpMV with direct neighbour loads, 1: LDS-staged "
                          "planes (default)")
+    ap.add_argument("--tzb", type=int, default=None,
+                    help="tiles per z-column of the search kernel "
+                         "(0: automatic)")
+    ap.add_argument("--update-group", type=int, default=None,
+                    help="1: update kernel over z-columns of tiles too")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
     ap.add_argument("--time-every", type=int, default=8,
@@ -124,6 +129,7 @@ def log(a, *msg):
 def run_form(lib, a, precision, warm, use_net=1):
     """Warm-up (and, for the learned guess, the network's training solves),
     then exactly a.steps timed timesteps."""
+    warm_mode = warm
     case = tempfile.mkdtemp(prefix="bb_bench_")
     write_case(a, case, precision, warm)
     ctypes.c_int.in_dll(lib, "bb_warm_use_net").value = use_net
@@ -173,6 +179,15 @@ def run_form(lib, a, precision, warm, use_net=1):
             ktimes[name] = (ms.value, cnt.value)
     stats = sim.pp_stats()
     nsolves = solves.value - sv0
+    wstat = None
+    if use_net and warm_mode == "LEARNED":
+        el = ctypes.c_double.in_dll(lib, "bb_warm_err_last_sum").value
+        eg = ctypes.c_double.in_dll(lib, "bb_warm_err_guess_sum").value
+        wstat = {"net_solves": ctypes.c_int.in_dll(
+                    lib, "bb_warm_net_solves").value,
+                "training_solves": train,
+                "energy_error_ratio_last_over_learned":
+                    el / eg if eg > 0 else None}
     sim.teardown()
     shutil.rmtree(case, ignore_errors=True)
     elapsed = t1 - t0
@@ -186,6 +201,7 @@ def run_form(lib, a, precision, warm, use_net=1):
         "last_true_resid": stats["true_resid"],
         "warmup_steps": nw,
         "warmup_s": tw1 - tw0,
+        "learned_guess": wstat,
         "ktimes": ktimes,
     }
 
@@ -267,6 +283,8 @@ def main():
                           "elements": 1 << 27}))
         return
     for name, val in (("bb_spmv_variant", a.spmv_variant),
+                      ("bb_spmv_tzb", a.tzb),
+                      ("bb_update_group", a.update_group),
                       ("bb_pcg_batch", a.batch),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:

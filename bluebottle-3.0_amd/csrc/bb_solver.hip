@@ -980,6 +980,7 @@ int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
                                           tiles, 1: one tile per workgroup,
                                           0: direct loads */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
+int bb_update_group = 0;               /* 1: update kernel over z-columns too */
 int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
@@ -1088,10 +1089,15 @@ double bb_true_residual(double sp_rhs)
  * (A = -Laplacian, symmetric positive semidefinite).  use_psi / use_q
  * select the directions; a singular or non-positive system drops the
  * direction at fault.  Returns 1 if phi was set (0: start from phi = 0). */
+/* Energy (b, phi0) captured by the last guess, and by q alone; with the
+ * solution phi* the A-norm errors squared are (b, phi*) minus these. */
+double bb_galerkin_e_guess = 0., bb_galerkin_e_q = 0.;
+
 int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
   int use_q, double *alpha, double *beta)
 {
   cg_buffers();
+  bb_galerkin_e_guess = bb_galerkin_e_q = 0.;
   const long nt = n_tiles(ddom);
   *alpha = *beta = 0.;
   if (!use_psi && !use_q) return 0;
@@ -1114,11 +1120,24 @@ int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
     b = bq / aqq;
   }
   if (!isfinite(a) || !isfinite(b) || (a == 0. && b == 0.)) return 0;
+  bb_galerkin_e_guess = a * bp + b * bq;
+  if (use_q && aqq > 0.) bb_galerkin_e_q = bq * bq / aqq;
   hipLaunchKernelGGL(combine_kernel, TILES_LAUNCH, ddom, a, ps, b, qq, _phi);
   BB_LAUNCH_CHECK();
   *alpha = a;
   *beta = b;
   return 1;
+}
+
+/* (b, x) over the fluid cells of the current right-hand side. */
+double bb_pp_energy(const double *x)
+{
+  cg_buffers();
+  const long nt = n_tiles(ddom);
+  hipLaunchKernelGGL(galerkin_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
+    x, x, _rhs_p, _part_w1);
+  bb_reduce_rows(_part_w1, nt, _bb_scalars + 16);
+  return bb_fetch_scalar(16);
 }
 
 /* Tiles per workgroup of the z-column search kernel: as many as keep at
@@ -1193,7 +1212,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   } else {
     bb_kt_begin(KT_UPDATE);
     int tzb = spmv_tzb();
-    if (bb_spmv_variant == 2 && tzb > 1) {
+    if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
       hipLaunchKernelGGL(update_group_kernel<double>, spmv_group_grid(tzb),
         tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
         pnew, _Apb_q, fluid_mask(), _phi, _r_q, _part_r);
@@ -1334,7 +1353,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   bb_kt_end(KT_SPMV32);
   bb_kt_begin(KT_UPDATE32);
   int tzb = spmv_tzb();
-  if (bb_spmv_variant == 2 && tzb > 1) {
+  if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
     hipLaunchKernelGGL(update_group_kernel<float>, spmv_group_grid(tzb),
       tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
       pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);

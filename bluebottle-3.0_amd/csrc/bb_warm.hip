@@ -52,6 +52,13 @@ int bb_warm_net_all_passes = 0;        /* 0: the network runs for the first
                                           later Lamb passes of the step use
                                           span{phi_last} */
 int bb_warm_use_last = 1;              /* 0: span{psi} only */
+/* What the network gained on the last solve that used it: A-norm errors
+ * squared of the guess from span{phi_last} alone and of the guess used,
+ * and their running sums over all such solves. */
+double bb_warm_err_last = 0., bb_warm_err_guess = 0.;
+double bb_warm_err_last_sum = 0., bb_warm_err_guess_sum = 0.;
+int bb_warm_net_solves = 0;
+static int net_this_solve = 0;
 double bb_warm_beta = 0.;
 static int have_last = 0;
 static double *_psi = NULL, *_phi_last = NULL, *_wres = NULL;
@@ -204,6 +211,10 @@ extern "C" void bb_warm_setup(void)
   BB_CHECK(hipMemset(_wres, 0, ddom.cc.n * sizeof(double)));
   have_last = 0;
   net_step = -1;
+  net_this_solve = 0;
+  bb_warm_err_last = bb_warm_err_guess = 0.;
+  bb_warm_err_last_sum = bb_warm_err_guess_sum = 0.;
+  bb_warm_net_solves = 0;
   BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
     hipHostMallocDefault));
   bb_warm_solves = 0;
@@ -249,9 +260,13 @@ static void build_input(const uint8_t *fluid)
     bb_stream, ddom, _wres, fluid, rm, ir, _win);
 }
 
-/* Galerkin guess on span{psi, q}, bb_solver.hip. */
+/* Galerkin guess on span{psi, q}, its captured energies and (b, x),
+ * bb_solver.hip. */
 int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
   int use_q, double *alpha, double *beta);
+extern double bb_galerkin_e_guess, bb_galerkin_e_q;
+double bb_pp_energy(const double *x);
+static double e_guess = 0., e_last = 0.;
 
 /* C ABI: the guess for the coming solve into _phi.  Returns 1 if it was
  * placed, 0 if the solve starts from phi = 0. */
@@ -275,6 +290,9 @@ extern "C" int cuda_PP_warmstart_infer(void)
   }
   bb_warm_used = bb_pp_galerkin_guess(_psi, net_ready, _phi_last, last,
     &bb_warm_alpha, &bb_warm_beta);
+  net_this_solve = net_ready && last && bb_warm_used;
+  e_guess = bb_galerkin_e_guess;
+  e_last = bb_galerkin_e_q;
   return bb_warm_used;
 }
 
@@ -319,6 +337,17 @@ extern "C" void cuda_PP_warmstart_train(void)
   if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
   int train = bb_warm_use_net && bb_warm_solves < bb_warm_train_solves;
   bb_warm_solves++;
+  if (net_this_solve) {
+    /* (b, phi*) - (b, phi0) is the A-norm error squared of a Galerkin
+     * guess phi0 against the converged phi* */
+    double es = bb_pp_energy(_phi);
+    bb_warm_err_last = es - e_last;
+    bb_warm_err_guess = es - e_guess;
+    bb_warm_err_last_sum += bb_warm_err_last;
+    bb_warm_err_guess_sum += bb_warm_err_guess;
+    bb_warm_net_solves++;
+    net_this_solve = 0;
+  }
   if (train) train_pair();
   BB_CHECK(hipMemcpyAsync(_phi_last, _phi, ddom.cc.n * sizeof(double),
     hipMemcpyDeviceToDevice, bb_stream));

@@ -20,18 +20,20 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def set_knobs(variant=2, tzb=0, batch=16):
+def set_knobs(variant=2, tzb=0, batch=16, group=0):
     """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
-    1: one tile per workgroup, 0: direct neighbour loads."""
+    1: one tile per workgroup, 0: direct neighbour loads; group = 1: the
+    update kernel over the same z-columns."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_spmv_tzb").value = tzb
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = batch
+    ctypes.c_int.in_dll(lib, "bb_update_group").value = group
 
 
-def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16):
+def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, group=0):
     n = kw["n"]
-    set_knobs(variant, tzb, batch)
+    set_knobs(variant, tzb, batch, group)
     bbconfig.write_case(case_dir, **kw)
     out = {"gpu": {}, "orc": {}, "iters_gpu": [], "iters_orc": []}
     with orc.Oracle(case_dir) as o:
@@ -90,19 +92,22 @@ def test_channel_small(case_dir):
     assert out["gpu_dt"] == out["orc_dt"]
 
 
-@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 4)])
-def test_channel_bitwise(case_dir, variant, tzb):
+@pytest.mark.parametrize("variant,tzb,group", [(0, 0, 0), (1, 0, 0),
+                                               (2, 1, 0), (2, 4, 0),
+                                               (2, 2, 1)])
+def test_channel_bitwise(case_dir, variant, tzb, group):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
-                   tzb)
+                   tzb, group=group)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,tzb,batch", [(0, 0, 16), (1, 0, 16),
-                                               (1, 0, 5), (2, 2, 1),
-                                               (2, 3, 16), (2, 4, 5)])
-def test_ragged_sizes(case_dir, variant, tzb, batch):
+@pytest.mark.parametrize("variant,tzb,batch,group",
+                         [(0, 0, 16, 0), (1, 0, 16, 0), (1, 0, 5, 0),
+                          (2, 2, 1, 0), (2, 3, 16, 1), (2, 4, 5, 0),
+                          (2, 2, 16, 1)])
+def test_ragged_sizes(case_dir, variant, tzb, batch, group):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
     3, not a multiple of the z-column length), shear walls."""
@@ -110,7 +115,7 @@ def test_ragged_sizes(case_dir, variant, tzb, batch):
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     kw["vel_bc"] = bbconfig.walls(lid=("v", "E", 0.5))
-    out = run_pair(case_dir, kw, 3, variant, tzb, batch)
+    out = run_pair(case_dir, kw, 3, variant, tzb, batch, group)
     check(out)
     assert out["iters_gpu"] == out["iters_orc"]
     for f in FIELDS:
