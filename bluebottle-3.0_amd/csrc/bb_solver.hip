@@ -464,7 +464,8 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
  * paid once per column instead of once per tile, and each tile's partial
  * is written exactly as above (same per-column z order, same tree).  The
  * ring of raw planes rotates through named registers: Ra holds plane
- * k + 1 (stored into LDS at the top of step k), Rb plane k + 2. */
+ * k + 1 (stored into LDS at the top of step k), Rb plane k + 2; the LDS
+ * planes rotate through three buffers with one barrier per plane. */
 template <bool FIRST, typename V, bool MIXED>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
@@ -474,7 +475,7 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   const uint8_t *__restrict__ fluid,
   V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
 {
-  __shared__ V lds[2 * LPLANE];
+  __shared__ V lds[3 * LPLANE];
   if (st->done | st->inner_done) return;
   const Grid &C = D.cc;
   const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
@@ -508,19 +509,20 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
   bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
 
-  plane_store<FIRST, V>(D, T, E0, E1, Rp, beta,
-                        lds + ((k0 - 1) & 1) * LPLANE, k0 - 1);
-  plane_store<FIRST, V>(D, T, E0, E1, Rc, beta, lds + (k0 & 1) * LPLANE, k0);
+  /* three LDS planes (k - 1, k, k + 1) rotate: the buffer that receives
+   * plane k + 1 held plane k - 2, whose last reads came before the barrier
+   * of step k - 1, so one barrier per plane suffices */
+  V *bprev = lds, *bcur = lds + LPLANE, *bnext = lds + 2 * LPLANE;
+  plane_store<FIRST, V>(D, T, E0, E1, Rp, beta, bprev, k0 - 1);
+  plane_store<FIRST, V>(D, T, E0, E1, Rc, beta, bcur, k0);
   __syncthreads();
-  V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
-  V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
+  V prev = bprev[yl * LX + xl];
+  V cur = bcur[yl * LX + xl];
   double acc = 0.0;
   long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
   const long tstride = (long) ntx_ * nty_;
   for (int k = k0; k <= kend; k++) {
-    __syncthreads();
-    plane_store<FIRST, V>(D, T, E0, E1, Ra, beta,
-                          lds + ((k + 1) & 1) * LPLANE, k + 1);
+    plane_store<FIRST, V>(D, T, E0, E1, Ra, beta, bnext, k + 1);
     if (TWO) {
       Ra = Rb;
       if (k + 3 <= kend + 1)
@@ -529,8 +531,8 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
       Ra = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + 2);
     }
     __syncthreads();
-    const V *P = lds + (k & 1) * LPLANE;
-    V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
+    const V *P = bcur;
+    V nxt = bnext[yl * LX + xl];
     if (colok) {
       V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
                         P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
@@ -543,6 +545,10 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
     }
     prev = cur;
     cur = nxt;
+    V *t3 = bprev;
+    bprev = bcur;
+    bcur = bnext;
+    bnext = t3;
     if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
       tile_partial(acc, gpart, tile);   /* uniform over the workgroup */
       acc = 0.0;
