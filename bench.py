@@ -90,6 +90,9 @@ def parse():
                          "(0: automatic)")
     ap.add_argument("--update-group", type=int, default=None,
                     help="1: update kernel over z-columns of tiles too")
+    ap.add_argument("--mixed-x2", type=int, default=None,
+                    help="1: FP32 kernels of the mixed form on x pairs "
+                         "(default), 0: one cell per lane")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
     ap.add_argument("--time-every", type=int, default=8,
@@ -291,6 +294,7 @@ def main():
     for name, val in (("bb_spmv_variant", a.spmv_variant),
                       ("bb_spmv_tzb", a.tzb),
                       ("bb_update_group", a.update_group),
+                      ("bb_mixed_x2", a.mixed_x2),
                       ("bb_pcg_batch", a.batch),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:

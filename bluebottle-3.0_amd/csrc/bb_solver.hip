@@ -557,6 +557,318 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   }
 }
 
+/* ------------- FP32 inner sweep: two cells per lane (x pairs) ------------- */
+/* The mixed form's FP32 vectors halve the bytes per cell, so one cell per
+ * lane would leave the FP32 kernels issue-bound.  These kernels give each
+ * lane the x pair (x, x + 1), x odd, moved as one float2 (the FP32
+ * vectors are offset so that the pair is 8-byte aligned; needs even nx).
+ * Tiles are 128 x 8 x 8 cells (64 lanes x 2, eight rows, eight planes)
+ * grouped into z-columns of tzb tiles like the FP64 kernels; each tile's
+ * partial is the per-lane sum over its planes of (c0 + c1) products,
+ * combined by the tile tree.  The LDS row holds [pad, x = xb, x = xb + 1
+ * .. xb + 128, x = xb + 129, pad], so interior pairs are 8-byte aligned;
+ * raw planes are loaded one plane ahead (Ra holds plane k + 1). */
+#define TX2 128
+#define LX2 (TX2 + 4)
+#define LPLANE2 (LX2 * LY)
+
+__device__ inline int n_tiles_x2(const DevDom &D)
+{
+  return (D.nx + TX2 - 1) / TX2;
+}
+
+/* One plane's raw values held by a thread: pair entries e = t and
+ * t + TILE_THREADS (of 64 pairs x LY rows) and halo entry t (of 2 x LY). */
+struct Plane2 {
+  float2 r0, p0, r1, p1;
+  float rh, ph;
+};
+
+/* Source cell (x) of local column xl (0: x = xb, 129: x = xb + 129) after
+ * the periodic wrap; 0 if the entry is a wall ghost or outside. */
+__device__ inline int x2_src(const DevDom &D, int xb, int xl)
+{
+  int xg = xb + xl;
+  if (xg < 1) return D.perx ? D.nx : 0;
+  if (xg > D.nx) return (D.perx && xg == D.nx + 1) ? 1 : 0;
+  return xg;
+}
+
+__device__ inline int y2_src(const DevDom &D, int jj)
+{
+  if (jj < 1) return D.pery ? D.ny : 0;
+  if (jj > D.ny) return (D.pery && jj == D.ny + 1) ? 1 : 0;
+  return jj;
+}
+
+template <bool FIRST>
+__device__ inline void pair_load(const DevDom &D, int xb, int jb, int e,
+  int ks, const float *__restrict__ r, const float *__restrict__ pold,
+  const uint8_t *__restrict__ fluid, float2 *rv, float2 *pv)
+{
+  *rv = make_float2(0.f, 0.f);
+  *pv = make_float2(0.f, 0.f);
+  int rr = e >> 6, pp = e & 63;
+  int js = y2_src(D, jb + rr);
+  if (!ks || !js || e >= 64 * LY) return;
+  int x0 = xb + 1 + 2 * pp;
+  unsigned base = (unsigned) js * (unsigned) D.cc.sy +
+                  (unsigned) ks * (unsigned) D.cc.sz;
+  if (x0 + 1 <= D.nx) {
+    unsigned c = base + (unsigned) x0;
+    *rv = *(const float2 *) (r + c);
+    if (!FIRST) *pv = *(const float2 *) (pold + c);
+    if (fluid != NULL) {
+      if (!fluid[c]) rv->x = 0.f;
+      if (!fluid[c + 1]) rv->y = 0.f;
+    }
+  } else {
+    /* past the last interior pair: only x = nx + 1 can be needed (the
+     * east neighbour of x = nx), as the wrapped cell 1 or a wall zero */
+    int xs = x2_src(D, xb, 1 + 2 * pp);
+    if (xs) {
+      unsigned c = base + (unsigned) xs;
+      rv->x = r[c];
+      if (!FIRST) pv->x = pold[c];
+      if (fluid != NULL && !fluid[c]) rv->x = 0.f;
+    }
+  }
+}
+
+template <bool FIRST>
+__device__ inline Plane2 plane2_load(const DevDom &D, int xb, int jb, int kk,
+  const float *__restrict__ r, const float *__restrict__ pold,
+  const uint8_t *__restrict__ fluid)
+{
+  Plane2 P;
+  int ks = plane_src(D, kk);
+  int t = threadIdx.x;
+  pair_load<FIRST>(D, xb, jb, t, ks, r, pold, fluid, &P.r0, &P.p0);
+  pair_load<FIRST>(D, xb, jb, t + TILE_THREADS, ks, r, pold, fluid, &P.r1,
+                   &P.p1);
+  P.rh = P.ph = 0.f;
+  if (t < 2 * LY && ks) {
+    int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
+    int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
+    if (js && xs) {
+      unsigned c = (unsigned) xs + (unsigned) js * (unsigned) D.cc.sy +
+                   (unsigned) ks * (unsigned) D.cc.sz;
+      P.rh = r[c];
+      if (!FIRST) P.ph = pold[c];
+      if (fluid != NULL && !fluid[c]) P.rh = 0.f;
+    }
+  }
+  return P;
+}
+
+/* invM of source cell (xs, js, ks); 0 for a zero entry (ks, js or xs 0) */
+__device__ inline float invm2(const InvMT<float> &T, const DevDom &D, int xs,
+  int js, int ks)
+{
+  if (!xs || !js || !ks) return 0.f;
+  return T.v[wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery) +
+             9 * wcount(ks, D.nz, D.perz)];
+}
+
+template <bool FIRST>
+__device__ inline float2 pair_value(const DevDom &D, const InvMT<float> &T,
+  int xb, int jb, int e, int ks, float2 rv, float2 pv, float beta)
+{
+  int rr = e >> 6, pp = e & 63;
+  int js = y2_src(D, jb + rr);
+  int x0 = x2_src(D, xb, 1 + 2 * pp), x1 = x2_src(D, xb, 2 + 2 * pp);
+  float z0 = rv.x * invm2(T, D, x0, js, ks);
+  float z1 = rv.y * invm2(T, D, x1, js, ks);
+  if (FIRST) return make_float2(z0, z1);
+  return make_float2(z0 + beta * pv.x, z1 + beta * pv.y);
+}
+
+template <bool FIRST>
+__device__ inline void plane2_store(const DevDom &D, const InvMT<float> &T,
+  int xb, int jb, int kk, const Plane2 &P, float beta, float *__restrict__ buf)
+{
+  int ks = plane_src(D, kk);
+  int t = threadIdx.x;
+  {
+    float2 v = pair_value<FIRST>(D, T, xb, jb, t, ks, P.r0, P.p0, beta);
+    int rr = t >> 6, pp = t & 63;
+    *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
+  }
+  int e1 = t + TILE_THREADS;
+  if (e1 < 64 * LY) {
+    float2 v = pair_value<FIRST>(D, T, xb, jb, e1, ks, P.r1, P.p1, beta);
+    int rr = e1 >> 6, pp = e1 & 63;
+    *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
+  }
+  if (t < 2 * LY) {
+    int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
+    int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
+    float z = P.rh * invm2(T, D, xs, js, ks);
+    buf[rr * LX2 + 1 + xl] = FIRST ? z : z + beta * P.ph;
+  }
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
+  int q, const double *__restrict__ rpart, long nrpart, int tzb,
+  const float *__restrict__ r, const float *__restrict__ pold,
+  const uint8_t *__restrict__ fluid,
+  float *__restrict__ pnew, float *__restrict__ Ap, double *__restrict__ gpart)
+{
+  __shared__ __attribute__((aligned(16))) float lds[3 * LPLANE2];
+  if (st->done | st->inner_done) return;
+  const Grid &C = D.cc;
+  const int ntx_ = n_tiles_x2(D);
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
+  const int ngz = (ntz_ + tzb - 1) / tzb;
+  long g = tile_of_block((long) ntx_ * nty_ * ngz);
+  int lane = threadIdx.x & (WAVE - 1);
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
+  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
+  int xb = tx * TX2, jb = ty * TILE_Y;
+  int k0 = tz0 * TILE_Z + 1;
+  int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
+  int x0 = xb + 2 * lane + 1, j = jb + wv + 1;
+  bool colok = (x0 <= D.nx) && (j <= D.ny);
+  Plane2 Rp = plane2_load<FIRST>(D, xb, jb, k0 - 1, r, pold, fluid);
+  Plane2 Rc = plane2_load<FIRST>(D, xb, jb, k0, r, pold, fluid);
+  Plane2 Ra = plane2_load<FIRST>(D, xb, jb, k0 + 1, r, pold, fluid);
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<true>(st, rpart, nrpart, q, &beta64)) return;
+  float beta = (float) beta64;
+  int yl = wv + 1, o = yl * LX2 + 2 + 2 * lane;
+  bool wW = wall_lo(x0, D.perx), wE = wall_hi(x0 + 1, D.nx, D.perx);
+  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+  float *bprev = lds, *bcur = lds + LPLANE2, *bnext = lds + 2 * LPLANE2;
+  plane2_store<FIRST>(D, T, xb, jb, k0 - 1, Rp, beta, bprev);
+  plane2_store<FIRST>(D, T, xb, jb, k0, Rc, beta, bcur);
+  __syncthreads();
+  float2 prev = *(const float2 *) &bprev[o];
+  float2 cur = *(const float2 *) &bcur[o];
+  double acc = 0.0;
+  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
+  const long tstride = (long) ntx_ * nty_;
+  for (int k = k0; k <= kend; k++) {
+    plane2_store<FIRST>(D, T, xb, jb, k + 1, Ra, beta, bnext);
+    if (k + 2 <= kend + 1)
+      Ra = plane2_load<FIRST>(D, xb, jb, k + 2, r, pold, fluid);
+    __syncthreads();
+    float2 nxt = *(const float2 *) &bnext[o];
+    if (colok) {
+      float2 S = *(const float2 *) &bcur[o - LX2];
+      float2 N = *(const float2 *) &bcur[o + LX2];
+      float Wv = bcur[o - 1], Ev = bcur[o + 2];
+      bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+      float a0 = neg_lap7<float>(K, cur.x, Wv, cur.y, S.x, N.x, prev.x, nxt.x,
+                                 wW, false, wS, wN, wB, wT);
+      float a1 = neg_lap7<float>(K, cur.y, cur.x, Ev, S.y, N.y, prev.y, nxt.y,
+                                 false, wE, wS, wN, wB, wT);
+      unsigned c = (unsigned) gidx(C, x0, j, k);
+      *(float2 *) (pnew + c) = cur;
+      *(float2 *) (Ap + c) = make_float2(a0, a1);
+      acc += (double) cur.x * (double) a0 + (double) cur.y * (double) a1;
+    }
+    prev = cur;
+    cur = nxt;
+    float *t3 = bprev;
+    bprev = bcur;
+    bcur = bnext;
+    bnext = t3;
+    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
+      tile_partial(acc, gpart, tile);   /* uniform over the workgroup */
+      acc = 0.0;
+      tile += tstride;
+    }
+  }
+}
+
+/* alpha = (r, z) / (p, Ap); e += alpha p, r -= alpha Ap, z = invM r and the
+ * tile partials of (r, z), on x pairs over z-columns of tiles. */
+__global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
+  const double *__restrict__ ppart, long nppart, int tzb,
+  const float *__restrict__ p, const float *__restrict__ Ap,
+  const uint8_t *__restrict__ fluid, float *__restrict__ e,
+  float *__restrict__ r, double *__restrict__ gpart)
+{
+  if (st->done | st->inner_done) return;
+  const Grid &C = D.cc;
+  const int ntx_ = n_tiles_x2(D);
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
+  const int ngz = (ntz_ + tzb - 1) / tzb;
+  long g = tile_of_block((long) ntx_ * nty_ * ngz);
+  int lane = threadIdx.x & (WAVE - 1);
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
+  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
+  int x0 = tx * TX2 + 2 * lane + 1, j = ty * TILE_Y + wv + 1;
+  bool colok = (x0 <= D.nx) && (j <= D.ny);
+  int tz1 = tz0 + tzb < ntz_ ? tz0 + tzb : ntz_;
+  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
+  const long tstride = (long) ntx_ * nty_;
+  /* the first tile's p and Ap are in flight during the reduction */
+  int kf = tz0 * TILE_Z + 1;
+  int kf1 = kf + TILE_Z - 1 < D.nz ? kf + TILE_Z - 1 : D.nz;
+  unsigned sz = (unsigned) C.sz;
+  float2 pv[TILE_Z], apv[TILE_Z];
+  {
+    unsigned c0 = (unsigned) gidx(C, x0, j, kf);
+#pragma unroll
+    for (int kk = 0; kk < TILE_Z; kk++) {
+      pv[kk] = apv[kk] = make_float2(0.f, 0.f);
+      if (colok && kf + kk <= kf1) {
+        pv[kk] = *(const float2 *) (p + c0 + kk * sz);
+        apv[kk] = *(const float2 *) (Ap + c0 + kk * sz);
+      }
+    }
+  }
+  float alpha = (float) (st->rzs[q & 1]/tile_block_sum(ppart, nppart));
+  for (int tz = tz0; tz < tz1; tz++, tile += tstride) {
+    int k0 = tz * TILE_Z + 1;
+    int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;
+    unsigned c0 = (unsigned) gidx(C, x0, j, k0);
+    if (tz > tz0) {
+#pragma unroll
+      for (int kk = 0; kk < TILE_Z; kk++) {
+        pv[kk] = apv[kk] = make_float2(0.f, 0.f);
+        if (colok && k0 + kk <= k1) {
+          pv[kk] = *(const float2 *) (p + c0 + kk * sz);
+          apv[kk] = *(const float2 *) (Ap + c0 + kk * sz);
+        }
+      }
+    }
+    double acc = 0.0;
+    if (colok) {
+#pragma unroll
+      for (int kk = 0; kk < TILE_Z; kk++) {
+        if (k0 + kk <= k1) {
+          unsigned c = c0 + kk * sz;
+          float2 ev = *(const float2 *) (e + c);
+          float2 rv = *(const float2 *) (r + c);
+          ev.x += alpha*pv[kk].x;
+          ev.y += alpha*pv[kk].y;
+          rv.x = rv.x - alpha*apv[kk].x;
+          rv.y = rv.y - alpha*apv[kk].y;
+          *(float2 *) (e + c) = ev;
+          *(float2 *) (r + c) = rv;
+          float m0 = solid(fluid, c) ? 0.f : invm_at<float>(T, D, x0, j, k0 + kk);
+          float m1 = solid(fluid, c + 1) ? 0.f
+                                         : invm_at<float>(T, D, x0 + 1, j, k0 + kk);
+          float z0 = rv.x*m0, z1 = rv.y*m1;
+          acc += (double) rv.x * (double) z0 + (double) rv.y * (double) z1;
+        }
+      }
+    }
+    tile_partial(acc, gpart, tile);
+  }
+}
+
 /* The same pass with every neighbour read from global memory instead of
  * LDS (a second implementation, used to cross-check the tiled one). */
 template <bool FIRST>
@@ -987,24 +1299,41 @@ int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
                                           0: direct loads */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_update_group = 0;               /* 1: update kernel over z-columns too */
+int bb_mixed_x2 = 1;                   /* 1: FP32 kernels on x pairs (even nx) */
 int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
 long bb_pcg_solves_total = 0;
 
-/* FP32 vectors of the mixed form's inner sweeps (Gcc layout). */
+/* FP32 vectors of the mixed form's inner sweeps (Gcc layout).  Each
+ * vector starts one float into its allocation, so that element 1 (the
+ * first interior x) and, for even nx, every odd x of every row sits on an
+ * 8-byte boundary: the two-cells-per-lane kernels move (x, x + 1) pairs
+ * with odd x as one float2. */
 static float *_r32 = NULL, *_p32a = NULL, *_p32b = NULL, *_Ap32 = NULL,
              *_e32 = NULL;
 static long buf32_n = 0;
 
+static void cg_buffers32_free(void)
+{
+  float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
+  for (int i = 0; i < 5; i++) {
+    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i] - 1));
+    *bufs[i] = NULL;
+  }
+  buf32_n = 0;
+}
+
 static void cg_buffers32(void)
 {
   if (buf32_n == ddom.cc.n) return;
+  cg_buffers32_free();
   float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
   for (int i = 0; i < 5; i++) {
-    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i]));
-    BB_CHECK(hipMalloc((void **) bufs[i], ddom.cc.n * sizeof(float)));
-    BB_CHECK(hipMemset(*bufs[i], 0, ddom.cc.n * sizeof(float)));
+    float *a = NULL;
+    BB_CHECK(hipMalloc((void **) &a, (ddom.cc.n + 2) * sizeof(float)));
+    BB_CHECK(hipMemset(a, 0, (ddom.cc.n + 2) * sizeof(float)));
+    *bufs[i] = a + 1;
   }
   buf32_n = ddom.cc.n;
 }
@@ -1047,12 +1376,7 @@ void bb_solver_free(void)
   _part_w1 = _part_w2 = NULL;
   if (_cg_state) BB_CHECK(hipFree(_cg_state));
   if (cg_host) BB_CHECK(hipHostFree(cg_host));
-  float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
-  for (int i = 0; i < 5; i++) {
-    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i]));
-    *bufs[i] = NULL;
-  }
-  buf32_n = 0;
+  cg_buffers32_free();
   _cg_state = NULL;
   cg_host = NULL;
   _p_alt = _part_p = _part_r = NULL;
@@ -1326,6 +1650,36 @@ extern "C" void cuda_PP_cg(void)
   cuda_PP_cg_noparts();
 }
 
+/* Number of tile partials now in _part_r (the FP64 tile kernels and the
+ * x-pair FP32 kernels use different tile counts). */
+static long rp_n = 0;
+
+static bool use_x2(void) { return bb_mixed_x2 && ddom.nx % 2 == 0; }
+
+static long n_tiles_x2_host(void)
+{
+  long ntx = (ddom.nx + TX2 - 1) / TX2;
+  long nty = (ddom.ny + TILE_Y - 1) / TILE_Y;
+  long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
+  return ntx * nty * ntz;
+}
+
+static dim3 x2_group_grid(int tzb)
+{
+  long ntx = (ddom.nx + TX2 - 1) / TX2;
+  long nty = (ddom.ny + TILE_Y - 1) / TILE_Y;
+  long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
+  return dim3((unsigned) (ntx * nty * ((ntz + tzb - 1) / tzb)));
+}
+
+/* z-column length of the x-pair kernels: at least four workgroups per CU */
+static int x2_tzb(void)
+{
+  if (bb_spmv_tzb > 0) return bb_spmv_tzb;
+  long t = n_tiles_x2_host() / 1024;
+  return t < 1 ? 1 : (t > 4 ? 4 : (int) t);
+}
+
 /* Enqueue inner iteration q of the mixed form (FP32 vectors); `first`
  * starts a new Krylov space (p = z). */
 static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
@@ -1334,6 +1688,28 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   const long nt = n_tiles(ddom);
   float *pnew = (q & 1) ? _p32a : _p32b;
   float *pold = (q & 1) ? _p32b : _p32a;
+  if (use_x2()) {
+    const long nt2 = n_tiles_x2_host();
+    int tzb = x2_tzb();
+    bb_kt_begin(KT_SPMV32);
+    if (first)
+      hipLaunchKernelGGL(search_spmv_x2_kernel<true>, x2_group_grid(tzb),
+        tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
+        tzb, _r32, pold, fluid_mask(), pnew, _Ap32, _part_p);
+    else
+      hipLaunchKernelGGL(search_spmv_x2_kernel<false>, x2_group_grid(tzb),
+        tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
+        tzb, _r32, pold, fluid_mask(), pnew, _Ap32, _part_p);
+    bb_kt_end(KT_SPMV32);
+    bb_kt_begin(KT_UPDATE32);
+    hipLaunchKernelGGL(update_x2_kernel, x2_group_grid(tzb), tiles_block(), 0,
+      bb_stream, ddom, T, _cg_state, q, _part_p, nt2, tzb, pnew, _Ap32,
+      fluid_mask(), _e32, _r32, _part_r);
+    bb_kt_end(KT_UPDATE32);
+    rp_n = nt2;
+    BB_LAUNCH_CHECK();
+    return;
+  }
   bb_kt_begin(KT_SPMV32);
   if (bb_spmv_variant == 2) {
     int tzb = spmv_tzb();
@@ -1368,6 +1744,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
       q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
   }
   bb_kt_end(KT_UPDATE32);
+  rp_n = nt;
   BB_LAUNCH_CHECK();
 }
 
@@ -1406,6 +1783,7 @@ static void pcg_mixed(double ts, int warm)
     hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
       fluid_mask(), _phi, _e32, _r32, _part_r);
   }
+  rp_n = nt;
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual,
     pp_inner_tol*pp_inner_tol);
@@ -1433,7 +1811,7 @@ static void pcg_mixed(double ts, int warm)
     for (int qq = q + 1; qq <= qend; qq++)
       enqueue_iteration32(qq, qq == first_q, K32, T32);
     hipLaunchKernelGGL(cg_check_kernel<true>, dim3(1), dim3(TILE_THREADS),
-      0, bb_stream, _cg_state, qend, _part_r, nt);
+      0, bb_stream, _cg_state, qend, _part_r, rp_n);
     fetch_state();
     if (!cg_host->inner_done) {
       q = qend;
@@ -1445,6 +1823,7 @@ static void pcg_mixed(double ts, int warm)
       _rhs_p, fluid_mask(), _r32, _part_r);
     hipLaunchKernelGGL(mx_state_kernel, dim3(1), dim3(TILE_THREADS), 0,
       bb_stream, _cg_state, _part_r, nt, q, bb_mixed_restart);
+    rp_n = nt;
     BB_LAUNCH_CHECK();
     nouter++;
     fetch_state();
