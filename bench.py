@@ -15,8 +15,9 @@ in -z, Lamb order 4).  It runs twice, once per form:
                  training solves happen in untimed warm-up steps).
 `value` is the mixed+learned form's timesteps/s; the plain form's figures
 sit beside it.  Prints one JSON line with the throughput, a live roofline
-of the plain form's dominant kernel (HIP events on the library stream) and
-a CPU baseline of the FP64 host oracle on a bounded sample.
+of the dominant kernel of each form (HIP events on the library stream:
+`roofline` for the timed form, `roofline_plain` for the plain one) and a
+CPU baseline of the FP64 host oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -45,7 +46,9 @@ KT_IDS = {"spmv_search_kernel": 0, "update_kernel": 1,
           "spmv_search_kernel_f32": 5, "update_kernel_f32": 6}
 
 # Algorithmic bytes per interior cell of each timed kernel (one element per
-# array the kernel semantically reads or writes; FP64 = 8 B, FP32 = 4 B).
+# array the kernel semantically reads or writes; FP64 = 8 B, FP32 = 4 B),
+# plus one byte for the particle mask that both kernels read when the box
+# holds particles.
 KERNEL_BYTES = {
     # update: reads phi, p, r, Ap; writes phi, r
     "update_kernel": 6 * 8,
@@ -55,6 +58,12 @@ KERNEL_BYTES = {
     "update_kernel_f32": 6 * 4,
     "spmv_search_kernel_f32": 4 * 4,
 }
+MASK_BYTES = 1
+
+# HBM bytes per interior cell per launch measured by rocprofv3 --pmc
+# (FETCH_SIZE x 2 + WRITE_SIZE, the factor 2 from a copy kernel of known
+# size) on the 256^3 sedimentation workload; profiles/ holds the runs.
+TRAFFIC_PER_CELL = {}
 
 
 def parse():
@@ -209,13 +218,18 @@ def run_form(lib, a, precision, warm, use_net=1):
     }
 
 
-def roofline_of(ktimes, cells, copy_gbs):
+def roofline_of(ktimes, cells, copy_gbs, mask):
+    """Live roofline of the kernel with the most timed time: algorithmic
+    bytes per launch over its average launch time (HIP events on the
+    library stream)."""
     if not ktimes:
         return None
     dom_name = max(ktimes, key=lambda k: ktimes[k][0])
     kms, kn = ktimes[dom_name]
     avg_s = (kms / 1e3) / kn
-    achieved = KERNEL_BYTES[dom_name] * cells / avg_s / 1e9
+    bpc = KERNEL_BYTES[dom_name] + (MASK_BYTES if mask else 0)
+    achieved = bpc * cells / avg_s / 1e9
+    tpc = TRAFFIC_PER_CELL.get(dom_name)
     return {
         "kernel": dom_name,
         "bound": "hbm",
@@ -225,8 +239,9 @@ def roofline_of(ktimes, cells, copy_gbs):
         "frac": achieved / HBM_PEAK_GBS,
         "frac_of_measured_copy": achieved / copy_gbs,
         "measured_copy_gbs": copy_gbs,
-        "traffic": None,
-        "bytes_per_cell": KERNEL_BYTES[dom_name],
+        "traffic": tpc * cells if tpc else None,
+        "bytes_per_cell": bpc,
+        "algorithmic_bytes_per_launch": bpc * cells,
         "avg_launch_us": avg_s * 1e6,
         "launches_timed": kn,
     }
@@ -309,6 +324,7 @@ def main():
     for key, prec, warm, use_net in forms:
         res[key] = run_form(lib, a, prec, warm, use_net)
     cells = a.n[0] * a.n[1] * a.n[2]
+    mask = a.workload == "sediment" and a.nparts > 0
     head = res[forms[-1][0]]
     plain = res.get("plain")
     if a.workload == "sediment":
@@ -346,8 +362,11 @@ def main():
                                  if plain and head is not plain else None),
             "spmv_variant": ctypes.c_int.in_dll(lib, "bb_spmv_variant").value,
         },
-        "roofline": roofline_of((plain or head)["ktimes"], cells, copy_gbs),
+        "roofline": roofline_of(head["ktimes"], cells, copy_gbs, mask),
     }
+    if plain is not None and plain is not head:
+        out["roofline_plain"] = roofline_of(plain["ktimes"], cells, copy_gbs,
+                                            mask)
     if not a.no_cpu:
         ref = plain or head
         out["cpu_baseline"] = cpu_baseline(a, ref["pcg_iters_per_step"],

@@ -601,21 +601,64 @@ __device__ inline int y2_src(const DevDom &D, int jj)
   return jj;
 }
 
+/* Where a thread's entry of every plane comes from, fixed per workgroup:
+ * cxy = x + y*sy of the source (first) cell, -1 for an all-zero entry;
+ * info = w0 | w1 << 4 | single << 8 with w0, w1 the x and y part of the
+ * invM table index of the two cells and `single` for an entry whose only
+ * nonzero cell is the first (the wrapped or wall neighbour x = nx + 1). */
+struct Entry2 {
+  int cxy, info;
+};
+
+__device__ inline Entry2 pair_entry(const DevDom &D, int xb, int jb, int e)
+{
+  Entry2 E;
+  E.cxy = -1;
+  E.info = 0;
+  int rr = e >> 6, pp = e & 63;
+  int js = y2_src(D, jb + rr);
+  if (e >= 64 * LY || !js) return E;
+  int x0 = xb + 1 + 2 * pp;
+  int wy = 3 * wcount(js, D.ny, D.pery);
+  if (x0 + 1 <= D.nx) {
+    E.cxy = x0 + js * D.cc.sy;
+    E.info = (wcount(x0, D.nx, D.perx) + wy) |
+             ((wcount(x0 + 1, D.nx, D.perx) + wy) << 4);
+  } else {
+    int xs = x2_src(D, xb, 1 + 2 * pp);
+    if (xs) {
+      E.cxy = xs + js * D.cc.sy;
+      E.info = (wcount(xs, D.nx, D.perx) + wy) | (1 << 8);
+    }
+  }
+  return E;
+}
+
+__device__ inline Entry2 halo_entry(const DevDom &D, int xb, int jb, int t)
+{
+  Entry2 E;
+  E.cxy = -1;
+  E.info = 0;
+  if (t >= 2 * LY) return E;
+  int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
+  int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
+  if (js && xs) {
+    E.cxy = xs + js * D.cc.sy;
+    E.info = wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery);
+  }
+  return E;
+}
+
 template <bool FIRST>
-__device__ inline void pair_load(const DevDom &D, int xb, int jb, int e,
-  int ks, const float *__restrict__ r, const float *__restrict__ pold,
+__device__ inline void pair_load(const Entry2 &E, unsigned off,
+  const float *__restrict__ r, const float *__restrict__ pold,
   const uint8_t *__restrict__ fluid, float2 *rv, float2 *pv)
 {
   *rv = make_float2(0.f, 0.f);
   *pv = make_float2(0.f, 0.f);
-  int rr = e >> 6, pp = e & 63;
-  int js = y2_src(D, jb + rr);
-  if (!ks || !js || e >= 64 * LY) return;
-  int x0 = xb + 1 + 2 * pp;
-  unsigned base = (unsigned) js * (unsigned) D.cc.sy +
-                  (unsigned) ks * (unsigned) D.cc.sz;
-  if (x0 + 1 <= D.nx) {
-    unsigned c = base + (unsigned) x0;
+  if (E.cxy < 0) return;
+  unsigned c = (unsigned) E.cxy + off;
+  if (!(E.info & (1 << 8))) {
     *rv = *(const float2 *) (r + c);
     if (!FIRST) *pv = *(const float2 *) (pold + c);
     if (fluid != NULL) {
@@ -623,88 +666,74 @@ __device__ inline void pair_load(const DevDom &D, int xb, int jb, int e,
       if (!fluid[c + 1]) rv->y = 0.f;
     }
   } else {
-    /* past the last interior pair: only x = nx + 1 can be needed (the
-     * east neighbour of x = nx), as the wrapped cell 1 or a wall zero */
-    int xs = x2_src(D, xb, 1 + 2 * pp);
-    if (xs) {
-      unsigned c = base + (unsigned) xs;
-      rv->x = r[c];
-      if (!FIRST) pv->x = pold[c];
-      if (fluid != NULL && !fluid[c]) rv->x = 0.f;
-    }
+    rv->x = r[c];
+    if (!FIRST) pv->x = pold[c];
+    if (fluid != NULL && !fluid[c]) rv->x = 0.f;
   }
 }
 
 template <bool FIRST>
-__device__ inline Plane2 plane2_load(const DevDom &D, int xb, int jb, int kk,
-  const float *__restrict__ r, const float *__restrict__ pold,
-  const uint8_t *__restrict__ fluid)
+__device__ inline Plane2 plane2_load(const DevDom &D, const Entry2 &E0,
+  const Entry2 &E1, const Entry2 &EH, int kk, const float *__restrict__ r,
+  const float *__restrict__ pold, const uint8_t *__restrict__ fluid)
 {
   Plane2 P;
-  int ks = plane_src(D, kk);
-  int t = threadIdx.x;
-  pair_load<FIRST>(D, xb, jb, t, ks, r, pold, fluid, &P.r0, &P.p0);
-  pair_load<FIRST>(D, xb, jb, t + TILE_THREADS, ks, r, pold, fluid, &P.r1,
-                   &P.p1);
+  P.r0 = P.p0 = P.r1 = P.p1 = make_float2(0.f, 0.f);
   P.rh = P.ph = 0.f;
-  if (t < 2 * LY && ks) {
-    int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
-    int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
-    if (js && xs) {
-      unsigned c = (unsigned) xs + (unsigned) js * (unsigned) D.cc.sy +
-                   (unsigned) ks * (unsigned) D.cc.sz;
-      P.rh = r[c];
-      if (!FIRST) P.ph = pold[c];
-      if (fluid != NULL && !fluid[c]) P.rh = 0.f;
-    }
+  int ks = plane_src(D, kk);
+  if (!ks) return P;
+  unsigned off = (unsigned) ks * (unsigned) D.cc.sz;
+  pair_load<FIRST>(E0, off, r, pold, fluid, &P.r0, &P.p0);
+  pair_load<FIRST>(E1, off, r, pold, fluid, &P.r1, &P.p1);
+  if (EH.cxy >= 0) {
+    unsigned c = (unsigned) EH.cxy + off;
+    P.rh = r[c];
+    if (!FIRST) P.ph = pold[c];
+    if (fluid != NULL && !fluid[c]) P.rh = 0.f;
   }
   return P;
 }
 
-/* invM of source cell (xs, js, ks); 0 for a zero entry (ks, js or xs 0) */
-__device__ inline float invm2(const InvMT<float> &T, const DevDom &D, int xs,
-  int js, int ks)
-{
-  if (!xs || !js || !ks) return 0.f;
-  return T.v[wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery) +
-             9 * wcount(ks, D.nz, D.perz)];
-}
-
+/* invM of the entry's cells from the workgroup's LDS copy of the table
+ * (wz: z part of the index, -1 for a zero plane) */
 template <bool FIRST>
-__device__ inline float2 pair_value(const DevDom &D, const InvMT<float> &T,
-  int xb, int jb, int e, int ks, float2 rv, float2 pv, float beta)
+__device__ inline float2 pair_value(const float *s_invm, const Entry2 &E,
+  int wz, float2 rv, float2 pv, float beta)
 {
-  int rr = e >> 6, pp = e & 63;
-  int js = y2_src(D, jb + rr);
-  int x0 = x2_src(D, xb, 1 + 2 * pp), x1 = x2_src(D, xb, 2 + 2 * pp);
-  float z0 = rv.x * invm2(T, D, x0, js, ks);
-  float z1 = rv.y * invm2(T, D, x1, js, ks);
+  if (E.cxy < 0 || wz < 0) return make_float2(0.f, 0.f);
+  float z0 = rv.x * s_invm[(E.info & 15) + wz];
+  float z1 = rv.y * s_invm[((E.info >> 4) & 15) + wz];
   if (FIRST) return make_float2(z0, z1);
   return make_float2(z0 + beta * pv.x, z1 + beta * pv.y);
 }
 
 template <bool FIRST>
-__device__ inline void plane2_store(const DevDom &D, const InvMT<float> &T,
-  int xb, int jb, int kk, const Plane2 &P, float beta, float *__restrict__ buf)
+__device__ inline void plane2_store(const DevDom &D, const float *s_invm,
+  const Entry2 &E0, const Entry2 &E1, const Entry2 &EH, int kk,
+  const Plane2 &P, float beta, float *__restrict__ buf)
 {
   int ks = plane_src(D, kk);
+  int wz = ks ? 9 * wcount(ks, D.nz, D.perz) : -1;
   int t = threadIdx.x;
   {
-    float2 v = pair_value<FIRST>(D, T, xb, jb, t, ks, P.r0, P.p0, beta);
+    float2 v = pair_value<FIRST>(s_invm, E0, wz, P.r0, P.p0, beta);
     int rr = t >> 6, pp = t & 63;
     *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
   }
   int e1 = t + TILE_THREADS;
   if (e1 < 64 * LY) {
-    float2 v = pair_value<FIRST>(D, T, xb, jb, e1, ks, P.r1, P.p1, beta);
+    float2 v = pair_value<FIRST>(s_invm, E1, wz, P.r1, P.p1, beta);
     int rr = e1 >> 6, pp = e1 & 63;
     *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
   }
   if (t < 2 * LY) {
     int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
-    int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
-    float z = P.rh * invm2(T, D, xs, js, ks);
-    buf[rr * LX2 + 1 + xl] = FIRST ? z : z + beta * P.ph;
+    float v = 0.f;
+    if (EH.cxy >= 0 && wz >= 0) {
+      float z = P.rh * s_invm[EH.info + wz];
+      v = FIRST ? z : z + beta * P.ph;
+    }
+    buf[rr * LX2 + 1 + xl] = v;
   }
 }
 
@@ -718,7 +747,9 @@ search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   float *__restrict__ pnew, float *__restrict__ Ap, double *__restrict__ gpart)
 {
   __shared__ __attribute__((aligned(16))) float lds[3 * LPLANE2];
+  __shared__ float s_invm[27];
   if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   const Grid &C = D.cc;
   const int ntx_ = n_tiles_x2(D);
   const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
@@ -734,18 +765,22 @@ search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
   int x0 = xb + 2 * lane + 1, j = jb + wv + 1;
   bool colok = (x0 <= D.nx) && (j <= D.ny);
-  Plane2 Rp = plane2_load<FIRST>(D, xb, jb, k0 - 1, r, pold, fluid);
-  Plane2 Rc = plane2_load<FIRST>(D, xb, jb, k0, r, pold, fluid);
-  Plane2 Ra = plane2_load<FIRST>(D, xb, jb, k0 + 1, r, pold, fluid);
+  Entry2 E0 = pair_entry(D, xb, jb, threadIdx.x);
+  Entry2 E1 = pair_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
+  Entry2 EH = halo_entry(D, xb, jb, threadIdx.x);
+  Plane2 Rp = plane2_load<FIRST>(D, E0, E1, EH, k0 - 1, r, pold, fluid);
+  Plane2 Rc = plane2_load<FIRST>(D, E0, E1, EH, k0, r, pold, fluid);
+  Plane2 Ra = plane2_load<FIRST>(D, E0, E1, EH, k0 + 1, r, pold, fluid);
   double beta64 = 0.;
   if (!FIRST && !cg_continue<true>(st, rpart, nrpart, q, &beta64)) return;
   float beta = (float) beta64;
+  __syncthreads();                      /* s_invm */
   int yl = wv + 1, o = yl * LX2 + 2 + 2 * lane;
   bool wW = wall_lo(x0, D.perx), wE = wall_hi(x0 + 1, D.nx, D.perx);
   bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
   float *bprev = lds, *bcur = lds + LPLANE2, *bnext = lds + 2 * LPLANE2;
-  plane2_store<FIRST>(D, T, xb, jb, k0 - 1, Rp, beta, bprev);
-  plane2_store<FIRST>(D, T, xb, jb, k0, Rc, beta, bcur);
+  plane2_store<FIRST>(D, s_invm, E0, E1, EH, k0 - 1, Rp, beta, bprev);
+  plane2_store<FIRST>(D, s_invm, E0, E1, EH, k0, Rc, beta, bcur);
   __syncthreads();
   float2 prev = *(const float2 *) &bprev[o];
   float2 cur = *(const float2 *) &bcur[o];
@@ -753,9 +788,9 @@ search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
   const long tstride = (long) ntx_ * nty_;
   for (int k = k0; k <= kend; k++) {
-    plane2_store<FIRST>(D, T, xb, jb, k + 1, Ra, beta, bnext);
+    plane2_store<FIRST>(D, s_invm, E0, E1, EH, k + 1, Ra, beta, bnext);
     if (k + 2 <= kend + 1)
-      Ra = plane2_load<FIRST>(D, xb, jb, k + 2, r, pold, fluid);
+      Ra = plane2_load<FIRST>(D, E0, E1, EH, k + 2, r, pold, fluid);
     __syncthreads();
     float2 nxt = *(const float2 *) &bnext[o];
     if (colok) {
@@ -796,7 +831,9 @@ update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
   const uint8_t *__restrict__ fluid, float *__restrict__ e,
   float *__restrict__ r, double *__restrict__ gpart)
 {
+  __shared__ float s_invm[27];
   if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   const Grid &C = D.cc;
   const int ntx_ = n_tiles_x2(D);
   const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
@@ -828,7 +865,10 @@ update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
       }
     }
   }
+  /* tile_block_sum's barriers also publish s_invm */
   float alpha = (float) (st->rzs[q & 1]/tile_block_sum(ppart, nppart));
+  int wj = 3 * wcount(j, D.ny, D.pery);
+  int w0 = wcount(x0, D.nx, D.perx) + wj, w1 = wcount(x0 + 1, D.nx, D.perx) + wj;
   for (int tz = tz0; tz < tz1; tz++, tile += tstride) {
     int k0 = tz * TILE_Z + 1;
     int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;
@@ -857,9 +897,9 @@ update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
           rv.y = rv.y - alpha*apv[kk].y;
           *(float2 *) (e + c) = ev;
           *(float2 *) (r + c) = rv;
-          float m0 = solid(fluid, c) ? 0.f : invm_at<float>(T, D, x0, j, k0 + kk);
-          float m1 = solid(fluid, c + 1) ? 0.f
-                                         : invm_at<float>(T, D, x0 + 1, j, k0 + kk);
+          int wz = 9 * wcount(k0 + kk, D.nz, D.perz);
+          float m0 = solid(fluid, c) ? 0.f : s_invm[w0 + wz];
+          float m1 = solid(fluid, c + 1) ? 0.f : s_invm[w1 + wz];
           float z0 = rv.x*m0, z1 = rv.y*m1;
           acc += (double) rv.x * (double) z0 + (double) rv.y * (double) z1;
         }
@@ -1117,6 +1157,39 @@ galerkin_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
       a[2] += ps * Aps;
       a[3] += ps * Aq;
       a[4] += qs * Aq;
+    }
+  }
+  long nt = (long) ntx_ * nty_ * ntz_;
+  for (int m = 0; m < 5; m++) {
+    tile_partial(a[m], part + m * nt, tile);
+    __syncthreads();                  /* tile_partial's LDS is reused */
+  }
+}
+
+/* Tile partials of the least-squares system for the Jacobi-weighted
+ * residual of phi0 = alpha psi + beta q: (b, W A psi), (b, W A q),
+ * (A psi, W A psi), (A psi, W A q), (A q, W A q) with A = -Laplacian and
+ * W = invM, over fluid cells. */
+__global__ void __launch_bounds__(TILE_THREADS)
+galerkin_mr_kernel(DevDom D, Coef K, InvMTab T,
+  const uint8_t *__restrict__ fluid, const double *__restrict__ psi,
+  const double *__restrict__ q, const double *__restrict__ rhs,
+  double *__restrict__ part)
+{
+  TILE_PROLOGUE
+  double a[5] = {0., 0., 0., 0., 0.};
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      if (solid(fluid, c)) continue;
+      double w = invm_at(T, D, x, j, k), b = rhs[c];
+      double Aps = neg_lap_at(D, K, psi, x, j, k);
+      double Aq = neg_lap_at(D, K, q, x, j, k);
+      a[0] += w * b * Aps;
+      a[1] += w * b * Aq;
+      a[2] += w * Aps * Aps;
+      a[3] += w * Aps * Aq;
+      a[4] += w * Aq * Aq;
     }
   }
   long nt = (long) ntx_ * nty_ * ntz_;
@@ -1419,9 +1492,14 @@ double bb_true_residual(double sp_rhs)
  * (A = -Laplacian, symmetric positive semidefinite).  use_psi / use_q
  * select the directions; a singular or non-positive system drops the
  * direction at fault.  Returns 1 if phi was set (0: start from phi = 0). */
-/* Energy (b, phi0) captured by the last guess, and by q alone; with the
- * solution phi* the A-norm errors squared are (b, phi*) minus these. */
+/* Energy 2 (b, phi0) - (phi0, A phi0) captured by the last guess, and by
+ * the Galerkin guess on q alone; with the solution phi* the A-norm errors
+ * squared are (b, phi*) minus these. */
 double bb_galerkin_e_guess = 0., bb_galerkin_e_q = 0.;
+/* 0: the guess minimises the A-norm error (Galerkin); 1: it minimises the
+ * Jacobi-weighted residual (b - A phi0, invM (b - A phi0)), the norm of the
+ * PCG stopping test. */
+int bb_galerkin_norm = 0;
 
 int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
   int use_q, double *alpha, double *beta)
@@ -1439,18 +1517,29 @@ int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
   bb_fetch_scalars(21);
   double *S = bb_host_scalars + 16;
   double bp = S[0], bq = S[1], app = S[2], apq = S[3], aqq = S[4];
+  /* the normal equations of the chosen norm */
+  double Bp = bp, Bq = bq, Ap = app, Apq = apq, Aq = aqq;
+  if (bb_galerkin_norm == 1) {
+    hipLaunchKernelGGL(galerkin_mr_kernel, TILES_LAUNCH, ddom, coef(),
+      invm_table(coef()), fluid_mask(), ps, qq, _rhs_p, _part_w1);
+    for (int m = 0; m < 5; m++)
+      bb_reduce_rows(_part_w1 + m * nt, nt, _bb_scalars + 16 + m);
+    bb_fetch_scalars(21);
+    Bp = S[0]; Bq = S[1]; Ap = S[2]; Apq = S[3]; Aq = S[4];
+  }
   double a = 0., b = 0.;
-  double det = app * aqq - apq * apq;
-  if (use_psi && use_q && det > 1e-12 * app * aqq && app > 0. && aqq > 0.) {
-    a = (bp * aqq - bq * apq) / det;
-    b = (bq * app - bp * apq) / det;
-  } else if (use_psi && app > 0.) {
-    a = bp / app;
-  } else if (use_q && aqq > 0.) {
-    b = bq / aqq;
+  double det = Ap * Aq - Apq * Apq;
+  if (use_psi && use_q && det > 1e-12 * Ap * Aq && Ap > 0. && Aq > 0.) {
+    a = (Bp * Aq - Bq * Apq) / det;
+    b = (Bq * Ap - Bp * Apq) / det;
+  } else if (use_psi && Ap > 0.) {
+    a = Bp / Ap;
+  } else if (use_q && Aq > 0.) {
+    b = Bq / Aq;
   }
   if (!isfinite(a) || !isfinite(b) || (a == 0. && b == 0.)) return 0;
-  bb_galerkin_e_guess = a * bp + b * bq;
+  bb_galerkin_e_guess = 2. * (a * bp + b * bq) -
+                        (a * a * app + 2. * a * b * apq + b * b * aqq);
   if (use_q && aqq > 0.) bb_galerkin_e_q = bq * bq / aqq;
   hipLaunchKernelGGL(combine_kernel, TILES_LAUNCH, ddom, a, ps, b, qq, _phi);
   BB_LAUNCH_CHECK();

@@ -135,14 +135,19 @@ def test_adam_reduces_loss(net):
 def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
     """PCG iterations per timestep (all Lamb passes) with a given start:
     'zero' (the reference's phi0 = 0), 'last' (span{previous solution}),
-    'learned' (span{network guess, previous solution})."""
+    'learned' (span{network guess, previous solution}); the suffix '_mr'
+    picks the guess in the span by the Jacobi-weighted residual instead of
+    the A-norm error."""
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})
     ext["pp_warmstart"] = "OFF" if mode == "zero" else "LEARNED"
     kw["extensions"] = ext
     bbconfig.write_case(case_dir, **kw)
     bbconfig.write_parts(case_dir, parts)
-    ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1 if mode == "learned" else 0
+    ctypes.c_int.in_dll(L, "bb_warm_use_net").value = \
+        1 if mode.startswith("learned") else 0
+    ctypes.c_int.in_dll(L, "bb_galerkin_norm").value = \
+        1 if mode.endswith("_mr") else 0
     tot = ctypes.c_long.in_dll(L, "bb_pcg_iters_total")
     per_step, resid = [], []
     with bbhip.Simulation(case_dir, kw["n"]) as s:
@@ -155,6 +160,7 @@ def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
                   ctypes.c_double.in_dll(L, "bb_warm_err_guess_sum").value,
                   ctypes.c_int.in_dll(L, "bb_warm_net_solves").value)
     ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1
+    ctypes.c_int.in_dll(L, "bb_galerkin_norm").value = 0
     return per_step, resid, energy
 
 
@@ -176,16 +182,20 @@ def test_warm_start_in_solver(case_dir):
         ctypes.c_int.in_dll(L, k).value = v
     try:
         res = {m: solve_iterations(case_dir, kw, parts, 4, m, L)
-               for m in ("zero", "last", "learned")}
+               for m in ("zero", "last", "learned", "last_mr",
+                         "learned_mr")}
     finally:
         for k, v in saved.items():
             ctypes.c_int.in_dll(L, k).value = v
     for m, (it, rs, _) in res.items():
         print(f"{m:8s} PCG iterations per step {it}  final resid {rs}")
         assert all(r <= 1e-8 for r in rs)
-    z, last, lrn = (sum(res[m][0][2:]) for m in ("zero", "last", "learned"))
+    z, last, lrn, lmr, nmr = (sum(res[m][0][2:]) for m in
+                              ("zero", "last", "learned", "last_mr",
+                               "learned_mr"))
     el, eg, nsolves = res["learned"][2]
-    print("steps 3-4: zero", z, "last", last, "learned", lrn)
+    print("steps 3-4: zero", z, "last", last, "learned", lrn, "last_mr", lmr,
+          "learned_mr", nmr)
     print(f"solves with the network: {nsolves}; A-norm error^2 summed over "
           f"them: span{{phi_last}} {el:.4e}, span{{phi_last, psi}} {eg:.4e} "
           f"(ratio {el / max(eg, 1e-300):.3f})")
