@@ -106,10 +106,12 @@ __device__ inline V invm_at(const InvMT<V> &T, const DevDom &D, int x, int j,
 /* Particle cells take no part in the solve: their right-hand side is zero
  * (src/particle_kernel.cu:1748-1753) and their rows of the particle SpMV
  * (src/solver_kernel.cu:652-712) give exactly zero while the search
- * direction is zero there.  With the Jacobi factor set to zero in particle
- * cells (`fluid` = 0) the search direction stays zero in them, so the
- * fluid rows see the same neighbours and every dot product the same terms;
- * the residual in particle cells is never used.  fluid == NULL: no
+ * direction is zero there.  Every kernel that writes the residual (CG
+ * init, the periodic recompute, the update, the mixed form's corrections)
+ * writes 0 in particle cells (`fluid` = 0) and the Jacobi factor is 0
+ * there, so z = invM r and the search direction stay zero in them: the
+ * fluid rows see the same neighbours and every dot product the same terms.
+ * The z-column search kernels therefore need no mask.  fluid == NULL: no
  * particles. */
 __device__ inline bool solid(const uint8_t *__restrict__ fluid, long c)
 {
@@ -202,7 +204,7 @@ cg_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
       long c = gidx(C, x, j, k);
       double t = rhs[c];
       double t2 = t * (solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
-      r[c] = t;
+      r[c] = solid(fluid, c) ? 0. : t;
       phi[c] = 0.;
       acc += t * t2;
     }
@@ -356,8 +358,53 @@ __device__ inline PlaneRaw<V> plane_load(const DevDom &D,
   return R;
 }
 
+/* The same loads without a branch: every thread issues all of them, an
+ * entry or plane that holds zeros reading a valid dummy cell instead (the
+ * residual is already zero in particle cells).  With no load inside a
+ * branch, and none waiting on loaded data, the compiler can keep the loads
+ * of the next planes in flight across the loop; a conditional load makes
+ * it wait for every outstanding load. */
+template <typename V> struct PlaneRawM {
+  V r0, p0, r1, p1;
+};
+
 template <bool FIRST, typename V>
-__device__ inline void plane_store(const DevDom &D, const InvMT<V> &T,
+__device__ inline PlaneRawM<V> plane_load_nb(const DevDom &D,
+  const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
+  const V *__restrict__ pold, int kk)
+{
+  PlaneRawM<V> R;
+  int ks = plane_src(D, kk);
+  unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) D.cc.sz;
+  unsigned c0 = (unsigned) (E0.cxy >= 0 ? E0.cxy : 0) + off;
+  unsigned c1 = (unsigned) (E1.cxy >= 0 ? E1.cxy : 0) + off;
+  R.r0 = ld32(r, c0);
+  R.r1 = ld32(r, c1);
+  R.p0 = FIRST ? (V) 0 : ld32(pold, c0);
+  R.p1 = FIRST ? (V) 0 : ld32(pold, c1);
+  return R;
+}
+
+template <bool FIRST, typename V>
+__device__ inline void plane_store_nb(const DevDom &D, const V *tab,
+  const PlaneEntry &E0, const PlaneEntry &E1, const PlaneRawM<V> &R, V beta,
+  V *__restrict__ buf, int kk)
+{
+  int ks = plane_src(D, kk);
+  int wz = 9 * wcount(ks, D.nz, D.perz);
+  V z0 = R.r0*tab[E0.wxy + wz], z1 = R.r1*tab[E1.wxy + wz];
+  V v0 = FIRST ? z0 : z0 + beta*R.p0;
+  V v1 = FIRST ? z1 : z1 + beta*R.p1;
+  v0 = (ks && E0.cxy >= 0) ? v0 : (V) 0;
+  v1 = (ks && E1.cxy >= 0) ? v1 : (V) 0;
+  buf[threadIdx.x] = v0;
+  if (threadIdx.x + TILE_THREADS < LPLANE) buf[threadIdx.x + TILE_THREADS] = v1;
+}
+
+/* `tab`: the 27-entry invM table (kernel argument or a workgroup's LDS
+ * copy). */
+template <bool FIRST, typename V>
+__device__ inline void plane_store(const DevDom &D, const V *tab,
   const PlaneEntry &E0, const PlaneEntry &E1, const PlaneRaw<V> &R, V beta,
   V *__restrict__ buf, int kk)
 {
@@ -365,11 +412,11 @@ __device__ inline void plane_store(const DevDom &D, const InvMT<V> &T,
   int wz = 9 * wcount(ks, D.nz, D.perz);
   V v0 = 0, v1 = 0;
   if (ks && E0.cxy >= 0) {
-    V z = R.r0*T.v[E0.wxy + wz];
+    V z = R.r0*tab[E0.wxy + wz];
     v0 = FIRST ? z : z + beta*R.p0;
   }
   if (ks && E1.cxy >= 0) {
-    V z = R.r1*T.v[E1.wxy + wz];
+    V z = R.r1*tab[E1.wxy + wz];
     v1 = FIRST ? z : z + beta*R.p1;
   }
   buf[threadIdx.x] = v0;
@@ -422,9 +469,10 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
   bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
 
-  plane_store<FIRST, V>(D, T, E0, E1, R[0], beta,
+  plane_store<FIRST, V>(D, T.v, E0, E1, R[0], beta,
                         lds + ((k0 - 1) & 1) * LPLANE, k0 - 1);
-  plane_store<FIRST, V>(D, T, E0, E1, R[1], beta, lds + (k0 & 1) * LPLANE, k0);
+  plane_store<FIRST, V>(D, T.v, E0, E1, R[1], beta, lds + (k0 & 1) * LPLANE,
+                        k0);
   __syncthreads();
   V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
   V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
@@ -434,7 +482,7 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     const int k = k0 + kk;
     if (k <= k1) {                    /* uniform over the workgroup */
       __syncthreads();
-      plane_store<FIRST, V>(D, T, E0, E1, R[(kk + 2) % RING], beta,
+      plane_store<FIRST, V>(D, T.v, E0, E1, R[(kk + 2) % RING], beta,
                             lds + ((k + 1) & 1) * LPLANE, k + 1);
       if (k + RING - 1 <= k1 + 1)
         R[kk % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + RING - 1);
@@ -472,11 +520,12 @@ __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   int q, const double *__restrict__ rpart, long ntiles, int tzb,
   const V *__restrict__ r, const V *__restrict__ pold,
-  const uint8_t *__restrict__ fluid,
   V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
 {
   __shared__ V lds[3 * LPLANE];
+  __shared__ V s_invm[27];
   if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   const Grid &C = D.cc;
   const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
   const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
@@ -494,17 +543,17 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   bool colok = (x <= D.nx) && (j <= D.ny);
   PlaneEntry E0 = plane_entry(D, xb, jb, threadIdx.x);
   PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
-  constexpr bool TWO = true;                /* two planes in flight */
-  PlaneRaw<V> Rp = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 - 1);
-  PlaneRaw<V> Rc = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0);
-  PlaneRaw<V> Ra = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 1);
-  PlaneRaw<V> Rb;
-  Rb.r0 = Rb.p0 = Rb.r1 = Rb.p1 = 0;
+  PlaneRawM<V> Rp = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0 - 1);
+  PlaneRawM<V> Rc = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0);
+  PlaneRawM<V> Ra = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0 + 1);
   double beta64 = 0.;
   if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
-  if (TWO && k0 + 2 <= kend + 1)
-    Rb = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 2);
+  /* loads are unconditional (plane index clamped to kend + 1): a load
+   * issued on one path only would make the compiler wait for all */
+  PlaneRawM<V> Rb = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
+                                                 min(k0 + 2, kend + 1));
   V beta = (V) beta64;
+  __syncthreads();                      /* s_invm */
   int xl = lane + 1, yl = wv + 1;
   bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
   bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
@@ -513,23 +562,20 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
    * plane k + 1 held plane k - 2, whose last reads came before the barrier
    * of step k - 1, so one barrier per plane suffices */
   V *bprev = lds, *bcur = lds + LPLANE, *bnext = lds + 2 * LPLANE;
-  plane_store<FIRST, V>(D, T, E0, E1, Rp, beta, bprev, k0 - 1);
-  plane_store<FIRST, V>(D, T, E0, E1, Rc, beta, bcur, k0);
+  plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rp, beta, bprev, k0 - 1);
+  plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rc, beta, bcur, k0);
   __syncthreads();
   V prev = bprev[yl * LX + xl];
   V cur = bcur[yl * LX + xl];
   double acc = 0.0;
   long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
   const long tstride = (long) ntx_ * nty_;
-  for (int k = k0; k <= kend; k++) {
-    plane_store<FIRST, V>(D, T, E0, E1, Ra, beta, bnext, k + 1);
-    if (TWO) {
-      Ra = Rb;
-      if (k + 3 <= kend + 1)
-        Rb = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + 3);
-    } else if (k + 2 <= kend + 1) {
-      Ra = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + 2);
-    }
+  /* one plane of the column: store plane k + 1 (raw values in Rs, loaded
+   * two steps earlier), refill Rs with plane k + 3, compute plane k */
+  auto step = [&](int k, PlaneRawM<V> &Rs) {
+    plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rs, beta, bnext, k + 1);
+    Rs = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
+                                       min(k + 3, kend + 1));
     __syncthreads();
     const V *P = bcur;
     V nxt = bnext[yl * LX + xl];
@@ -554,6 +600,12 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
       acc = 0.0;
       tile += tstride;
     }
+  };
+  /* unrolled by two so that Ra and Rb alternate without register copies
+   * (a copy of a register with a load in flight waits for that load) */
+  for (int k = k0; k <= kend; k += 2) {
+    step(k, Ra);
+    if (k + 1 <= kend) step(k + 1, Rb);
   }
 }
 
@@ -649,48 +701,34 @@ __device__ inline Entry2 halo_entry(const DevDom &D, int xb, int jb, int t)
   return E;
 }
 
+/* Every load is issued unconditionally (a zero entry or plane reads the
+ * aligned dummy pair at x = 1 of row 0 instead and is zeroed when stored;
+ * a `single` entry reads its pair and keeps the first value), so the
+ * compiler can keep loads of the next plane in flight across the loop.
+ * The residual is already zero in particle cells. */
 template <bool FIRST>
 __device__ inline void pair_load(const Entry2 &E, unsigned off,
-  const float *__restrict__ r, const float *__restrict__ pold,
-  const uint8_t *__restrict__ fluid, float2 *rv, float2 *pv)
+  const float *__restrict__ r, const float *__restrict__ pold, float2 *rv,
+  float2 *pv)
 {
-  *rv = make_float2(0.f, 0.f);
-  *pv = make_float2(0.f, 0.f);
-  if (E.cxy < 0) return;
-  unsigned c = (unsigned) E.cxy + off;
-  if (!(E.info & (1 << 8))) {
-    *rv = *(const float2 *) (r + c);
-    if (!FIRST) *pv = *(const float2 *) (pold + c);
-    if (fluid != NULL) {
-      if (!fluid[c]) rv->x = 0.f;
-      if (!fluid[c + 1]) rv->y = 0.f;
-    }
-  } else {
-    rv->x = r[c];
-    if (!FIRST) pv->x = pold[c];
-    if (fluid != NULL && !fluid[c]) rv->x = 0.f;
-  }
+  unsigned c = (unsigned) (E.cxy >= 0 ? E.cxy : 1) + off;
+  *rv = *(const float2 *) (r + c);
+  *pv = FIRST ? make_float2(0.f, 0.f) : *(const float2 *) (pold + c);
 }
 
 template <bool FIRST>
 __device__ inline Plane2 plane2_load(const DevDom &D, const Entry2 &E0,
   const Entry2 &E1, const Entry2 &EH, int kk, const float *__restrict__ r,
-  const float *__restrict__ pold, const uint8_t *__restrict__ fluid)
+  const float *__restrict__ pold)
 {
   Plane2 P;
-  P.r0 = P.p0 = P.r1 = P.p1 = make_float2(0.f, 0.f);
-  P.rh = P.ph = 0.f;
   int ks = plane_src(D, kk);
-  if (!ks) return P;
-  unsigned off = (unsigned) ks * (unsigned) D.cc.sz;
-  pair_load<FIRST>(E0, off, r, pold, fluid, &P.r0, &P.p0);
-  pair_load<FIRST>(E1, off, r, pold, fluid, &P.r1, &P.p1);
-  if (EH.cxy >= 0) {
-    unsigned c = (unsigned) EH.cxy + off;
-    P.rh = r[c];
-    if (!FIRST) P.ph = pold[c];
-    if (fluid != NULL && !fluid[c]) P.rh = 0.f;
-  }
+  unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) D.cc.sz;
+  pair_load<FIRST>(E0, off, r, pold, &P.r0, &P.p0);
+  pair_load<FIRST>(E1, off, r, pold, &P.r1, &P.p1);
+  unsigned c = (unsigned) (EH.cxy >= 0 ? EH.cxy : 1) + off;
+  P.rh = r[c];
+  P.ph = FIRST ? 0.f : pold[c];
   return P;
 }
 
@@ -700,11 +738,13 @@ template <bool FIRST>
 __device__ inline float2 pair_value(const float *s_invm, const Entry2 &E,
   int wz, float2 rv, float2 pv, float beta)
 {
-  if (E.cxy < 0 || wz < 0) return make_float2(0.f, 0.f);
-  float z0 = rv.x * s_invm[(E.info & 15) + wz];
-  float z1 = rv.y * s_invm[((E.info >> 4) & 15) + wz];
-  if (FIRST) return make_float2(z0, z1);
-  return make_float2(z0 + beta * pv.x, z1 + beta * pv.y);
+  int w = wz < 0 ? 0 : wz;
+  float z0 = rv.x * s_invm[(E.info & 15) + w];
+  float z1 = rv.y * s_invm[((E.info >> 4) & 15) + w];
+  float2 v = FIRST ? make_float2(z0, z1)
+                   : make_float2(z0 + beta * pv.x, z1 + beta * pv.y);
+  if (E.info & (1 << 8)) v.y = 0.f;
+  return (E.cxy < 0 || wz < 0) ? make_float2(0.f, 0.f) : v;
 }
 
 template <bool FIRST>
@@ -728,12 +768,9 @@ __device__ inline void plane2_store(const DevDom &D, const float *s_invm,
   }
   if (t < 2 * LY) {
     int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
-    float v = 0.f;
-    if (EH.cxy >= 0 && wz >= 0) {
-      float z = P.rh * s_invm[EH.info + wz];
-      v = FIRST ? z : z + beta * P.ph;
-    }
-    buf[rr * LX2 + 1 + xl] = v;
+    float z = P.rh * s_invm[EH.info + (wz < 0 ? 0 : wz)];
+    float v = FIRST ? z : z + beta * P.ph;
+    buf[rr * LX2 + 1 + xl] = (EH.cxy >= 0 && wz >= 0) ? v : 0.f;
   }
 }
 
@@ -743,7 +780,6 @@ __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   int q, const double *__restrict__ rpart, long nrpart, int tzb,
   const float *__restrict__ r, const float *__restrict__ pold,
-  const uint8_t *__restrict__ fluid,
   float *__restrict__ pnew, float *__restrict__ Ap, double *__restrict__ gpart)
 {
   __shared__ __attribute__((aligned(16))) float lds[3 * LPLANE2];
@@ -768,9 +804,9 @@ search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   Entry2 E0 = pair_entry(D, xb, jb, threadIdx.x);
   Entry2 E1 = pair_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
   Entry2 EH = halo_entry(D, xb, jb, threadIdx.x);
-  Plane2 Rp = plane2_load<FIRST>(D, E0, E1, EH, k0 - 1, r, pold, fluid);
-  Plane2 Rc = plane2_load<FIRST>(D, E0, E1, EH, k0, r, pold, fluid);
-  Plane2 Ra = plane2_load<FIRST>(D, E0, E1, EH, k0 + 1, r, pold, fluid);
+  Plane2 Rp = plane2_load<FIRST>(D, E0, E1, EH, k0 - 1, r, pold);
+  Plane2 Rc = plane2_load<FIRST>(D, E0, E1, EH, k0, r, pold);
+  Plane2 Ra = plane2_load<FIRST>(D, E0, E1, EH, k0 + 1, r, pold);
   double beta64 = 0.;
   if (!FIRST && !cg_continue<true>(st, rpart, nrpart, q, &beta64)) return;
   float beta = (float) beta64;
@@ -789,8 +825,7 @@ search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
   const long tstride = (long) ntx_ * nty_;
   for (int k = k0; k <= kend; k++) {
     plane2_store<FIRST>(D, s_invm, E0, E1, EH, k + 1, Ra, beta, bnext);
-    if (k + 2 <= kend + 1)
-      Ra = plane2_load<FIRST>(D, E0, E1, EH, k + 2, r, pold, fluid);
+    Ra = plane2_load<FIRST>(D, E0, E1, EH, min(k + 2, kend + 1), r, pold);
     __syncthreads();
     float2 nxt = *(const float2 *) &bnext[o];
     if (colok) {
@@ -893,8 +928,8 @@ update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
           float2 rv = *(const float2 *) (r + c);
           ev.x += alpha*pv[kk].x;
           ev.y += alpha*pv[kk].y;
-          rv.x = rv.x - alpha*apv[kk].x;
-          rv.y = rv.y - alpha*apv[kk].y;
+          rv.x = solid(fluid, c) ? 0.f : rv.x - alpha*apv[kk].x;
+          rv.y = solid(fluid, c + 1) ? 0.f : rv.y - alpha*apv[kk].y;
           *(float2 *) (e + c) = ev;
           *(float2 *) (r + c) = rv;
           int wz = 9 * wcount(k0 + kk, D.nz, D.perz);
@@ -984,7 +1019,7 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
       if (k0 + kk <= k1) {
         long c = c0 + (long) kk * C.sz;
         phi[c] += alpha*pv[kk];
-        V rr = r[c] - alpha*apv[kk];
+        V rr = solid(fluid, c) ? (V) 0 : r[c] - alpha*apv[kk];
         r[c] = rr;
         V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
         acc += (double) rr * (double) z;
@@ -1049,7 +1084,7 @@ update_group_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
           V *ph = (V *) ((char *) phi + c * (unsigned) sizeof(V));
           V *rp = (V *) ((char *) r + c * (unsigned) sizeof(V));
           *ph += alpha*pv[kk];
-          V rr = *rp - alpha*apv[kk];
+          V rr = solid(fluid, c) ? (V) 0 : *rp - alpha*apv[kk];
           *rp = rr;
           V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
           acc += (double) rr * (double) z;
@@ -1107,7 +1142,7 @@ residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
-      r[c] = rr;
+      r[c] = solid(fluid, c) ? 0. : rr;
       double z = rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
       acc += rr * z;
     }
@@ -1128,7 +1163,7 @@ cg_init_warm_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ rhs,
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
-      r[c] = rr;
+      r[c] = solid(fluid, c) ? 0. : rr;
       double z = rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k));
       acc += rr * z;
     }
@@ -1271,7 +1306,7 @@ mx_init_kernel(DevDom D, InvMTab T, const double *__restrict__ rhs,
       double t = rhs[c];
       phi[c] = 0.;
       e[c] = 0.f;
-      r[c] = (float) t;
+      r[c] = solid(fluid, c) ? 0.f : (float) t;
       acc += t * (t*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k)));
     }
   }
@@ -1313,7 +1348,7 @@ mx_residual_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ phi,
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
-      r[c] = (float) rr;
+      r[c] = solid(fluid, c) ? 0.f : (float) rr;
       acc += rr * (rr*(solid(fluid, c) ? 0. : invm_at(T, D, x, j, k)));
     }
   }
@@ -1586,6 +1621,17 @@ void bb_pp_defect(const double *x, double *out)
   BB_LAUNCH_CHECK();
 }
 
+/* The z-column search kernel. */
+template <bool FIRST, typename V, bool MIXED>
+static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
+  long nrp, const V *r, const V *pold, V *pnew, V *Ap)
+{
+  int tzb = spmv_tzb();
+  hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED>),
+    spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+    _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+}
+
 /* Enqueue iteration q; p buffers alternate by parity. */
 static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 {
@@ -1594,17 +1640,11 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   double *pold = (q & 1) ? _p_alt : _p_q;
   bb_kt_begin(KT_SPMV);
   if (bb_spmv_variant == 2) {
-    int tzb = spmv_tzb();
     if (q == 1)
-      hipLaunchKernelGGL((search_spmv_group_kernel<true, double, false>),
-        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-        _cg_state, q, _part_r, nt, tzb, _r_q, pold, fluid_mask(), pnew,
-        _Apb_q, _part_p);
+      launch_group<true, double, false>(K, T, q, nt, _r_q, pold, pnew, _Apb_q);
     else
-      hipLaunchKernelGGL((search_spmv_group_kernel<false, double, false>),
-        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-        _cg_state, q, _part_r, nt, tzb, _r_q, pold, fluid_mask(), pnew,
-        _Apb_q, _part_p);
+      launch_group<false, double, false>(K, T, q, nt, _r_q, pold, pnew,
+                                         _Apb_q);
   } else if (bb_spmv_variant == 1) {
     if (q == 1)
       hipLaunchKernelGGL((search_spmv_kernel<true, double, false>),
@@ -1784,11 +1824,11 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
     if (first)
       hipLaunchKernelGGL(search_spmv_x2_kernel<true>, x2_group_grid(tzb),
         tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
-        tzb, _r32, pold, fluid_mask(), pnew, _Ap32, _part_p);
+        tzb, _r32, pold, pnew, _Ap32, _part_p);
     else
       hipLaunchKernelGGL(search_spmv_x2_kernel<false>, x2_group_grid(tzb),
         tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
-        tzb, _r32, pold, fluid_mask(), pnew, _Ap32, _part_p);
+        tzb, _r32, pold, pnew, _Ap32, _part_p);
     bb_kt_end(KT_SPMV32);
     bb_kt_begin(KT_UPDATE32);
     hipLaunchKernelGGL(update_x2_kernel, x2_group_grid(tzb), tiles_block(), 0,
@@ -1801,17 +1841,10 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   }
   bb_kt_begin(KT_SPMV32);
   if (bb_spmv_variant == 2) {
-    int tzb = spmv_tzb();
     if (first)
-      hipLaunchKernelGGL((search_spmv_group_kernel<true, float, true>),
-        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-        _cg_state, q, _part_r, nt, tzb, _r32, pold, fluid_mask(), pnew,
-        _Ap32, _part_p);
+      launch_group<true, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
     else
-      hipLaunchKernelGGL((search_spmv_group_kernel<false, float, true>),
-        spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-        _cg_state, q, _part_r, nt, tzb, _r32, pold, fluid_mask(), pnew,
-        _Ap32, _part_p);
+      launch_group<false, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
   } else if (first) {
     hipLaunchKernelGGL((search_spmv_kernel<true, float, true>), TILES_LAUNCH,
       ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
