@@ -211,6 +211,21 @@ __device__ inline void tile_partial(double acc, double *__restrict__ gpart,
                   ((s_w[4] + s_w[5]) + (s_w[6] + s_w[7]));
 }
 
+/* tile_partial without a branch around the global store: every thread
+ * forms the same row combination and stores the same value (same order,
+ * same bits), so no memory operation depends on the thread index. */
+__device__ inline void tile_partial_all(double acc, double *__restrict__ gpart,
+  long tile)
+{
+  __shared__ double s_w2[TILE_Y];
+  int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  double v = wave_tree_sum(acc);
+  if (lane == 0) s_w2[w] = v;
+  __syncthreads();
+  gpart[tile] = ((s_w2[0] + s_w2[1]) + (s_w2[2] + s_w2[3])) +
+                ((s_w2[4] + s_w2[5]) + (s_w2[6] + s_w2[7]));
+}
+
 /* Step 4 inside a TILE_THREADS (512) workgroup: thread t plays accumulators
  * t and t + 512.  Returns the total to every thread. */
 __device__ inline double tile_block_sum(const double *__restrict__ part, long n)

@@ -928,8 +928,10 @@ update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
           float2 rv = *(const float2 *) (r + c);
           ev.x += alpha*pv[kk].x;
           ev.y += alpha*pv[kk].y;
-          rv.x = solid(fluid, c) ? 0.f : rv.x - alpha*apv[kk].x;
-          rv.y = solid(fluid, c + 1) ? 0.f : rv.y - alpha*apv[kk].y;
+          rv.x = rv.x - alpha*apv[kk].x;
+          rv.y = rv.y - alpha*apv[kk].y;
+          rv.x = solid(fluid, c) ? 0.f : rv.x;
+          rv.y = solid(fluid, c + 1) ? 0.f : rv.y;
           *(float2 *) (e + c) = ev;
           *(float2 *) (r + c) = rv;
           int wz = 9 * wcount(k0 + kk, D.nz, D.perz);
@@ -991,7 +993,14 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
  * r -= alpha Ap, z = invM r + tile partials of (r, z).  In the mixed form
  * phi is the FP32 correction accumulated since the last FP64 correction. */
 template <typename V>
+__device__ inline void st32(V *__restrict__ a, unsigned i, V v)
+{
+  *(V *) ((char *) a + i * (unsigned) sizeof(V)) = v;
+}
+
+template <typename V, bool MASK>
 __global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
 update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
   const double *__restrict__ ppart, long ntiles,
   const V *__restrict__ p, const V *__restrict__ Ap,
@@ -1000,26 +1009,83 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
 {
   if (st->done | st->inner_done) return;
   TILE_PROLOGUE
-  /* p and Ap are loaded before the reduction that yields alpha. */
-  long c0 = gidx(C, x, j, k0);
-  V pv[TILE_Z], apv[TILE_Z];
+  /* A whole tile (the common case) runs with no load or store behind a
+   * branch, so the waits the compiler inserts stay exact; same arithmetic
+   * and order as the general path below. */
+  if (xb + TILE_X <= D.nx && jb + TILE_Y <= D.ny && k0 + TILE_Z - 1 <= D.nz) {
+    unsigned c0 = (unsigned) gidx(C, x, j, k0), sz = (unsigned) C.sz;
+    /* planes 0-3 of p and Ap are in flight during the reduction; the tile
+     * then goes in two halves of four planes */
+    V pv[4], apv[4];
 #pragma unroll
-  for (int kk = 0; kk < TILE_Z; kk++) {
-    pv[kk] = apv[kk] = 0;
-    if (colok && k0 + kk <= k1) {
-      pv[kk] = p[c0 + (long) kk * C.sz];
-      apv[kk] = Ap[c0 + (long) kk * C.sz];
+    for (int i = 0; i < 4; i++) {
+      pv[i] = ld32(p, c0 + i * sz);
+      apv[i] = ld32(Ap, c0 + i * sz);
+    }
+    V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
+    double acc = 0.0;
+#pragma unroll
+    for (int h = 0; h < TILE_Z; h += 4) {
+      V ph[4], rv[4];
+      uint32_t f[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        unsigned c = c0 + (h + i) * sz;
+        if (h > 0) {
+          pv[i] = ld32(p, c);
+          apv[i] = ld32(Ap, c);
+        }
+        ph[i] = ld32(phi, c);
+        rv[i] = ld32(r, c);
+        f[i] = MASK ? (uint32_t) fluid[c] : 1u;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int kk = h + i;
+        unsigned c = c0 + kk * sz;
+        st32(phi, c, (V) (ph[i] + alpha*pv[i]));
+        V rr = rv[i] - alpha*apv[i];
+        rr = f[i] ? rr : (V) 0;
+        st32(r, c, rr);
+        V z = rr*(f[i] ? invm_at<V>(T, D, x, j, k0 + kk) : (V) 0);
+        acc += (double) rr * (double) z;
+      }
+      /* the second half's loads are issued after the first half's stores,
+       * not hoisted above them (register pressure) */
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    tile_partial_all(acc, gpart, tile);
+    return;
+  }
+  /* general tiles (domain edges): the same halves with every access
+   * guarded */
+  long cb = gidx(C, x, j, k0);
+  V pv[4], apv[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    pv[i] = apv[i] = 0;
+    if (colok && k0 + i <= k1) {
+      pv[i] = p[cb + (long) i * C.sz];
+      apv[i] = Ap[cb + (long) i * C.sz];
     }
   }
   V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
   double acc = 0.0;
   if (colok) {
 #pragma unroll
-    for (int kk = 0; kk < TILE_Z; kk++) {
-      if (k0 + kk <= k1) {
-        long c = c0 + (long) kk * C.sz;
-        phi[c] += alpha*pv[kk];
-        V rr = solid(fluid, c) ? (V) 0 : r[c] - alpha*apv[kk];
+    for (int h = 0; h < TILE_Z; h += 4) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int kk = h + i;
+        if (k0 + kk > k1) continue;
+        long c = cb + (long) kk * C.sz;
+        if (h > 0) {
+          pv[i] = p[c];
+          apv[i] = Ap[c];
+        }
+        phi[c] += alpha*pv[i];
+        V rr = r[c] - alpha*apv[i];
+        rr = solid(fluid, c) ? (V) 0 : rr;
         r[c] = rr;
         V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
         acc += (double) rr * (double) z;
@@ -1084,7 +1150,8 @@ update_group_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
           V *ph = (V *) ((char *) phi + c * (unsigned) sizeof(V));
           V *rp = (V *) ((char *) r + c * (unsigned) sizeof(V));
           *ph += alpha*pv[kk];
-          V rr = solid(fluid, c) ? (V) 0 : *rp - alpha*apv[kk];
+          V rr = *rp - alpha*apv[kk];
+          rr = solid(fluid, c) ? (V) 0 : rr;
           *rp = rr;
           V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
           acc += (double) rr * (double) z;
@@ -1676,9 +1743,14 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
         tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
         pnew, _Apb_q, fluid_mask(), _phi, _r_q, _part_r);
     } else {
-      hipLaunchKernelGGL(update_kernel<double>, TILES_LAUNCH, ddom, T,
-        _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi, _r_q,
-        _part_r);
+      if (fluid_mask())
+        hipLaunchKernelGGL((update_kernel<double, true>), TILES_LAUNCH, ddom,
+          T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi,
+          _r_q, _part_r);
+      else
+        hipLaunchKernelGGL((update_kernel<double, false>), TILES_LAUNCH, ddom,
+          T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi,
+          _r_q, _part_r);
     }
     bb_kt_end(KT_UPDATE);
   }
@@ -1862,8 +1934,14 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
       tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
       pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
   } else {
-    hipLaunchKernelGGL(update_kernel<float>, TILES_LAUNCH, ddom, T, _cg_state,
-      q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
+    if (fluid_mask())
+      hipLaunchKernelGGL((update_kernel<float, true>), TILES_LAUNCH, ddom, T,
+        _cg_state, q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32,
+        _part_r);
+    else
+      hipLaunchKernelGGL((update_kernel<float, false>), TILES_LAUNCH, ddom, T,
+        _cg_state, q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32,
+        _part_r);
   }
   bb_kt_end(KT_UPDATE32);
   rp_n = nt;
