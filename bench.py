@@ -100,8 +100,8 @@ def parse():
     ap.add_argument("--update-group", type=int, default=None,
                     help="1: update kernel over z-columns of tiles too")
     ap.add_argument("--mixed-x2", type=int, default=None,
-                    help="1: FP32 kernels of the mixed form on x pairs "
-                         "(default), 0: one cell per lane")
+                    help="1: FP32 kernels of the mixed form on x pairs, "
+                         "0: one cell per lane (default)")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
     ap.add_argument("--time-every", type=int, default=8,

@@ -478,8 +478,10 @@ __global__ void face_list_kernel(PDom P, const int *__restrict__ phase,
   list[atomicAdd(&count[dir], 1)] = (int) t;
 }
 
+/* One wavefront per workgroup: the Lamb series needs more registers than
+ * the default launch bound allows without spilling. */
 template <int dir>
-__global__ void part_bc_kernel(PDom P, double *vel,
+__global__ void __launch_bounds__(64) part_bc_kernel(PDom P, double *vel,
   const int *__restrict__ phase, const int *__restrict__ flag,
   const part_struct *__restrict__ parts, double nu,
   const int *__restrict__ list, int nlist)
@@ -571,13 +573,13 @@ static void part_bc(double *u, double *v, double *w)
 {
   PDom P = pdom();
   if (bc_n[0] > 0)
-    hipLaunchKernelGGL(part_bc_kernel<0>, dim3(nblk(bc_n[0], 256)), dim3(256),
+    hipLaunchKernelGGL(part_bc_kernel<0>, dim3(nblk(bc_n[0], 64)), dim3(64),
       0, bb_stream, P, u, _phase, _flag_u, _parts, nu, _bc_list[0], bc_n[0]);
   if (bc_n[1] > 0)
-    hipLaunchKernelGGL(part_bc_kernel<1>, dim3(nblk(bc_n[1], 256)), dim3(256),
+    hipLaunchKernelGGL(part_bc_kernel<1>, dim3(nblk(bc_n[1], 64)), dim3(64),
       0, bb_stream, P, v, _phase, _flag_v, _parts, nu, _bc_list[1], bc_n[1]);
   if (bc_n[2] > 0)
-    hipLaunchKernelGGL(part_bc_kernel<2>, dim3(nblk(bc_n[2], 256)), dim3(256),
+    hipLaunchKernelGGL(part_bc_kernel<2>, dim3(nblk(bc_n[2], 64)), dim3(64),
       0, bb_stream, P, w, _phase, _flag_w, _parts, nu, _bc_list[2], bc_n[2]);
   BB_LAUNCH_CHECK();
 }

@@ -1007,12 +1007,16 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
   const uint8_t *__restrict__ fluid, V *__restrict__ phi,
   V *__restrict__ r, double *__restrict__ gpart)
 {
+  __shared__ V s_invm[27];
   if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   TILE_PROLOGUE
   /* A whole tile (the common case) runs with no load or store behind a
    * branch, so the waits the compiler inserts stay exact; same arithmetic
-   * and order as the general path below. */
+   * and order as the general path below.  The invM table is read from LDS
+   * (tile_block_sum's barriers publish it). */
   if (xb + TILE_X <= D.nx && jb + TILE_Y <= D.ny && k0 + TILE_Z - 1 <= D.nz) {
+    const int wxy = wcount(x, D.nx, D.perx) + 3 * wcount(j, D.ny, D.pery);
     unsigned c0 = (unsigned) gidx(C, x, j, k0), sz = (unsigned) C.sz;
     /* planes 0-3 of p and Ap are in flight during the reduction; the tile
      * then goes in two halves of four planes */
@@ -1047,7 +1051,8 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
         V rr = rv[i] - alpha*apv[i];
         rr = f[i] ? rr : (V) 0;
         st32(r, c, rr);
-        V z = rr*(f[i] ? invm_at<V>(T, D, x, j, k0 + kk) : (V) 0);
+        V m = s_invm[wxy + 9 * wcount(k0 + kk, D.nz, D.perz)];
+        V z = rr*(f[i] ? m : (V) 0);
         acc += (double) rr * (double) z;
       }
       /* the second half's loads are issued after the first half's stores,
@@ -1474,7 +1479,9 @@ int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
                                           0: direct loads */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_update_group = 0;               /* 1: update kernel over z-columns too */
-int bb_mixed_x2 = 1;                   /* 1: FP32 kernels on x pairs (even nx) */
+int bb_mixed_x2 = 0;                   /* 1: FP32 kernels on x pairs (even nx);
+                                          0 (default): the one-cell z-column
+                                          kernels, measured as fast */
 int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */

@@ -27,9 +27,9 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p")
 
 
-def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=1):
+def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0):
     """x2 = 1: the FP32 kernels on x pairs (used when nx is even), 0: one
-    cell per lane."""
+    cell per lane (the default)."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = restart
     ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = x2
@@ -57,7 +57,7 @@ def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=1):
         for f in FIELDS:
             res["gpu"][f] = np.array(s.field(f))
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = 0
-    ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = 1
+    ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = 0
     return res
 
 
@@ -103,7 +103,7 @@ def test_mixed_channel_two_x_tiles(case_dir):
     """Channel 192x16x24: one full and one partial 128-cell tile in x (the
     halo between x-pair tiles and the periodic wrap at x = nx + 1), a
     partial tile in z, 3 steps."""
-    res = run_mixed(case_dir, bbconfig.channel(n=(192, 16, 24)), 3)
+    res = run_mixed(case_dir, bbconfig.channel(n=(192, 16, 24)), 3, x2=1)
     check_mixed(res, {"u": 1e-9, "v": 1e-7, "w": 1e-6, "p": 1e-6})
 
 
