@@ -62,8 +62,15 @@ MASK_BYTES = 1
 
 # HBM bytes per interior cell per launch measured by rocprofv3 --pmc
 # (FETCH_SIZE x 2 + WRITE_SIZE, the factor 2 from a copy kernel of known
-# size) on the 256^3 sedimentation workload; profiles/ holds the runs.
-TRAFFIC_PER_CELL = {}
+# size) on the 256^3 sedimentation workload with 500 spheres; the commands
+# and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
+# They apply to that workload (the mask byte included).
+TRAFFIC_PER_CELL = {
+    "spmv_search_kernel": 34.60,
+    "update_kernel": 50.32,
+    "spmv_search_kernel_f32": 17.10,
+    "update_kernel_f32": 25.32,
+}
 
 
 def parse():
@@ -239,7 +246,7 @@ def roofline_of(ktimes, cells, copy_gbs, mask):
         "frac": achieved / HBM_PEAK_GBS,
         "frac_of_measured_copy": achieved / copy_gbs,
         "measured_copy_gbs": copy_gbs,
-        "traffic": tpc * cells if tpc else None,
+        "traffic": tpc * cells if (tpc and mask) else None,
         "bytes_per_cell": bpc,
         "algorithmic_bytes_per_launch": bpc * cells,
         "avg_launch_us": avg_s * 1e6,

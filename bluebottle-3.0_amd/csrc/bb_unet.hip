@@ -734,7 +734,16 @@ static void free_acts(Acts &A)
   memset(&A, 0, sizeof(A));
 }
 
-#define CHUNK 4096L
+/* Voxel chunk of the weight- and bias-gradient partials: at most
+ * MAX_CHUNKS chunks of at least 256 voxels (a multiple of 16), so that a
+ * 64^3 training crop spreads over 512 workgroups per tile job. */
+#define MAX_CHUNKS 512L
+static long chunk_of(long nv)
+{
+  long c = (nv + MAX_CHUNKS - 1) / MAX_CHUNKS;
+  c = c < 256 ? 256 : c;
+  return (c + 15) / 16 * 16;
+}
 
 /* Activation buffers for a volume nx x ny x nz (each divisible by 4). */
 extern "C" int bb_unet_volume(int nx, int ny, int nz)
@@ -789,8 +798,7 @@ extern "C" int bb_unet_volume(int nx, int ny, int nz)
   A.g_d0 = (float *) dzalloc(n0 * 16 * 4);
   A.g_y = (float *) dzalloc(n0 * 4);
   A.g_in0 = (float *) dzalloc(n0 * 16 * 4);
-  long nchunks = (n0 + CHUNK - 1) / CHUNK;
-  A.part_n = nchunks * 27L * 64 * 96;
+  A.part_n = MAX_CHUNKS * 27L * 64 * 96;
   A.part = (float *) dzalloc(A.part_n * 4);
   A.loss_n = blocks(n0, 256);
   A.loss_part = (float *) dzalloc(A.loss_n * 4);
@@ -921,14 +929,15 @@ static void layer_bwd(int l, TV G, TV X, TV GX)
   Layer &Y = net->L[l];
   Acts &A = *net->A;
   long nv = (long) G.nx * G.ny * G.nz;
-  long nchunks = (nv + CHUNK - 1) / CHUNK;
+  long chunk = chunk_of(nv);
+  long nchunks = (nv + chunk - 1) / chunk;
   int jobs = Y.taps * (Y.coutp / 16) * (Y.cinp / 16);
   hipLaunchKernelGGL(conv_wgrad_kernel, dim3((unsigned) nchunks,
     (unsigned) ((jobs + 3) / 4)), dim3(256), 0, bb_stream, G, Y.cout, X,
-    Y.cin, Y.taps, Y.coutp, Y.cinp, CHUNK, A.part);
+    Y.cin, Y.taps, Y.coutp, Y.cinp, chunk, A.part);
   ew(sum_chunks_kernel, Y.nw, A.part, Y.nw, (int) nchunks, Y.gw);
   hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned) nchunks),
-    dim3(Y.coutp), 0, bb_stream, G, Y.cout, CHUNK, A.part);
+    dim3(Y.coutp), 0, bb_stream, G, Y.cout, chunk, A.part);
   ew(sum_chunks_kernel, (long) Y.coutp, A.part, (long) Y.coutp,
      (int) nchunks, Y.gb);
   if (GX.p) {
