@@ -484,7 +484,7 @@ template <int dir>
 __global__ void __launch_bounds__(64) part_bc_kernel(PDom P, double *vel,
   const int *__restrict__ phase, const int *__restrict__ flag,
   const part_struct *__restrict__ parts, double nu,
-  const int *__restrict__ list, int nlist)
+  const int *__restrict__ list, int nlist, double *__restrict__ vals)
 {
   const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
   long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -527,12 +527,28 @@ __global__ void __launch_bounds__(64) part_bc_kernel(PDom P, double *vel,
     val = Uz;
   }
   vel[C] = val;
+  vals[e] = val;
 }
 
-/* Face lists of the current cages (device), their lengths (host). */
+/* The values of the last part_bc_kernel pass again: a face's value depends
+ * only on the particle state, which the four BC calls of a Lamb pass
+ * share (face index t of the list is the face's array index). */
+__global__ void part_bc_apply_kernel(double *vel, const int *__restrict__ list,
+  int nlist, const double *__restrict__ vals)
+{
+  long e = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nlist) return;
+  vel[list[e]] = vals[e];
+}
+
+/* Face lists of the current cages (device), their lengths (host), and the
+ * BC values of the listed faces for the current particle state: valid
+ * until a function that changes the particles or the cages runs. */
 static int *_bc_list[3] = {NULL, NULL, NULL};
 static int *_bc_count = NULL;
 static int bc_n[3] = {0, 0, 0};
+static double *_bc_val[3] = {NULL, NULL, NULL};
+static int bc_vals_ok = 0;
 
 /* Called by cuda_build_cages (bb_bc.hip) once every flag is final. */
 void bb_particle_face_lists(void)
@@ -541,9 +557,12 @@ void bb_particle_face_lists(void)
   const long gn[3] = {ddom.fx.n, ddom.fy.n, ddom.fz.n};
   if (!_bc_count) {
     BB_CHECK(hipMalloc((void **) &_bc_count, 3 * sizeof(int)));
-    for (int d = 0; d < 3; d++)
+    for (int d = 0; d < 3; d++) {
       BB_CHECK(hipMalloc((void **) &_bc_list[d], gn[d] * sizeof(int)));
+      BB_CHECK(hipMalloc((void **) &_bc_val[d], gn[d] * sizeof(double)));
+    }
   }
+  bc_vals_ok = 0;
   BB_CHECK(hipMemsetAsync(_bc_count, 0, 3 * sizeof(int), bb_stream));
   hipLaunchKernelGGL(face_list_kernel<0>, dim3(nblk(gn[0], 256)), dim3(256),
     0, bb_stream, P, _phase, _flag_u, _bc_count, _bc_list[0]);
@@ -562,9 +581,12 @@ static void free_face_lists(void)
   if (!_bc_count) return;
   for (int d = 0; d < 3; d++) {
     BB_CHECK(hipFree(_bc_list[d]));
+    BB_CHECK(hipFree(_bc_val[d]));
     _bc_list[d] = NULL;
+    _bc_val[d] = NULL;
     bc_n[d] = 0;
   }
+  bc_vals_ok = 0;
   BB_CHECK(hipFree(_bc_count));
   _bc_count = NULL;
 }
@@ -572,16 +594,29 @@ static void free_face_lists(void)
 static void part_bc(double *u, double *v, double *w)
 {
   PDom P = pdom();
+  double *vel[3] = {u, v, w};
+  if (bc_vals_ok) {
+    for (int d = 0; d < 3; d++)
+      if (bc_n[d] > 0)
+        hipLaunchKernelGGL(part_bc_apply_kernel, dim3(nblk(bc_n[d], 256)),
+          dim3(256), 0, bb_stream, vel[d], _bc_list[d], bc_n[d], _bc_val[d]);
+    BB_LAUNCH_CHECK();
+    return;
+  }
   if (bc_n[0] > 0)
     hipLaunchKernelGGL(part_bc_kernel<0>, dim3(nblk(bc_n[0], 64)), dim3(64),
-      0, bb_stream, P, u, _phase, _flag_u, _parts, nu, _bc_list[0], bc_n[0]);
+      0, bb_stream, P, u, _phase, _flag_u, _parts, nu, _bc_list[0], bc_n[0],
+      _bc_val[0]);
   if (bc_n[1] > 0)
     hipLaunchKernelGGL(part_bc_kernel<1>, dim3(nblk(bc_n[1], 64)), dim3(64),
-      0, bb_stream, P, v, _phase, _flag_v, _parts, nu, _bc_list[1], bc_n[1]);
+      0, bb_stream, P, v, _phase, _flag_v, _parts, nu, _bc_list[1], bc_n[1],
+      _bc_val[1]);
   if (bc_n[2] > 0)
     hipLaunchKernelGGL(part_bc_kernel<2>, dim3(nblk(bc_n[2], 64)), dim3(64),
-      0, bb_stream, P, w, _phase, _flag_w, _parts, nu, _bc_list[2], bc_n[2]);
+      0, bb_stream, P, w, _phase, _flag_w, _parts, nu, _bc_list[2], bc_n[2],
+      _bc_val[2]);
   BB_LAUNCH_CHECK();
+  bc_vals_ok = 1;
 }
 
 extern "C" void cuda_part_BC(void)
@@ -709,6 +744,7 @@ extern "C" void cuda_parts_internal(void)
  * sit at theta = 0 + DIV_ST and PI - DIV_ST. */
 extern "C" void cuda_init_physalis(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (NPARTS == 0) return;
   int nn[NCOEFFS] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 4};
   int mm[NCOEFFS] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3, 0, 1, 2, 3, 4};
@@ -1027,6 +1063,7 @@ lamb_kernel(PDom P, part_struct *parts, const double *__restrict__ p,
 
 extern "C" void cuda_lamb(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) return;
   hipLaunchKernelGGL(lamb_kernel, dim3(nparts), dim3(64), 0, bb_stream,
     pdom(), _parts, _p, _u, _v, _w, rho_f, mu, nu, gradP, lamb_relax, s_beta,
@@ -1097,6 +1134,7 @@ lamb_max_store_kernel(int nparts, int ncoeffs_max, part_struct *parts,
 
 extern "C" void cuda_lamb_err(real *error, int *number)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) {
     /* The reference leaves its result undefined here; no particles means
      * the Lamb loop has nothing to converge. */
@@ -1493,6 +1531,7 @@ static void collide(void)
 
 extern "C" void cuda_move_parts_sub(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) return;
   collide();
   hipLaunchKernelGGL(move_parts_a_kernel, pgrid(), dim3(256), 0, bb_stream,
@@ -1503,6 +1542,7 @@ extern "C" void cuda_move_parts_sub(void)
 
 extern "C" void cuda_update_part_velocity(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) return;
   hipLaunchKernelGGL(move_parts_a_kernel, pgrid(), dim3(256), 0, bb_stream,
     nparts, _parts, dt, g, gradP, rho_f);
@@ -1511,6 +1551,7 @@ extern "C" void cuda_update_part_velocity(void)
 
 extern "C" void cuda_update_part_position(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) return;
   hipLaunchKernelGGL(move_parts_b_kernel, pgrid(), dim3(256), 0, bb_stream,
     nparts, _parts, dt, DOM, ddom.perx, ddom.pery, ddom.perz);
@@ -1520,6 +1561,7 @@ extern "C" void cuda_update_part_position(void)
 /* ------------------------------ memory ----------------------------------- */
 extern "C" void cuda_part_malloc_dev(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   int n = nparts > 0 ? nparts : 1;
   BB_CHECK(hipMalloc((void **) &_parts, n * sizeof(part_struct)));
   BB_CHECK(hipMalloc((void **) &_perr, n * sizeof(double)));
@@ -1535,6 +1577,7 @@ extern "C" void cuda_part_malloc_dev(void)
 
 extern "C" void cuda_part_push(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (nparts > 0)
     BB_CHECK(hipMemcpy(_parts, parts, nparts * sizeof(part_struct),
       hipMemcpyHostToDevice));
@@ -1551,6 +1594,7 @@ extern "C" void cuda_part_pull(void)
 
 extern "C" void cuda_part_free(void)
 {
+  bc_vals_ok = 0;                /* particle state changes */
   if (_parts) BB_CHECK(hipFree(_parts));
   if (_perr) BB_CHECK(hipFree(_perr));
   if (_pnum) BB_CHECK(hipFree(_pnum));
