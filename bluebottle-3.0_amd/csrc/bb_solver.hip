@@ -336,6 +336,12 @@ __device__ inline V ld32(const V *__restrict__ a, unsigned i)
   return *(const V *) ((const char *) a + i * (unsigned) sizeof(V));
 }
 
+template <typename V>
+__device__ inline void st32(V *__restrict__ a, unsigned i, V v)
+{
+  *(V *) ((char *) a + i * (unsigned) sizeof(V)) = v;
+}
+
 template <bool FIRST, typename V>
 __device__ inline PlaneRaw<V> plane_load(const DevDom &D,
   const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
@@ -568,10 +574,15 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   V prev = bprev[yl * LX + xl];
   V cur = bcur[yl * LX + xl];
   double acc = 0.0;
-  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
-  const long tstride = (long) ntx_ * nty_;
+  /* Tile partials of the column: each wave's tree sum goes to LDS at the
+   * end of each tile (no memory operation in that branch) and the row
+   * combination is stored after the loop; the same order as tile_partial. */
+  __shared__ double s_tp[8][TILE_Y];
+  int ti = 0;
   /* one plane of the column: store plane k + 1 (raw values in Rs, loaded
-   * two steps earlier), refill Rs with plane k + 3, compute plane k */
+   * two steps earlier), refill Rs with plane k + 3, compute plane k.
+   * Loads and stores are unconditional (lanes outside the domain store to
+   * the never-read corner ghost cell 0 and add nothing to the partial). */
   auto step = [&](int k, PlaneRawM<V> &Rs) {
     plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rs, beta, bnext, k + 1);
     Rs = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
@@ -579,16 +590,14 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
     __syncthreads();
     const V *P = bcur;
     V nxt = bnext[yl * LX + xl];
-    if (colok) {
-      V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
-                        P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
-                        nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
-                        wall_hi(k, D.nz, D.perz));
-      long c = gidx(C, x, j, k);
-      pnew[c] = cur;
-      Ap[c] = a;
-      acc += (double) cur * (double) a;
-    }
+    V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
+                      P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
+                      nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
+                      wall_hi(k, D.nz, D.perz));
+    unsigned c = colok ? (unsigned) gidx(C, x, j, k) : 0u;
+    st32(pnew, c, cur);
+    st32(Ap, c, a);
+    acc += colok ? (double) cur * (double) a : 0.0;
     prev = cur;
     cur = nxt;
     V *t3 = bprev;
@@ -596,16 +605,205 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
     bcur = bnext;
     bnext = t3;
     if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
-      tile_partial(acc, gpart, tile);   /* uniform over the workgroup */
+      double v = wave_tree_sum(acc);
+      if (lane == 0) s_tp[ti][wv] = v;
       acc = 0.0;
-      tile += tstride;
+      ti++;
     }
   };
-  /* unrolled by two so that Ra and Rb alternate without register copies
-   * (a copy of a register with a load in flight waits for that load) */
-  for (int k = k0; k <= kend; k += 2) {
+  /* pairs of planes, so that Ra and Rb alternate without register copies
+   * (a copy of a register with a load in flight waits for that load), and
+   * no step inside the loop is conditional; an odd last plane after it */
+  int k = k0;
+  for (; k + 1 <= kend; k += 2) {
     step(k, Ra);
-    if (k + 1 <= kend) step(k + 1, Rb);
+    step(k + 1, Rb);
+  }
+  if (k <= kend) step(k, Ra);
+  __syncthreads();
+  if (threadIdx.x < ti) {
+    const double *w = s_tp[threadIdx.x];
+    gpart[tx + (long) ntx_ * (ty + (long) nty_ * (tz0 + (int) threadIdx.x))] =
+      ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+  }
+}
+
+/* ----------------- search + SpMV without LDS planes ---------------------- */
+/* Each wavefront owns one row (64 cells along x) of a z-column of tiles and
+ * streams it through z on its own: no LDS planes and no barrier per plane.
+ * For plane k it forms p_new of its own row at plane k + 1 (the z
+ * neighbour) and of rows j - 1 and j + 1 at plane k (loaded again by the
+ * neighbouring wavefronts; the cache serves the repeats); the x neighbours
+ * come from the lanes on either side (cross-lane shuffles), and the two
+ * cells past the row's ends from one extra load.  Every p_new is formed
+ * with the same arithmetic as in search_spmv_group_kernel, so the results
+ * are the same bits.  Raw values are loaded two planes ahead into two
+ * alternating register sets. */
+
+/* Source offset and invM x/y index of cell (xg, jj) after the periodic
+ * wrap (cxy -1: a wall ghost or outside, value 0). */
+__device__ inline PlaneEntry cell_entry(const DevDom &D, int xg, int jj)
+{
+  PlaneEntry E;
+  int js = jj, xs = xg;
+  if (jj < 1) js = D.pery ? D.ny : 0;
+  else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
+  if (xg < 1) xs = D.perx ? D.nx : 0;
+  else if (xg > D.nx) xs = (D.perx && xg == D.nx + 1) ? 1 : 0;
+  bool ok = js && xs;
+  E.cxy = ok ? xs + js * D.cc.sy : -1;
+  E.wxy = ok ? wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery) : 0;
+  return E;
+}
+
+/* Raw r and p_old: own row at plane kc (c), rows j - 1 (s), j + 1 (n) and
+ * the end-of-row cell (h) at plane ks. */
+template <typename V> struct RowsRaw {
+  V rc, pc, rs, ps, rn, pn, rh, ph;
+};
+
+template <bool FIRST, typename V>
+__device__ inline void rows_ld(const PlaneEntry &E, unsigned off,
+  const V *__restrict__ r, const V *__restrict__ pold, V *rv, V *pv)
+{
+  unsigned c = (unsigned) (E.cxy >= 0 ? E.cxy : 0) + off;
+  *rv = ld32(r, c);
+  *pv = FIRST ? (V) 0 : ld32(pold, c);
+}
+
+template <bool FIRST, typename V>
+__device__ inline RowsRaw<V> rows_load(const DevDom &D, const PlaneEntry &EC,
+  const PlaneEntry &ES, const PlaneEntry &EN, const PlaneEntry &EH, int kc,
+  int ks, const V *__restrict__ r, const V *__restrict__ pold)
+{
+  RowsRaw<V> R;
+  int pc = plane_src(D, kc), psd = plane_src(D, ks);
+  unsigned oc = (unsigned) (pc ? pc : 1) * (unsigned) D.cc.sz;
+  unsigned os = (unsigned) (psd ? psd : 1) * (unsigned) D.cc.sz;
+  rows_ld<FIRST, V>(EC, oc, r, pold, &R.rc, &R.pc);
+  rows_ld<FIRST, V>(ES, os, r, pold, &R.rs, &R.ps);
+  rows_ld<FIRST, V>(EN, os, r, pold, &R.rn, &R.pn);
+  rows_ld<FIRST, V>(EH, os, r, pold, &R.rh, &R.ph);
+  return R;
+}
+
+/* p_new = z + beta p_old of one entry at plane kk (0 for a zero entry or a
+ * wall plane) */
+template <bool FIRST, typename V>
+__device__ inline V pnew_of(const DevDom &D, const V *tab,
+  const PlaneEntry &E, int kk, V rv, V pv, V beta)
+{
+  int ks = plane_src(D, kk);
+  int wz = 9 * wcount(ks, D.nz, D.perz);
+  V z = rv*tab[E.wxy + wz];
+  V v = FIRST ? z : z + beta*pv;
+  return (ks && E.cxy >= 0) ? v : (V) 0;
+}
+
+template <typename V>
+__device__ inline V shfl_up1(V v)
+{
+  return __shfl_up(v, 1, WAVE);
+}
+
+template <typename V>
+__device__ inline V shfl_down1(V v)
+{
+  return __shfl_down(v, 1, WAVE);
+}
+
+template <bool FIRST, typename V, bool MIXED>
+__global__ void __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(6, 8)))
+search_spmv_rows_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
+  int q, const double *__restrict__ rpart, long ntiles, int tzb,
+  const V *__restrict__ r, const V *__restrict__ pold,
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
+{
+  __shared__ V s_invm[27];
+  __shared__ double s_tp[8][TILE_Y];
+  if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
+  const Grid &C = D.cc;
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
+  const int ngz = (ntz_ + tzb - 1) / tzb;
+  long g = tile_of_block((long) ntx_ * nty_ * ngz);
+  int lane = threadIdx.x & (WAVE - 1);
+  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
+  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
+  int xb = tx * TILE_X, jb = ty * TILE_Y;
+  int k0 = tz0 * TILE_Z + 1;
+  int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
+  int x = xb + lane + 1, j = jb + wv + 1;
+  bool colok = (x <= D.nx) && (j <= D.ny);
+  PlaneEntry EC = cell_entry(D, x, j);
+  PlaneEntry ES = cell_entry(D, x, j - 1);
+  PlaneEntry EN = cell_entry(D, x, j + 1);
+  /* the cells past the row's ends: lane 0 needs x - 1, lane 63 x + 1; the
+   * other lanes repeat one of the two addresses (two cache lines a wave) */
+  PlaneEntry EH = cell_entry(D, lane < WAVE / 2 ? xb : xb + WAVE + 1, j);
+  /* raw values: centre planes k0 - 1 and k0; then sets A (centre k0 + 1,
+   * rows k0) and B (centre k0 + 2, rows k0 + 1) */
+  V rp0, pp0, rc0, pc0;
+  {
+    int pa = plane_src(D, k0 - 1);
+    unsigned oa = (unsigned) (pa ? pa : 1) * (unsigned) D.cc.sz;
+    rows_ld<FIRST, V>(EC, oa, r, pold, &rp0, &pp0);
+    unsigned ob = (unsigned) k0 * (unsigned) D.cc.sz;
+    rows_ld<FIRST, V>(EC, ob, r, pold, &rc0, &pc0);
+  }
+  RowsRaw<V> Ra = rows_load<FIRST, V>(D, EC, ES, EN, EH, k0 + 1, k0, r, pold);
+  RowsRaw<V> Rb = rows_load<FIRST, V>(D, EC, ES, EN, EH, min(k0 + 2, kend + 1),
+                                      min(k0 + 1, kend), r, pold);
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
+  V beta = (V) beta64;
+  __syncthreads();                      /* s_invm */
+  bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+  V prev = pnew_of<FIRST, V>(D, s_invm, EC, k0 - 1, rp0, pp0, beta);
+  V cur = pnew_of<FIRST, V>(D, s_invm, EC, k0, rc0, pc0, beta);
+  double acc = 0.0;
+  int ti = 0;
+  auto step = [&](int k, RowsRaw<V> &X) {
+    V nxt = pnew_of<FIRST, V>(D, s_invm, EC, k + 1, X.rc, X.pc, beta);
+    V Sv = pnew_of<FIRST, V>(D, s_invm, ES, k, X.rs, X.ps, beta);
+    V Nv = pnew_of<FIRST, V>(D, s_invm, EN, k, X.rn, X.pn, beta);
+    V Hv = pnew_of<FIRST, V>(D, s_invm, EH, k, X.rh, X.ph, beta);
+    X = rows_load<FIRST, V>(D, EC, ES, EN, EH, min(k + 3, kend + 1),
+                            min(k + 2, kend), r, pold);
+    V up = shfl_up1(cur), dn = shfl_down1(cur);
+    V Wv = lane == 0 ? Hv : up;
+    V Ev = lane == WAVE - 1 ? Hv : dn;
+    V a = neg_lap7<V>(K, cur, Wv, Ev, Sv, Nv, prev, nxt, wW, wE, wS, wN,
+                      wall_lo(k, D.perz), wall_hi(k, D.nz, D.perz));
+    unsigned c = colok ? (unsigned) gidx(C, x, j, k) : 0u;
+    st32(pnew, c, cur);
+    st32(Ap, c, a);
+    acc += colok ? (double) cur * (double) a : 0.0;
+    prev = cur;
+    cur = nxt;
+    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
+      double v = wave_tree_sum(acc);
+      if (lane == 0) s_tp[ti][wv] = v;
+      acc = 0.0;
+      ti++;
+    }
+  };
+  int k = k0;
+  for (; k + 1 <= kend; k += 2) {
+    step(k, Ra);
+    step(k + 1, Rb);
+  }
+  if (k <= kend) step(k, Ra);
+  __syncthreads();
+  if (threadIdx.x < ti) {
+    const double *w = s_tp[threadIdx.x];
+    gpart[tx + (long) ntx_ * (ty + (long) nty_ * (tz0 + (int) threadIdx.x))] =
+      ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
   }
 }
 
@@ -992,12 +1190,6 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
 /* (src/solver_kernel.cu:838-862): alpha = (r, z) / (p, Ap); phi += alpha p,
  * r -= alpha Ap, z = invM r + tile partials of (r, z).  In the mixed form
  * phi is the FP32 correction accumulated since the last FP64 correction. */
-template <typename V>
-__device__ inline void st32(V *__restrict__ a, unsigned i, V v)
-{
-  *(V *) ((char *) a + i * (unsigned) sizeof(V)) = v;
-}
-
 template <typename V, bool MASK>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
@@ -1475,7 +1667,9 @@ static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
 int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
-                                          tiles, 1: one tile per workgroup,
+                                          tiles, 3: the same columns with one
+                                          wavefront per row and no LDS
+                                          planes, 1: one tile per workgroup,
                                           0: direct loads */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_update_group = 0;               /* 1: update kernel over z-columns too */
@@ -1672,7 +1866,8 @@ double bb_pp_energy(const double *x)
  * least four workgroups per CU (1024 on MI355X), at most 4. */
 static int spmv_tzb(void)
 {
-  if (bb_spmv_tzb > 0) return bb_spmv_tzb;
+  /* the z-column kernels keep at most 8 tile partials per column */
+  if (bb_spmv_tzb > 0) return bb_spmv_tzb < 8 ? bb_spmv_tzb : 8;
   long nt = n_tiles(ddom);
   long t = nt / 1024;
   return t < 1 ? 1 : (t > 4 ? 4 : (int) t);
@@ -1701,6 +1896,12 @@ static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, const V *r, const V *pold, V *pnew, V *Ap)
 {
   int tzb = spmv_tzb();
+  if (bb_spmv_variant == 3) {
+    hipLaunchKernelGGL((search_spmv_rows_kernel<FIRST, V, MIXED>),
+      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+    return;
+  }
   hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED>),
     spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
     _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
@@ -1713,7 +1914,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   double *pnew = (q & 1) ? _p_q : _p_alt;
   double *pold = (q & 1) ? _p_alt : _p_q;
   bb_kt_begin(KT_SPMV);
-  if (bb_spmv_variant == 2) {
+  if (bb_spmv_variant >= 2) {
     if (q == 1)
       launch_group<true, double, false>(K, T, q, nt, _r_q, pold, pnew, _Apb_q);
     else
@@ -1919,7 +2120,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
     return;
   }
   bb_kt_begin(KT_SPMV32);
-  if (bb_spmv_variant == 2) {
+  if (bb_spmv_variant >= 2) {
     if (first)
       launch_group<true, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
     else

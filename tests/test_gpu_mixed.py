@@ -27,13 +27,15 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p")
 
 
-def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0):
+def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0,
+              variant=2):
     """x2 = 1: the FP32 kernels on x pairs (used when nx is even), 0: one
-    cell per lane (the default)."""
+    cell per lane (the default); variant: the search kernel (2: LDS planes,
+    3: one wavefront per row)."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = restart
     ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = x2
-    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 2
+    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = 16
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})
@@ -58,6 +60,7 @@ def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0):
             res["gpu"][f] = np.array(s.field(f))
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = 0
     ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = 0
+    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 2
     return res
 
 
@@ -90,12 +93,14 @@ def test_mixed_cavity_32_100_steps(case_dir):
     check_mixed(res, {"u": 1e-10, "v": 1e-9, "w": 1e-10, "p": 1e-9})
 
 
-@pytest.mark.parametrize("x2", [1, 0])
-def test_mixed_channel_64(case_dir, x2):
+@pytest.mark.parametrize("x2,variant", [(1, 2), (0, 2), (0, 3)])
+def test_mixed_channel_64(case_dir, x2, variant):
     """Periodic channel 64x32x32 (singular Neumann-periodic system), 5 steps,
     with the x-pair FP32 kernels (one partial 128-cell tile in x, periodic
-    wrap through the pair loader) and with one cell per lane."""
-    res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 5, x2=x2)
+    wrap through the pair loader) and with one cell per lane (both search
+    kernels)."""
+    res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 5, x2=x2,
+                    variant=variant)
     check_mixed(res, {"u": 1e-9, "v": 1e-7, "w": 1e-6, "p": 1e-6})
 
 

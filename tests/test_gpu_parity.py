@@ -22,6 +22,7 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
 
 def set_knobs(variant=2, tzb=0, batch=16, group=0):
     """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
+    3: the same z-columns with one wavefront per row and no LDS planes,
     1: one tile per workgroup, 0: direct neighbour loads; group = 1: the
     update kernel over the same z-columns."""
     lib = bbhip.load()
@@ -76,7 +77,8 @@ def test_cavity_32_full_100_steps(case_dir):
     assert out["iters_gpu"] == out["iters_orc"]
 
 
-@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 2),
+                                         (3, 1), (3, 2)])
 def test_cavity_bitwise(case_dir, variant, tzb):
     out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant, tzb)
     for f in FIELDS:
@@ -94,7 +96,7 @@ def test_channel_small(case_dir):
 
 @pytest.mark.parametrize("variant,tzb,group", [(0, 0, 0), (1, 0, 0),
                                                (2, 1, 0), (2, 4, 0),
-                                               (2, 2, 1)])
+                                               (2, 2, 1), (3, 4, 0)])
 def test_channel_bitwise(case_dir, variant, tzb, group):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
                    tzb, group=group)
@@ -106,7 +108,7 @@ def test_channel_bitwise(case_dir, variant, tzb, group):
 @pytest.mark.parametrize("variant,tzb,batch,group",
                          [(0, 0, 16, 0), (1, 0, 16, 0), (1, 0, 5, 0),
                           (2, 2, 1, 0), (2, 3, 16, 1), (2, 4, 5, 0),
-                          (2, 2, 16, 1)])
+                          (2, 2, 16, 1), (3, 3, 16, 0), (3, 2, 5, 0)])
 def test_ragged_sizes(case_dir, variant, tzb, batch, group):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
@@ -140,7 +142,8 @@ def test_periodic_all_and_wide_x(case_dir):
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-def test_periodic_all_25d(case_dir):
+@pytest.mark.parametrize("variant", [2, 3])
+def test_periodic_all_25d(case_dir, variant):
     per = ("PERIODIC",)
     vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
     kw = dict(n=(40, 12, 10), lo=(0.0, 0.0, 0.0), hi=(4.0, 1.2, 1.0),
@@ -148,7 +151,7 @@ def test_periodic_all_25d(case_dir):
               pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
               init_cond="QUIESCENT",
               extensions={"fixed_dt": 1e-3, "perturb": (6, 0.2)})
-    out = run_pair(case_dir, kw, 3, variant=2, tzb=2)
+    out = run_pair(case_dir, kw, 3, variant=variant, tzb=2)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
