@@ -251,17 +251,26 @@ def random_spheres(n, box, a, seed, gap=0.5, max_tries=2000000):
 
 def sediment(n=(64, 64, 64), nparts=8, a=6.0, seed=20260102, rho_ratio=2.0,
              order=4, nu=0.1, galileo=100.0, lamb_residual=1e-2,
-             lamb_max_iter=50, **over):
-    """Triply periodic sedimentation box [0, nx) x [0, ny) x [0, nz), dx = 1,
-    spheres of radius a at seeded random positions, gravity in -z set from
-    the Galileo number Ga = sqrt((rho_p/rho_f - 1) g d^3) / nu."""
+             lamb_max_iter=50, walls="", **over):
+    """Sedimentation box [0, nx) x [0, ny) x [0, nz), dx = 1, spheres of
+    radius a at seeded random positions, gravity in -z set from the Galileo
+    number Ga = sqrt((rho_p/rho_f - 1) g d^3) / nu.  Periodic in every
+    direction, except that the face pairs named in `walls` ("x", "y", "z")
+    are no-slip walls with a Neumann pressure condition."""
     per = ("PERIODIC",)
     vel = {c: {f: per for f in FACES} for c in "uvw"}
+    pbc = {f: per for f in FACES}
+    for ax, fs in (("x", "WE"), ("y", "SN"), ("z", "BT")):
+        if ax in walls:
+            for f in fs:
+                pbc[f] = ("NEUMANN", 0.0)
+                for c in "uvw":
+                    vel[c][f] = ("DIRICHLET", 0.0, 0.0)
     d = 2.0 * a
     gz = (galileo * nu) ** 2 / ((rho_ratio - 1.0) * d ** 3)
     kw = dict(n=n, lo=(0.0, 0.0, 0.0), hi=tuple(float(x) for x in n),
               rho_f=1.0, nu=nu, duration=1e9, CFL=0.5, pp_max_iter=100000,
-              pp_residual=1e-8, p_bc={f: per for f in FACES}, vel_bc=vel,
+              pp_residual=1e-8, p_bc=pbc, vel_bc=vel,
               init_cond="QUIESCENT", out_plane="HOMOGENEOUS",
               lamb_max_iter=lamb_max_iter, lamb_residual=lamb_residual,
               lamb_relax=1.0, lamb_cut=1e-4,

@@ -66,6 +66,44 @@ static PDom pdom(void)
   return P;
 }
 
+/* The six domain faces W E S N B T as the particle kernels see them
+ * (passed by value): the code a quadrature node beyond the face takes
+ * (src/physalis_kernel.cu:118-214: the _D code when any velocity component
+ * has a Dirichlet condition there, else the _N code when any has a Neumann
+ * condition, else -1), the face's Dirichlet velocities, whether its
+ * pressure condition is Neumann (wall contact forces act only then,
+ * src/particle_kernel.cu:2373) and its screen offset ds. */
+struct PWalls {
+  int code[6];
+  int pneu[6];
+  double vel[6][3];
+  double ds[6];
+};
+
+static PWalls pwalls(void)
+{
+  PWalls W;
+  const int vt[6][3] = {{bc.uW, bc.vW, bc.wW}, {bc.uE, bc.vE, bc.wE},
+                        {bc.uS, bc.vS, bc.wS}, {bc.uN, bc.vN, bc.wN},
+                        {bc.uB, bc.vB, bc.wB}, {bc.uT, bc.vT, bc.wT}};
+  const double vd[6][3] = {{bc.uWD, bc.vWD, bc.wWD}, {bc.uED, bc.vED, bc.wED},
+                           {bc.uSD, bc.vSD, bc.wSD}, {bc.uND, bc.vND, bc.wND},
+                           {bc.uBD, bc.vBD, bc.wBD}, {bc.uTD, bc.vTD, bc.wTD}};
+  const int pt[6] = {bc.pW, bc.pE, bc.pS, bc.pN, bc.pB, bc.pT};
+  const double ds[6] = {bc.dsW, bc.dsE, bc.dsS, bc.dsN, bc.dsB, bc.dsT};
+  for (int f = 0; f < 6; f++) {
+    bool d = vt[f][0] == DIRICHLET || vt[f][1] == DIRICHLET ||
+             vt[f][2] == DIRICHLET;
+    bool n = vt[f][0] == NEUMANN || vt[f][1] == NEUMANN ||
+             vt[f][2] == NEUMANN;
+    W.code[f] = d ? WEST_WALL_D - 10 * f : (n ? WEST_WALL_N - 10 * f : -1);
+    W.pneu[f] = pt[f] == NEUMANN;
+    for (int c = 0; c < 3; c++) W.vel[f][c] = vd[f][c];
+    W.ds[f] = ds[f];
+  }
+  return W;
+}
+
 /* Image position p + s L (s in {-1, 0, 1}), formed as the reference forms
  * the ghost copy's position. */
 __host__ __device__ inline double img(double p, int s, double l)
@@ -801,9 +839,9 @@ __device__ inline double node_coord(double xp, double X, double s,
   return x;
 }
 
-/* Rest-frame field values at the 26 nodes of each particle
- * (src/physalis_kernel.cu:118-640); node codes are all -1 in a periodic
- * domain.  One workgroup per particle, lanes 0..25 are the nodes. */
+/* Node codes and rest-frame field values at the 26 nodes of each particle
+ * (src/physalis_kernel.cu:118-640).  One workgroup per particle, lanes
+ * 0..25 are the nodes. */
 __device__ inline void cart2sphere(double u, double v, double w, double theta,
   double phi, double *ur, double *ut, double *up)
 {
@@ -834,7 +872,8 @@ struct NodeVals {
   double pp[NNODES], ur[NNODES], ut[NNODES], up[NNODES];
 };
 
-__device__ void node_values(const PDom &P, part_struct *Q, int node,
+__device__ void node_values(const PDom &P, const PWalls &W, part_struct *Q,
+  int node,
   const double *__restrict__ p, const double *__restrict__ u,
   const double *__restrict__ v, const double *__restrict__ w, double rho_f,
   double nu, gradP_struct gradP, double s_beta, double s_ref, g_struct g,
@@ -853,7 +892,6 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
   double a5 = Q->r * Q->r;
   a5 *= a5 * Q->r;
 
-  Q->nodes[node] = -1;
   double th = c_node_t[node], ph = c_node_p[node];
   double xp = Q->rs * sin(th) * cos(ph);
   double yp = Q->rs * sin(th) * sin(ph);
@@ -865,6 +903,25 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
   int oobi = ic < 1 || ic > P.nx, oobj = jc < 1 || jc > P.ny,
       oobk = kc < 1 || kc > P.nz;
   int oob = oobi || oobj || oobk;
+  /* node code (src/physalis_kernel.cu:137-213): the first face, in the
+   * order W E S N B T, that the node lies beyond; a periodic face has no
+   * code, and a periodic coordinate is already inside the domain */
+  {
+    const bool beyond[6] = {x - P.xs < 0, x - P.xe >= 0, y - P.ys < 0,
+                            y - P.ye >= 0, z - P.zs < 0, z - P.ze >= 0};
+    int code = -1;
+    for (int f = 0; f < 6; f++)
+      if (code == -1 && beyond[f]) code = W.code[f];
+    Q->nodes[node] = code;
+  }
+  const int ncode = Q->nodes[node];
+  const int wall = ncode < -1;
+  /* the value a node beyond a face takes: that face's Dirichlet velocity
+   * (0 at a Neumann face) in the rest frame, when the node is out of
+   * bounds in that one direction only (src/physalis_kernel.cu:437-460) */
+  const int wi = oobi * (1-oobj) * (1-oobk) * wall;
+  const int wj = (1-oobi) * oobj * (1-oobk) * wall;
+  const int wk = (1-oobi) * (1-oobj) * oobk * wall;
 
   /* pressure: cell centre (src/physalis_kernel.cu:305-364) */
   int i = ic, j = jc, k = kc;
@@ -899,7 +956,17 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
   double tmp_u = Q->u + ocrossr_x +
                  0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_x;
   uu -= tmp_u;
-  uu = (1-oobi) * (1-oobj) * (1-oobk) * (Q->nodes[node] == -1) * uu;
+  double uwi = (ncode == WEST_WALL_D) * W.vel[0][0]
+             + (ncode == EAST_WALL_D) * W.vel[1][0];
+  double uwj = (ncode == SOUTH_WALL_D) * W.vel[2][0]
+             + (ncode == NORTH_WALL_D) * W.vel[3][0];
+  double uwk = (ncode == BOTTOM_WALL_D) * W.vel[4][0]
+             + (ncode == TOP_WALL_D) * W.vel[5][0];
+  uwi -= tmp_u;
+  uwj -= tmp_u;
+  uwk -= tmp_u;
+  uu = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * uu
+     + wi * uwi + wj * uwj + wk * uwk;
 
   /* v: y-faces */
   i = ic; j = jc; k = kc;
@@ -914,7 +981,17 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
   double tmp_v = Q->v + ocrossr_y +
                  0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_y;
   vv -= tmp_v;
-  vv = (1-oobi) * (1-oobj) * (1-oobk) * (Q->nodes[node] == -1) * vv;
+  double vwi = (ncode == WEST_WALL_D) * W.vel[0][1]
+             + (ncode == EAST_WALL_D) * W.vel[1][1];
+  double vwj = (ncode == SOUTH_WALL_D) * W.vel[2][1]
+             + (ncode == NORTH_WALL_D) * W.vel[3][1];
+  double vwk = (ncode == BOTTOM_WALL_D) * W.vel[4][1]
+             + (ncode == TOP_WALL_D) * W.vel[5][1];
+  vwi -= tmp_v;
+  vwj -= tmp_v;
+  vwk -= tmp_v;
+  vv = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * vv
+     + wi * vwi + wj * vwj + wk * vwk;
 
   /* w: z-faces */
   i = ic; j = jc; k = kc;
@@ -929,7 +1006,17 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
   double tmp_w = Q->w + ocrossr_z +
                  0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_z;
   ww -= tmp_w;
-  ww = (1-oobi) * (1-oobj) * (1-oobk) * (Q->nodes[node] == -1) * ww;
+  double wwi = (ncode == WEST_WALL_D) * W.vel[0][2]
+             + (ncode == EAST_WALL_D) * W.vel[1][2];
+  double wwj = (ncode == SOUTH_WALL_D) * W.vel[2][2]
+             + (ncode == NORTH_WALL_D) * W.vel[3][2];
+  double wwk = (ncode == BOTTOM_WALL_D) * W.vel[4][2]
+             + (ncode == TOP_WALL_D) * W.vel[5][2];
+  wwi -= tmp_w;
+  wwj -= tmp_w;
+  wwk -= tmp_w;
+  ww = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * ww
+     + wi * wwi + wj * wwj + wk * wwk;
 
   out->pp[node] = pp;
   cart2sphere(uu, vv, ww, th, ph, &out->ur[node], &out->ut[node],
@@ -944,7 +1031,8 @@ __device__ void node_values(const PDom &P, part_struct *Q, int node,
 /* Quadrature (src/physalis_kernel.cu:642-722), Lamb coefficients
  * (:724-805) and forces (:807-841) for one particle per workgroup. */
 __global__ void __launch_bounds__(64)
-lamb_kernel(PDom P, part_struct *parts, const double *__restrict__ p,
+lamb_kernel(PDom P, PWalls W, part_struct *parts,
+  const double *__restrict__ p,
   const double *__restrict__ u, const double *__restrict__ v,
   const double *__restrict__ w, double rho_f, double mu, double nu,
   gradP_struct gradP, double relax, double s_beta, double s_ref, g_struct g)
@@ -954,7 +1042,8 @@ lamb_kernel(PDom P, part_struct *parts, const double *__restrict__ p,
   part_struct *Q = &parts[blockIdx.x];
   int t = threadIdx.x;
   if (t < NNODES)
-    node_values(P, Q, t, p, u, v, w, rho_f, nu, gradP, s_beta, s_ref, g, &N);
+    node_values(P, W, Q, t, p, u, v, w, rho_f, nu, gradP, s_beta, s_ref, g,
+                &N);
   __syncthreads();
 
   int ncoeff = Q->ncoeff;
@@ -1066,8 +1155,8 @@ extern "C" void cuda_lamb(void)
   bc_vals_ok = 0;                /* particle state changes */
   if (nparts == 0) return;
   hipLaunchKernelGGL(lamb_kernel, dim3(nparts), dim3(64), 0, bb_stream,
-    pdom(), _parts, _p, _u, _v, _w, rho_f, mu, nu, gradP, lamb_relax, s_beta,
-    s_ref, g);
+    pdom(), pwalls(), _parts, _p, _u, _v, _w, rho_f, mu, nu, gradP,
+    lamb_relax, s_beta, s_ref, g);
   BB_LAUNCH_CHECK();
 }
 
@@ -1426,6 +1515,147 @@ __global__ void spring_kernel(int nparts, part_struct *parts, dom_struct D)
   Q->kFz = - Q->spring_k * dz;
 }
 
+/* Lubrication and contact forces between particle i and the six domain
+ * faces (src/particle_kernel.cu:2347-3073), one thread per particle, faces
+ * in the order W E S N B T.  Face f has normal axis a = f / 2 and side
+ * sg = +1 (W, S, B: the particle must lie above the plane) or -1 (E, N,
+ * T); t1 < t2 are the two tangential axes.  Every face runs through the
+ * same code; the forms the reference writes out per face follow from
+ * e_a x w (the coupling of the rotation w to the tangential forces and of
+ * the tangential velocity to the torques) with these exceptions, kept as
+ * the reference has them:
+ *  - in contact, the wall's tangential velocity is always the south
+ *    face's Dirichlet value (bc.uSD, vSD, wSD);
+ *  - in contact with the W or E face, the z component of the contact
+ *    point velocity takes w_x (not w_y);
+ *  - in contact with the W face, the tangential spring takes the normal
+ *    stiffness k (the other faces the Mindlin stiffness kt);
+ *  - in contact with the E face, the z torque adds a variable that no
+ *    earlier statement of the thread sets except the W-face lubrication
+ *    branch, which stores 0: the term is 0 here.
+ * The contact-list bookkeeping (codes -10 - f in iSt) and ncoll_wall run
+ * for every face, periodic ones included; the forces are multiplied by
+ * (pressure condition of the face is Neumann). */
+#define CROSS_K ((const int[3][3]) {{0, 2, 1}, {2, 0, 0}, {1, 0, 0}})
+#define CROSS_S ((const int[3][3]) {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0}})
+
+__global__ void collision_walls_kernel(int nparts, part_struct *parts,
+  PWalls W, dom_struct D, double eps, double mu, double rhof, double nu,
+  int ilr, double dt)
+{
+  int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i >= nparts) return;
+  part_struct *Q = &parts[i];
+  const double ai = Q->r;
+  const double hN = ilr * Q->r;
+  const double pos[3] = {Q->x, Q->y, Q->z};
+  const double vel[3] = {Q->u, Q->v, Q->w};
+  const double vel0[3] = {Q->u0, Q->v0, Q->w0};
+  const double om[3] = {Q->ox, Q->oy, Q->oz};
+  const double om0[3] = {Q->ox0, Q->oy0, Q->oz0};
+  const double lo[3] = {D.xs, D.ys, D.zs}, hi[3] = {D.xe, D.ye, D.ze};
+  double iF[3] = {Q->iFx, Q->iFy, Q->iFz};
+  double iL[3] = {Q->iLx, Q->iLy, Q->iLz};
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    const int a = f / 2;
+    const double sg = (f & 1) ? -1. : 1.;
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    const int code = -10 - f;
+    double d = (f & 1) ? pos[a] - (hi[a] - W.ds[f])
+                       : pos[a] - (lo[a] + W.ds[f]);
+    double h = fabs(d) - ai;
+    const int isTrue = W.pneu[f];
+    const bool side = (f & 1) ? d < 0 : d > 0;
+    /* (e_a x v)_t = xs[t] * v[xk[t]] on the tangential axes t */
+    const int xk[3] = {CROSS_K[a][0], CROSS_K[a][1], CROSS_K[a][2]};
+    const int xs[3] = {CROSS_S[a][0], CROSS_S[a][1], CROSS_S[a][2]};
+    if (side && h < hN && h > 0) {
+      int q = 0;
+      while (q < MAX_NEIGHBORS && Q->iSt[q] != code) q++;
+      if (q < MAX_NEIGHBORS && Q->iSt[q] == code) {
+        Q->iSt[q] = -1;
+        Q->St[q] = 0.;
+      }
+      if (h < eps*Q->r) h = eps*Q->r;
+      double ah = ai/h - ai/hN;
+      double lnah = log(hN/h);
+      double F[3], L[3];
+      double Un = vel[a] - W.vel[f][a];
+      F[a] = -6.*PI*mu*ai*Un*ah;
+      L[a] = 0.;
+      const int ts[2] = {t1, t2};
+      for (int m = 0; m < 2; m++) {
+        int t = ts[m];
+        double Ut = vel[t] - W.vel[f][t];
+        F[t] = -6.*PI*mu*ai*Ut*8./15.*lnah;
+        F[t] += -sg * xs[t] * (8.*PI*mu*ai*ai*om[xk[t]]*1./10.*lnah);
+        double Utk = vel[xk[t]] - W.vel[f][xk[t]];
+        L[t] = sg * xs[t] * (8.*PI*mu*ai*ai*Utk*1./10.*lnah);
+        L[t] += -8.*PI*mu*ai*ai*ai*om[t]*2./5.*lnah;
+      }
+      for (int c = 0; c < 3; c++) {
+        iF[c] += isTrue * F[c];
+        iL[c] += isTrue * L[c];
+      }
+    }
+    if (side && h < 0) {
+      double Un = sg * (vel[a] - W.vel[f][a]);
+      double Ut[3];
+      Ut[t1] = 0.5*(vel[t1] + vel0[t1]) - W.vel[2][t1];
+      Ut[t2] = 0.5*(vel[t2] + vel0[t2]) - W.vel[2][t2];
+      int q = 0;
+      while (q < MAX_NEIGHBORS && Q->iSt[q] != code) q++;
+      if (q == MAX_NEIGHBORS) {
+        q = 0;
+        while (q < MAX_NEIGHBORS - 1 && Q->iSt[q] != -1) q++;
+        Q->iSt[q] = code;
+        Q->St[q] = 1./9.*Q->rho/rhof*2.*Q->r*fabs(Un)/nu;
+        Q->ncoll_wall += 1;
+      }
+      double omh[3];
+      for (int c = 0; c < 3; c++) omh[c] = 0.5*(om[c] + om0[c]);
+      /* contact point velocity sg (Ut + (a + h/2) e_a x w) */
+      double V[3];
+      for (int m = 0; m < 2; m++) {
+        int t = m ? t2 : t1;
+        int k = (a == 0 && t == 2) ? 0 : xk[t];
+        V[t] = sg * (Ut[t] + xs[t] * ((ai + 0.5*h)*omh[k]));
+      }
+      double Hi = 0.5*Q->E/(1.+Q->sigma);
+      double kt = 8./((1.-Q->sigma*Q->sigma)/Hi
+        +(1.-Q->sigma*Q->sigma)/Hi)/sqrt(1./ai)*sqrt(-h);
+      double k = 4./3./((1.-Q->sigma*Q->sigma)/Q->E
+        + (1.-Q->sigma*Q->sigma)/Q->E)/sqrt(1./ai);
+      double e = Q->e_dry + (1.+Q->e_dry)/Q->St[q]*log(1.e-4);
+      if (e < 0) e = 0;
+      double alpha = -2.263*pow(e,0.3948)+2.22;
+      double eta = alpha*sqrt(4./3.*PI*ai*ai*ai*Q->rho*k*sqrt(-h));
+      double Ks = f == 0 ? k : kt;
+      double Ft[3];
+      Ft[t1] = -Ks * (V[t1] * dt);
+      Ft[t2] = -Ks * (V[t2] * dt);
+      double Fm = sqrt(Ft[t1]*Ft[t1] + Ft[t2]*Ft[t2]);
+      double cf = Q->coeff_fric;
+      if (Fm > fabs(cf * (sqrt(-h*h*h)*k - eta*Un))) {
+        Ft[t1] = cf * (sqrt(-h*h*h)*k - eta*Un) * Ft[t1] / Fm;
+        Ft[t2] = cf * (sqrt(-h*h*h)*k - eta*Un) * Ft[t2] / Fm;
+      }
+      iF[a] += sg * (isTrue * (sqrt(-h*h*h)*k - eta*Un));
+      iF[t1] += isTrue * Ft[t1];
+      iF[t2] += isTrue * Ft[t2];
+      /* torque -sg (e_a x F_t) (a + h/2) on the tangential axes */
+      for (int m = 0; m < 2; m++) {
+        int t = m ? t2 : t1;
+        double Fk = (f == 1 && t == 2) ? 0. : Ft[xk[t]];
+        iL[t] += -sg * xs[t] * (isTrue * (ai+0.5*h) * Fk);
+      }
+    }
+  }
+  Q->iFx = iF[0]; Q->iFy = iF[1]; Q->iFz = iF[2];
+  Q->iLx = iL[0]; Q->iLy = iL[1]; Q->iLz = iL[2];
+}
+
 /* Accelerations and velocities (src/particle_kernel.cu:3368-3410). */
 __global__ void move_parts_a_kernel(int nparts, part_struct *parts, double dt,
   g_struct g, gradP_struct gradP, double rho_f)
@@ -1526,6 +1756,9 @@ static void collide(void)
   }
   hipLaunchKernelGGL(spring_kernel, pgrid(), dim3(256), 0, bb_stream, nparts,
     _parts, DOM);
+  hipLaunchKernelGGL(collision_walls_kernel, pgrid(), dim3(256), 0, bb_stream,
+    nparts, _parts, pwalls(), DOM, 0.01, mu, rho_f, nu,
+    (int) interaction_length_ratio, dt);
   BB_LAUNCH_CHECK();
 }
 

@@ -160,13 +160,6 @@ void parts_read_input(void)
       exit(EXIT_FAILURE);
     }
   }
-  /* Contact with walls (src/particle_kernel.cu:2347-3073) is not part of this
-   * build: particles need periodic boundaries in every direction. */
-  if (!(per[0] && per[1] && per[2])) {
-    fprintf(stderr, "bluebottle: particles need a triply periodic domain in "
-      "this build (wall collisions are not implemented)\n");
-    exit(EXIT_FAILURE);
-  }
 }
 
 void parts_init(void)

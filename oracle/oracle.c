@@ -44,6 +44,7 @@ static struct {
   int pp_max_iter;
   double pp_residual;
   int ptype[6];                       /* W E S N B T */
+  double ds[6];                       /* screen offsets of the p BCs */
   int vtype[3][6];                    /* [u,v,w][face] */
   double vDm[3][6], vDa[3][6], vD[3][6];
   double v_bc_tdelay, p_bc_tdelay, g_bc_tdelay;
@@ -156,7 +157,7 @@ static int parse(const char *dir)
   char key[32];
   for (int f6 = 0; f6 < 6; f6++) {
     snprintf(key, sizeof(key), "bc.p%c", F[f6]);
-    O.ptype[f6] = get_bc(key, NULL, NULL);
+    O.ptype[f6] = get_bc(key, &O.ds[f6], NULL);
     for (int c = 0; c < 3; c++) {
       snprintf(key, sizeof(key), "bc.%c%c", "uvw"[c], F[f6]);
       O.vtype[c][f6] = get_bc(key, &O.vDm[c][f6], &O.vDa[c][f6]);
@@ -440,13 +441,11 @@ static double dot_cc(const double *a, const double *b)
 }
 
 /* ---------------------------- flags / phase ----------------------------- */
-void orc_build_flags(void)
+/* Flags 0 on non-periodic external boundary faces over the whole ghosted
+ * plane (src/cuda_particle.cu:1606-1640, the last step of the flag build,
+ * after the particle cages). */
+static void o_flags_external(void)
 {
-  /* src/cuda_particle.cu:1516-1646 without particles: flags 1 everywhere,
-   * 0 on non-periodic external boundary faces over the whole ghosted plane. */
-  for (long c = 0; c < O.fx.n; c++) flu_[c] = 1;
-  for (long c = 0; c < O.fy.n; c++) flv_[c] = 1;
-  for (long c = 0; c < O.fz.n; c++) flw_[c] = 1;
   if (!O.per[0])
     for (int k = 0; k <= O.fx.nk + 1; k++) for (int j = 0; j <= O.fx.nj + 1; j++) {
       flu_[IDX(O.fx, 1, j, k)] = 0;
@@ -462,6 +461,16 @@ void orc_build_flags(void)
       flw_[IDX(O.fz, i, j, 1)] = 0;
       flw_[IDX(O.fz, i, j, O.fz.nk)] = 0;
     }
+}
+
+void orc_build_flags(void)
+{
+  /* src/cuda_particle.cu:1516-1646 without particles: flags 1 everywhere,
+   * then the external faces */
+  for (long c = 0; c < O.fx.n; c++) flu_[c] = 1;
+  for (long c = 0; c < O.fy.n; c++) flv_[c] = 1;
+  for (long c = 0; c < O.fz.n; c++) flw_[c] = 1;
+  o_flags_external();
 }
 
 /* ------------------------- boundary conditions -------------------------- */

@@ -59,6 +59,10 @@ int orc_nparts(void);
  * pnm_re pnm_im phinm_re phinm_im chinm_re chinm_im. */
 int orc_part_state(int n, double *out);
 int orc_part_set_coeffs(int n, const double *c90);
+int orc_part_set_motion(int n, const double *m15);
+int orc_part_contacts(int n, int *iSt, double *St, int *ncoll2);
+void orc_collide_once(void);
+void orc_move_parts_sub(void);
 /* Lamb velocity (U[0..2]) and n = 0 pressure term (U[3]) of particle n at
  * (x, y, z) relative to its centre. */
 void orc_lamb_eval(int n, double x, double y, double z, double *U);

@@ -62,6 +62,12 @@ def load():
         L.orc_lamb_eval.argtypes = [ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, D]
         L.orc_lamb_once.restype = ctypes.c_double
+        L.orc_part_set_motion.argtypes = [ctypes.c_int, D]
+        L.orc_part_set_motion.restype = ctypes.c_int
+        L.orc_part_contacts.argtypes = [ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), D,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.orc_part_contacts.restype = ctypes.c_int
         L.orc_lebedev.argtypes = [D, D, D]
         _lib = L
     return _lib
@@ -139,6 +145,36 @@ class Oracle:
         assert c.shape == (90,)
         self.lib.orc_part_set_coeffs(n, c.ctypes.data_as(
             ctypes.POINTER(ctypes.c_double)))
+
+    def set_motion(self, n, pos, vel, om, vel0=None, om0=None):
+        """Position, velocity and rotation of particle n (the previous
+        velocity and rotation default to the current ones)."""
+        vel0 = vel if vel0 is None else vel0
+        om0 = om if om0 is None else om0
+        m = np.ascontiguousarray(np.concatenate(
+            [pos, vel, om, vel0, om0]).astype(np.float64))
+        assert m.shape == (15,)
+        self.lib.orc_part_set_motion(n, m.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)))
+
+    def collide(self):
+        """Pair, spring and wall forces on the current particle state."""
+        self.lib.orc_collide_once()
+
+    def move_sub(self):
+        """Collisions, velocity update, collisions (one Lamb sub-step)."""
+        self.lib.orc_move_parts_sub()
+
+    def contacts(self, n):
+        """(iSt list, St list, ncoll_part, ncoll_wall) of particle n."""
+        i = np.zeros(12, dtype=np.int32)
+        s = np.zeros(12)
+        c = np.zeros(2, dtype=np.int32)
+        self.lib.orc_part_contacts(
+            n, i.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return list(i), list(s), int(c[0]), int(c[1])
 
     def lamb_eval(self, n, x, y, z):
         U = np.zeros(4)

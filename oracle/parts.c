@@ -24,7 +24,9 @@
  * sums all 26 nodes of a particle in one sequence (the reference sums per
  * copy and then adds the copies' partial sums; the two differ only in
  * rounding, and only for particles whose shell crosses a periodic face).
- * Wall collisions are not restated: particles need a triply periodic box.
+ * Walls: a node beyond a wall face takes that face's code and value
+ * (src/physalis_kernel.cu:118-214, 437-640), and the wall forces of
+ * src/particle_kernel.cu:2347-3073 are restated face by face below.
  */
 
 #define O_NNODES 26
@@ -49,7 +51,7 @@ typedef struct {
   double St[O_MAXNB];
   int iSt[O_MAXNB];
   double e_dry, coeff_fric, s;
-  int ncoll_part;
+  int ncoll_part, ncoll_wall;
   double pnm_re[O_NCOEFF], pnm_im[O_NCOEFF], phinm_re[O_NCOEFF],
          phinm_im[O_NCOEFF], chinm_re[O_NCOEFF], chinm_im[O_NCOEFF];
   double pnm_re0[O_NCOEFF], pnm_im0[O_NCOEFF], phinm_re0[O_NCOEFF],
@@ -108,10 +110,11 @@ static int parts_parse(const char *dir)
     ok &= fscanf(f, " rs %lf", &t_rs) == 1;
     ok &= fscanf(f, " s_order %d", &t_so) == 1;
     if (!ok) { fclose(f); return -5; }
-    /* periodic: a centre exactly at the end wraps to the start */
-    Q->x -= O.xl * (Q->x == O.xe);
-    Q->y -= O.yl * (Q->y == O.ye);
-    Q->z -= O.zl * (Q->z == O.ze);
+    /* periodic: a centre exactly at the end wraps to the start
+     * (src/particle.c:176-194) */
+    if (O.per[0]) Q->x -= O.xl * (Q->x == O.xe);
+    if (O.per[1]) Q->y -= O.yl * (Q->y == O.ye);
+    if (O.per[2]) Q->z -= O.zl * (Q->z == O.ze);
     Q->N = i;
     Q->rs = rs_r * Q->r;
     Q->s = 0.;
@@ -182,6 +185,8 @@ void orc_build_cages(void)
 {
   orc_build_flags();                    /* resets flags, external walls */
   if (NP_ == 0) return;
+  /* with particles the cage flags below rewrite every face; the external
+   * faces are set again after them */
   for (long c = 0; c < O.cc.n; c++) { phase_[c] = -1; shell_[c] = 1; }
   for (int pass = 0; pass < 2; pass++)
   for (int n = 0; n < NP_; n++)
@@ -239,6 +244,8 @@ void orc_build_cages(void)
     flw_[IDX(O.fz, i, j, k)] = 1 - 2*((phase_[B] < 0 && phase_[T] > -1) ||
       (phase_[B] > -1 && phase_[T] < 0) || (shell_[T] < 1 && shell_[B] < 1));
   }
+  /* the external faces last, as the reference orders the flag build */
+  o_flags_external();
 }
 
 /* ---------------------------- Lamb series -------------------------------- */
@@ -584,6 +591,35 @@ static void o_node(OP *Q, int node, double *pp, double *ur, double *ut,
   }
   int oobi = ic < 1 || ic > O.nx, oobj = jc < 1 || jc > O.ny,
       oobk = kc < 1 || kc > O.nz, oob = oobi || oobj || oobk;
+  /* node code (src/physalis_kernel.cu:137-213): faces in the order W E S N
+   * B T, the Dirichlet code before the Neumann one, the first that applies
+   * wins */
+  {
+    const double gap[6] = {x - O.xs, x - O.xe, y - O.ys, y - O.ye, z - O.zs,
+                           z - O.ze};
+    for (int f6 = 0; f6 < 6 && Q->nodes[node] == -1; f6++) {
+      int out = (f6 & 1) ? gap[f6] >= 0 : gap[f6] < 0;
+      if (!out) continue;
+      int anyD = 0, anyN = 0;
+      for (int c = 0; c < 3; c++) {
+        anyD |= O.vtype[c][f6] == DIRICHLET;
+        anyN |= O.vtype[c][f6] == NEUMANN;
+      }
+      if (anyD) Q->nodes[node] = -100 - 10 * f6;
+      else if (anyN) Q->nodes[node] = -101 - 10 * f6;
+    }
+  }
+  int code = Q->nodes[node];
+  /* Dirichlet velocity (component c) of the face a node lies beyond,
+   * along axis d (0: W/E, 1: S/N, 2: B/T) */
+  double wall[3][3];
+  for (int c = 0; c < 3; c++)
+    for (int d = 0; d < 3; d++)
+      wall[c][d] = (code == -100 - 20 * d) * O.vD[c][2 * d]
+                 + (code == -110 - 20 * d) * O.vD[c][2 * d + 1];
+  int only[3] = {oobi * (1-oobj) * (1-oobk) * (code < -1),
+                 (1-oobi) * oobj * (1-oobk) * (code < -1),
+                 (1-oobi) * (1-oobj) * oobk * (code < -1)};
   double ox = Q->ox, oy = Q->oy, oz = Q->oz;
   int i = o_clamp(ic, 1, O.nx), j = o_clamp(jc, 1, O.ny), k = o_clamp(kc, 1, O.nz);
   double P = o_interp(p_, &O.cc, i, j, k, x, y, z, (i - 0.5) * O.dx + O.xs,
@@ -604,21 +640,27 @@ static void o_node(OP *Q, int node, double *pp, double *ur, double *ut,
   i = o_clamp(ic, 1, O.nx + 1); j = o_clamp(jc, 1, O.ny); k = o_clamp(kc, 1, O.nz);
   double uu = o_interp(u_, &O.fx, i, j, k, x, y, z, (i - O_DOM_BUF) * O.dx + O.xs,
     (j - 0.5) * O.dy + O.ys, (k - 0.5) * O.dz + O.zs);
-  uu -= Q->u + (oy*zp - oz*yp) + 0.1 * inu * (rs5 - a5) * irs3 *
+  double tu = Q->u + (oy*zp - oz*yp) + 0.1 * inu * (rs5 - a5) * irs3 *
         (Q->oydot*zp - Q->ozdot*yp);
-  uu = in * uu;
+  uu -= tu;
+  uu = in * uu + only[0] * (wall[0][0] - tu) + only[1] * (wall[0][1] - tu)
+     + only[2] * (wall[0][2] - tu);
   i = o_clamp(ic, 1, O.nx); j = o_clamp(jc, 1, O.ny + 1); k = o_clamp(kc, 1, O.nz);
   double vv = o_interp(v_, &O.fy, i, j, k, x, y, z, (i - 0.5) * O.dx + O.xs,
     (j - O_DOM_BUF) * O.dy + O.ys, (k - 0.5) * O.dz + O.zs);
-  vv -= Q->v + -(ox*zp - oz*xp) + 0.1 * inu * (rs5 - a5) * irs3 *
+  double tv = Q->v + -(ox*zp - oz*xp) + 0.1 * inu * (rs5 - a5) * irs3 *
         -(Q->oxdot*zp - Q->ozdot*xp);
-  vv = in * vv;
+  vv -= tv;
+  vv = in * vv + only[0] * (wall[1][0] - tv) + only[1] * (wall[1][1] - tv)
+     + only[2] * (wall[1][2] - tv);
   i = o_clamp(ic, 1, O.nx); j = o_clamp(jc, 1, O.ny); k = o_clamp(kc, 1, O.nz + 1);
   double ww = o_interp(w_, &O.fz, i, j, k, x, y, z, (i - 0.5) * O.dx + O.xs,
     (j - 0.5) * O.dy + O.ys, (k - O_DOM_BUF) * O.dz + O.zs);
-  ww -= Q->w + (ox*yp - oy*xp) + 0.1 * inu * (rs5 - a5) * irs3 *
+  double tw = Q->w + (ox*yp - oy*xp) + 0.1 * inu * (rs5 - a5) * irs3 *
         (Q->oxdot*yp - Q->oydot*xp);
-  ww = in * ww;
+  ww -= tw;
+  ww = in * ww + only[0] * (wall[2][0] - tw) + only[1] * (wall[2][1] - tw)
+     + only[2] * (wall[2][2] - tw);
   *pp = P;
   double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
   *ur = st * (uu * cp + vv * sp) + ww * ct;
@@ -854,6 +896,264 @@ static void o_collide_pair(OP *Pi, const OP *Pj, double xj, double yj,
   Pi->iLx += Lox; Pi->iLy += Loy; Pi->iLz += Loz;
 }
 
+/* ------------------------- wall forces --------------------------------- */
+/* src/particle_kernel.cu:2347-3073, restated face by face.  Each face: the
+ * centre-to-plane distance d (planes at xs + dsW, xe - dsE, ...), gap
+ * h = |d| - a, lubrication for 0 < h < hN on the fluid side, contact for
+ * h < 0.  Contact-list codes -10 (W) .. -15 (T).  The forces count only when
+ * the face's pressure condition is Neumann; the bookkeeping runs always. */
+typedef struct {
+  double h, ah, lnah;
+} OLub;
+
+static int o_wall_lub_begin(OP *Q, int code, double h, double hN, OLub *L)
+{
+  int q = 0;
+  while (q < O_MAXNB && Q->iSt[q] != code) q++;
+  if (q < O_MAXNB && Q->iSt[q] == code) { Q->iSt[q] = -1; Q->St[q] = 0.; }
+  double eps = 0.01, ai = Q->r;
+  if (h < eps*Q->r) h = eps*Q->r;
+  L->h = h;
+  L->ah = ai/h - ai/hN;
+  L->lnah = log(hN/h);
+  return 0;
+}
+
+/* the contact slot of a face: found, or a new one (Stokes number from the
+ * normal approach speed) */
+static int o_wall_contact_slot(OP *Q, int code, double Un)
+{
+  int q = 0;
+  while (q < O_MAXNB && Q->iSt[q] != code) q++;
+  if (q == O_MAXNB) {
+    q = 0;
+    while (q < O_MAXNB - 1 && Q->iSt[q] != -1) q++;
+    Q->iSt[q] = code;
+    Q->St[q] = 1./9.*Q->rho/O.rho_f*2.*Q->r*fabs(Un)/O.nu;
+    Q->ncoll_wall += 1;
+  }
+  return q;
+}
+
+typedef struct {
+  double k, kt, eta, Fn;   /* Hertz and Mindlin stiffness, damping, normal */
+} OCon;
+
+static void o_wall_contact_coef(const OP *Q, int q, double h, double Un,
+  OCon *C)
+{
+  double ai = Q->r, sg = Q->sigma;
+  double Hi = 0.5*Q->E/(1.+sg);
+  C->kt = 8./((1.-sg*sg)/Hi + (1.-sg*sg)/Hi)/sqrt(1./ai)*sqrt(-h);
+  C->k = 4./3./((1.-sg*sg)/Q->E + (1.-sg*sg)/Q->E)/sqrt(1./ai);
+  double e = Q->e_dry + (1.+Q->e_dry)/Q->St[q]*log(1.e-4);
+  if (e < 0) e = 0;
+  double alpha = -2.263*pow(e,0.3948)+2.22;
+  C->eta = alpha*sqrt(4./3.*PI*ai*ai*ai*Q->rho*C->k*sqrt(-h));
+  C->Fn = sqrt(-h*h*h)*C->k - C->eta*Un;
+}
+
+/* Coulomb limit on the tangential pair (f1, f2) */
+static void o_fric(const OP *Q, double Fn, double *f1, double *f2)
+{
+  double Ft = sqrt(*f1 * *f1 + *f2 * *f2);
+  double cf = Q->coeff_fric;
+  if (Ft > fabs(cf * Fn)) {
+    *f1 = cf * Fn * *f1 / Ft;
+    *f2 = cf * Fn * *f2 / Ft;
+  }
+}
+
+static void o_collide_walls(void)
+{
+  const double mu = O.mu, dt = O.dt;
+  const double *uD = O.vD[0], *vD = O.vD[1], *wD = O.vD[2];
+  for (int i = 0; i < NP_; i++) {
+    OP *Q = &P_[i];
+    double ai = Q->r, hN = (int) ilr_ * Q->r;
+    double c6 = 6.*PI*mu*ai;            /* Stokes drag factor */
+    OLub L;
+    OCon C;
+    double d, h;
+    int on;
+    /* West: n = +x */
+    d = Q->x - (O.xs + O.ds[0]); h = fabs(d) - ai;
+    on = O.ptype[0] == NEUMANN;
+    if (d > 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -10, h, hN, &L);
+      double Un = Q->u - uD[0], Uy = Q->v - vD[0], Uz = Q->w - wD[0];
+      double Fx = -c6*Un*L.ah;
+      double Fy = -c6*Uy*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->oz*1./10.*L.lnah;
+      double Fz = -c6*Uz*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->oy*1./10.*L.lnah;
+      double Ly = -8.*PI*mu*ai*ai*Uz*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oy*2./5.*L.lnah;
+      double Lz = 8.*PI*mu*ai*ai*Uy*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oz*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLy += on * Ly; Q->iLz += on * Lz;
+    }
+    if (d > 0 && h < 0) {
+      double Un = Q->u - uD[0];
+      double Uy = 0.5*(Q->v + Q->v0) - vD[2], Uz = 0.5*(Q->w + Q->w0) - wD[2];
+      int q = o_wall_contact_slot(Q, -10, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oz = 0.5*(Q->oz + Q->oz0);
+      double lev = ai + 0.5*h;
+      double Vy = Uy - lev*oz, Vz = Uz + lev*ox;
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fy = -C.k * (Vy * dt), Fz = -C.k * (Vz * dt);
+      o_fric(Q, C.Fn, &Fy, &Fz);
+      Q->iFx += on * C.Fn; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLy += on * lev * Fz; Q->iLz -= on * lev * Fy;
+    }
+    /* East: n = -x */
+    d = Q->x - (O.xe - O.ds[1]); h = fabs(d) - ai;
+    on = O.ptype[1] == NEUMANN;
+    if (d < 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -11, h, hN, &L);
+      double Un = Q->u - uD[1], Uy = Q->v - vD[1], Uz = Q->w - wD[1];
+      double Fx = -c6*Un*L.ah;
+      double Fy = -c6*Uy*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->oz*1./10.*L.lnah;
+      double Fz = -c6*Uz*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->oy*1./10.*L.lnah;
+      double Ly = 8.*PI*mu*ai*ai*Uz*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oy*2./5.*L.lnah;
+      double Lz = -8.*PI*mu*ai*ai*Uy*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oz*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLy += on * Ly; Q->iLz += on * Lz;
+    }
+    if (d < 0 && h < 0) {
+      double Un = -(Q->u - uD[1]);
+      double Uy = 0.5*(Q->v + Q->v0) - vD[2], Uz = 0.5*(Q->w + Q->w0) - wD[2];
+      int q = o_wall_contact_slot(Q, -11, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oz = 0.5*(Q->oz + Q->oz0);
+      double lev = ai + 0.5*h;
+      double Vy = -(Uy - lev*oz), Vz = -(Uz + lev*ox);
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fy = -C.kt * (Vy * dt), Fz = -C.kt * (Vz * dt);
+      o_fric(Q, C.Fn, &Fy, &Fz);
+      Q->iFx -= on * C.Fn; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLy -= on * lev * Fz;
+      /* the z torque adds on * lev * 0 (see the note of this section) */
+    }
+    /* South: n = +y */
+    d = Q->y - (O.ys + O.ds[2]); h = fabs(d) - ai;
+    on = O.ptype[2] == NEUMANN;
+    if (d > 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -12, h, hN, &L);
+      double Un = Q->v - vD[2], Ux = Q->u - uD[2], Uz = Q->w - wD[2];
+      double Fy = -c6*Un*L.ah;
+      double Fx = -c6*Ux*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->oz*1./10.*L.lnah;
+      double Fz = -c6*Uz*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->ox*1./10.*L.lnah;
+      double Lx = 8.*PI*mu*ai*ai*Uz*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->ox*2./5.*L.lnah;
+      double Lz = -8.*PI*mu*ai*ai*Ux*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oz*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLx += on * Lx; Q->iLz += on * Lz;
+    }
+    if (d > 0 && h < 0) {
+      double Un = Q->v - vD[2];
+      double Ux = 0.5*(Q->u + Q->u0) - uD[2], Uz = 0.5*(Q->w + Q->w0) - wD[2];
+      int q = o_wall_contact_slot(Q, -12, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oz = 0.5*(Q->oz + Q->oz0);
+      double lev = ai + 0.5*h;
+      double Vx = Ux + lev*oz, Vz = Uz - lev*ox;
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fx = -C.kt * (Vx * dt), Fz = -C.kt * (Vz * dt);
+      o_fric(Q, C.Fn, &Fx, &Fz);
+      Q->iFx += on * Fx; Q->iFy += on * C.Fn; Q->iFz += on * Fz;
+      Q->iLx -= on * lev * Fz; Q->iLz += on * lev * Fx;
+    }
+    /* North: n = -y */
+    d = Q->y - (O.ye - O.ds[3]); h = fabs(d) - ai;
+    on = O.ptype[3] == NEUMANN;
+    if (d < 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -13, h, hN, &L);
+      double Un = Q->v - vD[3], Ux = Q->u - uD[3], Uz = Q->w - wD[3];
+      double Fy = -c6*Un*L.ah;
+      double Fx = -c6*Ux*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->oz*1./10.*L.lnah;
+      double Fz = -c6*Uz*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->ox*1./10.*L.lnah;
+      double Lx = -8.*PI*mu*ai*ai*Uz*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->ox*2./5.*L.lnah;
+      double Lz = 8.*PI*mu*ai*ai*Ux*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oz*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLx += on * Lx; Q->iLz += on * Lz;
+    }
+    if (d < 0 && h < 0) {
+      double Un = -(Q->v - vD[3]);
+      double Ux = 0.5*(Q->u + Q->u0) - uD[2], Uz = 0.5*(Q->w + Q->w0) - wD[2];
+      int q = o_wall_contact_slot(Q, -13, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oz = 0.5*(Q->oz + Q->oz0);
+      double lev = ai + 0.5*h;
+      double Vx = -(Ux + lev*oz), Vz = -(Uz - lev*ox);
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fx = -C.kt * (Vx * dt), Fz = -C.kt * (Vz * dt);
+      o_fric(Q, C.Fn, &Fx, &Fz);
+      Q->iFx += on * Fx; Q->iFy -= on * C.Fn; Q->iFz += on * Fz;
+      Q->iLx += on * lev * Fz; Q->iLz -= on * lev * Fx;
+    }
+    /* Bottom: n = +z */
+    d = Q->z - (O.zs + O.ds[4]); h = fabs(d) - ai;
+    on = O.ptype[4] == NEUMANN;
+    if (d > 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -14, h, hN, &L);
+      double Un = Q->w - wD[4], Ux = Q->u - uD[4], Uy = Q->v - vD[4];
+      double Fz = -c6*Un*L.ah;
+      double Fx = -c6*Ux*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->oy*1./10.*L.lnah;
+      double Fy = -c6*Uy*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->ox*1./10.*L.lnah;
+      double Lx = -8.*PI*mu*ai*ai*Uy*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->ox*2./5.*L.lnah;
+      double Ly = 8.*PI*mu*ai*ai*Ux*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oy*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLx += on * Lx; Q->iLy += on * Ly;
+    }
+    if (d > 0 && h < 0) {
+      double Un = Q->w - wD[4];
+      double Ux = 0.5*(Q->u + Q->u0) - uD[2], Uy = 0.5*(Q->v + Q->v0) - vD[2];
+      int q = o_wall_contact_slot(Q, -14, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oy = 0.5*(Q->oy + Q->oy0);
+      double lev = ai + 0.5*h;
+      double Vx = Ux - lev*oy, Vy = Uy + lev*ox;
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fx = -C.kt * (Vx * dt), Fy = -C.kt * (Vy * dt);
+      o_fric(Q, C.Fn, &Fx, &Fy);
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * C.Fn;
+      Q->iLx += on * lev * Fy; Q->iLy -= on * lev * Fx;
+    }
+    /* Top: n = -z */
+    d = Q->z - (O.ze - O.ds[5]); h = fabs(d) - ai;
+    on = O.ptype[5] == NEUMANN;
+    if (d < 0 && h < hN && h > 0) {
+      o_wall_lub_begin(Q, -15, h, hN, &L);
+      double Un = Q->w - wD[5], Ux = Q->u - uD[5], Uy = Q->v - vD[5];
+      double Fz = -c6*Un*L.ah;
+      double Fx = -c6*Ux*8./15.*L.lnah - 8.*PI*mu*ai*ai*Q->oy*1./10.*L.lnah;
+      double Fy = -c6*Uy*8./15.*L.lnah + 8.*PI*mu*ai*ai*Q->ox*1./10.*L.lnah;
+      double Lx = 8.*PI*mu*ai*ai*Uy*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->ox*2./5.*L.lnah;
+      double Ly = -8.*PI*mu*ai*ai*Ux*1./10.*L.lnah
+                  - 8.*PI*mu*ai*ai*ai*Q->oy*2./5.*L.lnah;
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz += on * Fz;
+      Q->iLx += on * Lx; Q->iLy += on * Ly;
+    }
+    if (d < 0 && h < 0) {
+      double Un = -(Q->w - wD[5]);
+      double Ux = 0.5*(Q->u + Q->u0) - uD[2], Uy = 0.5*(Q->v + Q->v0) - vD[2];
+      int q = o_wall_contact_slot(Q, -15, Un);
+      double ox = 0.5*(Q->ox + Q->ox0), oy = 0.5*(Q->oy + Q->oy0);
+      double lev = ai + 0.5*h;
+      double Vx = -(Ux - lev*oy), Vy = -(Uy + lev*ox);
+      o_wall_contact_coef(Q, q, h, Un, &C);
+      double Fx = -C.kt * (Vx * dt), Fy = -C.kt * (Vy * dt);
+      o_fric(Q, C.Fn, &Fx, &Fy);
+      Q->iFx += on * Fx; Q->iFy += on * Fy; Q->iFz -= on * C.Fn;
+      Q->iLx -= on * lev * Fy; Q->iLy += on * lev * Fx;
+    }
+  }
+}
+
 static void o_collide(void)
 {
   for (int i = 0; i < NP_; i++) {
@@ -908,6 +1208,7 @@ static void o_collide(void)
     Q->kFy = - Q->spring_k * (y - Q->spring_y - ly);
     Q->kFz = - Q->spring_k * (z - Q->spring_z - lz);
   }
+  o_collide_walls();
 }
 
 static void o_move_a(void)
@@ -1025,6 +1326,47 @@ int orc_part_set_coeffs(int n, const double *c)
                   Q->chinm_re, Q->chinm_im};
   for (int i = 0; i < 6; i++) memcpy(d[i], c + 15 * i, 15 * sizeof(double));
   return 0;
+}
+
+/* Set the motion of particle n: x y z u v w ox oy oz, then the previous
+ * u0 v0 w0 ox0 oy0 oz0 (15 values), for tests of the contact forces. */
+int orc_part_set_motion(int n, const double *m)
+{
+  if (n < 0 || n >= NP_) return -1;
+  OP *Q = &P_[n];
+  Q->x = m[0]; Q->y = m[1]; Q->z = m[2];
+  Q->u = m[3]; Q->v = m[4]; Q->w = m[5];
+  Q->ox = m[6]; Q->oy = m[7]; Q->oz = m[8];
+  Q->u0 = m[9]; Q->v0 = m[10]; Q->w0 = m[11];
+  Q->ox0 = m[12]; Q->oy0 = m[13]; Q->oz0 = m[14];
+  return 0;
+}
+
+/* Contact list of particle n: iSt and St (MAX_NEIGHBORS each), and the
+ * particle and wall collision counts. */
+int orc_part_contacts(int n, int *iSt, double *St, int *ncoll)
+{
+  if (n < 0 || n >= NP_) return -1;
+  const OP *Q = &P_[n];
+  memcpy(iSt, Q->iSt, sizeof(Q->iSt));
+  memcpy(St, Q->St, sizeof(Q->St));
+  ncoll[0] = Q->ncoll_part;
+  ncoll[1] = Q->ncoll_wall;
+  return 0;
+}
+
+/* One evaluation of the collision forces (pairs, springs, walls) on the
+ * current particle state. */
+void orc_collide_once(void)
+{
+  o_collide();
+}
+
+/* The sub-step particle update of the Lamb loop (collisions, velocities,
+ * collisions again: src/cuda_particle.cu:1335-1473). */
+void orc_move_parts_sub(void)
+{
+  o_move_sub();
 }
 
 /* Lamb velocity (Cartesian) of particle n's current coefficients at the
