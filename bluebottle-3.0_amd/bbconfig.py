@@ -28,15 +28,19 @@ def flow_config(*, n=(32, 32, 32), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                 gradP=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)),
                 g=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)),
                 p_bc_tdelay=0.0, g_bc_tdelay=0.0, v_bc_tdelay=0.0,
-                osci_f=0.0, extensions=None):
+                osci_f=0.0, scalar=None, extensions=None):
     """Return flow.config text.
 
     p_bc:   dict face -> ("PERIODIC",) or ("NEUMANN", 0.0)
     vel_bc: dict comp ('u','v','w') -> dict face -> ("PERIODIC",) |
             ("DIRICHLET", Dm, Da) | ("NEUMANN",)
+    scalar: None (SCALAR 0) or a dict with s_D, s_k, s_beta, s_ref,
+            init_s=(s_init, s_init_rand) and bc_s: face -> ("PERIODIC",) |
+            ("DIRICHLET", value) | ("NEUMANN", flux); SCALAR 1.
     extensions: dict of optional keys (fixed_dt, perturb=(seed, amp),
             pp_precision, pp_inner_tol, pp_warmstart).
     """
+    sc = scalar or {}
     p_bc = p_bc or {f: ("NEUMANN", 0.0) for f in FACES}
     vel_bc = vel_bc or {c: {f: ("DIRICHLET", 0.0, 0.0) for f in FACES}
                         for c in "uvw"}
@@ -53,12 +57,12 @@ def flow_config(*, n=(32, 32, 32), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
     L.append(f"rho_f {_fmt(float(rho_f))}")
     L.append(f"nu {_fmt(float(nu))}")
     for k in ("s_D", "s_k", "s_beta", "s_ref"):
-        L.append(f"{k} 0")
+        L.append(f"{k} {_fmt(float(sc.get(k, 0.0)))}")
     L.append("")
     L.append("SIMULATION PARAMETERS")
     L.append(f"duration {_fmt(float(duration))}")
     L.append(f"CFL {_fmt(float(CFL))}")
-    L.append("SCALAR 0")
+    L.append(f"SCALAR {1 if scalar else 0}")
     L.append(f"pp_max_iter {int(pp_max_iter)}")
     L.append(f"pp_residual {_fmt(float(pp_residual))}")
     L.append(f"lamb_max_iter {int(lamb_max_iter)}")
@@ -80,14 +84,18 @@ def flow_config(*, n=(32, 32, 32), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
             L.append(f"bc.{c}{f} " + " ".join(_fmt(x) for x in t))
     L.append("SCALAR")
     for f in FACES:
-        if p_bc[f][0] == "PERIODIC":
+        if "bc_s" in sc:
+            t = sc["bc_s"][f]
+            L.append(f"bc_s.s{f} " + " ".join(_fmt(x) for x in t))
+        elif p_bc[f][0] == "PERIODIC":
             L.append(f"bc_s.s{f} PERIODIC")
         else:
             L.append(f"bc_s.s{f} NEUMANN 0")
     L.append("")
     L.append("INITIAL CONDITION")
     L.append(f"init_cond {init_cond}")
-    L.append("init_s 0 0")
+    si = sc.get("init_s", (0.0, 0.0))
+    L.append(f"init_s {_fmt(float(si[0]))} {_fmt(float(si[1]))}")
     L.append("")
     L.append("SOLVABILITY ENFORCEMENT PLANE")
     L.append(f"out_plane {out_plane}")
@@ -280,3 +288,30 @@ def sediment(n=(64, 64, 64), nparts=8, a=6.0, seed=20260102, rho_ratio=2.0,
     parts = [dict(r=a, x=p[0], y=p[1], z=p[2], rho=rho_ratio, order=order)
              for p in pos]
     return kw, parts
+
+
+def rayleigh_benard(n=(256, 256, 128), Ra=1e6, Pr=1.0, amp=0.01, **over):
+    """Rayleigh-Benard convection: [0, 2) x [0, 2) x [0, 1], periodic in x
+    and y, no-slip walls at z = 0 and z = 1 with s = +0.5 and -0.5 there,
+    Boussinesq coupling with s_beta g = 1 (g = -z), nu = sqrt(Pr / Ra),
+    s_D = 1 / sqrt(Ra Pr); s starts at 0 plus the reference's seeded
+    uniform(-amp, amp) noise per cell."""
+    per = ("PERIODIC",)
+    wall = ("DIRICHLET", 0.0, 0.0)
+    vel = {c: {"W": per, "E": per, "S": per, "N": per, "B": wall, "T": wall}
+           for c in "uvw"}
+    pb = {"W": per, "E": per, "S": per, "N": per,
+          "B": ("NEUMANN", 0.0), "T": ("NEUMANN", 0.0)}
+    sbc = {"W": per, "E": per, "S": per, "N": per,
+           "B": ("DIRICHLET", 0.5), "T": ("DIRICHLET", -0.5)}
+    nu = math.sqrt(Pr / Ra)
+    scalar = dict(s_D=1.0 / math.sqrt(Ra * Pr), s_k=0.0, s_beta=1.0,
+                  s_ref=0.0, init_s=(0.0, amp), bc_s=sbc)
+    kw = dict(n=n, lo=(0.0, 0.0, 0.0), hi=(2.0, 2.0, 1.0), rho_f=1.0, nu=nu,
+              duration=1e9, CFL=0.5, pp_max_iter=100000, pp_residual=1e-8,
+              p_bc=pb, vel_bc=vel, init_cond="QUIESCENT",
+              out_plane="HOMOGENEOUS",
+              g=((0.0, 0.0), (0.0, 0.0), (-1.0, 0.0)), scalar=scalar,
+              extensions={})
+    kw.update(over)
+    return kw

@@ -26,6 +26,8 @@ _FIELDS = {
     "diff_v": GFY, "diff0_v": GFY, "f_y": GFY,
     "w": GFZ, "w0": GFZ, "w_star": GFZ, "conv_w": GFZ, "conv0_w": GFZ,
     "diff_w": GFZ, "diff0_w": GFZ, "f_z": GFZ,
+    "s": GCC, "s0": GCC, "s_conv": GCC, "s_conv0": GCC, "s_diff": GCC,
+    "s_diff0": GCC,
 }
 _IFIELDS = {"flag_u": GFX, "flag_v": GFY, "flag_w": GFZ, "phase": GCC,
             "phase_shell": GCC}
@@ -148,6 +150,8 @@ class Simulation:
 
     def pull(self):
         self.lib.cuda_dom_pull()
+        if ctypes.c_int.in_dll(self.lib, "SCALAR").value >= 1:
+            self.lib.cuda_scalar_pull()
 
     def push(self):
         self.lib.cuda_dom_push()

@@ -24,6 +24,7 @@
 
 extern double bb_force[3];
 int bb_history_rotated = 0;
+extern int bb_bous_on;                 /* bb_scalar.hip */
 
 /* ------------------------------ forcing --------------------------------- */
 extern "C" void cuda_compute_forcing(void)
@@ -56,14 +57,20 @@ extern "C" void cuda_compute_forcing(void)
     else g.z = delta*g.za;
   }
   /* f = 0; f += -gradP/rho_f  (forcing_reset_* then forcing_add_*_const) */
+  bb_bous_on = 0;                      /* f = 0 ... (forcing_reset_*) */
   bb_force[0] = 0. + (-gradP.x/rho_f);
   bb_force[1] = 0. + (-gradP.y/rho_f);
   bb_force[2] = 0. + (-gradP.z/rho_f);
 }
 
 /* --------------------------------- u* ----------------------------------- */
+/* f: the constant part of the forcing (-gradP / rho_f); with the
+ * Boussinesq term on (sb != NULL), f + bg (s_face - s_ref) with
+ * bg = -g s_beta and s_face the mean of the two cells beside the face
+ * (src/scalar_kernel.cu:160-206, added after the constant as the
+ * reference adds it). */
 struct StarArgs {
-  double rho_f, nu, dt, dt0, ab0, ab, ddx, ddy, ddz, f;
+  double rho_f, nu, dt, dt0, ab0, ab, ddx, ddy, ddz, f, bg, sref;
 };
 
 __global__ void __launch_bounds__(256)
@@ -71,6 +78,7 @@ u_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   const double *__restrict__ v0, const double *__restrict__ w0,
   const double *__restrict__ p, const double *__restrict__ conv0,
   const double *__restrict__ diff0, const int *__restrict__ phase,
+  const double *__restrict__ sb,
   double *__restrict__ ust, double *__restrict__ conv,
   double *__restrict__ diff)
 {
@@ -113,7 +121,9 @@ u_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   double sd = A.nu * (ddudxx + ddudyy + ddudzz);
   if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
   else s += sd;
-  s += A.f;
+  double fo = A.f;
+  if (sb) fo += A.bg * (0.5*(sb[cc - 1] + sb[cc]) - A.sref);
+  s += fo;
   s *= A.dt;
   s += u111;
   s *= (double) (phase[cc - 1] < 0 && phase[cc] < 0);
@@ -127,6 +137,7 @@ v_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   const double *__restrict__ v0, const double *__restrict__ w0,
   const double *__restrict__ p, const double *__restrict__ conv0,
   const double *__restrict__ diff0, const int *__restrict__ phase,
+  const double *__restrict__ sb,
   double *__restrict__ vst, double *__restrict__ conv,
   double *__restrict__ diff)
 {
@@ -169,7 +180,9 @@ v_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   double sd = A.nu * (ddvdxx + ddvdyy + ddvdzz);
   if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
   else s += sd;
-  s += A.f;
+  double fo = A.f;
+  if (sb) fo += A.bg * (0.5*(sb[cc - C.sy] + sb[cc]) - A.sref);
+  s += fo;
   s *= A.dt;
   s += v111;
   s *= (double) (phase[cc - C.sy] < 0 && phase[cc] < 0);
@@ -183,6 +196,7 @@ w_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   const double *__restrict__ v0, const double *__restrict__ w0,
   const double *__restrict__ p, const double *__restrict__ conv0,
   const double *__restrict__ diff0, const int *__restrict__ phase,
+  const double *__restrict__ sb,
   double *__restrict__ wst, double *__restrict__ conv,
   double *__restrict__ diff)
 {
@@ -225,7 +239,9 @@ w_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   double sd = A.nu * (ddwdxx + ddwdyy + ddwdzz);
   if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
   else s += sd;
-  s += A.f;
+  double fo = A.f;
+  if (sb) fo += A.bg * (0.5*(sb[cc - C.sz] + sb[cc]) - A.sref);
+  s += fo;
   s *= A.dt;
   s += w111;
   s *= (double) (phase[cc - C.sz] < 0 && phase[cc] < 0);
@@ -234,9 +250,11 @@ w_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   diff[c] = sd;
 }
 
-static StarArgs star_args(double f)
+static StarArgs star_args(double f, double gc)
 {
   StarArgs A;
+  A.bg = -gc * s_beta;
+  A.sref = s_ref;
   A.rho_f = rho_f; A.nu = nu; A.dt = dt; A.dt0 = dt0;
   A.ab0 = 0.5 * dt / dt0;
   A.ab = 1. + A.ab0;
@@ -252,15 +270,20 @@ extern "C" void cuda_U_star(void)
   /* src/cuda_bluebottle.cu:2095-2108 */
   dim3 bl(64, 4, 1);
   const DevDom &D = ddom;
+  const double *sb = (SCALAR >= 1 && bb_bous_on) ? _s : NULL;
   hipLaunchKernelGGL(u_star_kernel, dim3((D.fx.ni + 63) / 64,
-    (D.fx.nj + 3) / 4, D.fx.nk), bl, 0, bb_stream, D, star_args(bb_force[0]),
-    _u0, _v0, _w0, _p0, _conv0_u, _diff0_u, _phase, _u_star, _conv_u, _diff_u);
+    (D.fx.nj + 3) / 4, D.fx.nk), bl, 0, bb_stream, D,
+    star_args(bb_force[0], g.x),
+    _u0, _v0, _w0, _p0, _conv0_u, _diff0_u, _phase, sb, _u_star, _conv_u,
+    _diff_u);
   hipLaunchKernelGGL(v_star_kernel, dim3((D.fy.ni + 63) / 64,
-    (D.fy.nj + 3) / 4, D.fy.nk), bl, 0, bb_stream, D, star_args(bb_force[1]),
-    _u0, _v0, _w0, _p0, _conv0_v, _diff0_v, _phase, _v_star, _conv_v, _diff_v);
+    (D.fy.nj + 3) / 4, D.fy.nk), bl, 0, bb_stream, D, star_args(bb_force[1], g.y),
+    _u0, _v0, _w0, _p0, _conv0_v, _diff0_v, _phase, sb, _v_star, _conv_v,
+    _diff_v);
   hipLaunchKernelGGL(w_star_kernel, dim3((D.fz.ni + 63) / 64,
-    (D.fz.nj + 3) / 4, D.fz.nk), bl, 0, bb_stream, D, star_args(bb_force[2]),
-    _u0, _v0, _w0, _p0, _conv0_w, _diff0_w, _phase, _w_star, _conv_w, _diff_w);
+    (D.fz.nj + 3) / 4, D.fz.nk), bl, 0, bb_stream, D, star_args(bb_force[2], g.z),
+    _u0, _v0, _w0, _p0, _conv0_w, _diff0_w, _phase, sb, _w_star, _conv_w,
+    _diff_w);
   BB_LAUNCH_CHECK();
   bb_history_rotated = 0;
 }
@@ -549,9 +572,15 @@ extern "C" void cuda_find_dt(void)
   real v_max = sqrt(bb_host_scalars[1]);
   real w_max = sqrt(bb_host_scalars[2]);
   dom_struct *d = &dom[0];
-  dt = u_max/d->dx + 2.*nu/(d->dx * d->dx);
-  dt += v_max/d->dy + 2.*nu/(d->dy * d->dy);
-  dt += w_max/d->dz + 2.*nu/(d->dz * d->dz);
+  if (SCALAR >= 1 && s_D > nu) {
+    dt = u_max/d->dx + 2.*s_D/(d->dx * d->dx);
+    dt += v_max/d->dy + 2.*s_D/(d->dy * d->dy);
+    dt += w_max/d->dz + 2.*s_D/(d->dz * d->dz);
+  } else {
+    dt = u_max/d->dx + 2.*nu/(d->dx * d->dx);
+    dt += v_max/d->dy + 2.*nu/(d->dy * d->dy);
+    dt += w_max/d->dz + 2.*nu/(d->dz * d->dz);
+  }
   dt = CFL/dt;
 }
 

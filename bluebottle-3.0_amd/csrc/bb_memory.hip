@@ -352,3 +352,54 @@ extern "C" void cuda_dom_free(void)
   diff_u = diff_v = diff_w = diff0_u = diff0_v = diff0_w = NULL;
   flag_u = flag_v = flag_w = phase = phase_shell = NULL;
 }
+
+/* ------------------------- scalar fields -------------------------------- */
+/* src/scalar.c:41-53; src/cuda_scalar.cu:18-146.  Gcc arrays, host in the
+ * reference layout, device x-fastest. */
+real *s = NULL, *s0 = NULL, *s_conv = NULL, *s_conv0 = NULL, *s_diff = NULL,
+     *s_diff0 = NULL;
+real *_s = NULL, *_s0 = NULL, *_s_conv = NULL, *_s_conv0 = NULL,
+     *_s_diff = NULL, *_s_diff0 = NULL;
+
+extern "C" void cuda_scalar_malloc_host(void)
+{
+  size_t cc = dom[0].Gcc.s3b * sizeof(real);
+  s = (real *) host_alloc(cc); s0 = (real *) host_alloc(cc);
+  s_conv = (real *) host_alloc(cc); s_conv0 = (real *) host_alloc(cc);
+  s_diff = (real *) host_alloc(cc); s_diff0 = (real *) host_alloc(cc);
+}
+
+extern "C" void cuda_scalar_malloc_dev(void)
+{
+  size_t cc = ddom.cc.n * sizeof(real);
+  _s = (real *) dev_alloc(cc); _s0 = (real *) dev_alloc(cc);
+  _s_conv = (real *) dev_alloc(cc); _s_conv0 = (real *) dev_alloc(cc);
+  _s_diff = (real *) dev_alloc(cc); _s_diff0 = (real *) dev_alloc(cc);
+}
+
+extern "C" void cuda_scalar_push(void)
+{
+  BB_CHECK(hipDeviceSynchronize());
+  push_field(s, _s, L_GCC); push_field(s0, _s0, L_GCC);
+  push_field(s_conv, _s_conv, L_GCC); push_field(s_conv0, _s_conv0, L_GCC);
+  push_field(s_diff, _s_diff, L_GCC); push_field(s_diff0, _s_diff0, L_GCC);
+}
+
+extern "C" void cuda_scalar_pull(void)
+{
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(s, _s, L_GCC); pull_field(s0, _s0, L_GCC);
+  pull_field(s_conv, _s_conv, L_GCC); pull_field(s_conv0, _s_conv0, L_GCC);
+  pull_field(s_diff, _s_diff, L_GCC); pull_field(s_diff0, _s_diff0, L_GCC);
+}
+
+extern "C" void cuda_scalar_free(void)
+{
+  BB_CHECK(hipDeviceSynchronize());
+  real *dd[] = {_s, _s0, _s_conv, _s_conv0, _s_diff, _s_diff0};
+  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) dfree(dd[i]);
+  real *hh[] = {s, s0, s_conv, s_conv0, s_diff, s_diff0};
+  for (size_t i = 0; i < sizeof(hh) / sizeof(hh[0]); i++) hfree(hh[i]);
+  _s = _s0 = _s_conv = _s_conv0 = _s_diff = _s_diff0 = NULL;
+  s = s0 = s_conv = s_conv0 = s_diff = s_diff0 = NULL;
+}

@@ -364,11 +364,6 @@ void domain_read_input(void)
       "(Kp, Ki, Kd > 0) is not supported by this build\n");
     exit(EXIT_FAILURE);
   }
-  if (SCALAR >= 1) {
-    fprintf(stderr, "flow.config: scalar transport (SCALAR >= 1) is not "
-      "supported by this build\n");
-    exit(EXIT_FAILURE);
-  }
   if (init_cond == TURBULENT || init_cond == TURB_CHANNEL) {
     fprintf(stderr, "flow.config: init_cond TURBULENT/TURB_CHANNEL are not "
       "supported by this build; use CHANNEL with the `perturb` key\n");

@@ -63,8 +63,13 @@ int bluebottle_setup(void)
   domain_fill();
   cuda_dom_malloc_host();
   cuda_dom_malloc_dev();
+  if (SCALAR >= 1) {
+    cuda_scalar_malloc_host();
+    cuda_scalar_malloc_dev();
+  }
   cuda_blocks_init();
   domain_init_fields();
+  if (SCALAR >= 1) scalar_init_fields();
   NPARTS = 0;
   nparts = 0;
   if (parts_read_input) parts_read_input();
@@ -73,11 +78,21 @@ int bluebottle_setup(void)
   cuda_dom_push();
   cuda_exchange(_p, GCC);
   exchange_vel(_u, _v, _w);
+  if (SCALAR >= 1) {
+    cuda_scalar_push();
+    cuda_exchange(_s, GCC);
+  }
   cuda_build_cages();
   cuda_find_dt();
   cuda_dom_BC();
   cuda_exchange(_p, GCC);
   exchange_vel(_u, _v, _w);
+  if (SCALAR >= 1) {
+    cuda_scalar_BC(_s);
+    cuda_scalar_BC(_s0);
+    cuda_exchange(_s, GCC);
+    cuda_exchange(_s0, GCC);
+  }
   cuda_PP_init_jacobi_preconditioner();
   bb_warm_setup();
   return 0;
@@ -97,6 +112,7 @@ int bluebottle_step(void)
     fflush(stdout);
   }
   cuda_compute_forcing();
+  if (SCALAR >= 1) cuda_compute_boussinesq();
   compute_vel_BC();
 
   while (iter_err > lamb_residual) {
@@ -148,6 +164,28 @@ int bluebottle_step(void)
       printf("  Reached the max number of Lamb's iterations. Continuing...\n");
   }
 
+  /* scalar inner iteration (src/bluebottle.c:293-341) */
+  if (SCALAR >= 1) {
+    int scalar_iter = 0;
+    real scalar_iter_err = FLT_MAX;
+    while (scalar_iter_err > lamb_residual) {
+      cuda_scalar_BC(_s0);
+      cuda_scalar_part_BC(_s0);
+      cuda_exchange(_s0, GCC);
+      cuda_scalar_solve();
+      cuda_exchange(_s, GCC);
+      cuda_scalar_BC(_s);
+      cuda_scalar_part_fill();
+      cuda_exchange(_s, GCC);
+      cuda_scalar_lamb();
+      scalar_iter_err = cuda_scalar_lamb_err();
+      scalar_iter++;
+      if (scalar_iter == lamb_max_iter) break;
+    }
+    cuda_store_s();
+    cuda_scalar_update_part();
+  }
+
   if (NPARTS > 0) {
     cuda_move_parts_sub();
     cuda_update_part_velocity();
@@ -180,6 +218,7 @@ void bluebottle_teardown(void)
   cuda_block();
   bb_warm_teardown();
   if (bb_parts_teardown) bb_parts_teardown();
+  if (SCALAR >= 1) cuda_scalar_free();
   cuda_dom_free();
   domain_free();
 }

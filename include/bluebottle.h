@@ -183,8 +183,7 @@ extern real pid_int, pid_back, Kp, Ki, Kd;
 extern real turbA, turbl, turb_k0;
 extern long int cpumem, gpumem;
 
-/* Scalar parameters are parsed (src/domain.c:166-206) although scalar
- * transport itself is not part of this library. */
+/* Scalar transport parameters (src/domain.c:166-206). */
 extern int SCALAR;
 extern real s_D, s_k, s_beta, s_ref, lamb_cut_scalar, s_init, s_init_rand;
 
@@ -434,6 +433,26 @@ void cuda_update_part_position(void);        /* :1498-1513 + periodic wrap */
  * solve to the network while it is learning. */
 int cuda_PP_warmstart_infer(void);
 void cuda_PP_warmstart_train(void);
+/* Scalar transport (src/scalar.h, src/cuda_scalar.cu): host arrays in the
+ * reference layout and device arrays, both Gcc-sized (src/scalar.c:41-53). */
+extern real *s, *s0, *s_conv, *s_conv0, *s_diff, *s_diff0;
+extern real *_s, *_s0, *_s_conv, *_s_conv0, *_s_diff, *_s_diff0;
+void scalar_init_fields(void);               /* src/scalar.c:55-88 */
+void scalar_part_init(void);                 /* src/scalar.c:90-115 */
+void cuda_scalar_malloc_host(void);          /* src/cuda_scalar.cu:18-32 */
+void cuda_scalar_malloc_dev(void);           /* :35-61 */
+void cuda_scalar_push(void);                 /* :64-99 */
+void cuda_scalar_pull(void);                 /* :102-106 */
+void cuda_scalar_free(void);                 /* :129-146 */
+void cuda_compute_boussinesq(void);          /* :149-154 */
+void cuda_scalar_BC(real *array);            /* :157-237 */
+void cuda_scalar_part_BC(real *array);       /* :240-244 */
+void cuda_scalar_part_fill(void);            /* :247-250 */
+void cuda_scalar_solve(void);                /* :253-257 */
+void cuda_scalar_lamb(void);                 /* :866-911 */
+real cuda_scalar_lamb_err(void);             /* :914-946 */
+void cuda_store_s(void);                     /* :949-957 */
+void cuda_scalar_update_part(void);          /* :959-969 */
 /* Periodic ghost refresh of one field; replaces mpi_cuda_exchange_G*
  * (src/mpi_comm.c:257-405) on a single GPU.  `grid` is GCC/GFX/GFY/GFZ. */
 void cuda_exchange(real *array, int grid);

@@ -125,6 +125,7 @@ static int get_bc(const char *key, double *a, double *b)
   exit(1);
 }
 
+static void o_scalar_parse(void);
 static int O_lamb_max_iter;
 static double O_lamb_residual, O_lamb_relax, O_lamb_cut;
 static double O_sbeta, O_sref, O_g[3], O_gm[3], O_ga[3];
@@ -202,6 +203,7 @@ static int parse(const char *dir)
   if (find_line("perturb", rest))
     sscanf(rest, "%llu %lf", &O.perturb_seed, &O.perturb_amp);
   for (int d = 0; d < 3; d++) O.per[d] = (O.ptype[2 * d] == PERIODIC);
+  o_scalar_parse();
   return 0;
 }
 
@@ -826,6 +828,7 @@ void orc_solvability(void)
                for (int i = 1; i <= O.nx; i++)
 
 #include "parts.c"
+#include "scalar.c"
 
 void orc_jacobi(void)
 {
@@ -1122,9 +1125,10 @@ void orc_find_dt(void)
   if (O.fixed_dt > 0.) { O.dt = O.fixed_dt; return; }
   double um = sqrt(sqmax(u_, &O.fx)), vm = sqrt(sqmax(v_, &O.fy)),
          wm = sqrt(sqmax(w_, &O.fz));
-  O.dt = um/O.dx + 2.*O.nu/(O.dx * O.dx);
-  O.dt += vm/O.dy + 2.*O.nu/(O.dy * O.dy);
-  O.dt += wm/O.dz + 2.*O.nu/(O.dz * O.dz);
+  double k = (O_scalar >= 1 && O_sD > O.nu) ? O_sD : O.nu;
+  O.dt = um/O.dx + 2.*k/(O.dx * O.dx);
+  O.dt += vm/O.dy + 2.*k/(O.dy * O.dy);
+  O.dt += wm/O.dz + 2.*k/(O.dz * O.dz);
   O.dt = O.CFL/O.dt;
 }
 
@@ -1154,15 +1158,26 @@ int orc_setup(const char *dir)
   tbuf_ = dal((long) O.nx * O.ny * O.nz);
   for (long c = 0; c < O.cc.n; c++) { phase_[c] = -1; shell_[c] = -1; }
   init_fields();
+  if (O_scalar >= 1) {
+    o_scalar_alloc();
+    o_scalar_init();
+  }
   rc = parts_parse(dir);
   if (rc) return rc;
   o_init_physalis();
   /* src/bluebottle.c:113-135 */
   orc_exchange(p_, 0); exch_vel(u_, v_, w_);
+  if (O_scalar >= 1) orc_exchange(s_, 0);
   orc_build_cages();
   orc_find_dt();
   orc_bc_all();
   orc_exchange(p_, 0); exch_vel(u_, v_, w_);
+  if (O_scalar >= 1) {
+    o_scalar_bc(s_);
+    o_scalar_bc(s0_);
+    orc_exchange(s_, 0);
+    orc_exchange(s0_, 0);
+  }
   orc_jacobi();
   return 0;
 }
@@ -1174,6 +1189,7 @@ int orc_step(void)
   O.ttime += O.dt;
   O.stepnum++;
   orc_forcing();
+  if (O_scalar >= 1) o_boussinesq();
   orc_vel_bc_ramp();
   int q = 0, iter = 0;
   double err = DBL_MAX;
@@ -1214,6 +1230,7 @@ int orc_step(void)
   }
   O.lamb_iters = iter;
   O.lamb_err = err;
+  if (O_scalar >= 1) o_scalar_step();
   if (NP_ > 0) {
     o_move_sub();
     o_move_a();
@@ -1239,6 +1256,7 @@ void orc_teardown(void)
     w0_, ws_, cw_, c0w_, dw_, d0w_, fz_, tbuf_};
   for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); i++) free(d[i]);
   free(flu_); free(flv_); free(flw_); free(phase_); free(shell_);
+  if (s_) o_scalar_free();
   free(P_);
   P_ = NULL;
   NP_ = 0;
@@ -1254,7 +1272,8 @@ double *orc_field(const char *n)
     {"conv0_u", c0u_}, {"conv0_v", c0v_}, {"conv0_w", c0w_}, {"diff_u", du_},
     {"diff_v", dv_}, {"diff_w", dw_}, {"diff0_u", d0u_}, {"diff0_v", d0v_},
     {"diff0_w", d0w_}, {"rhs", rhs_}, {"invM", invM_}, {"f_x", fx_},
-    {"f_y", fy_}, {"f_z", fz_}};
+    {"f_y", fy_}, {"f_z", fz_}, {"s", s_}, {"s0", s0_}, {"s_conv", sc_},
+    {"s_diff", sd_}};
   for (size_t i = 0; i < sizeof(t) / sizeof(t[0]); i++)
     if (strcmp(t[i].n, n) == 0) return t[i].a;
   return NULL;

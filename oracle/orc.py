@@ -21,7 +21,8 @@ for _c, _g in (("u", GFX), ("v", GFY), ("w", GFZ)):
     for _pre in ("conv_", "conv0_", "diff_", "diff0_"):
         _GRID[_pre + _c] = _g
     _GRID["flag_" + _c] = _g
-_GRID.update({"f_x": GFX, "f_y": GFY, "f_z": GFZ})
+_GRID.update({"f_x": GFX, "f_y": GFY, "f_z": GFZ, "s": GCC, "s0": GCC,
+              "s_conv": GCC, "s_diff": GCC})
 
 _lib = None
 

@@ -186,6 +186,27 @@ def test_rejects_out_of_scope_options(tmp_path):
     open(path, "w").write(good.replace("Kp 0", "Kp 1.5"))
     r = run_parse(str(tmp_path))
     assert r.returncode != 0 and "PID" in r.stderr
+    # scalar transport parses; with particles it is refused when
+    # part.config is read
     open(path, "w").write(good.replace("SCALAR 0", "SCALAR 1"))
     r = run_parse(str(tmp_path))
-    assert r.returncode != 0 and "SCALAR" in r.stderr
+    assert r.returncode == 0, r.stderr
+    kw, parts = bbconfig.sediment(n=(16, 16, 16), nparts=1, a=3.0, seed=1)
+    bbconfig.write_case(str(tmp_path), **kw)
+    bbconfig.write_parts(str(tmp_path), parts)
+    good = open(path).read()
+    open(path, "w").write(good.replace("SCALAR 0", "SCALAR 1"))
+    code = textwrap.dedent(f"""
+        import os, sys
+        sys.path.insert(0, {os.path.join(ROOT, 'bluebottle-3.0_amd')!r})
+        import bbhip
+        lib = bbhip.load()
+        os.chdir({str(tmp_path)!r})
+        lib.domain_read_input()
+        lib.domain_fill()
+        lib.parts_read_input()
+        print("OK")
+    """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "SCALAR" in r.stderr, r.stderr
