@@ -197,7 +197,8 @@ def part_config(parts, l_over_a=1.0):
     """Return part.config text in the grammar of src/particle.c:66-391.
 
     parts: list of dicts with keys r, x, y, z and optional u v w, rho, E,
-    sigma, e_dry, coeff_fric, order, rs_r, translating, rotating.
+    sigma, e_dry, coeff_fric, order, rs_r, translating, rotating, and the
+    scalar keys s, update, cp, srs_r (scalar node radius / r), s_order.
     """
     L = [f"n {len(parts)}", f"(l/a) {_fmt(float(l_over_a))}"]
     for p in parts:
@@ -222,11 +223,11 @@ def part_config(parts, l_over_a=1.0):
         L.append("spring_l 0")
         L.append(f"translating {int(p.get('translating', 1))}")
         L.append(f"rotating {int(p.get('rotating', 1))}")
-        L.append("s 0")
-        L.append("update 0")
-        L.append("cp 0")
-        L.append("rs 0")
-        L.append("s_order 0")
+        L.append(f"s {_fmt(g('s', 0))}")
+        L.append(f"update {int(p.get('update', 0))}")
+        L.append(f"cp {_fmt(g('cp', 0))}")
+        L.append(f"rs {_fmt(g('srs_r', 0))}")
+        L.append(f"s_order {int(p.get('s_order', 0))}")
     return "\n".join(L) + "\n"
 
 

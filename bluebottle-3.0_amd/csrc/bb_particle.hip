@@ -1243,6 +1243,300 @@ extern "C" void cuda_lamb_err(real *error, int *number)
   *number = (int) err_host[1];
 }
 
+/* ---------------------------- particle heat ----------------------------- */
+/* The scalar (heat) exchange of each particle with the fluid
+ * (src/scalar_kernel.cu:303-577, 1143-1360; src/cuda_scalar.cu:866-969):
+ * the scalar on a sphere of radius srs at the 26 nodes, its spherical
+ * harmonic coefficients anm (n <= s_order, -n <= m <= n), the heat flux
+ * q = 2 sqrt(pi) a a_00, the analytic scalar in particle cells, and the
+ * particle's scalar from q.  The reference's Legendre table has its m < 0
+ * cases commented out and returns 0 for them through its switch
+ * (src/particle_kernel.cu:1928-1990): every m < 0 term is 0, here as
+ * there (Pnm returns 0 for m < 0). */
+__constant__ int c_snn[S_MAX_COEFFS] = {0, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3,
+  3, 3, 3, 3, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+__constant__ int c_smm[S_MAX_COEFFS] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -3, -2,
+  -1, 0, 1, 2, 3, -4, -3, -2, -1, 0, 1, 2, 3, 4};
+
+/* the scalar BC types and values of the six faces */
+struct PScalarBC {
+  int type[6];
+  double val[6];
+};
+
+static PScalarBC pscalar_bc(void)
+{
+  PScalarBC B;
+  const int t[6] = {bc_s.sW, bc_s.sE, bc_s.sS, bc_s.sN, bc_s.sB, bc_s.sT};
+  const double d[6] = {bc_s.sWD, bc_s.sED, bc_s.sSD, bc_s.sND, bc_s.sBD,
+                       bc_s.sTD};
+  for (int f = 0; f < 6; f++) {
+    B.type[f] = t[f];
+    B.val[f] = d[f];
+  }
+  return B;
+}
+
+/* s - s_p at one node (src/scalar_kernel.cu:303-527) */
+__device__ double scalar_node_value(const PDom &P, const PScalarBC &B,
+  part_struct *Q, int node, const double *__restrict__ s)
+{
+  double ddx = 1. / P.dx, ddy = 1. / P.dy, ddz = 1. / P.dz;
+  double th = c_node_t[node], ph = c_node_p[node];
+  double xp = Q->srs * sin(th) * cos(ph);
+  double yp = Q->srs * sin(th) * sin(ph);
+  double zp = Q->srs * cos(th);
+  int ic, jc, kc;
+  double x = node_coord(xp, Q->x, P.xs, P.dx, P.xl, P.nx, P.perx, &ic);
+  double y = node_coord(yp, Q->y, P.ys, P.dy, P.yl, P.ny, P.pery, &jc);
+  double z = node_coord(zp, Q->z, P.zs, P.dz, P.zl, P.nz, P.perz, &kc);
+  const bool beyond[6] = {x - P.xs < 0, x - P.xe >= 0, y - P.ys < 0,
+                          y - P.ye >= 0, z - P.zs < 0, z - P.ze >= 0};
+  int code = -1;
+  for (int f = 0; f < 6; f++) {
+    if (code != -1 || !beyond[f]) continue;
+    if (B.type[f] == DIRICHLET) code = WEST_WALL_D - 10 * f;
+    else if (B.type[f] == NEUMANN) code = WEST_WALL_N - 10 * f;
+  }
+  Q->nodes[node] = code;
+  int oobi = ic < 1 || ic > P.nx, oobj = jc < 1 || jc > P.ny,
+      oobk = kc < 1 || kc > P.nz;
+  int i = ic, j = jc, k = kc;
+  i += (1 - i) * (i < 1); j += (1 - j) * (j < 1); k += (1 - k) * (k < 1);
+  i += (P.nx - i) * (i > P.nx); j += (P.ny - j) * (j > P.ny);
+  k += (P.nz - k) * (k > P.nz);
+  long C = gidx(P.cc, i, j, k);
+  double ss = node_interp(s, P.cc, i, j, k, x, y, z,
+    (i - 0.5) * P.dx + P.xs, (j - 0.5) * P.dy + P.ys,
+    (k - 0.5) * P.dz + P.zs, ddx, ddy, ddz) - Q->s;
+  double swi = (code == WEST_WALL_D) * B.val[0]
+             + (code == EAST_WALL_D) * B.val[1]
+             + (code == WEST_WALL_N) * s[C - 1]
+             + (code == EAST_WALL_N) * s[C + 1]
+             - Q->s;
+  double swj = (code == SOUTH_WALL_D) * B.val[2]
+             + (code == NORTH_WALL_D) * B.val[3]
+             + (code == SOUTH_WALL_N) * s[C - P.cc.sy]
+             + (code == NORTH_WALL_N) * s[C + P.cc.sy]
+             - Q->s;
+  double swk = (code == BOTTOM_WALL_D) * B.val[4]
+             + (code == TOP_WALL_D) * B.val[5]
+             + (code == BOTTOM_WALL_N) * s[C - P.cc.sz]
+             + (code == TOP_WALL_N) * s[C + P.cc.sz]
+             - Q->s;
+  return (1-oobi) * (1-oobj) * (1-oobk) * (code == -1) * ss
+       + oobi * (1-oobj) * (1-oobk) * (code < -1) * swi
+       + (1-oobi) * oobj * (1-oobk) * (code < -1) * swj
+       + (1-oobi) * (1-oobj) * oobk * (code < -1) * swk;
+}
+
+/* Node values, quadrature (nodes in the reference's order and weights),
+ * anm = <Y_nm, s - s_p> / (rsa^n - ars^(n+1)) and q, one workgroup per
+ * particle (src/scalar_kernel.cu:529-577, 1247-1272). */
+__global__ void __launch_bounds__(64)
+scalar_lamb_kernel(PDom P, PScalarBC B, part_struct *parts,
+  const double *__restrict__ s)
+{
+  __shared__ double ss[NNODES];
+  part_struct *Q = &parts[blockIdx.x];
+  int t = threadIdx.x;
+  if (t < NNODES) ss[t] = scalar_node_value(P, B, Q, t, s);
+  __syncthreads();
+  int sncoeff = Q->sncoeff;
+  double ars = Q->r / Q->srs;
+  double rsa = Q->srs / Q->r;
+  if (t < sncoeff) {
+    int n = c_snn[t], m = c_smm[t];
+    double re = 0., im = 0.;
+    for (int node = 0; node < NNODES; node++) {
+      double theta = c_node_t[node];
+      double phi = c_node_p[node];
+      double N_nm = Nnm(n, m);
+      double P_nm = m < 0 ? 0. : Pnm(n, m, theta);
+      double Ire = N_nm*P_nm*ss[node]*cos(m * phi);
+      double Iim = -N_nm*P_nm*ss[node]*sin(m * phi);
+      if (node == 0) { re = Ire * c_A1; im = Iim * c_A1; }
+      else if (node < 6) { re += c_A1 * Ire; im += c_A1 * Iim; }
+      else if (node < 18) { re += c_A2 * Ire; im += c_A2 * Iim; }
+      else { re += c_A3 * Ire; im += c_A3 * Iim; }
+    }
+    double A = pow(rsa, n) - pow(ars, n+1.);
+    Q->anm_re[t] = re / A;
+    Q->anm_im[t] = im / A;
+  }
+  __syncthreads();
+  if (t == 0 && sncoeff > 0) Q->q = 2. * sqrt(PI) * Q->r * Q->anm_re[0];
+}
+
+/* Largest relative change of each particle's anm (SNSP = 2 arrays), then
+ * the maximum over particles (thrust::reduce with 0 as the start) and
+ * anm0 <- anm (src/scalar_kernel.cu:1274-1349). */
+__global__ void __launch_bounds__(64)
+scalar_error_kernel(double lamb_cut_s, int s_ncoeffs_max, part_struct *parts,
+  double *perr)
+{
+  __shared__ double s_max[S_MAX_COEFFS];
+  part_struct *Q = &parts[blockIdx.x];
+  int coeff = threadIdx.x;
+  if (coeff < s_ncoeffs_max) {
+    const double *cur[2] = {Q->anm_re, Q->anm_im};
+    const double *old[2] = {Q->anm_re0, Q->anm_im0};
+    double m = DBL_MIN;
+    for (int i = 0; i < 2; i++) {
+      double curr_val = cur[i][coeff];
+      double zeroth_val = cur[i][0];
+      int flag = (fabs(curr_val) > fabs(lamb_cut_s*zeroth_val)) *
+                 (fabs(curr_val) > 1.e-16) *
+                 (fabs(zeroth_val) > DBL_MIN);
+      double div = fabs(curr_val);
+      div += (1.e-16 - div) * (div < 1.e-16);
+      double e = (double) flag * fabs(curr_val - old[i][coeff]) / div;
+      m += (e - m) * (e > m);
+    }
+    s_max[coeff] = m;
+  }
+  __syncthreads();
+  if (coeff == 0) {
+    double mx = DBL_MIN;
+    for (int i = 0; i < s_ncoeffs_max; i++)
+      mx += (s_max[i] - mx) * (s_max[i] > mx);
+    perr[blockIdx.x] = mx;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+scalar_max_store_kernel(int nparts, int s_ncoeffs_max, part_struct *parts,
+  const double *__restrict__ perr, double *out)
+{
+  if (threadIdx.x == 0) {
+    double best = 0.;
+    for (int n = 0; n < nparts; n++) best = perr[n] > best ? perr[n] : best;
+    out[0] = best;
+  }
+  for (int t = threadIdx.x; t < nparts * s_ncoeffs_max; t += blockDim.x) {
+    part_struct *Q = &parts[t / s_ncoeffs_max];
+    int c = t % s_ncoeffs_max;
+    Q->anm_re0[c] = Q->anm_re[c];
+    Q->anm_im0[c] = Q->anm_im[c];
+  }
+}
+
+/* The analytic scalar in particle cells: the series in shell cells, the
+ * particle's scalar inside; fluid cells keep theirs
+ * (src/scalar_kernel.cu:1143-1227). */
+__global__ void __launch_bounds__(256)
+scalar_part_bc_kernel(PDom P, const part_struct *__restrict__ parts,
+  const int *__restrict__ phase, const int *__restrict__ shell, double *s)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > P.nx) return;
+  long C = gidx(P.cc, i, j, k);
+  int pn = phase[C];
+  if (pn < 0) return;
+  const part_struct *Q = &parts[pn];
+  if (shell[C] > 0) {
+    s[C] = Q->s;
+    return;
+  }
+  double x = (i - 0.5) * P.dx + P.xs;
+  double y = (j - 0.5) * P.dy + P.ys;
+  double z = (k - 0.5) * P.dz + P.zs;
+  x -= near_img(x, Q->x, P.xl, P.perx);
+  y -= near_img(y, Q->y, P.yl, P.pery);
+  z -= near_img(z, Q->z, P.zl, P.perz);
+  double r, theta, phi;
+  xyz2rtp(x, y, z, &r, &theta, &phi);
+  double ar = Q->r / r;
+  double ra = r / Q->r;
+  double v = Q->s;
+  for (int n = 0; n <= Q->sorder; n++) {
+    int coeff = 0;
+    for (int jj = 0; jj < n; jj++) coeff += 2*jj + 1;
+    double X = 0.;
+    for (int m = -n; m <= n; m++) {
+      X += Nnm(n, m) * (m < 0 ? 0. : Pnm(n, m, theta))
+        * (Q->anm_re[coeff]*cos(m*phi) - Q->anm_im[coeff]*sin(m*phi));
+      coeff++;
+    }
+    v += (pow(ra, n) - pow(ar, n + 1)) * X;
+  }
+  s[C] = v;
+}
+
+__global__ void __launch_bounds__(256)
+scalar_part_fill_kernel(PDom P, const part_struct *__restrict__ parts,
+  const int *__restrict__ phase, double *s)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > P.nx) return;
+  long C = gidx(P.cc, i, j, k);
+  int pn = phase[C];
+  if (pn > -1) s[C] = parts[pn].s;
+}
+
+__global__ void update_part_scalar_kernel(int nparts, part_struct *parts,
+  double dt, double s_k)
+{
+  int pp = threadIdx.x + blockIdx.x * blockDim.x;
+  if (pp >= nparts) return;
+  part_struct *Q = &parts[pp];
+  double vol = 4./3. * PI * Q->r*Q->r*Q->r;
+  double m = vol * Q->rho;
+  Q->s += (float) Q->update * Q->q * s_k * dt / m / Q->cp;
+}
+
+extern "C" void cuda_scalar_part_BC(real *array)
+{
+  if (nparts == 0) return;
+  PDom P = pdom();
+  hipLaunchKernelGGL(scalar_part_bc_kernel, dim3((P.nx + 255) / 256, P.ny,
+    P.nz), dim3(256), 0, bb_stream, P, _parts, _phase, _phase_shell, array);
+  BB_LAUNCH_CHECK();
+}
+
+extern "C" void cuda_scalar_part_fill(void)
+{
+  if (nparts == 0) return;
+  PDom P = pdom();
+  hipLaunchKernelGGL(scalar_part_fill_kernel, dim3((P.nx + 255) / 256, P.ny,
+    P.nz), dim3(256), 0, bb_stream, P, _parts, _phase, _s);
+  BB_LAUNCH_CHECK();
+}
+
+extern "C" void cuda_scalar_lamb(void)
+{
+  if (nparts == 0) return;
+  hipLaunchKernelGGL(scalar_lamb_kernel, dim3(nparts), dim3(64), 0, bb_stream,
+    pdom(), pscalar_bc(), _parts, _s);
+  BB_LAUNCH_CHECK();
+}
+
+extern "C" real cuda_scalar_lamb_err(void)
+{
+  /* src/cuda_scalar.cu:914-946 */
+  if (nparts == 0) return DBL_MIN;
+  hipLaunchKernelGGL(scalar_error_kernel, dim3(nparts), dim3(64), 0,
+    bb_stream, lamb_cut_scalar, s_ncoeffs_max, _parts, _perr);
+  hipLaunchKernelGGL(scalar_max_store_kernel, dim3(1), dim3(256), 0,
+    bb_stream, nparts, s_ncoeffs_max, _parts, _perr, _err_out);
+  BB_LAUNCH_CHECK();
+  BB_CHECK(hipMemcpyAsync(err_host, _err_out, sizeof(double),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  return err_host[0];
+}
+
+extern "C" void cuda_scalar_update_part(void)
+{
+  if (nparts == 0) return;
+  hipLaunchKernelGGL(update_part_scalar_kernel, dim3(nblk(nparts, 256)),
+    dim3(256), 0,
+    bb_stream, nparts, _parts, dt, s_k);
+  BB_LAUNCH_CHECK();
+}
+
 /* ------------------------- collisions and motion ------------------------- */
 __global__ void collision_init_kernel(int nparts, part_struct *parts)
 {

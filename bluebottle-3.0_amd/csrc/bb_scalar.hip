@@ -15,9 +15,9 @@
  *   The u*, v*, w* kernels form it from the two cells beside the face when
  *   this call has followed the step's cuda_compute_forcing (they hold the
  *   constant part of f in registers; no f arrays exist on the device).
+ * The particle heat exchange (cuda_scalar_part_BC, _part_fill, _lamb,
+ * _lamb_err, _update_part) is with the particle kernels (bb_particle.hip).
  */
-#include <float.h>
-
 #include "bb_common.h"
 
 /* Set by cuda_compute_boussinesq, cleared by cuda_compute_forcing. */
@@ -152,23 +152,3 @@ extern "C" void cuda_store_s(void)
   BB_CHECK(hipMemcpyAsync(_s_diff0, _s_diff, n, hipMemcpyDeviceToDevice,
     bb_stream));
 }
-
-/* Particle heat transfer (src/scalar_kernel.cu:303-1360) is not part of
- * this build: a case with SCALAR >= 1 and particles is refused when the
- * input is read, so these are the calls of a particle-free run. */
-extern "C" void cuda_scalar_part_BC(real *array)
-{
-  (void) array;
-}
-
-extern "C" void cuda_scalar_part_fill(void) {}
-
-extern "C" void cuda_scalar_lamb(void) {}
-
-extern "C" real cuda_scalar_lamb_err(void)
-{
-  /* src/cuda_scalar.cu:917: the error of a run without particles */
-  return DBL_MIN;
-}
-
-extern "C" void cuda_scalar_update_part(void) {}

@@ -55,6 +55,8 @@ static void pressure_solve(void)
 
 int bb_lamb_iters = 0;
 real bb_lamb_err = 0.;
+int bb_scalar_iters = 0;               /* scalar iterations of the last step */
+real bb_scalar_err = 0.;
 
 int bluebottle_setup(void)
 {
@@ -182,6 +184,8 @@ int bluebottle_step(void)
       scalar_iter++;
       if (scalar_iter == lamb_max_iter) break;
     }
+    bb_scalar_iters = scalar_iter;
+    bb_scalar_err = scalar_iter_err;
     cuda_store_s();
     cuda_scalar_update_part();
   }

@@ -141,12 +141,6 @@ void parts_read_input(void)
   fclose(pf);
   nparts = NPARTS;
   if (NPARTS == 0) return;
-  if (SCALAR >= 1) {
-    /* particle heat transfer, src/scalar_kernel.cu:303-1360 */
-    fprintf(stderr, "part.config: particles with scalar transport (SCALAR "
-      ">= 1) are not supported by this build\n");
-    exit(EXIT_FAILURE);
-  }
 
   volume_fraction = total_parts_volume / (DOM.xl * DOM.yl * DOM.zl);
   rho_avg = total_parts_mass / total_parts_volume * volume_fraction

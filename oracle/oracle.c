@@ -126,6 +126,7 @@ static int get_bc(const char *key, double *a, double *b)
 }
 
 static void o_scalar_parse(void);
+static int O_scalar;                 /* SCALAR (scalar.c) */
 static int O_lamb_max_iter;
 static double O_lamb_residual, O_lamb_relax, O_lamb_cut;
 static double O_sbeta, O_sref, O_g[3], O_gm[3], O_ga[3];
@@ -1164,6 +1165,7 @@ int orc_setup(const char *dir)
   }
   rc = parts_parse(dir);
   if (rc) return rc;
+  o_scalar_part_init();
   o_init_physalis();
   /* src/bluebottle.c:113-135 */
   orc_exchange(p_, 0); exch_vel(u_, v_, w_);
@@ -1308,6 +1310,8 @@ double orc_scalar(const char *n)
   if (strcmp(n, "lamb_iters") == 0) return O.lamb_iters;
   if (strcmp(n, "lamb_err") == 0) return O.lamb_err;
   if (strcmp(n, "nparts") == 0) return NP_;
+  if (strcmp(n, "scalar_iters") == 0) return O_scalar_iters;
+  if (strcmp(n, "scalar_err") == 0) return O_scalar_err;
   return NAN;
 }
 

@@ -63,6 +63,10 @@ int orc_part_set_motion(int n, const double *m15);
 int orc_part_contacts(int n, int *iSt, double *St, int *ncoll2);
 void orc_collide_once(void);
 void orc_move_parts_sub(void);
+int orc_part_scalar(int n, double *s_q_anm52);
+int orc_part_set_scalar(int n, const double *s_q_anm52);
+void orc_scalar_lamb_once(void);
+void orc_scalar_part_bc_s(void);
 /* Lamb velocity (U[0..2]) and n = 0 pressure term (U[3]) of particle n at
  * (x, y, z) relative to its centre. */
 void orc_lamb_eval(int n, double x, double y, double z, double *U);

@@ -162,6 +162,21 @@ class Oracle:
         """Pair, spring and wall forces on the current particle state."""
         self.lib.orc_collide_once()
 
+    def part_scalar(self, n):
+        """(s, q, anm_re[25], anm_im[25]) of particle n."""
+        out = np.zeros(52)
+        self.lib.orc_part_scalar(n, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)))
+        return out[0], out[1], out[2:27].copy(), out[27:52].copy()
+
+    def set_part_scalar(self, n, s, q, anm_re, anm_im):
+        v = np.zeros(52)
+        v[0], v[1] = s, q
+        v[2:27] = anm_re
+        v[27:52] = anm_im
+        self.lib.orc_part_set_scalar(n, v.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)))
+
     def move_sub(self):
         """Collisions, velocity update, collisions (one Lamb sub-step)."""
         self.lib.orc_move_parts_sub()

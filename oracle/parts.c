@@ -31,6 +31,7 @@
 
 #define O_NNODES 26
 #define O_NCOEFF 15
+#define O_SNCOEFF 25
 #define O_MAXNB 12
 #ifndef PI
 #define PI 3.14159265358979323846
@@ -52,6 +53,11 @@ typedef struct {
   int iSt[O_MAXNB];
   double e_dry, coeff_fric, s;
   int ncoll_part, ncoll_wall;
+  /* scalar (heat) state, src/particle.h:426-445 */
+  int sorder, sncoeff, update;
+  double srs, q, cp;
+  double anm_re[O_SNCOEFF], anm_im[O_SNCOEFF], anm_re0[O_SNCOEFF],
+         anm_im0[O_SNCOEFF];
   double pnm_re[O_NCOEFF], pnm_im[O_NCOEFF], phinm_re[O_NCOEFF],
          phinm_im[O_NCOEFF], chinm_re[O_NCOEFF], chinm_im[O_NCOEFF];
   double pnm_re0[O_NCOEFF], pnm_im0[O_NCOEFF], phinm_re0[O_NCOEFF],
@@ -117,7 +123,12 @@ static int parts_parse(const char *dir)
     if (O.per[2]) Q->z -= O.zl * (Q->z == O.ze);
     Q->N = i;
     Q->rs = rs_r * Q->r;
-    Q->s = 0.;
+    /* src/particle.c:131-137: the scalar state counts only with SCALAR */
+    Q->s = t_s * (O_scalar >= 1);
+    Q->update = t_up * (O_scalar >= 1);
+    Q->cp = t_cp;
+    Q->srs = t_rs;
+    Q->sorder = t_so;
     Q->ncoeff = 0;
     for (int j = 0; j <= Q->order; j++) Q->ncoeff += j + 1;
     if (Q->ncoeff > ncmax_) ncmax_ = Q->ncoeff;

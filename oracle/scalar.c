@@ -10,12 +10,11 @@
  *   transport step      src/scalar_kernel.cu:208-301
  *   store               src/cuda_scalar.cu:949-957
  *   step order          src/bluebottle.c:189-192, 293-341
- * Particle heat transfer is not restated (the library refuses scalar runs
- * with particles).
+ *   particle heat       src/scalar.c:90-115, src/scalar_kernel.cu:303-577,
+ *                       1143-1360, src/cuda_scalar.cu:866-969
  */
 
-static int O_scalar;
-static double O_sD, O_sk, O_sinit, O_sinit_rand;
+static double O_sD, O_sk, O_sinit, O_sinit_rand, O_slamb_cut;
 static int O_sbc_type[6];
 static double O_sbc_val[6];
 static double *s_, *s0_, *sc_, *sc0_, *sd_, *sd0_;
@@ -26,6 +25,7 @@ static void o_scalar_parse(void)
   O_scalar = find_line("SCALAR", rest) ? atoi(rest) : 0;
   O_sD = find_line("s_D", rest) ? atof(rest) : 0.;
   O_sk = find_line("s_k", rest) ? atof(rest) : 0.;
+  O_slamb_cut = find_line("s_lamb_cut", rest) ? atof(rest) : 0.;
   O_sinit = O_sinit_rand = 0.;
   if (find_line("init_s", rest)) sscanf(rest, "%lf %lf", &O_sinit, &O_sinit_rand);
   const char *F = "WESNBT";
@@ -164,22 +164,286 @@ static void o_store_s(void)
   memcpy(sd0_, sd_, O.cc.n * sizeof(double));
 }
 
-/* the scalar iteration after the Lamb loop; without particles its error is
- * DBL_MIN after the first pass */
+/* --------------------------- particle heat ------------------------------ */
+static int o_sncmax_;                 /* s_ncoeffs_max */
+static const int o_snn[O_SNCOEFF] = {0, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3,
+  3, 3, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+static const int o_smm[O_SNCOEFF] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -3, -2, -1,
+  0, 1, 2, 3, -4, -3, -2, -1, 0, 1, 2, 3, 4};
+
+/* The reference's associated Legendre table has its m < 0 cases commented
+ * out, and its switch falls through every n to `return 0`
+ * (src/particle_kernel.cu:1928-1990, src/physalis_kernel.cu:57-116): the
+ * m < 0 terms of the scalar series and quadrature are zero. */
+static double o_sPnm(int n, int m, double th)
+{
+  return m < 0 ? 0. : o_Pnm(n, m, th);
+}
+
+static void o_scalar_part_init(void)
+{
+  o_sncmax_ = 0;
+  for (int n = 0; n < NP_; n++) {
+    int snc = 0;
+    for (int j = 0; j <= P_[n].sorder; j++) snc += 2*j + 1;
+    if (snc > o_sncmax_) o_sncmax_ = snc;
+  }
+  for (int n = 0; n < NP_; n++) {
+    OP *Q = &P_[n];
+    Q->srs = Q->srs * Q->r;
+    Q->q = 0.0;
+    Q->sncoeff = 0;
+    for (int j = 0; j <= Q->sorder; j++) Q->sncoeff += 2*j + 1;
+    for (int j = 0; j < O_SNCOEFF; j++)
+      Q->anm_re[j] = Q->anm_im[j] = Q->anm_re0[j] = Q->anm_im0[j] = 0.;
+  }
+}
+
+/* scalar at one node on the srs sphere, minus the particle's scalar
+ * (src/scalar_kernel.cu:303-527); the node's code from the scalar BCs */
+static double o_snode(OP *Q, int node)
+{
+  double th = o_nt[node], ph = o_np[node];
+  double xp = Q->srs * sin(th) * cos(ph);
+  double yp = Q->srs * sin(th) * sin(ph);
+  double zp = Q->srs * cos(th);
+  double x = xp + Q->x, y = yp + Q->y, z = zp + Q->z;
+  int ic = 0, jc = 0, kc = 0;
+  for (int im = 0; im < 27; im++) {
+    int sx = im % 3 - 1, sy = (im / 3) % 3 - 1, sz = im / 9 - 1;
+    if ((sx && !O.per[0]) || (sy && !O.per[1]) || (sz && !O.per[2])) continue;
+    double xi = xp + oimg(Q->x, sx, O.xl), yi = yp + oimg(Q->y, sy, O.yl),
+           zi = zp + oimg(Q->z, sz, O.zl);
+    int a = o_cell((xi - (O.xs - O.dx)) * (1. / O.dx));
+    int b = o_cell((yi - (O.ys - O.dy)) * (1. / O.dy));
+    int c = o_cell((zi - (O.zs - O.dz)) * (1. / O.dz));
+    if (im == 13 || (a >= 1 && a <= O.nx && b >= 1 && b <= O.ny &&
+                     c >= 1 && c <= O.nz)) {
+      x = xi; y = yi; z = zi; ic = a; jc = b; kc = c;
+      if (a >= 1 && a <= O.nx && b >= 1 && b <= O.ny && c >= 1 && c <= O.nz)
+        break;
+    }
+  }
+  /* node code: faces W E S N B T, Dirichlet before Neumann */
+  int code = -1;
+  const double gap[6] = {x - O.xs, x - O.xe, y - O.ys, y - O.ye, z - O.zs,
+                         z - O.ze};
+  for (int f6 = 0; f6 < 6 && code == -1; f6++) {
+    int out = (f6 & 1) ? gap[f6] >= 0 : gap[f6] < 0;
+    if (!out) continue;
+    if (O_sbc_type[f6] == DIRICHLET) code = -100 - 10 * f6;
+    else if (O_sbc_type[f6] == NEUMANN) code = -101 - 10 * f6;
+  }
+  Q->nodes[node] = code;
+  int oobi = ic < 1 || ic > O.nx, oobj = jc < 1 || jc > O.ny,
+      oobk = kc < 1 || kc > O.nz;
+  int i = o_clamp(ic, 1, O.nx), j = o_clamp(jc, 1, O.ny), k = o_clamp(kc, 1, O.nz);
+  long C = IDX(O.cc, i, j, k);
+  double ss = o_interp(s_, &O.cc, i, j, k, x, y, z, (i - 0.5) * O.dx + O.xs,
+    (j - 0.5) * O.dy + O.ys, (k - 0.5) * O.dz + O.zs) - Q->s;
+  /* a node beyond a face: the face's Dirichlet value, or the ghost cell
+   * beside the clamped cell for a Neumann face */
+  double wall[3];
+  long nb[6] = {C - 1, C + 1, C - O.cc.sy, C + O.cc.sy, C - O.cc.sz, C + O.cc.sz};
+  for (int d = 0; d < 3; d++) {
+    wall[d] = (code == -100 - 20 * d) * O_sbc_val[2 * d]
+            + (code == -110 - 20 * d) * O_sbc_val[2 * d + 1]
+            + (code == -101 - 20 * d) * s_[nb[2 * d]]
+            + (code == -111 - 20 * d) * s_[nb[2 * d + 1]]
+            - Q->s;
+  }
+  return (1-oobi) * (1-oobj) * (1-oobk) * (code == -1) * ss
+       + oobi * (1-oobj) * (1-oobk) * (code < -1) * wall[0]
+       + (1-oobi) * oobj * (1-oobk) * (code < -1) * wall[1]
+       + (1-oobi) * (1-oobj) * oobk * (code < -1) * wall[2];
+}
+
+/* quadrature and coefficients anm = <Y_nm, s - s_p> / (rsa^n - ars^(n+1)),
+ * heat flux q = 2 sqrt(pi) a a_00 (src/scalar_kernel.cu:529-577,
+ * 1247-1272) */
+static void o_scalar_lamb(void)
+{
+  for (int n = 0; n < NP_; n++) {
+    OP *Q = &P_[n];
+    double ss[O_NNODES];
+    for (int node = 0; node < O_NNODES; node++) ss[node] = o_snode(Q, node);
+    double ars = Q->r / Q->srs, rsa = Q->srs / Q->r;
+    for (int coeff = 0; coeff < Q->sncoeff; coeff++) {
+      int nn = o_snn[coeff], m = o_smm[coeff];
+      double re = 0., im = 0.;
+      for (int node = 0; node < O_NNODES; node++) {
+        double th = o_nt[node], ph = o_np[node];
+        double N_nm = o_Nnm(nn, m), P_nm = o_sPnm(nn, m, th);
+        double Ire = N_nm*P_nm*ss[node]*cos(m * ph);
+        double Iim = -N_nm*P_nm*ss[node]*sin(m * ph);
+        double wgt = node < 6 ? o_A1 : (node < 18 ? o_A2 : o_A3);
+        if (node == 0) { re = Ire * o_A1; im = Iim * o_A1; }
+        else { re += wgt * Ire; im += wgt * Iim; }
+      }
+      double A = pow(rsa, nn) - pow(ars, nn + 1.);
+      Q->anm_re[coeff] = re / A;
+      Q->anm_im[coeff] = im / A;
+    }
+    if (Q->sncoeff > 0) Q->q = 2. * sqrt(PI) * Q->r * Q->anm_re[0];
+  }
+}
+
+/* largest relative coefficient change (src/scalar_kernel.cu:1274-1339,
+ * src/cuda_scalar.cu:914-946), then anm0 = anm */
+static double o_scalar_err(void)
+{
+  double err = DBL_MIN;
+  if (NP_ > 0) {
+    double best = 0.;
+    for (int n = 0; n < NP_; n++) {
+      OP *Q = &P_[n];
+      const double *cur[2] = {Q->anm_re, Q->anm_im};
+      const double *old[2] = {Q->anm_re0, Q->anm_im0};
+      double mx = DBL_MIN;
+      for (int coeff = 0; coeff < o_sncmax_; coeff++) {
+        double m = DBL_MIN;
+        for (int i = 0; i < 2; i++) {
+          double cv = cur[i][coeff], zv = cur[i][0];
+          int flag = (fabs(cv) > fabs(O_slamb_cut*zv)) * (fabs(cv) > 1.e-16) *
+                     (fabs(zv) > DBL_MIN);
+          double div = fabs(cv);
+          div += (1.e-16 - div) * (div < 1.e-16);
+          double e = (double) flag * fabs(cv - old[i][coeff]) / div;
+          m += (e - m) * (e > m);
+        }
+        mx += (m - mx) * (m > mx);
+      }
+      if (mx > best) best = mx;
+    }
+    err = best;
+    for (int n = 0; n < NP_; n++)
+      for (int coeff = 0; coeff < o_sncmax_; coeff++) {
+        P_[n].anm_re0[coeff] = P_[n].anm_re[coeff];
+        P_[n].anm_im0[coeff] = P_[n].anm_im[coeff];
+      }
+  }
+  return err;
+}
+
+/* the analytic scalar in particle cells: the series in shell cells, the
+ * particle's scalar inside (src/scalar_kernel.cu:1143-1227) */
+static void o_scalar_part_bc(double *a)
+{
+  if (NP_ == 0) return;
+  for (int k = 1; k <= O.nz; k++)
+    for (int j = 1; j <= O.ny; j++)
+      for (int i = 1; i <= O.nx; i++) {
+        long C = IDX(O.cc, i, j, k);
+        int P = phase_[C];
+        if (P < 0) continue;
+        const OP *Q = &P_[P];
+        double x = (i - 0.5) * O.dx + O.xs, y = (j - 0.5) * O.dy + O.ys,
+               z = (k - 0.5) * O.dz + O.zs;
+        x -= onear(x, Q->x, O.xl, O.per[0]);
+        y -= onear(y, Q->y, O.yl, O.per[1]);
+        z -= onear(z, Q->z, O.zl, O.per[2]);
+        double r, th, ph;
+        o_xyz2rtp(x, y, z, &r, &th, &ph);
+        double ar = Q->r / r, ra = r / Q->r;
+        double v = Q->s;
+        for (int n = 0; n <= Q->sorder; n++) {
+          int coeff = 0;
+          for (int jj = 0; jj < n; jj++) coeff += 2*jj + 1;
+          double X = 0.;
+          for (int m = -n; m <= n; m++) {
+            X += o_Nnm(n, m) * o_sPnm(n, m, th)
+              * (Q->anm_re[coeff]*cos(m*ph) - Q->anm_im[coeff]*sin(m*ph));
+            coeff++;
+          }
+          v += (pow(ra, n) - pow(ar, n + 1)) * X;
+        }
+        a[C] = shell_[C] < 1 ? v : Q->s;
+      }
+}
+
+static void o_scalar_part_fill(double *a)
+{
+  if (NP_ == 0) return;
+  for (int k = 1; k <= O.nz; k++)
+    for (int j = 1; j <= O.ny; j++)
+      for (int i = 1; i <= O.nx; i++) {
+        long C = IDX(O.cc, i, j, k);
+        if (phase_[C] > -1) a[C] = P_[phase_[C]].s;
+      }
+}
+
+/* particle scalar from the heat flux (src/scalar_kernel.cu:1351-1360) */
+static void o_update_part_scalar(void)
+{
+  for (int n = 0; n < NP_; n++) {
+    OP *Q = &P_[n];
+    double vol = 4./3. * PI * Q->r*Q->r*Q->r;
+    double m = vol * Q->rho;
+    Q->s += (float) Q->update * Q->q * O_sk * O.dt / m / Q->cp;
+  }
+}
+
+/* the scalar iteration after the Lamb loop (src/bluebottle.c:293-341) */
+static int O_scalar_iters;
+static double O_scalar_err;
+
 static void o_scalar_step(void)
 {
   int it = 0;
   double err = FLT_MAX;
   while (err > O_lamb_residual) {
     o_scalar_bc(s0_);
+    o_scalar_part_bc(s0_);
     orc_exchange(s0_, 0);
     o_scalar_solve();
     orc_exchange(s_, 0);
     o_scalar_bc(s_);
+    o_scalar_part_fill(s_);
     orc_exchange(s_, 0);
-    err = DBL_MIN;
+    if (NP_ > 0) o_scalar_lamb();
+    err = o_scalar_err();
     it++;
     if (it == O_lamb_max_iter) break;
   }
+  O_scalar_iters = it;
+  O_scalar_err = err;
   o_store_s();
+  o_update_part_scalar();
+}
+
+/* Scalar state of particle n: s, q, then anm_re and anm_im (25 each). */
+int orc_part_scalar(int n, double *out)
+{
+  if (n < 0 || n >= NP_) return -1;
+  const OP *Q = &P_[n];
+  out[0] = Q->s;
+  out[1] = Q->q;
+  memcpy(out + 2, Q->anm_re, sizeof(Q->anm_re));
+  memcpy(out + 2 + O_SNCOEFF, Q->anm_im, sizeof(Q->anm_im));
+  return 0;
+}
+
+/* Set the scalar state of particle n (s, q, anm_re[25], anm_im[25]). */
+int orc_part_set_scalar(int n, const double *in)
+{
+  if (n < 0 || n >= NP_) return -1;
+  OP *Q = &P_[n];
+  Q->s = in[0];
+  Q->q = in[1];
+  memcpy(Q->anm_re, in + 2, sizeof(Q->anm_re));
+  memcpy(Q->anm_im, in + 2 + O_SNCOEFF, sizeof(Q->anm_im));
+  return 0;
+}
+
+/* One pass of the particle scalar quadrature on the current s (the
+ * coefficients and heat fluxes), and the analytic particle values written
+ * into s, for closed-form tests. */
+void orc_scalar_lamb_once(void)
+{
+  o_scalar_lamb();
+}
+
+void orc_scalar_part_bc_s(void)
+{
+  o_scalar_part_bc(s_);
 }
