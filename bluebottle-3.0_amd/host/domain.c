@@ -35,11 +35,6 @@ void parse_cmdline_args(int argc, char *argv[])
       exit(EXIT_FAILURE);
     }
   }
-  if (use_restart) {
-    fprintf(stderr, "bluebottle: restart input is not supported by this "
-      "build\n");
-    exit(EXIT_FAILURE);
-  }
 }
 
 /* ---- rigid-order reader ---- */

@@ -62,6 +62,7 @@ int bluebottle_setup(void)
 {
   /* src/bluebottle.c:31-142 */
   domain_read_input();
+  recorder_read_config();
   domain_fill();
   cuda_dom_malloc_host();
   cuda_dom_malloc_dev();
@@ -76,6 +77,14 @@ int bluebottle_setup(void)
   nparts = 0;
   if (parts_read_input) parts_read_input();
   if (!use_restart) recorder_PP_init("solver_expd.rec");
+  if (use_restart) {
+    in_restart();
+    if (ttime >= duration) {
+      printf("\n...simulation completed");
+      restart_stop = 1;
+      exit(EXIT_SUCCESS);
+    }
+  }
   if (bb_parts_setup) bb_parts_setup();
   cuda_dom_push();
   cuda_exchange(_p, GCC);
@@ -208,6 +217,9 @@ int bluebottle_step(void)
     cuda_PP_init_jacobi_preconditioner();
   }
 
+  /* restart file (src/bluebottle.c:397-398); 1 asks the caller to stop */
+  if (restart_recorder_write()) return 1;
+
   if (dt < 1.e-10 || dt > 1.e10) {
     printf("N%d >> The solution time step has diverged. Ending simulation.\n",
       rank);
@@ -237,7 +249,8 @@ int bluebottle_main(int argc, char *argv[])
     printf("======================================\n");
     fflush(stdout);
   }
-  while (ttime <= duration) bluebottle_step();
+  while (ttime <= duration)
+    if (bluebottle_step()) break;
   if (rank == 0) {
     printf("\nEXPD: The simulation has reached its specified duration\n");
     printf("========================================");

@@ -300,8 +300,10 @@ void bb_parts_setup(void)
 {
   if (NPARTS == 0) return;
   init_bins();
-  parts_init();
-  scalar_part_init();
+  if (!use_restart) {               /* a restart reads the particle state */
+    parts_init();
+    scalar_part_init();
+  }
   cuda_part_malloc_dev();
   cuda_part_push();
   cuda_init_physalis();

@@ -366,13 +366,26 @@ void rng_init(unsigned long long seed);            /* src/rng.c:30-41 */
 unsigned long long rng_uint64(void);               /* src/rng.c:44-51 */
 double rng_dbl(void);                              /* src/rng.c:78-81 */
 void recorder_PP_init(const char *name);           /* src/recorder.c:157-188 */
+/* Restart (src/domain.c:3005-3261, src/recorder.c:43-98, 338-391): the
+ * reference's binary restart.config-<rank> in the reference layouts. */
+extern int use_restart, restart_stop, rec_restart_complete;
+extern real rec_restart_dt, rec_cgns_flow_dt, rec_cgns_part_dt, rec_vtk_dt;
+extern int rec_vtk_stepnum_out;
+extern real rec_cgns_flow_ttime_out, rec_cgns_part_ttime_out,
+  rec_vtk_ttime_out;
+void recorder_read_config(void);                   /* src/recorder.c:43-98 */
+void out_restart(void);                            /* src/domain.c:3005-3092 */
+void in_restart(void);                             /* src/domain.c:3094-3261 */
+int restart_recorder_write(void);                  /* src/recorder.c:338-391 */
 void recorder_PP(const char *name, int niter, real resid, real etime); /* :190-221 */
 
 /* Driver: src/bluebottle.c:28-436 split into its phases so that a caller can
  * step the solver one timestep at a time.  bluebottle_main() is the whole
  * program (setup, `while (ttime <= duration)` loop, teardown). */
 int bluebottle_setup(void);          /* :31-142, returns 0 on success */
-int bluebottle_step(void);           /* one pass of :168-407 */
+int bluebottle_step(void);           /* one pass of :168-407; 1 when a
+                                        restart file was written and
+                                        RESTART_STOP asks to end the run */
 void bluebottle_teardown(void);      /* :416-430 */
 int bluebottle_main(int argc, char *argv[]);
 
