@@ -47,8 +47,9 @@ KT_IDS = {"spmv_search_kernel": 0, "update_kernel": 1,
 
 # Algorithmic bytes per interior cell of each timed kernel (one element per
 # array the kernel semantically reads or writes; FP64 = 8 B, FP32 = 4 B),
-# plus one byte for the particle mask that both kernels read when the box
-# holds particles.
+# plus one byte for the particle mask that the update kernels read when the
+# box holds particles (the search kernels read none: the residual is zero
+# in particle cells).
 KERNEL_BYTES = {
     # update: reads phi, p, r, Ap; writes phi, r
     "update_kernel": 6 * 8,
@@ -66,10 +67,10 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 34.60,
-    "update_kernel": 50.32,
-    "spmv_search_kernel_f32": 17.10,
-    "update_kernel_f32": 25.32,
+    "spmv_search_kernel": 37.86,
+    "update_kernel": 51.19,
+    "spmv_search_kernel_f32": 17.87,
+    "update_kernel_f32": 26.14,
 }
 
 
@@ -234,7 +235,8 @@ def roofline_of(ktimes, cells, copy_gbs, mask):
     dom_name = max(ktimes, key=lambda k: ktimes[k][0])
     kms, kn = ktimes[dom_name]
     avg_s = (kms / 1e3) / kn
-    bpc = KERNEL_BYTES[dom_name] + (MASK_BYTES if mask else 0)
+    reads_mask = mask and dom_name.startswith("update")
+    bpc = KERNEL_BYTES[dom_name] + (MASK_BYTES if reads_mask else 0)
     achieved = bpc * cells / avg_s / 1e9
     tpc = TRAFFIC_PER_CELL.get(dom_name)
     return {
