@@ -100,8 +100,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
-                    help="0: SpMV with direct neighbour loads, 1: LDS-staged "
-                         "planes (default)")
+                    help="search kernel: 2 z-columns through LDS planes "
+                         "(default), 3 z-columns with one wavefront per row, "
+                         "1 one tile per workgroup, 0 direct neighbour loads")
     ap.add_argument("--tzb", type=int, default=None,
                     help="tiles per z-column of the search kernel "
                          "(0: automatic)")

@@ -545,10 +545,12 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
  * (ring fill, the reduction that yields beta) and the two halo planes are
  * paid once per column instead of once per tile, and each tile's partial
  * is written exactly as above (same per-column z order, same tree).  The
- * ring of raw planes rotates through named registers: Ra holds plane
- * k + 1 (stored into LDS at the top of step k), Rb plane k + 2; the LDS
- * planes rotate through three buffers with one barrier per plane. */
-template <bool FIRST, typename V, bool MIXED>
+ * ring of raw planes rotates through DEPTH register sets: set d holds
+ * plane k + 1 + d at step k (set 0 is stored into LDS at the top of the
+ * step and reloaded with plane k + 1 + DEPTH), so DEPTH planes of loads
+ * are in flight behind the compute; the LDS planes rotate through three
+ * buffers with one barrier per plane. */
+template <bool FIRST, typename V, bool MIXED, int DEPTH>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
@@ -579,13 +581,15 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
   PlaneRawM<V> Rp = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0 - 1);
   PlaneRawM<V> Rc = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0);
-  PlaneRawM<V> Ra = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0 + 1);
-  double beta64 = 0.;
-  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
   /* loads are unconditional (plane index clamped to kend + 1): a load
    * issued on one path only would make the compiler wait for all */
-  PlaneRawM<V> Rb = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
-                                                 min(k0 + 2, kend + 1));
+  PlaneRawM<V> R[DEPTH];
+#pragma unroll
+  for (int d = 0; d < DEPTH; d++)
+    R[d] = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
+                                   min(k0 + 1 + d, kend + 1));
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
   V beta = (V) beta64;
   __syncthreads();                      /* s_invm */
   int xl = lane + 1, yl = wv + 1;
@@ -614,7 +618,7 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   auto step = [&](int k, PlaneRawM<V> &Rs) {
     plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rs, beta, bnext, k + 1);
     Rs = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
-                                       min(k + 3, kend + 1));
+                                 min(k + 1 + DEPTH, kend + 1));
     __syncthreads();
     const V *P = bcur;
     V nxt = bnext[yl * LX + xl];
@@ -639,15 +643,18 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
       ti++;
     }
   };
-  /* pairs of planes, so that Ra and Rb alternate without register copies
-   * (a copy of a register with a load in flight waits for that load), and
-   * no step inside the loop is conditional; an odd last plane after it */
+  /* DEPTH planes per trip, so that the register sets take their turns
+   * without register copies (a copy of a register with a load in flight
+   * waits for that load) and no step inside the loop is conditional; the
+   * last planes of the column after it */
   int k = k0;
-  for (; k + 1 <= kend; k += 2) {
-    step(k, Ra);
-    step(k + 1, Rb);
+  for (; k + DEPTH - 1 <= kend; k += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) step(k + d, R[d]);
   }
-  if (k <= kend) step(k, Ra);
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++)
+    if (k + d <= kend) step(k + d, R[d]);
   __syncthreads();
   if (threadIdx.x < ti) {
     const double *w = s_tp[threadIdx.x];
@@ -1930,7 +1937,9 @@ static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
       _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
     return;
   }
-  hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED>),
+  /* two planes in flight: three (FP32: 72.4 against 71.5 us at 256^3) and
+   * four did not help, and past two the FP64 kernel spills at 8 waves */
+  hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2>),
     spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
     _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
 }
