@@ -10,6 +10,7 @@
  * recomputes the same fields, so exactly one pass is made.
  */
 #include <float.h>
+#include <sys/time.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,6 +39,13 @@ void cuda_parts_internal(void) __attribute__((weak));
  * pp_warmstart is LEARNED. */
 void bb_warm_setup(void);
 void bb_warm_teardown(void);
+
+static double wall_seconds(void)
+{
+  struct timeval tv;
+  gettimeofday(&tv, NULL);
+  return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
 
 static void exchange_vel(real *a, real *b, real *c)
 {
@@ -76,7 +84,10 @@ int bluebottle_setup(void)
   NPARTS = 0;
   nparts = 0;
   if (parts_read_input) parts_read_input();
-  if (!use_restart) recorder_PP_init("solver_expd.rec");
+  if (!use_restart) {
+    recorder_PP_init("solver_expd.rec");
+    recorder_scalar_init("iter_etime.rec");
+  }
   if (use_restart) {
     in_restart();
     if (ttime >= duration) {
@@ -126,6 +137,7 @@ int bluebottle_step(void)
   if (SCALAR >= 1) cuda_compute_boussinesq();
   compute_vel_BC();
 
+  double mitertimestart = wall_seconds();
   while (iter_err > lamb_residual) {
     cuda_U_star();
     exchange_vel(_u_star, _v_star, _w_star);
@@ -166,6 +178,7 @@ int bluebottle_step(void)
     iter++;
     if (iter == lamb_max_iter) break;
   }
+  double mitertime = wall_seconds() - mitertimestart;
   bb_lamb_iters = iter;
   bb_lamb_err = iter_err;
   if (rank == 0 && bb_verbose && NPARTS > 0) {
@@ -176,9 +189,11 @@ int bluebottle_step(void)
   }
 
   /* scalar inner iteration (src/bluebottle.c:293-341) */
+  int scalar_iter = 0;
+  real scalar_iter_err = FLT_MAX;
+  double sitertime = 0.;
   if (SCALAR >= 1) {
-    int scalar_iter = 0;
-    real scalar_iter_err = FLT_MAX;
+    double sitertimestart = wall_seconds();
     while (scalar_iter_err > lamb_residual) {
       cuda_scalar_BC(_s0);
       cuda_scalar_part_BC(_s0);
@@ -193,6 +208,7 @@ int bluebottle_step(void)
       scalar_iter++;
       if (scalar_iter == lamb_max_iter) break;
     }
+    sitertime = wall_seconds() - sitertimestart;
     bb_scalar_iters = scalar_iter;
     bb_scalar_err = scalar_iter_err;
     cuda_store_s();
@@ -210,6 +226,8 @@ int bluebottle_step(void)
   cuda_store_u();
   dt0 = dt;
   cuda_find_dt();
+  recorder_scalar("iter_etime.rec", ttime, iter, mitertime, iter_err,
+    scalar_iter, sitertime, scalar_iter_err);
 
   if (NPARTS > 0) {
     cuda_update_part_position();

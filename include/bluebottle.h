@@ -378,6 +378,10 @@ void out_restart(void);                            /* src/domain.c:3005-3092 */
 void in_restart(void);                             /* src/domain.c:3094-3261 */
 int restart_recorder_write(void);                  /* src/recorder.c:338-391 */
 void recorder_PP(const char *name, int niter, real resid, real etime); /* :190-221 */
+void recorder_scalar_init(const char *name);       /* src/scalar.c:226-262 */
+void recorder_scalar(const char *name, real ttime, int mniter,
+  real mitertime, real merr, int sniter, real sitertime,
+  real serr);                                      /* src/scalar.c:264-295 */
 
 /* Driver: src/bluebottle.c:28-436 split into its phases so that a caller can
  * step the solver one timestep at a time.  bluebottle_main() is the whole
