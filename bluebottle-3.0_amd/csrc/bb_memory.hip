@@ -293,6 +293,14 @@ extern "C" void cuda_dom_pull(void)
   for (int i = 0; i < d->Gfz.s3b; i++) f_z[i] = bb_force[2];
 }
 
+extern "C" void cuda_dom_pull_phase(void)
+{
+  /* src/cuda_bluebottle.cu:1455 */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(phase, _phase, L_GCC);
+  pull_field(phase_shell, _phase_shell, L_GCC);
+}
+
 /* Pull one Gcc device array (x-fastest, ghosted) into a host buffer of the
  * same order; used by tests and the solver statistics. */
 extern "C" void bb_pull_raw(const void *dev, void *host, size_t bytes)

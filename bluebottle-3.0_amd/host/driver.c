@@ -85,6 +85,7 @@ int bluebottle_setup(void)
   nparts = 0;
   if (parts_read_input) parts_read_input();
   if (!use_restart) {
+    vtk_recorder_init();
     recorder_PP_init("solver_expd.rec");
     recorder_scalar_init("iter_etime.rec");
   }
@@ -117,6 +118,8 @@ int bluebottle_setup(void)
   }
   cuda_PP_init_jacobi_preconditioner();
   bb_warm_setup();
+  /* initial output (src/bluebottle.c:159-162) */
+  if (!use_restart) recorder_out();
   return 0;
 }
 
@@ -124,6 +127,9 @@ int bluebottle_step(void)
 {
   /* One pass of src/bluebottle.c:168-407. */
   ttime += dt;
+  rec_cgns_flow_ttime_out += dt;
+  rec_cgns_part_ttime_out += dt;
+  rec_vtk_ttime_out += dt;
   stepnum++;
   int iter = 0;
   real iter_err = FLT_MAX;
@@ -228,6 +234,7 @@ int bluebottle_step(void)
   cuda_find_dt();
   recorder_scalar("iter_etime.rec", ttime, iter, mitertime, iter_err,
     scalar_iter, sitertime, scalar_iter_err);
+  recorder_out();
 
   if (NPARTS > 0) {
     cuda_update_part_position();

@@ -14,8 +14,9 @@
 
 #include "bluebottle.h"
 
-/* record.config (src/recorder.c:43-98); output writers are not part of
- * this library, their clocks are kept for the restart file's layout */
+/* record.config (src/recorder.c:43-98).  VTK output uses rec_vtk_*; there
+ * are no CGNS writers (libcgns), their clocks are kept for the restart
+ * file's layout. */
 real rec_cgns_flow_dt = -1., rec_cgns_part_dt = -1., rec_vtk_dt = -1.;
 real rec_restart_dt = -1.;
 int rec_restart_complete = -1, restart_stop = 0;

@@ -382,6 +382,17 @@ void recorder_scalar_init(const char *name);       /* src/scalar.c:226-262 */
 void recorder_scalar(const char *name, real ttime, int mniter,
   real mitertime, real merr, int sniter, real sitertime,
   real serr);                                      /* src/scalar.c:264-295 */
+/* ParaView output under <case>/output (src/vtk.c:29-405): out.pvd, and per
+ * output out_<n>.pvtr/.vtr (p, phase, phase_shell, vel) and out_<n>.pvtp/.vtp
+ * (particle centres, n, r, pvel).  recorder_out pulls and writes when
+ * VTK_FLOW_FIELD in record.config says an output is due. */
+void recorder_out(void);                           /* src/recorder.c:91-155 */
+void vtk_recorder_init(void);                      /* src/vtk.c:29-38 */
+void vtk_recorder_write(void);                     /* src/vtk.c:40-60 */
+void init_VTK(void);                               /* src/vtk.c:62-92 */
+void out_VTK(void);                                /* src/vtk.c:94-129 */
+void dom_out_VTK(void);                            /* src/vtk.c:131-309 */
+void part_out_VTK(void);                           /* src/vtk.c:311-404 */
 
 /* Driver: src/bluebottle.c:28-436 split into its phases so that a caller can
  * step the solver one timestep at a time.  bluebottle_main() is the whole
@@ -401,6 +412,7 @@ void cuda_dom_malloc_host(void);             /* src/cuda_bluebottle.cu:76-141 */
 void cuda_dom_malloc_dev(void);              /* src/cuda_bluebottle.cu:144-432 */
 void cuda_dom_push(void);                    /* :443-490, host layouts -> device */
 void cuda_dom_pull(void);                    /* :492-509, device -> host layouts */
+void cuda_dom_pull_phase(void);              /* :1455, phase and phase_shell */
 void cuda_blocks_init(void);                 /* :511-1006 (launch shapes) */
 void cuda_update_bc(void);                   /* :435-440 */
 void cuda_dom_BC(void);                      /* :1218-1438 + periodic planes */
