@@ -43,7 +43,9 @@ METRIC = ("timesteps/sec @256^3+500 spheres (plain FP64 vs mixed+learned); "
 HBM_PEAK_GBS = 8000.0
 # Kernel-timer ids (the KT_* enum of the library).
 KT_IDS = {"spmv_search_kernel": 0, "update_kernel": 1,
-          "spmv_search_kernel_f32": 5, "update_kernel_f32": 6}
+          "spmv_search_kernel_f32": 5, "update_kernel_f32": 6,
+          "update_kernel_skip": 8, "update_kernel_pair": 9,
+          "update_kernel_f32_skip": 10, "update_kernel_f32_pair": 11}
 
 # Algorithmic bytes per interior cell of each timed kernel (one element per
 # array the kernel semantically reads or writes; FP64 = 8 B, FP32 = 4 B),
@@ -53,6 +55,12 @@ KT_IDS = {"spmv_search_kernel": 0, "update_kernel": 1,
 KERNEL_BYTES = {
     # update: reads phi, p, r, Ap; writes phi, r
     "update_kernel": 6 * 8,
+    # paired phi updates (the default): every other update reads r, Ap and
+    # writes r; the next also reads phi and both directions and writes phi
+    "update_kernel_skip": 3 * 8,
+    "update_kernel_pair": 7 * 8,
+    "update_kernel_f32_skip": 3 * 4,
+    "update_kernel_f32_pair": 7 * 4,
     # search + SpMV: reads r, p_old; writes p_new, Ap
     "spmv_search_kernel": 4 * 8,
     # the mixed form's inner sweep, FP32 vectors (phi is the correction e)
@@ -67,10 +75,13 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 37.86,
-    "update_kernel": 51.19,
-    "spmv_search_kernel_f32": 17.87,
-    "update_kernel_f32": 26.14,
+    "spmv_search_kernel": 37.80,
+    "update_kernel": 51.15,
+    "update_kernel_skip": 26.05,
+    "update_kernel_pair": 59.84,
+    "spmv_search_kernel_f32": 17.88,
+    "update_kernel_f32_skip": 13.51,
+    "update_kernel_f32_pair": 30.28,
 }
 
 
@@ -106,6 +117,12 @@ def parse():
     ap.add_argument("--tzb", type=int, default=None,
                     help="tiles per z-column of the search kernel "
                          "(0: automatic)")
+    ap.add_argument("--spmv-ntp", type=int, default=None,
+                    help="1: FP64 search kernel loads p_old with the "
+                         "non-temporal hint (default), 0: ordinary loads")
+    ap.add_argument("--defer-phi", type=int, default=None,
+                    help="1: the plain form's update kernel adds the "
+                         "phi terms of two iterations at once")
     ap.add_argument("--update-group", type=int, default=None,
                     help="1: update kernel over z-columns of tiles too")
     ap.add_argument("--mixed-x2", type=int, default=None,
@@ -318,6 +335,8 @@ def main():
         return
     for name, val in (("bb_spmv_variant", a.spmv_variant),
                       ("bb_spmv_tzb", a.tzb),
+                      ("bb_spmv_ntp", a.spmv_ntp),
+                      ("bb_defer_phi", a.defer_phi),
                       ("bb_update_group", a.update_group),
                       ("bb_mixed_x2", a.mixed_x2),
                       ("bb_pcg_batch", a.batch),

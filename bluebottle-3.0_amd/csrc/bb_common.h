@@ -262,7 +262,11 @@ void bb_fetch_scalars(int n);
 /* Optional per-kernel timing with HIP events on bb_stream (bb_reduce.hip).
  * Ids of the timed kernels: */
 enum { KT_SPMV = 0, KT_UPDATE, KT_SEARCH, KT_RHS, KT_USTAR, KT_SPMV32,
-       KT_UPDATE32, KT_SEARCH32, KT_COUNT };
+       KT_UPDATE32, KT_SEARCH32,
+       /* update kernels of paired phi updates: the one that leaves phi
+        * alone and the one that adds two terms */
+       KT_UPDATE_SKIP, KT_UPDATE_PAIR, KT_UPDATE32_SKIP, KT_UPDATE32_PAIR,
+       KT_COUNT };
 extern int bb_time_kernels;
 void bb_kt_begin(int id);
 void bb_kt_end(int id);

@@ -141,6 +141,7 @@ struct CGState {
   double rz_ref;      /* mixed form: (r, z) at the last FP64 correction */
   double relup;       /* mixed form: inner reduction of (r, z) per sweep */
   int q, done, nan_flag, inner_done;
+  double alpha;       /* alpha of the last update that left phi alone */
 };
 
 /* ------------------------------ Jacobi ---------------------------------- */
@@ -232,6 +233,7 @@ cg_state_init_kernel(const double *__restrict__ bpart,
     st->done = 0;
     st->nan_flag = 0;
     st->inner_done = 0;
+    st->alpha = 0.;
   }
 }
 
@@ -402,7 +404,7 @@ template <typename V> struct PlaneRawM {
   V r0, p0, r1, p1;
 };
 
-template <bool FIRST, typename V>
+template <bool FIRST, typename V, bool NTP = true>
 __device__ inline PlaneRawM<V> plane_load_nb(const DevDom &D,
   const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
   const V *__restrict__ pold, int kk)
@@ -414,8 +416,8 @@ __device__ inline PlaneRawM<V> plane_load_nb(const DevDom &D,
   unsigned c1 = (unsigned) (E1.cxy >= 0 ? E1.cxy : 0) + off;
   R.r0 = ld32(r, c0);
   R.r1 = ld32(r, c1);
-  R.p0 = FIRST ? (V) 0 : ld32_pold(pold, c0);
-  R.p1 = FIRST ? (V) 0 : ld32_pold(pold, c1);
+  R.p0 = FIRST ? (V) 0 : (NTP ? ld32_pold(pold, c0) : ld32(pold, c0));
+  R.p1 = FIRST ? (V) 0 : (NTP ? ld32_pold(pold, c1) : ld32(pold, c1));
   return R;
 }
 
@@ -550,7 +552,7 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
  * step and reloaded with plane k + 1 + DEPTH), so DEPTH planes of loads
  * are in flight behind the compute; the LDS planes rotate through three
  * buffers with one barrier per plane. */
-template <bool FIRST, typename V, bool MIXED, int DEPTH>
+template <bool FIRST, typename V, bool MIXED, int DEPTH, bool NTP>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
@@ -579,14 +581,14 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   bool colok = (x <= D.nx) && (j <= D.ny);
   PlaneEntry E0 = plane_entry(D, xb, jb, threadIdx.x);
   PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
-  PlaneRawM<V> Rp = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0 - 1);
-  PlaneRawM<V> Rc = plane_load_nb<FIRST, V>(D, E0, E1, r, pold, k0);
+  PlaneRawM<V> Rp = plane_load_nb<FIRST, V, NTP>(D, E0, E1, r, pold, k0 - 1);
+  PlaneRawM<V> Rc = plane_load_nb<FIRST, V, NTP>(D, E0, E1, r, pold, k0);
   /* loads are unconditional (plane index clamped to kend + 1): a load
    * issued on one path only would make the compiler wait for all */
   PlaneRawM<V> R[DEPTH];
 #pragma unroll
   for (int d = 0; d < DEPTH; d++)
-    R[d] = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
+    R[d] = plane_load_nb<FIRST, V, NTP>(D, E0, E1, r, pold,
                                    min(k0 + 1 + d, kend + 1));
   double beta64 = 0.;
   if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
@@ -617,7 +619,7 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
    * the never-read corner ghost cell 0 and add nothing to the partial). */
   auto step = [&](int k, PlaneRawM<V> &Rs) {
     plane_store_nb<FIRST, V>(D, s_invm, E0, E1, Rs, beta, bnext, k + 1);
-    Rs = plane_load_nb<FIRST, V>(D, E0, E1, r, pold,
+    Rs = plane_load_nb<FIRST, V, NTP>(D, E0, E1, r, pold,
                                  min(k + 1 + DEPTH, kend + 1));
     __syncthreads();
     const V *P = bcur;
@@ -1225,17 +1227,27 @@ search_spmv_direct_kernel(DevDom D, Coef K, InvMTab T, CGState *st, int q,
 /* (src/solver_kernel.cu:838-862): alpha = (r, z) / (p, Ap); phi += alpha p,
  * r -= alpha Ap, z = invM r + tile partials of (r, z).  In the mixed form
  * phi is the FP32 correction accumulated since the last FP64 correction. */
-template <typename V, bool MASK>
+/* PHI selects the phi update: PHI_EACH adds alpha p; PHI_SKIP leaves phi
+ * alone and keeps alpha in the CG state; PHI_PAIR then adds both terms,
+ * (phi + alpha' p') + alpha p with p' = pprev the previous iteration's
+ * direction (still in the other p buffer): the same operations in the same
+ * order as two PHI_EACH updates, hence the same bits, with phi read and
+ * written once per two iterations (8 B per cell per iteration less). */
+enum { PHI_EACH = 0, PHI_SKIP = 1, PHI_PAIR = 2 };
+
+template <typename V, bool MASK, int PHI = PHI_EACH>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
-update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
+update_kernel(DevDom D, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ ppart, long ntiles,
   const V *__restrict__ p, const V *__restrict__ Ap,
   const uint8_t *__restrict__ fluid, V *__restrict__ phi,
-  V *__restrict__ r, double *__restrict__ gpart)
+  V *__restrict__ r, double *__restrict__ gpart,
+  const V *__restrict__ pprev = NULL)
 {
   __shared__ V s_invm[27];
   if (st->done | st->inner_done) return;
+  const V alpha1 = PHI == PHI_PAIR ? (V) st->alpha : (V) 0;
   if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   TILE_PROLOGUE
   /* A whole tile (the common case) runs with no load or store behind a
@@ -1250,23 +1262,26 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
     V pv[4], apv[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      pv[i] = ld32(p, c0 + i * sz);
+      pv[i] = PHI == PHI_SKIP ? (V) 0 : ld32(p, c0 + i * sz);
       apv[i] = ld32_nt(Ap, c0 + i * sz);
     }
     V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
+    if (PHI == PHI_SKIP && blockIdx.x == 0 && threadIdx.x == 0)
+      st->alpha = alpha;
     double acc = 0.0;
 #pragma unroll
     for (int h = 0; h < TILE_Z; h += 4) {
-      V ph[4], rv[4];
+      V ph[4], rv[4], pp[4];
       uint32_t f[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         unsigned c = c0 + (h + i) * sz;
         if (h > 0) {
-          pv[i] = ld32(p, c);
+          if (PHI != PHI_SKIP) pv[i] = ld32(p, c);
           apv[i] = ld32_nt(Ap, c);
         }
-        ph[i] = ld32(phi, c);
+        ph[i] = PHI == PHI_SKIP ? (V) 0 : ld32(phi, c);
+        pp[i] = PHI == PHI_PAIR ? ld32(pprev, c) : (V) 0;
         rv[i] = ld32(r, c);
         f[i] = MASK ? (uint32_t) fluid[c] : 1u;
       }
@@ -1274,7 +1289,11 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
       for (int i = 0; i < 4; i++) {
         int kk = h + i;
         unsigned c = c0 + kk * sz;
-        st32(phi, c, (V) (ph[i] + alpha*pv[i]));
+        if (PHI == PHI_EACH) st32(phi, c, (V) (ph[i] + alpha*pv[i]));
+        if (PHI == PHI_PAIR) {
+          V t = ph[i] + alpha1*pp[i];
+          st32(phi, c, (V) (t + alpha*pv[i]));
+        }
         V rr = rv[i] - alpha*apv[i];
         rr = f[i] ? rr : (V) 0;
         st32(r, c, rr);
@@ -1297,11 +1316,13 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
   for (int i = 0; i < 4; i++) {
     pv[i] = apv[i] = 0;
     if (colok && k0 + i <= k1) {
-      pv[i] = p[cb + (long) i * C.sz];
+      if (PHI != PHI_SKIP) pv[i] = p[cb + (long) i * C.sz];
       apv[i] = Ap[cb + (long) i * C.sz];
     }
   }
   V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
+  if (PHI == PHI_SKIP && blockIdx.x == 0 && threadIdx.x == 0)
+    st->alpha = alpha;
   double acc = 0.0;
   if (colok) {
 #pragma unroll
@@ -1312,10 +1333,14 @@ update_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
         if (k0 + kk > k1) continue;
         long c = cb + (long) kk * C.sz;
         if (h > 0) {
-          pv[i] = p[c];
+          if (PHI != PHI_SKIP) pv[i] = p[c];
           apv[i] = Ap[c];
         }
-        phi[c] += alpha*pv[i];
+        if (PHI == PHI_EACH) phi[c] += alpha*pv[i];
+        if (PHI == PHI_PAIR) {
+          V t = phi[c] + alpha1*pprev[c];
+          phi[c] = t + alpha*pv[i];
+        }
         V rr = r[c] - alpha*apv[i];
         rr = solid(fluid, c) ? (V) 0 : rr;
         r[c] = rr;
@@ -1402,6 +1427,22 @@ update_solution_kernel(DevDom D, const CGState *st, int q,
 {
   if (st->done) return;
   double alpha = st->rzs[q & 1]/tile_block_sum(ppart, ntiles);
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
+    phi[c] += alpha*p[c];
+  }
+}
+
+/* phi += alpha p for the term a PHI_SKIP update left when the solve stops
+ * right after it; alpha is the one that update kept. */
+template <typename V>
+__global__ void __launch_bounds__(TILE_THREADS)
+phi_finish_kernel(DevDom D, const CGState *st,
+  const V *__restrict__ p, V *__restrict__ phi)
+{
+  V alpha = (V) st->alpha;
   TILE_PROLOGUE
   if (!colok) return;
   for (int k = k0; k <= k1; k++) {
@@ -1707,6 +1748,11 @@ int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
                                           planes, 1: one tile per workgroup,
                                           0: direct loads */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
+int bb_spmv_ntp = 1;                   /* 1: FP64 search loads p_old with the
+                                          non-temporal hint (0: ordinary
+                                          loads, measured slower) */
+int bb_defer_phi = 1;                  /* 1: phi updated every second
+                                          iteration with both terms */
 int bb_update_group = 0;               /* 1: update kernel over z-columns too */
 int bb_mixed_x2 = 0;                   /* 1: FP32 kernels on x pairs (even nx);
                                           0 (default): the one-cell z-column
@@ -1939,9 +1985,41 @@ static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   }
   /* two planes in flight: three (FP32: 72.4 against 71.5 us at 256^3) and
    * four did not help, and past two the FP64 kernel spills at 8 waves */
-  hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2>),
-    spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-    _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+  if (bb_spmv_ntp)
+    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, true>),
+      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+  else
+    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, false>),
+      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+}
+
+/* Paired phi updates (bb_defer_phi, plain form, one-tile update kernel):
+ * odd iterations leave phi to the next one, except before an iteration
+ * that recomputes the true residual (every 50th), whose phi update adds
+ * its own term only. */
+static bool update_is_group(void)
+{
+  return bb_spmv_variant == 2 && bb_update_group && spmv_tzb() > 1;
+}
+
+static bool phi_skips(int q)
+{
+  return bb_defer_phi && !update_is_group() && (q & 1) && (q + 1) % 50 != 0;
+}
+
+/* The mixed form's inner sweep pairs its updates of the correction e the
+ * same way, counting from the first iteration after each FP64 correction
+ * (where e is folded into phi and must be whole). */
+static int pair_base32 = 1;
+
+static bool use_x2(void);
+
+static bool phi_skips32(int q)
+{
+  return bb_defer_phi && !update_is_group() && !use_x2() &&
+         ((q - pair_base32) & 1) == 0;
 }
 
 /* Enqueue iteration q; p buffers alternate by parity. */
@@ -1981,23 +2059,33 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
     hipLaunchKernelGGL(residual_kernel, TILES_LAUNCH, ddom, K, T, _cg_state,
       _phi, _rhs_p, fluid_mask(), _r_q, _part_r);
   } else {
-    bb_kt_begin(KT_UPDATE);
+    int mode = phi_skips(q) ? PHI_SKIP
+             : (q > 1 && phi_skips(q - 1)) ? PHI_PAIR : PHI_EACH;
+    int kt = mode == PHI_SKIP ? KT_UPDATE_SKIP
+           : mode == PHI_PAIR ? KT_UPDATE_PAIR : KT_UPDATE;
+    bb_kt_begin(kt);
     int tzb = spmv_tzb();
     if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
       hipLaunchKernelGGL(update_group_kernel<double>, spmv_group_grid(tzb),
         tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
         pnew, _Apb_q, fluid_mask(), _phi, _r_q, _part_r);
     } else {
-      if (fluid_mask())
-        hipLaunchKernelGGL((update_kernel<double, true>), TILES_LAUNCH, ddom,
-          T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi,
-          _r_q, _part_r);
-      else
-        hipLaunchKernelGGL((update_kernel<double, false>), TILES_LAUNCH, ddom,
-          T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fluid_mask(), _phi,
-          _r_q, _part_r);
+      const uint8_t *fm = fluid_mask();
+#define UPD(M, P) hipLaunchKernelGGL((update_kernel<double, M, P>),          \
+        TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fm, \
+        _phi, _r_q, _part_r, pold)
+      if (fm) {
+        if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+        else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+        else UPD(true, PHI_EACH);
+      } else {
+        if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+        else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+        else UPD(false, PHI_EACH);
+      }
+#undef UPD
     }
-    bb_kt_end(KT_UPDATE);
+    bb_kt_end(kt);
   }
   BB_LAUNCH_CHECK();
 }
@@ -2047,6 +2135,12 @@ static void pcg_plain(double ts, int warm)
     q = qend;
     fetch_state();
     if (cg_host->done) break;
+  }
+  /* a solve that stops right after a PHI_SKIP update: add its term */
+  if (cg_host->done && cg_host->q >= 1 && phi_skips(cg_host->q)) {
+    hipLaunchKernelGGL(phi_finish_kernel<double>, TILES_LAUNCH, ddom,
+      _cg_state, (cg_host->q & 1) ? _p_q : _p_alt, _phi);
+    BB_LAUNCH_CHECK();
   }
   real sp_rq1 = cg_host->rz1;
   if (cg_host->done && cg_host->nan_flag) {
@@ -2172,23 +2266,33 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
       _Ap32, _part_p);
   }
   bb_kt_end(KT_SPMV32);
-  bb_kt_begin(KT_UPDATE32);
+  int mode = phi_skips32(q) ? PHI_SKIP
+           : (q > pair_base32 && phi_skips32(q - 1)) ? PHI_PAIR : PHI_EACH;
+  int kt = mode == PHI_SKIP ? KT_UPDATE32_SKIP
+         : mode == PHI_PAIR ? KT_UPDATE32_PAIR : KT_UPDATE32;
+  bb_kt_begin(kt);
   int tzb = spmv_tzb();
   if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
     hipLaunchKernelGGL(update_group_kernel<float>, spmv_group_grid(tzb),
       tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
       pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
   } else {
-    if (fluid_mask())
-      hipLaunchKernelGGL((update_kernel<float, true>), TILES_LAUNCH, ddom, T,
-        _cg_state, q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32,
-        _part_r);
-    else
-      hipLaunchKernelGGL((update_kernel<float, false>), TILES_LAUNCH, ddom, T,
-        _cg_state, q, _part_p, nt, pnew, _Ap32, fluid_mask(), _e32, _r32,
-        _part_r);
+    const uint8_t *fm = fluid_mask();
+#define UPD(M, P) hipLaunchKernelGGL((update_kernel<float, M, P>),           \
+      TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Ap32, fm,    \
+      _e32, _r32, _part_r, pold)
+    if (fm) {
+      if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+      else UPD(true, PHI_EACH);
+    } else {
+      if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+      else UPD(false, PHI_EACH);
+    }
+#undef UPD
   }
-  bb_kt_end(KT_UPDATE32);
+  bb_kt_end(kt);
   rp_n = nt;
   BB_LAUNCH_CHECK();
 }
@@ -2249,6 +2353,7 @@ static void pcg_mixed(double ts, int warm)
     return;
   }
   int q = 0, first_q = 1, nouter = 1;
+  pair_base32 = 1;
   int done = cg_host->rz1 <= pp_residual*pp_residual*sp_rhs;
   while (!done && q <= pp_max_iter && nouter <= pp_max_iter) {
     int qend = q + bb_pcg_batch;
@@ -2263,6 +2368,11 @@ static void pcg_mixed(double ts, int warm)
       continue;
     }
     q = cg_host->q;
+    /* e must be whole before the fold: add a skipped update's term */
+    if (q >= pair_base32 && phi_skips32(q))
+      hipLaunchKernelGGL(phi_finish_kernel<float>, TILES_LAUNCH, ddom,
+        _cg_state, (q & 1) ? _p32a : _p32b, _e32);
+    pair_base32 = q + 1;
     hipLaunchKernelGGL(mx_fold_kernel, TILES_LAUNCH, ddom, _phi, _e32);
     hipLaunchKernelGGL(mx_residual_kernel, TILES_LAUNCH, ddom, K, T, _phi,
       _rhs_p, fluid_mask(), _r32, _part_r);

@@ -28,11 +28,16 @@ FIELDS = ("u", "v", "w", "p")
 
 
 def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0,
-              variant=2):
+              variant=2, defer=None):
     """x2 = 1: the FP32 kernels on x pairs (used when nx is even), 0: one
     cell per lane (the default); variant: the search kernel (2: LDS planes,
-    3: one wavefront per row)."""
+    3: one wavefront per row); defer: 1 pairs the inner sweep's updates of
+    the correction, 0 updates it every iteration (None: library default)."""
     lib = bbhip.load()
+    defer_knob = ctypes.c_int.in_dll(lib, "bb_defer_phi")
+    defer_default = defer_knob.value
+    if defer is not None:
+        defer_knob.value = defer
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = restart
     ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = x2
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
@@ -61,6 +66,7 @@ def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0,
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = 0
     ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = 0
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 2
+    defer_knob.value = defer_default
     return res
 
 
@@ -132,3 +138,23 @@ def test_mixed_inner_tolerances(case_dir, inner_tol):
     kw["hi"] = (1.3, 0.7, 1.1)
     res = run_mixed(case_dir, kw, 5, inner_tol=inner_tol)
     check_mixed(res, {"u": 1e-8, "v": 2e-5, "w": 1e-8, "p": 1e-8})
+
+
+@pytest.mark.parametrize("inner_tol", [None, 0.3])
+def test_mixed_paired_updates_same_bits(case_dir, inner_tol):
+    """Pairing the inner sweep's updates of the correction (two terms added
+    in one pass, in the same order) gives the same bits as updating it
+    every iteration, across FP64 corrections (inner_tol 0.3 makes many)."""
+    kw = bbconfig.cavity(n=16, steps=3)
+    kw["n"] = (37, 11, 23)
+    kw["hi"] = (1.3, 0.7, 1.1)
+    a = run_mixed(case_dir, kw, 3, inner_tol=inner_tol, defer=0)
+    b = run_mixed(case_dir, kw, 3, inner_tol=inner_tol, defer=1)
+    assert [st["niter"] for st in a["stats"]] == \
+        [st["niter"] for st in b["stats"]]
+    assert [st["nouter"] for st in a["stats"]] == \
+        [st["nouter"] for st in b["stats"]]
+    if inner_tol is not None:
+        assert max(st["nouter"] for st in b["stats"]) > 2
+    for f in FIELDS:
+        assert np.array_equal(a["gpu"][f], b["gpu"][f]), f

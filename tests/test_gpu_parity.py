@@ -20,21 +20,32 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def set_knobs(variant=2, tzb=0, batch=16, group=0):
+def set_knobs(variant=2, tzb=0, batch=16, group=0, defer=0):
     """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
     3: the same z-columns with one wavefront per row and no LDS planes,
     1: one tile per workgroup, 0: direct neighbour loads; group = 1: the
-    update kernel over the same z-columns."""
+    update kernel over the same z-columns; defer = 1: phi updated every
+    second iteration with both terms (odd iterations leave it)."""
     lib = bbhip.load()
+    ctypes.c_int.in_dll(lib, "bb_defer_phi").value = defer
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_spmv_tzb").value = tzb
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = batch
     ctypes.c_int.in_dll(lib, "bb_update_group").value = group
 
 
-def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, group=0):
+@pytest.fixture(autouse=True)
+def default_knobs():
+    """Each test sets its knobs; the library keeps them, so put the
+    defaults back for the tests that follow."""
+    yield
+    set_knobs()
+
+
+def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, group=0,
+             defer=0):
     n = kw["n"]
-    set_knobs(variant, tzb, batch, group)
+    set_knobs(variant, tzb, batch, group, defer)
     bbconfig.write_case(case_dir, **kw)
     out = {"gpu": {}, "orc": {}, "iters_gpu": [], "iters_orc": []}
     with orc.Oracle(case_dir) as o:
@@ -94,22 +105,24 @@ def test_channel_small(case_dir):
     assert out["gpu_dt"] == out["orc_dt"]
 
 
-@pytest.mark.parametrize("variant,tzb,group", [(0, 0, 0), (1, 0, 0),
-                                               (2, 1, 0), (2, 4, 0),
-                                               (2, 2, 1), (3, 4, 0)])
-def test_channel_bitwise(case_dir, variant, tzb, group):
+@pytest.mark.parametrize("variant,tzb,group,defer",
+                         [(0, 0, 0, 0), (1, 0, 0, 0), (2, 1, 0, 0),
+                          (2, 4, 0, 0), (2, 2, 1, 0), (3, 4, 0, 0),
+                          (2, 4, 0, 1), (2, 1, 0, 1)])
+def test_channel_bitwise(case_dir, variant, tzb, group, defer):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
-                   tzb, group=group)
+                   tzb, group=group, defer=defer)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,tzb,batch,group",
-                         [(0, 0, 16, 0), (1, 0, 16, 0), (1, 0, 5, 0),
-                          (2, 2, 1, 0), (2, 3, 16, 1), (2, 4, 5, 0),
-                          (2, 2, 16, 1), (3, 3, 16, 0), (3, 2, 5, 0)])
-def test_ragged_sizes(case_dir, variant, tzb, batch, group):
+@pytest.mark.parametrize("variant,tzb,batch,group,defer",
+                         [(0, 0, 16, 0, 0), (1, 0, 16, 0, 0), (1, 0, 5, 0, 0),
+                          (2, 2, 1, 0, 0), (2, 3, 16, 1, 0), (2, 4, 5, 0, 0),
+                          (2, 2, 16, 1, 0), (3, 3, 16, 0, 0), (3, 2, 5, 0, 0),
+                          (2, 2, 1, 0, 1), (2, 4, 5, 0, 1), (2, 3, 16, 0, 1)])
+def test_ragged_sizes(case_dir, variant, tzb, batch, group, defer):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
     3, not a multiple of the z-column length), shear walls."""
@@ -117,7 +130,7 @@ def test_ragged_sizes(case_dir, variant, tzb, batch, group):
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     kw["vel_bc"] = bbconfig.walls(lid=("v", "E", 0.5))
-    out = run_pair(case_dir, kw, 3, variant, tzb, batch, group)
+    out = run_pair(case_dir, kw, 3, variant, tzb, batch, group, defer)
     check(out)
     assert out["iters_gpu"] == out["iters_orc"]
     for f in FIELDS:
@@ -142,8 +155,8 @@ def test_periodic_all_and_wide_x(case_dir):
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant", [2, 3])
-def test_periodic_all_25d(case_dir, variant):
+@pytest.mark.parametrize("variant,defer", [(2, 0), (3, 0), (2, 1)])
+def test_periodic_all_25d(case_dir, variant, defer):
     per = ("PERIODIC",)
     vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
     kw = dict(n=(40, 12, 10), lo=(0.0, 0.0, 0.0), hi=(4.0, 1.2, 1.0),
@@ -151,7 +164,7 @@ def test_periodic_all_25d(case_dir, variant):
               pp_residual=1e-8, p_bc={f: per for f in "WESNBT"}, vel_bc=vel,
               init_cond="QUIESCENT",
               extensions={"fixed_dt": 1e-3, "perturb": (6, 0.2)})
-    out = run_pair(case_dir, kw, 3, variant=variant, tzb=2)
+    out = run_pair(case_dir, kw, 3, variant=variant, tzb=2, defer=defer)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
