@@ -79,7 +79,7 @@ TRAFFIC_PER_CELL = {
     "update_kernel": 51.15,
     "update_kernel_skip": 26.05,
     "update_kernel_pair": 59.84,
-    "spmv_search_kernel_f32": 17.88,
+    "spmv_search_kernel_f32": 17.71,
     "update_kernel_f32_skip": 13.51,
     "update_kernel_f32_pair": 30.28,
 }
@@ -120,6 +120,14 @@ def parse():
     ap.add_argument("--spmv-ntp", type=int, default=None,
                     help="1: FP64 search kernel loads p_old with the "
                          "non-temporal hint (default), 0: ordinary loads")
+    ap.add_argument("--spmv-nts", type=int, default=None,
+                    help="1: the FP64 search stores p_new and Ap with the "
+                         "non-temporal hint (default), 0: ordinary stores")
+    ap.add_argument("--spmv-nts32", type=int, default=None,
+                    help="the same for the FP32 search (default 1)")
+    ap.add_argument("--tzb32", type=int, default=None,
+                    help="tiles per z-column of the FP32 search "
+                         "(0: automatic)")
     ap.add_argument("--defer-phi", type=int, default=None,
                     help="1: the plain form's update kernel adds the "
                          "phi terms of two iterations at once")
@@ -337,6 +345,9 @@ def main():
                       ("bb_spmv_tzb", a.tzb),
                       ("bb_spmv_ntp", a.spmv_ntp),
                       ("bb_defer_phi", a.defer_phi),
+                      ("bb_spmv_nts", a.spmv_nts),
+                      ("bb_spmv_nts32", a.spmv_nts32),
+                      ("bb_spmv_tzb32", a.tzb32),
                       ("bb_update_group", a.update_group),
                       ("bb_mixed_x2", a.mixed_x2),
                       ("bb_pcg_batch", a.batch),

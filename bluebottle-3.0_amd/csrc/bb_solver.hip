@@ -552,7 +552,8 @@ search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
  * step and reloaded with plane k + 1 + DEPTH), so DEPTH planes of loads
  * are in flight behind the compute; the LDS planes rotate through three
  * buffers with one barrier per plane. */
-template <bool FIRST, typename V, bool MIXED, int DEPTH, bool NTP>
+template <bool FIRST, typename V, bool MIXED, int DEPTH, bool NTP,
+          bool NTS = true>
 __global__ void __launch_bounds__(TILE_THREADS)
 __attribute__((amdgpu_waves_per_eu(8, 8)))
 search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
@@ -629,8 +630,13 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
                       nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
                       wall_hi(k, D.nz, D.perz));
     unsigned c = colok ? (unsigned) gidx(C, x, j, k) : 0u;
-    st32_nt(pnew, c, cur);
-    st32_nt(Ap, c, a);
+    if (NTS) {
+      st32_nt(pnew, c, cur);
+      st32_nt(Ap, c, a);
+    } else {
+      st32(pnew, c, cur);
+      st32(Ap, c, a);
+    }
     acc += colok ? (double) cur * (double) a : 0.0;
     prev = cur;
     cur = nxt;
@@ -1751,6 +1757,15 @@ int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_spmv_ntp = 1;                   /* 1: FP64 search loads p_old with the
                                           non-temporal hint (0: ordinary
                                           loads, measured slower) */
+int bb_spmv_nts = 1;                   /* 1: the FP64 search stores p_new and
+                                          Ap with the non-temporal hint */
+int bb_spmv_nts32 = 1;                 /* the same for the FP32 search (with
+                                          8-tile columns ordinary stores take
+                                          the same time per step, but the
+                                          update kernels absorb their
+                                          write-back) */
+int bb_spmv_tzb32 = 0;                 /* tiles per z-column of the FP32
+                                          search (0: automatic) */
 int bb_defer_phi = 1;                  /* 1: phi updated every second
                                           iteration with both terms */
 int bb_update_group = 0;               /* 1: update kernel over z-columns too */
@@ -1954,6 +1969,17 @@ static int spmv_tzb(void)
   return t < 1 ? 1 : (t > 4 ? 4 : (int) t);
 }
 
+/* The FP32 search moves half the bytes per cell: longer columns (up to 8
+ * tiles, two workgroups per CU at 256^3) measured faster for it, and
+ * slower for the FP64 one. */
+static int spmv_tzb32(void)
+{
+  if (bb_spmv_tzb32 > 0) return bb_spmv_tzb32 < 8 ? bb_spmv_tzb32 : 8;
+  if (bb_spmv_tzb > 0) return spmv_tzb();
+  long t = n_tiles(ddom) / 512;
+  return t < 1 ? 1 : (t > 8 ? 8 : (int) t);
+}
+
 static dim3 spmv_group_grid(int tzb)
 {
   long ntx = (ddom.nx + TILE_X - 1) / TILE_X;
@@ -1976,7 +2002,9 @@ template <bool FIRST, typename V, bool MIXED>
 static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, const V *r, const V *pold, V *pnew, V *Ap)
 {
-  int tzb = spmv_tzb();
+  const bool f32 = sizeof(V) == 4;
+  int tzb = f32 ? spmv_tzb32() : spmv_tzb();
+  int nts = f32 ? bb_spmv_nts32 : bb_spmv_nts;
   if (bb_spmv_variant == 3) {
     hipLaunchKernelGGL((search_spmv_rows_kernel<FIRST, V, MIXED>),
       spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
@@ -1985,7 +2013,11 @@ static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   }
   /* two planes in flight: three (FP32: 72.4 against 71.5 us at 256^3) and
    * four did not help, and past two the FP64 kernel spills at 8 waves */
-  if (bb_spmv_ntp)
+  if (!nts)
+    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, true,
+      false>), spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
+      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+  else if (bb_spmv_ntp)
     hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, true>),
       spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
       _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);

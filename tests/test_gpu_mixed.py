@@ -158,3 +158,25 @@ def test_mixed_paired_updates_same_bits(case_dir, inner_tol):
         assert max(st["nouter"] for st in b["stats"]) > 2
     for f in FIELDS:
         assert np.array_equal(a["gpu"][f], b["gpu"][f]), f
+
+
+def test_mixed_search_settings_same_bits(case_dir):
+    """The FP32 search's column length (tiles per z-column) and store hint
+    change neither its arithmetic nor the order of its partial sums: the
+    runs agree bit for bit (16 z-tiles: columns of 1, 3 and 8 tiles)."""
+    lib = bbhip.load()
+    kw = bbconfig.channel(n=(64, 32, 128))
+    out = []
+    for tzb32, nts32 in ((1, 1), (3, 0), (8, 0)):
+        ctypes.c_int.in_dll(lib, "bb_spmv_tzb32").value = tzb32
+        ctypes.c_int.in_dll(lib, "bb_spmv_nts32").value = nts32
+        try:
+            out.append(run_mixed(case_dir, kw, 2))
+        finally:
+            ctypes.c_int.in_dll(lib, "bb_spmv_tzb32").value = 0
+            ctypes.c_int.in_dll(lib, "bb_spmv_nts32").value = 1
+    for res in out[1:]:
+        assert [st["niter"] for st in res["stats"]] == \
+            [st["niter"] for st in out[0]["stats"]]
+        for f in FIELDS:
+            assert np.array_equal(res["gpu"][f], out[0]["gpu"][f]), f

@@ -108,7 +108,7 @@ def test_channel_small(case_dir):
 @pytest.mark.parametrize("variant,tzb,group,defer",
                          [(0, 0, 0, 0), (1, 0, 0, 0), (2, 1, 0, 0),
                           (2, 4, 0, 0), (2, 2, 1, 0), (3, 4, 0, 0),
-                          (2, 4, 0, 1), (2, 1, 0, 1)])
+                          (2, 4, 0, 1), (2, 1, 0, 1), (2, 8, 0, 1)])
 def test_channel_bitwise(case_dir, variant, tzb, group, defer):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
                    tzb, group=group, defer=defer)
@@ -121,7 +121,8 @@ def test_channel_bitwise(case_dir, variant, tzb, group, defer):
                          [(0, 0, 16, 0, 0), (1, 0, 16, 0, 0), (1, 0, 5, 0, 0),
                           (2, 2, 1, 0, 0), (2, 3, 16, 1, 0), (2, 4, 5, 0, 0),
                           (2, 2, 16, 1, 0), (3, 3, 16, 0, 0), (3, 2, 5, 0, 0),
-                          (2, 2, 1, 0, 1), (2, 4, 5, 0, 1), (2, 3, 16, 0, 1)])
+                          (2, 2, 1, 0, 1), (2, 4, 5, 0, 1), (2, 3, 16, 0, 1),
+                          (2, 8, 16, 0, 1)])
 def test_ragged_sizes(case_dir, variant, tzb, batch, group, defer):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
