@@ -626,6 +626,75 @@ extern "C" void cuda_find_dt(void)
   dt = CFL/dt;
 }
 
+/* ------------------------- turbulence forcing ---------------------------- */
+extern "C" void cuda_compute_turb_forcing(void)
+{
+  /* src/cuda_bluebottle.cu:1985-2093 acts only when init_cond is TURBULENT
+   * (linear forcing proportional to the velocity field, with kinetic energy
+   * and dissipation records).  That forcing needs per-face force arrays,
+   * which this library does not keep (its u* kernels add a constant force
+   * per component), so such a case stops here with a message. */
+  if (init_cond != TURBULENT) return;
+  fprintf(stderr, "N%d >> init_cond TURBULENT (linear turbulence forcing) is "
+    "not supported by this library\n", rank);
+  exit(EXIT_FAILURE);
+}
+
+/* ------------------------- wall shear stress ----------------------------- */
+/* Row sums over x of (u(i, j + 1, k) - u(i, j, k)) / dy for the interior
+ * x faces i = 1..ni of plane k (one wavefront per k). */
+__global__ void __launch_bounds__(256)
+dudy_rows_kernel(Grid G, const double *__restrict__ u, int j, double idy,
+  double *__restrict__ rows)
+{
+  int lane = threadIdx.x & (WAVE - 1);
+  long row = (long) blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
+  if (row >= G.nk) return;
+  int k = (int) row + 1;
+  double acc = 0.;
+  for (int i = lane + 1; i <= G.ni; i += WAVE)
+    acc += (u[gidx(G, i, j + 1, k)] - u[gidx(G, i, j, k)]) * idy;
+  acc = wave_tree_sum(acc);
+  if (lane == 0) rows[row] = acc;
+}
+
+extern "C" void cuda_wall_shear_stress(void)
+{
+  /* src/cuda_bluebottle.cu:2764-2835: the mean du/dy on the south face
+   * (between the ghost row and the first row) and on the north face
+   * (between the last row and the ghost row), over the in x kn interior x
+   * faces, written as rho_f nu du/dy to output/wss.dat.  The reference's
+   * per-face kernel maps (i, k) and (i + 1, 0) onto one slot for k = kn;
+   * here each face is summed once. */
+  const Grid &G = ddom.fx;
+  long nrows = G.nk;
+  dim3 grid((unsigned) ((nrows + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK));
+  hipLaunchKernelGGL(dudy_rows_kernel, grid, rows_block(), 0, bb_stream, G,
+    _u, 0, 1. / dom[0].dy, _bb_rows);
+  bb_reduce_rows(_bb_rows, nrows, _bb_scalars + 12);
+  hipLaunchKernelGGL(dudy_rows_kernel, grid, rows_block(), 0, bb_stream, G,
+    _u, G.nj, 1. / dom[0].dy, _bb_rows2);
+  bb_reduce_rows(_bb_rows2, nrows, _bb_scalars + 13);
+  BB_LAUNCH_CHECK();
+  bb_fetch_scalars(14);
+  double nface = (double) dom[0].Gfx.in * dom[0].Gfx.kn;
+  double dudy_s = bb_host_scalars[12] / nface;
+  double dudy_n = bb_host_scalars[13] / nface;
+  if (rank != 0) return;
+  char fname[FILE_NAME_SIZE];
+  snprintf(fname, sizeof fname, "%s/%s/wss.dat", ROOT_DIR, OUTPUT_DIR);
+  FILE *file = fopen(fname, stepnum == 1 ? "w" : "a");
+  if (file == NULL) {
+    fprintf(stderr, "Could not open file %s\n", fname);
+    exit(EXIT_FAILURE);
+  }
+  if (stepnum == 1)
+    fprintf(file, "%-9s%-11s%-11s%-11s", "stepnum", "ttime", "wss-s", "wss-n");
+  fprintf(file, "\n%-9d%-11.3e%-11.3e%-11.3e", stepnum, ttime,
+    rho_f*nu*dudy_s, rho_f*nu*dudy_n);
+  fclose(file);
+}
+
 /* ----------------------------- AB2 history ------------------------------ */
 static void swap_ptr(real **a, real **b)
 {

@@ -111,8 +111,15 @@ extern "C" void cuda_dom_malloc_host(void)
   flag_u = (int *) host_alloc(d->Gfx.s3b * sizeof(int));
   flag_v = (int *) host_alloc(d->Gfy.s3b * sizeof(int));
   flag_w = (int *) host_alloc(d->Gfz.s3b * sizeof(int));
-  phase = (int *) host_alloc(d->Gcc.s3b * sizeof(int));
-  phase_shell = (int *) host_alloc(d->Gcc.s3b * sizeof(int));
+}
+
+extern "C" void cuda_part_malloc_host(void)
+{
+  /* src/cuda_particle.cu:47-55: the host phase arrays, needed with or
+   * without particles (called once after parts_read_input) */
+  size_t cc = dom[0].Gcc.s3b * sizeof(int);
+  if (phase == NULL) phase = (int *) host_alloc(cc);
+  if (phase_shell == NULL) phase_shell = (int *) host_alloc(cc);
 }
 
 static void *dev_alloc(size_t bytes)
@@ -285,8 +292,8 @@ extern "C" void cuda_dom_pull(void)
   pull_field(diff0_w, _diff0_w, L_GFZ);
   pull_field(flag_u, _flag_u, L_GFX); pull_field(flag_v, _flag_v, L_GFY);
   pull_field(flag_w, _flag_w, L_GFZ);
-  pull_field(phase, _phase, L_GCC);
-  pull_field(phase_shell, _phase_shell, L_GCC);
+  if (phase != NULL) pull_field(phase, _phase, L_GCC);
+  if (phase_shell != NULL) pull_field(phase_shell, _phase_shell, L_GCC);
   dom_struct *d = &dom[0];
   for (int i = 0; i < d->Gfx.s3b; i++) f_x[i] = bb_force[0];
   for (int i = 0; i < d->Gfy.s3b; i++) f_y[i] = bb_force[1];
@@ -297,8 +304,57 @@ extern "C" void cuda_dom_pull_phase(void)
 {
   /* src/cuda_bluebottle.cu:1455 */
   BB_CHECK(hipDeviceSynchronize());
+  cuda_part_malloc_host();
   pull_field(phase, _phase, L_GCC);
   pull_field(phase_shell, _phase_shell, L_GCC);
+}
+
+extern "C" void cuda_dom_pull_debug(void)
+{
+  /* src/cuda_bluebottle.cu:1464-1505: phi, the intermediate velocity, the
+   * convection and diffusion terms of this step and the face flags */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(phi, _phi, L_GCC);
+  pull_field(u_star, _u_star, L_GFX); pull_field(v_star, _v_star, L_GFY);
+  pull_field(w_star, _w_star, L_GFZ);
+  pull_field(conv_u, bb_history_rotated ? _conv0_u : _conv_u, L_GFX);
+  pull_field(conv_v, bb_history_rotated ? _conv0_v : _conv_v, L_GFY);
+  pull_field(conv_w, bb_history_rotated ? _conv0_w : _conv_w, L_GFZ);
+  pull_field(diff_u, bb_history_rotated ? _diff0_u : _diff_u, L_GFX);
+  pull_field(diff_v, bb_history_rotated ? _diff0_v : _diff_v, L_GFY);
+  pull_field(diff_w, bb_history_rotated ? _diff0_w : _diff_w, L_GFZ);
+  pull_field(flag_u, _flag_u, L_GFX); pull_field(flag_v, _flag_v, L_GFY);
+  pull_field(flag_w, _flag_w, L_GFZ);
+}
+
+extern "C" void cuda_dom_pull_restart(void)
+{
+  /* src/cuda_bluebottle.cu:1507-1530: the fields of the previous step that
+   * a restart needs besides those of cuda_dom_pull */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(p0, _p0, L_GCC);
+  pull_field(u0, _u0, L_GFX); pull_field(v0, _v0, L_GFY);
+  pull_field(w0, _w0, L_GFZ);
+  pull_field(conv0_u, _conv0_u, L_GFX); pull_field(conv0_v, _conv0_v, L_GFY);
+  pull_field(conv0_w, _conv0_w, L_GFZ);
+  pull_field(diff0_u, _diff0_u, L_GFX); pull_field(diff0_v, _diff0_v, L_GFY);
+  pull_field(diff0_w, _diff0_w, L_GFZ);
+}
+
+extern "C" void printMemInfo(void)
+{
+  /* src/cuda_scalar.cu:972-986: device memory in use.  One GPU holds every
+   * particle, so the subdomain count is nparts. */
+  size_t free_byte = 0, total_byte = 0;
+  hipError_t e = hipMemGetInfo(&free_byte, &total_byte);
+  if (e != hipSuccess) {
+    printf("Error: hipMemGetInfo fails, %s\n", hipGetErrorString(e));
+    exit(1);
+  }
+  printf("N%d >> nparts = %d, nparts_subdom = %d, GPU memory usage: used = "
+    "%zu B, free = %zu B, total = %zu B\n", rank, nparts, nparts,
+    total_byte - free_byte, free_byte, total_byte);
+  fflush(stdout);
 }
 
 /* Pull one Gcc device array (x-fastest, ghosted) into a host buffer of the
@@ -399,6 +455,21 @@ extern "C" void cuda_scalar_pull(void)
   pull_field(s, _s, L_GCC); pull_field(s0, _s0, L_GCC);
   pull_field(s_conv, _s_conv, L_GCC); pull_field(s_conv0, _s_conv0, L_GCC);
   pull_field(s_diff, _s_diff, L_GCC); pull_field(s_diff0, _s_diff0, L_GCC);
+}
+
+extern "C" void cuda_scalar_pull_debug(void)
+{
+  /* src/cuda_scalar.cu:108-115 */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(s_conv, _s_conv, L_GCC); pull_field(s_diff, _s_diff, L_GCC);
+}
+
+extern "C" void cuda_scalar_pull_restart(void)
+{
+  /* src/cuda_scalar.cu:117-126 */
+  BB_CHECK(hipDeviceSynchronize());
+  pull_field(s0, _s0, L_GCC); pull_field(s_conv0, _s_conv0, L_GCC);
+  pull_field(s_diff0, _s_diff0, L_GCC);
 }
 
 extern "C" void cuda_scalar_free(void)

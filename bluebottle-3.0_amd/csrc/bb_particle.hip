@@ -2133,4 +2133,25 @@ extern "C" void cuda_part_free(void)
   free_face_lists();
   _parts = NULL; _perr = NULL; _pnum = NULL; _pbin = NULL; _err_out = NULL;
   err_host = NULL;
+  /* the host phase arrays of cuda_part_malloc_host (src/cuda_particle.cu:
+   * 1271-1272 frees them here) */
+  if (phase) BB_CHECK(hipHostFree(phase));
+  if (phase_shell) BB_CHECK(hipHostFree(phase_shell));
+  phase = phase_shell = NULL;
 }
+
+extern "C" void cuda_part_pull_debug(void)
+{
+  /* src/cuda_particle.cu:254-265 pulls every particle, ghost copies
+   * included; one GPU holds each particle once and keeps no copies */
+  cuda_part_pull();
+}
+
+/* src/cuda_particle.cu:287-1186 move the particles that left a rank into
+ * its neighbour and copy those within a bin of a periodic face as ghosts.
+ * On one GPU no particle changes owner; cuda_update_part_position wraps a
+ * centre that leaves through a periodic face, and every kernel evaluates
+ * the periodic images x +- L itself, so there is nothing to move. */
+extern "C" void cuda_transfer_parts_i(void) {}
+extern "C" void cuda_transfer_parts_j(void) {}
+extern "C" void cuda_transfer_parts_k(void) {}

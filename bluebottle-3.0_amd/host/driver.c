@@ -84,6 +84,7 @@ int bluebottle_setup(void)
   NPARTS = 0;
   nparts = 0;
   if (parts_read_input) parts_read_input();
+  cuda_part_malloc_host();
   if (!use_restart) {
     vtk_recorder_init();
     recorder_PP_init("solver_expd.rec");

@@ -413,6 +413,14 @@ void cuda_dom_malloc_dev(void);              /* src/cuda_bluebottle.cu:144-432 *
 void cuda_dom_push(void);                    /* :443-490, host layouts -> device */
 void cuda_dom_pull(void);                    /* :492-509, device -> host layouts */
 void cuda_dom_pull_phase(void);              /* :1455, phase and phase_shell */
+void cuda_dom_pull_debug(void);              /* :1464-1505, phi, u*, conv, diff,
+                                                flags */
+void cuda_dom_pull_restart(void);            /* :1507-1530, p0, u0, conv0, diff0 */
+void cuda_compute_turb_forcing(void);        /* :1985-2093; exits with a message
+                                                when init_cond is TURBULENT
+                                                (not supported), else nothing */
+void cuda_wall_shear_stress(void);           /* :2764-2835, output/wss.dat */
+void printMemInfo(void);                     /* src/cuda_scalar.cu:972-986 */
 void cuda_blocks_init(void);                 /* :511-1006 (launch shapes) */
 void cuda_update_bc(void);                   /* :435-440 */
 void cuda_dom_BC(void);                      /* :1218-1438 + periodic planes */
@@ -440,10 +448,20 @@ void parts_read_input(void);                 /* src/particle.c:66-391 */
 void parts_init(void);                       /* src/particle.c:393-554 */
 void init_bins(void);                        /* src/particle.c:744-834 */
 void parts_free(void);                       /* src/particle.c:876-879 */
+void cuda_part_malloc_host(void);            /* src/cuda_particle.cu:47-55, host
+                                                phase and phase_shell */
 void cuda_part_malloc_dev(void);             /* src/cuda_particle.cu */
 void cuda_part_push(void);                   /* host parts -> device */
 void cuda_part_pull(void);                   /* device parts -> host */
-void cuda_part_free(void);
+void cuda_part_pull_debug(void);             /* src/cuda_particle.cu:254-265 */
+void cuda_part_free(void);                   /* :1268-1284 */
+/* src/cuda_particle.cu:287-1186 move particles between ranks and copy
+ * ghosts near periodic faces; on one GPU the kernels evaluate the periodic
+ * images themselves and cuda_update_part_position wraps centres, so these
+ * have nothing to do. */
+void cuda_transfer_parts_i(void);
+void cuda_transfer_parts_j(void);
+void cuda_transfer_parts_k(void);
 void cuda_init_physalis(void);               /* src/cuda_physalis.cu:61-135 */
 void cuda_part_BC(void);                     /* src/cuda_particle.cu:1649-1663 */
 void cuda_part_BC_star(void);                /* :1665-1679 */
@@ -472,6 +490,8 @@ void cuda_scalar_malloc_host(void);          /* src/cuda_scalar.cu:18-32 */
 void cuda_scalar_malloc_dev(void);           /* :35-61 */
 void cuda_scalar_push(void);                 /* :64-99 */
 void cuda_scalar_pull(void);                 /* :102-106 */
+void cuda_scalar_pull_debug(void);           /* :108-115, s_conv, s_diff */
+void cuda_scalar_pull_restart(void);         /* :117-126, s0, s_conv0, s_diff0 */
 void cuda_scalar_free(void);                 /* :129-146 */
 void cuda_compute_boussinesq(void);          /* :149-154 */
 void cuda_scalar_BC(real *array);            /* :157-237 */
@@ -485,6 +505,27 @@ void cuda_scalar_update_part(void);          /* :959-969 */
 /* Periodic ghost refresh of one field; replaces mpi_cuda_exchange_G*
  * (src/mpi_comm.c:257-405) on a single GPU.  `grid` is GCC/GFX/GFY/GFZ. */
 void cuda_exchange(real *array, int grid);
+/* The MPI layer (src/mpi_comm.h:267-631) on one rank: no neighbour and no
+ * window; the exchanges refresh the periodic ghosts on the GPU
+ * (cuda_exchange).  MPI_Init/MPI_Finalize run when the program links an
+ * MPI library (found at run time; the library does not link MPI). */
+void mpi_startup(int argc, char *argv[]);    /* src/mpi_comm.c:39-79 */
+void mpi_dom_comm_init(void);                /* :81-96 */
+void mpi_parts_init(void);                   /* :471 */
+void mpi_cuda_exchange_Gcc(real *array);     /* :257-405 */
+void mpi_cuda_exchange_Gfx(real *array);
+void mpi_cuda_exchange_Gfy(real *array);
+void mpi_cuda_exchange_Gfz(real *array);
+void mpi_free(void);                         /* :1262-1309 */
+void mpi_end(void);                          /* :1311-1315 */
+/* Exchange buffers and window handles that src/scalar.c:115-224 names
+ * (src/cuda_physalis.cu:41-58, src/mpi_comm.c:71-136); unused on one rank. */
+extern real *_sum_send_e, *_sum_send_w, *_sum_send_n, *_sum_send_s,
+  *_sum_send_t, *_sum_send_b;
+extern real *_sum_recv_e, *_sum_recv_w, *_sum_recv_n, *_sum_recv_s,
+  *_sum_recv_t, *_sum_recv_b;
+extern void *parts_recv_win_e, *parts_recv_win_w, *parts_recv_win_n,
+  *parts_recv_win_s, *parts_recv_win_t, *parts_recv_win_b;
 void cuda_check_errors(int line);            /* src/cuda_bluebottle.cu:37-42 */
 
 #ifdef __cplusplus

@@ -57,6 +57,41 @@ def test_library_exports_every_declared_symbol():
     assert not missing_v, f"globals not exported: {missing_v}"
 
 
+def test_library_exports_what_the_reference_host_calls(tmp_path):
+    """Every function and global that the reference's host C files
+    (src/bluebottle.c, domain.c, particle.c, recorder.c, scalar.c, rng.c,
+    vtk.c) take from the files this library replaces (the CUDA sources and
+    src/mpi_comm.c) is exported, so those objects link against the library
+    in their place.  The list is tests/golden/host_calls.json, extracted
+    from the reference sources by tests/golden/extract_host_calls.py.  A C
+    program that references every one of them links against the library."""
+    import json
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden",
+                                     "host_calls.json")))
+    funcs = {k for k, v in fx["functions"].items()
+             if not v["test_build_only"]}
+    globs = set(fx["globals"])
+    assert len(funcs) >= 65 and "mpi_cuda_exchange_Gcc" in funcs
+    lib_path = bbhip.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path],
+                         capture_output=True, text=True, check=True).stdout
+    syms = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = sorted((funcs | globs) - syms)
+    assert not missing, f"not exported: {missing}"
+    # link check: a program that takes the address of each symbol
+    src = tmp_path / "host_link.c"
+    body = ["extern char " + n + "[];" for n in sorted(funcs | globs)]
+    body.append("void *tab[] = {" + ", ".join(sorted(funcs | globs)) + "};")
+    body.append("int main(void) { return tab[0] == 0; }")
+    src.write_text("\n".join(body) + "\n")
+    exe = tmp_path / "host_link"
+    r = subprocess.run(["gcc", "-o", str(exe), str(src), "-L",
+                        os.path.dirname(lib_path), "-lbluebottle",
+                        "-Wl,--no-as-needed"], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
 def run_parse(case_dir, extra=""):
     """Run domain_read_input + domain_fill in a child process (the parser
     exits the process on malformed input, like the reference)."""
