@@ -112,8 +112,7 @@ def parse():
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
                     help="search kernel: 2 z-columns through LDS planes "
-                         "(default), 3 z-columns with one wavefront per row, "
-                         "1 one tile per workgroup, 0 direct neighbour loads")
+                         "(default), 0 direct neighbour loads")
     ap.add_argument("--tzb", type=int, default=None,
                     help="tiles per z-column of the search kernel "
                          "(0: automatic)")
@@ -131,11 +130,6 @@ def parse():
     ap.add_argument("--defer-phi", type=int, default=None,
                     help="1: the plain form's update kernel adds the "
                          "phi terms of two iterations at once")
-    ap.add_argument("--update-group", type=int, default=None,
-                    help="1: update kernel over z-columns of tiles too")
-    ap.add_argument("--mixed-x2", type=int, default=None,
-                    help="1: FP32 kernels of the mixed form on x pairs, "
-                         "0: one cell per lane (default)")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
     ap.add_argument("--time-every", type=int, default=8,
@@ -348,8 +342,6 @@ def main():
                       ("bb_spmv_nts", a.spmv_nts),
                       ("bb_spmv_nts32", a.spmv_nts32),
                       ("bb_spmv_tzb32", a.tzb32),
-                      ("bb_update_group", a.update_group),
-                      ("bb_mixed_x2", a.mixed_x2),
                       ("bb_pcg_batch", a.batch),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:

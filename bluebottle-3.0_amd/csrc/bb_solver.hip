@@ -325,11 +325,6 @@ __device__ inline int plane_src(const DevDom &D, int kk)
   return kk;
 }
 
-/* Raw r and p_old of a thread's two entries of one plane. */
-template <typename V> struct PlaneRaw {
-  V r0, p0, r1, p1;
-};
-
 /* Element `i` of a Gcc array by a 32-bit byte offset from the (uniform)
  * base, so the load takes a scalar base and one vector offset register. */
 template <typename V>
@@ -370,28 +365,6 @@ template <typename V>
 __device__ inline void st32_nt(V *__restrict__ a, unsigned i, V v)
 {
   __builtin_nontemporal_store(v, (V *) ((char *) a + i * (unsigned) sizeof(V)));
-}
-
-template <bool FIRST, typename V>
-__device__ inline PlaneRaw<V> plane_load(const DevDom &D,
-  const PlaneEntry &E0, const PlaneEntry &E1, const V *__restrict__ r,
-  const V *__restrict__ pold, const uint8_t *__restrict__ fluid, int kk)
-{
-  PlaneRaw<V> R;
-  R.r0 = R.p0 = R.r1 = R.p1 = 0;
-  int ks = plane_src(D, kk);
-  unsigned off = (unsigned) ks * (unsigned) D.cc.sz;
-  if (ks && E0.cxy >= 0) {
-    R.r0 = ld32(r, E0.cxy + off);
-    if (!FIRST) R.p0 = ld32(pold, E0.cxy + off);
-    if (fluid != NULL && fluid[E0.cxy + off] == 0) R.r0 = 0;
-  }
-  if (ks && E1.cxy >= 0) {
-    R.r1 = ld32(r, E1.cxy + off);
-    if (!FIRST) R.p1 = ld32(pold, E1.cxy + off);
-    if (fluid != NULL && fluid[E1.cxy + off] == 0) R.r1 = 0;
-  }
-  return R;
 }
 
 /* The same loads without a branch: every thread issues all of them, an
@@ -437,27 +410,6 @@ __device__ inline void plane_store_nb(const DevDom &D, const V *tab,
   if (threadIdx.x + TILE_THREADS < LPLANE) buf[threadIdx.x + TILE_THREADS] = v1;
 }
 
-/* `tab`: the 27-entry invM table (kernel argument or a workgroup's LDS
- * copy). */
-template <bool FIRST, typename V>
-__device__ inline void plane_store(const DevDom &D, const V *tab,
-  const PlaneEntry &E0, const PlaneEntry &E1, const PlaneRaw<V> &R, V beta,
-  V *__restrict__ buf, int kk)
-{
-  int ks = plane_src(D, kk);
-  int wz = 9 * wcount(ks, D.nz, D.perz);
-  V v0 = 0, v1 = 0;
-  if (ks && E0.cxy >= 0) {
-    V z = R.r0*tab[E0.wxy + wz];
-    v0 = FIRST ? z : z + beta*R.p0;
-  }
-  if (ks && E1.cxy >= 0) {
-    V z = R.r1*tab[E1.wxy + wz];
-    v1 = FIRST ? z : z + beta*R.p1;
-  }
-  buf[threadIdx.x] = v0;
-  if (threadIdx.x + TILE_THREADS < LPLANE) buf[threadIdx.x + TILE_THREADS] = v1;
-}
 
 /* (-A) p from the centre value, four in-plane neighbours and the two
  * z neighbours (src/solver_kernel.cu:715-836); the w* say which faces are
@@ -469,77 +421,6 @@ __device__ inline V neg_lap7(const CoefT<V> &K, V pC, V pW, V pE, V pS, V pN,
   return -K.idx2 * (wmul(wE, pE - pC) - wmul(wW, pC - pW))
          -K.idy2 * (wmul(wN, pN - pC) - wmul(wS, pC - pS))
          -K.idz2 * (wmul(wT, pT - pC) - wmul(wB, pC - pB));
-}
-
-/* The tile's planes stream through two LDS buffers; the raw r and p_old of
- * each plane are loaded one (FP64) or two (FP32) planes ahead of their use,
- * the first three before the reduction that yields beta, so memory latency
- * overlaps the work of the planes in between. */
-template <bool FIRST, typename V, bool MIXED>
-__global__ void __launch_bounds__(TILE_THREADS)
-__attribute__((amdgpu_waves_per_eu(8, 8)))
-search_spmv_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
-  const double *__restrict__ rpart, long ntiles,
-  const V *__restrict__ r, const V *__restrict__ pold,
-  const uint8_t *__restrict__ fluid,
-  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
-{
-  __shared__ V lds[2 * LPLANE];
-  if (st->done | st->inner_done) return;
-  TILE_PROLOGUE
-  PlaneEntry E0 = plane_entry(D, xb, jb, threadIdx.x);
-  PlaneEntry E1 = plane_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
-  /* Ring of raw planes in registers: slot (p - k0 + 1) mod RING holds plane
-   * p; plane k + RING - 1 is loaded while plane k is computed. */
-  constexpr int RING = sizeof(V) == 8 ? 3 : 4;
-  PlaneRaw<V> R[RING];
-  R[0] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 - 1);
-  R[1] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0);
-  R[2] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 1);
-  double beta64 = 0.;
-  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
-  if (RING > 3 && k0 + 2 <= k1 + 1)
-    R[3 % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k0 + 2);
-  V beta = (V) beta64;
-  int xl = lane + 1, yl = wv + 1;
-  bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
-  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
-
-  plane_store<FIRST, V>(D, T.v, E0, E1, R[0], beta,
-                        lds + ((k0 - 1) & 1) * LPLANE, k0 - 1);
-  plane_store<FIRST, V>(D, T.v, E0, E1, R[1], beta, lds + (k0 & 1) * LPLANE,
-                        k0);
-  __syncthreads();
-  V prev = lds[((k0 - 1) & 1) * LPLANE + yl * LX + xl];
-  V cur = lds[(k0 & 1) * LPLANE + yl * LX + xl];
-  double acc = 0.0;
-#pragma unroll
-  for (int kk = 0; kk < TILE_Z; kk++) {
-    const int k = k0 + kk;
-    if (k <= k1) {                    /* uniform over the workgroup */
-      __syncthreads();
-      plane_store<FIRST, V>(D, T.v, E0, E1, R[(kk + 2) % RING], beta,
-                            lds + ((k + 1) & 1) * LPLANE, k + 1);
-      if (k + RING - 1 <= k1 + 1)
-        R[kk % RING] = plane_load<FIRST, V>(D, E0, E1, r, pold, fluid, k + RING - 1);
-      __syncthreads();
-      const V *P = lds + (k & 1) * LPLANE;
-      V nxt = lds[((k + 1) & 1) * LPLANE + yl * LX + xl];
-      if (colok) {
-        V a = neg_lap7<V>(K, cur, P[yl * LX + xl - 1], P[yl * LX + xl + 1],
-                          P[(yl - 1) * LX + xl], P[(yl + 1) * LX + xl], prev,
-                          nxt, wW, wE, wS, wN, wall_lo(k, D.perz),
-                          wall_hi(k, D.nz, D.perz));
-        long c = gidx(C, x, j, k);
-        pnew[c] = cur;
-        Ap[c] = a;
-        acc += (double) cur * (double) a;
-      }
-      prev = cur;
-      cur = nxt;
-    }
-  }
-  tile_partial(acc, gpart, tile);
 }
 
 /* The same pass over a column of `tzb` tiles stacked in z: one workgroup
@@ -671,521 +552,6 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
   }
 }
 
-/* ----------------- search + SpMV without LDS planes ---------------------- */
-/* Each wavefront owns one row (64 cells along x) of a z-column of tiles and
- * streams it through z on its own: no LDS planes and no barrier per plane.
- * For plane k it forms p_new of its own row at plane k + 1 (the z
- * neighbour) and of rows j - 1 and j + 1 at plane k (loaded again by the
- * neighbouring wavefronts; the cache serves the repeats); the x neighbours
- * come from the lanes on either side (cross-lane shuffles), and the two
- * cells past the row's ends from one extra load.  Every p_new is formed
- * with the same arithmetic as in search_spmv_group_kernel, so the results
- * are the same bits.  Raw values are loaded two planes ahead into two
- * alternating register sets. */
-
-/* Source offset and invM x/y index of cell (xg, jj) after the periodic
- * wrap (cxy -1: a wall ghost or outside, value 0). */
-__device__ inline PlaneEntry cell_entry(const DevDom &D, int xg, int jj)
-{
-  PlaneEntry E;
-  int js = jj, xs = xg;
-  if (jj < 1) js = D.pery ? D.ny : 0;
-  else if (jj > D.ny) js = (D.pery && jj == D.ny + 1) ? 1 : 0;
-  if (xg < 1) xs = D.perx ? D.nx : 0;
-  else if (xg > D.nx) xs = (D.perx && xg == D.nx + 1) ? 1 : 0;
-  bool ok = js && xs;
-  E.cxy = ok ? xs + js * D.cc.sy : -1;
-  E.wxy = ok ? wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery) : 0;
-  return E;
-}
-
-/* Raw r and p_old: own row at plane kc (c), rows j - 1 (s), j + 1 (n) and
- * the end-of-row cell (h) at plane ks. */
-template <typename V> struct RowsRaw {
-  V rc, pc, rs, ps, rn, pn, rh, ph;
-};
-
-template <bool FIRST, typename V>
-__device__ inline void rows_ld(const PlaneEntry &E, unsigned off,
-  const V *__restrict__ r, const V *__restrict__ pold, V *rv, V *pv)
-{
-  unsigned c = (unsigned) (E.cxy >= 0 ? E.cxy : 0) + off;
-  *rv = ld32(r, c);
-  *pv = FIRST ? (V) 0 : ld32(pold, c);
-}
-
-template <bool FIRST, typename V>
-__device__ inline RowsRaw<V> rows_load(const DevDom &D, const PlaneEntry &EC,
-  const PlaneEntry &ES, const PlaneEntry &EN, const PlaneEntry &EH, int kc,
-  int ks, const V *__restrict__ r, const V *__restrict__ pold)
-{
-  RowsRaw<V> R;
-  int pc = plane_src(D, kc), psd = plane_src(D, ks);
-  unsigned oc = (unsigned) (pc ? pc : 1) * (unsigned) D.cc.sz;
-  unsigned os = (unsigned) (psd ? psd : 1) * (unsigned) D.cc.sz;
-  rows_ld<FIRST, V>(EC, oc, r, pold, &R.rc, &R.pc);
-  rows_ld<FIRST, V>(ES, os, r, pold, &R.rs, &R.ps);
-  rows_ld<FIRST, V>(EN, os, r, pold, &R.rn, &R.pn);
-  rows_ld<FIRST, V>(EH, os, r, pold, &R.rh, &R.ph);
-  return R;
-}
-
-/* p_new = z + beta p_old of one entry at plane kk (0 for a zero entry or a
- * wall plane) */
-template <bool FIRST, typename V>
-__device__ inline V pnew_of(const DevDom &D, const V *tab,
-  const PlaneEntry &E, int kk, V rv, V pv, V beta)
-{
-  int ks = plane_src(D, kk);
-  int wz = 9 * wcount(ks, D.nz, D.perz);
-  V z = rv*tab[E.wxy + wz];
-  V v = FIRST ? z : z + beta*pv;
-  return (ks && E.cxy >= 0) ? v : (V) 0;
-}
-
-template <typename V>
-__device__ inline V shfl_up1(V v)
-{
-  return __shfl_up(v, 1, WAVE);
-}
-
-template <typename V>
-__device__ inline V shfl_down1(V v)
-{
-  return __shfl_down(v, 1, WAVE);
-}
-
-template <bool FIRST, typename V, bool MIXED>
-__global__ void __launch_bounds__(TILE_THREADS)
-__attribute__((amdgpu_waves_per_eu(6, 8)))
-search_spmv_rows_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
-  int q, const double *__restrict__ rpart, long ntiles, int tzb,
-  const V *__restrict__ r, const V *__restrict__ pold,
-  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
-{
-  __shared__ V s_invm[27];
-  __shared__ double s_tp[8][TILE_Y];
-  if (st->done | st->inner_done) return;
-  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
-  const Grid &C = D.cc;
-  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
-  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
-  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
-  const int ngz = (ntz_ + tzb - 1) / tzb;
-  long g = tile_of_block((long) ntx_ * nty_ * ngz);
-  int lane = threadIdx.x & (WAVE - 1);
-  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
-  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
-  int xb = tx * TILE_X, jb = ty * TILE_Y;
-  int k0 = tz0 * TILE_Z + 1;
-  int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
-  int x = xb + lane + 1, j = jb + wv + 1;
-  bool colok = (x <= D.nx) && (j <= D.ny);
-  PlaneEntry EC = cell_entry(D, x, j);
-  PlaneEntry ES = cell_entry(D, x, j - 1);
-  PlaneEntry EN = cell_entry(D, x, j + 1);
-  /* the cells past the row's ends: lane 0 needs x - 1, lane 63 x + 1; the
-   * other lanes repeat one of the two addresses (two cache lines a wave) */
-  PlaneEntry EH = cell_entry(D, lane < WAVE / 2 ? xb : xb + WAVE + 1, j);
-  /* raw values: centre planes k0 - 1 and k0; then sets A (centre k0 + 1,
-   * rows k0) and B (centre k0 + 2, rows k0 + 1) */
-  V rp0, pp0, rc0, pc0;
-  {
-    int pa = plane_src(D, k0 - 1);
-    unsigned oa = (unsigned) (pa ? pa : 1) * (unsigned) D.cc.sz;
-    rows_ld<FIRST, V>(EC, oa, r, pold, &rp0, &pp0);
-    unsigned ob = (unsigned) k0 * (unsigned) D.cc.sz;
-    rows_ld<FIRST, V>(EC, ob, r, pold, &rc0, &pc0);
-  }
-  RowsRaw<V> Ra = rows_load<FIRST, V>(D, EC, ES, EN, EH, k0 + 1, k0, r, pold);
-  RowsRaw<V> Rb = rows_load<FIRST, V>(D, EC, ES, EN, EH, min(k0 + 2, kend + 1),
-                                      min(k0 + 1, kend), r, pold);
-  double beta64 = 0.;
-  if (!FIRST && !cg_continue<MIXED>(st, rpart, ntiles, q, &beta64)) return;
-  V beta = (V) beta64;
-  __syncthreads();                      /* s_invm */
-  bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
-  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
-  V prev = pnew_of<FIRST, V>(D, s_invm, EC, k0 - 1, rp0, pp0, beta);
-  V cur = pnew_of<FIRST, V>(D, s_invm, EC, k0, rc0, pc0, beta);
-  double acc = 0.0;
-  int ti = 0;
-  auto step = [&](int k, RowsRaw<V> &X) {
-    V nxt = pnew_of<FIRST, V>(D, s_invm, EC, k + 1, X.rc, X.pc, beta);
-    V Sv = pnew_of<FIRST, V>(D, s_invm, ES, k, X.rs, X.ps, beta);
-    V Nv = pnew_of<FIRST, V>(D, s_invm, EN, k, X.rn, X.pn, beta);
-    V Hv = pnew_of<FIRST, V>(D, s_invm, EH, k, X.rh, X.ph, beta);
-    X = rows_load<FIRST, V>(D, EC, ES, EN, EH, min(k + 3, kend + 1),
-                            min(k + 2, kend), r, pold);
-    V up = shfl_up1(cur), dn = shfl_down1(cur);
-    V Wv = lane == 0 ? Hv : up;
-    V Ev = lane == WAVE - 1 ? Hv : dn;
-    V a = neg_lap7<V>(K, cur, Wv, Ev, Sv, Nv, prev, nxt, wW, wE, wS, wN,
-                      wall_lo(k, D.perz), wall_hi(k, D.nz, D.perz));
-    unsigned c = colok ? (unsigned) gidx(C, x, j, k) : 0u;
-    st32(pnew, c, cur);
-    st32(Ap, c, a);
-    acc += colok ? (double) cur * (double) a : 0.0;
-    prev = cur;
-    cur = nxt;
-    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
-      double v = wave_tree_sum(acc);
-      if (lane == 0) s_tp[ti][wv] = v;
-      acc = 0.0;
-      ti++;
-    }
-  };
-  int k = k0;
-  for (; k + 1 <= kend; k += 2) {
-    step(k, Ra);
-    step(k + 1, Rb);
-  }
-  if (k <= kend) step(k, Ra);
-  __syncthreads();
-  if (threadIdx.x < ti) {
-    const double *w = s_tp[threadIdx.x];
-    gpart[tx + (long) ntx_ * (ty + (long) nty_ * (tz0 + (int) threadIdx.x))] =
-      ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
-  }
-}
-
-/* ------------- FP32 inner sweep: two cells per lane (x pairs) ------------- */
-/* The mixed form's FP32 vectors halve the bytes per cell, so one cell per
- * lane would leave the FP32 kernels issue-bound.  These kernels give each
- * lane the x pair (x, x + 1), x odd, moved as one float2 (the FP32
- * vectors are offset so that the pair is 8-byte aligned; needs even nx).
- * Tiles are 128 x 8 x 8 cells (64 lanes x 2, eight rows, eight planes)
- * grouped into z-columns of tzb tiles like the FP64 kernels; each tile's
- * partial is the per-lane sum over its planes of (c0 + c1) products,
- * combined by the tile tree.  The LDS row holds [pad, x = xb, x = xb + 1
- * .. xb + 128, x = xb + 129, pad], so interior pairs are 8-byte aligned;
- * raw planes are loaded one plane ahead (Ra holds plane k + 1). */
-#define TX2 128
-#define LX2 (TX2 + 4)
-#define LPLANE2 (LX2 * LY)
-
-__device__ inline int n_tiles_x2(const DevDom &D)
-{
-  return (D.nx + TX2 - 1) / TX2;
-}
-
-/* One plane's raw values held by a thread: pair entries e = t and
- * t + TILE_THREADS (of 64 pairs x LY rows) and halo entry t (of 2 x LY). */
-struct Plane2 {
-  float2 r0, p0, r1, p1;
-  float rh, ph;
-};
-
-/* Source cell (x) of local column xl (0: x = xb, 129: x = xb + 129) after
- * the periodic wrap; 0 if the entry is a wall ghost or outside. */
-__device__ inline int x2_src(const DevDom &D, int xb, int xl)
-{
-  int xg = xb + xl;
-  if (xg < 1) return D.perx ? D.nx : 0;
-  if (xg > D.nx) return (D.perx && xg == D.nx + 1) ? 1 : 0;
-  return xg;
-}
-
-__device__ inline int y2_src(const DevDom &D, int jj)
-{
-  if (jj < 1) return D.pery ? D.ny : 0;
-  if (jj > D.ny) return (D.pery && jj == D.ny + 1) ? 1 : 0;
-  return jj;
-}
-
-/* Where a thread's entry of every plane comes from, fixed per workgroup:
- * cxy = x + y*sy of the source (first) cell, -1 for an all-zero entry;
- * info = w0 | w1 << 4 | single << 8 with w0, w1 the x and y part of the
- * invM table index of the two cells and `single` for an entry whose only
- * nonzero cell is the first (the wrapped or wall neighbour x = nx + 1). */
-struct Entry2 {
-  int cxy, info;
-};
-
-__device__ inline Entry2 pair_entry(const DevDom &D, int xb, int jb, int e)
-{
-  Entry2 E;
-  E.cxy = -1;
-  E.info = 0;
-  int rr = e >> 6, pp = e & 63;
-  int js = y2_src(D, jb + rr);
-  if (e >= 64 * LY || !js) return E;
-  int x0 = xb + 1 + 2 * pp;
-  int wy = 3 * wcount(js, D.ny, D.pery);
-  if (x0 + 1 <= D.nx) {
-    E.cxy = x0 + js * D.cc.sy;
-    E.info = (wcount(x0, D.nx, D.perx) + wy) |
-             ((wcount(x0 + 1, D.nx, D.perx) + wy) << 4);
-  } else {
-    int xs = x2_src(D, xb, 1 + 2 * pp);
-    if (xs) {
-      E.cxy = xs + js * D.cc.sy;
-      E.info = (wcount(xs, D.nx, D.perx) + wy) | (1 << 8);
-    }
-  }
-  return E;
-}
-
-__device__ inline Entry2 halo_entry(const DevDom &D, int xb, int jb, int t)
-{
-  Entry2 E;
-  E.cxy = -1;
-  E.info = 0;
-  if (t >= 2 * LY) return E;
-  int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
-  int js = y2_src(D, jb + rr), xs = x2_src(D, xb, xl);
-  if (js && xs) {
-    E.cxy = xs + js * D.cc.sy;
-    E.info = wcount(xs, D.nx, D.perx) + 3 * wcount(js, D.ny, D.pery);
-  }
-  return E;
-}
-
-/* Every load is issued unconditionally (a zero entry or plane reads the
- * aligned dummy pair at x = 1 of row 0 instead and is zeroed when stored;
- * a `single` entry reads its pair and keeps the first value), so the
- * compiler can keep loads of the next plane in flight across the loop.
- * The residual is already zero in particle cells. */
-template <bool FIRST>
-__device__ inline void pair_load(const Entry2 &E, unsigned off,
-  const float *__restrict__ r, const float *__restrict__ pold, float2 *rv,
-  float2 *pv)
-{
-  unsigned c = (unsigned) (E.cxy >= 0 ? E.cxy : 1) + off;
-  *rv = *(const float2 *) (r + c);
-  *pv = FIRST ? make_float2(0.f, 0.f) : *(const float2 *) (pold + c);
-}
-
-template <bool FIRST>
-__device__ inline Plane2 plane2_load(const DevDom &D, const Entry2 &E0,
-  const Entry2 &E1, const Entry2 &EH, int kk, const float *__restrict__ r,
-  const float *__restrict__ pold)
-{
-  Plane2 P;
-  int ks = plane_src(D, kk);
-  unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) D.cc.sz;
-  pair_load<FIRST>(E0, off, r, pold, &P.r0, &P.p0);
-  pair_load<FIRST>(E1, off, r, pold, &P.r1, &P.p1);
-  unsigned c = (unsigned) (EH.cxy >= 0 ? EH.cxy : 1) + off;
-  P.rh = r[c];
-  P.ph = FIRST ? 0.f : pold[c];
-  return P;
-}
-
-/* invM of the entry's cells from the workgroup's LDS copy of the table
- * (wz: z part of the index, -1 for a zero plane) */
-template <bool FIRST>
-__device__ inline float2 pair_value(const float *s_invm, const Entry2 &E,
-  int wz, float2 rv, float2 pv, float beta)
-{
-  int w = wz < 0 ? 0 : wz;
-  float z0 = rv.x * s_invm[(E.info & 15) + w];
-  float z1 = rv.y * s_invm[((E.info >> 4) & 15) + w];
-  float2 v = FIRST ? make_float2(z0, z1)
-                   : make_float2(z0 + beta * pv.x, z1 + beta * pv.y);
-  if (E.info & (1 << 8)) v.y = 0.f;
-  return (E.cxy < 0 || wz < 0) ? make_float2(0.f, 0.f) : v;
-}
-
-template <bool FIRST>
-__device__ inline void plane2_store(const DevDom &D, const float *s_invm,
-  const Entry2 &E0, const Entry2 &E1, const Entry2 &EH, int kk,
-  const Plane2 &P, float beta, float *__restrict__ buf)
-{
-  int ks = plane_src(D, kk);
-  int wz = ks ? 9 * wcount(ks, D.nz, D.perz) : -1;
-  int t = threadIdx.x;
-  {
-    float2 v = pair_value<FIRST>(s_invm, E0, wz, P.r0, P.p0, beta);
-    int rr = t >> 6, pp = t & 63;
-    *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
-  }
-  int e1 = t + TILE_THREADS;
-  if (e1 < 64 * LY) {
-    float2 v = pair_value<FIRST>(s_invm, E1, wz, P.r1, P.p1, beta);
-    int rr = e1 >> 6, pp = e1 & 63;
-    *(float2 *) &buf[rr * LX2 + 2 + 2 * pp] = v;
-  }
-  if (t < 2 * LY) {
-    int rr = t >> 1, xl = (t & 1) ? TX2 + 1 : 0;
-    float z = P.rh * s_invm[EH.info + (wz < 0 ? 0 : wz)];
-    float v = FIRST ? z : z + beta * P.ph;
-    buf[rr * LX2 + 1 + xl] = (EH.cxy >= 0 && wz >= 0) ? v : 0.f;
-  }
-}
-
-template <bool FIRST>
-__global__ void __launch_bounds__(TILE_THREADS)
-__attribute__((amdgpu_waves_per_eu(8, 8)))
-search_spmv_x2_kernel(DevDom D, CoefT<float> K, InvMT<float> T, CGState *st,
-  int q, const double *__restrict__ rpart, long nrpart, int tzb,
-  const float *__restrict__ r, const float *__restrict__ pold,
-  float *__restrict__ pnew, float *__restrict__ Ap, double *__restrict__ gpart)
-{
-  __shared__ __attribute__((aligned(16))) float lds[3 * LPLANE2];
-  __shared__ float s_invm[27];
-  if (st->done | st->inner_done) return;
-  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
-  const Grid &C = D.cc;
-  const int ntx_ = n_tiles_x2(D);
-  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
-  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
-  const int ngz = (ntz_ + tzb - 1) / tzb;
-  long g = tile_of_block((long) ntx_ * nty_ * ngz);
-  int lane = threadIdx.x & (WAVE - 1);
-  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
-  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
-  int xb = tx * TX2, jb = ty * TILE_Y;
-  int k0 = tz0 * TILE_Z + 1;
-  int kend = k0 + tzb * TILE_Z - 1 < D.nz ? k0 + tzb * TILE_Z - 1 : D.nz;
-  int x0 = xb + 2 * lane + 1, j = jb + wv + 1;
-  bool colok = (x0 <= D.nx) && (j <= D.ny);
-  Entry2 E0 = pair_entry(D, xb, jb, threadIdx.x);
-  Entry2 E1 = pair_entry(D, xb, jb, threadIdx.x + TILE_THREADS);
-  Entry2 EH = halo_entry(D, xb, jb, threadIdx.x);
-  Plane2 Rp = plane2_load<FIRST>(D, E0, E1, EH, k0 - 1, r, pold);
-  Plane2 Rc = plane2_load<FIRST>(D, E0, E1, EH, k0, r, pold);
-  Plane2 Ra = plane2_load<FIRST>(D, E0, E1, EH, k0 + 1, r, pold);
-  double beta64 = 0.;
-  if (!FIRST && !cg_continue<true>(st, rpart, nrpart, q, &beta64)) return;
-  float beta = (float) beta64;
-  __syncthreads();                      /* s_invm */
-  int yl = wv + 1, o = yl * LX2 + 2 + 2 * lane;
-  bool wW = wall_lo(x0, D.perx), wE = wall_hi(x0 + 1, D.nx, D.perx);
-  bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
-  float *bprev = lds, *bcur = lds + LPLANE2, *bnext = lds + 2 * LPLANE2;
-  plane2_store<FIRST>(D, s_invm, E0, E1, EH, k0 - 1, Rp, beta, bprev);
-  plane2_store<FIRST>(D, s_invm, E0, E1, EH, k0, Rc, beta, bcur);
-  __syncthreads();
-  float2 prev = *(const float2 *) &bprev[o];
-  float2 cur = *(const float2 *) &bcur[o];
-  double acc = 0.0;
-  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
-  const long tstride = (long) ntx_ * nty_;
-  for (int k = k0; k <= kend; k++) {
-    plane2_store<FIRST>(D, s_invm, E0, E1, EH, k + 1, Ra, beta, bnext);
-    Ra = plane2_load<FIRST>(D, E0, E1, EH, min(k + 2, kend + 1), r, pold);
-    __syncthreads();
-    float2 nxt = *(const float2 *) &bnext[o];
-    if (colok) {
-      float2 S = *(const float2 *) &bcur[o - LX2];
-      float2 N = *(const float2 *) &bcur[o + LX2];
-      float Wv = bcur[o - 1], Ev = bcur[o + 2];
-      bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
-      float a0 = neg_lap7<float>(K, cur.x, Wv, cur.y, S.x, N.x, prev.x, nxt.x,
-                                 wW, false, wS, wN, wB, wT);
-      float a1 = neg_lap7<float>(K, cur.y, cur.x, Ev, S.y, N.y, prev.y, nxt.y,
-                                 false, wE, wS, wN, wB, wT);
-      unsigned c = (unsigned) gidx(C, x0, j, k);
-      *(float2 *) (pnew + c) = cur;
-      *(float2 *) (Ap + c) = make_float2(a0, a1);
-      acc += (double) cur.x * (double) a0 + (double) cur.y * (double) a1;
-    }
-    prev = cur;
-    cur = nxt;
-    float *t3 = bprev;
-    bprev = bcur;
-    bcur = bnext;
-    bnext = t3;
-    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
-      tile_partial(acc, gpart, tile);   /* uniform over the workgroup */
-      acc = 0.0;
-      tile += tstride;
-    }
-  }
-}
-
-/* alpha = (r, z) / (p, Ap); e += alpha p, r -= alpha Ap, z = invM r and the
- * tile partials of (r, z), on x pairs over z-columns of tiles. */
-__global__ void __launch_bounds__(TILE_THREADS)
-__attribute__((amdgpu_waves_per_eu(8, 8)))
-update_x2_kernel(DevDom D, InvMT<float> T, const CGState *st, int q,
-  const double *__restrict__ ppart, long nppart, int tzb,
-  const float *__restrict__ p, const float *__restrict__ Ap,
-  const uint8_t *__restrict__ fluid, float *__restrict__ e,
-  float *__restrict__ r, double *__restrict__ gpart)
-{
-  __shared__ float s_invm[27];
-  if (st->done | st->inner_done) return;
-  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
-  const Grid &C = D.cc;
-  const int ntx_ = n_tiles_x2(D);
-  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
-  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
-  const int ngz = (ntz_ + tzb - 1) / tzb;
-  long g = tile_of_block((long) ntx_ * nty_ * ngz);
-  int lane = threadIdx.x & (WAVE - 1);
-  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
-  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
-  int x0 = tx * TX2 + 2 * lane + 1, j = ty * TILE_Y + wv + 1;
-  bool colok = (x0 <= D.nx) && (j <= D.ny);
-  int tz1 = tz0 + tzb < ntz_ ? tz0 + tzb : ntz_;
-  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
-  const long tstride = (long) ntx_ * nty_;
-  /* the first tile's p and Ap are in flight during the reduction */
-  int kf = tz0 * TILE_Z + 1;
-  int kf1 = kf + TILE_Z - 1 < D.nz ? kf + TILE_Z - 1 : D.nz;
-  unsigned sz = (unsigned) C.sz;
-  float2 pv[TILE_Z], apv[TILE_Z];
-  {
-    unsigned c0 = (unsigned) gidx(C, x0, j, kf);
-#pragma unroll
-    for (int kk = 0; kk < TILE_Z; kk++) {
-      pv[kk] = apv[kk] = make_float2(0.f, 0.f);
-      if (colok && kf + kk <= kf1) {
-        pv[kk] = *(const float2 *) (p + c0 + kk * sz);
-        apv[kk] = *(const float2 *) (Ap + c0 + kk * sz);
-      }
-    }
-  }
-  /* tile_block_sum's barriers also publish s_invm */
-  float alpha = (float) (st->rzs[q & 1]/tile_block_sum(ppart, nppart));
-  int wj = 3 * wcount(j, D.ny, D.pery);
-  int w0 = wcount(x0, D.nx, D.perx) + wj, w1 = wcount(x0 + 1, D.nx, D.perx) + wj;
-  for (int tz = tz0; tz < tz1; tz++, tile += tstride) {
-    int k0 = tz * TILE_Z + 1;
-    int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;
-    unsigned c0 = (unsigned) gidx(C, x0, j, k0);
-    if (tz > tz0) {
-#pragma unroll
-      for (int kk = 0; kk < TILE_Z; kk++) {
-        pv[kk] = apv[kk] = make_float2(0.f, 0.f);
-        if (colok && k0 + kk <= k1) {
-          pv[kk] = *(const float2 *) (p + c0 + kk * sz);
-          apv[kk] = *(const float2 *) (Ap + c0 + kk * sz);
-        }
-      }
-    }
-    double acc = 0.0;
-    if (colok) {
-#pragma unroll
-      for (int kk = 0; kk < TILE_Z; kk++) {
-        if (k0 + kk <= k1) {
-          unsigned c = c0 + kk * sz;
-          float2 ev = *(const float2 *) (e + c);
-          float2 rv = *(const float2 *) (r + c);
-          ev.x += alpha*pv[kk].x;
-          ev.y += alpha*pv[kk].y;
-          rv.x = rv.x - alpha*apv[kk].x;
-          rv.y = rv.y - alpha*apv[kk].y;
-          rv.x = solid(fluid, c) ? 0.f : rv.x;
-          rv.y = solid(fluid, c + 1) ? 0.f : rv.y;
-          *(float2 *) (e + c) = ev;
-          *(float2 *) (r + c) = rv;
-          int wz = 9 * wcount(k0 + kk, D.nz, D.perz);
-          float m0 = solid(fluid, c) ? 0.f : s_invm[w0 + wz];
-          float m1 = solid(fluid, c + 1) ? 0.f : s_invm[w1 + wz];
-          float z0 = rv.x*m0, z1 = rv.y*m1;
-          acc += (double) rv.x * (double) z0 + (double) rv.y * (double) z1;
-        }
-      }
-    }
-    tile_partial(acc, gpart, tile);
-  }
-}
 
 /* The same pass with every neighbour read from global memory instead of
  * LDS (a second implementation, used to cross-check the tiled one). */
@@ -1358,72 +724,6 @@ update_kernel(DevDom D, InvMT<V> T, CGState *st, int q,
   tile_partial(acc, gpart, tile);
 }
 
-/* The same update over a column of `tzb` tiles stacked in z (the grouping
- * of search_spmv_group_kernel): alpha's reduction once per column, each
- * tile's planes of p and Ap loaded together, each tile's partial written
- * as above. */
-template <typename V>
-__global__ void __launch_bounds__(TILE_THREADS)
-__attribute__((amdgpu_waves_per_eu(8, 8)))
-update_group_kernel(DevDom D, InvMT<V> T, const CGState *st, int q,
-  const double *__restrict__ ppart, long ntiles, int tzb,
-  const V *__restrict__ p, const V *__restrict__ Ap,
-  const uint8_t *__restrict__ fluid, V *__restrict__ phi,
-  V *__restrict__ r, double *__restrict__ gpart)
-{
-  if (st->done | st->inner_done) return;
-  const Grid &C = D.cc;
-  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
-  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
-  const int ntz_ = (D.nz + TILE_Z - 1) / TILE_Z;
-  const int ngz = (ntz_ + tzb - 1) / tzb;
-  long g = tile_of_block((long) ntx_ * nty_ * ngz);
-  int lane = threadIdx.x & (WAVE - 1);
-  int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int tx = (int) (g % ntx_), ty = (int) ((g / ntx_) % nty_);
-  int tz0 = (int) (g / ((long) ntx_ * nty_)) * tzb;
-  int x = tx * TILE_X + lane + 1, j = ty * TILE_Y + wv + 1;
-  bool colok = (x <= D.nx) && (j <= D.ny);
-  int tz1 = tz0 + tzb < ntz_ ? tz0 + tzb : ntz_;
-  long tile = tx + (long) ntx_ * (ty + (long) nty_ * tz0);
-  const long tstride = (long) ntx_ * nty_;
-  /* the reduction runs once per column (the single-tile kernel overlaps
-   * it with its loads; here it is amortised over the column instead) */
-  V alpha = (V) (st->rzs[q & 1]/tile_block_sum(ppart, ntiles));
-  for (int tz = tz0; tz < tz1; tz++, tile += tstride) {
-    int k0 = tz * TILE_Z + 1;
-    int k1 = k0 + TILE_Z - 1 < D.nz ? k0 + TILE_Z - 1 : D.nz;
-    /* 32-bit element offsets: one vector register per address */
-    unsigned c0 = (unsigned) gidx(C, x, j, k0), sz = (unsigned) C.sz;
-    V pv[TILE_Z], apv[TILE_Z];
-#pragma unroll
-    for (int kk = 0; kk < TILE_Z; kk++) {
-      pv[kk] = apv[kk] = 0;
-      if (colok && k0 + kk <= k1) {
-        pv[kk] = ld32(p, c0 + kk * sz);
-        apv[kk] = ld32(Ap, c0 + kk * sz);
-      }
-    }
-    double acc = 0.0;
-    if (colok) {
-#pragma unroll
-      for (int kk = 0; kk < TILE_Z; kk++) {
-        if (k0 + kk <= k1) {
-          unsigned c = c0 + kk * sz;
-          V *ph = (V *) ((char *) phi + c * (unsigned) sizeof(V));
-          V *rp = (V *) ((char *) r + c * (unsigned) sizeof(V));
-          *ph += alpha*pv[kk];
-          V rr = *rp - alpha*apv[kk];
-          rr = solid(fluid, c) ? (V) 0 : rr;
-          *rp = rr;
-          V z = rr*(solid(fluid, c) ? (V) 0 : invm_at<V>(T, D, x, j, k0 + kk));
-          acc += (double) rr * (double) z;
-        }
-      }
-    }
-    tile_partial(acc, gpart, tile);
-  }
-}
 
 /* Every 50th iteration (src/cuda_solver.cu:208-224): phi += alpha p ... */
 __global__ void __launch_bounds__(TILE_THREADS)
@@ -1748,11 +1048,11 @@ static double *_part_r = NULL;         /* tile partials of (r, z) */
 static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
-int bb_spmv_variant = 2;               /* 2: LDS kernel over z-columns of
-                                          tiles, 3: the same columns with one
-                                          wavefront per row and no LDS
-                                          planes, 1: one tile per workgroup,
-                                          0: direct loads */
+int bb_spmv_variant = 2;               /* 2: search kernel over z-columns of
+                                          tiles with LDS planes; 0: direct
+                                          neighbour loads (a second
+                                          implementation, one tile per
+                                          workgroup, for cross-checks) */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_spmv_ntp = 1;                   /* 1: FP64 search loads p_old with the
                                           non-temporal hint (0: ordinary
@@ -1768,20 +1068,13 @@ int bb_spmv_tzb32 = 0;                 /* tiles per z-column of the FP32
                                           search (0: automatic) */
 int bb_defer_phi = 1;                  /* 1: phi updated every second
                                           iteration with both terms */
-int bb_update_group = 0;               /* 1: update kernel over z-columns too */
-int bb_mixed_x2 = 0;                   /* 1: FP32 kernels on x pairs (even nx);
-                                          0 (default): the one-cell z-column
-                                          kernels, measured as fast */
-int bb_spmv_kchunk = 8;                /* kept for the C ABI; tiles fix 8 */
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
 long bb_pcg_solves_total = 0;
 
 /* FP32 vectors of the mixed form's inner sweeps (Gcc layout).  Each
- * vector starts one float into its allocation, so that element 1 (the
- * first interior x) and, for even nx, every odd x of every row sits on an
- * 8-byte boundary: the two-cells-per-lane kernels move (x, x + 1) pairs
- * with odd x as one float2. */
+ * vector starts one float into its allocation, so that the first interior
+ * cell of a row (x = 1) sits on an 8-byte boundary. */
 static float *_r32 = NULL, *_p32a = NULL, *_p32b = NULL, *_Ap32 = NULL,
              *_e32 = NULL;
 static long buf32_n = 0;
@@ -1997,48 +1290,36 @@ void bb_pp_defect(const double *x, double *out)
   BB_LAUNCH_CHECK();
 }
 
-/* The z-column search kernel. */
+/* The z-column search kernel; the two hint switches select the template
+ * (non-temporal p_old loads: FP64 only, the FP32 kernel loads p_old
+ * ordinarily either way). */
 template <bool FIRST, typename V, bool MIXED>
 static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, const V *r, const V *pold, V *pnew, V *Ap)
 {
   const bool f32 = sizeof(V) == 4;
   int tzb = f32 ? spmv_tzb32() : spmv_tzb();
-  int nts = f32 ? bb_spmv_nts32 : bb_spmv_nts;
-  if (bb_spmv_variant == 3) {
-    hipLaunchKernelGGL((search_spmv_rows_kernel<FIRST, V, MIXED>),
-      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
-    return;
-  }
+  bool nts = (f32 ? bb_spmv_nts32 : bb_spmv_nts) != 0;
+  bool ntp = bb_spmv_ntp != 0;
+  dim3 grid = spmv_group_grid(tzb);
   /* two planes in flight: three (FP32: 72.4 against 71.5 us at 256^3) and
    * four did not help, and past two the FP64 kernel spills at 8 waves */
-  if (!nts)
-    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, true,
-      false>), spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
-  else if (bb_spmv_ntp)
-    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, true>),
-      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
-  else
-    hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, MIXED, 2, false>),
-      spmv_group_grid(tzb), tiles_block(), 0, bb_stream, ddom, K, T,
-      _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p);
+#define SG(NTP, NTS) hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, \
+    MIXED, 2, NTP, NTS>), grid, tiles_block(), 0, bb_stream, ddom, K, T,    \
+    _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p)
+  if (ntp && nts) SG(true, true);
+  else if (ntp) SG(true, false);
+  else if (nts) SG(false, true);
+  else SG(false, false);
+#undef SG
 }
 
-/* Paired phi updates (bb_defer_phi, plain form, one-tile update kernel):
- * odd iterations leave phi to the next one, except before an iteration
- * that recomputes the true residual (every 50th), whose phi update adds
- * its own term only. */
-static bool update_is_group(void)
-{
-  return bb_spmv_variant == 2 && bb_update_group && spmv_tzb() > 1;
-}
-
+/* Paired phi updates (bb_defer_phi, plain form): odd iterations leave phi
+ * to the next one, except before an iteration that recomputes the true
+ * residual (every 50th), whose phi update adds its own term only. */
 static bool phi_skips(int q)
 {
-  return bb_defer_phi && !update_is_group() && (q & 1) && (q + 1) % 50 != 0;
+  return bb_defer_phi && (q & 1) && (q + 1) % 50 != 0;
 }
 
 /* The mixed form's inner sweep pairs its updates of the correction e the
@@ -2046,12 +1327,9 @@ static bool phi_skips(int q)
  * (where e is folded into phi and must be whole). */
 static int pair_base32 = 1;
 
-static bool use_x2(void);
-
 static bool phi_skips32(int q)
 {
-  return bb_defer_phi && !update_is_group() && !use_x2() &&
-         ((q - pair_base32) & 1) == 0;
+  return bb_defer_phi && ((q - pair_base32) & 1) == 0;
 }
 
 /* Enqueue iteration q; p buffers alternate by parity. */
@@ -2061,28 +1339,21 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   double *pnew = (q & 1) ? _p_q : _p_alt;
   double *pold = (q & 1) ? _p_alt : _p_q;
   bb_kt_begin(KT_SPMV);
-  if (bb_spmv_variant >= 2) {
+  if (bb_spmv_variant != 0) {
     if (q == 1)
       launch_group<true, double, false>(K, T, q, nt, _r_q, pold, pnew, _Apb_q);
     else
       launch_group<false, double, false>(K, T, q, nt, _r_q, pold, pnew,
                                          _Apb_q);
-  } else if (bb_spmv_variant == 1) {
-    if (q == 1)
-      hipLaunchKernelGGL((search_spmv_kernel<true, double, false>),
-        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
-        _Apb_q, _part_p);
-    else
-      hipLaunchKernelGGL((search_spmv_kernel<false, double, false>),
-        TILES_LAUNCH, ddom, K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
-        _Apb_q, _part_p);
   } else {
     if (q == 1)
       hipLaunchKernelGGL(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
-        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew, _Apb_q, _part_p);
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
+        _Apb_q, _part_p);
     else
       hipLaunchKernelGGL(search_spmv_direct_kernel<false>, TILES_LAUNCH, ddom,
-        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew, _Apb_q, _part_p);
+        K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
+        _Apb_q, _part_p);
   }
   bb_kt_end(KT_SPMV);
   if (q % 50 == 0) {
@@ -2096,27 +1367,20 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
     int kt = mode == PHI_SKIP ? KT_UPDATE_SKIP
            : mode == PHI_PAIR ? KT_UPDATE_PAIR : KT_UPDATE;
     bb_kt_begin(kt);
-    int tzb = spmv_tzb();
-    if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
-      hipLaunchKernelGGL(update_group_kernel<double>, spmv_group_grid(tzb),
-        tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
-        pnew, _Apb_q, fluid_mask(), _phi, _r_q, _part_r);
-    } else {
-      const uint8_t *fm = fluid_mask();
+    const uint8_t *fm = fluid_mask();
 #define UPD(M, P) hipLaunchKernelGGL((update_kernel<double, M, P>),          \
-        TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fm, \
-        _phi, _r_q, _part_r, pold)
-      if (fm) {
-        if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
-        else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
-        else UPD(true, PHI_EACH);
-      } else {
-        if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
-        else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
-        else UPD(false, PHI_EACH);
-      }
-#undef UPD
+      TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fm,   \
+      _phi, _r_q, _part_r, pold)
+    if (fm) {
+      if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+      else UPD(true, PHI_EACH);
+    } else {
+      if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+      else UPD(false, PHI_EACH);
     }
+#undef UPD
     bb_kt_end(kt);
   }
   BB_LAUNCH_CHECK();
@@ -2222,36 +1486,6 @@ extern "C" void cuda_PP_cg(void)
   cuda_PP_cg_noparts();
 }
 
-/* Number of tile partials now in _part_r (the FP64 tile kernels and the
- * x-pair FP32 kernels use different tile counts). */
-static long rp_n = 0;
-
-static bool use_x2(void) { return bb_mixed_x2 && ddom.nx % 2 == 0; }
-
-static long n_tiles_x2_host(void)
-{
-  long ntx = (ddom.nx + TX2 - 1) / TX2;
-  long nty = (ddom.ny + TILE_Y - 1) / TILE_Y;
-  long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
-  return ntx * nty * ntz;
-}
-
-static dim3 x2_group_grid(int tzb)
-{
-  long ntx = (ddom.nx + TX2 - 1) / TX2;
-  long nty = (ddom.ny + TILE_Y - 1) / TILE_Y;
-  long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
-  return dim3((unsigned) (ntx * nty * ((ntz + tzb - 1) / tzb)));
-}
-
-/* z-column length of the x-pair kernels: at least four workgroups per CU */
-static int x2_tzb(void)
-{
-  if (bb_spmv_tzb > 0) return bb_spmv_tzb;
-  long t = n_tiles_x2_host() / 1024;
-  return t < 1 ? 1 : (t > 4 ? 4 : (int) t);
-}
-
 /* Enqueue inner iteration q of the mixed form (FP32 vectors); `first`
  * starts a new Krylov space (p = z). */
 static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
@@ -2260,72 +1494,32 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   const long nt = n_tiles(ddom);
   float *pnew = (q & 1) ? _p32a : _p32b;
   float *pold = (q & 1) ? _p32b : _p32a;
-  if (use_x2()) {
-    const long nt2 = n_tiles_x2_host();
-    int tzb = x2_tzb();
-    bb_kt_begin(KT_SPMV32);
-    if (first)
-      hipLaunchKernelGGL(search_spmv_x2_kernel<true>, x2_group_grid(tzb),
-        tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
-        tzb, _r32, pold, pnew, _Ap32, _part_p);
-    else
-      hipLaunchKernelGGL(search_spmv_x2_kernel<false>, x2_group_grid(tzb),
-        tiles_block(), 0, bb_stream, ddom, K, T, _cg_state, q, _part_r, rp_n,
-        tzb, _r32, pold, pnew, _Ap32, _part_p);
-    bb_kt_end(KT_SPMV32);
-    bb_kt_begin(KT_UPDATE32);
-    hipLaunchKernelGGL(update_x2_kernel, x2_group_grid(tzb), tiles_block(), 0,
-      bb_stream, ddom, T, _cg_state, q, _part_p, nt2, tzb, pnew, _Ap32,
-      fluid_mask(), _e32, _r32, _part_r);
-    bb_kt_end(KT_UPDATE32);
-    rp_n = nt2;
-    BB_LAUNCH_CHECK();
-    return;
-  }
   bb_kt_begin(KT_SPMV32);
-  if (bb_spmv_variant >= 2) {
-    if (first)
-      launch_group<true, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
-    else
-      launch_group<false, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
-  } else if (first) {
-    hipLaunchKernelGGL((search_spmv_kernel<true, float, true>), TILES_LAUNCH,
-      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
-      _Ap32, _part_p);
-  } else {
-    hipLaunchKernelGGL((search_spmv_kernel<false, float, true>), TILES_LAUNCH,
-      ddom, K, T, _cg_state, q, _part_r, nt, _r32, pold, fluid_mask(), pnew,
-      _Ap32, _part_p);
-  }
+  if (first)
+    launch_group<true, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
+  else
+    launch_group<false, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
   bb_kt_end(KT_SPMV32);
   int mode = phi_skips32(q) ? PHI_SKIP
            : (q > pair_base32 && phi_skips32(q - 1)) ? PHI_PAIR : PHI_EACH;
   int kt = mode == PHI_SKIP ? KT_UPDATE32_SKIP
          : mode == PHI_PAIR ? KT_UPDATE32_PAIR : KT_UPDATE32;
   bb_kt_begin(kt);
-  int tzb = spmv_tzb();
-  if (bb_spmv_variant == 2 && bb_update_group && tzb > 1) {
-    hipLaunchKernelGGL(update_group_kernel<float>, spmv_group_grid(tzb),
-      tiles_block(), 0, bb_stream, ddom, T, _cg_state, q, _part_p, nt, tzb,
-      pnew, _Ap32, fluid_mask(), _e32, _r32, _part_r);
-  } else {
-    const uint8_t *fm = fluid_mask();
+  const uint8_t *fm = fluid_mask();
 #define UPD(M, P) hipLaunchKernelGGL((update_kernel<float, M, P>),           \
-      TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Ap32, fm,    \
-      _e32, _r32, _part_r, pold)
-    if (fm) {
-      if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
-      else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
-      else UPD(true, PHI_EACH);
-    } else {
-      if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
-      else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
-      else UPD(false, PHI_EACH);
-    }
-#undef UPD
+    TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Ap32, fm,      \
+    _e32, _r32, _part_r, pold)
+  if (fm) {
+    if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+    else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+    else UPD(true, PHI_EACH);
+  } else {
+    if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+    else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+    else UPD(false, PHI_EACH);
   }
+#undef UPD
   bb_kt_end(kt);
-  rp_n = nt;
   BB_LAUNCH_CHECK();
 }
 
@@ -2364,7 +1558,6 @@ static void pcg_mixed(double ts, int warm)
     hipLaunchKernelGGL(mx_init_kernel, TILES_LAUNCH, ddom, T, _rhs_p,
       fluid_mask(), _phi, _e32, _r32, _part_r);
   }
-  rp_n = nt;
   hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(TILE_THREADS), 0,
     bb_stream, _part_p, _part_r, nt, _cg_state, pp_residual*pp_residual,
     pp_inner_tol*pp_inner_tol);
@@ -2393,7 +1586,7 @@ static void pcg_mixed(double ts, int warm)
     for (int qq = q + 1; qq <= qend; qq++)
       enqueue_iteration32(qq, qq == first_q, K32, T32);
     hipLaunchKernelGGL(cg_check_kernel<true>, dim3(1), dim3(TILE_THREADS),
-      0, bb_stream, _cg_state, qend, _part_r, rp_n);
+      0, bb_stream, _cg_state, qend, _part_r, nt);
     fetch_state();
     if (!cg_host->inner_done) {
       q = qend;
@@ -2410,7 +1603,6 @@ static void pcg_mixed(double ts, int warm)
       _rhs_p, fluid_mask(), _r32, _part_r);
     hipLaunchKernelGGL(mx_state_kernel, dim3(1), dim3(TILE_THREADS), 0,
       bb_stream, _cg_state, _part_r, nt, q, bb_mixed_restart);
-    rp_n = nt;
     BB_LAUNCH_CHECK();
     nouter++;
     fetch_state();

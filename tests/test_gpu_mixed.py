@@ -27,20 +27,15 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p")
 
 
-def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0,
-              variant=2, defer=None):
-    """x2 = 1: the FP32 kernels on x pairs (used when nx is even), 0: one
-    cell per lane (the default); variant: the search kernel (2: LDS planes,
-    3: one wavefront per row); defer: 1 pairs the inner sweep's updates of
-    the correction, 0 updates it every iteration (None: library default)."""
+def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, defer=None):
+    """defer: 1 pairs the inner sweep's updates of the correction, 0 updates
+    it every iteration (None: library default)."""
     lib = bbhip.load()
     defer_knob = ctypes.c_int.in_dll(lib, "bb_defer_phi")
     defer_default = defer_knob.value
     if defer is not None:
         defer_knob.value = defer
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = restart
-    ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = x2
-    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = 16
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})
@@ -64,8 +59,6 @@ def run_mixed(case_dir, kw, nsteps, restart=0, inner_tol=None, x2=0,
         for f in FIELDS:
             res["gpu"][f] = np.array(s.field(f))
     ctypes.c_int.in_dll(lib, "bb_mixed_restart").value = 0
-    ctypes.c_int.in_dll(lib, "bb_mixed_x2").value = 0
-    ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = 2
     defer_knob.value = defer_default
     return res
 
@@ -99,22 +92,17 @@ def test_mixed_cavity_32_100_steps(case_dir):
     check_mixed(res, {"u": 1e-10, "v": 1e-9, "w": 1e-10, "p": 1e-9})
 
 
-@pytest.mark.parametrize("x2,variant", [(1, 2), (0, 2), (0, 3)])
-def test_mixed_channel_64(case_dir, x2, variant):
-    """Periodic channel 64x32x32 (singular Neumann-periodic system), 5 steps,
-    with the x-pair FP32 kernels (one partial 128-cell tile in x, periodic
-    wrap through the pair loader) and with one cell per lane (both search
-    kernels)."""
-    res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 5, x2=x2,
-                    variant=variant)
+def test_mixed_channel_64(case_dir):
+    """Periodic channel 64x32x32 (singular Neumann-periodic system), 5
+    steps."""
+    res = run_mixed(case_dir, bbconfig.channel(n=(64, 32, 32)), 5)
     check_mixed(res, {"u": 1e-9, "v": 1e-7, "w": 1e-6, "p": 1e-6})
 
 
-def test_mixed_channel_two_x_tiles(case_dir):
-    """Channel 192x16x24: one full and one partial 128-cell tile in x (the
-    halo between x-pair tiles and the periodic wrap at x = nx + 1), a
-    partial tile in z, 3 steps."""
-    res = run_mixed(case_dir, bbconfig.channel(n=(192, 16, 24)), 3, x2=1)
+def test_mixed_channel_three_x_tiles(case_dir):
+    """Channel 192x16x24: three 64-cell tiles in x (the halo between them
+    and the periodic wrap at x = nx + 1), a partial tile in z, 3 steps."""
+    res = run_mixed(case_dir, bbconfig.channel(n=(192, 16, 24)), 3)
     check_mixed(res, {"u": 1e-9, "v": 1e-7, "w": 1e-6, "p": 1e-6})
 
 

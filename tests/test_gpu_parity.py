@@ -20,18 +20,16 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def set_knobs(variant=2, tzb=0, batch=16, group=0, defer=0):
+def set_knobs(variant=2, tzb=0, batch=16, defer=0):
     """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
-    3: the same z-columns with one wavefront per row and no LDS planes,
-    1: one tile per workgroup, 0: direct neighbour loads; group = 1: the
-    update kernel over the same z-columns; defer = 1: phi updated every
-    second iteration with both terms (odd iterations leave it)."""
+    0: direct neighbour loads (the cross-check implementation); defer = 1:
+    phi updated every second iteration with both terms (odd iterations
+    leave it)."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_defer_phi").value = defer
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_spmv_tzb").value = tzb
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = batch
-    ctypes.c_int.in_dll(lib, "bb_update_group").value = group
 
 
 @pytest.fixture(autouse=True)
@@ -42,10 +40,9 @@ def default_knobs():
     set_knobs()
 
 
-def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, group=0,
-             defer=0):
+def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, defer=0):
     n = kw["n"]
-    set_knobs(variant, tzb, batch, group, defer)
+    set_knobs(variant, tzb, batch, defer)
     bbconfig.write_case(case_dir, **kw)
     out = {"gpu": {}, "orc": {}, "iters_gpu": [], "iters_orc": []}
     with orc.Oracle(case_dir) as o:
@@ -88,8 +85,7 @@ def test_cavity_32_full_100_steps(case_dir):
     assert out["iters_gpu"] == out["iters_orc"]
 
 
-@pytest.mark.parametrize("variant,tzb", [(0, 0), (1, 0), (2, 1), (2, 2),
-                                         (3, 1), (3, 2)])
+@pytest.mark.parametrize("variant,tzb", [(0, 0), (2, 1), (2, 2)])
 def test_cavity_bitwise(case_dir, variant, tzb):
     out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant, tzb)
     for f in FIELDS:
@@ -105,25 +101,22 @@ def test_channel_small(case_dir):
     assert out["gpu_dt"] == out["orc_dt"]
 
 
-@pytest.mark.parametrize("variant,tzb,group,defer",
-                         [(0, 0, 0, 0), (1, 0, 0, 0), (2, 1, 0, 0),
-                          (2, 4, 0, 0), (2, 2, 1, 0), (3, 4, 0, 0),
-                          (2, 4, 0, 1), (2, 1, 0, 1), (2, 8, 0, 1)])
-def test_channel_bitwise(case_dir, variant, tzb, group, defer):
+@pytest.mark.parametrize("variant,tzb,defer",
+                         [(0, 0, 0), (2, 1, 0), (2, 4, 0), (2, 4, 1),
+                          (2, 1, 1), (2, 8, 1)])
+def test_channel_bitwise(case_dir, variant, tzb, defer):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
-                   tzb, group=group, defer=defer)
+                   tzb, defer=defer)
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,tzb,batch,group,defer",
-                         [(0, 0, 16, 0, 0), (1, 0, 16, 0, 0), (1, 0, 5, 0, 0),
-                          (2, 2, 1, 0, 0), (2, 3, 16, 1, 0), (2, 4, 5, 0, 0),
-                          (2, 2, 16, 1, 0), (3, 3, 16, 0, 0), (3, 2, 5, 0, 0),
-                          (2, 2, 1, 0, 1), (2, 4, 5, 0, 1), (2, 3, 16, 0, 1),
-                          (2, 8, 16, 0, 1)])
-def test_ragged_sizes(case_dir, variant, tzb, batch, group, defer):
+@pytest.mark.parametrize("variant,tzb,batch,defer",
+                         [(0, 0, 16, 0), (0, 0, 5, 0), (2, 2, 1, 0),
+                          (2, 4, 5, 0), (2, 3, 16, 0), (2, 2, 1, 1),
+                          (2, 4, 5, 1), (2, 3, 16, 1), (2, 8, 16, 1)])
+def test_ragged_sizes(case_dir, variant, tzb, batch, defer):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
     3, not a multiple of the z-column length), shear walls."""
@@ -131,7 +124,7 @@ def test_ragged_sizes(case_dir, variant, tzb, batch, group, defer):
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     kw["vel_bc"] = bbconfig.walls(lid=("v", "E", 0.5))
-    out = run_pair(case_dir, kw, 3, variant, tzb, batch, group, defer)
+    out = run_pair(case_dir, kw, 3, variant, tzb, batch, defer)
     check(out)
     assert out["iters_gpu"] == out["iters_orc"]
     for f in FIELDS:
@@ -141,7 +134,7 @@ def test_ragged_sizes(case_dir, variant, tzb, batch, group, defer):
 
 def test_periodic_all_and_wide_x(case_dir):
     """Triply periodic box (wrapped neighbours in every direction) with an x
-    extent past the 2.5-D kernel's limit (row-kernel fallback)."""
+    extent of several 64-cell tiles and a ragged last one."""
     per = ("PERIODIC",)
     vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
     kw = dict(n=(450, 6, 5), lo=(0.0, 0.0, 0.0), hi=(4.5, 0.6, 0.5),
@@ -156,7 +149,7 @@ def test_periodic_all_and_wide_x(case_dir):
             describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,defer", [(2, 0), (3, 0), (2, 1)])
+@pytest.mark.parametrize("variant,defer", [(2, 0), (0, 0), (2, 1)])
 def test_periodic_all_25d(case_dir, variant, defer):
     per = ("PERIODIC",)
     vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
