@@ -70,6 +70,12 @@ def load():
                                         ctypes.POINTER(ctypes.c_int)]
         L.orc_part_contacts.restype = ctypes.c_int
         L.orc_lebedev.argtypes = [D, D, D]
+        L.orc_lamb_eval_p.argtypes = [ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double]
+        L.orc_lamb_eval_p.restype = ctypes.c_double
+        L.orc_part_nodes.argtypes = [ctypes.c_int, D, D, D, D,
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.orc_lamb_from_nodes.argtypes = [ctypes.c_int, D, D, D, D]
         _lib = L
     return _lib
 
@@ -191,6 +197,34 @@ class Oracle:
             s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return list(i), list(s), int(c[0]), int(c[1])
+
+    def lamb_eval_p(self, n, x, y, z):
+        """Lamb pressure series of particle n at (x, y, z) from its centre
+        (no rest-frame terms)."""
+        return self.lib.orc_lamb_eval_p(n, x, y, z)
+
+    def nodes(self, n):
+        """Rest-frame (p, u_r, u_theta, u_phi, code) at the 26 nodes of
+        particle n on the current fields."""
+        P = ctypes.POINTER(ctypes.c_double)
+        v = [np.zeros(26) for _ in range(4)]
+        code = np.zeros(26, dtype=np.int32)
+        self.lib.orc_part_nodes(n, *[a.ctypes.data_as(P) for a in v],
+                                code.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return v[0], v[1], v[2], v[3], code
+
+    def lamb_from_nodes(self, n, pp, ur, ut, up):
+        """Quadrature, coefficients and forces of particle n from given
+        rest-frame node values."""
+        P = ctypes.POINTER(ctypes.c_double)
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+                for a in (pp, ur, ut, up)]
+        self.lib.orc_lamb_from_nodes(n, *[a.ctypes.data_as(P) for a in arrs])
+
+    def part_bc_once(self):
+        """Particle boundary values (faces next to particles, p inside)
+        from the current coefficients."""
+        self.lib.orc_part_bc_once()
 
     def lamb_eval(self, n, x, y, z):
         U = np.zeros(4)

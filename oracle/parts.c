@@ -679,14 +679,15 @@ static void o_node(OP *Q, int node, double *pp, double *ur, double *ut,
   *up = -uu * sp + vv * cp;
 }
 
-static void o_lamb(void)
+/* Quadrature, Lamb coefficients (with relaxation) and forces of one
+ * particle from its rest-frame node values (src/physalis_kernel.cu:
+ * 642-841). */
+static void o_lamb_solve(OP *Q, const double *pp, const double *ur,
+  const double *ut, const double *up)
 {
   double relax = O_lamb_relax, mu = O.mu, nu = O.nu;
-  for (int n = 0; n < NP_; n++) {
-    OP *Q = &P_[n];
-    double pp[O_NNODES], ur[O_NNODES], ut[O_NNODES], up[O_NNODES];
-    for (int node = 0; node < O_NNODES; node++)
-      o_node(Q, node, &pp[node], &ur[node], &ut[node], &up[node]);
+  (void) ur;
+  {
     for (int coeff = 0; coeff < Q->ncoeff; coeff++) {
       int nn = o_nn[coeff], m = o_mm[coeff];
       double S[6] = {0, 0, 0, 0, 0, 0};
@@ -763,6 +764,17 @@ static void o_lamb(void)
       + 8. * PI * mu * nu * 2.*N11 * Q->r * Q->chinm_im[2];
     Q->Lz = O.rho_f * vol * Q->r*Q->r * Q->ozdot
       + 8. * PI * mu * nu * N10 * Q->r * Q->chinm_re[1];
+  }
+}
+
+static void o_lamb(void)
+{
+  for (int n = 0; n < NP_; n++) {
+    OP *Q = &P_[n];
+    double pp[O_NNODES], ur[O_NNODES], ut[O_NNODES], up[O_NNODES];
+    for (int node = 0; node < O_NNODES; node++)
+      o_node(Q, node, &pp[node], &ur[node], &ut[node], &up[node]);
+    o_lamb_solve(Q, pp, ur, ut, up);
   }
 }
 
@@ -1389,6 +1401,54 @@ void orc_lamb_eval(int n, double x, double y, double z, double *U)
   o_xyz2rtp(x, y, z, &r, &th, &ph);
   o_lamb_vel(Q->order, Q->r, r, th, ph, Q, O.nu, &U[0], &U[1], &U[2]);
   U[3] = o_X(0, th, ph, Q->pnm_re, Q->pnm_im) * O.mu * O.nu / (Q->r * Q->r);
+}
+
+/* The Lamb pressure series of particle n's coefficients at (x, y, z)
+ * relative to its centre, without the rest-frame terms (the series of
+ * src/particle_kernel.cu:1716-1729). */
+double orc_lamb_eval_p(int n, double x, double y, double z)
+{
+  const OP *Q = &P_[n];
+  double r, th, ph;
+  o_xyz2rtp(x, y, z, &r, &th, &ph);
+  double a = Q->r, ar = a / r, ra = r / a;
+  double pp = o_X(0, th, ph, Q->pnm_re, Q->pnm_im);
+  for (int nn = 1; nn <= Q->order; nn++) {
+    pp += (1.-0.5*nn*(2.*nn-1.)/(nn+1.)*pow(ar,2.*nn+1.))*pow(ra,nn)
+      * o_X(nn, th, ph, Q->pnm_re, Q->pnm_im);
+    pp -= nn*(2.*nn-1.)*(2.*nn+1.)/(nn+1.)*pow(ar,nn+1.)
+      * o_X(nn, th, ph, Q->phinm_re, Q->phinm_im);
+  }
+  return pp * O.mu * O.nu / (a * a);
+}
+
+/* The rest-frame node values of particle n on the current fields
+ * (pressure, u_r, u_theta, u_phi at the 26 nodes) and the node codes. */
+void orc_part_nodes(int n, double *pp, double *ur, double *ut, double *up,
+  int *code)
+{
+  OP *Q = &P_[n];
+  for (int node = 0; node < O_NNODES; node++) {
+    o_node(Q, node, &pp[node], &ur[node], &ut[node], &up[node]);
+    code[node] = Q->nodes[node];
+  }
+}
+
+/* Quadrature, coefficients and forces of particle n from given rest-frame
+ * node values (the part of the Physalis update after the interpolation). */
+void orc_lamb_from_nodes(int n, const double *pp, const double *ur,
+  const double *ut, const double *up)
+{
+  o_lamb_solve(&P_[n], pp, ur, ut, up);
+}
+
+/* The particle boundary values on the current coefficients: u, v, w on
+ * the faces next to particles (o_part_bc) and p inside them
+ * (o_part_p_fill). */
+void orc_part_bc_once(void)
+{
+  o_part_bc(u_, v_, w_);
+  o_part_p_fill();
 }
 
 /* Run the Physalis update once on the current fields (quadrature, Lamb
