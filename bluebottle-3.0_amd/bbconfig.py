@@ -197,8 +197,9 @@ def part_config(parts, l_over_a=1.0):
     """Return part.config text in the grammar of src/particle.c:66-391.
 
     parts: list of dicts with keys r, x, y, z and optional u v w, rho, E,
-    sigma, e_dry, coeff_fric, order, rs_r, translating, rotating, and the
-    scalar keys s, update, cp, srs_r (scalar node radius / r), s_order.
+    sigma, e_dry, coeff_fric, order, rs_r, spring_k, spring_x, spring_y,
+    spring_z, spring_l, translating, rotating, and the scalar keys s,
+    update, cp, srs_r (scalar node radius / r), s_order.
     """
     L = [f"n {len(parts)}", f"(l/a) {_fmt(float(l_over_a))}"]
     for p in parts:
@@ -218,9 +219,10 @@ def part_config(parts, l_over_a=1.0):
         L.append(f"coeff_fric {_fmt(g('coeff_fric', 0.5))}")
         L.append(f"order {int(p.get('order', 4))}")
         L.append(f"rs/r {_fmt(g('rs_r', 1.2))}")
-        L.append("spring_k 0")
-        L.append("spring (x, y, z) 0 0 0")
-        L.append("spring_l 0")
+        L.append(f"spring_k {_fmt(g('spring_k', 0))}")
+        L.append(f"spring (x, y, z) {_fmt(g('spring_x', 0))} "
+                 f"{_fmt(g('spring_y', 0))} {_fmt(g('spring_z', 0))}")
+        L.append(f"spring_l {_fmt(g('spring_l', 0))}")
         L.append(f"translating {int(p.get('translating', 1))}")
         L.append(f"rotating {int(p.get('rotating', 1))}")
         L.append(f"s {_fmt(g('s', 0))}")

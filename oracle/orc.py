@@ -187,6 +187,10 @@ class Oracle:
         """Collisions, velocity update, collisions (one Lamb sub-step)."""
         self.lib.orc_move_parts_sub()
 
+    def move_b(self):
+        """End-of-step update: positions, body axes, previous values."""
+        self.lib.orc_move_parts_b()
+
     def contacts(self, n):
         """(iSt list, St list, ncoll_part, ncoll_wall) of particle n."""
         i = np.zeros(12, dtype=np.int32)

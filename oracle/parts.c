@@ -1392,6 +1392,14 @@ void orc_move_parts_sub(void)
   o_move_sub();
 }
 
+/* The end-of-step particle update: positions, body axes, stored
+ * previous values and the periodic wrap (src/particle_kernel.cu:
+ * 3412-3484). */
+void orc_move_parts_b(void)
+{
+  o_move_b();
+}
+
 /* Lamb velocity (Cartesian) of particle n's current coefficients at the
  * point (x, y, z) relative to its centre, and the n = 0 pressure term. */
 void orc_lamb_eval(int n, double x, double y, double z, double *U)
