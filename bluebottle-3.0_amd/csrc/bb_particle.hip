@@ -121,191 +121,171 @@ __device__ inline double near_img(double q, double p, double l, int per)
   return p;
 }
 
-/* ------------------------- spherical helpers ----------------------------- */
-/* src/particle_kernel.cu:1905-1917 (the DIV_ST guards compare against a
- * negative number and never fire). */
-__device__ inline void xyz2rtp(double x, double y, double z, double *r,
+/* ------------------------- small vector algebra -------------------------- */
+struct v3 { double x, y, z; };
+
+__device__ inline v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline v3 operator*(double s, v3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ inline double dot(v3 a, v3 b) { return a.x*b.x + a.y*b.y + a.z*b.z; }
+__device__ inline v3 cross(v3 a, v3 b)
+{
+  return {a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x};
+}
+__device__ inline double norm(v3 a) { return sqrt(dot(a, a)); }
+
+/* Acceleration of the fluid the particle frame sees from the imposed
+ * pressure gradient and the Boussinesq buoyancy of the particle's scalar,
+ * -gradP/rho_f - s_beta (s - s_ref) g. */
+__device__ inline v3 frame_accel(gradP_struct gradP, double rho_f,
+  double s_beta, double s, double s_ref, g_struct g)
+{
+  const double b = -s_beta * (s - s_ref);
+  return (-1. / rho_f) * v3{gradP.x, gradP.y, gradP.z} + b * v3{g.x, g.y, g.z};
+}
+
+/* Velocity of a particle's rigid frame at offset rv (|rv| = r) from its
+ * centre, with the rest frame's correction for unsteady rotation:
+ * U + w x r + (r^5 - a^5)/(10 nu r^3) dw/dt x r. */
+__device__ inline v3 frame_velocity(const part_struct *Q, v3 rv, double r,
+  double nu)
+{
+  const double a2 = Q->r * Q->r, r2 = r * r;
+  const double unsteady = 0.1 / nu * (r2*r2*r - a2*a2*Q->r) / (r2*r);
+  return v3{Q->u, Q->v, Q->w} + cross(v3{Q->ox, Q->oy, Q->oz}, rv)
+       + unsteady * cross(v3{Q->oxdot, Q->oydot, Q->ozdot}, rv);
+}
+
+/* ------------------------- spherical harmonics -------------------------- */
+/* Spherical coordinates of (x, y, z) about a particle centre, phi in
+ * [0, 2 pi) (the convention of src/particle_kernel.cu:1905-1917, whose
+ * guards compare against the negative DIV_ST and never act). */
+__device__ inline void to_spherical(double x, double y, double z, double *r,
   double *theta, double *phi)
 {
-  double XY = x*x + y*y;
-  double XYZ = XY + z*z;
-  if (XYZ >= 0 && XYZ < DIV_ST) XYZ = DIV_ST;
-  *r = sqrt(XYZ);
+  double rho2 = x*x + y*y;
+  *r = sqrt(rho2 + z*z);
   *theta = acos(z / *r);
-  if (XY >= 0 && XY < DIV_ST) XY = DIV_ST;
-  *phi = acos(x / sqrt(XY));
+  *phi = acos(x / sqrt(rho2));
   if (y < 0.) *phi = 2.*PI - *phi;
 }
 
-/* src/particle_kernel.cu:1919-1929 */
-__device__ inline double Nnm(int n, int m)
+/* Highest Lamb order: NCOEFFS = (LMAX + 1)(LMAX + 2)/2 coefficients. */
+#define LMAX 4
+
+/* Normalisation of the surface harmonic Y_n^m = N_n^m P_n^m(cos t) e^{imp}:
+ * N_n^m = sqrt((2n+1)/(4 pi) (n-m)!/(n+m)!). */
+__device__ inline double harmonic_norm(int n, int m)
 {
-  double fact_top = 1;
-  double fact_bot = 1;
-  for (int i = 1; i <= (n-m); i++) fact_top *= (double) i;
-  for (int i = 1; i <= (n+m); i++) fact_bot *= (double) i;
-  return sqrt((2.*n+1.) / 4. / PI * fact_top / fact_bot);
+  double ratio = 1.;
+  for (int q = n - m + 1; q <= n + m; q++) ratio /= (double) q;
+  return sqrt((2.*n+1.) / (4.*PI) * ratio);
 }
 
-/* Associated Legendre functions P_n^m(cos theta), n <= 5, m >= 0
- * (src/particle_kernel.cu:1931-1991). */
-__device__ inline double Pnm(int n, int m, double theta)
+/* Associated Legendre functions P_n^m(cos t), Condon-Shortley phase, for
+ * 0 <= m <= n <= nmax, from the recurrences
+ *   P_m^m = (-1)^m (2m-1)!! sin^m t,   P_{m+1}^m = (2m+1) cos t P_m^m,
+ *   (n - m) P_n^m = (2n-1) cos t P_{n-1}^m - (n+m-1) P_{n-2}^m.
+ * P is indexed [n][m]; entries with m > n are left untouched. */
+template <int NMAX>
+__device__ inline void legendre_table(double ct, double st,
+  double P[NMAX + 1][NMAX + 1])
 {
-  double x = cos(theta);
-  double y = sin(theta);
-  switch (n) {
-    case 0: return 1;
-    case 1:
-      if (m == 0) return x;
-      if (m == 1) return -y;
-      break;
-    case 2:
-      if (m == 0) return 0.5*(3.*x*x - 1.);
-      if (m == 1) return -3.*x*y;
-      if (m == 2) return 3.*y*y;
-      break;
-    case 3:
-      if (m == 0) return 0.5*x*(5.*x*x - 3.);
-      if (m == 1) return -1.5*(5.*x*x - 1.)*y;
-      if (m == 2) return 15.*x*y*y;
-      if (m == 3) return -15.*y*y*y;
-      break;
-    case 4:
-      if (m == 0) return 0.125*(35.*x*x*x*x - 30.*x*x + 3.);
-      if (m == 1) return -2.5*(7.*x*x - 3.)*x*y;
-      if (m == 2) return 7.5*(7.*x*x - 1.)*y*y;
-      if (m == 3) return -105.*x*y*y*y;
-      if (m == 4) return 105.*y*y*y*y;
-      break;
-    case 5:
-      if (m == 0) return 0.125*x*(63.*x*x*x*x - 70.*x*x + 15.);
-      if (m == 1) return -1.875*y*(21.*x*x*x*x - 14.*x*x + 1.);
-      if (m == 2) return 52.5*x*y*y*(3.*x*x - 1.);
-      if (m == 3) return -52.5*y*y*y*(9.*x*x - 1.);
-      if (m == 4) return 945.*x*y*y*y*y;
-      if (m == 5) return -945.*y*y*y*y*y;
-      break;
+  double pmm = 1.;
+#pragma unroll
+  for (int m = 0; m <= NMAX; m++) {
+    if (m > 0) pmm *= -(2.*m - 1.) * st;
+    P[m][m] = pmm;
+    if (m + 1 <= NMAX) P[m + 1][m] = (2.*m + 1.) * ct * pmm;
+#pragma unroll
+    for (int n = m + 2; n <= NMAX; n++)
+      P[n][m] = ((2.*n - 1.) * ct * P[n - 1][m]
+                 - (n + m - 1.) * P[n - 2][m]) / (double) (n - m);
   }
-  return 0;
 }
 
-/* The series terms of one order n for coefficient arrays (re, im)
- * (src/particle_kernel.cu:1993-2170): X = sum_m N P (c e^{i m phi}),
- * Y = d/dtheta form, Z = 1/sin(theta) d/dphi form. */
-__device__ inline int coeff0(int n)
+/* One P_n^m(cos theta), n <= LMAX + 1. */
+__device__ inline double legendre_nm(int n, int m, double theta)
 {
-  int c = 0;
-  for (int j = 0; j < n; j++) c += j + 1;
-  return c;
+  double P[LMAX + 2][LMAX + 2];
+  legendre_table<LMAX + 1>(cos(theta), sin(theta), P);
+  return (m > n) ? 0. : P[n][m];
 }
 
-__device__ inline double Xn(int n, double theta, double phi,
-  const double *re, const double *im)
-{
-  int coeff = coeff0(n);
-  int m = 0;
-  double sum = Nnm(n,m)*Pnm(n,m,theta)*re[coeff];
-  for (m = 1; m <= n; m++) {
-    coeff++;
-    sum += 2.*Nnm(n,m)*Pnm(n,m,theta)
-      *(re[coeff]*cos(m*phi) - im[coeff]*sin(m*phi));
-  }
-  return sum;
-}
-
-__device__ inline double st_guard(double theta)
-{
-  double st = sin(theta);
-  if (st >= 0 && st < DIV_ST) st = DIV_ST;
-  else if (st < 0 && st > -DIV_ST) st = -DIV_ST;
-  return st;
-}
-
-__device__ inline double Yn(int n, double theta, double phi,
-  const double *re, const double *im)
-{
-  double ct = cos(theta);
-  double st = st_guard(theta);
-  int coeff = coeff0(n);
-  int m = 0;
-  double sum = Nnm(n,m)
-    *(-(n+1)*ct/st*Pnm(n,m,theta)+(n-m+1)/st*Pnm(n+1,m,theta))
-    *re[coeff];
-  for (m = 1; m <= n; m++) {
-    coeff++;
-    sum += 2.*Nnm(n,m)
-      *(-(n+1)*ct/st*Pnm(n,m,theta)+(n-m+1)/st*Pnm(n+1,m,theta))
-      *(re[coeff]*cos(m*phi) - im[coeff]*sin(m*phi));
-  }
-  return sum;
-}
-
-__device__ inline double Zn(int n, double theta, double phi,
-  const double *re, const double *im)
-{
-  double st = st_guard(theta);
-  int coeff = coeff0(n);
-  double sum = 0.;
-  for (int m = 1; m <= n; m++) {
-    coeff++;
-    sum += -2.*m/st*Nnm(n,m)*Pnm(n,m,theta)
-      *(re[coeff]*sin(m*phi) + im[coeff]*cos(m*phi));
-  }
-  return sum;
-}
-
-/* Lamb's velocity at (r, theta, phi) in Cartesian components
- * (src/particle_kernel.cu:1839-1903). */
+/* Velocity of Lamb's solution for the flow outside a sphere of radius a, at
+ * (r, theta, phi) about its centre, in Cartesian components (the series of
+ * src/particle_kernel.cu:1839-1903).  With rho = r/a, s = a/r and the three
+ * harmonic coefficient sets p (pressure), f (phi) and c (chi) of each order
+ * n, their surface fields at the point
+ *   X_n = Re sum_m w_m c_nm Y_n^m,  T_n = Re sum_m w_m c_nm dY_n^m/dt,
+ *   F_n = Re sum_m w_m c_nm (1/sin t) dY_n^m/dp
+ * (w_0 = 1, w_m = 2: the m < 0 terms are the conjugates) give
+ *   u_r = (nu/a) sum_n  Ar_n X_n[p] + Br_n X_n[f]
+ *   u_t = (nu/a) sum_n  At_n T_n[p] + Bt_n T_n[f] + C_n F_n[c]
+ *   u_p = (nu/a) sum_n  At_n F_n[p] + Bt_n F_n[f] - C_n T_n[c]
+ * with the radial functions
+ *   Ar_n = n rho^(n+1) / (2(2n+3)) + (n/4)((2n+1)/(2n+3) s^2 - 1) s^n,
+ *   Br_n = n rho^(n-1) + (n/2)(2n-1 - (2n+1) rho^2) s^(n+2),
+ *   At_n = (n+3) rho^(n+1) / (2(n+1)(2n+3))
+ *          + (n-2 - n(2n+1)/(2n+3) s^2) s^n / (4(n+1)),
+ *   Bt_n = rho^(n-1) + ((n-2)(2n+1) rho^2 - n(2n-1)) s^(n+2) / (2(n+1)),
+ *   C_n  = rho^(n-1) - s^(n+1).
+ * The n = 0 pressure term (a constant) drives no flow.  dP_n^m/dt is formed
+ * as ((n-m+1) P_{n+1}^m - (n+1) cos t P_n^m) / sin t. */
 __device__ inline void lamb_vel(int order, double a, double r, double theta,
-  double phi, const part_struct *P, double nu, double *Ux, double *Uy,
+  double phi, const part_struct *Q, double nu, double *Ux, double *Uy,
   double *Uz)
 {
-  double ia = 1. / a;
-  double ar = a / r;
-  double ra = r * ia;
-  double ur = 0.;
-  double ut = 0.5*ra*Yn(0, theta, phi, P->pnm_re, P->pnm_im);
-  double up = 0.5*ra*Zn(0, theta, phi, P->pnm_re, P->pnm_im);
+  const double ct = cos(theta), st = sin(theta);
+  const double cp = cos(phi), sp = sin(phi);
+  double P[LMAX + 2][LMAX + 2];
+  legendre_table<LMAX + 1>(ct, st, P);
+  const double rho = r / a, s = a / r;
+  double ur = 0., ut = 0., up = 0.;
+  double rho_nm1 = 1., s_n = s;             /* rho^(n-1), s^n at n = 1 */
   for (int n = 1; n <= order; n++) {
-    double powranm1 = pow(ra,n-1.);
-    double powranp1 = powranm1*ra*ra;
-    double powarn = pow(ar,n);
-    double powarnp1 = powarn*ar;
-    double powarnp2 = powarnp1*ar;
-    double od2np3 = 1./(2.*n+3.);
-    double odnp1 = 1./(n+1.);
-    ur += (0.5*n*od2np3*powranp1
-      + 0.25*n*((2.*n+1.)*od2np3*ar*ar-1.)*powarn)
-      * Xn(n, theta, phi, P->pnm_re, P->pnm_im);
-    ur += (n*powranm1
-      + 0.5*n*(2.*n-1.-(2.*n+1.)*ra*ra)*powarnp2)
-      * Xn(n, theta, phi, P->phinm_re, P->phinm_im);
-    ut += (0.5*(n+3.)*odnp1*od2np3*powranp1
-      + 0.25*odnp1*(n-2.-n*(2.*n+1.)*od2np3*ar*ar)*powarn)
-      * Yn(n, theta, phi, P->pnm_re, P->pnm_im);
-    ut += (powranm1
-      + 0.5*odnp1*((n-2.)*(2.*n+1.)*ra*ra-n*(2.*n-1.))*powarnp2)
-      * Yn(n, theta, phi, P->phinm_re, P->phinm_im);
-    ut += (powranm1
-      - powarnp1)
-      * Zn(n, theta, phi, P->chinm_re, P->chinm_im);
-    up += (0.5*(n+3.)*odnp1*od2np3*powranp1
-      + 0.25*odnp1*(n-2.-n*(2.*n+1.)*od2np3*ar*ar)*powarn)
-      * Zn(n, theta, phi, P->pnm_re, P->pnm_im);
-    up += (powranm1
-      + 0.5*odnp1*((n-2.)*(2.*n+1.)*ra*ra-n*(2.*n-1.))*powarnp2)
-      * Zn(n, theta, phi, P->phinm_re, P->phinm_im);
-    up += (-powranm1
-      + powarnp1)
-      * Yn(n, theta, phi, P->chinm_re, P->chinm_im);
+    double X[3] = {0., 0., 0.}, T[3] = {0., 0., 0.}, F[3] = {0., 0., 0.};
+    const double *re[3] = {Q->pnm_re, Q->phinm_re, Q->chinm_re};
+    const double *im[3] = {Q->pnm_im, Q->phinm_im, Q->chinm_im};
+    const int c0 = n * (n + 1) / 2;
+    double cm = 1., sm = 0.;                /* cos(m phi), sin(m phi) */
+    for (int m = 0; m <= n; m++) {
+      const double wN = (m ? 2. : 1.) * harmonic_norm(n, m);
+      const double Y = wN * P[n][m];
+      const double dY = wN * ((n - m + 1.) * P[n + 1][m]
+                              - (n + 1.) * ct * P[n][m]) / st;
+      const double mY = -m * Y / st;
+      for (int c = 0; c < 3; c++) {
+        const double cr = re[c][c0 + m] * cm - im[c][c0 + m] * sm;
+        const double ci = re[c][c0 + m] * sm + im[c][c0 + m] * cm;
+        X[c] += Y * cr;
+        T[c] += dY * cr;
+        F[c] += mY * ci;
+      }
+      const double cn = cm * cp - sm * sp;
+      sm = sm * cp + cm * sp;
+      cm = cn;
+    }
+    const double rho_np1 = rho_nm1 * rho * rho;
+    const double s2 = s * s;
+    const double i2n3 = 1. / (2.*n + 3.), in1 = 1. / (n + 1.);
+    const double Ar = 0.5*n*i2n3*rho_np1 + 0.25*n*((2.*n+1.)*i2n3*s2 - 1.)*s_n;
+    const double Br = n*rho_nm1 + 0.5*n*(2.*n-1. - (2.*n+1.)*rho*rho)*s_n*s2;
+    const double At = 0.5*(n+3.)*in1*i2n3*rho_np1
+                    + 0.25*in1*(n-2. - n*(2.*n+1.)*i2n3*s2)*s_n;
+    const double Bt = rho_nm1
+                    + 0.5*in1*((n-2.)*(2.*n+1.)*rho*rho - n*(2.*n-1.))*s_n*s2;
+    const double Cc = rho_nm1 - s_n * s;
+    ur += Ar * X[0] + Br * X[1];
+    ut += At * T[0] + Bt * T[1] + Cc * F[2];
+    up += At * F[0] + Bt * F[1] - Cc * T[2];
+    rho_nm1 *= rho;
+    s_n *= s;
   }
-  ur *= nu * ia;
-  ut *= nu * ia;
-  up *= nu * ia;
-  double st = sin(theta);
-  double ct = cos(theta);
-  double sp = sin(phi);
-  double cp = cos(phi);
+  const double scale = nu / a;
+  ur *= scale;
+  ut *= scale;
+  up *= scale;
   *Ux = ur*st*cp + ut*ct*cp - up*sp;
   *Uy = ur*st*sp + ut*ct*sp + up*cp;
   *Uz = ur*ct - ut*st;
@@ -540,30 +520,12 @@ __global__ void __launch_bounds__(64) part_bc_kernel(PDom P, double *vel,
   y -= near_img(y, Q->y, P.yl, P.pery);
   z -= near_img(z, Q->z, P.zl, P.perz);
   double r, theta, phi;
-  xyz2rtp(x, y, z, &r, &theta, &phi);
-  double Ux, Uy, Uz;
-  double a = Q->r;
-  lamb_vel(Q->order, a, r, theta, phi, Q, nu, &Ux, &Uy, &Uz);
-  double val;
-  if (dir == 0) {
-    double ocrossr_x = Q->oy*z - Q->oz*y;
-    double odotcrossr_x = Q->oydot*z - Q->ozdot*y;
-    Ux += Q->u + ocrossr_x;
-    Ux += 0.1/nu *(r*r*r*r*r-a*a*a*a*a)/(r*r*r) * odotcrossr_x;
-    val = Ux;
-  } else if (dir == 1) {
-    double ocrossr_y = -(Q->ox*z - Q->oz*x);
-    double odotcrossr_y = -(Q->oxdot*z - Q->ozdot*x);
-    Uy += Q->v + ocrossr_y;
-    Uy += 0.1/nu *(r*r*r*r*r-a*a*a*a*a)/(r*r*r) * odotcrossr_y;
-    val = Uy;
-  } else {
-    double ocrossr_z = Q->ox*y - Q->oy*x;
-    double odotcrossr_z = Q->oxdot*y - Q->oydot*x;
-    Uz += Q->w + ocrossr_z;
-    Uz += 0.1/nu *(r*r*r*r*r-a*a*a*a*a)/(r*r*r) * odotcrossr_z;
-    val = Uz;
-  }
+  to_spherical(x, y, z, &r, &theta, &phi);
+  double U[3];
+  lamb_vel(Q->order, Q->r, r, theta, phi, Q, nu, &U[0], &U[1], &U[2]);
+  const v3 Uf = frame_velocity(Q, v3{x, y, z}, r, nu);
+  const double frame[3] = {Uf.x, Uf.y, Uf.z};
+  const double val = U[dir] + frame[dir];
   vel[C] = val;
   vals[e] = val;
 }
@@ -695,23 +657,19 @@ __global__ void part_p_fill_kernel(PDom P, double *p,
   y -= near_img(y, Q->y, P.yl, P.pery);
   z -= near_img(z, Q->z, P.zl, P.perz);
   double r, theta, phi;
-  xyz2rtp(x, y, z, &r, &theta, &phi);
-  double a = Q->r;
-  double ox = Q->ox, oy = Q->oy, oz = Q->oz;
-  double pp_tmp = Xn(0, theta, phi, Q->pnm_re, Q->pnm_im);
-  pp_tmp *= mu*nu/(a*a);
-  double ocrossr2 = (oy*z - oz*y) * (oy*z - oz*y);
-  ocrossr2 += (ox*z - oz*x) * (ox*z - oz*x);
-  ocrossr2 += (ox*y - oy*x) * (ox*y - oy*x);
-  double rhoV = rho_f;
-  double bousiq_x = -s_beta*(Q->s - s_ref)*g.x;
-  double bousiq_y = -s_beta*(Q->s - s_ref)*g.y;
-  double bousiq_z = -s_beta*(Q->s - s_ref)*g.z;
-  double accdotr = (-gradP.x/rhoV - Q->udot + bousiq_x)*x +
-                   (-gradP.y/rhoV - Q->vdot + bousiq_y)*y +
-                   (-gradP.z/rhoV - Q->wdot + bousiq_z)*z;
-  pp_tmp += 0.5 * rho_f * ocrossr2 + rho_f * accdotr;
-  p[t] = pp_tmp;
+  to_spherical(x, y, z, &r, &theta, &phi);
+  const double a = Q->r;
+  const v3 rv = {x, y, z};
+  const v3 om = {Q->ox, Q->oy, Q->oz};
+  const v3 acc = {Q->udot, Q->vdot, Q->wdot};
+  const v3 wxr = cross(om, rv);
+  /* n = 0 pressure of the series plus the rigid-frame terms, as in the
+   * node values below */
+  double pc = harmonic_norm(0, 0) * Q->pnm_re[0] * (mu*nu/(a*a));
+  pc += 0.5 * rho_f * dot(wxr, wxr)
+      + rho_f * dot(frame_accel(gradP, rho_f, s_beta, Q->s, s_ref, g) - acc,
+                    rv);
+  p[t] = pc;
 }
 
 extern "C" void cuda_part_p_fill(void)
@@ -872,6 +830,13 @@ struct NodeVals {
   double pp[NNODES], ur[NNODES], ut[NNODES], up[NNODES];
 };
 
+/* Index of a cell or face coordinate clamped to the grid's interior range
+ * lo..hi. */
+__device__ inline int clamp_index(int i, int lo, int hi)
+{
+  return i < lo ? lo : (i > hi ? hi : i);
+}
+
 __device__ void node_values(const PDom &P, const PWalls &W, part_struct *Q,
   int node,
   const double *__restrict__ p, const double *__restrict__ u,
@@ -879,30 +844,18 @@ __device__ void node_values(const PDom &P, const PWalls &W, part_struct *Q,
   double nu, gradP_struct gradP, double s_beta, double s_ref, g_struct g,
   NodeVals *out)
 {
-  double ddx = 1. / P.dx, ddy = 1. / P.dy, ddz = 1. / P.dz;
-  double irho_f = 1. / rho_f;
-  double inu = 1. / nu;
-  double ox = Q->ox, oy = Q->oy, oz = Q->oz;
-  double oxdot = Q->oxdot, oydot = Q->oydot, ozdot = Q->ozdot;
-  double udot = Q->udot, vdot = Q->vdot, wdot = Q->wdot;
-  double rs2 = Q->rs * Q->rs;
-  double rs3 = rs2 * Q->rs;
-  double rs5 = rs3 * rs2;
-  double irs3 = 1./rs3;
-  double a5 = Q->r * Q->r;
-  a5 *= a5 * Q->r;
-
-  double th = c_node_t[node], ph = c_node_p[node];
-  double xp = Q->rs * sin(th) * cos(ph);
-  double yp = Q->rs * sin(th) * sin(ph);
-  double zp = Q->rs * cos(th);
+  const double ddx = 1. / P.dx, ddy = 1. / P.dy, ddz = 1. / P.dz;
+  const double th = c_node_t[node], ph = c_node_p[node];
+  /* node offset from the centre on the sphere of radius rs */
+  const v3 rn = {Q->rs * sin(th) * cos(ph), Q->rs * sin(th) * sin(ph),
+                 Q->rs * cos(th)};
   int ic, jc, kc;
-  double x = node_coord(xp, Q->x, P.xs, P.dx, P.xl, P.nx, P.perx, &ic);
-  double y = node_coord(yp, Q->y, P.ys, P.dy, P.yl, P.ny, P.pery, &jc);
-  double z = node_coord(zp, Q->z, P.zs, P.dz, P.zl, P.nz, P.perz, &kc);
-  int oobi = ic < 1 || ic > P.nx, oobj = jc < 1 || jc > P.ny,
-      oobk = kc < 1 || kc > P.nz;
-  int oob = oobi || oobj || oobk;
+  const double x = node_coord(rn.x, Q->x, P.xs, P.dx, P.xl, P.nx, P.perx, &ic);
+  const double y = node_coord(rn.y, Q->y, P.ys, P.dy, P.yl, P.ny, P.pery, &jc);
+  const double z = node_coord(rn.z, Q->z, P.zs, P.dz, P.zl, P.nz, P.perz, &kc);
+  const int oob[3] = {ic < 1 || ic > P.nx, jc < 1 || jc > P.ny,
+                      kc < 1 || kc > P.nz};
+  const int outside = oob[0] || oob[1] || oob[2];
   /* node code (src/physalis_kernel.cu:137-213): the first face, in the
    * order W E S N B T, that the node lies beyond; a periodic face has no
    * code, and a periodic coordinate is already inside the domain */
@@ -916,120 +869,92 @@ __device__ void node_values(const PDom &P, const PWalls &W, part_struct *Q,
   }
   const int ncode = Q->nodes[node];
   const int wall = ncode < -1;
-  /* the value a node beyond a face takes: that face's Dirichlet velocity
-   * (0 at a Neumann face) in the rest frame, when the node is out of
-   * bounds in that one direction only (src/physalis_kernel.cu:437-460) */
-  const int wi = oobi * (1-oobj) * (1-oobk) * wall;
-  const int wj = (1-oobi) * oobj * (1-oobk) * wall;
-  const int wk = (1-oobi) * (1-oobj) * oobk * wall;
+  const int fluid = (1 - outside) * (ncode == -1);
+  /* a node beyond a face in one direction only takes that face's Dirichlet
+   * velocity (0 at a Neumann face) in the rest frame
+   * (src/physalis_kernel.cu:437-460) */
+  int beyond_only[3];
+  for (int d = 0; d < 3; d++)
+    beyond_only[d] = oob[d] * (1 - oob[(d+1)%3]) * (1 - oob[(d+2)%3]) * wall;
 
-  /* pressure: cell centre (src/physalis_kernel.cu:305-364) */
-  int i = ic, j = jc, k = kc;
-  i += (1 - i) * (i < 1); j += (1 - j) * (j < 1); k += (1 - k) * (k < 1);
-  i += (P.nx - i) * (i > P.nx); j += (P.ny - j) * (j > P.ny);
-  k += (P.nz - k) * (k > P.nz);
-  double pp = node_interp(p, P.cc, i, j, k, x, y, z,
-    (i - 0.5) * P.dx + P.xs, (j - 0.5) * P.dy + P.ys,
-    (k - 0.5) * P.dz + P.zs, ddx, ddy, ddz);
-  double ocrossr2 = (oy*zp - oz*yp) * (oy*zp - oz*yp);
-  ocrossr2 += (ox*zp - oz*xp) * (ox*zp - oz*xp);
-  ocrossr2 += (ox*yp - oy*xp) * (ox*yp - oy*xp);
-  double bousiq_x = -s_beta*(Q->s - s_ref)*g.x;
-  double bousiq_y = -s_beta*(Q->s - s_ref)*g.y;
-  double bousiq_z = -s_beta*(Q->s - s_ref)*g.z;
-  double accdotr = (-gradP.x * irho_f - udot + bousiq_x)*xp +
-                   (-gradP.y * irho_f - vdot + bousiq_y)*yp +
-                   (-gradP.z * irho_f - wdot + bousiq_z)*zp;
-  pp -= 0.5 * rho_f * ocrossr2 + rho_f * accdotr;
-  pp = pp * (1 - oob) * (Q->nodes[node] == -1);
+  /* the particle's rigid frame at the node: its velocity and the pressure
+   * of the frame's centrifugal and linear accelerations */
+  const v3 om = {Q->ox, Q->oy, Q->oz};
+  const v3 acc = {Q->udot, Q->vdot, Q->wdot};
+  const v3 wxr = cross(om, rn);
+  const v3 rigid = frame_velocity(Q, rn, Q->rs, nu);
 
-  /* u: x-faces, the face index clamps to nx + 1 at ">=" as the reference */
-  i = ic; j = jc; k = kc;
-  i += (1 - i) * (i < 1); j += (1 - j) * (j < 1); k += (1 - k) * (k < 1);
-  i += (P.nx + 1 - i) * (i >= P.nx + 1); j += (P.ny - j) * (j > P.ny);
-  k += (P.nz - k) * (k > P.nz);
-  double uu = node_interp(u, P.fx, i, j, k, x, y, z,
-    (i - DOM_BUF) * P.dx + P.xs, (j - 0.5) * P.dy + P.ys,
-    (k - 0.5) * P.dz + P.zs, ddx, ddy, ddz);
-  double ocrossr_x = oy*zp - oz*yp;
-  double odotcrossr_x = oydot*zp - ozdot*yp;
-  double tmp_u = Q->u + ocrossr_x +
-                 0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_x;
-  uu -= tmp_u;
-  double uwi = (ncode == WEST_WALL_D) * W.vel[0][0]
-             + (ncode == EAST_WALL_D) * W.vel[1][0];
-  double uwj = (ncode == SOUTH_WALL_D) * W.vel[2][0]
-             + (ncode == NORTH_WALL_D) * W.vel[3][0];
-  double uwk = (ncode == BOTTOM_WALL_D) * W.vel[4][0]
-             + (ncode == TOP_WALL_D) * W.vel[5][0];
-  uwi -= tmp_u;
-  uwj -= tmp_u;
-  uwk -= tmp_u;
-  uu = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * uu
-     + wi * uwi + wj * uwj + wk * uwk;
+  /* pressure: cell centres (src/physalis_kernel.cu:305-364) */
+  {
+    const int i = clamp_index(ic, 1, P.nx), j = clamp_index(jc, 1, P.ny),
+              k = clamp_index(kc, 1, P.nz);
+    double pn = node_interp(p, P.cc, i, j, k, x, y, z,
+      (i - 0.5) * P.dx + P.xs, (j - 0.5) * P.dy + P.ys,
+      (k - 0.5) * P.dz + P.zs, ddx, ddy, ddz);
+    pn -= 0.5 * rho_f * dot(wxr, wxr)
+        + rho_f * dot(frame_accel(gradP, rho_f, s_beta, Q->s, s_ref, g) - acc,
+                      rn);
+    out->pp[node] = pn * fluid;
+  }
 
-  /* v: y-faces */
-  i = ic; j = jc; k = kc;
-  i += (1 - i) * (i < 1); j += (1 - j) * (j < 1); k += (1 - k) * (k < 1);
-  i += (P.nx - i) * (i > P.nx); j += (P.ny + 1 - j) * (j >= P.ny + 1);
-  k += (P.nz - k) * (k > P.nz);
-  double vv = node_interp(v, P.fy, i, j, k, x, y, z,
-    (i - 0.5) * P.dx + P.xs, (j - DOM_BUF) * P.dy + P.ys,
-    (k - 0.5) * P.dz + P.zs, ddx, ddy, ddz);
-  double ocrossr_y = -(ox*zp - oz*xp);
-  double odotcrossr_y = -(oxdot*zp - ozdot*xp);
-  double tmp_v = Q->v + ocrossr_y +
-                 0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_y;
-  vv -= tmp_v;
-  double vwi = (ncode == WEST_WALL_D) * W.vel[0][1]
-             + (ncode == EAST_WALL_D) * W.vel[1][1];
-  double vwj = (ncode == SOUTH_WALL_D) * W.vel[2][1]
-             + (ncode == NORTH_WALL_D) * W.vel[3][1];
-  double vwk = (ncode == BOTTOM_WALL_D) * W.vel[4][1]
-             + (ncode == TOP_WALL_D) * W.vel[5][1];
-  vwi -= tmp_v;
-  vwj -= tmp_v;
-  vwk -= tmp_v;
-  vv = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * vv
-     + wi * vwi + wj * vwj + wk * vwk;
-
-  /* w: z-faces */
-  i = ic; j = jc; k = kc;
-  i += (1 - i) * (i < 1); j += (1 - j) * (j < 1); k += (1 - k) * (k < 1);
-  i += (P.nx - i) * (i > P.nx); j += (P.ny - j) * (j > P.ny);
-  k += (P.nz + 1 - k) * (k >= P.nz + 1);
-  double ww = node_interp(w, P.fz, i, j, k, x, y, z,
-    (i - 0.5) * P.dx + P.xs, (j - 0.5) * P.dy + P.ys,
-    (k - DOM_BUF) * P.dz + P.zs, ddx, ddy, ddz);
-  double ocrossr_z = ox*yp - oy*xp;
-  double odotcrossr_z = oxdot*yp - oydot*xp;
-  double tmp_w = Q->w + ocrossr_z +
-                 0.1 * inu * (rs5 - a5) * irs3 * odotcrossr_z;
-  ww -= tmp_w;
-  double wwi = (ncode == WEST_WALL_D) * W.vel[0][2]
-             + (ncode == EAST_WALL_D) * W.vel[1][2];
-  double wwj = (ncode == SOUTH_WALL_D) * W.vel[2][2]
-             + (ncode == NORTH_WALL_D) * W.vel[3][2];
-  double wwk = (ncode == BOTTOM_WALL_D) * W.vel[4][2]
-             + (ncode == TOP_WALL_D) * W.vel[5][2];
-  wwi -= tmp_w;
-  wwj -= tmp_w;
-  wwk -= tmp_w;
-  ww = (1-oobi) * (1-oobj) * (1-oobk) * (ncode == -1) * ww
-     + wi * wwi + wj * wwj + wk * wwk;
-
-  out->pp[node] = pp;
-  cart2sphere(uu, vv, ww, th, ph, &out->ur[node], &out->ut[node],
+  /* velocity components on their face grids (the face index of the
+   * component's own direction clamps to n + 1) */
+  const double *vel[3] = {u, v, w};
+  const Grid *grid[3] = {&P.fx, &P.fy, &P.fz};
+  const double rig[3] = {rigid.x, rigid.y, rigid.z};
+  double un[3];
+  for (int d = 0; d < 3; d++) {
+    const int i = clamp_index(ic, 1, P.nx + (d == 0));
+    const int j = clamp_index(jc, 1, P.ny + (d == 1));
+    const int k = clamp_index(kc, 1, P.nz + (d == 2));
+    double val = node_interp(vel[d], *grid[d], i, j, k, x, y, z,
+      (i - (d == 0 ? DOM_BUF : 0.5)) * P.dx + P.xs,
+      (j - (d == 1 ? DOM_BUF : 0.5)) * P.dy + P.ys,
+      (k - (d == 2 ? DOM_BUF : 0.5)) * P.dz + P.zs, ddx, ddy, ddz);
+    val -= rig[d];
+    double at_wall[3];
+    for (int e = 0; e < 3; e++) {
+      at_wall[e] = (ncode == WEST_WALL_D - 10 * (2*e)) * W.vel[2*e][d]
+                 + (ncode == WEST_WALL_D - 10 * (2*e + 1)) * W.vel[2*e + 1][d];
+      at_wall[e] -= rig[d];
+    }
+    un[d] = fluid * val + beyond_only[0] * at_wall[0]
+          + beyond_only[1] * at_wall[1] + beyond_only[2] * at_wall[2];
+  }
+  cart2sphere(un[0], un[1], un[2], th, ph, &out->ur[node], &out->ut[node],
     &out->up[node]);
 }
 
-/* Legendre function of the quadrature, same forms as Pnm
- * (src/physalis_kernel.cu:55-116). */
-#define pnm Pnm
-#define nnm Nnm
+/* Cartesian vector of an order-1 harmonic coefficient set: the
+ * combination (-2 N_1^1 Re c_11, 2 N_1^1 Im c_11, N_1^0 Re c_10) that the
+ * force and torque of Lamb's solution take (coefficients 1 = (1,0) and
+ * 2 = (1,1)). */
+__device__ inline v3 order1_vector(const double *re, const double *im)
+{
+  const double N10 = sqrt(3./4./PI), N11 = sqrt(3./8./PI);
+  return {-2.*N11 * re[2], 2.*N11 * im[2], N10 * re[1]};
+}
 
-/* Quadrature (src/physalis_kernel.cu:642-722), Lamb coefficients
- * (:724-805) and forces (:807-841) for one particle per workgroup. */
+/* Per particle (one workgroup of 64 lanes): the rest-frame pressure and
+ * velocity at the 26 nodes (lanes 0..25); then one lane per coefficient
+ * (n, m) projects them on the sphere by the Lebedev rule and solves for
+ * the Lamb coefficients of order n (with under-relaxation); then lane 0
+ * forms the force and torque (src/physalis_kernel.cu:642-841).
+ * Projections of one (n, m), with Y = N_n^m P_n^m(cos t) e^{-imp}:
+ *   Pr = int p Y,
+ *   Dv = int (u_t dY/dt + u_p (1/sin t) dY/dp)   (surface divergence),
+ *   Cu = int (u_t (1/sin t) dY/dp - u_p dY/dt)   (surface curl),
+ * each as (re, im).  On the node sphere rs, with s = a/rs and q = rs/a,
+ * the order-n coefficients (p, f, c) of the pressure, phi and chi series
+ * satisfy
+ *   K11 p - K12 f = (a^2/(mu nu)) Pr,    K21 p + K22 f = (a/nu) Dv,
+ *   E c = (a/nu) Cu,
+ *   K11 = (1 - n(2n-1) s^(2n+1) / (2(n+1))) q^n,
+ *   K12 = n(2n-1)(2n+1) s^(n+1) / (n+1),
+ *   K21 = (n/4)(2(n+3)/(2n+3) + (n-2 - n(2n+1) s^2/(2n+3)) s^(2n+1)) q^(n+1),
+ *   K22 = n(n+1 + ((n-2)(2n+1) q^2 - n(2n-1)) s^(2n+1) / 2) q^(n-1),
+ *   E   = n(n+1)(s^(2n+1) - 1) q^n;
+ * order 0 has the pressure only, p = (a^2/(mu nu)) Pr. */
 __global__ void __launch_bounds__(64)
 lamb_kernel(PDom P, PWalls W, part_struct *parts,
   const double *__restrict__ p,
@@ -1038,7 +963,6 @@ lamb_kernel(PDom P, PWalls W, part_struct *parts,
   gradP_struct gradP, double relax, double s_beta, double s_ref, g_struct g)
 {
   __shared__ NodeVals N;
-  __shared__ double sums[6][NCOEFFS];
   part_struct *Q = &parts[blockIdx.x];
   int t = threadIdx.x;
   if (t < NNODES)
@@ -1046,107 +970,96 @@ lamb_kernel(PDom P, PWalls W, part_struct *parts,
                 &N);
   __syncthreads();
 
-  int ncoeff = Q->ncoeff;
-  if (t < ncoeff) {
-    int n = c_nn[t], m = c_mm[t];
-    double S[6];
+  if (t < Q->ncoeff) {
+    const int n = c_nn[t], m = c_mm[t];
+    const double Nnm = harmonic_norm(n, m);
+    double Pr[2], Dv[2], Cu[2];
     for (int node = 0; node < NNODES; node++) {
-      double theta = c_node_t[node];
-      double phi = c_node_p[node];
-      double N_nm = nnm(n, m);
-      double P_nm = pnm(n, m, theta);
-      double P_n1m = pnm(n + 1, m, theta);
-      double dPdt = (n - m + 1.)*P_n1m - (n + 1.)*cos(theta)*P_nm;
-      double dPdp = m*P_nm;
-      double isth = 1./sin(theta);
-      double cmphi = cos(m * phi);
-      double smphi = sin(m * phi);
-      double I[6];
-      I[0] = N_nm*P_nm*N.pp[node]*cmphi;
-      I[1] = -N_nm*P_nm*N.pp[node]*smphi;
-      I[2] = N_nm*isth*(dPdt * N.ut[node] * cmphi - dPdp * N.up[node] * smphi);
-      I[3] = N_nm*isth*(-dPdt * N.ut[node] * smphi - dPdp * N.up[node] * cmphi);
-      I[4] = N_nm*isth*(dPdp * N.ut[node] * smphi + dPdt * N.up[node] * cmphi);
-      I[5] = N_nm*isth*(dPdp * N.ut[node] * cmphi - dPdt * N.up[node] * smphi);
-      for (int q = 0; q < 6; q++) {
-        if (node == 0) S[q] = I[q] * c_A1;
-        else if (node < 6) S[q] += c_A1 * I[q];
-        else if (node < 18) S[q] += c_A2 * I[q];
-        else S[q] += c_A3 * I[q];
+      const double theta = c_node_t[node];
+      const double phi = c_node_p[node];
+      const double ct = cos(theta), st = sin(theta);
+      double Pl[LMAX + 2][LMAX + 2];
+      legendre_table<LMAX + 1>(ct, st, Pl);
+      const double Pn = Pl[n][m];
+      const double dPt = (n - m + 1.)*Pl[n + 1][m] - (n + 1.)*ct*Pn;
+      const double dPp = m*Pn;
+      const double ist = 1./st;
+      const double cm = cos(m * phi), sm = sin(m * phi);
+      const double pn = N.pp[node], utn = N.ut[node], upn = N.up[node];
+      const double term[6] = {
+        Nnm*Pn*pn*cm, -Nnm*Pn*pn*sm,
+        Nnm*ist*(dPt*utn*cm - dPp*upn*sm), Nnm*ist*(-dPt*utn*sm - dPp*upn*cm),
+        Nnm*ist*(dPp*utn*sm + dPt*upn*cm), Nnm*ist*(dPp*utn*cm - dPt*upn*sm)};
+      /* Lebedev weights: 6 axis nodes, 12 edge midpoints, 8 corners */
+      const double wgt = node < 6 ? c_A1 : (node < 18 ? c_A2 : c_A3);
+      double *acc[6] = {&Pr[0], &Pr[1], &Dv[0], &Dv[1], &Cu[0], &Cu[1]};
+      for (int q = 0; q < 6; q++)
+        *acc[q] = node == 0 ? term[q] * wgt : *acc[q] + wgt * term[q];
+    }
+
+    const double a = Q->r;
+    const double sP = a * a / (mu * nu);       /* scale of Pr */
+    const double sV = a / nu;                  /* scale of Dv and Cu */
+    double cp[2], cf[2], cc[2];
+    if (n == 0) {
+      for (int c = 0; c < 2; c++) {
+        cp[c] = sP * Pr[c];
+        cf[c] = 0.;
+        cc[c] = 0.;
+      }
+    } else {
+      const double s = a / Q->rs, q = Q->rs / a;
+      const double s_np1 = pow(s, n + 1.);
+      const double s_2np1 = s_np1 * s_np1 * q;
+      const double q_nm1 = pow(q, n - 1.);
+      const double q_n = q_nm1 * q, q_np1 = q_n * q;
+      const double in1 = 1./(n + 1.), i2n3 = 1./(2.*n + 3.);
+      const double K11 = (1. - 0.5*n*(2.*n - 1.)*in1*s_2np1) * q_n;
+      const double K12 = n*(2.*n - 1.)*(2.*n + 1.)*in1*s_np1;
+      const double K21 = 0.25*n*(2.*(n + 3.)*i2n3
+        + (n - 2. - n*(2.*n + 1.)*i2n3*s*s)*s_2np1)*q_np1;
+      const double K22 = n*(n + 1. + 0.5*((n - 2.)*(2.*n + 1.)*q*q
+        - n*(2.*n - 1.))*s_2np1)*q_nm1;
+      const double E = n*(n + 1.)*(s_2np1 - 1.)*q_n;
+      const double idet = 1./(K11*K22 + K12*K21);
+      for (int c = 0; c < 2; c++) {
+        const double rp = sP * Pr[c], rd = sV * Dv[c];
+        cp[c] = (rp*K22 + rd*K12) * idet;
+        cf[c] = (rd*K11 - rp*K21) * idet;
+        cc[c] = sV * Cu[c] * (1./E);
       }
     }
-    for (int q = 0; q < 6; q++) sums[q][t] = S[q];
-
-    /* Lamb coefficients with under-relaxation */
-    double ars = Q->r / Q->rs;
-    double rsa = Q->rs / Q->r;
-    double r2 = Q->r * Q->r;
-    double inu = 1./nu;
-    double imunu = inu / mu;
-    int coeff = t;
-    double Yre = S[0], Yim = S[1], rDre = S[2], rDim = S[3], xXre = S[4],
-           xXim = S[5];
-    if (n == 0) {
-      Q->pnm_re[coeff] = (1. - relax) * Q->pnm_re0[coeff] +
-                         relax * r2 * imunu * Yre;
-      Q->pnm_im[coeff] = (1. - relax) * Q->pnm_im0[coeff] +
-                         relax * r2 * imunu * Yim;
-      Q->phinm_re[coeff] = 0.;
-      Q->phinm_im[coeff] = 0.;
-      Q->chinm_re[coeff] = 0.;
-      Q->chinm_im[coeff] = 0.;
-    } else {
-      double pow_ars_np1 = pow(ars, n + 1.);
-      double pow_ars_2np1 = pow_ars_np1 * pow_ars_np1 * rsa;
-      double pow_rsa_nm1 = pow(rsa, n - 1.);
-      double pow_rsa_n = pow_rsa_nm1 * rsa;
-      double pow_rsa_np1 = pow_rsa_n * rsa;
-      double i_np1 = 1./(n + 1.);
-      double i_2np3 = 1./(2.*n + 3.);
-      double A = (1. - 0.5*n*(2.*n - 1.) * i_np1 * pow_ars_2np1) * pow_rsa_n;
-      double B = n*(2.*n - 1.)*(2.*n + 1.) * i_np1*pow_ars_np1;
-      double C = 0.25*n*(2.*(n + 3.)*i_2np3
-        + (n - 2. - n*(2.*n + 1.)*i_2np3*ars*ars)*pow_ars_2np1)*pow_rsa_np1;
-      double D = n*(n + 1. + 0.5*((n - 2.)*(2.*n + 1.)*rsa*rsa
-        - n*(2.*n - 1.))*pow_ars_2np1)*pow_rsa_nm1;
-      double idet = 1./ (A*D + B*C);
-      double pre = (r2*imunu*Yre*D + Q->r*inu*rDre*B) * idet;
-      double pim = (r2*imunu*Yim*D + Q->r*inu*rDim*B) * idet;
-      double fre = (Q->r*inu*rDre*A - r2*imunu*Yre*C) * idet;
-      double fim = (Q->r*inu*rDim*A - r2*imunu*Yim*C) * idet;
-      double E = n*(n + 1.)*(pow_ars_2np1 - 1.)*pow_rsa_n;
-      double iE = 1./ E;
-      double cre = Q->r*inu*xXre * iE;
-      double cim = Q->r*inu*xXim * iE;
-      Q->pnm_re[coeff] = Q->pnm_re0[coeff]*(1. - relax) + relax*pre;
-      Q->pnm_im[coeff] = Q->pnm_im0[coeff]*(1. - relax) + relax*pim;
-      Q->phinm_re[coeff] = Q->phinm_re0[coeff]*(1. - relax) + relax*fre;
-      Q->phinm_im[coeff] = Q->phinm_im0[coeff]*(1. - relax) + relax*fim;
-      Q->chinm_re[coeff] = Q->chinm_re0[coeff]*(1. - relax) + relax*cre;
-      Q->chinm_im[coeff] = Q->chinm_im0[coeff]*(1. - relax) + relax*cim;
-    }
+    double *now[6] = {Q->pnm_re, Q->pnm_im, Q->phinm_re, Q->phinm_im,
+                      Q->chinm_re, Q->chinm_im};
+    const double *old[6] = {Q->pnm_re0, Q->pnm_im0, Q->phinm_re0,
+                            Q->phinm_im0, Q->chinm_re0, Q->chinm_im0};
+    const double fresh[6] = {cp[0], cp[1], cf[0], cf[1], cc[0], cc[1]};
+    for (int q = 0; q < 6; q++)
+      now[q][t] = (n == 0 && q >= 2) ? 0.
+                                     : old[q][t]*(1. - relax) + relax*fresh[q];
   }
   __syncthreads();
   if (t == 0) {
-    double irho_f = 1./ rho_f;
-    double vol = 4./3. * PI *  Q->r*Q->r*Q->r;
-    double N10 = sqrt(3./4./PI);
-    double N11 = sqrt(3./8./PI);
-    double bousiq_x = -s_beta*(Q->s - s_ref)*g.x;
-    double bousiq_y = -s_beta*(Q->s - s_ref)*g.y;
-    double bousiq_z = -s_beta*(Q->s - s_ref)*g.z;
-    Q->Fx = rho_f * vol * (Q->udot + gradP.x * irho_f - bousiq_x)
-      - PI * mu * nu * 2.*N11 * (Q->pnm_re[2] + 6.*Q->phinm_re[2]);
-    Q->Fy = rho_f * vol * (Q->vdot + gradP.y * irho_f - bousiq_y)
-      + PI * mu * nu * 2.*N11 * (Q->pnm_im[2] + 6.*Q->phinm_im[2]);
-    Q->Fz = rho_f * vol * (Q->wdot + gradP.z * irho_f - bousiq_z)
-      + PI * mu * nu * N10 * (Q->pnm_re[1] + 6.*Q->phinm_re[1]);
-    Q->Lx = rho_f * vol * Q->r*Q->r * Q->oxdot
-      - 8. * PI * mu * nu * 2.*N11 * Q->r * Q->chinm_re[2];
-    Q->Ly = rho_f * vol * Q->r*Q->r * Q->oydot
-      + 8. * PI * mu * nu * 2.*N11 * Q->r * Q->chinm_im[2];
-    Q->Lz = rho_f * vol * Q->r*Q->r * Q->ozdot
-      + 8. * PI * mu * nu * N10 * Q->r * Q->chinm_re[1];
+    /* F = rho_f V (dU/dt - frame acceleration) + pi mu nu vec(p + 6 phi),
+     * L = rho_f V a^2 dw/dt + 8 pi mu nu a vec(chi) */
+    const double a = Q->r;
+    const double vol = 4./3. * PI * a*a*a;
+    double pf_re[3], pf_im[3];
+    for (int c = 1; c <= 2; c++) {
+      pf_re[c] = Q->pnm_re[c] + 6.*Q->phinm_re[c];
+      pf_im[c] = Q->pnm_im[c] + 6.*Q->phinm_im[c];
+    }
+    const v3 Fh = (PI * mu * nu) * order1_vector(pf_re, pf_im);
+    const v3 Lh = (8. * PI * mu * nu * a) * order1_vector(Q->chinm_re,
+                                                        Q->chinm_im);
+    const v3 acc = {Q->udot, Q->vdot, Q->wdot};
+    const v3 omdot = {Q->oxdot, Q->oydot, Q->ozdot};
+    const v3 F = (rho_f * vol)
+               * (acc - frame_accel(gradP, rho_f, s_beta, Q->s, s_ref, g))
+               + Fh;
+    const v3 L = (rho_f * vol * a*a) * omdot + Lh;
+    Q->Fx = F.x; Q->Fy = F.y; Q->Fz = F.z;
+    Q->Lx = L.x; Q->Ly = L.y; Q->Lz = L.z;
   }
 }
 
@@ -1252,7 +1165,7 @@ extern "C" void cuda_lamb_err(real *error, int *number)
  * particle's scalar from q.  The reference's Legendre table has its m < 0
  * cases commented out and returns 0 for them through its switch
  * (src/particle_kernel.cu:1928-1990): every m < 0 term is 0, here as
- * there (Pnm returns 0 for m < 0). */
+ * there (legendre_nm is not called for m < 0). */
 __constant__ int c_snn[S_MAX_COEFFS] = {0, 1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3,
   3, 3, 3, 3, 4, 4, 4, 4, 4, 4, 4, 4, 4};
 __constant__ int c_smm[S_MAX_COEFFS] = {0, -1, 0, 1, -2, -1, 0, 1, 2, -3, -2,
@@ -1351,8 +1264,8 @@ scalar_lamb_kernel(PDom P, PScalarBC B, part_struct *parts,
     for (int node = 0; node < NNODES; node++) {
       double theta = c_node_t[node];
       double phi = c_node_p[node];
-      double N_nm = Nnm(n, m);
-      double P_nm = m < 0 ? 0. : Pnm(n, m, theta);
+      double N_nm = harmonic_norm(n, m);
+      double P_nm = m < 0 ? 0. : legendre_nm(n, m, theta);
       double Ire = N_nm*P_nm*ss[node]*cos(m * phi);
       double Iim = -N_nm*P_nm*ss[node]*sin(m * phi);
       if (node == 0) { re = Ire * c_A1; im = Iim * c_A1; }
@@ -1446,7 +1359,7 @@ scalar_part_bc_kernel(PDom P, const part_struct *__restrict__ parts,
   y -= near_img(y, Q->y, P.yl, P.pery);
   z -= near_img(z, Q->z, P.zl, P.perz);
   double r, theta, phi;
-  xyz2rtp(x, y, z, &r, &theta, &phi);
+  to_spherical(x, y, z, &r, &theta, &phi);
   double ar = Q->r / r;
   double ra = r / Q->r;
   double v = Q->s;
@@ -1455,7 +1368,7 @@ scalar_part_bc_kernel(PDom P, const part_struct *__restrict__ parts,
     for (int jj = 0; jj < n; jj++) coeff += 2*jj + 1;
     double X = 0.;
     for (int m = -n; m <= n; m++) {
-      X += Nnm(n, m) * (m < 0 ? 0. : Pnm(n, m, theta))
+      X += harmonic_norm(n, m) * (m < 0 ? 0. : legendre_nm(n, m, theta))
         * (Q->anm_re[coeff]*cos(m*phi) - Q->anm_im[coeff]*sin(m*phi));
       coeff++;
     }
@@ -1570,12 +1483,154 @@ __global__ void bin_kernel(int nparts, const part_struct *__restrict__ parts,
   pbin[pp] = ib + jb * G.s1b + kb * G.s2b;
 }
 
-/* Lubrication and contact forces on particle i from its neighbours
- * (src/particle_kernel.cu:3075-3366).  Neighbours are visited bin by bin
- * (z, then y, then x offsets -1..1, as the reference) and, inside a bin,
- * in particle order; a bin across a periodic face holds the neighbours'
- * images (positions x +- L). */
-/* One wavefront per particle i: the lanes test 64 candidates j at a time
+/* Velocity and rotation of a particle centred on the sub-step: the mean of
+ * the current and the stored value. */
+__device__ inline v3 mid_velocity(const part_struct *Q)
+{
+  return 0.5 * (v3{Q->u, Q->v, Q->w} + v3{Q->u0, Q->v0, Q->w0});
+}
+__device__ inline v3 mid_rotation(const part_struct *Q)
+{
+  return 0.5 * (v3{Q->ox, Q->oy, Q->oz} + v3{Q->ox0, Q->oy0, Q->oz0});
+}
+
+/* ------------------------- contact bookkeeping --------------------------- */
+/* Each particle lists up to MAX_NEIGHBORS partners it is in contact with
+ * (iSt: partner number, or -10 - face for a wall; -1 = free slot) and the
+ * Stokes number of each contact at its start (St). */
+__device__ inline int contact_find(const part_struct *Q, int partner)
+{
+  int q = 0;
+  while (q < MAX_NEIGHBORS && Q->iSt[q] != partner) q++;
+  return q;
+}
+
+/* A partner outside contact range leaves the list. */
+__device__ inline void contact_forget(part_struct *Q, int partner)
+{
+  int q = contact_find(Q, partner);
+  if (q < MAX_NEIGHBORS) {
+    Q->iSt[q] = -1;
+    Q->St[q] = 0.;
+  }
+}
+
+/* Slot of a partner in contact; a new contact takes the first free slot
+ * (the last one when all are taken), records St = (rho_p/rho_f)(2a)|U_n|/(9
+ * nu) and counts one more collision in *count. */
+__device__ inline int contact_slot(part_struct *Q, int partner, double Un,
+  double rhof, double nu, int *count)
+{
+  int q = contact_find(Q, partner);
+  if (q == MAX_NEIGHBORS) {
+    q = 0;
+    while (q < MAX_NEIGHBORS - 1 && Q->iSt[q] != -1) q++;
+    Q->iSt[q] = partner;
+    Q->St[q] = 1./9.*Q->rho/rhof*2.*Q->r*fabs(Un)/nu;
+    *count += 1;
+  }
+  return q;
+}
+
+/* Coefficient of restitution of a wet contact from the dry one and the
+ * contact's Stokes number, e = e_dry + (1 + e_dry) ln(x_c/x_0) / St with
+ * x_c/x_0 = 1e-4 (clipped at 0), and the damping factor of the soft-sphere
+ * dashpot fitted to it, alpha(e) = 2.22 - 2.263 e^0.3948. */
+__device__ inline double dashpot_alpha(double e_dry, double St)
+{
+  double e = e_dry + (1.+e_dry)/St*log(1.e-4);
+  if (e < 0) e = 0;
+  return -2.263*pow(e,0.3948)+2.22;
+}
+
+struct PairForce { v3 F, L; };
+
+/* Near-contact lubrication on sphere i (radius a) from sphere j (radius
+ * beta a), surfaces h apart with 0 < h < h_N: the leading terms of the
+ * two-sphere resistance functions (Jeffrey & Onishi 1984; Kim & Karrila
+ * 1991, ch. 9) for the relative velocity U = U_n n + U_t of i against j and
+ * the summed rotation w of the pair,
+ *   F = -6 pi mu a (X1 g1 + X2 g2) U_n - 6 pi mu a Y1 g2 U_t
+ *       + 8 pi mu a^2 Y2 g2 (w x n),
+ *   L = -8 pi mu a^2 Y2 g2 (U_t x n) - 8 pi mu a^3 Y3 g2 w,
+ * X1 = beta^2/(1+beta)^2, X2 = beta(1+7beta+beta^2)/(5(1+beta)^3),
+ * Y1 = 4beta(2+beta+2beta^2)/(15(1+beta)^3), Y2 = beta(4+beta)/(10(1+beta)^2),
+ * Y3 = 2beta/(5(1+beta)).  The singular gap functions are measured from the
+ * cut-off, g1 = a/h - a/h_N and g2 = ln(h_N/h), so the force vanishes
+ * continuously at h_N; h is floored at eps a. */
+__device__ inline PairForce pair_lubrication(double a, double beta, double h,
+  double hN, double eps, double mu, v3 n, v3 Un, v3 Ut, v3 w)
+{
+  if (h < eps*a) h = eps*a;
+  const double g1 = a/h - a/hN;
+  const double g2 = log(hN/h);
+  const double b1 = 1. + beta;
+  const double X1 = beta*beta / (b1*b1);
+  const double X2 = beta*(1.+7.*beta+beta*beta) / (5.*b1*b1*b1);
+  const double Y1 = 4.*beta*(2.+beta+2.*beta*beta) / (15.*b1*b1*b1);
+  const double Y2 = beta*(4.+beta) / (10.*b1*b1);
+  const double Y3 = 2.*beta / (5.*b1);
+  const double stokes = 6.*PI*mu*a;           /* translational drag */
+  const double rotlet = 8.*PI*mu*a*a;         /* rotational coupling */
+  PairForce out;
+  out.F = (-stokes*(X1*g1 + X2*g2)) * Un + (-stokes*Y1*g2) * Ut
+        + (rotlet*Y2*g2) * cross(w, n);
+  out.L = (-rotlet*Y2*g2) * cross(Ut, n) + (-rotlet*a*Y3*g2) * w;
+  return out;
+}
+
+/* Soft-sphere contact on sphere i (radius a_i, density rho_i, Young's
+ * modulus E_i, Poisson ratio s_i) from sphere j, overlapping by
+ * delta = -h > 0:
+ *  - normal: a Hertz spring k delta^(3/2) with k = (4/3) E* sqrt(R*),
+ *    1/E* = (1-s_i^2)/E_i + (1-s_j^2)/E_j, 1/R* = 1/a_i + 1/a_j, and a
+ *    dashpot -eta U.n with eta = alpha sqrt(m_i k delta^(1/2)),
+ *    m_i = (4/3) pi a_i^3 rho_i (alpha from dashpot_alpha);
+ *  - tangential: a Mindlin spring k_t = 8 G* sqrt(a_i delta) acting on the
+ *    tangential slip V_t dt of the contact point, V = -U_t + ((a_i+a_j+h)/2)
+ *    (w x n), capped at Coulomb friction cf |F_n| (cf the mean of the two
+ *    friction coefficients); G* is formed from sphere i's shear modulus
+ *    G_i = E_i/(2(1+s_i)) for both surfaces, 1/G* = 2(1-s_i^2)/G_i, and
+ *    R from a_i alone, as the reference forms it;
+ *  - torque: (a_i + h/2) n x F about the centre of i. */
+__device__ inline PairForce pair_contact(const part_struct *Pi,
+  const part_struct *Pj, double h, double alpha, double dt, v3 n, double un,
+  v3 Ut, v3 w)
+{
+  const double ai = Pi->r, aj = Pj->r;
+  const double delta = -h;
+  const double Ei = (1.-Pi->sigma*Pi->sigma)/Pi->E;
+  const double Ej = (1.-Pj->sigma*Pj->sigma)/Pj->E;
+  const double k = 4./3./(Ei + Ej)/sqrt(1./ai + 1./aj);
+  const double mass = 4./3.*PI*ai*ai*ai*Pi->rho;
+  const double eta = alpha*sqrt(mass*k*sqrt(delta));
+  const double Fn_mag = sqrt(delta*delta*delta)*k - eta*un;
+  const v3 Fn = Fn_mag * n;
+
+  const double Gi = 0.5*Pi->E/(1.+Pi->sigma);
+  const double Gs = (1.-Pi->sigma*Pi->sigma)/Gi;
+  const double kt = 8./(Gs + Gs)/sqrt(1./ai)*sqrt(delta);
+  const v3 V = (0.5*(ai + aj + h)) * cross(w, n) - Ut;
+  const v3 slip = dt * (V - dot(V, n) * n);
+  v3 Ft = (-kt) * slip;
+  Ft = Ft - dot(Ft, n) * n;
+  const double cf = 0.5 * (Pi->coeff_fric + Pj->coeff_fric);
+  const double Ft_mag = norm(Ft);
+  const double Fn_abs = norm(Fn);
+  if (Ft_mag > cf * Fn_abs) Ft = (cf * Fn_abs / Ft_mag) * Ft;
+
+  PairForce out;
+  out.F = Fn + Ft;
+  out.L = (ai + 0.5*h) * cross(n, out.F);
+  return out;
+}
+
+/* Lubrication and contact forces on particle i from its neighbours, the
+ * physics of src/particle_kernel.cu:3075-3366 (forces above).  Neighbours
+ * are visited bin by bin (z, then y, then x offsets -1..1, as the
+ * reference) and, inside a bin, in particle order; a bin across a periodic
+ * face holds the neighbours' images (positions x +- L).
+ * One wavefront per particle i: the lanes test 64 candidates j at a time
  * for membership of the target bin, and lane 0 applies the matches in
  * increasing j, so the forces add up in the order of a scan over j. */
 __global__ void __launch_bounds__(WAVE)
@@ -1594,6 +1649,11 @@ collision_parts_kernel(int nparts, part_struct *parts,
   if (!(ib >= G._is && ib <= G._ie && jb >= G._js && jb <= G._je &&
         kb >= G._ks && kb <= G._ke)) return;
   part_struct *Pi = &parts[i];
+  const v3 xi = {Pi->x, Pi->y, Pi->z};
+  const v3 ui = mid_velocity(Pi), wi = mid_rotation(Pi);
+  const double hN = interaction_length_ratio * Pi->r;
+  v3 F = {Pi->iFx, Pi->iFy, Pi->iFz};
+  v3 L = {Pi->iLx, Pi->iLy, Pi->iLz};
   for (int n = -1; n <= 1; n++)
   for (int m = -1; m <= 1; m++)
   for (int l = -1; l <= 1; l++) {
@@ -1620,193 +1680,72 @@ collision_parts_kernel(int nparts, part_struct *parts,
       unsigned long long hits =
         __ballot(jl < nparts && pbin[jl] == tbin && jl != i);
       while (hits) {
-      int j = j0 + __ffsll((unsigned long long) hits) - 1;
-      hits &= hits - 1;
-      if (lane != 0) continue;
-      const part_struct *Pj = &parts[j];
-      double xj = img(Pj->x, sh[0], xl), yj = img(Pj->y, sh[1], yl),
-             zj = img(Pj->z, sh[2], zl);
-      double ai = Pi->r;
-      double aj = Pj->r;
-      double Bq = aj / ai;
-      double hN = interaction_length_ratio * Pi->r;
-      double ux, uy, uz, rx, ry, rz, r, h, ah, lnah, nx, ny, nz, udotn;
-      double unx, uny, unz, utx, uty, utz, ut, tx, ty, tz, t, bx, by, bz, b;
-      double omegax, omegay, omegaz, omega;
-      double ocrossnx, ocrossny, ocrossnz, utcrossnx, utcrossny, utcrossnz;
-      double opB, Fnx, Fny, Fnz, Ftx, Fty, Ftz, Lox, Loy, Loz;
-      int q;
-      rx = Pi->x - xj;
-      ry = Pi->y - yj;
-      rz = Pi->z - zj;
-      ux = 0.5*((Pi->u - Pj->u) + (Pi->u0 - Pj->u0));
-      uy = 0.5*((Pi->v - Pj->v) + (Pi->v0 - Pj->v0));
-      uz = 0.5*((Pi->w - Pj->w) + (Pi->w0 - Pj->w0));
-      r = sqrt(rx*rx + ry*ry + rz*rz);
-      omegax = 0.5*((Pi->ox + Pj->ox) + (Pi->ox0 + Pj->ox0));
-      omegay = 0.5*((Pi->oy + Pj->oy) + (Pi->oy0 + Pj->oy0));
-      omegaz = 0.5*((Pi->oz + Pj->oz) + (Pi->oz0 + Pj->oz0));
-      omega = sqrt(omegax*omegax + omegay*omegay + omegaz*omegaz);
-      h = r - ai - aj;
-      nx = rx / r;
-      ny = ry / r;
-      nz = rz / r;
-      udotn = ux * nx + uy * ny + uz * nz;
-      unx = udotn * nx;
-      uny = udotn * ny;
-      unz = udotn * nz;
-      utx = ux - unx;
-      uty = uy - uny;
-      utz = uz - unz;
-      ut = sqrt(utx*utx + uty*uty + utz*utz);
-      if (ut > 0) {
-        tx = utx / ut; ty = uty / ut; tz = utz / ut;
-        bx = ny*tz - nz*ty; by = -nx*tz + nz*tx; bz = nx*ty - ny*tx;
-        b = sqrt(bx*bx + by*by + bz*bz);
-        bx = bx / b; by = by / b; bz = bz / b;
-      } else if (omega > 0) {
-        bx = omegax / omega; by = omegay / omega; bz = omegaz / omega;
-        tx = by*nz - bz*ny; ty = -bx*nz + bz*nx; tz = bx*ny - by*nx;
-        t = sqrt(tx*tx + ty*ty + tz*tz);
-        tx = tx / t; ty = ty / t; tz = tz / t;
-      } else {
-        tx = 1.; ty = 0.; tz = 0.;
-        bx = ny*tz - nz*ty; by = -nx*tz + nz*tx; bz = nx*ty - ny*tx;
-        b = sqrt(bx*bx + by*by + bz*bz);
-        bx = bx / b; by = by / b; bz = bz / b;
-      }
-      (void) bx; (void) by; (void) bz; (void) tx; (void) ty; (void) tz;
-      opB = 1 + Bq;
-      ocrossnx = omegay*nz - omegaz*ny;
-      ocrossny = -omegax*nz + omegaz*nx;
-      ocrossnz = omegax*ny - omegay*nx;
-      utcrossnx = uty*nz - utz*ny;
-      utcrossny = -utx*nz + utz*nx;
-      utcrossnz = utx*ny - uty*nx;
-      if (h < hN && h > 0) {
-        q = 0;
-        while (q < MAX_NEIGHBORS && Pi->iSt[q] != Pj->N) q++;
-        if (q < MAX_NEIGHBORS && Pi->iSt[q] == Pj->N) {
-          Pi->iSt[q] = -1;
-          Pi->St[q] = 0.;
+        int j = j0 + __ffsll((unsigned long long) hits) - 1;
+        hits &= hits - 1;
+        if (lane != 0) continue;
+        const part_struct *Pj = &parts[j];
+        const v3 xj = {img(Pj->x, sh[0], xl), img(Pj->y, sh[1], yl),
+                       img(Pj->z, sh[2], zl)};
+        const v3 sep = xi - xj;
+        const double dist = norm(sep);
+        const v3 nrm = (1. / dist) * sep;
+        const double h = dist - Pi->r - Pj->r;
+        const v3 U = ui - mid_velocity(Pj);
+        const v3 w = wi + mid_rotation(Pj);
+        const double un = dot(U, nrm);
+        const v3 Un = un * nrm;
+        const v3 Ut = U - Un;
+        if (h < hN && h > 0) {
+          contact_forget(Pi, Pj->N);
+          PairForce f = pair_lubrication(Pi->r, Pj->r / Pi->r, h, hN, eps,
+                                         mu, nrm, Un, Ut, w);
+          F = F + f.F;
+          L = L + f.L;
         }
-        if (h < eps*Pi->r) h = eps*Pi->r;
-        ah = ai/h - ai/hN;
-        lnah = log(hN/h);
-        Fnx = -1. * Bq*Bq / (opB*opB) * ah
-          - Bq*(1.+7.*Bq+Bq*Bq)/(5.*opB*opB*opB)*lnah;
-        Fny = Fnx;
-        Fnz = Fnx;
-        Fnx *= 6.*PI*mu*ai*unx;
-        Fny *= 6.*PI*mu*ai*uny;
-        Fnz *= 6.*PI*mu*ai*unz;
-        Ftx = -6.*PI*mu*ai*utx*4.*Bq*(2.+Bq+2.*Bq*Bq)/(15.*opB*opB*opB)*lnah;
-        Fty = -6.*PI*mu*ai*uty*4.*Bq*(2.+Bq+2.*Bq*Bq)/(15.*opB*opB*opB)*lnah;
-        Ftz = -6.*PI*mu*ai*utz*4.*Bq*(2.+Bq+2.*Bq*Bq)/(15.*opB*opB*opB)*lnah;
-        Ftx += 8.*PI*mu*ai*ai*ocrossnx*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Fty += 8.*PI*mu*ai*ai*ocrossny*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Ftz += 8.*PI*mu*ai*ai*ocrossnz*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Lox = -8.*PI*mu*ai*ai*utcrossnx*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Loy = -8.*PI*mu*ai*ai*utcrossny*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Loz = -8.*PI*mu*ai*ai*utcrossnz*Bq*(4.+Bq)/(10.*opB*opB)*lnah;
-        Lox += -8.*PI*mu*ai*ai*ai*omegax*2.*Bq/(5.*opB)*lnah;
-        Loy += -8.*PI*mu*ai*ai*ai*omegay*2.*Bq/(5.*opB)*lnah;
-        Loz += -8.*PI*mu*ai*ai*ai*omegaz*2.*Bq/(5.*opB)*lnah;
-      } else {
-        ah = 0; lnah = 0;
-        Fnx = Fny = Fnz = 0;
-        Ftx = Fty = Ftz = 0;
-        Lox = Loy = Loz = 0;
-      }
-      if (h < 0) {
-        q = 0;
-        while (q < MAX_NEIGHBORS && Pi->iSt[q] != Pj->N) q++;
-        if (q == MAX_NEIGHBORS) {
-          q = 0;
-          while (q < MAX_NEIGHBORS - 1 && Pi->iSt[q] != -1) q++;
-          Pi->iSt[q] = Pj->N;
-          Pi->St[q] = 1./9.*Pi->rho/rhof*2.*Pi->r*fabs(udotn)/nu;
-          Pi->ncoll_part += 1;
+        if (h < 0) {
+          int q = contact_slot(Pi, Pj->N, un, rhof, nu, &Pi->ncoll_part);
+          double alpha = dashpot_alpha(Pi->e_dry, Pi->St[q]);
+          PairForce f = pair_contact(Pi, Pj, h, alpha, dt, nrm, un, Ut, w);
+          F = F + f.F;
+          L = L + f.L;
         }
-        double Vx = -utx + 0.5*(ai + aj + h)*ocrossnx;
-        double Vy = -uty + 0.5*(ai + aj + h)*ocrossny;
-        double Vz = -utz + 0.5*(ai + aj + h)*ocrossnz;
-        double Hi = 0.5*Pi->E/(1.+Pi->sigma);
-        double kt = 8./((1.-Pi->sigma*Pi->sigma)/Hi
-          +(1.-Pi->sigma*Pi->sigma)/Hi)/sqrt(1./ai)*sqrt(-h);
-        double Vdotn = Vx*nx + Vy*ny + Vz*nz;
-        double sx = (Vx - Vdotn * nx) * dt;
-        double sy = (Vy - Vdotn * ny) * dt;
-        double sz = (Vz - Vdotn * nz) * dt;
-        ah = 0;
-        lnah = 0;
-        double k = 4./3./((1.-Pi->sigma*Pi->sigma)/Pi->E
-          + (1.-Pj->sigma*Pj->sigma)/Pj->E)/sqrt(1./ai + 1./aj);
-        double xcx0 = 1.e-4;
-        double e = Pi->e_dry + (1.+Pi->e_dry)/Pi->St[q]*log(xcx0);
-        if (e < 0) e = 0;
-        double alpha = -2.263*pow(e,0.3948)+2.22;
-        double eta = alpha*sqrt(4./3.*PI*ai*ai*ai*Pi->rho*k*sqrt(-h));
-        Fnx = (sqrt(-h*h*h)*k - eta*udotn)*nx;
-        Fny = (sqrt(-h*h*h)*k - eta*udotn)*ny;
-        Fnz = (sqrt(-h*h*h)*k - eta*udotn)*nz;
-        double coeff_fric = 0.5 * (Pi->coeff_fric + Pj->coeff_fric);
-        Ftx = -kt * sx;
-        Fty = -kt * sy;
-        Ftz = -kt * sz;
-        double Ftdotn = Ftx*nx + Fty*ny + Ftz*nz;
-        Ftx = Ftx - Ftdotn * nx;
-        Fty = Fty - Ftdotn * ny;
-        Ftz = Ftz - Ftdotn * nz;
-        double Ft = sqrt(Ftx*Ftx + Fty*Fty + Ftz*Ftz);
-        double Fn = sqrt(Fnx*Fnx + Fny*Fny + Fnz*Fnz);
-        if (Ft > coeff_fric * Fn) {
-          Ftx = coeff_fric * Fn * Ftx / Ft;
-          Fty = coeff_fric * Fn * Fty / Ft;
-          Ftz = coeff_fric * Fn * Ftz / Ft;
-        }
-        Lox = -(ai+0.5*h)*((Fny+Fty)*nz-(Fnz+Ftz)*ny);
-        Loy =  (ai+0.5*h)*((Fnx+Ftx)*nz-(Fnz+Ftz)*nx);
-        Loz = -(ai+0.5*h)*((Fnx+Ftx)*ny-(Fny+Fty)*nx);
-      }
-      (void) ah; (void) lnah;
-      Pi->iFx += Fnx + Ftx;
-      Pi->iFy += Fny + Fty;
-      Pi->iFz += Fnz + Ftz;
-      Pi->iLx += Lox;
-      Pi->iLy += Loy;
-      Pi->iLz += Loz;
       }
     }
   }
+  if (lane == 0) {                     /* the lane that summed the pairs */
+    Pi->iFx = F.x; Pi->iFy = F.y; Pi->iFz = F.z;
+    Pi->iLx = L.x; Pi->iLy = L.y; Pi->iLz = L.z;
+  }
 }
 
-/* Spring force toward an anchor (src/particle_kernel.cu:2303-2345). */
+/* Linear spring of stiffness k and rest length l between the particle
+ * centre x (first wrapped back into the domain) and its anchor A:
+ * kF = -k (d - l d/|d|) with d = x - A; a component of d below 1e-16 in
+ * magnitude takes no rest-length part (the spring of
+ * src/particle_kernel.cu:2303-2345). */
 __global__ void spring_kernel(int nparts, part_struct *parts, dom_struct D)
 {
   int i = threadIdx.x + blockIdx.x * blockDim.x;
   if (i >= nparts || !(parts[i].spring_k > 0.)) return;
   part_struct *Q = &parts[i];
-  double x = Q->x, y = Q->y, z = Q->z;
-  x += D.xl * ((x < D.xs) - (x > D.xe));
-  y += D.yl * ((y < D.ys) - (y > D.ye));
-  z += D.zl * ((z < D.zs) - (z > D.ze));
-  double nx = x - Q->spring_x;
-  double ny = y - Q->spring_y;
-  double nz = z - Q->spring_z;
-  double n = sqrt(nx*nx+ny*ny+nz*nz);
-  double lx = Q->spring_l * (nx / n);
-  double ly = Q->spring_l * (ny / n);
-  double lz = Q->spring_l * (nz / n);
-  lx *= (nx*nx > 1e-16*1e-16);
-  ly *= (ny*ny > 1e-16*1e-16);
-  lz *= (nz*nz > 1e-16*1e-16);
-  double dx = x - Q->spring_x - lx;
-  double dy = y - Q->spring_y - ly;
-  double dz = z - Q->spring_z - lz;
-  Q->kFx = - Q->spring_k * dx;
-  Q->kFy = - Q->spring_k * dy;
-  Q->kFz = - Q->spring_k * dz;
+  const v3 lo = {D.xs, D.ys, D.zs}, hi = {D.xe, D.ye, D.ze},
+           len = {D.xl, D.yl, D.zl};
+  const double xc[3] = {Q->x, Q->y, Q->z};
+  const double l3[3] = {len.x, len.y, len.z}, lo3[3] = {lo.x, lo.y, lo.z},
+               hi3[3] = {hi.x, hi.y, hi.z};
+  double xw[3];
+  for (int c = 0; c < 3; c++)
+    xw[c] = xc[c] + l3[c] * ((xc[c] < lo3[c]) - (xc[c] > hi3[c]));
+  const v3 d = v3{xw[0], xw[1], xw[2]}
+             - v3{Q->spring_x, Q->spring_y, Q->spring_z};
+  const v3 dir = (1. / norm(d)) * d;
+  const double dd[3] = {d.x, d.y, d.z}, u3[3] = {dir.x, dir.y, dir.z};
+  double F[3];
+  for (int c = 0; c < 3; c++) {
+    const double rest = (dd[c]*dd[c] > 1e-16*1e-16) ? Q->spring_l * u3[c] : 0.;
+    F[c] = -Q->spring_k * (dd[c] - rest);
+  }
+  Q->kFx = F[0]; Q->kFy = F[1]; Q->kFz = F[2];
 }
 
 /* Lubrication and contact forces between particle i and the six domain
@@ -1865,12 +1804,7 @@ __global__ void collision_walls_kernel(int nparts, part_struct *parts,
     const int xk[3] = {CROSS_K[a][0], CROSS_K[a][1], CROSS_K[a][2]};
     const int xs[3] = {CROSS_S[a][0], CROSS_S[a][1], CROSS_S[a][2]};
     if (side && h < hN && h > 0) {
-      int q = 0;
-      while (q < MAX_NEIGHBORS && Q->iSt[q] != code) q++;
-      if (q < MAX_NEIGHBORS && Q->iSt[q] == code) {
-        Q->iSt[q] = -1;
-        Q->St[q] = 0.;
-      }
+      contact_forget(Q, code);
       if (h < eps*Q->r) h = eps*Q->r;
       double ah = ai/h - ai/hN;
       double lnah = log(hN/h);
@@ -1898,15 +1832,7 @@ __global__ void collision_walls_kernel(int nparts, part_struct *parts,
       double Ut[3];
       Ut[t1] = 0.5*(vel[t1] + vel0[t1]) - W.vel[2][t1];
       Ut[t2] = 0.5*(vel[t2] + vel0[t2]) - W.vel[2][t2];
-      int q = 0;
-      while (q < MAX_NEIGHBORS && Q->iSt[q] != code) q++;
-      if (q == MAX_NEIGHBORS) {
-        q = 0;
-        while (q < MAX_NEIGHBORS - 1 && Q->iSt[q] != -1) q++;
-        Q->iSt[q] = code;
-        Q->St[q] = 1./9.*Q->rho/rhof*2.*Q->r*fabs(Un)/nu;
-        Q->ncoll_wall += 1;
-      }
+      int q = contact_slot(Q, code, Un, rhof, nu, &Q->ncoll_wall);
       double omh[3];
       for (int c = 0; c < 3; c++) omh[c] = 0.5*(om[c] + om0[c]);
       /* contact point velocity sg (Ut + (a + h/2) e_a x w) */
@@ -1921,9 +1847,7 @@ __global__ void collision_walls_kernel(int nparts, part_struct *parts,
         +(1.-Q->sigma*Q->sigma)/Hi)/sqrt(1./ai)*sqrt(-h);
       double k = 4./3./((1.-Q->sigma*Q->sigma)/Q->E
         + (1.-Q->sigma*Q->sigma)/Q->E)/sqrt(1./ai);
-      double e = Q->e_dry + (1.+Q->e_dry)/Q->St[q]*log(1.e-4);
-      if (e < 0) e = 0;
-      double alpha = -2.263*pow(e,0.3948)+2.22;
+      double alpha = dashpot_alpha(Q->e_dry, Q->St[q]);
       double eta = alpha*sqrt(4./3.*PI*ai*ai*ai*Q->rho*k*sqrt(-h));
       double Ks = f == 0 ? k : kt;
       double Ft[3];
@@ -1950,50 +1874,61 @@ __global__ void collision_walls_kernel(int nparts, part_struct *parts,
   Q->iLx = iL[0]; Q->iLy = iL[1]; Q->iLz = iL[2];
 }
 
-/* Accelerations and velocities (src/particle_kernel.cu:3368-3410). */
+/* Linear and angular acceleration from the hydrodynamic (F, L), spring
+ * (kF), interaction (iF, iL) and applied (aF, aL) loads, the mean pressure
+ * gradient's buoyancy on the particle volume and reduced gravity; velocities
+ * advanced by the trapezoidal rule over the stored accelerations
+ * (the motion of src/particle_kernel.cu:3368-3410). */
 __global__ void move_parts_a_kernel(int nparts, part_struct *parts, double dt,
   g_struct g, gradP_struct gradP, double rho_f)
 {
   int pp = threadIdx.x + blockIdx.x * blockDim.x;
   if (pp >= nparts) return;
   part_struct *Q = &parts[pp];
-  double vol = 4./3. * PI * Q->r*Q->r*Q->r;
-  double m = vol * Q->rho;
+  const double vol = 4./3. * PI * Q->r*Q->r*Q->r;
+  const double mass = vol * Q->rho;
   if (Q->translating) {
-    Q->udot = (Q->Fx + Q->kFx + Q->iFx + Q->aFx - vol*gradP.x) / m
-      + (Q->rho - rho_f) / Q->rho * g.x;
-    Q->vdot = (Q->Fy + Q->kFy + Q->iFy + Q->aFy - vol*gradP.y) / m
-      + (Q->rho - rho_f) / Q->rho * g.y;
-    Q->wdot = (Q->Fz + Q->kFz + Q->iFz + Q->aFz - vol*gradP.z) / m
-      + (Q->rho - rho_f) / Q->rho * g.z;
-    Q->u = Q->u0 + 0.5*dt*(Q->udot + Q->udot0);
-    Q->v = Q->v0 + 0.5*dt*(Q->vdot + Q->vdot0);
-    Q->w = Q->w0 + 0.5*dt*(Q->wdot + Q->wdot0);
+    const double g_red = (Q->rho - rho_f) / Q->rho;
+    const v3 load = v3{Q->Fx, Q->Fy, Q->Fz} + v3{Q->kFx, Q->kFy, Q->kFz}
+                  + v3{Q->iFx, Q->iFy, Q->iFz} + v3{Q->aFx, Q->aFy, Q->aFz}
+                  - vol * v3{gradP.x, gradP.y, gradP.z};
+    const v3 acc = (1. / mass) * load + g_red * v3{g.x, g.y, g.z};
+    const v3 acc0 = {Q->udot0, Q->vdot0, Q->wdot0};
+    const v3 vel = v3{Q->u0, Q->v0, Q->w0} + (0.5*dt) * (acc + acc0);
+    Q->udot = acc.x; Q->vdot = acc.y; Q->wdot = acc.z;
+    Q->u = vel.x; Q->v = vel.y; Q->w = vel.z;
   }
   if (Q->rotating) {
-    double I = 0.4 * m * Q->r*Q->r;
-    Q->oxdot = (Q->Lx + Q->iLx + Q->aLx) / I;
-    Q->oydot = (Q->Ly + Q->iLy + Q->aLy) / I;
-    Q->ozdot = (Q->Lz + Q->iLz + Q->aLz) / I;
-    Q->ox = Q->ox0 + 0.5*dt*(Q->oxdot + Q->oxdot0);
-    Q->oy = Q->oy0 + 0.5*dt*(Q->oydot + Q->oydot0);
-    Q->oz = Q->oz0 + 0.5*dt*(Q->ozdot + Q->ozdot0);
+    const double inertia = 0.4 * mass * Q->r*Q->r;
+    const v3 torque = v3{Q->Lx, Q->Ly, Q->Lz} + v3{Q->iLx, Q->iLy, Q->iLz}
+                    + v3{Q->aLx, Q->aLy, Q->aLz};
+    const v3 alpha = (1. / inertia) * torque;
+    const v3 alpha0 = {Q->oxdot0, Q->oydot0, Q->ozdot0};
+    const v3 om = v3{Q->ox0, Q->oy0, Q->oz0} + (0.5*dt) * (alpha + alpha0);
+    Q->oxdot = alpha.x; Q->oydot = alpha.y; Q->ozdot = alpha.z;
+    Q->ox = om.x; Q->oy = om.y; Q->oz = om.z;
   }
 }
 
-__device__ inline void rotate(double qr, double qi, double qj, double qk,
-  double *pi, double *pj, double *pk)
+/* Unit quaternion s + v (v = (x, y, z)). */
+struct quat { double s; v3 v; };
+
+__device__ inline quat qmul(quat a, quat b)
 {
-  double Pr = *pi*qi + *pj*qj + *pk*qk;
-  double Pi = *pi*qr - *pj*qk + *pk*qj;
-  double Pj = *pi*qk + *pj*qr - *pk*qi;
-  double Pk = -*pi*qj + *pj*qi + *pk*qr;
-  *pi = qr*Pi + qi*Pr + qj*Pk - qk*Pj;
-  *pj = qr*Pj - qi*Pk + qj*Pr + qk*Pi;
-  *pk = qr*Pk + qi*Pj - qj*Pi + qk*Pr;
+  return {a.s*b.s - dot(a.v, b.v), a.s*b.v + b.s*a.v + cross(a.v, b.v)};
 }
 
-/* Positions, body axes, stored *0 fields (src/particle_kernel.cu:3412-3470),
+/* Vector p turned by the rotation q: the vector part of q (0, p) q*. */
+__device__ inline v3 qrotate(quat q, v3 p)
+{
+  quat qc = {q.s, -1. * q.v};
+  return qmul(qmul(q, quat{0., p}), qc).v;
+}
+
+/* Centre advanced by the trapezoidal rule; body axes turned through the
+ * angle |w| dt about the axis of the mean rotation rate w = (w + w0)/2
+ * (|w| taken as the mean of the two magnitudes); current velocities and
+ * accelerations stored as the old ones (src/particle_kernel.cu:3412-3470);
  * then the periodic wrap of a centre that left the domain (the reference's
  * particle transfer, src/particle_kernel.cu:1166-1210). */
 __global__ void move_parts_b_kernel(int nparts, part_struct *parts, double dt,
@@ -2003,30 +1938,26 @@ __global__ void move_parts_b_kernel(int nparts, part_struct *parts, double dt,
   if (pp >= nparts) return;
   part_struct *Q = &parts[pp];
   if (Q->translating) {
-    Q->x += 0.5*dt*(Q->u + Q->u0);
-    Q->y += 0.5*dt*(Q->v + Q->v0);
-    Q->z += 0.5*dt*(Q->w + Q->w0);
-    Q->u0 = Q->u; Q->v0 = Q->v; Q->w0 = Q->w;
+    const v3 vel = {Q->u, Q->v, Q->w}, vel0 = {Q->u0, Q->v0, Q->w0};
+    const v3 x = v3{Q->x, Q->y, Q->z} + (0.5*dt) * (vel + vel0);
+    Q->x = x.x; Q->y = x.y; Q->z = x.z;
+    Q->u0 = vel.x; Q->v0 = vel.y; Q->w0 = vel.z;
     Q->udot0 = Q->udot; Q->vdot0 = Q->vdot; Q->wdot0 = Q->wdot;
   }
   if (Q->rotating) {
-    double mag = 0.5*sqrt(Q->ox*Q->ox + Q->oy*Q->oy + Q->oz*Q->oz);
-    mag += 0.5*sqrt(Q->ox0*Q->ox0 + Q->oy0*Q->oy0 + Q->oz0*Q->oz0);
-    double X = 0, Y = 0, Z = 0;
-    if (mag > 0) {
-      X = 0.5 * (Q->ox + Q->ox0) / mag;
-      Y = 0.5 * (Q->oy + Q->oy0) / mag;
-      Z = 0.5 * (Q->oz + Q->oz0) / mag;
-    }
-    double theta = mag * dt;
-    double qr = cos(0.5*theta);
-    double qi = X * sin(0.5*theta);
-    double qj = Y * sin(0.5*theta);
-    double qk = Z * sin(0.5*theta);
-    rotate(qr, qi, qj, qk, &Q->axx, &Q->axy, &Q->axz);
-    rotate(qr, qi, qj, qk, &Q->ayx, &Q->ayy, &Q->ayz);
-    rotate(qr, qi, qj, qk, &Q->azx, &Q->azy, &Q->azz);
-    Q->ox0 = Q->ox; Q->oy0 = Q->oy; Q->oz0 = Q->oz;
+    const v3 om = {Q->ox, Q->oy, Q->oz}, om0 = {Q->ox0, Q->oy0, Q->oz0};
+    const double rate = 0.5 * norm(om) + 0.5 * norm(om0);
+    v3 axis = {0., 0., 0.};
+    if (rate > 0) axis = (0.5 / rate) * (om + om0);
+    const double half = 0.5 * (rate * dt);
+    const quat q = {cos(half), sin(half) * axis};
+    const v3 ex = qrotate(q, v3{Q->axx, Q->axy, Q->axz});
+    const v3 ey = qrotate(q, v3{Q->ayx, Q->ayy, Q->ayz});
+    const v3 ez = qrotate(q, v3{Q->azx, Q->azy, Q->azz});
+    Q->axx = ex.x; Q->axy = ex.y; Q->axz = ex.z;
+    Q->ayx = ey.x; Q->ayy = ey.y; Q->ayz = ey.z;
+    Q->azx = ez.x; Q->azy = ez.y; Q->azz = ez.z;
+    Q->ox0 = om.x; Q->oy0 = om.y; Q->oz0 = om.z;
     Q->oxdot0 = Q->oxdot; Q->oydot0 = Q->oydot; Q->ozdot0 = Q->ozdot;
   }
   if (perx) { if (Q->x >= D.xe) Q->x = Q->x - D.xl; else if (Q->x < D.xs) Q->x = Q->x + D.xl; }
