@@ -10,11 +10,20 @@ Default workload: triply periodic sedimentation box 256^3 with 500 spheres
 of radius 6 dx (seeded non-overlapping positions, density ratio 2, gravity
 in -z, Lamb order 4).  It runs twice, once per form:
   plain          FP64 PCG from phi = 0 (the reference's solver);
-  mixed+learned  FP32 PCG inside FP64 defect correction, initial guess from
-                 the U-Net trained online on the solver's own solves (its
-                 training solves happen in untimed warm-up steps).
-`value` is the mixed+learned form's timesteps/s; the plain form's figures
-sit beside it.  Prints one JSON line with the throughput, a live roofline
+  mixed+last     FP32 PCG inside FP64 defect correction, initial guess from
+                 the previous solution alone (no network);
+  mixed+history  the same, initial guess from the span of the last four
+                 solutions (no network);
+  mixed+learned  the same, initial guess from the span of the last four
+                 solutions and the U-Net's guess, the network trained
+                 online on the solver's own solves (its training solves
+                 happen in untimed warm-up steps).
+The learned form runs first; every form then takes its number of warm-up
+steps, so all time the same physical timesteps.
+`value` is the mixed+learned form's timesteps/s; the other forms' figures
+sit beside it.  Kernel averages cover full launches only: the launches a
+batch enqueues past the iteration at which a solve stopped return at once
+and their timing samples are dropped.  Prints one JSON line with the throughput, a live roofline
 of the dominant kernel of each form (HIP events on the library stream:
 `roofline` for the timed form, `roofline_plain` for the plain one) and a
 CPU baseline of the FP64 host oracle on a bounded sample.
@@ -96,10 +105,11 @@ def parse():
                          "channel: single-phase channel at Re_tau 180")
     ap.add_argument("--form", default="both",
                     choices=["both", "plain", "mixed-learned", "mixed",
-                             "mixed-last"],
+                             "mixed-last", "mixed-history"],
                     help="which forms to time (mixed: FP32-PCG from phi = 0; "
                          "mixed-last: FP32-PCG with the guess from the "
-                         "previous solution only, no network)")
+                         "previous solution only; mixed-history: from the "
+                         "last four solutions; no network in either)")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -166,13 +176,15 @@ def log(a, *msg):
         print(*msg, file=sys.stderr, flush=True)
 
 
-def run_form(lib, a, precision, warm, use_net=1):
+def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None):
     """Warm-up (and, for the learned guess, the network's training solves),
-    then exactly a.steps timed timesteps."""
+    then exactly a.steps timed timesteps.  nwarm fixes the warm-up length
+    (so that every form times the same physical steps)."""
     warm_mode = warm
     case = tempfile.mkdtemp(prefix="bb_bench_")
     write_case(a, case, precision, warm)
     ctypes.c_int.in_dll(lib, "bb_warm_use_net").value = use_net
+    ctypes.c_int.in_dll(lib, "bb_warm_history").value = history
     tot = ctypes.c_long.in_dll(lib, "bb_pcg_iters_total")
     solves = ctypes.c_long.in_dll(lib, "bb_pcg_solves_total")
     wsolves = ctypes.c_int.in_dll(lib, "bb_warm_solves")
@@ -181,8 +193,9 @@ def run_form(lib, a, precision, warm, use_net=1):
     tw0 = time.perf_counter()
     sim.setup()
     nw = 0
-    while nw < a.warmup or (warm == "LEARNED" and use_net and
-                            wsolves.value < train):
+    while (nw < nwarm) if nwarm is not None else (
+            nw < a.warmup or (warm == "LEARNED" and use_net and
+                              wsolves.value < train)):
         sim.step()
         nw += 1
         log(a, f"[{precision}/{warm}] warm-up step {nw}: solves "
@@ -196,13 +209,17 @@ def run_form(lib, a, precision, warm, use_net=1):
     ctypes.c_int.in_dll(lib, "bb_time_every").value = max(a.time_every, 1)
     lib.bb_ktimer_enable(1 if a.time_every > 0 else 0)
     it0, sv0 = tot.value, solves.value
-    per_step, lamb, resid = [], [], []
+    per_step, lamb, resid, solve_iters = [], [], [], []
+    slog = (ctypes.c_int * 8192).in_dll(lib, "bb_solve_log")
     sim.sync()
     t0 = time.perf_counter()
     for s in range(a.steps):
-        i0 = tot.value
+        i0, n0 = tot.value, solves.value
         sim.step()
         per_step.append(tot.value - i0)
+        if s < 2:
+            solve_iters.append([slog[m % 8192]
+                                for m in range(n0, solves.value)])
         lamb.append(sim.lamb_stats()[0])
         resid.append(sim.pp_stats()["resid"])
         log(a, f"[{precision}/{warm}] step {s}: PCG iterations "
@@ -213,21 +230,27 @@ def run_form(lib, a, precision, warm, use_net=1):
     ms = ctypes.c_double()
     cnt = ctypes.c_long()
     ktimes = {}
+    dropped = {}
+    lib.bb_ktimer_dropped.restype = ctypes.c_long
     for name, kid in KT_IDS.items():
         lib.bb_ktimer_read(kid, ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value > 0:
             ktimes[name] = (ms.value, cnt.value)
+            dropped[name] = lib.bb_ktimer_dropped(kid)
     stats = sim.pp_stats()
     nsolves = solves.value - sv0
     wstat = None
     if use_net and warm_mode == "LEARNED":
         el = ctypes.c_double.in_dll(lib, "bb_warm_err_last_sum").value
+        eh = ctypes.c_double.in_dll(lib, "bb_warm_err_hist_sum").value
         eg = ctypes.c_double.in_dll(lib, "bb_warm_err_guess_sum").value
         wstat = {"net_solves": ctypes.c_int.in_dll(
                     lib, "bb_warm_net_solves").value,
                 "training_solves": train,
                 "energy_error_ratio_last_over_learned":
-                    el / eg if eg > 0 else None}
+                    el / eg if eg > 0 else None,
+                "energy_error_ratio_history_over_learned":
+                    eh / eg if eg > 0 else None}
     sim.teardown()
     shutil.rmtree(case, ignore_errors=True)
     elapsed = t1 - t0
@@ -240,9 +263,12 @@ def run_form(lib, a, precision, warm, use_net=1):
         "max_final_resid": max(resid),
         "last_true_resid": stats["true_resid"],
         "warmup_steps": nw,
+        "history": history if warm_mode == "LEARNED" else 0,
         "warmup_s": tw1 - tw0,
         "learned_guess": wstat,
         "ktimes": ktimes,
+        "samples_dropped_past_convergence": dropped,
+        "iters_per_solve_first_timed_steps": solve_iters,
     }
 
 
@@ -346,17 +372,30 @@ def main():
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:
             ctypes.c_int.in_dll(lib, name).value = val
-    forms = {"both": [("plain", "PLAIN", "OFF", 1),
-                      ("mixed_learned", "MIXED", "LEARNED", 1)],
-             "plain": [("plain", "PLAIN", "OFF", 1)],
-             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED", 1)],
-             "mixed": [("mixed", "MIXED", "OFF", 1)],
-             "mixed-last": [("mixed_last", "MIXED", "LEARNED", 0)]}[a.form]
+    # (key, precision, warm start, network, earlier solutions in the span)
+    forms = {"both": [("plain", "PLAIN", "OFF", 1, 0),
+                      ("mixed_last", "MIXED", "LEARNED", 0, 1),
+                      ("mixed_history", "MIXED", "LEARNED", 0, 4),
+                      ("mixed_learned", "MIXED", "LEARNED", 1, 4)],
+             "plain": [("plain", "PLAIN", "OFF", 1, 0)],
+             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED", 1, 4)],
+             "mixed": [("mixed", "MIXED", "OFF", 1, 0)],
+             "mixed-last": [("mixed_last", "MIXED", "LEARNED", 0, 1)],
+             "mixed-history": [("mixed_history", "MIXED", "LEARNED", 0, 4)]
+             }[a.form]
     res = {}
-    for key, prec, warm, use_net in forms:
-        res[key] = run_form(lib, a, prec, warm, use_net)
+    # the learned form first: its warm-up (the network's training solves)
+    # sets the warm-up length of the others, so all time the same steps
+    order = sorted(forms, key=lambda f: 0 if f[3] and f[2] == "LEARNED"
+                   else 1)
+    nwarm = None
+    for key, prec, warm, use_net, hist in order:
+        res[key] = run_form(lib, a, prec, warm, use_net, hist, nwarm)
+        if nwarm is None:
+            nwarm = res[key]["warmup_steps"]
     cells = a.n[0] * a.n[1] * a.n[2]
     mask = a.workload == "sediment" and a.nparts > 0
+    res = {f[0]: res[f[0]] for f in forms}
     head = res[forms[-1][0]]
     plain = res.get("plain")
     if a.workload == "sediment":

@@ -271,6 +271,8 @@ extern int bb_time_kernels;
 void bb_kt_begin(int id);
 void bb_kt_end(int id);
 void bb_kt_collect(void);     /* call after a stream synchronisation */
+void bb_kt_collect_upto(long qmax);   /* drops samples tagged > qmax */
+extern long bb_kt_q;
 
 /* Ghost refresh (periodic planes), BCs and flags, bb_bc.hip. */
 void bb_wrap(double *a, int grid);

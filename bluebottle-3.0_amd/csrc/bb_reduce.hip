@@ -4,6 +4,8 @@
  * Replaces the thrust::reduce / thrust::inner_product calls of
  * src/cuda_solver.cu:151-231 and src/cuda_bluebottle.cu:2341-2526.
  */
+#include <limits.h>
+
 #include "bb_common.h"
 
 __global__ void __launch_bounds__(REDUCE_THREADS)
@@ -39,13 +41,22 @@ double bb_fetch_scalar(int slot)
 }
 
 /* ------------------------------ kernel timing ---------------------------- */
+/* One launch in bb_time_every of each timed kernel is bracketed by a pair
+ * of HIP events on the library stream.  A sample is tagged with the PCG
+ * iteration (bb_kt_q) it belongs to and resolved when the solver has
+ * fetched its state: the launches a batch enqueues past the iteration at
+ * which the solve stopped return at once, and their samples are dropped
+ * (bb_kt_collect_upto) so that averages cover full launches only. */
 int bb_time_kernels = 0;
 int bb_time_every = 8;                 /* time one launch in this many */
-static hipEvent_t kt_a[KT_COUNT], kt_b[KT_COUNT];
+long bb_kt_q = 0;                      /* iteration of the next launch */
+#define KT_SLOTS 32
+static hipEvent_t kt_a[KT_COUNT][KT_SLOTS], kt_b[KT_COUNT][KT_SLOTS];
+static long kt_tag[KT_COUNT][KT_SLOTS];
 static int kt_pending[KT_COUNT], kt_init = 0, kt_active[KT_COUNT];
 static long kt_seen[KT_COUNT];
 static double kt_ms[KT_COUNT];
-static long kt_n[KT_COUNT];
+static long kt_n[KT_COUNT], kt_dropped[KT_COUNT];
 
 void bb_kt_begin(int id)
 {
@@ -54,35 +65,53 @@ void bb_kt_begin(int id)
   if (bb_time_every > 1 && (kt_seen[id]++ % bb_time_every) != 0) return;
   kt_active[id] = 1;
   if (!kt_init) {
-    for (int i = 0; i < KT_COUNT; i++) {
-      BB_CHECK(hipEventCreate(&kt_a[i]));
-      BB_CHECK(hipEventCreate(&kt_b[i]));
-    }
+    for (int i = 0; i < KT_COUNT; i++)
+      for (int j = 0; j < KT_SLOTS; j++) {
+        BB_CHECK(hipEventCreate(&kt_a[i][j]));
+        BB_CHECK(hipEventCreate(&kt_b[i][j]));
+      }
     kt_init = 1;
   }
-  if (kt_pending[id]) bb_kt_collect();
-  BB_CHECK(hipEventRecord(kt_a[id], bb_stream));
+  if (kt_pending[id] == KT_SLOTS) bb_kt_collect();
+  kt_tag[id][kt_pending[id]] = bb_kt_q;
+  BB_CHECK(hipEventRecord(kt_a[id][kt_pending[id]], bb_stream));
 }
 
 void bb_kt_end(int id)
 {
   if (!bb_time_kernels || !kt_active[id]) return;
-  BB_CHECK(hipEventRecord(kt_b[id], bb_stream));
-  kt_pending[id] = 1;
+  BB_CHECK(hipEventRecord(kt_b[id][kt_pending[id]], bb_stream));
+  kt_pending[id]++;
+}
+
+void bb_kt_collect_upto(long qmax)
+{
+  if (!kt_init) return;
+  for (int i = 0; i < KT_COUNT; i++) {
+    for (int j = 0; j < kt_pending[i]; j++) {
+      BB_CHECK(hipEventSynchronize(kt_b[i][j]));
+      if (kt_tag[i][j] > qmax) {
+        kt_dropped[i]++;
+        continue;
+      }
+      float ms = 0.f;
+      BB_CHECK(hipEventElapsedTime(&ms, kt_a[i][j], kt_b[i][j]));
+      kt_ms[i] += ms;
+      kt_n[i] += 1;
+    }
+    kt_pending[i] = 0;
+  }
 }
 
 void bb_kt_collect(void)
 {
-  if (!kt_init) return;
-  for (int i = 0; i < KT_COUNT; i++) {
-    if (!kt_pending[i]) continue;
-    BB_CHECK(hipEventSynchronize(kt_b[i]));
-    float ms = 0.f;
-    BB_CHECK(hipEventElapsedTime(&ms, kt_a[i], kt_b[i]));
-    kt_ms[i] += ms;
-    kt_n[i] += 1;
-    kt_pending[i] = 0;
-  }
+  bb_kt_collect_upto(LONG_MAX);
+}
+
+/* Samples dropped as launches past the end of their solve. */
+extern "C" long bb_ktimer_dropped(int id)
+{
+  return (id >= 0 && id < KT_COUNT) ? kt_dropped[id] : 0;
 }
 
 /* Accumulated milliseconds and launch count of timed kernel `id`. */
@@ -96,7 +125,9 @@ extern "C" void bb_ktimer_read(int id, double *ms, long *n)
 extern "C" void bb_ktimer_reset(void)
 {
   bb_kt_collect();
-  for (int i = 0; i < KT_COUNT; i++) { kt_ms[i] = 0.; kt_n[i] = 0; kt_seen[i] = 0; }
+  for (int i = 0; i < KT_COUNT; i++) {
+    kt_ms[i] = 0.; kt_n[i] = 0; kt_seen[i] = 0; kt_dropped[i] = 0;
+  }
 }
 
 extern "C" void bb_ktimer_enable(int on)

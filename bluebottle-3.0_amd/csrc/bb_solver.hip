@@ -35,6 +35,8 @@
 
 #include <stdint.h>
 
+#include <limits.h>
+
 #include "bb_common.h"
 
 /* Stencil weights 1/dx^2, ... in the arithmetic type V of the vectors. */
@@ -847,49 +849,113 @@ galerkin_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
   }
 }
 
-/* Tile partials of the least-squares system for the Jacobi-weighted
- * residual of phi0 = alpha psi + beta q: (b, W A psi), (b, W A q),
- * (A psi, W A psi), (A psi, W A q), (A q, W A q) with A = -Laplacian and
- * W = invM, over fluid cells. */
+/* One damped Jacobi sweep on (-A) phi = b over the fluid cells,
+ * out = phi + omega invM (b - (-A) phi); particle cells hold 0 (the value
+ * a solve from phi = 0 leaves there, which the fluid rows see as their
+ * neighbours). */
 __global__ void __launch_bounds__(TILE_THREADS)
-galerkin_mr_kernel(DevDom D, Coef K, InvMTab T,
-  const uint8_t *__restrict__ fluid, const double *__restrict__ psi,
-  const double *__restrict__ q, const double *__restrict__ rhs,
-  double *__restrict__ part)
-{
-  TILE_PROLOGUE
-  double a[5] = {0., 0., 0., 0., 0.};
-  if (colok) {
-    for (int k = k0; k <= k1; k++) {
-      long c = gidx(C, x, j, k);
-      if (solid(fluid, c)) continue;
-      double w = invm_at(T, D, x, j, k), b = rhs[c];
-      double Aps = neg_lap_at(D, K, psi, x, j, k);
-      double Aq = neg_lap_at(D, K, q, x, j, k);
-      a[0] += w * b * Aps;
-      a[1] += w * b * Aq;
-      a[2] += w * Aps * Aps;
-      a[3] += w * Aps * Aq;
-      a[4] += w * Aq * Aq;
-    }
-  }
-  long nt = (long) ntx_ * nty_ * ntz_;
-  for (int m = 0; m < 5; m++) {
-    tile_partial(a[m], part + m * nt, tile);
-    __syncthreads();                  /* tile_partial's LDS is reused */
-  }
-}
-
-/* phi = alpha psi + beta q on interior cells. */
-__global__ void __launch_bounds__(TILE_THREADS)
-combine_kernel(DevDom D, double al, const double *__restrict__ psi, double be,
-  const double *__restrict__ q, double *__restrict__ phi)
+jacobi_sweep_kernel(DevDom D, Coef K, InvMTab T, double omega,
+  const double *__restrict__ rhs, const uint8_t *__restrict__ fluid,
+  const double *__restrict__ phi, double *__restrict__ out)
 {
   TILE_PROLOGUE
   if (!colok) return;
   for (int k = k0; k <= k1; k++) {
     long c = gidx(C, x, j, k);
-    phi[c] = al * psi[c] + be * q[c];
+    double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
+    out[c] = solid(fluid, c) ? 0.
+                             : phi[c] + omega * (invm_at(T, D, x, j, k) * rr);
+  }
+}
+
+/* ---------------- Galerkin guess on several directions ------------------ */
+/* Up to GMAX directions v_0..v_{m-1} (Gcc arrays, zero in particle cells):
+ * per workgroup, the partial sums over its tile's fluid cells of the
+ * m(m+1)/2 entries (v_a, -A v_b), a <= b, and of the m entries (b, v_a),
+ * in the order [(b, v_0) .. (b, v_{m-1}), (v_0, A v_0), (v_0, A v_1) ..,
+ * (v_1, A v_1) ..].  These are plain FP64 sums in a fixed order; the
+ * guess they give is not part of the bit-exact plain solve. */
+#define GMAX 6
+#define GTERMS (GMAX + GMAX * (GMAX + 1) / 2)
+struct GVecs { const double *v[GMAX]; };
+
+__global__ void __launch_bounds__(TILE_THREADS)
+gram_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid, GVecs V,
+  int m, const double *__restrict__ rhs, double *__restrict__ part)
+{
+  TILE_PROLOGUE
+  double a[GTERMS];
+#pragma unroll
+  for (int t = 0; t < GTERMS; t++) a[t] = 0.;
+  if (colok) {
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      if (solid(fluid, c)) continue;
+      double vc[GMAX], Av[GMAX];
+#pragma unroll
+      for (int q = 0; q < GMAX; q++) {
+        if (q < m) {
+          vc[q] = V.v[q][c];
+          Av[q] = neg_lap_at(D, K, V.v[q], x, j, k);
+        }
+      }
+      const double b = rhs[c];
+      int t = 0;
+#pragma unroll
+      for (int q = 0; q < GMAX; q++, t++)
+        if (q < m) a[t] += b * vc[q];
+#pragma unroll
+      for (int p = 0; p < GMAX; p++)
+#pragma unroll
+        for (int q = p; q < GMAX; q++, t++)
+          if (q < m) a[t] += vc[p] * Av[q];
+    }
+  }
+  /* per-workgroup sums: wavefront trees, then the 8 rows */
+  __shared__ double s[TILE_Y][GTERMS];
+#pragma unroll
+  for (int t = 0; t < GTERMS; t++) {
+    double v = wave_tree_sum(a[t]);
+    if (lane == 0) s[wv][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < GTERMS) {
+    double v = 0.;
+    for (int w = 0; w < TILE_Y; w++) v += s[w][threadIdx.x];
+    part[(long) threadIdx.x * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+/* Sum of each of the GTERMS rows of n partials (one workgroup per row). */
+__global__ void __launch_bounds__(256)
+gram_sum_kernel(const double *__restrict__ part, long n,
+  double *__restrict__ out)
+{
+  __shared__ double s[256];
+  const double *row = part + (long) blockIdx.x * n;
+  double a = 0.;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) a += row[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+/* phi = sum_q c_q v_q over the fluid cells, 0 in particle cells. */
+__global__ void __launch_bounds__(TILE_THREADS)
+combine_many_kernel(DevDom D, GVecs V, int m, const double *__restrict__ cf,
+  const uint8_t *__restrict__ fluid, double *__restrict__ phi)
+{
+  TILE_PROLOGUE
+  if (!colok) return;
+  for (int k = k0; k <= k1; k++) {
+    long c = gidx(C, x, j, k);
+    double s = 0.;
+    for (int q = 0; q < m; q++) s += cf[q] * V.v[q][c];
+    phi[c] = solid(fluid, c) ? 0. : s;
   }
 }
 
@@ -1042,6 +1108,10 @@ static const uint8_t *fluid_mask(void)
 
 static CGState *_cg_state = NULL;
 static CGState *cg_host = NULL;
+/* Gram-system buffers of the multi-direction guess (bb_pp_galerkin_many). */
+static double *_gram_part = NULL, *_gram_out = NULL, *_gram_cf = NULL;
+static double *gram_host = NULL;
+static long gram_blocks = 0;
 static double *_p_alt = NULL;          /* second search-direction buffer */
 static double *_part_p = NULL;         /* tile partials of (p, Ap) */
 static double *_part_r = NULL;         /* tile partials of (r, z) */
@@ -1071,6 +1141,16 @@ int bb_defer_phi = 1;                  /* 1: phi updated every second
 int bb_mixed_restart = 0;              /* 1: restart CG at every correction */
 long bb_pcg_iters_total = 0;           /* PCG iterations over all solves */
 long bb_pcg_solves_total = 0;
+/* Iterations of each solve, in order (entry n % BB_SOLVE_LOG of solve n). */
+#define BB_SOLVE_LOG 8192
+int bb_solve_log[BB_SOLVE_LOG];
+
+static void log_solve(int niter)
+{
+  bb_solve_log[bb_pcg_solves_total % BB_SOLVE_LOG] = niter;
+  bb_pcg_iters_total += niter;
+  bb_pcg_solves_total++;
+}
 
 /* FP32 vectors of the mixed form's inner sweeps (Gcc layout).  Each
  * vector starts one float into its allocation, so that the first interior
@@ -1146,6 +1226,12 @@ void bb_solver_free(void)
   cg_host = NULL;
   _p_alt = _part_p = _part_r = NULL;
   buf_n = part_n = 0;
+  if (_gram_part) BB_CHECK(hipFree(_gram_part));
+  if (_gram_out) BB_CHECK(hipFree(_gram_out));
+  if (_gram_cf) BB_CHECK(hipFree(_gram_cf));
+  if (gram_host) BB_CHECK(hipHostFree(gram_host));
+  _gram_part = _gram_out = _gram_cf = gram_host = NULL;
+  gram_blocks = 0;
 }
 
 static void fetch_state(void)
@@ -1153,7 +1239,11 @@ static void fetch_state(void)
   BB_CHECK(hipMemcpyAsync(cg_host, _cg_state, sizeof(CGState),
     hipMemcpyDeviceToHost, bb_stream));
   BB_CHECK(hipStreamSynchronize(bb_stream));
-  if (bb_time_kernels) bb_kt_collect();
+  /* launches past the iteration the solve (or inner sweep) stopped at
+   * returned at once: their timing samples are dropped */
+  if (bb_time_kernels)
+    bb_kt_collect_upto((cg_host->done || cg_host->inner_done)
+                       ? (long) cg_host->q : LONG_MAX);
 }
 
 /* rhs and the tile partials of (b, b) in _part_p. */
@@ -1177,67 +1267,133 @@ double bb_true_residual(double sp_rhs)
   return sp_rhs > 0 ? sqrt(rr) / sqrt(sp_rhs) : 0.;
 }
 
-/* The initial guess with the least A-norm error in span{psi, q}:
- * phi = alpha psi + beta q solving the 2 x 2 Galerkin system
- *   [(psi,A psi) (psi,A q); (q,A psi) (q,A q)] [alpha; beta]
- *     = [(b, psi); (b, q)]
- * (A = -Laplacian, symmetric positive semidefinite).  use_psi / use_q
- * select the directions; a singular or non-positive system drops the
- * direction at fault.  Returns 1 if phi was set (0: start from phi = 0). */
-/* Energy 2 (b, phi0) - (phi0, A phi0) captured by the last guess, and by
- * the Galerkin guess on q alone; with the solution phi* the A-norm errors
- * squared are (b, phi*) minus these. */
-double bb_galerkin_e_guess = 0., bb_galerkin_e_q = 0.;
-/* 0: the guess minimises the A-norm error (Galerkin); 1: it minimises the
- * Jacobi-weighted residual (b - A phi0, invM (b - A phi0)), the norm of the
- * PCG stopping test. */
-int bb_galerkin_norm = 0;
 
-int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
-  int use_q, double *alpha, double *beta)
+/* Eigen-decomposition of a symmetric m x m matrix (m <= GMAX) by cyclic
+ * Jacobi rotations: on return the diagonal of a holds the eigenvalues and
+ * the columns of v the eigenvectors. */
+static void sym_eigen(int m, double a[GMAX][GMAX], double v[GMAX][GMAX])
+{
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++) v[i][j] = i == j ? 1. : 0.;
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = 0.;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) off += a[p][q] * a[p][q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) {
+        if (fabs(a[p][q]) < 1e-300) continue;
+        double th = 0.5 * (a[q][q] - a[p][p]) / a[p][q];
+        double t = (th >= 0. ? 1. : -1.) / (fabs(th) + sqrt(th * th + 1.));
+        double c = 1. / sqrt(t * t + 1.), sn = t * c;
+        for (int k = 0; k < m; k++) {
+          double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - sn * akq;
+          a[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < m; k++) {
+          double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - sn * aqk;
+          a[q][k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < m; k++) {
+          double vkp = v[k][p], vkq = v[k][q];
+          v[k][p] = c * vkp - sn * vkq;
+          v[k][q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+
+/* The initial guess with the least A-norm error in span{v_0 .. v_{m-1}}
+ * (m <= GMAX, each zero in particle cells) into _phi: the Gram system
+ * G c = g, G_ab = (v_a, -A v_b), g_a = (b, v_a), solved through its
+ * eigen-decomposition with directions of relative eigenvalue below 1e-12
+ * dropped (nearly parallel history vectors).  Returns the energy
+ * 2 (b, phi0) - (phi0, -A phi0) = c . g the guess captures (0: no guess,
+ * phi = 0), and in *e_first the same for the Galerkin guess on v_0 alone
+ * when first_alone is set. */
+double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
+  double *e_first)
 {
   cg_buffers();
-  bb_galerkin_e_guess = bb_galerkin_e_q = 0.;
+  *e_first = 0.;
+  if (m <= 0) return 0.;
+  if (m > GMAX) m = GMAX;
   const long nt = n_tiles(ddom);
-  *alpha = *beta = 0.;
-  if (!use_psi && !use_q) return 0;
-  const double *ps = use_psi ? psi : q, *qq = use_q ? q : psi;
-  hipLaunchKernelGGL(galerkin_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
-    ps, qq, _rhs_p, _part_w1);
-  for (int m = 0; m < 5; m++)
-    bb_reduce_rows(_part_w1 + m * nt, nt, _bb_scalars + 16 + m);
-  bb_fetch_scalars(21);
-  double *S = bb_host_scalars + 16;
-  double bp = S[0], bq = S[1], app = S[2], apq = S[3], aqq = S[4];
-  /* the normal equations of the chosen norm */
-  double Bp = bp, Bq = bq, Ap = app, Apq = apq, Aq = aqq;
-  if (bb_galerkin_norm == 1) {
-    hipLaunchKernelGGL(galerkin_mr_kernel, TILES_LAUNCH, ddom, coef(),
-      invm_table(coef()), fluid_mask(), ps, qq, _rhs_p, _part_w1);
-    for (int m = 0; m < 5; m++)
-      bb_reduce_rows(_part_w1 + m * nt, nt, _bb_scalars + 16 + m);
-    bb_fetch_scalars(21);
-    Bp = S[0]; Bq = S[1]; Ap = S[2]; Apq = S[3]; Aq = S[4];
+  if (gram_blocks != nt) {
+    if (_gram_part) BB_CHECK(hipFree(_gram_part));
+    if (!_gram_out) {
+      BB_CHECK(hipMalloc((void **) &_gram_out, GTERMS * sizeof(double)));
+      BB_CHECK(hipMalloc((void **) &_gram_cf, GMAX * sizeof(double)));
+      BB_CHECK(hipHostMalloc((void **) &gram_host,
+        (GTERMS + GMAX) * sizeof(double), hipHostMallocDefault));
+    }
+    BB_CHECK(hipMalloc((void **) &_gram_part, GTERMS * nt * sizeof(double)));
+    gram_blocks = nt;
   }
-  double a = 0., b = 0.;
-  double det = Ap * Aq - Apq * Apq;
-  if (use_psi && use_q && det > 1e-12 * Ap * Aq && Ap > 0. && Aq > 0.) {
-    a = (Bp * Aq - Bq * Apq) / det;
-    b = (Bq * Ap - Bp * Apq) / det;
-  } else if (use_psi && Ap > 0.) {
-    a = Bp / Ap;
-  } else if (use_q && Aq > 0.) {
-    b = Bq / Aq;
+  GVecs V;
+  for (int q = 0; q < GMAX; q++) V.v[q] = q < m ? vecs[q] : vecs[0];
+  hipLaunchKernelGGL(gram_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
+    V, m, _rhs_p, _gram_part);
+  hipLaunchKernelGGL(gram_sum_kernel, dim3(GTERMS), dim3(256), 0, bb_stream,
+    _gram_part, nt, _gram_out);
+  BB_CHECK(hipMemcpyAsync(gram_host, _gram_out, GTERMS * sizeof(double),
+    hipMemcpyDeviceToHost, bb_stream));
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  double g[GMAX], G[GMAX][GMAX], E[GMAX][GMAX];
+  int t = 0;
+  for (int q = 0; q < GMAX; q++, t++) g[q] = gram_host[t];
+  for (int p = 0; p < GMAX; p++)
+    for (int q = p; q < GMAX; q++, t++) G[p][q] = G[q][p] = gram_host[t];
+  if (first_alone && G[0][0] > 0.) *e_first = g[0] * g[0] / G[0][0];
+  sym_eigen(m, G, E);
+  double lmax = 0.;
+  for (int q = 0; q < m; q++) lmax = fmax(lmax, G[q][q]);
+  double c[GMAX] = {0., 0., 0., 0., 0., 0.};
+  if (!(lmax > 0.) || !isfinite(lmax)) return 0.;
+  for (int q = 0; q < m; q++) {
+    double lam = G[q][q];
+    if (!(lam > 1e-12 * lmax)) continue;
+    double proj = 0.;
+    for (int k = 0; k < m; k++) proj += E[k][q] * g[k];
+    for (int k = 0; k < m; k++) c[k] += E[k][q] * proj / lam;
   }
-  if (!isfinite(a) || !isfinite(b) || (a == 0. && b == 0.)) return 0;
-  bb_galerkin_e_guess = 2. * (a * bp + b * bq) -
-                        (a * a * app + 2. * a * b * apq + b * b * aqq);
-  if (use_q && aqq > 0.) bb_galerkin_e_q = bq * bq / aqq;
-  hipLaunchKernelGGL(combine_kernel, TILES_LAUNCH, ddom, a, ps, b, qq, _phi);
+  double energy = 0.;
+  for (int q = 0; q < m; q++) energy += c[q] * g[q];
+  for (int q = 0; q < m; q++)
+    if (!isfinite(c[q])) return 0.;
+  double *ch = gram_host + GTERMS;
+  for (int q = 0; q < m; q++) ch[q] = c[q];
+  BB_CHECK(hipMemcpyAsync(_gram_cf, ch, m * sizeof(double),
+    hipMemcpyHostToDevice, bb_stream));
+  hipLaunchKernelGGL(combine_many_kernel, TILES_LAUNCH, ddom, V, m, _gram_cf,
+    fluid_mask(), _phi);
   BB_LAUNCH_CHECK();
-  *alpha = a;
-  *beta = b;
-  return 1;
+  return energy > 0. ? energy : 0.;
+}
+
+/* nsweeps damped Jacobi sweeps (rounded up to even) on the initial guess
+ * in _phi for the right-hand side in _rhs_p.  The iteration matrix
+ * I - omega invM (-A) is self-adjoint in the A inner product with
+ * eigenvalues in (-1, 1) for 0 < omega < 1, so each sweep lowers the
+ * guess's A-norm error, damping its high-frequency part fastest.  The
+ * second search-direction buffer serves as the other half of the
+ * ping-pong (the solve that follows overwrites it). */
+void bb_pp_smooth(int nsweeps, double omega)
+{
+  if (nsweeps <= 0) return;
+  cg_buffers();
+  Coef K = coef();
+  InvMTab T = invm_table(K);
+  for (int s = 0; s < nsweeps; s += 2) {
+    hipLaunchKernelGGL(jacobi_sweep_kernel, TILES_LAUNCH, ddom, K, T, omega,
+      _rhs_p, fluid_mask(), _phi, _p_alt);
+    hipLaunchKernelGGL(jacobi_sweep_kernel, TILES_LAUNCH, ddom, K, T, omega,
+      _rhs_p, fluid_mask(), _p_alt, _phi);
+  }
+  BB_LAUNCH_CHECK();
 }
 
 /* (b, x) over the fluid cells of the current right-hand side. */
@@ -1338,6 +1494,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   const long nt = n_tiles(ddom);
   double *pnew = (q & 1) ? _p_q : _p_alt;
   double *pold = (q & 1) ? _p_alt : _p_q;
+  bb_kt_q = q;
   bb_kt_begin(KT_SPMV);
   if (bb_spmv_variant != 0) {
     if (q == 1)
@@ -1459,8 +1616,7 @@ static void pcg_plain(double ts, int warm)
   pp_last.etime = etime;
   pp_last.niter = cg_host->q;
   pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
-  bb_pcg_iters_total += cg_host->q;
-  bb_pcg_solves_total++;
+  log_solve(cg_host->q);
 }
 
 int bb_log_true_residual = 1;
@@ -1494,6 +1650,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
   const long nt = n_tiles(ddom);
   float *pnew = (q & 1) ? _p32a : _p32b;
   float *pold = (q & 1) ? _p32b : _p32a;
+  bb_kt_q = q;
   bb_kt_begin(KT_SPMV32);
   if (first)
     launch_group<true, float, true>(K, T, q, nt, _r32, pold, pnew, _Ap32);
@@ -1631,8 +1788,7 @@ static void pcg_mixed(double ts, int warm)
   pp_last.niter = q;
   pp_last.nouter = nouter;
   pp_last.resid = sqrt(sp_rq1)/sqrt(sp_rhs);
-  bb_pcg_iters_total += q;
-  bb_pcg_solves_total++;
+  log_solve(q);
 }
 
 extern "C" void cuda_PP_cg_mixed(void)

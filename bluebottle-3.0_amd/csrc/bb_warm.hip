@@ -2,22 +2,27 @@
  * Learned initial guess for the pressure solve (pp_warmstart LEARNED).
  *
  * Each solve of -A phi = -b starts from the guess with the least A-norm
- * error in span{phi_last, psi}: phi_last is the previous solve's solution,
- * and psi = U-Net(r0 / |r0|_rms, fluid mask) is the network's prediction of
- * the correction phi - phi_last from the defect r0 = b - (-A) phi_last of
- * the coming solve (for a linear operator the correction is a fixed
- * function of the defect, which is what the network learns).  The 2 x 2
- * Galerkin system fixes both weights, so the guess is never worse in the
- * A-norm than phi0 = 0 (the reference's start) or than the last solution
- * alone.  Every solve still ends at the same test
+ * error in span{phi_1 .. phi_h, psi}: phi_1 .. phi_h are the solutions of
+ * the last h solves (most recent first; h <= bb_warm_history), and
+ * psi = U-Net(r0 / |r0|_rms, fluid mask) is the network's prediction of
+ * the correction phi - phi_1 from the defect r0 = b - (-A) phi_1 of the
+ * coming solve (for a linear operator the correction is a fixed function
+ * of the defect, which is what the network learns).  The Galerkin system
+ * fixes every weight, so the guess is never worse in the A-norm than
+ * phi0 = 0 (the reference's start) or than phi_1 alone (the fallback).
+ * Successive solves of a timestep (the Lamb passes) change the right-hand
+ * side by amounts that shrink as the particle boundary values converge, so
+ * the earlier solutions span most of the next correction.  A guess that
+ * came from the network is then smoothed by bb_warm_sweeps damped Jacobi
+ * sweeps (its BF16 output carries high-frequency error that the PCG's
+ * residual test sees first).  Every solve still ends at the same test
  * (r, M^-1 r) <= pp_residual^2 (b, b).
  *
  * The network is trained online: the first bb_warm_train_solves solves
- * each contribute one r0 -> (phi - phi_last) pair (both scaled to unit
+ * each contribute one r0 -> (phi - phi_1) pair (both scaled to unit
  * RMS), on which bb_warm_steps Adam steps are taken over random cubic
  * crops of bb_warm_crop cells; afterwards the network only infers, by
- * default for the first solve of each timestep (the later Lamb passes of
- * a step start from span{phi_last}, whose defect is already small).
+ * default for the first solve of each timestep.
  *
  * No counterpart in the reference, which always starts from phi = 0
  * (src/solver_kernel.cu:258-283).
@@ -52,16 +57,25 @@ int bb_warm_net_all_passes = 0;        /* 0: the network runs for the first
                                           later Lamb passes of the step use
                                           span{phi_last} */
 int bb_warm_use_last = 1;              /* 0: span{psi} only */
+int bb_warm_history = 4;               /* earlier solutions in the span
+                                          (1: the last one only) */
+int bb_warm_sweeps = 4;                /* damped Jacobi sweeps on a guess
+                                          that came from the network */
+double bb_warm_omega = 0.8;            /* their damping factor */
 /* What the network gained on the last solve that used it: A-norm errors
  * squared of the guess from span{phi_last} alone and of the guess used,
  * and their running sums over all such solves. */
 double bb_warm_err_last = 0., bb_warm_err_guess = 0.;
 double bb_warm_err_last_sum = 0., bb_warm_err_guess_sum = 0.;
+/* the same for the guess from the earlier solutions without psi */
+double bb_warm_err_hist = 0., bb_warm_err_hist_sum = 0.;
 int bb_warm_net_solves = 0;
 static int net_this_solve = 0;
 double bb_warm_beta = 0.;
-static int have_last = 0;
-static double *_psi = NULL, *_phi_last = NULL, *_wres = NULL;
+#define HIST_MAX 5
+static double *_hist[HIST_MAX];        /* ring of earlier solutions */
+static int hist_n = 0, hist_head = 0;  /* count, next slot */
+static double *_psi = NULL, *_wres = NULL;
 static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
 static float *_cin = NULL, *_ctgt = NULL;
 static double *_wpart = NULL, *wpart_host = NULL;
@@ -173,6 +187,20 @@ __global__ void warm_place_kernel(DevDom D, const float *__restrict__ y,
   phi[c] = f ? (double) y[v] : 0.;
 }
 
+/* Zero in the particle cells of the current mask (interior cells).  A
+ * solution kept from an earlier solve holds values in cells that particles
+ * have entered since; the fluid rows of the coming solve would see them
+ * as boundary data, where a solve from phi = 0 sees zeros. */
+__global__ void mask_solid_kernel(DevDom D, const uint8_t *__restrict__ fluid,
+  double *__restrict__ a)
+{
+  long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  long nv = (long) D.nx * D.ny * D.nz;
+  if (v >= nv) return;
+  long c = cell_of_v(D, v);
+  if (!fluid[c]) a[c] = 0.;
+}
+
 static inline unsigned nb(long n) { return (unsigned) ((n + 255) / 256); }
 
 static bool dims_ok(void)
@@ -205,15 +233,17 @@ extern "C" void bb_warm_setup(void)
   BB_CHECK(hipMalloc((void **) &_wpart, MBLK * 4 * sizeof(double)));
   BB_CHECK(hipMalloc((void **) &_psi, ddom.cc.n * sizeof(double)));
   BB_CHECK(hipMemset(_psi, 0, ddom.cc.n * sizeof(double)));
-  BB_CHECK(hipMalloc((void **) &_phi_last, ddom.cc.n * sizeof(double)));
-  BB_CHECK(hipMemset(_phi_last, 0, ddom.cc.n * sizeof(double)));
+  for (int h = 0; h < HIST_MAX; h++) {
+    BB_CHECK(hipMalloc((void **) &_hist[h], ddom.cc.n * sizeof(double)));
+    BB_CHECK(hipMemset(_hist[h], 0, ddom.cc.n * sizeof(double)));
+  }
   BB_CHECK(hipMalloc((void **) &_wres, ddom.cc.n * sizeof(double)));
   BB_CHECK(hipMemset(_wres, 0, ddom.cc.n * sizeof(double)));
-  have_last = 0;
+  hist_n = hist_head = 0;
   net_step = -1;
   net_this_solve = 0;
-  bb_warm_err_last = bb_warm_err_guess = 0.;
-  bb_warm_err_last_sum = bb_warm_err_guess_sum = 0.;
+  bb_warm_err_last = bb_warm_err_guess = bb_warm_err_hist = 0.;
+  bb_warm_err_last_sum = bb_warm_err_guess_sum = bb_warm_err_hist_sum = 0.;
   bb_warm_net_solves = 0;
   BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
     hipHostMallocDefault));
@@ -228,9 +258,12 @@ extern "C" void bb_warm_teardown(void)
   BB_CHECK(hipFree(_wpart));
   BB_CHECK(hipHostFree(wpart_host));
   BB_CHECK(hipFree(_psi));
-  BB_CHECK(hipFree(_phi_last));
+  for (int h = 0; h < HIST_MAX; h++) {
+    BB_CHECK(hipFree(_hist[h]));
+    _hist[h] = NULL;
+  }
   BB_CHECK(hipFree(_wres));
-  _psi = _phi_last = _wres = NULL;
+  _psi = _wres = NULL;
   _win = _wtgt = _wout = _cin = _ctgt = NULL;
   _wpart = NULL;
   wpart_host = NULL;
@@ -247,11 +280,17 @@ static double rms_scale(double sum, double sum2, long n, double *mean)
 /* The defect b - (-A) x, bb_solver.hip. */
 void bb_pp_defect(const double *x, double *out);
 
-/* Network input from the defect of phi_last for the current right-hand
- * side (phi_last = 0 before the first solve: the defect is b). */
+/* The most recent earlier solution (a zero array before the first). */
+static double *phi_last(void)
+{
+  return _hist[(hist_head + HIST_MAX - 1) % HIST_MAX];
+}
+
+/* Network input from the defect of phi_1 for the current right-hand
+ * side (phi_1 = 0 before the first solve: the defect is b). */
 static void build_input(const uint8_t *fluid)
 {
-  bb_pp_defect(_phi_last, _wres);
+  bb_pp_defect(phi_last(), _wres);
   double m[4];
   moments(_wres, _wres, 0., m);
   double rm;
@@ -260,13 +299,12 @@ static void build_input(const uint8_t *fluid)
     bb_stream, ddom, _wres, fluid, rm, ir, _win);
 }
 
-/* Galerkin guess on span{psi, q}, its captured energies and (b, x),
- * bb_solver.hip. */
-int bb_pp_galerkin_guess(const double *psi, int use_psi, const double *q,
-  int use_q, double *alpha, double *beta);
-extern double bb_galerkin_e_guess, bb_galerkin_e_q;
+/* Galerkin guess on several directions and (b, x), bb_solver.hip. */
+double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
+  double *e_first);
 double bb_pp_energy(const double *x);
-static double e_guess = 0., e_last = 0.;
+void bb_pp_smooth(int nsweeps, double omega);
+static double e_guess = 0., e_last = 0., e_hist = 0.;
 
 /* C ABI: the guess for the coming solve into _phi.  Returns 1 if it was
  * placed, 0 if the solve starts from phi = 0. */
@@ -279,7 +317,20 @@ extern "C" int cuda_PP_warmstart_infer(void)
   int net_ready = bb_warm_use_net && bb_warm_solves >= bb_warm_train_solves &&
                   (bb_warm_net_all_passes || stepnum != net_step);
   if (net_ready) net_step = stepnum;
-  int last = bb_warm_use_last && have_last;
+  int hmax = bb_warm_history < 1 ? 1 : (bb_warm_history > HIST_MAX
+                                          ? HIST_MAX : bb_warm_history);
+  int nh = bb_warm_use_last ? (hist_n < hmax ? hist_n : hmax) : 0;
+  /* the earlier solutions, most recent first; particles may have moved
+   * since they were solved, and a value kept in a cell a particle has
+   * entered would act as boundary data of the fluid rows */
+  const double *vecs[HIST_MAX + 1];
+  for (int h = 0; h < nh; h++) {
+    double *v = _hist[(hist_head + HIST_MAX - 1 - h) % HIST_MAX];
+    if (fluid)
+      hipLaunchKernelGGL(mask_solid_kernel, dim3(nb(wnv)), dim3(256), 0,
+        bb_stream, ddom, fluid, v);
+    vecs[h] = v;
+  }
   if (net_ready) {
     build_input(fluid);
     bb_unet_volume(ddom.nx, ddom.ny, ddom.nz);
@@ -288,11 +339,24 @@ extern "C" int cuda_PP_warmstart_infer(void)
       bb_stream, ddom, _wout, fluid, _psi);
     BB_LAUNCH_CHECK();
   }
-  bb_warm_used = bb_pp_galerkin_guess(_psi, net_ready, _phi_last, last,
-    &bb_warm_alpha, &bb_warm_beta);
-  net_this_solve = net_ready && last && bb_warm_used;
-  e_guess = bb_galerkin_e_guess;
-  e_last = bb_galerkin_e_q;
+  int m = nh;
+  if (net_ready) vecs[m++] = _psi;
+  e_guess = e_last = e_hist = 0.;
+  if (m == 0) return 0;
+  if (net_ready && nh > 0) {
+    /* the same span without psi, for the record of what psi adds */
+    e_hist = bb_pp_galerkin_many(vecs, nh, 1, &e_last);
+  }
+  double ef = 0.;
+  e_guess = bb_pp_galerkin_many(vecs, m, 1, &ef);
+  if (!(net_ready && nh > 0)) {
+    e_last = nh > 0 ? ef : 0.;
+    e_hist = e_guess;
+  }
+  bb_warm_used = e_guess > 0.;
+  net_this_solve = net_ready && nh > 0 && bb_warm_used;
+  /* smoothing sweeps on the network's guess */
+  if (net_ready && bb_warm_used) bb_pp_smooth(bb_warm_sweeps, bb_warm_omega);
   return bb_warm_used;
 }
 
@@ -310,12 +374,12 @@ static void train_pair(void)
   const uint8_t *fluid = NPARTS > 0 ? _bb_fluid : NULL;
   build_input(fluid);
   double m[4];
-  moments(_phi, _phi_last, 1., m);
+  moments(_phi, phi_last(), 1., m);
   double em;
   double is = rms_scale(m[0], m[1], wnv, &em);
   if (is == 0.) return;
   hipLaunchKernelGGL(warm_target_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _phi, _phi_last, fluid, em, is, _wtgt);
+    bb_stream, ddom, _phi, phi_last(), fluid, em, is, _wtgt);
   int e = crop_edge;
   bb_unet_volume(e, e, e);
   for (int s = 0; s < bb_warm_steps; s++) {
@@ -331,25 +395,28 @@ static void train_pair(void)
 }
 
 /* C ABI: one training round on the solve just finished (input: the defect
- * of phi_last; target: phi - phi_last), then phi_last = phi. */
+ * of phi_1; target: phi - phi_1), then phi joins the earlier solutions. */
 extern "C" void cuda_PP_warmstart_train(void)
 {
   if (pp_warmstart != PP_WARM_LEARNED || !_win) return;
   int train = bb_warm_use_net && bb_warm_solves < bb_warm_train_solves;
   bb_warm_solves++;
   if (net_this_solve) {
-    /* (b, phi*) - (b, phi0) is the A-norm error squared of a Galerkin
-     * guess phi0 against the converged phi* */
+    /* (b, phi*) - E is the A-norm error squared of a Galerkin guess that
+     * captures the energy E, against the converged phi* */
     double es = bb_pp_energy(_phi);
     bb_warm_err_last = es - e_last;
+    bb_warm_err_hist = es - e_hist;
     bb_warm_err_guess = es - e_guess;
     bb_warm_err_last_sum += bb_warm_err_last;
+    bb_warm_err_hist_sum += bb_warm_err_hist;
     bb_warm_err_guess_sum += bb_warm_err_guess;
     bb_warm_net_solves++;
     net_this_solve = 0;
   }
   if (train) train_pair();
-  BB_CHECK(hipMemcpyAsync(_phi_last, _phi, ddom.cc.n * sizeof(double),
-    hipMemcpyDeviceToDevice, bb_stream));
-  have_last = 1;
+  BB_CHECK(hipMemcpyAsync(_hist[hist_head], _phi,
+    ddom.cc.n * sizeof(double), hipMemcpyDeviceToDevice, bb_stream));
+  hist_head = (hist_head + 1) % HIST_MAX;
+  if (hist_n < HIST_MAX) hist_n++;
 }

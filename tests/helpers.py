@@ -30,3 +30,55 @@ def describe(name, a, b):
     return (f"{name}: rel_l2={rel_l2(a, b):.3e} max_abs={diff.max():.3e} "
             f"at {idx} gpu={a[idx] if diff.size else None} "
             f"oracle={b[idx] if diff.size else None}")
+
+
+def grid_eigen_range(kw):
+    """Smallest nonzero and largest eigenvalue bound and the largest
+    diagonal entry of (-A) (the 7-point Laplacian the PCG solves) on the
+    uniform grid of a case without particles: per direction the smallest
+    nonzero eigenvalue is 4 sin^2(pi/n)/h^2 (periodic) or
+    4 sin^2(pi/(2n))/h^2 (zero-flux walls); the largest is below
+    sum 4/h^2."""
+    n, lo, hi = kw["n"], kw["lo"], kw["hi"]
+    h = [(hi[a] - lo[a]) / n[a] for a in range(3)]
+    per = [kw["p_bc"][f][0] == "PERIODIC" for f in "WSB"]
+    lmin = min(4.0 * np.sin(np.pi / (n[a] if per[a] else 2 * n[a])) ** 2
+               / h[a] ** 2 for a in range(3))
+    lmax = sum(4.0 / h[a] ** 2 for a in range(3))
+    return lmin, lmax, sum(2.0 / h[a] ** 2 for a in range(3))
+
+
+def solve_difference_bounds(lam_min, lam_max, diag_max, nsolves, tol=1e-8):
+    """Largest relative difference between two runs whose pressure solves
+    all pass the PCG's stopping test (r, M^-1 r) <= tol^2 (b, b), with
+    M = diag(-A) <= diag_max:
+      ||r|| <= sqrt(diag_max) tol ||b|| for each solve;
+      the error e = A^-1 r of a solve against the exact discrete solution
+      phi* then has ||e|| / ||phi*|| <= kappa sqrt(diag_max) tol
+      (||e|| <= ||r|| / lam_min, ||phi*|| >= ||b|| / lam_max), and
+      its gradient ||G e|| / ||G phi*|| = ||e||_A / ||phi*||_A
+      <= sqrt(kappa diag_max) tol;
+      two runs differ by at most twice that per solve, and the differences
+      of successive solves add (the step map is taken as non-expansive).
+    Returns the bound for phi-like fields (phi, p) and for the projected
+    velocity relative to the norm of the whole velocity field."""
+    kappa = lam_max / lam_min
+    r = np.sqrt(diag_max) * tol
+    return {"phi": 2.0 * nsolves * kappa * r,
+            "grad": 2.0 * nsolves * np.sqrt(kappa) * r}
+
+
+def field_bounds(b, fields_ref):
+    """Per-field bounds from solve_difference_bounds: p and phi take the
+    phi bound, u, v and w the gradient bound scaled by ||U|| / ||u_c||
+    (the reference run's fields)."""
+    U = np.sqrt(sum(np.linalg.norm(interior(fields_ref[c])) ** 2
+                    for c in ("u", "v", "w")))
+    out = {}
+    for f in fields_ref:
+        if f in ("u", "v", "w"):
+            nc = np.linalg.norm(interior(fields_ref[f]))
+            out[f] = b["grad"] * U / max(nc, 1e-300)
+        elif f in ("p", "phi"):
+            out[f] = b["phi"]
+    return out

@@ -20,7 +20,8 @@ import pytest
 import bbconfig
 import bbhip
 import orc
-from helpers import describe, interior, rel_l2
+from helpers import (describe, field_bounds, grid_eigen_range, interior,
+                     rel_l2, solve_difference_bounds)
 
 pytestmark = pytest.mark.gpu
 
@@ -120,12 +121,23 @@ def test_mixed_restart_variant(case_dir):
 def test_mixed_inner_tolerances(case_dir, inner_tol):
     """Loose and tight inner sweeps reach the same FP64 target.  v is two
     orders smaller than u in this lid-driven box, so its relative error is
-    the largest (measured 1.4e-6 and 4.3e-6)."""
+    the largest (measured 1.4e-6 and 4.3e-6).  Its bound, 2e-5, lies below
+    the difference two runs can show when each of their five solves passes
+    the stopping test (solve_difference_bounds on this grid's closed-form
+    eigenvalues: about 1e-2 for v), which the test also checks."""
     kw = bbconfig.cavity(n=16, steps=5)
     kw["n"] = (37, 11, 23)
     kw["hi"] = (1.3, 0.7, 1.1)
     res = run_mixed(case_dir, kw, 5, inner_tol=inner_tol)
-    check_mixed(res, {"u": 1e-8, "v": 2e-5, "w": 1e-8, "p": 1e-8})
+    stated = {"u": 1e-8, "v": 2e-5, "w": 1e-8, "p": 1e-8}
+    derived = field_bounds(
+        solve_difference_bounds(*grid_eigen_range(kw), nsolves=5),
+        res["orc"])
+    print("bounds derived from the stopping test:",
+          {f: f"{b:.2e}" for f, b in derived.items()})
+    for f in FIELDS:
+        assert stated[f] <= derived[f], f
+    check_mixed(res, stated)
 
 
 @pytest.mark.parametrize("inner_tol", [None, 0.3])
