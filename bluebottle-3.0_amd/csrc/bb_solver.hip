@@ -1183,8 +1183,13 @@ static void cg_buffers32(void)
   buf32_n = ddom.cc.n;
 }
 
+/* Set while cuda_PP_cg_noalloc runs: the CG buffers then point into the
+ * static scratch array and must not be (re)allocated. */
+static int cg_in_scratch = 0;
+
 static void cg_buffers(void)
 {
+  if (cg_in_scratch) return;
   if (!_cg_state) {
     BB_CHECK(hipMalloc((void **) &_cg_state, sizeof(CGState)));
     BB_CHECK(hipMemset(_cg_state, 0, sizeof(CGState)));
@@ -1640,6 +1645,110 @@ extern "C" void cuda_PP_cg(void)
   /* The particle form (src/cuda_solver.cu:39-300): the same iteration with
    * the particle cells masked out (fluid_mask). */
   cuda_PP_cg_noparts();
+}
+
+/* ---------------------- allocation-free plain solve ---------------------- */
+/* Scratch of cuda_PP_cg_noalloc: the CG state, the second search-direction
+ * buffer and the two tile-partial vectors of the plain solve, carved from
+ * one static device array (each device holds its own copy until the
+ * process ends).  Host code never passes `scratch` itself to a kernel: its
+ * device address comes from hipGetSymbolAddress, once per device. */
+#define BB_SCRATCH_BYTES ((size_t) 64 << 20)
+#define BB_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[BB_SCRATCH_BYTES];
+static unsigned char *scratch_dev[BB_MAX_DEVICES];
+static CGState cg_host_plain;          /* pageable host copy of the state */
+extern "C" int recorder_enabled;       /* host/recorder.c */
+
+__global__ void scratch_zero_kernel(unsigned long long *p, long n)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t < n; t += (long) gridDim.x * blockDim.x) p[t] = 0ull;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
+
+/* Bytes of scratch the plain solve needs on the current grid. */
+static size_t noalloc_bytes(void)
+{
+  const long nt = n_tiles(ddom);
+  return align256(sizeof(CGState)) + align256(ddom.cc.n * sizeof(double))
+       + 2 * align256(nt * sizeof(double));
+}
+
+extern "C" int cuda_PP_cg_noalloc(void)
+{
+  int dev = 0;
+  BB_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= BB_MAX_DEVICES) {
+    fprintf(stderr, "cuda_PP_cg_noalloc: device %d beyond the %d supported\n",
+      dev, BB_MAX_DEVICES);
+    return 1;
+  }
+  if (!scratch_dev[dev]) {
+    void *ptr = NULL;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == NULL) {
+      fprintf(stderr, "cuda_PP_cg_noalloc: no scratch address on device %d "
+        "(%s)\n", dev, hipGetErrorString(e));
+      return 1;
+    }
+    scratch_dev[dev] = (unsigned char *) ptr;
+  }
+  const size_t need = noalloc_bytes();
+  if (need > BB_SCRATCH_BYTES) {
+    fprintf(stderr, "cuda_PP_cg_noalloc: the grid needs %zu bytes of scratch, "
+      "more than the %zu available (at most about %zu cells)\n", need,
+      BB_SCRATCH_BYTES, BB_SCRATCH_BYTES / sizeof(double) * 15 / 16);
+    return 2;
+  }
+  if (pp_warmstart == PP_WARM_LEARNED) {
+    fprintf(stderr, "cuda_PP_cg_noalloc: the learned initial guess "
+      "allocates its network; use cuda_PP_cg\n");
+    return 3;
+  }
+  /* carve the scratch and clear it (a fill kernel, no memset node) */
+  unsigned char *base = scratch_dev[dev];
+  const long nt = n_tiles(ddom);
+  CGState *st = (CGState *) base;
+  base += align256(sizeof(CGState));
+  double *palt = (double *) base;
+  base += align256(ddom.cc.n * sizeof(double));
+  double *pp = (double *) base;
+  base += align256(nt * sizeof(double));
+  double *pr = (double *) base;
+  hipLaunchKernelGGL(scratch_zero_kernel, dim3(1024), dim3(256), 0, bb_stream,
+    (unsigned long long *) scratch_dev[dev], (long) (need / 8));
+  BB_LAUNCH_CHECK();
+
+  /* run the plain solve on the scratch buffers, then put the library's
+   * own buffers back */
+  CGState *s_state = _cg_state, *s_host = cg_host;
+  double *s_alt = _p_alt, *s_pp = _part_p, *s_pr = _part_r;
+  const int s_timers = bb_time_kernels, s_rec = recorder_enabled;
+  _cg_state = st;
+  cg_host = &cg_host_plain;
+  _p_alt = palt;
+  _part_p = pp;
+  _part_r = pr;
+  bb_time_kernels = 0;                 /* its events are created lazily */
+  recorder_enabled = 0;
+  cg_in_scratch = 1;
+  double ts = now_s();
+  bb_pp_rhs();
+  pcg_plain(ts, 0);
+  pp_last.true_resid = (bb_log_true_residual && pp_last.sp_rhs > 0)
+    ? bb_true_residual(pp_last.sp_rhs) : 0.;
+  BB_CHECK(hipStreamSynchronize(bb_stream));
+  cg_in_scratch = 0;
+  _cg_state = s_state;
+  cg_host = s_host;
+  _p_alt = s_alt;
+  _part_p = s_pp;
+  _part_r = s_pr;
+  bb_time_kernels = s_timers;
+  recorder_enabled = s_rec;
+  return 0;
 }
 
 /* Enqueue inner iteration q of the mixed form (FP32 vectors); `first`

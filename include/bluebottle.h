@@ -440,6 +440,15 @@ void cuda_PP_cg(void);                       /* src/cuda_solver.cu:39-300 */
 void cuda_PP_cg_noparts(void);               /* src/cuda_solver.cu:574-761 */
 /* No counterpart in the reference: the FP32-inner / FP64-outer form. */
 void cuda_PP_cg_mixed(void);
+/* No counterpart in the reference: cuda_PP_cg (plain FP64 Jacobi-PCG, the
+ * same kernels and bits) for callers that cannot allocate device memory
+ * (code under graph capture, serving loops).  Its scratch (CG state, the
+ * second search direction, tile partials) lives in a static 64 MiB device
+ * array, which limits the grid to about 7.9 million cells including ghosts;
+ * calls that overlap on one GPU race on it.  Returns 0, or 1 (no scratch
+ * address on this device), 2 (grid too large) or 3 (learned initial guess
+ * selected: its network allocates) before doing any work. */
+int cuda_PP_cg_noalloc(void);
 /* Particles and Physalis.  On one GPU every particle is held once; a
  * particle whose cage or quadrature nodes cross a periodic boundary is
  * handled through its periodic image positions x +- L (the same values the
