@@ -73,29 +73,22 @@ struct StarArgs {
   double rho_f, nu, dt, dt0, ab0, ab, ddx, ddy, ddz, f, bg, sref;
 };
 
-/* u* at face (i, j, k) */
-__device__ inline void u_star_cell(const DevDom &D, const StarArgs &A,
-  const double *__restrict__ u0, const double *__restrict__ v0,
-  const double *__restrict__ w0, const double *__restrict__ p,
-  const double *__restrict__ conv0, const double *__restrict__ diff0,
-  const int *__restrict__ phase, const double *__restrict__ sb,
-  double *__restrict__ ust, double *__restrict__ conv,
-  double *__restrict__ diff, int i, int j, int k)
-{
-  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
-  long c = gidx(X, i, j, k);
-  double u011 = u0[c - 1], u111 = u0[c], u211 = u0[c + 1];
-  double u101 = u0[c - X.sy], u121 = u0[c + X.sy];
-  double u110 = u0[c - X.sz], u112 = u0[c + X.sz];
-  long cv = gidx(Y, i, j, k);
-  double v011 = v0[cv - 1], v111 = v0[cv];
-  double v021 = v0[cv - 1 + Y.sy], v121 = v0[cv + Y.sy];
-  long cw = gidx(Z, i, j, k);
-  double w011 = w0[cw - 1], w111 = w0[cw];
-  double w012 = w0[cw - 1 + Z.sz], w112 = w0[cw + Z.sz];
-  long cc = gidx(C, i, j, k);
+/* The arithmetic of u*, v* and w* at one face from its loaded
+ * neighbourhood: the star value s, the convective and diffusive terms sc
+ * and sd.  Names follow the reference: u111 is the face itself, u011 /
+ * u211 its x neighbours, u101 / u121 y, u110 / u112 z; the other two
+ * components at the four faces around it; pm, pc the two cells beside it;
+ * sbm, sbc the scalar there (Boussinesq); fluid = both cells are fluid. */
+struct StarOut { double s, sc, sd; };
 
-  double s = (p[cc - 1] - p[cc]) * A.ddx/A.rho_f;
+__device__ inline StarOut u_star_math(const StarArgs &A, double u011,
+  double u111, double u211, double u101, double u121, double u110,
+  double u112, double v011, double v111, double v021, double v121,
+  double w011, double w111, double w012, double w112, double pm, double pc,
+  double conv0c, double diff0c, bool bous, double sbm, double sbc,
+  bool fluid)
+{
+  double s = (pm - pc) * A.ddx/A.rho_f;
   double duudx = (u211 + u111)*(u211 + u111) - (u111 + u011)*(u111 + u011);
   duudx *= 0.25 * A.ddx;
   double duvdy = (u121 + u111)*(v121 + v021) - (u111 + u101)*(v111 + v011);
@@ -103,7 +96,7 @@ __device__ inline void u_star_cell(const DevDom &D, const StarArgs &A,
   double duwdz = (u112 + u111)*(w112 + w012) - (u111 + u110)*(w111 + w011);
   duwdz *= 0.25 * A.ddz;
   double sc = duudx + duvdy + duwdz;
-  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0[c]);
+  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0c);
   else s += -sc;
   double dud1 = (u211 - u111) * A.ddx;
   double dud0 = (u111 - u011) * A.ddx;
@@ -115,59 +108,26 @@ __device__ inline void u_star_cell(const DevDom &D, const StarArgs &A,
   dud0 = (u111 - u110) * A.ddz;
   double ddudzz = (dud1 - dud0) * A.ddz;
   double sd = A.nu * (ddudxx + ddudyy + ddudzz);
-  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
+  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0c);
   else s += sd;
   double fo = A.f;
-  if (sb) fo += A.bg * (0.5*(sb[cc - 1] + sb[cc]) - A.sref);
+  if (bous) fo += A.bg * (0.5*(sbm + sbc) - A.sref);
   s += fo;
   s *= A.dt;
   s += u111;
-  s *= (double) (phase[cc - 1] < 0 && phase[cc] < 0);
-  ust[c] = s;
-  conv[c] = sc;
-  diff[c] = sd;
+  s *= (double) fluid;
+  return {s, sc, sd};
 }
 
-__global__ void __launch_bounds__(256)
-u_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
-  const double *__restrict__ v0, const double *__restrict__ w0,
-  const double *__restrict__ p, const double *__restrict__ conv0,
-  const double *__restrict__ diff0, const int *__restrict__ phase,
-  const double *__restrict__ sb,
-  double *__restrict__ ust, double *__restrict__ conv,
-  double *__restrict__ diff)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > D.fx.ni || j > D.fx.nj) return;
-  u_star_cell(D, A, u0, v0, w0, p, conv0, diff0, phase, sb, ust, conv,
-    diff, i, j, k);
-}
 
-/* v* at face (i, j, k) */
-__device__ inline void v_star_cell(const DevDom &D, const StarArgs &A,
-  const double *__restrict__ u0, const double *__restrict__ v0,
-  const double *__restrict__ w0, const double *__restrict__ p,
-  const double *__restrict__ conv0, const double *__restrict__ diff0,
-  const int *__restrict__ phase, const double *__restrict__ sb,
-  double *__restrict__ vst, double *__restrict__ conv,
-  double *__restrict__ diff, int i, int j, int k)
+__device__ inline StarOut v_star_math(const StarArgs &A, double v011,
+  double v111, double v211, double v101, double v121, double v110,
+  double v112, double u101, double u111, double u201, double u211,
+  double w101, double w111, double w102, double w112, double pm, double pc,
+  double conv0c, double diff0c, bool bous, double sbm, double sbc,
+  bool fluid)
 {
-  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
-  long c = gidx(Y, i, j, k);
-  double v101 = v0[c - Y.sy], v111 = v0[c], v121 = v0[c + Y.sy];
-  double v110 = v0[c - Y.sz], v112 = v0[c + Y.sz];
-  double v011 = v0[c - 1], v211 = v0[c + 1];
-  long cw = gidx(Z, i, j, k);
-  double w101 = w0[cw - Z.sy], w111 = w0[cw];
-  double w102 = w0[cw - Z.sy + Z.sz], w112 = w0[cw + Z.sz];
-  long cu = gidx(X, i, j, k);
-  double u101 = u0[cu - X.sy], u111 = u0[cu];
-  double u201 = u0[cu - X.sy + 1], u211 = u0[cu + 1];
-  long cc = gidx(C, i, j, k);
-
-  double s = (p[cc - C.sy] - p[cc]) * A.ddy/A.rho_f;
+  double s = (pm - pc) * A.ddy/A.rho_f;
   double dvudx = (v211 + v111)*(u211 + u201) - (v111 + v011)*(u111 + u101);
   dvudx *= 0.25 * A.ddx;
   double dvvdy = (v121 + v111)*(v121 + v111) - (v111 + v101)*(v111 + v101);
@@ -175,7 +135,7 @@ __device__ inline void v_star_cell(const DevDom &D, const StarArgs &A,
   double dvwdz = (v112 + v111)*(w112 + w102) - (v111 + v110)*(w111 + w101);
   dvwdz *= 0.25 * A.ddz;
   double sc = dvudx + dvvdy + dvwdz;
-  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0[c]);
+  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0c);
   else s += -sc;
   double dvd1 = (v211 - v111) * A.ddx;
   double dvd0 = (v111 - v011) * A.ddx;
@@ -187,59 +147,26 @@ __device__ inline void v_star_cell(const DevDom &D, const StarArgs &A,
   dvd0 = (v111 - v110) * A.ddz;
   double ddvdzz = (dvd1 - dvd0) * A.ddz;
   double sd = A.nu * (ddvdxx + ddvdyy + ddvdzz);
-  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
+  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0c);
   else s += sd;
   double fo = A.f;
-  if (sb) fo += A.bg * (0.5*(sb[cc - C.sy] + sb[cc]) - A.sref);
+  if (bous) fo += A.bg * (0.5*(sbm + sbc) - A.sref);
   s += fo;
   s *= A.dt;
   s += v111;
-  s *= (double) (phase[cc - C.sy] < 0 && phase[cc] < 0);
-  vst[c] = s;
-  conv[c] = sc;
-  diff[c] = sd;
+  s *= (double) fluid;
+  return {s, sc, sd};
 }
 
-__global__ void __launch_bounds__(256)
-v_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
-  const double *__restrict__ v0, const double *__restrict__ w0,
-  const double *__restrict__ p, const double *__restrict__ conv0,
-  const double *__restrict__ diff0, const int *__restrict__ phase,
-  const double *__restrict__ sb,
-  double *__restrict__ vst, double *__restrict__ conv,
-  double *__restrict__ diff)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > D.fy.ni || j > D.fy.nj) return;
-  v_star_cell(D, A, u0, v0, w0, p, conv0, diff0, phase, sb, vst, conv,
-    diff, i, j, k);
-}
 
-/* w* at face (i, j, k) */
-__device__ inline void w_star_cell(const DevDom &D, const StarArgs &A,
-  const double *__restrict__ u0, const double *__restrict__ v0,
-  const double *__restrict__ w0, const double *__restrict__ p,
-  const double *__restrict__ conv0, const double *__restrict__ diff0,
-  const int *__restrict__ phase, const double *__restrict__ sb,
-  double *__restrict__ wst, double *__restrict__ conv,
-  double *__restrict__ diff, int i, int j, int k)
+__device__ inline StarOut w_star_math(const StarArgs &A, double w011,
+  double w111, double w211, double w101, double w121, double w110,
+  double w112, double u110, double u111, double u210, double u211,
+  double v110, double v111, double v120, double v121, double pm, double pc,
+  double conv0c, double diff0c, bool bous, double sbm, double sbc,
+  bool fluid)
 {
-  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
-  long c = gidx(Z, i, j, k);
-  double w110 = w0[c - Z.sz], w111 = w0[c], w112 = w0[c + Z.sz];
-  double w011 = w0[c - 1], w211 = w0[c + 1];
-  double w101 = w0[c - Z.sy], w121 = w0[c + Z.sy];
-  long cu = gidx(X, i, j, k);
-  double u110 = u0[cu - X.sz], u111 = u0[cu];
-  double u210 = u0[cu - X.sz + 1], u211 = u0[cu + 1];
-  long cv = gidx(Y, i, j, k);
-  double v110 = v0[cv - Y.sz], v111 = v0[cv];
-  double v120 = v0[cv - Y.sz + Y.sy], v121 = v0[cv + Y.sy];
-  long cc = gidx(C, i, j, k);
-
-  double s = (p[cc - C.sz] - p[cc]) * A.ddz/A.rho_f;
+  double s = (pm - pc) * A.ddz/A.rho_f;
   double dwudx = (w211 + w111)*(u211 + u210) - (w111 + w011)*(u111 + u110);
   dwudx *= 0.25 * A.ddx;
   double dwvdy = (w121 + w111)*(v121 + v120) - (w111 + w101)*(v111 + v110);
@@ -247,7 +174,7 @@ __device__ inline void w_star_cell(const DevDom &D, const StarArgs &A,
   double dwwdz = (w112 + w111)*(w112 + w111) - (w111 + w110)*(w111 + w110);
   dwwdz *= 0.25 * A.ddz;
   double sc = dwudx + dwvdy + dwwdz;
-  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0[c]);
+  if (A.dt0 > 0) s += (-A.ab * sc + A.ab0 * conv0c);
   else s += -sc;
   double dwd1 = (w211 - w111) * A.ddx;
   double dwd0 = (w111 - w011) * A.ddx;
@@ -259,17 +186,114 @@ __device__ inline void w_star_cell(const DevDom &D, const StarArgs &A,
   dwd0 = (w111 - w110) * A.ddz;
   double ddwdzz = (dwd1 - dwd0) * A.ddz;
   double sd = A.nu * (ddwdxx + ddwdyy + ddwdzz);
-  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0[c]);
+  if (A.dt0 > 0) s += (A.ab * sd - A.ab0 * diff0c);
   else s += sd;
   double fo = A.f;
-  if (sb) fo += A.bg * (0.5*(sb[cc - C.sz] + sb[cc]) - A.sref);
+  if (bous) fo += A.bg * (0.5*(sbm + sbc) - A.sref);
   s += fo;
   s *= A.dt;
   s += w111;
-  s *= (double) (phase[cc - C.sz] < 0 && phase[cc] < 0);
-  wst[c] = s;
-  conv[c] = sc;
-  diff[c] = sd;
+  s *= (double) fluid;
+  return {s, sc, sd};
+}
+
+
+/* u*, v* and w* over z-columns: a thread owns face column (i, j) for
+ * STAR_KZ consecutive planes (the workgroup: 64 x 4 columns, numbered
+ * through tile_of_block so that each XCD's L2 holds a contiguous slab).
+ * Values a face shares with the face one plane up (its own component's z
+ * neighbours, the other components on the planes k and k + 1, and for w*
+ * the cells below) are carried in registers from plane to plane instead
+ * of being loaded again; every face computes the same expression as
+ * before (u_star_math etc.), so the results are unchanged bit for bit. */
+#define STAR_KZ 4
+
+struct StarCol { int i, j, k0, k1; bool ok; };
+
+__device__ inline StarCol star_column(const Grid &G)
+{
+  const int nbx = (G.ni + 63) / 64, nby = (G.nj + 3) / 4;
+  const int nbz = (G.nk + STAR_KZ - 1) / STAR_KZ;
+  long t = tile_of_block((long) nbx * nby * nbz);
+  StarCol S;
+  S.i = (int) (t % nbx) * 64 + (int) (threadIdx.x & 63) + 1;
+  S.j = (int) ((t / nbx) % nby) * 4 + (int) (threadIdx.x >> 6) + 1;
+  S.k0 = (int) (t / ((long) nbx * nby)) * STAR_KZ + 1;
+  S.k1 = S.k0 + STAR_KZ - 1 < G.nk ? S.k0 + STAR_KZ - 1 : G.nk;
+  S.ok = S.i <= G.ni && S.j <= G.nj;
+  return S;
+}
+
+static unsigned star_blocks(const Grid &G)
+{
+  return (unsigned) (((G.ni + 63) / 64) * ((G.nj + 3) / 4) *
+                     ((G.nk + STAR_KZ - 1) / STAR_KZ));
+}
+
+__global__ void __launch_bounds__(256)
+u_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
+  const double *__restrict__ v0, const double *__restrict__ w0,
+  const double *__restrict__ p, const double *__restrict__ conv0,
+  const double *__restrict__ diff0, const int *__restrict__ phase,
+  const double *__restrict__ sb,
+  double *__restrict__ ust, double *__restrict__ conv,
+  double *__restrict__ diff)
+{
+  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
+  const StarCol S = star_column(X);
+  if (!S.ok) return;
+  long c = gidx(X, S.i, S.j, S.k0), cv = gidx(Y, S.i, S.j, S.k0);
+  long cw = gidx(Z, S.i, S.j, S.k0), cc = gidx(C, S.i, S.j, S.k0);
+  double u110 = u0[c - X.sz], u111 = u0[c];
+  double w011 = w0[cw - 1], w111 = w0[cw];
+  for (int k = S.k0; k <= S.k1; k++) {
+    const double u112 = u0[c + X.sz];
+    const double w012 = w0[cw - 1 + Z.sz], w112 = w0[cw + Z.sz];
+    StarOut o = u_star_math(A, u0[c - 1], u111, u0[c + 1], u0[c - X.sy],
+      u0[c + X.sy], u110, u112, v0[cv - 1], v0[cv], v0[cv - 1 + Y.sy],
+      v0[cv + Y.sy], w011, w111, w012, w112, p[cc - 1], p[cc], conv0[c],
+      diff0[c], sb != NULL, sb ? sb[cc - 1] : 0., sb ? sb[cc] : 0.,
+      phase[cc - 1] < 0 && phase[cc] < 0);
+    ust[c] = o.s;
+    conv[c] = o.sc;
+    diff[c] = o.sd;
+    u110 = u111; u111 = u112;
+    w011 = w012; w111 = w112;
+    c += X.sz; cv += Y.sz; cw += Z.sz; cc += C.sz;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+v_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
+  const double *__restrict__ v0, const double *__restrict__ w0,
+  const double *__restrict__ p, const double *__restrict__ conv0,
+  const double *__restrict__ diff0, const int *__restrict__ phase,
+  const double *__restrict__ sb,
+  double *__restrict__ vst, double *__restrict__ conv,
+  double *__restrict__ diff)
+{
+  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
+  const StarCol S = star_column(Y);
+  if (!S.ok) return;
+  long c = gidx(Y, S.i, S.j, S.k0), cu = gidx(X, S.i, S.j, S.k0);
+  long cw = gidx(Z, S.i, S.j, S.k0), cc = gidx(C, S.i, S.j, S.k0);
+  double v110 = v0[c - Y.sz], v111 = v0[c];
+  double w101 = w0[cw - Z.sy], w111 = w0[cw];
+  for (int k = S.k0; k <= S.k1; k++) {
+    const double v112 = v0[c + Y.sz];
+    const double w102 = w0[cw - Z.sy + Z.sz], w112 = w0[cw + Z.sz];
+    StarOut o = v_star_math(A, v0[c - 1], v111, v0[c + 1], v0[c - Y.sy],
+      v0[c + Y.sy], v110, v112, u0[cu - X.sy], u0[cu], u0[cu - X.sy + 1],
+      u0[cu + 1], w101, w111, w102, w112, p[cc - C.sy], p[cc], conv0[c],
+      diff0[c], sb != NULL, sb ? sb[cc - C.sy] : 0., sb ? sb[cc] : 0.,
+      phase[cc - C.sy] < 0 && phase[cc] < 0);
+    vst[c] = o.s;
+    conv[c] = o.sc;
+    diff[c] = o.sd;
+    v110 = v111; v111 = v112;
+    w101 = w102; w111 = w112;
+    c += Y.sz; cu += X.sz; cw += Z.sz; cc += C.sz;
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -281,17 +305,44 @@ w_star_kernel(DevDom D, StarArgs A, const double *__restrict__ u0,
   double *__restrict__ wst, double *__restrict__ conv,
   double *__restrict__ diff)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  int k = blockIdx.z + 1;
-  if (i > D.fz.ni || j > D.fz.nj) return;
-  w_star_cell(D, A, u0, v0, w0, p, conv0, diff0, phase, sb, wst, conv,
-    diff, i, j, k);
+  const Grid &X = D.fx, &Y = D.fy, &Z = D.fz, &C = D.cc;
+  const StarCol S = star_column(Z);
+  if (!S.ok) return;
+  long c = gidx(Z, S.i, S.j, S.k0), cu = gidx(X, S.i, S.j, S.k0);
+  long cv = gidx(Y, S.i, S.j, S.k0), cc = gidx(C, S.i, S.j, S.k0);
+  double w110 = w0[c - Z.sz], w111 = w0[c];
+  double u110 = u0[cu - X.sz], u210 = u0[cu - X.sz + 1];
+  double v110 = v0[cv - Y.sz], v120 = v0[cv - Y.sz + Y.sy];
+  double pm = p[cc - C.sz];
+  int phm = phase[cc - C.sz];
+  double sbm = sb ? sb[cc - C.sz] : 0.;
+  for (int k = S.k0; k <= S.k1; k++) {
+    const double w112 = w0[c + Z.sz];
+    const double u111 = u0[cu], u211 = u0[cu + 1];
+    const double v111 = v0[cv], v121 = v0[cv + Y.sy];
+    const double pc = p[cc];
+    const int phc = phase[cc];
+    const double sbc = sb ? sb[cc] : 0.;
+    StarOut o = w_star_math(A, w0[c - 1], w111, w0[c + 1], w0[c - Z.sy],
+      w0[c + Z.sy], w110, w112, u110, u111, u210, u211, v110, v111, v120,
+      v121, pm, pc, conv0[c], diff0[c], sb != NULL, sbm, sbc,
+      phm < 0 && phc < 0);
+    wst[c] = o.s;
+    conv[c] = o.sc;
+    diff[c] = o.sd;
+    w110 = w111; w111 = w112;
+    u110 = u111; u210 = u211;
+    v110 = v111; v120 = v121;
+    pm = pc; phm = phc; sbm = sbc;
+    c += Z.sz; cu += X.sz; cv += Y.sz; cc += C.sz;
+  }
 }
 
 /* (A kernel forming u*, v* and w* in one pass, reading u0, v0, w0, p and
  * phase once, took 884 us at 256^3 against 3 x 292 us for the three
- * kernels, which share those reads through L2.) */
+ * kernels, which share those reads through L2.  At 256^3 the three
+ * kernels take 303-321 us each whether a thread owns 1, 4 or 8 planes
+ * and with or without non-temporal stores.) */
 static StarArgs star_args(double f, double gc)
 {
   StarArgs A;
@@ -310,22 +361,17 @@ static StarArgs star_args(double f, double gc)
 extern "C" void cuda_U_star(void)
 {
   /* src/cuda_bluebottle.cu:2095-2108 */
-  dim3 bl(64, 4, 1);
   const DevDom &D = ddom;
   const double *sb = (SCALAR >= 1 && bb_bous_on) ? _s : NULL;
-  hipLaunchKernelGGL(u_star_kernel, dim3((D.fx.ni + 63) / 64,
-    (D.fx.nj + 3) / 4, D.fx.nk), bl, 0, bb_stream, D,
-    star_args(bb_force[0], g.x),
-    _u0, _v0, _w0, _p0, _conv0_u, _diff0_u, _phase, sb, _u_star, _conv_u,
-    _diff_u);
-  hipLaunchKernelGGL(v_star_kernel, dim3((D.fy.ni + 63) / 64,
-    (D.fy.nj + 3) / 4, D.fy.nk), bl, 0, bb_stream, D, star_args(bb_force[1], g.y),
-    _u0, _v0, _w0, _p0, _conv0_v, _diff0_v, _phase, sb, _v_star, _conv_v,
-    _diff_v);
-  hipLaunchKernelGGL(w_star_kernel, dim3((D.fz.ni + 63) / 64,
-    (D.fz.nj + 3) / 4, D.fz.nk), bl, 0, bb_stream, D, star_args(bb_force[2], g.z),
-    _u0, _v0, _w0, _p0, _conv0_w, _diff0_w, _phase, sb, _w_star, _conv_w,
-    _diff_w);
+  hipLaunchKernelGGL(u_star_kernel, dim3(star_blocks(D.fx)), dim3(256), 0,
+    bb_stream, D, star_args(bb_force[0], g.x), _u0, _v0, _w0, _p0, _conv0_u,
+    _diff0_u, _phase, sb, _u_star, _conv_u, _diff_u);
+  hipLaunchKernelGGL(v_star_kernel, dim3(star_blocks(D.fy)), dim3(256), 0,
+    bb_stream, D, star_args(bb_force[1], g.y), _u0, _v0, _w0, _p0, _conv0_v,
+    _diff0_v, _phase, sb, _v_star, _conv_v, _diff_v);
+  hipLaunchKernelGGL(w_star_kernel, dim3(star_blocks(D.fz)), dim3(256), 0,
+    bb_stream, D, star_args(bb_force[2], g.z), _u0, _v0, _w0, _p0, _conv0_w,
+    _diff0_w, _phase, sb, _w_star, _conv_w, _diff_w);
   BB_LAUNCH_CHECK();
   bb_history_rotated = 0;
 }

@@ -879,50 +879,71 @@ jacobi_sweep_kernel(DevDom D, Coef K, InvMTab T, double omega,
 #define GTERMS (GMAX + GMAX * (GMAX + 1) / 2)
 struct GVecs { const double *v[GMAX]; };
 
+template <int M>
 __global__ void __launch_bounds__(TILE_THREADS)
 gram_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid, GVecs V,
-  int m, const double *__restrict__ rhs, double *__restrict__ part)
+  const double *__restrict__ rhs, double *__restrict__ part)
 {
   TILE_PROLOGUE
-  double a[GTERMS];
+  constexpr int T = M + M * (M + 1) / 2;
+  double a[T];
 #pragma unroll
-  for (int t = 0; t < GTERMS; t++) a[t] = 0.;
+  for (int t = 0; t < T; t++) a[t] = 0.;
   if (colok) {
+    const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+    const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+    const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+    const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+#pragma unroll 1
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       if (solid(fluid, c)) continue;
-      double vc[GMAX], Av[GMAX];
+      const int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+      const long cW = gidx(C, xm, j, k), cE = gidx(C, xp, j, k);
+      const long cS = gidx(C, x, jm, k), cN = gidx(C, x, jp, k);
+      const long cB = gidx(C, x, j, km), cT = gidx(C, x, j, kp);
+      const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+      double vc[M], Av[M];
 #pragma unroll
-      for (int q = 0; q < GMAX; q++) {
-        if (q < m) {
-          vc[q] = V.v[q][c];
-          Av[q] = neg_lap_at(D, K, V.v[q], x, j, k);
-        }
+      for (int q = 0; q < M; q++) {
+        const double *f = V.v[q];
+        vc[q] = f[c];
+        Av[q] = neg_lap7(K, vc[q], f[cW], f[cE], f[cS], f[cN], f[cB], f[cT],
+                         wW, wE, wS, wN, wB, wT);
       }
       const double b = rhs[c];
       int t = 0;
 #pragma unroll
-      for (int q = 0; q < GMAX; q++, t++)
-        if (q < m) a[t] += b * vc[q];
+      for (int q = 0; q < M; q++, t++) a[t] += b * vc[q];
 #pragma unroll
-      for (int p = 0; p < GMAX; p++)
+      for (int p = 0; p < M; p++)
 #pragma unroll
-        for (int q = p; q < GMAX; q++, t++)
-          if (q < m) a[t] += vc[p] * Av[q];
+        for (int q = p; q < M; q++, t++) a[t] += vc[p] * Av[q];
     }
   }
-  /* per-workgroup sums: wavefront trees, then the 8 rows */
-  __shared__ double s[TILE_Y][GTERMS];
+  /* per-workgroup sums: wavefront trees, then the 8 rows; the terms go
+   * to the rows of `part` in the GMAX layout */
+  __shared__ double s[TILE_Y][T];
 #pragma unroll
-  for (int t = 0; t < GTERMS; t++) {
+  for (int t = 0; t < T; t++) {
     double v = wave_tree_sum(a[t]);
     if (lane == 0) s[wv][t] = v;
   }
   __syncthreads();
-  if (threadIdx.x < GTERMS) {
+  if (threadIdx.x < T) {
+    const int t = threadIdx.x;
+    int row;
+    if (t < M) {
+      row = t;
+    } else {                            /* pair (p, q), p <= q < M */
+      int r = t - M, p = 0;
+      while (r >= M - p) { r -= M - p; p++; }
+      const int q = p + r;
+      row = GMAX + p * GMAX - p * (p - 1) / 2 + (q - p);
+    }
     double v = 0.;
-    for (int w = 0; w < TILE_Y; w++) v += s[w][threadIdx.x];
-    part[(long) threadIdx.x * gridDim.x + blockIdx.x] = v;
+    for (int w = 0; w < TILE_Y; w++) v += s[w][t];
+    part[(long) row * gridDim.x + blockIdx.x] = v;
   }
 }
 
@@ -943,6 +964,14 @@ gram_sum_kernel(const double *__restrict__ part, long n,
   }
   if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
+
+__global__ void zero_rows_kernel(double *__restrict__ a, long n)
+{
+  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = 0.;
+}
+
+static unsigned nblk_zero(long n) { return (unsigned) ((n + 255) / 256); }
 
 /* phi = sum_q c_q v_q over the fluid cells, 0 in particle cells. */
 __global__ void __launch_bounds__(TILE_THREADS)
@@ -1340,8 +1369,15 @@ double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
   }
   GVecs V;
   for (int q = 0; q < GMAX; q++) V.v[q] = q < m ? vecs[q] : vecs[0];
-  hipLaunchKernelGGL(gram_kernel, TILES_LAUNCH, ddom, coef(), fluid_mask(),
-    V, m, _rhs_p, _gram_part);
+  /* rows of the terms a smaller m leaves out are summed too: zero them */
+  hipLaunchKernelGGL(zero_rows_kernel, dim3(nblk_zero(GTERMS * nt)),
+    dim3(256), 0, bb_stream, _gram_part, (long) GTERMS * nt);
+  switch (m) {
+#define GK(M) case M: hipLaunchKernelGGL(gram_kernel<M>, TILES_LAUNCH, ddom, \
+    coef(), fluid_mask(), V, _rhs_p, _gram_part); break;
+    GK(1) GK(2) GK(3) GK(4) GK(5) GK(6)
+#undef GK
+  }
   hipLaunchKernelGGL(gram_sum_kernel, dim3(GTERMS), dim3(256), 0, bb_stream,
     _gram_part, nt, _gram_out);
   BB_CHECK(hipMemcpyAsync(gram_host, _gram_out, GTERMS * sizeof(double),
@@ -1581,6 +1617,7 @@ static void pcg_plain(double ts, int warm)
         rank, RHS_TOL);
     pp_last.niter = 0; pp_last.resid = 0.; pp_last.true_resid = 0.;
     pp_last.etime = etime;
+    log_solve(0);
     return;
   }
   int q = 0;
@@ -1841,6 +1878,7 @@ static void pcg_mixed(double ts, int warm)
         rank, RHS_TOL);
     pp_last.niter = 0; pp_last.resid = 0.; pp_last.true_resid = 0.;
     pp_last.etime = etime;
+    log_solve(0);
     return;
   }
   int q = 0, first_q = 1, nouter = 1;

@@ -128,6 +128,10 @@ static int get_bc(const char *key, double *a, double *b)
 static void o_scalar_parse(void);
 static int O_scalar;                 /* SCALAR (scalar.c) */
 static int O_lamb_max_iter;
+/* 1: a solve stopped at pp_max_iter lets the timestep go on (projection,
+ * pressure update, Physalis, motion) instead of ending it; used only to
+ * time the components of a timestep on a bounded sample. */
+static int O_pcg_continue = 0;
 static double O_lamb_residual, O_lamb_relax, O_lamb_cut;
 static double O_sbeta, O_sref, O_g[3], O_gm[3], O_ga[3];
 
@@ -1028,7 +1032,7 @@ int orc_pcg(void)
       sp_rq = sp_rq1;
     }
   }
-  if (q > O.pp_max_iter) {
+  if (q > O.pp_max_iter && !O_pcg_continue) {
     fprintf(stderr, "oracle: PCG did not converge in %d iterations\n", q);
     O.niter = -1;
     return -1;
@@ -1322,4 +1326,5 @@ void orc_set_scalar(const char *n, double v)
   else if (strcmp(n, "ttime") == 0) O.ttime = v;
   else if (strcmp(n, "pp_max_iter") == 0) O.pp_max_iter = (int) v;
   else if (strcmp(n, "lamb_max_iter") == 0) O_lamb_max_iter = (int) v;
+  else if (strcmp(n, "pcg_continue") == 0) O_pcg_continue = (int) v;
 }

@@ -1,4 +1,6 @@
 """Comparison helpers shared by the parity tests."""
+import ctypes
+
 import numpy as np
 
 
@@ -81,4 +83,78 @@ def field_bounds(b, fields_ref):
             out[f] = b["grad"] * U / max(nc, 1e-300)
         elif f in ("p", "phi"):
             out[f] = b["phi"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Reader of the reference's restart file: block order of
+# src/domain.c:3025-3086; host arrays indexed as src/domain.c:918-1487 lays
+# them out (Gcc and Gfz x-fastest, Gfx y-fastest, Gfy z-fastest, one ghost
+# layer, the face grid one longer in its own direction).
+
+def restart_grid(n, kind):
+    nx, ny, nz = n
+    ib, jb, kb = nx + 2, ny + 2, nz + 2
+    if kind == "x":
+        ib += 1
+    elif kind == "y":
+        jb += 1
+    elif kind == "z":
+        kb += 1
+    return ib, jb, kb
+
+
+def restart_to_ijk(flat, n, kind):
+    """[i, j, k] view of a host array in the reference layout."""
+    ib, jb, kb = restart_grid(n, kind)
+    if kind == "x":                     # j + k jb + i jb kb
+        return flat.reshape(ib, kb, jb).transpose(0, 2, 1)
+    if kind == "y":                     # k + i kb + j kb ib
+        return flat.reshape(jb, ib, kb).transpose(1, 0, 2)
+    return flat.reshape(kb, jb, ib).transpose(2, 1, 0)   # x-fastest
+
+
+def read_restart(path, n, nparts, scalar):
+    """Dict of the blocks of a restart file (fields as [i, j, k] arrays,
+    particles as part_struct records); asserts the file ends there."""
+    import bbhip
+    data = open(path, "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(data, dtype=dtype, count=count, offset=off)
+        off += a.nbytes
+        return a
+
+    out = {}
+    out["ttime"], out["dt0"], out["dt"] = take("<f8", 3)
+    out["stepnum"], out["vtk_stepnum"] = take("<i4", 2)
+    out["clocks"] = take("<f8", 3)
+    for comp, kind in (("u", "x"), ("v", "y"), ("w", "z")):
+        size = int(np.prod(restart_grid(n, kind)))
+        for name in (comp, comp + "0", "diff0_" + comp, "conv0_" + comp,
+                     "diff_" + comp, "conv_" + comp, comp + "_star"):
+            out[name] = restart_to_ijk(take("<f8", size), n, kind)
+    size = int(np.prod(restart_grid(n, "c")))
+    for name in ("p", "phi", "p0"):
+        out[name] = restart_to_ijk(take("<f8", size), n, "c")
+    for name in ("phase", "phase_shell"):
+        out[name] = restart_to_ijk(take("<i4", size), n, "c")
+    for name, kind in (("flag_u", "x"), ("flag_v", "y"), ("flag_w", "z")):
+        out[name] = restart_to_ijk(
+            take("<i4", int(np.prod(restart_grid(n, kind)))), n, kind)
+    out["nparts"] = int(take("<i4", 1)[0])
+    assert out["nparts"] == nparts
+    psz = ctypes.sizeof(bbhip.PartStruct)
+    out["parts"] = [bbhip.PartStruct.from_buffer_copy(data, off + k * psz)
+                    for k in range(nparts)]
+    off += nparts * psz
+    out["ncoeffs_max"] = int(take("<i4", 1)[0])
+    out["pid_int"], out["pid_back"], out["gradPz"] = take("<f8", 3)
+    if scalar:
+        for name in ("s", "s0", "s_conv", "s_conv0", "s_diff", "s_diff0"):
+            out[name] = restart_to_ijk(take("<f8", size), n, "c")
+        out["s_ncoeffs_max"] = int(take("<i4", 1)[0])
+    assert off == len(data), (off, len(data))
     return out

@@ -244,3 +244,87 @@ def test_rejects_out_of_scope_options(tmp_path):
     r = subprocess.run([sys.executable, "-c", code], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0 and "OK" in r.stdout, r.stderr
+
+
+def test_restart_file_layout(tmp_path):
+    """out_restart (no device involved: host arrays supplied here) writes
+    the reference's blocks in the reference's order and layouts: every
+    array is filled with (its number in the file) * 1e6 + its host index,
+    and tests/helpers.read_restart, which maps [i, j, k] to the host index
+    of src/domain.c (Gfx y-fastest, Gfy z-fastest, Gcc and Gfz
+    x-fastest), must find each value where that formula puts it; the
+    particle and scalar blocks and the scalars around them included."""
+    n = (6, 5, 4)
+    kw = bbconfig.channel(n=n)
+    kw["scalar"] = dict(s_D=0.1, s_k=1.0, s_beta=0.1, s_ref=0.0,
+                        init_s=(0.0, 0.0),
+                        bc_s={f: ("PERIODIC",) for f in "WESNBT"})
+    bbconfig.write_case(str(tmp_path), **kw)
+    extra = textwrap.dedent("""
+        import numpy as np
+        sys.path.insert(0, {tests!r})
+        nx, ny, nz = {n!r}
+        sizes = {{"x": (nx + 3) * (ny + 2) * (nz + 2),
+                  "y": (nx + 2) * (ny + 3) * (nz + 2),
+                  "z": (nx + 2) * (ny + 2) * (nz + 3),
+                  "c": (nx + 2) * (ny + 2) * (nz + 2)}}
+        order = []
+        for comp, kind in (("u", "x"), ("v", "y"), ("w", "z")):
+            for nm in (comp, comp + "0", "diff0_" + comp, "conv0_" + comp,
+                       "diff_" + comp, "conv_" + comp, comp + "_star"):
+                order.append((nm, kind, "d"))
+        order += [("p", "c", "d"), ("phi", "c", "d"), ("p0", "c", "d"),
+                  ("phase", "c", "i"), ("phase_shell", "c", "i"),
+                  ("flag_u", "x", "i"), ("flag_v", "y", "i"),
+                  ("flag_w", "z", "i")]
+        order += [(nm, "c", "d") for nm in ("s", "s0", "s_conv", "s_conv0",
+                                            "s_diff", "s_diff0")]
+        keep = []
+        for num, (nm, kind, t) in enumerate(order):
+            m = sizes[kind]
+            if t == "d":
+                a = (num * 1e6 + np.arange(m)).astype(np.float64)
+            else:
+                a = (num * 100000 + np.arange(m)).astype(np.int32)
+            keep.append(a)
+            ctypes.c_void_p.in_dll(lib, nm).value = a.ctypes.data
+        parts = (bbhip.PartStruct * 2)()
+        for k in range(2):
+            parts[k].N = k
+            parts[k].x = 1.5 + k
+            parts[k].pnm_re[3] = 7.0 + k
+        ctypes.c_void_p.in_dll(lib, "parts").value = ctypes.addressof(parts)
+        ctypes.c_int.in_dll(lib, "nparts").value = 2
+        ctypes.c_int.in_dll(lib, "ncoeffs_max").value = 15
+        ctypes.c_int.in_dll(lib, "s_ncoeffs_max").value = 4
+        ctypes.c_double.in_dll(lib, "ttime").value = 0.25
+        ctypes.c_double.in_dll(lib, "dt").value = 0.125
+        ctypes.c_int.in_dll(lib, "stepnum").value = 7
+        lib.out_restart()
+        from helpers import read_restart
+        f = read_restart(os.path.join("input", "restart.config-0"),
+                         (nx, ny, nz), 2, True)
+        ib = {{"x": nx + 3, "y": nx + 2, "z": nx + 2, "c": nx + 2}}
+        jb = {{"x": ny + 2, "y": ny + 3, "z": ny + 2, "c": ny + 2}}
+        kb = {{"x": nz + 2, "y": nz + 2, "z": nz + 3, "c": nz + 2}}
+        bad = 0
+        for num, (nm, kind, t) in enumerate(order):
+            a = f[nm]
+            I, J, K = np.meshgrid(np.arange(ib[kind]), np.arange(jb[kind]),
+                                  np.arange(kb[kind]), indexing="ij")
+            if kind == "x":
+                idx = J + K * jb[kind] + I * jb[kind] * kb[kind]
+            elif kind == "y":
+                idx = K + I * kb[kind] + J * kb[kind] * ib[kind]
+            else:
+                idx = I + J * ib[kind] + K * ib[kind] * jb[kind]
+            want = num * (1e6 if t == "d" else 100000) + idx
+            bad += int(not np.array_equal(a, want.astype(a.dtype)))
+        print("LAYOUT", bad, f["stepnum"], f["ttime"], f["dt"],
+              [p.N for p in f["parts"]], [p.x for p in f["parts"]],
+              f["parts"][1].pnm_re[3], f["ncoeffs_max"], f["s_ncoeffs_max"])
+    """).format(tests=os.path.join(ROOT, "tests"), n=n)
+    r = run_parse(str(tmp_path), extra)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith("LAYOUT")][0]
+    assert line == ("LAYOUT 0 7 0.25 0.125 [0, 1] [1.5, 2.5] 8.0 15 4"), line

@@ -15,7 +15,7 @@ import pytest
 
 import bbconfig
 import bbhip
-from helpers import bitwise_equal, describe, interior
+from helpers import bitwise_equal, describe, interior, read_restart
 
 pytestmark = pytest.mark.gpu
 
@@ -102,3 +102,95 @@ def test_restart_round_trip(case_dir):
     for pa, pb in zip(rest["parts"], full["parts"]):
         for k in PART_KEYS:
             assert np.array_equal(np.asarray(pa[k]), np.asarray(pb[k])), k
+
+
+# ---------------------------------------------------------------------------
+# The file against the FP64 oracle: read by helpers.read_restart (the
+# reference's block order and layouts), its fields and particles compared
+# with the oracle's state after the same two steps, then the run restarted
+# from it compared with the oracle's next two steps.
+
+def test_restart_file_against_oracle(case_dir):
+    """After two steps the GPU's restart file, read by the reader above,
+    holds the oracle's fields, flags and particles (1e-10; integers equal);
+    restarted from it, the GPU's next two steps match the oracle's."""
+    import orc
+    from helpers import rel_l2
+    kw, parts = case()
+    bbconfig.write_case(case_dir, **kw)
+    bbconfig.write_parts(case_dir, parts)
+    n = kw["n"]
+    run(case_dir, n, 2, write=True)
+    path = os.path.join(case_dir, "input", "restart.config-0")
+    f = read_restart(path, n, len(parts), True)
+    fields = ("u", "v", "w", "p", "phi", "u0", "v0", "w0", "p0", "conv0_u",
+              "diff0_v", "s", "s0")
+    with orc.Oracle(case_dir) as o:
+        o.step(2)
+        assert int(f["stepnum"]) == 2
+        for q in ("dt", "ttime"):         # dt follows the velocities
+            assert abs(f[q] - o.scalar(q)) <= 1e-12 * abs(o.scalar(q)), q
+        worst = 0.
+        for name in fields:
+            a, b = interior(f[name]), interior(np.array(o.field(name)))
+            e = rel_l2(a, b)
+            worst = max(worst, e)
+            assert e <= 1e-10, describe(name, a, b)
+        for name in ("phase", "phase_shell", "flag_u", "flag_v", "flag_w"):
+            assert np.array_equal(interior(f[name]),
+                                  interior(np.array(o.field(name)))), name
+        for k in range(len(parts)):
+            want = o.part_state(k)
+            for q in ("x", "y", "z", "u", "v", "w", "ox", "Fz", "Lx"):
+                a, b = getattr(f["parts"][k], q), want[q]
+                assert abs(a - b) <= 1e-10 * max(abs(b), 1e-300), (k, q)
+            assert np.allclose(np.array(f["parts"][k].pnm_re[:]),
+                               want["pnm_re"], rtol=1e-9, atol=1e-12)
+        print("restart file vs oracle: worst field rel L2", f"{worst:.2e}")
+        o.step(2)
+        want = {name: np.array(o.field(name)) for name in FIELDS}
+        lamb_orc = int(o.scalar("lamb_iters"))
+    rest = run(case_dir, n, 2, restart=True)
+    for name in FIELDS:
+        a, b = interior(rest[name]), interior(want[name])
+        e = rel_l2(a, b)
+        print(name, f"after restart vs oracle rel L2 {e:.2e}")
+        assert e <= 1e-10, describe(name, a, b)
+    assert rest["stepnum"] == 4
+    assert lamb_orc > 0
+
+
+def test_rec_logs(case_dir):
+    """solver_expd.rec holds one line per pressure solve with its step and
+    iteration count as the library counted them and a residual at the
+    stopping test; iter_etime.rec holds one line per step with the Lamb
+    iteration count of the oracle's same step."""
+    import orc
+    kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=3, a=4.0, seed=7)
+    bbconfig.write_case(case_dir, **kw)
+    bbconfig.write_parts(case_dir, parts)
+    lib = bbhip.load()
+    slog = (ctypes.c_int * 8192).in_dll(lib, "bb_solve_log")
+    solves = ctypes.c_long.in_dll(lib, "bb_pcg_solves_total")
+    with orc.Oracle(case_dir) as o:
+        lamb_orc = []
+        for _ in range(3):
+            o.step()
+            lamb_orc.append(int(o.scalar("lamb_iters")))
+    expect = []
+    with bbhip.Simulation(case_dir, kw["n"], record=True) as s:
+        for step in range(1, 4):
+            n0 = solves.value
+            s.step()
+            expect += [(step, slog[m % 8192]) for m in range(n0, solves.value)]
+    rec = os.path.join(case_dir, "record")
+    rows = [l.split() for l in open(os.path.join(rec, "solver_expd.rec"))
+            if l.strip()][1:]
+    got = [(int(r[0]), int(r[3])) for r in rows]
+    assert got == expect
+    assert all(float(r[4]) <= 1e-8 for r in rows if int(r[3]) > 0)
+    rows = [l.split() for l in open(os.path.join(rec, "iter_etime.rec"))
+            if l.strip()][1:]
+    assert [int(r[0]) for r in rows] == [1, 2, 3]
+    assert [int(r[2]) for r in rows] == lamb_orc
+    print("solves logged", len(got), "Lamb iterations per step", lamb_orc)
