@@ -118,6 +118,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20260103)
     ap.add_argument("--train-solves", type=int, default=None,
                     help="solves that feed the network's online training")
+    ap.add_argument("--history", type=int, default=None,
+                    help="earlier solutions in the span of the history and "
+                         "learned forms (default 4)")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
@@ -272,7 +275,7 @@ def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None):
     }
 
 
-def roofline_of(ktimes, cells, copy_gbs, mask):
+def roofline_of(ktimes, cells, copy_gbs, mask, stream4_gbs=None):
     """Live roofline of the kernel with the most timed time: algorithmic
     bytes per launch over its average launch time (HIP events on the
     library stream)."""
@@ -294,6 +297,10 @@ def roofline_of(ktimes, cells, copy_gbs, mask):
         "frac": achieved / HBM_PEAK_GBS,
         "frac_of_measured_copy": achieved / copy_gbs,
         "measured_copy_gbs": copy_gbs,
+        "frac_of_stream4": (achieved / stream4_gbs
+                            if stream4_gbs and dom_name.endswith("f32")
+                            else None),
+        "stream4_gbs": stream4_gbs,
         "traffic": tpc * cells if (tpc and mask) else None,
         "bytes_per_cell": bpc,
         "algorithmic_bytes_per_launch": bpc * cells,
@@ -362,9 +369,15 @@ def main():
     lib.bb_stream_copy_gbs.restype = ctypes.c_double
     lib.bb_stream_copy_gbs.argtypes = [ctypes.c_long, ctypes.c_int]
     copy_gbs = lib.bb_stream_copy_gbs(1 << 27, 10)
+    lib.bb_stream4_gbs.restype = ctypes.c_double
+    lib.bb_stream4_gbs.argtypes = [ctypes.c_long, ctypes.c_int]
+    cells = a.n[0] * a.n[1] * a.n[2]
+    stream4_gbs = lib.bb_stream4_gbs(cells, 10)
     if a.calib_only:
         print(json.dumps({"stream_copy_gbs": copy_gbs,
-                          "elements": 1 << 27}))
+                          "elements": 1 << 27,
+                          "stream4_gbs": stream4_gbs,
+                          "stream4_elements": cells}))
         return
     for name, val in (("bb_spmv_variant", a.spmv_variant),
                       ("bb_spmv_tzb", a.tzb),
@@ -395,10 +408,11 @@ def main():
                    else 1)
     nwarm = None
     for key, prec, warm, use_net, hist in order:
+        if a.history is not None and hist > 1:
+            hist = a.history
         res[key] = run_form(lib, a, prec, warm, use_net, hist, nwarm)
         if nwarm is None:
             nwarm = res[key]["warmup_steps"]
-    cells = a.n[0] * a.n[1] * a.n[2]
     mask = a.workload == "sediment" and a.nparts > 0
     res = {f[0]: res[f[0]] for f in forms}
     head = res[forms[-1][0]]
@@ -438,11 +452,12 @@ def main():
                                  if plain and head is not plain else None),
             "spmv_variant": ctypes.c_int.in_dll(lib, "bb_spmv_variant").value,
         },
-        "roofline": roofline_of(head["ktimes"], cells, copy_gbs, mask),
+        "roofline": roofline_of(head["ktimes"], cells, copy_gbs, mask,
+                                stream4_gbs),
     }
     if plain is not None and plain is not head:
         out["roofline_plain"] = roofline_of(plain["ktimes"], cells, copy_gbs,
-                                            mask)
+                                            mask, stream4_gbs)
     if not a.no_cpu:
         ref = plain or head
         out["cpu_baseline"] = cpu_baseline(a, ref["pcg_iters_per_step"],
