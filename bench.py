@@ -124,8 +124,21 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
-                    help="search kernel: 2 z-columns through LDS planes "
-                         "(default), 0 direct neighbour loads")
+                    help="search kernel: 3 one wavefront per x row where "
+                         "the grid allows it, else 2 (default); 2 z-columns "
+                         "through LDS planes; 0 direct neighbour loads")
+    ap.add_argument("--row-ry", type=int, default=None,
+                    help="rows per workgroup of the FP32 row kernel (8 or "
+                         "16, default 8)")
+    ap.add_argument("--row-zc", type=int, default=None,
+                    help="planes per workgroup of the row kernel (multiple "
+                         "of 8 up to 64; 0: automatic)")
+    ap.add_argument("--row-depth", type=int, default=None,
+                    help="planes of loads in flight in the FP32 row kernel "
+                         "(2, 3 or 4; default 2)")
+    ap.add_argument("--row-fp64", type=int, default=None,
+                    help="1: the FP64 search uses the row kernel too "
+                         "(default 0)")
     ap.add_argument("--tzb", type=int, default=None,
                     help="tiles per z-column of the search kernel "
                          "(0: automatic)")
@@ -386,6 +399,10 @@ def main():
                       ("bb_spmv_nts", a.spmv_nts),
                       ("bb_spmv_nts32", a.spmv_nts32),
                       ("bb_spmv_tzb32", a.tzb32),
+                      ("bb_row_ry", a.row_ry),
+                      ("bb_row_zc", a.row_zc),
+                      ("bb_row_depth", a.row_depth),
+                      ("bb_row_fp64", a.row_fp64),
                       ("bb_pcg_batch", a.batch),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:

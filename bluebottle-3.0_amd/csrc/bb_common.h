@@ -58,12 +58,25 @@
 #define REDUCE_THREADS 1024
 
 /* A ghosted x-fastest grid.  ni/nj/nk count the interior points (the
- * reference's in/jn/kn); sy/sz are the strides of the ghosted array. */
+ * reference's in/jn/kn), ib/jb/kb the points with ghosts; sy/sz are the
+ * strides of the device array.  The row stride is ib rounded up to a
+ * multiple of 32 elements and every device grid array starts 31 elements
+ * into its allocation (bb_grid_alloc), so point x = 1 of every row sits on
+ * a 128-byte boundary for 4-byte elements and a 256-byte boundary for
+ * 8-byte ones: a wavefront's 64 consecutive elements of a row fill whole
+ * cache lines (unaligned rows cost the streaming kernels a third of their
+ * bandwidth at 256^3).  The points i >= ib of a row are padding. */
 struct Grid {
   int ni, nj, nk;
+  int ib, jb, kb;
   int sy, sz;
-  long n;                             /* elements including ghosts */
+  long n;                             /* elements of the device array */
 };
+
+#define GRID_ROW_ALIGN 32             /* row stride multiple (elements) */
+#define GRID_ORIGIN 31                /* leading elements of an array */
+void *bb_grid_alloc(long n, size_t esize);
+void bb_grid_free(void *p);
 
 __host__ __device__ inline long gidx(const Grid &G, int i, int j, int k)
 {
@@ -226,20 +239,25 @@ __device__ inline void tile_partial_all(double acc, double *__restrict__ gpart,
                 ((s_w2[4] + s_w2[5]) + (s_w2[6] + s_w2[7]));
 }
 
-/* Step 4 inside a TILE_THREADS (512) workgroup: thread t plays accumulators
- * t and t + 512.  Returns the total to every thread. */
+/* Step 4 inside a workgroup of NT >= TILE_THREADS (512) threads: thread
+ * t < 512 plays accumulators t and t + 512, the others only take part in
+ * the barriers.  Returns the total to every thread. */
+template <int NT = TILE_THREADS>
 __device__ inline double tile_block_sum(const double *__restrict__ part, long n)
 {
+  static_assert(NT >= TILE_THREADS, "tile_block_sum needs 512 threads");
   __shared__ double s[TILE_THREADS];
   __shared__ double s_tot;
   const long RT = REDUCE_THREADS;
   int t = threadIdx.x;
-  double a = 0.0, b = 0.0;
+  if (NT == TILE_THREADS || t < TILE_THREADS) {
+    double a = 0.0, b = 0.0;
 #pragma unroll 4
-  for (long q = t; q < n; q += RT) a += part[q];
+    for (long q = t; q < n; q += RT) a += part[q];
 #pragma unroll 4
-  for (long q = t + TILE_THREADS; q < n; q += RT) b += part[q];
-  s[t] = a + b;
+    for (long q = t + TILE_THREADS; q < n; q += RT) b += part[q];
+    s[t] = a + b;
+  }
   __syncthreads();
   for (int off = TILE_THREADS / 2; off >= WAVE; off >>= 1) {
     if (t < off) s[t] += s[t + off];

@@ -68,10 +68,49 @@ static Grid make_grid(const grid_info &G)
 {
   Grid g;
   g.ni = G.in; g.nj = G.jn; g.nk = G.kn;
-  g.sy = G.inb;
-  g.sz = G.inb * G.jnb;
-  g.n = (long) G.inb * G.jnb * G.knb;
+  g.ib = G.inb; g.jb = G.jnb; g.kb = G.knb;
+  g.sy = (G.inb + GRID_ROW_ALIGN - 1) / GRID_ROW_ALIGN * GRID_ROW_ALIGN;
+  g.sz = g.sy * G.jnb;
+  g.n = (long) g.sz * G.knb;
   return g;
+}
+
+/* Device arrays on a grid: n elements of esize bytes starting GRID_ORIGIN
+ * elements into a zeroed allocation (the allocation holds GRID_ORIGIN + 1
+ * spare elements, so index n is still inside).  The table maps the
+ * returned pointer back to the allocation for bb_grid_free. */
+#define GRID_ALLOCS 512
+static struct { void *user, *raw; } grid_tab[GRID_ALLOCS];
+
+void *bb_grid_alloc(long n, size_t esize)
+{
+  size_t bytes = (size_t) (n + GRID_ORIGIN + 1) * esize;
+  char *raw = NULL;
+  BB_CHECK(hipMalloc((void **) &raw, bytes));
+  BB_CHECK(hipMemset(raw, 0, bytes));
+  gpumem += bytes;
+  void *user = raw + (size_t) GRID_ORIGIN * esize;
+  for (int i = 0; i < GRID_ALLOCS; i++)
+    if (grid_tab[i].user == NULL) {
+      grid_tab[i].user = user;
+      grid_tab[i].raw = raw;
+      return user;
+    }
+  fprintf(stderr, "bb_grid_alloc: more than %d grid arrays\n", GRID_ALLOCS);
+  exit(EXIT_FAILURE);
+}
+
+void bb_grid_free(void *p)
+{
+  if (p == NULL) return;
+  for (int i = 0; i < GRID_ALLOCS; i++)
+    if (grid_tab[i].user == p) {
+      BB_CHECK(hipFree(grid_tab[i].raw));
+      grid_tab[i].user = grid_tab[i].raw = NULL;
+      return;
+    }
+  fprintf(stderr, "bb_grid_free: %p is not a grid array\n", p);
+  exit(EXIT_FAILURE);
 }
 
 static void *host_alloc(size_t bytes)
@@ -147,34 +186,33 @@ extern "C" void cuda_dom_malloc_dev(void)
   ddom.pery = (bc.pS == PERIODIC);
   ddom.perz = (bc.pB == PERIODIC);
 
-  size_t cc = ddom.cc.n * sizeof(real);
-  size_t fx = ddom.fx.n * sizeof(real);
-  size_t fy = ddom.fy.n * sizeof(real);
-  size_t fz = ddom.fz.n * sizeof(real);
-  _p = (real *) dev_alloc(cc); _p0 = (real *) dev_alloc(cc);
-  _phi = (real *) dev_alloc(cc); _invM = (real *) dev_alloc(cc);
-  _rhs_p = (real *) dev_alloc(cc); _r_q = (real *) dev_alloc(cc);
-  _z_q = (real *) dev_alloc(cc); _p_q = (real *) dev_alloc(cc);
+  const long cc = ddom.cc.n, fx = ddom.fx.n, fy = ddom.fy.n, fz = ddom.fz.n;
+#define GA(n) ((real *) bb_grid_alloc(n, sizeof(real)))
+  _p = GA(cc); _p0 = GA(cc);
+  _phi = GA(cc); _invM = GA(cc);
+  _rhs_p = GA(cc); _r_q = GA(cc);
+  _z_q = GA(cc); _p_q = GA(cc);
   _pb_q = _p_q;                       /* one ghosted search direction */
-  _Apb_q = (real *) dev_alloc(cc);
-  _u = (real *) dev_alloc(fx); _u0 = (real *) dev_alloc(fx);
-  _v = (real *) dev_alloc(fy); _v0 = (real *) dev_alloc(fy);
-  _w = (real *) dev_alloc(fz); _w0 = (real *) dev_alloc(fz);
-  _u_star = (real *) dev_alloc(fx); _v_star = (real *) dev_alloc(fy);
-  _w_star = (real *) dev_alloc(fz);
-  _conv_u = (real *) dev_alloc(fx); _conv_v = (real *) dev_alloc(fy);
-  _conv_w = (real *) dev_alloc(fz);
-  _conv0_u = (real *) dev_alloc(fx); _conv0_v = (real *) dev_alloc(fy);
-  _conv0_w = (real *) dev_alloc(fz);
-  _diff_u = (real *) dev_alloc(fx); _diff_v = (real *) dev_alloc(fy);
-  _diff_w = (real *) dev_alloc(fz);
-  _diff0_u = (real *) dev_alloc(fx); _diff0_v = (real *) dev_alloc(fy);
-  _diff0_w = (real *) dev_alloc(fz);
-  _flag_u = (int *) dev_alloc(ddom.fx.n * sizeof(int));
-  _flag_v = (int *) dev_alloc(ddom.fy.n * sizeof(int));
-  _flag_w = (int *) dev_alloc(ddom.fz.n * sizeof(int));
-  _phase = (int *) dev_alloc(ddom.cc.n * sizeof(int));
-  _phase_shell = (int *) dev_alloc(ddom.cc.n * sizeof(int));
+  _Apb_q = GA(cc);
+  _u = GA(fx); _u0 = GA(fx);
+  _v = GA(fy); _v0 = GA(fy);
+  _w = GA(fz); _w0 = GA(fz);
+  _u_star = GA(fx); _v_star = GA(fy);
+  _w_star = GA(fz);
+  _conv_u = GA(fx); _conv_v = GA(fy);
+  _conv_w = GA(fz);
+  _conv0_u = GA(fx); _conv0_v = GA(fy);
+  _conv0_w = GA(fz);
+  _diff_u = GA(fx); _diff_v = GA(fy);
+  _diff_w = GA(fz);
+  _diff0_u = GA(fx); _diff0_v = GA(fy);
+  _diff0_w = GA(fz);
+#undef GA
+  _flag_u = (int *) bb_grid_alloc(fx, sizeof(int));
+  _flag_v = (int *) bb_grid_alloc(fy, sizeof(int));
+  _flag_w = (int *) bb_grid_alloc(fz, sizeof(int));
+  _phase = (int *) bb_grid_alloc(cc, sizeof(int));
+  _phase_shell = (int *) bb_grid_alloc(cc, sizeof(int));
 
   long rows = (long) ddom.ny * ddom.nz + n_tiles(ddom);
   long faces = (long) (ddom.nx + 1) * (ddom.ny + 1);
@@ -216,17 +254,29 @@ static const grid_info &ginfo(int kind)
   }
 }
 
+static const Grid &dgrid(int kind)
+{
+  switch (kind) {
+    case L_GFX: return ddom.fx;
+    case L_GFY: return ddom.fy;
+    case L_GFZ: return ddom.fz;
+    default:    return ddom.cc;
+  }
+}
+
+/* Host (reference layout, dense) to device (x-fastest, padded rows) and
+ * back; the padding is zero on the device. */
 template <typename T>
 static void push_field(const T *host, T *dev, int kind)
 {
   const grid_info &G = ginfo(kind);
-  size_t n = (size_t) G.s3b;
-  T *stage = (T *) malloc(n * sizeof(T));
-  long d = 0;
+  const Grid &D = dgrid(kind);
+  size_t n = (size_t) D.n;
+  T *stage = (T *) calloc(n, sizeof(T));
   for (int k = 0; k < G.knb; k++)
     for (int j = 0; j < G.jnb; j++)
       for (int i = 0; i < G.inb; i++)
-        stage[d++] = host[host_loc(G, kind, i, j, k)];
+        stage[gidx(D, i, j, k)] = host[host_loc(G, kind, i, j, k)];
   BB_CHECK(hipMemcpy(dev, stage, n * sizeof(T), hipMemcpyHostToDevice));
   free(stage);
 }
@@ -235,14 +285,14 @@ template <typename T>
 static void pull_field(T *host, const T *dev, int kind)
 {
   const grid_info &G = ginfo(kind);
-  size_t n = (size_t) G.s3b;
+  const Grid &D = dgrid(kind);
+  size_t n = (size_t) D.n;
   T *stage = (T *) malloc(n * sizeof(T));
   BB_CHECK(hipMemcpy(stage, dev, n * sizeof(T), hipMemcpyDeviceToHost));
-  long d = 0;
   for (int k = 0; k < G.knb; k++)
     for (int j = 0; j < G.jnb; j++)
       for (int i = 0; i < G.inb; i++)
-        host[host_loc(G, kind, i, j, k)] = stage[d++];
+        host[host_loc(G, kind, i, j, k)] = stage[gidx(D, i, j, k)];
   free(stage);
 }
 
@@ -392,9 +442,9 @@ extern "C" void cuda_dom_free(void)
     _u, _u0, _v, _v0, _w, _w0, _u_star, _v_star, _w_star,
     _conv_u, _conv_v, _conv_w, _conv0_u, _conv0_v, _conv0_w,
     _diff_u, _diff_v, _diff_w, _diff0_u, _diff0_v, _diff0_w};
-  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) dfree(dd[i]);
-  dfree(_flag_u); dfree(_flag_v); dfree(_flag_w);
-  dfree(_phase); dfree(_phase_shell);
+  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) bb_grid_free(dd[i]);
+  bb_grid_free(_flag_u); bb_grid_free(_flag_v); bb_grid_free(_flag_w);
+  bb_grid_free(_phase); bb_grid_free(_phase_shell);
   dfree(_bb_rows); dfree(_bb_rows2); dfree(_bb_scalars);
   hfree(bb_host_scalars);
   _p = _p0 = _phi = _invM = _rhs_p = _r_q = _z_q = _p_q = _pb_q = _Apb_q = NULL;
@@ -435,10 +485,13 @@ extern "C" void cuda_scalar_malloc_host(void)
 
 extern "C" void cuda_scalar_malloc_dev(void)
 {
-  size_t cc = ddom.cc.n * sizeof(real);
-  _s = (real *) dev_alloc(cc); _s0 = (real *) dev_alloc(cc);
-  _s_conv = (real *) dev_alloc(cc); _s_conv0 = (real *) dev_alloc(cc);
-  _s_diff = (real *) dev_alloc(cc); _s_diff0 = (real *) dev_alloc(cc);
+  const long cc = ddom.cc.n;
+  _s = (real *) bb_grid_alloc(cc, sizeof(real));
+  _s0 = (real *) bb_grid_alloc(cc, sizeof(real));
+  _s_conv = (real *) bb_grid_alloc(cc, sizeof(real));
+  _s_conv0 = (real *) bb_grid_alloc(cc, sizeof(real));
+  _s_diff = (real *) bb_grid_alloc(cc, sizeof(real));
+  _s_diff0 = (real *) bb_grid_alloc(cc, sizeof(real));
 }
 
 extern "C" void cuda_scalar_push(void)
@@ -476,7 +529,7 @@ extern "C" void cuda_scalar_free(void)
 {
   BB_CHECK(hipDeviceSynchronize());
   real *dd[] = {_s, _s0, _s_conv, _s_conv0, _s_diff, _s_diff0};
-  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) dfree(dd[i]);
+  for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); i++) bb_grid_free(dd[i]);
   real *hh[] = {s, s0, s_conv, s_conv0, s_diff, s_diff0};
   for (size_t i = 0; i < sizeof(hh) / sizeof(hh[0]); i++) hfree(hh[i]);
   _s = _s0 = _s_conv = _s_conv0 = _s_diff = _s_diff0 = NULL;

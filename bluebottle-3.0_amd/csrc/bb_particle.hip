@@ -412,6 +412,7 @@ __global__ void cage_flag_kernel(PDom P, const int *__restrict__ phase,
   if (t >= G.n) return;
   int i = (int) (t % G.sy), j = (int) ((t / G.sy) % (G.sz / G.sy)),
       k = (int) (t / G.sz);
+  if (i >= G.ib) return;                /* row padding */
   int a = dir == 0 ? i : (dir == 1 ? j : k);
   int amax = dir == 0 ? P.nx + 1 : (dir == 1 ? P.ny + 1 : P.nz + 1);
   if (a < 1 || a > amax) return;
@@ -2028,7 +2029,7 @@ extern "C" void cuda_part_malloc_dev(void)
   BB_CHECK(hipMalloc((void **) &_err_out, 2 * sizeof(double)));
   BB_CHECK(hipHostMalloc((void **) &err_host, 2 * sizeof(double),
     hipHostMallocDefault));
-  BB_CHECK(hipMalloc((void **) &_bb_fluid, ddom.cc.n));
+  _bb_fluid = (uint8_t *) bb_grid_alloc(ddom.cc.n, 1);
   BB_CHECK(hipMemset(_bb_fluid, 1, ddom.cc.n));
   gpumem += n * sizeof(part_struct) + ddom.cc.n;
 }
@@ -2059,7 +2060,7 @@ extern "C" void cuda_part_free(void)
   if (_pbin) BB_CHECK(hipFree(_pbin));
   if (_err_out) BB_CHECK(hipFree(_err_out));
   if (err_host) BB_CHECK(hipHostFree(err_host));
-  if (_bb_fluid) BB_CHECK(hipFree(_bb_fluid));
+  bb_grid_free(_bb_fluid);
   _bb_fluid = NULL;
   free_face_lists();
   _parts = NULL; _perr = NULL; _pnum = NULL; _pbin = NULL; _err_out = NULL;

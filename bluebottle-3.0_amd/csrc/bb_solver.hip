@@ -36,6 +36,7 @@
 #include <stdint.h>
 
 #include <limits.h>
+#include <type_traits>
 
 #include "bb_common.h"
 
@@ -246,11 +247,11 @@ cg_state_init_kernel(const double *__restrict__ bpart,
  * inner FP32 sweep instead (inner_done), either at the target or once (r, z)
  * has fallen by the factor relup since the last FP64 correction; the host
  * then forms the FP64 residual and decides. */
-template <bool MIXED>
+template <bool MIXED, int NT = TILE_THREADS>
 __device__ inline bool cg_continue(CGState *st, const double *__restrict__ rpart,
   long n, int q, double *beta)
 {
-  double s = tile_block_sum(rpart, n);
+  double s = tile_block_sum<NT>(rpart, n);
   bool first = blockIdx.x == 0 && threadIdx.x == 0;
   bool stop = s <= st->tol2*st->sp_rhs || isnan(s);
   if (MIXED && !stop) stop = s <= st->relup*st->rz_ref;
@@ -551,6 +552,291 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
     const double *w = s_tp[threadIdx.x];
     gpart[tx + (long) ntx_ * (ty + (long) nty_ * (tz0 + (int) threadIdx.x))] =
       ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+  }
+}
+
+
+/* ------------------- search kernel, one wavefront per row ----------------- */
+/* The same pass with one wavefront per x row of the domain: lane l holds the
+ * CPR cells x = CPR l + 1 .. CPR l + CPR of its row, read with one vector
+ * load per array and plane, so x neighbours come from the lane itself and,
+ * at the ends of its run, from lanes l - 1 and l + 1 (cross-lane moves, no
+ * memory access).  A workgroup of RY wavefronts takes RY consecutive rows
+ * and streams `zc` planes of them: y neighbours from an LDS plane of RY + 2
+ * rows (its own rows and a halo row on either side, loaded by wavefronts 0
+ * and 1), z neighbours in registers.  Per plane it loads RY + 2 rows for RY
+ * rows and no x halo, where the tile plane of the z-column kernel takes
+ * 66 x 10 entries for 64 x 8 cells, 20 of them single elements on cache
+ * lines of their own.  Arithmetic per cell and the tile partials (64 x 8 x 8
+ * tiles: per cell over the tile's planes in z order, the 64-term tree of
+ * wave_tree_sum over the tile's x, then the rows) are those of the z-column
+ * kernel, so the two give the same bits.  Needs nx <= 64 CPR, nx a multiple
+ * of CPR, and point x = 1 of every row aligned to CPR elements (the grid
+ * layout gives 128 bytes); zc a multiple of TILE_Z, at most ROW_ZCMAX. */
+#define ROW_ZCMAX 64
+
+template <typename V, int N> struct Run { V v[N]; };
+
+/* N consecutive elements from element i of a grid array (one vector load;
+ * NT: with the non-temporal hint). */
+template <typename V, int N, bool NT>
+__device__ inline Run<V, N> run_load(const V *__restrict__ a, unsigned i)
+{
+  Run<V, N> o;
+  const char *p = (const char *) a + i * (unsigned) sizeof(V);
+  if constexpr (N == 1) {
+    o.v[0] = NT ? __builtin_nontemporal_load((const V *) p) : *(const V *) p;
+  } else {
+    typedef V VN __attribute__((ext_vector_type(N)));
+    VN t = NT ? __builtin_nontemporal_load((const VN *) p) : *(const VN *) p;
+#pragma unroll
+    for (int e = 0; e < N; e++) o.v[e] = t[e];
+  }
+  return o;
+}
+
+template <typename V, int N>
+__device__ inline void run_store_nt(V *__restrict__ a, unsigned i,
+  const Run<V, N> &o)
+{
+  char *p = (char *) a + i * (unsigned) sizeof(V);
+  if constexpr (N == 1) {
+    __builtin_nontemporal_store(o.v[0], (V *) p);
+  } else {
+    typedef V VN __attribute__((ext_vector_type(N)));
+    VN t;
+#pragma unroll
+    for (int e = 0; e < N; e++) t[e] = o.v[e];
+    __builtin_nontemporal_store(t, (VN *) p);
+  }
+}
+
+/* Source row of LDS row jj (0 and ny + 1 are the halo rows): the row
+ * itself, the wrapped row, or -1 for a row that holds zeros (a wall ghost,
+ * or a row past the halo in the domain's last workgroup). */
+__device__ inline int row_src(const DevDom &D, int jj)
+{
+  if (jj >= 1 && jj <= D.ny) return jj;
+  if (jj == 0) return D.pery ? D.ny : -1;
+  if (jj == D.ny + 1) return D.pery ? 1 : -1;
+  return -1;
+}
+
+template <bool FIRST, typename V, bool MIXED, int CPR, int RY, int DEPTH>
+__global__ void __launch_bounds__(RY * WAVE)
+search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
+  const double *__restrict__ rpart, long ntiles, int zc,
+  const V *__restrict__ r, const V *__restrict__ pold,
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
+{
+  typedef Run<V, CPR> R_t;
+  /* FP64 p_old is read with the non-temporal hint, FP32 p_old ordinarily
+   * (as in the z-column kernel) */
+  constexpr bool NTP = sizeof(V) == 8;
+  __shared__ R_t lds[3 * (RY + 2) * WAVE];
+  __shared__ V s_invm[27];
+  __shared__ double s_tp[ROW_ZCMAX / TILE_Z][CPR][RY];
+  if (st->done | st->inner_done) return;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
+  const Grid &C = D.cc;
+  const int nyb = (D.ny + RY - 1) / RY;
+  const int nzc = (D.nz + zc - 1) / zc;
+  const long g = tile_of_block((long) nyb * nzc);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jb = (int) (g % nyb) * RY;
+  const int k0 = (int) (g / nyb) * zc + 1;
+  const int kend = k0 + zc - 1 < D.nz ? k0 + zc - 1 : D.nz;
+  const int j = jb + wv + 1;
+  const int x0 = CPR * lane + 1;
+  const bool xok = x0 <= D.nx;            /* nx % CPR == 0: the whole run */
+  const bool colok = xok && j <= D.ny;
+  /* the wavefront's own row is LDS row wv + 1; wavefronts 0 and 1 also
+   * load halo row 0 and RY + 1, the others their own row a second time
+   * (every load unconditional) */
+  const bool hw = wv < 2;
+  const int hs = wv == 0 ? 0 : RY + 1;
+  const int so = row_src(D, j), sh = row_src(D, jb + hs);
+  const unsigned xo = xok ? (unsigned) x0 : 1u;
+  const unsigned co = xo + (unsigned) (so >= 0 ? so : 1) * (unsigned) C.sy;
+  const unsigned ch = hw ? xo + (unsigned) (sh >= 0 ? sh : 1) * (unsigned) C.sy
+                         : co;
+  const bool oko = xok && so >= 0, okh = xok && sh >= 0;
+  int wx[CPR];
+#pragma unroll
+  for (int e = 0; e < CPR; e++) wx[e] = wcount(x0 + e, D.nx, D.perx);
+  const int wyo = 3 * wcount(so, D.ny, D.pery);
+  const int wyh = 3 * wcount(sh, D.ny, D.pery);
+
+  struct Raw { R_t r, p, hr, hp; };
+  auto load = [&](int kk) {
+    Raw R;
+    int ks = plane_src(D, kk);
+    unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) C.sz;
+    R.r = run_load<V, CPR, false>(r, co + off);
+    R.hr = run_load<V, CPR, false>(r, ch + off);
+    if (FIRST) {
+#pragma unroll
+      for (int e = 0; e < CPR; e++) R.p.v[e] = R.hp.v[e] = (V) 0;
+    } else {
+      R.p = run_load<V, CPR, NTP>(pold, co + off);
+      R.hp = run_load<V, CPR, NTP>(pold, ch + off);
+    }
+    return R;
+  };
+  Raw Rm = load(k0 - 1), Rc = load(k0);
+  Raw Rq[DEPTH];
+#pragma unroll
+  for (int d = 0; d < DEPTH; d++) Rq[d] = load(min(k0 + 1 + d, kend + 1));
+  double beta64 = 0.;
+  if (!FIRST && !cg_continue<MIXED, RY * WAVE>(st, rpart, ntiles, q, &beta64))
+    return;
+  const V beta = (V) beta64;
+  __syncthreads();                      /* s_invm */
+  /* Jacobi factors of the run's cells in the two z classes a source plane
+   * can have: one wall face (planes 1 and nz between walls) or none */
+  V mo[2][CPR], mh[2][CPR];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int e = 0; e < CPR; e++) {
+      mo[c][e] = s_invm[wx[e] + wyo + 9 * (c + 1)];
+      mh[c][e] = s_invm[wx[e] + wyh + 9 * (c + 1)];
+    }
+  /* p_new of a loaded plane into LDS buffer `buf`; returns the own row.
+   * Only wavefronts 0 and 1 form and store a halo row. */
+  auto put = [&](const Raw &R, R_t *buf, int kk) {
+    int ks = plane_src(D, kk);
+    /* a ghost plane (ks = 0) holds zeros whatever the class */
+    const int zc2 = wcount(ks, D.nz, D.perz) == 2;
+    R_t o;
+#pragma unroll
+    for (int e = 0; e < CPR; e++) {
+      V z = R.r.v[e]*(zc2 ? mo[1][e] : mo[0][e]);
+      V v = FIRST ? z : z + beta*R.p.v[e];
+      o.v[e] = (ks && oko) ? v : (V) 0;
+    }
+    buf[(wv + 1) * WAVE + lane] = o;
+    if (hw) {
+      R_t h;
+#pragma unroll
+      for (int e = 0; e < CPR; e++) {
+        V zh = R.hr.v[e]*(zc2 ? mh[1][e] : mh[0][e]);
+        V vh = FIRST ? zh : zh + beta*R.hp.v[e];
+        h.v[e] = (ks && okh) ? vh : (V) 0;
+      }
+      buf[hs * WAVE + lane] = h;
+    }
+    return o;
+  };
+  R_t *bprev = lds, *bcur = lds + (RY + 2) * WAVE,
+      *bnext = lds + 2 * (RY + 2) * WAVE;
+  R_t prev = put(Rm, bprev, k0 - 1);
+  R_t cur = put(Rc, bcur, k0);
+  __syncthreads();
+  const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+  /* x = 1 and x = nx: the first cell of lane 0 and the last of lane
+   * nx / CPR - 1, the only cells that can have an x wall */
+  const int lend = D.nx / CPR - 1;
+  const bool xw_lo = lane == 0 && !D.perx, xw_hi = lane == lend && !D.perx;
+  /* (-A) p over the run; YZ: the row or the plane has a wall face in y or
+   * z (else those weights are 1 and left out) */
+  auto lap = [&](auto yz, const R_t &S, const R_t &N, const R_t &nxt, V wl,
+                 V er, V xn, V x1, bool wB, bool wT) {
+    constexpr bool YZ = decltype(yz)::value;
+    R_t a;
+#pragma unroll
+    for (int e = 0; e < CPR; e++) {
+      V pw = e > 0 ? cur.v[e > 0 ? e - 1 : 0]
+                   : (lane == 0 ? (D.perx ? xn : (V) 0) : wl);
+      V pe = e < CPR - 1 ? cur.v[e < CPR - 1 ? e + 1 : 0]
+                         : (lane == lend ? (D.perx ? x1 : (V) 0) : er);
+      const bool ww = e == 0 && xw_lo, we = e == CPR - 1 && xw_hi;
+      a.v[e] = neg_lap7<V>(K, cur.v[e], pw, pe, S.v[e], N.v[e], prev.v[e],
+                           nxt.v[e], ww, we, YZ && wS, YZ && wN, YZ && wB,
+                           YZ && wT);
+    }
+    return a;
+  };
+  double acc[CPR];
+#pragma unroll
+  for (int e = 0; e < CPR; e++) acc[e] = 0.0;
+  int ti = 0;
+  auto step = [&](int k, Raw &Rs) {
+    R_t nxt = put(Rs, bnext, k + 1);
+    Rs = load(min(k + 1 + DEPTH, kend + 1));
+    __syncthreads();
+    const R_t S = bcur[wv * WAVE + lane], N = bcur[(wv + 2) * WAVE + lane];
+    /* run neighbours: the last cell of lane l - 1, the first of l + 1, and
+     * the wrapped cells x = nx and x = 1 (0: a wall ghost) */
+    V wl = __shfl(cur.v[CPR - 1], (lane + WAVE - 1) & (WAVE - 1));
+    V er = __shfl(cur.v[0], (lane + 1) & (WAVE - 1));
+    V xn = __shfl(cur.v[CPR - 1], lend);
+    V x1 = __shfl(cur.v[0], 0);
+    const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+    /* wavefront-uniform: interior rows of interior planes skip the y and
+     * z weights (a weight of 1 multiplies exactly) */
+    R_t a;
+    if (wS || wN || wB || wT)
+      a = lap(std::true_type(), S, N, nxt, wl, er, xn, x1, wB, wT);
+    else
+      a = lap(std::false_type(), S, N, nxt, wl, er, xn, x1, wB, wT);
+    /* cells outside the domain store to the never-read edge ghost
+     * (x = 1, j = 0, k = 0), aligned like every run */
+    unsigned c = colok ? co + (unsigned) k * (unsigned) C.sz : 1u;
+    run_store_nt<V, CPR>(pnew, c, cur);
+    run_store_nt<V, CPR>(Ap, c, a);
+#pragma unroll
+    for (int e = 0; e < CPR; e++)
+      acc[e] += colok ? (double) cur.v[e] * (double) a.v[e] : 0.0;
+    prev = cur;
+    cur = nxt;
+    R_t *t3 = bprev;
+    bprev = bcur;
+    bcur = bnext;
+    bnext = t3;
+    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
+      /* wave_tree_sum over each 64-cell tile of the row: steps of CPR cells
+       * or more are lane moves within the tile's 64 / CPR lanes, the last
+       * log2(CPR) steps add within the lane */
+#pragma unroll
+      for (int off = 32; off >= CPR; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < CPR; e++)
+          acc[e] += __shfl_down(acc[e], off / CPR, WAVE);
+#pragma unroll
+      for (int off = CPR / 2; off >= 1; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < off; e++) acc[e] += acc[e + off];
+      if ((lane & (WAVE / CPR - 1)) == 0) s_tp[ti][lane / (WAVE / CPR)][wv] = acc[0];
+#pragma unroll
+      for (int e = 0; e < CPR; e++) acc[e] = 0.0;
+      ti++;
+    }
+  };
+  int k = k0;
+  for (; k + DEPTH - 1 <= kend; k += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) step(k + d, Rq[d]);
+  }
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++)
+    if (k + d <= kend) step(k + d, Rq[d]);
+  __syncthreads();
+  /* one partial per (z-tile, x-tile, 8-row group) of the workgroup, rows
+   * combined as in tile_partial */
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int nout = ti * CPR * (RY / TILE_Y);
+  if ((int) threadIdx.x < nout) {
+    int t = threadIdx.x;
+    int tx = t % CPR, rg = (t / CPR) % (RY / TILE_Y), tz = t / (CPR * (RY / TILE_Y));
+    int ty = jb / TILE_Y + rg;
+    if (tx < ntx_ && ty < nty_) {
+      const double *w = &s_tp[tz][tx][rg * TILE_Y];
+      gpart[tx + (long) ntx_ * (ty + (long) nty_ * ((k0 - 1) / TILE_Z + tz))] =
+        ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    }
   }
 }
 
@@ -1147,11 +1433,32 @@ static double *_part_r = NULL;         /* tile partials of (r, z) */
 static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
-int bb_spmv_variant = 2;               /* 2: search kernel over z-columns of
-                                          tiles with LDS planes; 0: direct
-                                          neighbour loads (a second
-                                          implementation, one tile per
-                                          workgroup, for cross-checks) */
+int bb_spmv_variant = 3;               /* 3: search kernel with one
+                                          wavefront per x row where the grid
+                                          allows it (else 2); 2: over
+                                          z-columns of tiles with LDS
+                                          planes; 0: direct neighbour loads
+                                          (one tile per workgroup, for
+                                          cross-checks) */
+int bb_row_fp64 = 0;                   /* 1: the FP64 search uses the row
+                                          kernel too (at 256^3 it measured
+                                          144 us against 132 for the
+                                          z-column kernel: 172 VGPRs leave
+                                          two wavefronts per SIMD) */
+int bb_row_ry = 0;                     /* rows per workgroup of the FP32 row
+                                          kernel: 16, else 8 (FP64: 8) */
+int bb_spmv_last = -1;                 /* search kernel of the last
+                                          launch: 3 row, 2 z-column, 0
+                                          direct */
+int bb_row_last[4] = {0, 0, 0, 0};     /* the row kernel's last cells per
+                                          lane, rows and planes per
+                                          workgroup, planes in flight */
+int bb_row_depth = 0;                  /* planes of loads in flight in the
+                                          FP32 row kernel: 3 or 4, else 2
+                                          (FP64: 2) */
+int bb_row_zc = 0;                     /* planes per workgroup of the row
+                                          kernel, a multiple of 8 up to 64
+                                          (0: automatic) */
 int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_spmv_ntp = 1;                   /* 1: FP64 search loads p_old with the
                                           non-temporal hint (0: ordinary
@@ -1181,9 +1488,8 @@ static void log_solve(int niter)
   bb_pcg_solves_total++;
 }
 
-/* FP32 vectors of the mixed form's inner sweeps (Gcc layout).  Each
- * vector starts one float into its allocation, so that the first interior
- * cell of a row (x = 1) sits on an 8-byte boundary. */
+/* FP32 vectors of the mixed form's inner sweeps (Gcc layout, grid arrays
+ * of bb_grid_alloc: x = 1 of every row on a 128-byte boundary). */
 static float *_r32 = NULL, *_p32a = NULL, *_p32b = NULL, *_Ap32 = NULL,
              *_e32 = NULL;
 static long buf32_n = 0;
@@ -1192,7 +1498,7 @@ static void cg_buffers32_free(void)
 {
   float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
   for (int i = 0; i < 5; i++) {
-    if (*bufs[i]) BB_CHECK(hipFree(*bufs[i] - 1));
+    bb_grid_free(*bufs[i]);
     *bufs[i] = NULL;
   }
   buf32_n = 0;
@@ -1203,12 +1509,8 @@ static void cg_buffers32(void)
   if (buf32_n == ddom.cc.n) return;
   cg_buffers32_free();
   float **bufs[5] = {&_r32, &_p32a, &_p32b, &_Ap32, &_e32};
-  for (int i = 0; i < 5; i++) {
-    float *a = NULL;
-    BB_CHECK(hipMalloc((void **) &a, (ddom.cc.n + 2) * sizeof(float)));
-    BB_CHECK(hipMemset(a, 0, (ddom.cc.n + 2) * sizeof(float)));
-    *bufs[i] = a + 1;
-  }
+  for (int i = 0; i < 5; i++)
+    *bufs[i] = (float *) bb_grid_alloc(ddom.cc.n, sizeof(float));
   buf32_n = ddom.cc.n;
 }
 
@@ -1226,9 +1528,8 @@ static void cg_buffers(void)
       hipHostMallocDefault));
   }
   if (buf_n != ddom.cc.n) {
-    if (_p_alt) BB_CHECK(hipFree(_p_alt));
-    BB_CHECK(hipMalloc((void **) &_p_alt, ddom.cc.n * sizeof(double)));
-    BB_CHECK(hipMemset(_p_alt, 0, ddom.cc.n * sizeof(double)));
+    bb_grid_free(_p_alt);
+    _p_alt = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
     buf_n = ddom.cc.n;
   }
   long nt = n_tiles(ddom);
@@ -1247,7 +1548,7 @@ static void cg_buffers(void)
 
 void bb_solver_free(void)
 {
-  if (_p_alt) BB_CHECK(hipFree(_p_alt));
+  bb_grid_free(_p_alt);
   if (_part_p) BB_CHECK(hipFree(_part_p));
   if (_part_r) BB_CHECK(hipFree(_part_r));
   if (_part_w1) BB_CHECK(hipFree(_part_w1));
@@ -1487,6 +1788,100 @@ void bb_pp_defect(const double *x, double *out)
   BB_LAUNCH_CHECK();
 }
 
+/* Cells per lane of the row kernel on this grid (a whole x row in one
+ * wavefront, nx a multiple of it), 0 where it does not apply. */
+static int row_cpr(void)
+{
+  const int nx = ddom.nx;
+  if (nx <= WAVE) return 1;
+  if (nx <= 2 * WAVE && nx % 2 == 0) return 2;
+  if (nx <= 4 * WAVE && nx % 4 == 0) return 4;
+  return 0;
+}
+
+/* Point x = 1 of every row of `a` aligned to `bytes`. */
+static bool rows_aligned(const void *a, size_t esize, size_t bytes)
+{
+  const Grid &C = ddom.cc;
+  uintptr_t x1 = (uintptr_t) a + esize;
+  return x1 % bytes == 0 && (C.sy * esize) % bytes == 0;
+}
+
+template <bool FIRST, typename V, bool MIXED, int CPR, int RY, int DEPTH>
+static void launch_row_t(const CoefT<V> &K, const InvMT<V> &T, int q,
+  long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
+{
+  const long nyb = (ddom.ny + RY - 1) / RY, nzc = (ddom.nz + zc - 1) / zc;
+  hipLaunchKernelGGL((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
+    dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K, T,
+    _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);
+}
+
+template <bool FIRST, typename V, bool MIXED, int RY, int DEPTH>
+static void launch_row_ry(int cpr, const CoefT<V> &K, const InvMT<V> &T,
+  int q, long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
+{
+  if (cpr == 4)
+    launch_row_t<FIRST, V, MIXED, 4, RY, DEPTH>(K, T, q, nrp, zc, r, pold,
+                                                pnew, Ap);
+  else if (cpr == 2)
+    launch_row_t<FIRST, V, MIXED, 2, RY, DEPTH>(K, T, q, nrp, zc, r, pold,
+                                                pnew, Ap);
+  else
+    launch_row_t<FIRST, V, MIXED, 1, RY, DEPTH>(K, T, q, nrp, zc, r, pold,
+                                                pnew, Ap);
+}
+
+/* The row kernel where it applies; false: the caller launches another. */
+template <bool FIRST, typename V, bool MIXED>
+static bool launch_row(const CoefT<V> &K, const InvMT<V> &T, int q,
+  long nrp, const V *r, const V *pold, V *pnew, V *Ap)
+{
+  if (sizeof(V) == 8 && !bb_row_fp64) return false;
+  const int cpr = row_cpr();
+  if (!cpr) return false;
+  const size_t run = (size_t) cpr * sizeof(V);
+  if (!rows_aligned(r, sizeof(V), run) || !rows_aligned(pold, sizeof(V), run)
+      || !rows_aligned(pnew, sizeof(V), run) || !rows_aligned(Ap, sizeof(V), run))
+    return false;
+  /* 8 rows per workgroup (FP32 at 256^3 inside the solver: 61.5 us with 8
+   * rows and 32 planes, 64.1 with 16 rows and 16 planes); FP64 always 8
+   * (with 16 the 1024-thread workgroup leaves 128 VGPRs per lane and the
+   * FP64 kernel spills) */
+  int ry = sizeof(V) == 4 && bb_row_ry == 16 ? 16 : 8;
+  /* planes per workgroup: about 256 workgroups of RY rows over the grid
+   * (one per CU of MI355X), in whole z-tiles, at most 8 of them */
+  int zc = bb_row_zc;
+  if (zc <= 0 || zc % TILE_Z || zc > ROW_ZCMAX) {
+    long nyb = (ddom.ny + ry - 1) / ry;
+    long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
+    long t = nyb * ntz / 256;
+    zc = TILE_Z * (int) (t < 1 ? 1 : (t > ROW_ZCMAX / TILE_Z ? ROW_ZCMAX / TILE_Z : t));
+  }
+  bb_spmv_last = 3;
+  bb_row_last[0] = cpr;
+  bb_row_last[1] = ry;
+  bb_row_last[2] = zc;
+  bb_row_last[3] = 2;
+  if constexpr (sizeof(V) == 4) {
+    /* planes of loads in flight: 2 (as the z-column kernel), 3 or 4 */
+    const int depth = bb_row_depth == 3 || bb_row_depth == 4 ? bb_row_depth : 2;
+    bb_row_last[3] = depth;
+#define RL(RY_, D_) launch_row_ry<FIRST, V, MIXED, RY_, D_>(cpr, K, T, q, nrp, \
+    zc, r, pold, pnew, Ap)
+    if (ry == 16) {
+      if (depth == 4) RL(16, 4); else if (depth == 3) RL(16, 3); else RL(16, 2);
+    } else {
+      if (depth == 4) RL(8, 4); else if (depth == 3) RL(8, 3); else RL(8, 2);
+    }
+#undef RL
+    return true;
+  }
+  launch_row_ry<FIRST, V, MIXED, 8, 2>(cpr, K, T, q, nrp, zc, r, pold, pnew,
+                                       Ap);
+  return true;
+}
+
 /* The z-column search kernel; the two hint switches select the template
  * (non-temporal p_old loads: FP64 only, the FP32 kernel loads p_old
  * ordinarily either way). */
@@ -1494,6 +1889,10 @@ template <bool FIRST, typename V, bool MIXED>
 static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, const V *r, const V *pold, V *pnew, V *Ap)
 {
+  if (bb_spmv_variant == 3 &&
+      launch_row<FIRST, V, MIXED>(K, T, q, nrp, r, pold, pnew, Ap))
+    return;
+  bb_spmv_last = 2;
   const bool f32 = sizeof(V) == 4;
   int tzb = f32 ? spmv_tzb32() : spmv_tzb();
   bool nts = (f32 ? bb_spmv_nts32 : bb_spmv_nts) != 0;
@@ -1544,6 +1943,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
       launch_group<false, double, false>(K, T, q, nt, _r_q, pold, pnew,
                                          _Apb_q);
   } else {
+    bb_spmv_last = 0;
     if (q == 1)
       hipLaunchKernelGGL(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
         K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
@@ -1709,7 +2109,8 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
 static size_t noalloc_bytes(void)
 {
   const long nt = n_tiles(ddom);
-  return align256(sizeof(CGState)) + align256(ddom.cc.n * sizeof(double))
+  return align256(sizeof(CGState))
+       + align256((ddom.cc.n + GRID_ORIGIN + 1) * sizeof(double))
        + 2 * align256(nt * sizeof(double));
 }
 
@@ -1749,8 +2150,8 @@ extern "C" int cuda_PP_cg_noalloc(void)
   const long nt = n_tiles(ddom);
   CGState *st = (CGState *) base;
   base += align256(sizeof(CGState));
-  double *palt = (double *) base;
-  base += align256(ddom.cc.n * sizeof(double));
+  double *palt = (double *) base + GRID_ORIGIN;   /* as bb_grid_alloc */
+  base += align256((ddom.cc.n + GRID_ORIGIN + 1) * sizeof(double));
   double *pp = (double *) base;
   base += align256(nt * sizeof(double));
   double *pr = (double *) base;

@@ -231,14 +231,10 @@ extern "C" void bb_warm_setup(void)
   BB_CHECK(hipMalloc((void **) &_cin, 2 * ne * sizeof(float)));
   BB_CHECK(hipMalloc((void **) &_ctgt, ne * sizeof(float)));
   BB_CHECK(hipMalloc((void **) &_wpart, MBLK * 4 * sizeof(double)));
-  BB_CHECK(hipMalloc((void **) &_psi, ddom.cc.n * sizeof(double)));
-  BB_CHECK(hipMemset(_psi, 0, ddom.cc.n * sizeof(double)));
-  for (int h = 0; h < HIST_MAX; h++) {
-    BB_CHECK(hipMalloc((void **) &_hist[h], ddom.cc.n * sizeof(double)));
-    BB_CHECK(hipMemset(_hist[h], 0, ddom.cc.n * sizeof(double)));
-  }
-  BB_CHECK(hipMalloc((void **) &_wres, ddom.cc.n * sizeof(double)));
-  BB_CHECK(hipMemset(_wres, 0, ddom.cc.n * sizeof(double)));
+  _psi = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
+  for (int h = 0; h < HIST_MAX; h++)
+    _hist[h] = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
+  _wres = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
   hist_n = hist_head = 0;
   net_step = -1;
   net_this_solve = 0;
@@ -257,12 +253,12 @@ extern "C" void bb_warm_teardown(void)
   for (int i = 0; i < 5; i++) BB_CHECK(hipFree(f[i]));
   BB_CHECK(hipFree(_wpart));
   BB_CHECK(hipHostFree(wpart_host));
-  BB_CHECK(hipFree(_psi));
+  bb_grid_free(_psi);
   for (int h = 0; h < HIST_MAX; h++) {
-    BB_CHECK(hipFree(_hist[h]));
+    bb_grid_free(_hist[h]);
     _hist[h] = NULL;
   }
-  BB_CHECK(hipFree(_wres));
+  bb_grid_free(_wres);
   _psi = _wres = NULL;
   _win = _wtgt = _wout = _cin = _ctgt = NULL;
   _wpart = NULL;

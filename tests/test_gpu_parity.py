@@ -20,8 +20,9 @@ FIELDS = ("u", "v", "w", "p", "phi", "u_star", "v_star", "w_star",
           "conv_u", "conv_v", "conv_w", "diff_u", "diff_v", "diff_w")
 
 
-def set_knobs(variant=2, tzb=0, batch=16, defer=0):
-    """variant 2: search kernel over z-columns of tzb tiles (0: automatic),
+def set_knobs(variant=3, tzb=0, batch=16, defer=0):
+    """variant 3: search kernel with one wavefront per x row where the grid
+    allows it, else as 2; 2: over z-columns of tzb tiles (0: automatic);
     0: direct neighbour loads (the cross-check implementation); defer = 1:
     phi updated every second iteration with both terms (odd iterations
     leave it)."""
@@ -37,10 +38,10 @@ def default_knobs():
     """Each test sets its knobs; the library keeps them, so put the
     defaults back for the tests that follow."""
     yield
-    set_knobs()
+    set_knobs(defer=1)
 
 
-def run_pair(case_dir, kw, nsteps, variant=2, tzb=0, batch=16, defer=0):
+def run_pair(case_dir, kw, nsteps, variant=3, tzb=0, batch=16, defer=0):
     n = kw["n"]
     set_knobs(variant, tzb, batch, defer)
     bbconfig.write_case(case_dir, **kw)
@@ -85,7 +86,7 @@ def test_cavity_32_full_100_steps(case_dir):
     assert out["iters_gpu"] == out["iters_orc"]
 
 
-@pytest.mark.parametrize("variant,tzb", [(0, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("variant,tzb", [(0, 0), (2, 1), (2, 2), (3, 0)])
 def test_cavity_bitwise(case_dir, variant, tzb):
     out = run_pair(case_dir, bbconfig.cavity(n=16, steps=5), 5, variant, tzb)
     for f in FIELDS:
@@ -103,7 +104,7 @@ def test_channel_small(case_dir):
 
 @pytest.mark.parametrize("variant,tzb,defer",
                          [(0, 0, 0), (2, 1, 0), (2, 4, 0), (2, 4, 1),
-                          (2, 1, 1), (2, 8, 1)])
+                          (2, 1, 1), (2, 8, 1), (3, 0, 0), (3, 0, 1)])
 def test_channel_bitwise(case_dir, variant, tzb, defer):
     out = run_pair(case_dir, bbconfig.channel(n=(64, 32, 32)), 3, variant,
                    tzb, defer=defer)
@@ -115,7 +116,8 @@ def test_channel_bitwise(case_dir, variant, tzb, defer):
 @pytest.mark.parametrize("variant,tzb,batch,defer",
                          [(0, 0, 16, 0), (0, 0, 5, 0), (2, 2, 1, 0),
                           (2, 4, 5, 0), (2, 3, 16, 0), (2, 2, 1, 1),
-                          (2, 4, 5, 1), (2, 3, 16, 1), (2, 8, 16, 1)])
+                          (2, 4, 5, 1), (2, 3, 16, 1), (2, 8, 16, 1),
+                          (3, 0, 16, 0), (3, 0, 5, 1)])
 def test_ragged_sizes(case_dir, variant, tzb, batch, defer):
     """x not a multiple of the 64-lane wavefront, y not a multiple of the
     row group, z not a multiple of the tile depth (and the tile count in z,
@@ -147,9 +149,18 @@ def test_periodic_all_and_wide_x(case_dir):
     for f in FIELDS:
         assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
             describe(f, out["gpu"][f], out["orc"][f])
+    # rows of 450 cells are wider than one wavefront's 256: the row variant
+    # falls back to the z-column kernel
+    out = run_pair(case_dir, kw, 3, variant=3)
+    assert ctypes.c_int.in_dll(bbhip.load(), "bb_spmv_last").value == 2
+    check(out)
+    for f in FIELDS:
+        assert bitwise_equal(out["gpu"][f], out["orc"][f]), \
+            describe(f, out["gpu"][f], out["orc"][f])
 
 
-@pytest.mark.parametrize("variant,defer", [(2, 0), (0, 0), (2, 1)])
+@pytest.mark.parametrize("variant,defer", [(2, 0), (0, 0), (2, 1), (3, 0),
+                                           (3, 1)])
 def test_periodic_all_25d(case_dir, variant, defer):
     per = ("PERIODIC",)
     vel = {c: {f: per for f in "WESNBT"} for c in "uvw"}
