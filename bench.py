@@ -136,6 +136,18 @@ def parse():
     ap.add_argument("--row-depth", type=int, default=None,
                     help="planes of loads in flight in the FP32 row kernel "
                          "(2, 3 or 4; default 2)")
+    ap.add_argument("--update-variant", type=int, default=None,
+                    help="update kernel: 1 one wavefront per x row where "
+                         "the grid allows it (default), 0 one tile per "
+                         "workgroup")
+    ap.add_argument("--urow-zc", type=int, default=None,
+                    help="planes per workgroup of the row update (0: "
+                         "automatic)")
+    ap.add_argument("--urow-depth", type=int, default=None,
+                    help="planes of loads in flight in the FP32 row update "
+                         "(2 or 4)")
+    ap.add_argument("--urow-fp64", type=int, default=None,
+                    help="1: the FP64 update uses the row kernel too")
     ap.add_argument("--row-fp64", type=int, default=None,
                     help="1: the FP64 search uses the row kernel too "
                          "(default 0)")
@@ -403,6 +415,10 @@ def main():
                       ("bb_row_zc", a.row_zc),
                       ("bb_row_depth", a.row_depth),
                       ("bb_row_fp64", a.row_fp64),
+                      ("bb_update_variant", a.update_variant),
+                      ("bb_urow_zc", a.urow_zc),
+                      ("bb_urow_depth", a.urow_depth),
+                      ("bb_urow_fp64", a.urow_fp64),
                       ("bb_pcg_batch", a.batch),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:

@@ -563,8 +563,8 @@ search_spmv_group_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st,
  * at the ends of its run, from lanes l - 1 and l + 1 (cross-lane moves, no
  * memory access).  A workgroup of RY wavefronts takes RY consecutive rows
  * and streams `zc` planes of them: y neighbours from an LDS plane of RY + 2
- * rows (its own rows and a halo row on either side, loaded by wavefronts 0
- * and 1), z neighbours in registers.  Per plane it loads RY + 2 rows for RY
+ * rows (its own rows and a halo row on either side, shared out over the
+ * wavefronts), z neighbours in registers.  Per plane it loads RY + 2 rows for RY
  * rows and no x halo, where the tile plane of the z-column kernel takes
  * 66 x 10 entries for 64 x 8 cells, 20 of them single elements on cache
  * lines of their own.  Arithmetic per cell and the tile partials (64 x 8 x 8
@@ -651,36 +651,43 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const int x0 = CPR * lane + 1;
   const bool xok = x0 <= D.nx;            /* nx % CPR == 0: the whole run */
   const bool colok = xok && j <= D.ny;
-  /* the wavefront's own row is LDS row wv + 1; wavefronts 0 and 1 also
-   * load halo row 0 and RY + 1, the others their own row a second time
-   * (every load unconditional) */
-  const bool hw = wv < 2;
-  const int hs = wv == 0 ? 0 : RY + 1;
+  /* the wavefront's own row is LDS row wv + 1.  The two halo rows (LDS
+   * rows 0 and RY + 1) are shared out evenly, so that every wavefront
+   * reaches the plane barrier with the same work: wavefront wv loads one
+   * cell per lane of segment wv % (RY / 2) (HC cells) of halo row 0 (wv <
+   * RY / 2) or RY + 1.  Every load is unconditional: lanes past the
+   * segment or the row read x = 1 and store nothing. */
+  constexpr int HSEG = RY / 2, HC = WAVE * CPR / HSEG;
+  static_assert(HC <= WAVE, "a halo segment must fit one wavefront");
+  const int hs = wv < HSEG ? 0 : RY + 1;
+  const int xh = (wv % HSEG) * HC + lane + 1;
+  const bool hlane = lane < HC && xh <= D.nx;
   const int so = row_src(D, j), sh = row_src(D, jb + hs);
   const unsigned xo = xok ? (unsigned) x0 : 1u;
   const unsigned co = xo + (unsigned) (so >= 0 ? so : 1) * (unsigned) C.sy;
-  const unsigned ch = hw ? xo + (unsigned) (sh >= 0 ? sh : 1) * (unsigned) C.sy
-                         : co;
-  const bool oko = xok && so >= 0, okh = xok && sh >= 0;
+  const unsigned ch = (hlane ? (unsigned) xh : 1u)
+                    + (unsigned) (sh >= 0 ? sh : 1) * (unsigned) C.sy;
+  const bool oko = xok && so >= 0, okh = sh >= 0;
   int wx[CPR];
 #pragma unroll
   for (int e = 0; e < CPR; e++) wx[e] = wcount(x0 + e, D.nx, D.perx);
   const int wyo = 3 * wcount(so, D.ny, D.pery);
-  const int wyh = 3 * wcount(sh, D.ny, D.pery);
+  const int wxyh = wcount(xh, D.nx, D.perx) + 3 * wcount(sh, D.ny, D.pery);
 
-  struct Raw { R_t r, p, hr, hp; };
+  struct Raw { R_t r, p; V hr, hp; };
   auto load = [&](int kk) {
     Raw R;
     int ks = plane_src(D, kk);
     unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) C.sz;
     R.r = run_load<V, CPR, false>(r, co + off);
-    R.hr = run_load<V, CPR, false>(r, ch + off);
+    R.hr = ld32(r, ch + off);
     if (FIRST) {
 #pragma unroll
-      for (int e = 0; e < CPR; e++) R.p.v[e] = R.hp.v[e] = (V) 0;
+      for (int e = 0; e < CPR; e++) R.p.v[e] = (V) 0;
+      R.hp = (V) 0;
     } else {
       R.p = run_load<V, CPR, NTP>(pold, co + off);
-      R.hp = run_load<V, CPR, NTP>(pold, ch + off);
+      R.hp = NTP ? ld32_nt(pold, ch + off) : ld32(pold, ch + off);
     }
     return R;
   };
@@ -695,16 +702,15 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   __syncthreads();                      /* s_invm */
   /* Jacobi factors of the run's cells in the two z classes a source plane
    * can have: one wall face (planes 1 and nz between walls) or none */
-  V mo[2][CPR], mh[2][CPR];
+  V mo[2][CPR], mh[2];
 #pragma unroll
-  for (int c = 0; c < 2; c++)
+  for (int c = 0; c < 2; c++) {
 #pragma unroll
-    for (int e = 0; e < CPR; e++) {
-      mo[c][e] = s_invm[wx[e] + wyo + 9 * (c + 1)];
-      mh[c][e] = s_invm[wx[e] + wyh + 9 * (c + 1)];
-    }
-  /* p_new of a loaded plane into LDS buffer `buf`; returns the own row.
-   * Only wavefronts 0 and 1 form and store a halo row. */
+    for (int e = 0; e < CPR; e++) mo[c][e] = s_invm[wx[e] + wyo + 9 * (c + 1)];
+    mh[c] = s_invm[wxyh + 9 * (c + 1)];
+  }
+  /* p_new of a loaded plane into LDS buffer `buf` (own run and halo cell,
+   * the halo cell at its x in the row); returns the own run */
   auto put = [&](const Raw &R, R_t *buf, int kk) {
     int ks = plane_src(D, kk);
     /* a ghost plane (ks = 0) holds zeros whatever the class */
@@ -717,16 +723,10 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       o.v[e] = (ks && oko) ? v : (V) 0;
     }
     buf[(wv + 1) * WAVE + lane] = o;
-    if (hw) {
-      R_t h;
-#pragma unroll
-      for (int e = 0; e < CPR; e++) {
-        V zh = R.hr.v[e]*(zc2 ? mh[1][e] : mh[0][e]);
-        V vh = FIRST ? zh : zh + beta*R.hp.v[e];
-        h.v[e] = (ks && okh) ? vh : (V) 0;
-      }
-      buf[hs * WAVE + lane] = h;
-    }
+    V zh = R.hr*(zc2 ? mh[1] : mh[0]);
+    V vh = FIRST ? zh : zh + beta*R.hp;
+    vh = (ks && okh) ? vh : (V) 0;
+    if (hlane) ((V *) buf)[hs * WAVE * CPR + xh - 1] = vh;
     return o;
   };
   R_t *bprev = lds, *bcur = lds + (RY + 2) * WAVE,
@@ -1012,6 +1012,171 @@ update_kernel(DevDom D, InvMT<V> T, CGState *st, int q,
   tile_partial(acc, gpart, tile);
 }
 
+
+/* ----------------- update kernel, one wavefront per row ------------------ */
+/* The update pass in the row shape of search_row_kernel: lane l holds the
+ * CPR cells x = CPR l + 1 .. CPR l + CPR of its row (vector loads), a
+ * workgroup takes RY rows and `zc` planes of them with DEPTH planes of
+ * loads in flight.  alpha (the reduction of every tile partial) is formed
+ * once per workgroup of RY rows by zc planes instead of once per 64 x 8 x 8
+ * tile, each wavefront reads whole rows, and no barrier is needed inside
+ * the plane loop.  Arithmetic per cell is that of update_kernel and the
+ * tile partials of (r, z) are formed as in search_row_kernel, so the two
+ * update kernels give the same bits.  Needs what the search row kernel
+ * needs, the fluid mask included (CPR bytes per run, aligned). */
+template <typename V, int N>
+__device__ inline void run_store(V *__restrict__ a, unsigned i,
+  const Run<V, N> &o)
+{
+  char *p = (char *) a + i * (unsigned) sizeof(V);
+  if constexpr (N == 1) {
+    *(V *) p = o.v[0];
+  } else {
+    typedef V VN __attribute__((ext_vector_type(N)));
+    VN t;
+#pragma unroll
+    for (int e = 0; e < N; e++) t[e] = o.v[e];
+    *(VN *) p = t;
+  }
+}
+
+/* N fluid flags from byte i (byte e of the result: cell e). */
+template <int N>
+__device__ inline uint32_t flags_load(const uint8_t *__restrict__ f,
+  unsigned i)
+{
+  if constexpr (N == 4) return *(const uint32_t *) (f + i);
+  else if constexpr (N == 2) return *(const uint16_t *) (f + i);
+  else return f[i];
+}
+
+template <typename V, bool MASK, int PHI, int CPR, int RY, int DEPTH>
+__global__ void __launch_bounds__(RY * WAVE)
+update_row_kernel(DevDom D, InvMT<V> T, CGState *st, int q,
+  const double *__restrict__ ppart, long ntiles, int zc,
+  const V *__restrict__ p, const V *__restrict__ Ap,
+  const uint8_t *__restrict__ fluid, V *__restrict__ phi,
+  V *__restrict__ r, double *__restrict__ gpart,
+  const V *__restrict__ pprev)
+{
+  typedef Run<V, CPR> R_t;
+  __shared__ V s_invm[27];
+  __shared__ double s_tp[ROW_ZCMAX / TILE_Z][CPR][RY];
+  if (st->done | st->inner_done) return;
+  const V alpha1 = PHI == PHI_PAIR ? (V) st->alpha : (V) 0;
+  if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
+  const Grid &C = D.cc;
+  const int nyb = (D.ny + RY - 1) / RY;
+  const int nzc = (D.nz + zc - 1) / zc;
+  const long g = tile_of_block((long) nyb * nzc);
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jb = (int) (g % nyb) * RY;
+  const int k0 = (int) (g / nyb) * zc + 1;
+  const int kend = k0 + zc - 1 < D.nz ? k0 + zc - 1 : D.nz;
+  const int j = jb + wv + 1;
+  const int x0 = CPR * lane + 1;
+  const bool colok = x0 <= D.nx && j <= D.ny;
+  /* cells outside the domain read x = 1 of row 1 and store nothing */
+  const unsigned co = (colok ? (unsigned) x0 : 1u)
+                    + (unsigned) (colok ? j : 1) * (unsigned) C.sy;
+  struct Raw { R_t p, ap, ph, pp, r; uint32_t f; };
+  auto load = [&](int k) {
+    Raw R;
+    unsigned c = co + (unsigned) k * (unsigned) C.sz;
+    R.ap = run_load<V, CPR, true>(Ap, c);
+    R.r = run_load<V, CPR, false>(r, c);
+    if (PHI != PHI_SKIP) {
+      R.p = run_load<V, CPR, false>(p, c);
+      R.ph = run_load<V, CPR, false>(phi, c);
+    }
+    if (PHI == PHI_PAIR) R.pp = run_load<V, CPR, false>(pprev, c);
+    R.f = MASK ? flags_load<CPR>(fluid, c) : 0xffffffffu;
+    return R;
+  };
+  /* the first DEPTH planes are in flight during the reduction */
+  Raw R[DEPTH];
+#pragma unroll
+  for (int d = 0; d < DEPTH; d++) R[d] = load(min(k0 + d, kend));
+  const V alpha = (V) (st->rzs[q & 1]/tile_block_sum<RY * WAVE>(ppart, ntiles));
+  if (PHI == PHI_SKIP && blockIdx.x == 0 && threadIdx.x == 0)
+    st->alpha = alpha;
+  /* Jacobi factors by z class (one wall face, none); s_invm was published
+   * by tile_block_sum's barriers */
+  const int wy = 3 * wcount(j, D.ny, D.pery);
+  V m[2][CPR];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int e = 0; e < CPR; e++)
+      m[c][e] = s_invm[wcount(x0 + e, D.nx, D.perx) + wy + 9 * (c + 1)];
+  double acc[CPR];
+#pragma unroll
+  for (int e = 0; e < CPR; e++) acc[e] = 0.0;
+  int ti = 0;
+  auto step = [&](int k, Raw &Rs) {
+    const unsigned c = co + (unsigned) k * (unsigned) C.sz;
+    const bool zc2 = wcount(k, D.nz, D.perz) == 2;
+    R_t ph, rv;
+#pragma unroll
+    for (int e = 0; e < CPR; e++) {
+      if (PHI == PHI_EACH) ph.v[e] = Rs.ph.v[e] + alpha*Rs.p.v[e];
+      if (PHI == PHI_PAIR) {
+        V t = Rs.ph.v[e] + alpha1*Rs.pp.v[e];
+        ph.v[e] = t + alpha*Rs.p.v[e];
+      }
+      const bool fl = ((Rs.f >> (8 * e)) & 0xffu) != 0;
+      V rr = Rs.r.v[e] - alpha*Rs.ap.v[e];
+      rr = fl ? rr : (V) 0;
+      rv.v[e] = rr;
+      V z = rr*(fl ? (zc2 ? m[1][e] : m[0][e]) : (V) 0);
+      acc[e] += colok ? (double) rr * (double) z : 0.0;
+    }
+    if (colok) {
+      if (PHI != PHI_SKIP) run_store<V, CPR>(phi, c, ph);
+      run_store<V, CPR>(r, c, rv);
+    }
+    Rs = load(min(k + DEPTH, kend));
+    if (((k - k0 + 1) & (TILE_Z - 1)) == 0 || k == kend) {
+#pragma unroll
+      for (int off = 32; off >= CPR; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < CPR; e++)
+          acc[e] += __shfl_down(acc[e], off / CPR, WAVE);
+#pragma unroll
+      for (int off = CPR / 2; off >= 1; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < off; e++) acc[e] += acc[e + off];
+      if ((lane & (WAVE / CPR - 1)) == 0)
+        s_tp[ti][lane / (WAVE / CPR)][wv] = acc[0];
+#pragma unroll
+      for (int e = 0; e < CPR; e++) acc[e] = 0.0;
+      ti++;
+    }
+  };
+  int k = k0;
+  for (; k + DEPTH - 1 <= kend; k += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) step(k + d, R[d]);
+  }
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; d++)
+    if (k + d <= kend) step(k + d, R[d]);
+  __syncthreads();
+  const int ntx_ = (D.nx + TILE_X - 1) / TILE_X;
+  const int nty_ = (D.ny + TILE_Y - 1) / TILE_Y;
+  const int nout = ti * CPR * (RY / TILE_Y);
+  if ((int) threadIdx.x < nout) {
+    int t = threadIdx.x;
+    int tx = t % CPR, rg = (t / CPR) % (RY / TILE_Y), tz = t / (CPR * (RY / TILE_Y));
+    int ty = jb / TILE_Y + rg;
+    if (tx < ntx_ && ty < nty_) {
+      const double *w = &s_tp[tz][tx][rg * TILE_Y];
+      gpart[tx + (long) ntx_ * (ty + (long) nty_ * ((k0 - 1) / TILE_Z + tz))] =
+        ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    }
+  }
+}
 
 /* Every 50th iteration (src/cuda_solver.cu:208-224): phi += alpha p ... */
 __global__ void __launch_bounds__(TILE_THREADS)
@@ -1456,6 +1621,23 @@ int bb_row_last[4] = {0, 0, 0, 0};     /* the row kernel's last cells per
 int bb_row_depth = 0;                  /* planes of loads in flight in the
                                           FP32 row kernel: 3 or 4, else 2
                                           (FP64: 2) */
+int bb_update_variant = 1;             /* 1: update kernel with one
+                                          wavefront per x row where the grid
+                                          allows it (FP32; FP64 with
+                                          bb_urow_fp64), 0: one tile per
+                                          workgroup */
+int bb_urow_fp64 = 0;                  /* 1: the FP64 update takes the row
+                                          kernel too (at 256^3 it measured
+                                          slower: pair 177.6 us against
+                                          167.2 for the tile kernel) */
+int bb_urow_zc = 0;                    /* planes per workgroup of the row
+                                          update (multiple of 8 up to 64;
+                                          0: automatic) */
+int bb_urow_depth = 0;                 /* planes of loads in flight in the
+                                          FP32 row update: 4, else 2 (FP64:
+                                          2) */
+int bb_update_last = -1;               /* update kernel of the last launch:
+                                          1 row, 0 tile */
 int bb_row_zc = 0;                     /* planes per workgroup of the row
                                           kernel, a multiple of 8 up to 64
                                           (0: automatic) */
@@ -1928,6 +2110,76 @@ static bool phi_skips32(int q)
   return bb_defer_phi && ((q - pair_base32) & 1) == 0;
 }
 
+template <typename V, bool MASK, int PHI, int CPR, int DEPTH>
+static void urow_t(const InvMT<V> &T, int q, long nt, int zc, const V *p,
+  const V *Ap, const uint8_t *fm, V *phi, V *r, const V *pprev)
+{
+  const long nyb = (ddom.ny + 7) / 8, nzc = (ddom.nz + zc - 1) / zc;
+  hipLaunchKernelGGL((update_row_kernel<V, MASK, PHI, CPR, 8, DEPTH>),
+    dim3((unsigned) (nyb * nzc)), dim3(8 * WAVE), 0, bb_stream, ddom, T,
+    _cg_state, q, _part_p, nt, zc, p, Ap, fm, phi, r, _part_r, pprev);
+}
+
+template <typename V, bool MASK, int PHI, int DEPTH>
+static void urow_cpr(int cpr, const InvMT<V> &T, int q, long nt, int zc,
+  const V *p, const V *Ap, const uint8_t *fm, V *phi, V *r, const V *pprev)
+{
+  if (cpr == 4)
+    urow_t<V, MASK, PHI, 4, DEPTH>(T, q, nt, zc, p, Ap, fm, phi, r, pprev);
+  else if (cpr == 2)
+    urow_t<V, MASK, PHI, 2, DEPTH>(T, q, nt, zc, p, Ap, fm, phi, r, pprev);
+  else
+    urow_t<V, MASK, PHI, 1, DEPTH>(T, q, nt, zc, p, Ap, fm, phi, r, pprev);
+}
+
+/* The row update where it applies (phi update `mode`); false: the caller
+ * launches update_kernel. */
+template <typename V>
+static bool launch_update_row(int mode, const InvMT<V> &T, int q, long nt,
+  const V *p, const V *Ap, const uint8_t *fm, V *phi, V *r, const V *pprev)
+{
+  if (!bb_update_variant || (sizeof(V) == 8 && !bb_urow_fp64)) return false;
+  const int cpr = row_cpr();
+  if (!cpr) return false;
+  const size_t run = (size_t) cpr * sizeof(V);
+  if (!rows_aligned(p, sizeof(V), run) || !rows_aligned(Ap, sizeof(V), run)
+      || !rows_aligned(phi, sizeof(V), run) || !rows_aligned(r, sizeof(V), run)
+      || !rows_aligned(pprev, sizeof(V), run)
+      || (fm && !rows_aligned(fm, 1, (size_t) cpr)))
+    return false;
+  /* planes per workgroup: about 256 workgroups of 8 rows (one per CU of
+   * MI355X), in whole z-tiles, at most 8 of them (FP32 at 256^3: skip
+   * update 38.0 us with 32 planes, 40.1 with 16; tile kernel 44.8) */
+  int zc = bb_urow_zc;
+  if (zc <= 0 || zc % TILE_Z || zc > ROW_ZCMAX) {
+    long nyb = (ddom.ny + 7) / 8;
+    long ntz = (ddom.nz + TILE_Z - 1) / TILE_Z;
+    long t = nyb * ntz / 256;
+    zc = TILE_Z * (int) (t < 1 ? 1 : (t > ROW_ZCMAX / TILE_Z ? ROW_ZCMAX / TILE_Z : t));
+  }
+  /* two planes in flight (FP32 pair update at 256^3: 76.2 us, 77.6 with
+   * four, which take 184 VGPRs; FP64 with four spills) */
+  const int depth = sizeof(V) == 4 && bb_urow_depth == 4 ? 4 : 2;
+  bb_update_last = 1;
+#define UR(M, P, D) urow_cpr<V, M, P, D>(cpr, T, q, nt, zc, p, Ap, fm, phi, r, \
+    pprev)
+#define UR_MODE(M, D) do {                                                    \
+    if (mode == PHI_SKIP) UR(M, PHI_SKIP, D);                                 \
+    else if (mode == PHI_PAIR) UR(M, PHI_PAIR, D);                            \
+    else UR(M, PHI_EACH, D);                                                  \
+  } while (0)
+  if constexpr (sizeof(V) == 4) {
+    if (depth == 4) {
+      if (fm) UR_MODE(true, 4); else UR_MODE(false, 4);
+      return true;
+    }
+  }
+  if (fm) UR_MODE(true, 2); else UR_MODE(false, 2);
+#undef UR_MODE
+#undef UR
+  return true;
+}
+
 /* Enqueue iteration q; p buffers alternate by parity. */
 static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 {
@@ -1969,14 +2221,18 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
 #define UPD(M, P) hipLaunchKernelGGL((update_kernel<double, M, P>),          \
       TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fm,   \
       _phi, _r_q, _part_r, pold)
-    if (fm) {
-      if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
-      else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
-      else UPD(true, PHI_EACH);
-    } else {
-      if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
-      else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
-      else UPD(false, PHI_EACH);
+    if (!launch_update_row<double>(mode, T, q, nt, pnew, _Apb_q, fm, _phi,
+                                   _r_q, pold)) {
+      bb_update_last = 0;
+      if (fm) {
+        if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+        else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+        else UPD(true, PHI_EACH);
+      } else {
+        if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+        else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+        else UPD(false, PHI_EACH);
+      }
     }
 #undef UPD
     bb_kt_end(kt);
@@ -2213,14 +2469,18 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
 #define UPD(M, P) hipLaunchKernelGGL((update_kernel<float, M, P>),           \
     TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Ap32, fm,      \
     _e32, _r32, _part_r, pold)
-  if (fm) {
-    if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
-    else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
-    else UPD(true, PHI_EACH);
-  } else {
-    if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
-    else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
-    else UPD(false, PHI_EACH);
+  if (!launch_update_row<float>(mode, T, q, nt, pnew, _Ap32, fm, _e32, _r32,
+                                pold)) {
+    bb_update_last = 0;
+    if (fm) {
+      if (mode == PHI_SKIP) UPD(true, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(true, PHI_PAIR);
+      else UPD(true, PHI_EACH);
+    } else {
+      if (mode == PHI_SKIP) UPD(false, PHI_SKIP);
+      else if (mode == PHI_PAIR) UPD(false, PHI_PAIR);
+      else UPD(false, PHI_EACH);
+    }
   }
 #undef UPD
   bb_kt_end(kt);

@@ -1,5 +1,6 @@
-"""The search kernel with one wavefront per x row (variant 3) against the
-FP64 oracle and against the z-column kernel (variant 2).
+"""The search and update kernels with one wavefront per x row against the
+FP64 oracle and against the kernels that take one z-column or one tile per
+workgroup.
 
 The row kernel holds 1, 2 or 4 cells per lane (rows of up to 64, 128 or 256
 cells), loads the y neighbours from an LDS plane of its rows plus a halo row
@@ -7,7 +8,10 @@ on either side and keeps the z neighbours in registers.  Its arithmetic per
 cell and the order of its tile partials are those of the z-column kernel, so
 the plain form must match the oracle bit for bit and the mixed form must
 give the same bits as the z-column kernel, whatever the rows and planes per
-workgroup."""
+workgroup.  The row update streams whole rows of r, A p, p, phi (and the
+previous direction, the fluid flags) through a ring of planes in flight,
+with the arithmetic and partials of the tile update kernel: the same
+bits again, with and without particles."""
 import ctypes
 
 import numpy as np
@@ -23,15 +27,21 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "phi")
 
 
-def knobs(variant=3, ry=0, zc=0, fp64=0, depth=0):
-    """fp64 = 1: the plain form's FP64 search takes the row kernel too (by
-    default only the mixed form's FP32 search does)."""
+def knobs(variant=3, ry=0, zc=0, fp64=0, depth=0, uvar=1, ufp64=0, uzc=0,
+          udepth=0):
+    """fp64 / ufp64 = 1: the plain form's FP64 search / update take the row
+    kernels too (by default only the mixed form's FP32 ones do); uvar 0:
+    the tile update kernel."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
     ctypes.c_int.in_dll(lib, "bb_row_fp64").value = fp64
     ctypes.c_int.in_dll(lib, "bb_row_ry").value = ry
     ctypes.c_int.in_dll(lib, "bb_row_zc").value = zc
     ctypes.c_int.in_dll(lib, "bb_row_depth").value = depth
+    ctypes.c_int.in_dll(lib, "bb_update_variant").value = uvar
+    ctypes.c_int.in_dll(lib, "bb_urow_fp64").value = ufp64
+    ctypes.c_int.in_dll(lib, "bb_urow_zc").value = uzc
+    ctypes.c_int.in_dll(lib, "bb_urow_depth").value = udepth
     ctypes.c_int.in_dll(lib, "bb_defer_phi").value = 1
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = 16
 
@@ -46,6 +56,10 @@ def last_kernel():
     lib = bbhip.load()
     row = (ctypes.c_int * 4).in_dll(lib, "bb_row_last")
     return ctypes.c_int.in_dll(lib, "bb_spmv_last").value, tuple(row)
+
+
+def last_update():
+    return ctypes.c_int.in_dll(bbhip.load(), "bb_update_last").value
 
 
 def box(n, per_x, per_y):
@@ -69,9 +83,11 @@ def box(n, per_x, per_y):
                 extensions={"fixed_dt": 1e-3, "perturb": (7, 0.2)})
 
 
-def run_gpu(case_dir, kw, nsteps):
+def run_gpu(case_dir, kw, nsteps, parts=None):
     n = kw["n"]
     bbconfig.write_case(case_dir, **kw)
+    if parts is not None:
+        bbconfig.write_parts(case_dir, parts)
     out = {"f": {}, "iters": []}
     with bbhip.Simulation(case_dir, n) as s:
         for _ in range(nsteps):
@@ -81,6 +97,7 @@ def run_gpu(case_dir, kw, nsteps):
         for f in FIELDS:
             out["f"][f] = np.array(s.field(f))
     out["kernel"] = last_kernel()
+    out["update"] = last_update()
     return out
 
 
@@ -105,10 +122,11 @@ def run_oracle(case_dir, kw, nsteps):
 def test_row_kernel_plain_bitwise(case_dir, nx, per_x, per_y, zc, cpr):
     kw = box((nx, 11, 13), per_x, per_y)
     ref = run_oracle(case_dir, kw, 2)
-    knobs(variant=3, zc=zc, fp64=1)
+    knobs(variant=3, zc=zc, fp64=1, ufp64=1, uzc=zc)
     out = run_gpu(case_dir, kw, 2)
     kern, (c, ry, z, _) = out["kernel"]
     assert kern == 3 and c == cpr and ry == 8, out["kernel"]
+    assert out["update"] == 1
     assert z == (zc if zc else 8)
     print("iters gpu", out["iters"], "oracle", ref["iters"])
     assert out["iters"] == ref["iters"]
@@ -152,3 +170,38 @@ def test_row_kernel_mixed_same_bits(case_dir, n):
         assert r["iters"] == res[0]["iters"]
         for f in FIELDS:
             assert bitwise_equal(r["f"][f], res[0]["f"][f]), f
+
+
+@pytest.mark.parametrize("n", [(128, 20, 40), (256, 12, 24)])
+def test_row_update_mixed_same_bits(case_dir, n):
+    """The mixed form's FP32 update: row kernel with 2 and 4 planes in
+    flight and several plane groups against the tile kernel."""
+    kw = bbconfig.channel(n=n, precision="MIXED")
+    res = []
+    for uvar, uzc, udepth in ((0, 0, 0), (1, 0, 4), (1, 8, 2), (1, 24, 4)):
+        knobs(uvar=uvar, uzc=uzc, udepth=udepth)
+        res.append(run_gpu(case_dir, kw, 2))
+        assert res[-1]["update"] == uvar
+    for r in res[1:]:
+        assert r["iters"] == res[0]["iters"]
+        for f in FIELDS:
+            assert bitwise_equal(r["f"][f], res[0]["f"][f]), f
+
+
+@pytest.mark.parametrize("precision", ["PLAIN", "MIXED"])
+def test_row_kernels_particles_same_bits(case_dir, precision):
+    """Sedimentation, 64^3 with 6 spheres: the row search and update (the
+    update reading the fluid flags of the particle cells) against the
+    z-column search and tile update, in both forms."""
+    kw, parts = bbconfig.sediment(n=(64, 64, 64), nparts=6, a=5.0, seed=13)
+    kw["extensions"] = dict(kw.get("extensions") or {},
+                            pp_precision=precision)
+    res = []
+    for variant, uvar in ((2, 0), (3, 1)):
+        knobs(variant=variant, fp64=1, uvar=uvar, ufp64=1)
+        res.append(run_gpu(case_dir, kw, 2, parts))
+        assert res[-1]["kernel"][0] == variant
+        assert res[-1]["update"] == uvar
+    assert res[1]["iters"] == res[0]["iters"]
+    for f in FIELDS:
+        assert bitwise_equal(res[1]["f"][f], res[0]["f"][f]), f
