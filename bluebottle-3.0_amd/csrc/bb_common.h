@@ -35,6 +35,7 @@
 #define BB_COMMON_H
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -288,6 +289,15 @@ enum { KT_SPMV = 0, KT_UPDATE, KT_SEARCH, KT_RHS, KT_USTAR, KT_SPMV32,
 extern int bb_time_kernels;
 void bb_kt_begin(int id);
 void bb_kt_end(int id);
+/* The event pair of the timed launch in progress (NULL when this launch
+ * is not timed).  A timed kernel is launched through BB_TIMED_LAUNCH, which
+ * hands the pair to hipExtLaunchKernelGGL: the runtime records them at the
+ * kernel's own start and end, so a sample is the kernel's duration without
+ * the dispatch gap that events recorded on the stream around it add. */
+extern hipEvent_t bb_kt_start, bb_kt_stop;
+#define BB_TIMED_LAUNCH(...) BB_TIMED_LAUNCH_(__VA_ARGS__)
+#define BB_TIMED_LAUNCH_(K, G, B, S, ST, ...)                                 \
+  hipExtLaunchKernelGGL(K, G, B, S, ST, bb_kt_start, bb_kt_stop, 0, __VA_ARGS__)
 void bb_kt_collect(void);     /* call after a stream synchronisation */
 void bb_kt_collect_upto(long qmax);   /* drops samples tagged > qmax */
 extern long bb_kt_q;

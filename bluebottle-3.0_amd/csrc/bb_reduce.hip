@@ -41,8 +41,9 @@ double bb_fetch_scalar(int slot)
 }
 
 /* ------------------------------ kernel timing ---------------------------- */
-/* One launch in bb_time_every of each timed kernel is bracketed by a pair
- * of HIP events on the library stream.  A sample is tagged with the PCG
+/* One launch in bb_time_every of each timed kernel carries a pair of HIP
+ * events, recorded by the runtime at the kernel's start and end
+ * (BB_TIMED_LAUNCH).  A sample is tagged with the PCG
  * iteration (bb_kt_q) it belongs to and resolved when the solver has
  * fetched its state: the launches a batch enqueues past the iteration at
  * which the solve stopped return at once, and their samples are dropped
@@ -58,9 +59,12 @@ static long kt_seen[KT_COUNT];
 static double kt_ms[KT_COUNT];
 static long kt_n[KT_COUNT], kt_dropped[KT_COUNT];
 
+hipEvent_t bb_kt_start = NULL, bb_kt_stop = NULL;
+
 void bb_kt_begin(int id)
 {
   kt_active[id] = 0;
+  bb_kt_start = bb_kt_stop = NULL;
   if (!bb_time_kernels) return;
   if (bb_time_every > 1 && (kt_seen[id]++ % bb_time_every) != 0) return;
   kt_active[id] = 1;
@@ -74,13 +78,15 @@ void bb_kt_begin(int id)
   }
   if (kt_pending[id] == KT_SLOTS) bb_kt_collect();
   kt_tag[id][kt_pending[id]] = bb_kt_q;
-  BB_CHECK(hipEventRecord(kt_a[id][kt_pending[id]], bb_stream));
+  bb_kt_start = kt_a[id][kt_pending[id]];
+  bb_kt_stop = kt_b[id][kt_pending[id]];
 }
 
+/* Every timed region launches exactly one kernel through BB_TIMED_LAUNCH. */
 void bb_kt_end(int id)
 {
+  bb_kt_start = bb_kt_stop = NULL;
   if (!bb_time_kernels || !kt_active[id]) return;
-  BB_CHECK(hipEventRecord(kt_b[id][kt_pending[id]], bb_stream));
   kt_pending[id]++;
 }
 

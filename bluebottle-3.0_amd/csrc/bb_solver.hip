@@ -1994,7 +1994,7 @@ static void launch_row_t(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
 {
   const long nyb = (ddom.ny + RY - 1) / RY, nzc = (ddom.nz + zc - 1) / zc;
-  hipLaunchKernelGGL((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
+  BB_TIMED_LAUNCH((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
     dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K, T,
     _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);
 }
@@ -2082,7 +2082,7 @@ static void launch_group(const CoefT<V> &K, const InvMT<V> &T, int q,
   dim3 grid = spmv_group_grid(tzb);
   /* two planes in flight: three (FP32: 72.4 against 71.5 us at 256^3) and
    * four did not help, and past two the FP64 kernel spills at 8 waves */
-#define SG(NTP, NTS) hipLaunchKernelGGL((search_spmv_group_kernel<FIRST, V, \
+#define SG(NTP, NTS) BB_TIMED_LAUNCH((search_spmv_group_kernel<FIRST, V,    \
     MIXED, 2, NTP, NTS>), grid, tiles_block(), 0, bb_stream, ddom, K, T,    \
     _cg_state, q, _part_r, nrp, tzb, r, pold, pnew, Ap, _part_p)
   if (ntp && nts) SG(true, true);
@@ -2115,7 +2115,7 @@ static void urow_t(const InvMT<V> &T, int q, long nt, int zc, const V *p,
   const V *Ap, const uint8_t *fm, V *phi, V *r, const V *pprev)
 {
   const long nyb = (ddom.ny + 7) / 8, nzc = (ddom.nz + zc - 1) / zc;
-  hipLaunchKernelGGL((update_row_kernel<V, MASK, PHI, CPR, 8, DEPTH>),
+  BB_TIMED_LAUNCH((update_row_kernel<V, MASK, PHI, CPR, 8, DEPTH>),
     dim3((unsigned) (nyb * nzc)), dim3(8 * WAVE), 0, bb_stream, ddom, T,
     _cg_state, q, _part_p, nt, zc, p, Ap, fm, phi, r, _part_r, pprev);
 }
@@ -2197,11 +2197,11 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
   } else {
     bb_spmv_last = 0;
     if (q == 1)
-      hipLaunchKernelGGL(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
+      BB_TIMED_LAUNCH(search_spmv_direct_kernel<true>, TILES_LAUNCH, ddom,
         K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
         _Apb_q, _part_p);
     else
-      hipLaunchKernelGGL(search_spmv_direct_kernel<false>, TILES_LAUNCH, ddom,
+      BB_TIMED_LAUNCH(search_spmv_direct_kernel<false>, TILES_LAUNCH, ddom,
         K, T, _cg_state, q, _part_r, nt, _r_q, pold, fluid_mask(), pnew,
         _Apb_q, _part_p);
   }
@@ -2218,7 +2218,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
            : mode == PHI_PAIR ? KT_UPDATE_PAIR : KT_UPDATE;
     bb_kt_begin(kt);
     const uint8_t *fm = fluid_mask();
-#define UPD(M, P) hipLaunchKernelGGL((update_kernel<double, M, P>),          \
+#define UPD(M, P) BB_TIMED_LAUNCH((update_kernel<double, M, P>),             \
       TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Apb_q, fm,   \
       _phi, _r_q, _part_r, pold)
     if (!launch_update_row<double>(mode, T, q, nt, pnew, _Apb_q, fm, _phi,
@@ -2466,7 +2466,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
          : mode == PHI_PAIR ? KT_UPDATE32_PAIR : KT_UPDATE32;
   bb_kt_begin(kt);
   const uint8_t *fm = fluid_mask();
-#define UPD(M, P) hipLaunchKernelGGL((update_kernel<float, M, P>),           \
+#define UPD(M, P) BB_TIMED_LAUNCH((update_kernel<float, M, P>),              \
     TILES_LAUNCH, ddom, T, _cg_state, q, _part_p, nt, pnew, _Ap32, fm,      \
     _e32, _r32, _part_r, pold)
   if (!launch_update_row<float>(mode, T, q, nt, pnew, _Ap32, fm, _e32, _r32,
