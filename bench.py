@@ -150,7 +150,7 @@ def parse():
                     help="1: the FP64 update uses the row kernel too")
     ap.add_argument("--row-fp64", type=int, default=None,
                     help="1: the FP64 search uses the row kernel too "
-                         "(default 0)")
+                         "(default), 0: the z-column kernel")
     ap.add_argument("--tzb", type=int, default=None,
                     help="tiles per z-column of the search kernel "
                          "(0: automatic)")

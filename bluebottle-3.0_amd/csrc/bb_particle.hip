@@ -491,10 +491,17 @@ __global__ void face_list_kernel(PDom P, const int *__restrict__ phase,
 {
   const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
   long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G.n) return;
   int i, j, k, np;
-  if (!bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np)) return;
-  list[atomicAdd(&count[dir], 1)] = (int) t;
+  const bool ok = t < G.n && bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np);
+  /* one atomic per wavefront: lane 0 reserves the wavefront's entries and
+   * each listed lane writes at its rank among them (every lane of a
+   * launched wavefront reaches the ballot) */
+  const unsigned long long m = __ballot(ok);
+  const int lane = threadIdx.x & (WAVE - 1);
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(&count[dir], __popcll(m));
+  base = __shfl(base, 0);
+  if (ok) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int) t;
 }
 
 /* One wavefront per workgroup: the Lamb series needs more registers than

@@ -1605,11 +1605,12 @@ int bb_spmv_variant = 3;               /* 3: search kernel with one
                                           planes; 0: direct neighbour loads
                                           (one tile per workgroup, for
                                           cross-checks) */
-int bb_row_fp64 = 0;                   /* 1: the FP64 search uses the row
-                                          kernel too (at 256^3 it measured
-                                          144 us against 132 for the
-                                          z-column kernel: 172 VGPRs leave
-                                          two wavefronts per SIMD) */
+int bb_row_fp64 = 1;                   /* 1: the FP64 search uses the row
+                                          kernel too (256^3 with 500
+                                          spheres: 123.8 us against 130.4
+                                          for the z-column kernel, with 172
+                                          VGPRs, two wavefronts per SIMD);
+                                          0: the z-column kernel */
 int bb_row_ry = 0;                     /* rows per workgroup of the FP32 row
                                           kernel: 16, else 8 (FP64: 8) */
 int bb_spmv_last = -1;                 /* search kernel of the last

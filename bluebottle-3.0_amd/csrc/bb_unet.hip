@@ -260,6 +260,32 @@ conv_fwd_kernel(TV X, int cin, int cin_pad, const uint16_t *__restrict__ W,
   }
 }
 
+/* Four consecutive channels of one voxel as loaded by the staging loop
+ * of conv_lds_kernel: BF16 inputs stay packed (two registers), FP32 inputs
+ * are converted to BF16 when they are written to LDS. */
+template <typename IN> struct RawUnit;
+template <> struct RawUnit<uint16_t> {
+  typedef u16x4 T;
+  __device__ static T zero() { return (T) 0; }
+  __device__ static void set(T &r, int e, uint16_t x) { r[e] = x; }
+  __device__ static u16x4 bf16(const T &r) { return r; }
+};
+template <> struct RawUnit<float> {
+  typedef float4 T;
+  __device__ static T zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ static void set(T &r, int e, float x)
+  {
+    if (e == 0) r.x = x; else if (e == 1) r.y = x; else if (e == 2) r.z = x;
+    else r.w = x;
+  }
+  __device__ static u16x4 bf16(const T &r)
+  {
+    u16x4 h;
+    h[0] = f2bf(r.x); h[1] = f2bf(r.y); h[2] = f2bf(r.z); h[3] = f2bf(r.w);
+    return h;
+  }
+};
+
 /* The same convolution with the input staged through LDS.  A workgroup
  * owns a 16 x 4 x 4 block of output voxels (16 rows of 16 voxels along x,
  * four rows per wavefront); for each chunk of KS input channels it stages
@@ -281,6 +307,10 @@ conv_lds_kernel(TV X, int cin, int cin_pad, const uint16_t *__restrict__ W,
 {
   constexpr int NT = 4, KPL = KS / 4;
   constexpr int KCP = KS + 8;           /* LDS row: KS channels + padding */
+  /* taps per weight group: a z-plane of taps (9) where their fragments
+   * take at most 72 registers (KPL / 2 per fragment), else a row (3);
+   * all 27 at once cost the 16-channel kernel an occupancy step */
+  constexpr int TG = NCO * KPL * 9 <= 144 ? 9 : 3;
   typedef typename Frag<KS>::T F;
   typedef typename U16V<KPL>::T U;
   __shared__ __attribute__((aligned(16))) uint16_t tile[CB_V * KCP];
@@ -298,53 +328,72 @@ conv_lds_kernel(TV X, int cin, int cin_pad, const uint16_t *__restrict__ W,
     for (int c = 0; c < NCO; c++) acc[t][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
   const uint16_t *Wl = W + (long) (co0 + n) * cin_pad + KPL * q;
   for (int ch = 0; ch < cin_pad; ch += KS) {
-    /* stage: units of 4 channels of one voxel */
-    for (int u = threadIdx.x; u < CB_V * (KS / 4); u += 256) {
-      int v = u / (KS / 4), c4 = 4 * (u % (KS / 4));
-      int vx = v % (CB_X + 2), vy = (v / (CB_X + 2)) % (CB_Y + 2),
-          vz = v / ((CB_X + 2) * (CB_Y + 2));
-      int gx = x0 + vx - 1, gy = y0 + vy - 1, gz = z0 + vz - 1;
-      u16x4 h = (u16x4) 0;
-      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < X.nx && gy < X.ny &&
-          gz < X.nz) {
+    /* stage: units of 4 channels of one voxel, NU per thread.  Every load
+     * of the chunk is issued before the first LDS store, so the
+     * workgroup waits for its loads once and not once per unit. */
+    constexpr int NUNIT = CB_V * (KS / 4), NU = (NUNIT + 255) / 256;
+    /* BF16 input: the four channels packed as loaded; FP32: four floats */
+    typedef typename RawUnit<IN>::T RU;
+    RU rv[NU];
+#pragma unroll
+    for (int i = 0; i < NU; i++) {
+      const int u = threadIdx.x + 256 * i;
+      rv[i] = RawUnit<IN>::zero();
+      const int v = u / (KS / 4), c4 = 4 * (u % (KS / 4));
+      const int vx = v % (CB_X + 2), vy = (v / (CB_X + 2)) % (CB_Y + 2),
+                vz = v / ((CB_X + 2) * (CB_Y + 2));
+      const int gx = x0 + vx - 1, gy = y0 + vy - 1, gz = z0 + vz - 1;
+      if (u < NUNIT && gx >= 0 && gy >= 0 && gz >= 0 && gx < X.nx &&
+          gy < X.ny && gz < X.nz) {
         long row = (long) gx + (long) X.nx * (gy + (long) X.ny * gz);
         int c = ch + c4;
         const IN *p = (const IN *) X.p + row * X.ld + X.c0 + c;
         if (vec && c + 4 <= cin) {
-          if (sizeof(IN) == 2) {
-            h = *(const u16x4 *) p;
-          } else {
-            float4 f = *(const float4 *) p;
-            h[0] = f2bf(f.x); h[1] = f2bf(f.y); h[2] = f2bf(f.z); h[3] = f2bf(f.w);
-          }
+          rv[i] = *(const RU *) p;
         } else {
 #pragma unroll
           for (int e = 0; e < 4; e++)
-            if (c + e < cin)
-              h[e] = sizeof(IN) == 2 ? ((const uint16_t *) p)[e]
-                                     : f2bf(((const float *) p)[e]);
+            if (c + e < cin) RawUnit<IN>::set(rv[i], e, p[e]);
         }
       }
-      *(u16x4 *) &tile[v * KCP + c4] = h;
+    }
+#pragma unroll
+    for (int i = 0; i < NU; i++) {
+      const int u = threadIdx.x + 256 * i;
+      if (u < NUNIT) {
+        const int v = u / (KS / 4), c4 = 4 * (u % (KS / 4));
+        *(u16x4 *) &tile[v * KCP + c4] = RawUnit<IN>::bf16(rv[i]);
+      }
     }
     __syncthreads();
-    for (int tap = 0; tap < 27; tap++) {
-      int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   /* offset + 1 */
-      F bfr[NT];
+    /* taps in groups of TG: the group's weight fragments are loaded
+     * together before its MFMAs, so their latency is paid once per group
+     * and not once per tap (same MFMAs in the same order) */
+    for (int g0 = 0; g0 < 27; g0 += TG) {
+      F aw[TG][NCO];
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
-        int r = wv * NT + t;
-        int yy = r % CB_Y, zz = r / CB_Y;
-        int v = (n + dx) + (CB_X + 2) * ((yy + dy) + (CB_Y + 2) * (zz + dz));
-        bfr[t] = __builtin_bit_cast(F, *(const U *) &tile[v * KCP + KPL * q]);
-      }
-      const uint16_t *Wt = Wl + (long) tap * cout_pad * cin_pad + ch;
+      for (int tg = 0; tg < TG; tg++)
 #pragma unroll
-      for (int c = 0; c < NCO; c++) {
-        F a = load_w<KS>(Wt + (long) c * 16 * cin_pad);
+        for (int c = 0; c < NCO; c++)
+          aw[tg][c] = load_w<KS>(Wl + (long) (g0 + tg) * cout_pad * cin_pad +
+                                 ch + (long) c * 16 * cin_pad);
 #pragma unroll
-        for (int t = 0; t < NT; t++)
-          acc[t][c] = mfma_bf16<KS>(a, bfr[t], acc[t][c]);
+      for (int tg = 0; tg < TG; tg++) {
+        const int tap = g0 + tg;
+        int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;   /* offset + 1 */
+        F bfr[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          int r = wv * NT + t;
+          int yy = r % CB_Y, zz = r / CB_Y;
+          int v = (n + dx) + (CB_X + 2) * ((yy + dy) + (CB_Y + 2) * (zz + dz));
+          bfr[t] = __builtin_bit_cast(F, *(const U *) &tile[v * KCP + KPL * q]);
+        }
+#pragma unroll
+        for (int c = 0; c < NCO; c++)
+#pragma unroll
+          for (int t = 0; t < NT; t++)
+            acc[t][c] = mfma_bf16<KS>(aw[tg][c], bfr[t], acc[t][c]);
       }
     }
     __syncthreads();
@@ -492,22 +541,32 @@ __global__ void relu_bwd_kernel(TV G, TV Y, int C)
   if (!(bf2f(y) > 0.f)) ((float *) G.p)[v * G.ld + G.c0 + c] = 0.f;
 }
 
-/* 2x2x2 average pool, BF16 in and out (coarse dims from C). */
+/* 2x2x2 average pool, BF16 in and out (coarse dims from C): one thread per
+ * four channels of a coarse voxel (C, both row lengths and channel offsets
+ * multiples of 4, as every pooled view of the network has), 32-bit voxel
+ * indices; the eight children are summed in the same order for every
+ * channel. */
 __global__ void pool_fwd_kernel(TV F, TV Cr, int C)
 {
-  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  long nv = (long) Cr.nx * Cr.ny * Cr.nz;
-  if (t >= nv * C) return;
-  long v = t / C;
-  int c = (int) (t % C), x, y, z;
-  vox_xyz(Cr, v, &x, &y, &z);
-  float s = 0.f;
+  /* blockDim.x is a multiple of C / 4: a workgroup covers whole voxels */
+  const unsigned nv = (unsigned) Cr.nx * Cr.ny * Cr.nz, C4 = (unsigned) C / 4;
+  const unsigned v = blockIdx.x * (blockDim.x / C4) + threadIdx.x / C4;
+  const unsigned c = 4 * (threadIdx.x % C4);
+  if (v >= nv) return;
+  const unsigned x = v % Cr.nx, yz = v / Cr.nx;
+  const unsigned y = yz % Cr.ny, z = yz / Cr.ny;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
   for (int q = 0; q < 8; q++) {
-    int xf = 2 * x + (q & 1), yf = 2 * y + ((q >> 1) & 1), zf = 2 * z + (q >> 2);
-    long vf = (long) xf + (long) F.nx * (yf + (long) F.ny * zf);
-    s += bf2f(((const uint16_t *) F.p)[vf * F.ld + F.c0 + c]);
+    unsigned xf = 2 * x + (q & 1), yf = 2 * y + ((q >> 1) & 1), zf = 2 * z + (q >> 2);
+    unsigned vf = xf + F.nx * (yf + F.ny * zf);
+    u16x4 h = *(const u16x4 *) ((const uint16_t *) F.p + (size_t) vf * F.ld + F.c0 + c);
+#pragma unroll
+    for (int e = 0; e < 4; e++) s[e] += bf2f(h[e]);
   }
-  ((uint16_t *) Cr.p)[v * Cr.ld + Cr.c0 + c] = f2bf(0.125f * s);
+  u16x4 o;
+#pragma unroll
+  for (int e = 0; e < 4; e++) o[e] = f2bf(0.125f * s[e]);
+  *(u16x4 *) ((uint16_t *) Cr.p + (size_t) v * Cr.ld + Cr.c0 + c) = o;
 }
 
 /* Pool backward: G_fine[v, c] += 0.125 G_coarse[parent(v), c] (FP32). */
@@ -524,18 +583,20 @@ __global__ void pool_bwd_kernel(TV Gf, TV Gc, int C)
     0.125f * ((const float *) Gc.p)[vc * Gc.ld + Gc.c0 + c];
 }
 
-/* Nearest upsample x2, BF16. */
+/* Nearest upsample x2, BF16: one thread per four channels of a fine voxel
+ * (the same alignment as the pool), 32-bit voxel indices. */
 __global__ void up_fwd_kernel(TV Cr, TV F, int C)
 {
-  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  long nv = (long) F.nx * F.ny * F.nz;
-  if (t >= nv * C) return;
-  long v = t / C;
-  int c = (int) (t % C), x, y, z;
-  vox_xyz(F, v, &x, &y, &z);
-  long vc = (long) (x / 2) + (long) Cr.nx * ((y / 2) + (long) Cr.ny * (z / 2));
-  ((uint16_t *) F.p)[v * F.ld + F.c0 + c] =
-    ((const uint16_t *) Cr.p)[vc * Cr.ld + Cr.c0 + c];
+  /* blockDim.x is a multiple of C / 4: a workgroup covers whole voxels */
+  const unsigned nv = (unsigned) F.nx * F.ny * F.nz, C4 = (unsigned) C / 4;
+  const unsigned v = blockIdx.x * (blockDim.x / C4) + threadIdx.x / C4;
+  const unsigned c = 4 * (threadIdx.x % C4);
+  if (v >= nv) return;
+  const unsigned x = v % F.nx, yz = v / F.nx;
+  const unsigned y = yz % F.ny, z = yz / F.ny;
+  const unsigned vc = x / 2 + Cr.nx * (y / 2 + Cr.ny * (z / 2));
+  *(u16x4 *) ((uint16_t *) F.p + (size_t) v * F.ld + F.c0 + c) =
+    *(const u16x4 *) ((const uint16_t *) Cr.p + (size_t) vc * Cr.ld + Cr.c0 + c);
 }
 
 /* Upsample backward: G_coarse[v, c] = sum of the 8 children (FP32). */
@@ -895,6 +956,20 @@ static void ew(K kern, long n, Args... args)
     args...);
 }
 
+/* Launch of a kernel with one thread per four channels of each of nv
+ * voxels (C channels, C / 4 dividing the 256-thread workgroup). */
+template <typename K, typename... Args>
+static void vox4(K kern, long nv, int C, Args... args)
+{
+  const long vpb = 256 / (C / 4);
+  if (nv > 0xffffffffL) {
+    fprintf(stderr, "U-Net volume too large (%ld voxels)\n", nv);
+    exit(EXIT_FAILURE);
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned) ((nv + vpb - 1) / vpb)), dim3(256),
+    0, bb_stream, args...);
+}
+
 /* Forward pass on the BF16 2-channel input already in A.in0 (16-channel
  * rows, channels 0 and 1); output FP32 in A.y. */
 static void forward(void)
@@ -902,20 +977,20 @@ static void forward(void)
   Acts &A = *net->A;
   conv(0, tv(A.in0, 16, 0, A, 0), false, none(), tv(A.e0a, 16, 0, A, 0));
   conv(1, tv(A.e0a, 16, 0, A, 0), false, none(), tv(A.cat0, 48, 32, A, 0));
-  ew(pool_fwd_kernel, A.nv[1] * 16, tv(A.cat0, 48, 32, A, 0),
-     tv(A.p0, 16, 0, A, 1), 16);
+  vox4(pool_fwd_kernel, A.nv[1], 16, tv(A.cat0, 48, 32, A, 0),
+       tv(A.p0, 16, 0, A, 1), 16);
   conv(2, tv(A.p0, 16, 0, A, 1), false, none(), tv(A.e1a, 32, 0, A, 1));
   conv(3, tv(A.e1a, 32, 0, A, 1), false, none(), tv(A.cat1, 96, 64, A, 1));
-  ew(pool_fwd_kernel, A.nv[2] * 32, tv(A.cat1, 96, 64, A, 1),
-     tv(A.p1, 32, 0, A, 2), 32);
+  vox4(pool_fwd_kernel, A.nv[2], 32, tv(A.cat1, 96, 64, A, 1),
+       tv(A.p1, 32, 0, A, 2), 32);
   conv(4, tv(A.p1, 32, 0, A, 2), false, none(), tv(A.b0, 64, 0, A, 2));
   conv(5, tv(A.b0, 64, 0, A, 2), false, none(), tv(A.b1, 64, 0, A, 2));
-  ew(up_fwd_kernel, A.nv[1] * 64, tv(A.b1, 64, 0, A, 2),
-     tv(A.cat1, 96, 0, A, 1), 64);
+  vox4(up_fwd_kernel, A.nv[1], 64, tv(A.b1, 64, 0, A, 2),
+       tv(A.cat1, 96, 0, A, 1), 64);
   conv(6, tv(A.cat1, 96, 0, A, 1), false, none(), tv(A.d1a, 32, 0, A, 1));
   conv(7, tv(A.d1a, 32, 0, A, 1), false, none(), tv(A.d1, 32, 0, A, 1));
-  ew(up_fwd_kernel, A.nv[0] * 32, tv(A.d1, 32, 0, A, 1),
-     tv(A.cat0, 48, 0, A, 0), 32);
+  vox4(up_fwd_kernel, A.nv[0], 32, tv(A.d1, 32, 0, A, 1),
+       tv(A.cat0, 48, 0, A, 0), 32);
   conv(8, tv(A.cat0, 48, 0, A, 0), false, none(), tv(A.d0a, 16, 0, A, 0));
   conv(9, tv(A.d0a, 16, 0, A, 0), false, none(), tv(A.d0, 16, 0, A, 0));
   conv(10, tv(A.d0, 16, 0, A, 0), false, tv(A.y, 1, 0, A, 0), none());
@@ -1010,8 +1085,8 @@ __global__ void pack_input_kernel(const float *__restrict__ in, long nv,
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
-  x[v * 16] = f2bf(in[2 * v]);
-  x[v * 16 + 1] = f2bf(in[2 * v + 1]);
+  const float2 f = ((const float2 *) in)[v];
+  *(uint32_t *) &x[v * 16] = (uint32_t) f2bf(f.x) | ((uint32_t) f2bf(f.y) << 16);
 }
 
 /* Forward on a device input (FP32, 2 channels, channels-last) of the

@@ -48,6 +48,10 @@ unsigned long long bb_warm_seed = 20260104ULL;
 int bb_warm_used = 0;                  /* 1: phi0 came from the network */
 double bb_warm_alpha = 0.;
 double bb_warm_loss = 0.;              /* last training loss */
+#define LOSS_LOG 4096
+double bb_warm_loss_log[LOSS_LOG];     /* loss of every Adam step, in order
+                                          (the first LOSS_LOG) */
+int bb_warm_loss_n = 0;
 int bb_warm_solves = 0;                /* solves seen */
 
 int bb_warm_use_net = 1;               /* 0: span{phi_last} only */
@@ -241,6 +245,7 @@ extern "C" void bb_warm_setup(void)
   bb_warm_err_last = bb_warm_err_guess = bb_warm_err_hist = 0.;
   bb_warm_err_last_sum = bb_warm_err_guess_sum = bb_warm_err_hist_sum = 0.;
   bb_warm_net_solves = 0;
+  bb_warm_loss_n = 0;
   BB_CHECK(hipHostMalloc((void **) &wpart_host, MBLK * 4 * sizeof(double),
     hipHostMallocDefault));
   bb_warm_solves = 0;
@@ -385,6 +390,7 @@ static void train_pair(void)
       bb_stream, ddom, _win, _wtgt, e, ox, oy, oz, _cin, _ctgt);
     bb_unet_forward(_cin, _wout);
     bb_warm_loss = bb_unet_backward(_ctgt);
+    if (bb_warm_loss_n < LOSS_LOG) bb_warm_loss_log[bb_warm_loss_n++] = bb_warm_loss;
     bb_unet_adam(bb_warm_lr);
   }
   BB_LAUNCH_CHECK();

@@ -170,7 +170,10 @@ void in_restart(void)
       parts[i].rotating = cfg[N].rotating;
       if (wng) { printf("N%d >> part moving updated\n", rank); wng = 0; }
     }
-    if (fabs(parts[i].E - cfg[N].E) > 1e-6) {
+    /* src/domain.c:3224 tests abs() of the difference, the int function:
+     * the difference is truncated toward zero first, so an E that changed
+     * by less than 1 keeps its restart value; the same test here */
+    if (fabs(trunc(parts[i].E - cfg[N].E)) > 1e-6) {
       parts[i].E = cfg[N].E;
       if (wng) { printf("N%d >> part E updated\n", rank); wng = 0; }
     }

@@ -17,8 +17,14 @@
  * definition here.
  * Compile with -ffp-contract=off.
  *
- * Pinning: see tests/test_oracle.py (closed forms of src/cuda_testing.cu,
- * exactness on polynomials, dense brute-force Poisson solves, invariants).
+ * Pinning: tests/test_oracle.py (u* on exponential and sine fields with
+ * second-order convergence, exact Dirichlet ghost formulas, periodic
+ * planes, the operator against an independent sparse matrix, the Laplacian
+ * on a quadratic, PCG against a dense solve, Taylor-Green decay, cavity
+ * invariants); forcing by a steady Poiseuille channel and find_dt's
+ * velocity branch (tests/test_oracle_dynamics.py).  Parity unpinned: the
+ * reference's thrust summation order (the canonical order above is the
+ * definition).
  */
 #include <float.h>
 #include <math.h>

@@ -27,6 +27,37 @@
  * Walls: a node beyond a wall face takes that face's code and value
  * (src/physalis_kernel.cu:118-214, 437-640), and the wall forces of
  * src/particle_kernel.cu:2347-3073 are restated face by face below.
+ *
+ * Pinning (each group of functions against mathematics that does not come
+ * from the reference's source):
+ *   Lebedev nodes/weights  closed-form weights, exactness on spherical
+ *                          harmonics to degree 7 (test_oracle_particles.py)
+ *   Lamb series (BCs, p fill, internal velocity)
+ *                          Stokes equations and no slip of every p/phi
+ *                          term, Lamb's toroidal radial function, the
+ *                          reference's own cuda_lamb_test boundary values
+ *                          (test_oracle_lamb.py)
+ *   node interpolation     exact on linear fields, second order with
+ *                          periodic images (test_oracle_lamb.py)
+ *   quadrature + coefficient solve
+ *                          round trip of chosen coefficients of orders
+ *                          0-3, order-4 aliasing stays out of n <= 3,
+ *                          Stokes drag, rotlet torque, Batchelor's strain
+ *                          flow (test_oracle_lamb.py)
+ *   cages, masked operator brute-force cage geometry, the masked-operator
+ *                          identity (test_oracle_particles.py)
+ *   pair lubrication/contact
+ *                          Jeffrey-Onishi asymptotes (equal and unequal
+ *                          spheres, reciprocity), Hertz normal force
+ *                          (test_oracle_dynamics.py)
+ *   wall forces            closed-form lubrication, Hertz contact, mirror
+ *                          symmetries between faces (test_oracle_walls.py)
+ *   springs                Hooke's law, periodic wrap, oscillator period
+ *                          (test_oracle_dynamics.py)
+ *   motion, rotation       free fall, Rodrigues' formula for the body axes
+ *                          (test_oracle_dynamics.py)
+ * Parity unpinned: the reference's thrust summation order (the canonical
+ * order of the library is the definition here).
  */
 
 #define O_NNODES 26

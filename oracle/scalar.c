@@ -12,6 +12,12 @@
  *   step order          src/bluebottle.c:189-192, 293-341
  *   particle heat       src/scalar.c:90-115, src/scalar_kernel.cu:303-577,
  *                       1143-1360, src/cuda_scalar.cu:866-969
+ *
+ * Pinning (tests/test_oracle_scalar.py): Fourier-mode decay, conservation
+ * of the flux form, steady wall profiles, the Boussinesq force of a
+ * uniform field, the diffusivity limit of the time step; particle heat by
+ * exact quadrature of a linear field, the conducting-sphere flux 4 pi a
+ * dT, the series in particle cells and the particle temperature update.
  */
 
 static double O_sD, O_sk, O_sinit, O_sinit_rand, O_slamb_cut;

@@ -27,10 +27,11 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("u", "v", "w", "p", "phi")
 
 
-def knobs(variant=3, ry=0, zc=0, fp64=0, depth=0, uvar=1, ufp64=0, uzc=0,
+def knobs(variant=3, ry=0, zc=0, fp64=1, depth=0, uvar=1, ufp64=0, uzc=0,
           udepth=0):
-    """fp64 / ufp64 = 1: the plain form's FP64 search / update take the row
-    kernels too (by default only the mixed form's FP32 ones do); uvar 0:
+    """fp64 = 0: the plain form's FP64 search takes the z-column kernel
+    (the row kernel by default); ufp64 = 1: its FP64 update takes the row
+    kernel too (by default only the mixed form's FP32 one does); uvar 0:
     the tile update kernel."""
     lib = bbhip.load()
     ctypes.c_int.in_dll(lib, "bb_spmv_variant").value = variant
