@@ -84,13 +84,13 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 37.80,
-    "update_kernel": 51.15,
-    "update_kernel_skip": 26.05,
-    "update_kernel_pair": 59.84,
-    "spmv_search_kernel_f32": 17.71,
-    "update_kernel_f32_skip": 13.51,
-    "update_kernel_f32_pair": 30.28,
+    "spmv_search_kernel": 37.29,
+    "update_kernel": 49.16,
+    "update_kernel_skip": 25.15,
+    "update_kernel_pair": 57.17,
+    "spmv_search_kernel_f32": 16.53,
+    "update_kernel_f32_skip": 13.20,
+    "update_kernel_f32_pair": 29.34,
 }
 
 
@@ -178,6 +178,11 @@ def parse():
     ap.add_argument("--calib-only", action="store_true",
                     help="run only the stream-copy calibration kernel "
                          "(2^27 doubles per array) and print its GB/s")
+    ap.add_argument("--unet", default=None, metavar="N[,N...]",
+                    help="run only the learned guess's network: five "
+                         "forward passes at each N^3 volume and eight "
+                         "training steps at 64^3, timed with device "
+                         "synchronisation (for the kernel trace)")
     a = ap.parse_args()
     dflt = (256, 256, 256) if a.workload == "sediment" else (256, 128, 128)
     a.n = tuple(v if v is not None else d
@@ -384,6 +389,42 @@ def cpu_baseline(a, iters_per_step, lamb_per_step):
                       f"(the oracle's own counts: same arithmetic)"}
 
 
+def unet_only(lib, vols):
+    """The network alone: five forward passes per N^3 volume (random
+    inputs in torch device tensors), then eight forward + backward + Adam
+    steps at 64^3; prints the mean ms per forward pass and per training
+    step."""
+    import torch
+    lib.bb_unet_create.argtypes = [ctypes.c_ulonglong]
+    lib.bb_unet_volume.argtypes = [ctypes.c_int] * 3
+    lib.bb_unet_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.bb_unet_backward.argtypes = [ctypes.c_void_p]
+    lib.bb_unet_backward.restype = ctypes.c_double
+    lib.bb_unet_adam.argtypes = [ctypes.c_float]
+    lib.bb_unet_create(7)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    out = {}
+    for n, reps, train in [(v, 5, False) for v in vols] + [(64, 8, True)]:
+        nv = n ** 3
+        x = torch.randn(2 * nv, device="cuda", generator=gen)
+        t = torch.randn(nv, device="cuda", generator=gen)
+        y = torch.zeros(nv, device="cuda")
+        lib.bb_unet_volume(n, n, n)
+        torch.cuda.synchronize()
+        lib.cuda_block()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lib.bb_unet_forward(x.data_ptr(), y.data_ptr())
+            if train:
+                lib.bb_unet_backward(t.data_ptr())
+                lib.bb_unet_adam(1e-3)
+        lib.cuda_block()
+        key = f"train_{n}" if train else f"forward_{n}"
+        out[key + "_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+    lib.bb_unet_destroy()
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -398,6 +439,9 @@ def main():
     lib.bb_stream4_gbs.argtypes = [ctypes.c_long, ctypes.c_int]
     cells = a.n[0] * a.n[1] * a.n[2]
     stream4_gbs = lib.bb_stream4_gbs(cells, 10)
+    if a.unet:
+        unet_only(lib, [int(v) for v in a.unet.split(",")])
+        return
     if a.calib_only:
         print(json.dumps({"stream_copy_gbs": copy_gbs,
                           "elements": 1 << 27,

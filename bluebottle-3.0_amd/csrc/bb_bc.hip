@@ -71,10 +71,16 @@ __global__ void bc_pair_kernel(double *a, Grid G, int dir, int klo,
   apply_side(a, b, S, e, 0, khi, vhi);
 }
 
-/* Periodic planes of one direction; `stag` = 1 if the grid is staggered in
- * that direction. */
-__global__ void wrap_kernel(double *a, Grid G, int dir, int stag)
+/* The periodic planes of every periodic direction in one launch
+ * (blockIdx.z picks the direction; `stag` = 1 if the grid is staggered in
+ * that direction).  Each direction's copies read and write only points
+ * whose other two indices are interior, so the directions are independent
+ * and one launch gives what one launch per direction gives. */
+struct WrapDirs { int n; int dir[3]; int stag[3]; };
+
+__global__ void wrap_dirs_kernel(double *a, Grid G, WrapDirs W)
 {
+  const int dir = W.dir[blockIdx.z], stag = W.stag[blockIdx.z];
   int n1 = (dir == 0) ? G.nj : G.ni;
   int n2 = (dir == 2) ? G.nj : G.nk;
   int t1 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,14 +120,23 @@ void bb_wrap(double *a, int grid)
 {
   const Grid &G = grid_of(grid);
   int per[3] = {ddom.perx, ddom.pery, ddom.perz};
+  WrapDirs W;
+  W.n = 0;
+  unsigned gx = 0, gy = 0;
   for (int dir = 0; dir < 3; dir++) {
     if (!per[dir]) continue;
-    int stag = (grid == GFX && dir == 0) || (grid == GFY && dir == 1) ||
-               (grid == GFZ && dir == 2);
+    W.dir[W.n] = dir;
+    W.stag[W.n] = (grid == GFX && dir == 0) || (grid == GFY && dir == 1) ||
+                  (grid == GFZ && dir == 2);
+    W.n++;
     dim3 gr, bl;
     launch_plane(G, dir, gr, bl);
-    hipLaunchKernelGGL(wrap_kernel, gr, bl, 0, bb_stream, a, G, dir, stag);
+    gx = gr.x > gx ? gr.x : gx;
+    gy = gr.y > gy ? gr.y : gy;
   }
+  if (W.n == 0) return;
+  hipLaunchKernelGGL(wrap_dirs_kernel, dim3(gx, gy, (unsigned) W.n),
+    dim3(64, 4), 0, bb_stream, a, G, W);
   BB_LAUNCH_CHECK();
 }
 

@@ -471,8 +471,12 @@ __device__ inline bool bc_face(const PDom &P, long t, const int *phase,
   const int *flag, int *i_, int *j_, int *k_, int *np)
 {
   const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
-  int i = (int) (t % G.sy), j = (int) ((t / G.sy) % (G.sz / G.sy)),
-      k = (int) (t / G.sz);
+  /* 32-bit index arithmetic: the face lists are built only for grids of
+   * fewer than 2^31 elements (bb_particle_face_lists) */
+  const unsigned tt = (unsigned) t, sy = (unsigned) G.sy;
+  const unsigned row = tt / sy;
+  int i = (int) (tt - row * sy), j = (int) (row % (unsigned) (G.sz / G.sy)),
+      k = (int) (tt / (unsigned) G.sz);
   if (dir == 0 && (i < 1 || i > P.nx + 1 || j > P.ny + 1 || k > P.nz + 1)) return false;
   if (dir == 1 && (j < 1 || j > P.ny + 1 || i > P.nx + 1 || k > P.nz + 1)) return false;
   if (dir == 2 && (k < 1 || k > P.nz + 1 || i > P.nx + 1 || j > P.ny + 1)) return false;
@@ -485,23 +489,98 @@ __device__ inline bool bc_face(const PDom &P, long t, const int *phase,
   return true;
 }
 
+/* The face lists are compacted without a shared counter: a counter that
+ * every wavefront holding a listed face bumps serialises those atomics in
+ * L2 (about 75 000 per direction at 256^3 with 500 spheres, 0.7 ms).
+ * Instead each workgroup counts the listed faces of its FL_SPAN
+ * consecutive faces (face_count_kernel), one workgroup turns the counts
+ * into offsets (face_offsets_kernel), and each workgroup writes its faces
+ * at its offset in increasing index order (face_write_kernel): the list
+ * comes out sorted. */
+#define FL_ITEMS 8
+#define FL_SPAN (FL_ITEMS * 256)
+
+/* Listed faces among this thread's item `it` of workgroup b, as the
+ * ballot of its wavefront. */
 template <int dir>
-__global__ void face_list_kernel(PDom P, const int *__restrict__ phase,
-  const int *__restrict__ flag, int *__restrict__ count, int *__restrict__ list)
+__device__ inline unsigned long long fl_ballot(const PDom &P, long n,
+  const int *phase, const int *flag, long b, int it, long *t_out)
 {
-  const Grid G = dir == 0 ? P.fx : (dir == 1 ? P.fy : P.fz);
-  long t = (long) blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = b * FL_SPAN + (long) it * 256 + threadIdx.x;
   int i, j, k, np;
-  const bool ok = t < G.n && bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np);
-  /* one atomic per wavefront: lane 0 reserves the wavefront's entries and
-   * each listed lane writes at its rank among them (every lane of a
-   * launched wavefront reaches the ballot) */
-  const unsigned long long m = __ballot(ok);
-  const int lane = threadIdx.x & (WAVE - 1);
-  int base = 0;
-  if (lane == 0 && m) base = atomicAdd(&count[dir], __popcll(m));
-  base = __shfl(base, 0);
-  if (ok) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int) t;
+  const bool ok = t < n && bc_face<dir>(P, t, phase, flag, &i, &j, &k, &np);
+  *t_out = ok ? t : -1;
+  return __ballot(ok);
+}
+
+template <int dir>
+__global__ void __launch_bounds__(256)
+face_count_kernel(PDom P, long n, const int *__restrict__ phase,
+  const int *__restrict__ flag, int *__restrict__ bcount)
+{
+  __shared__ int s_n[4];
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x >> 6;
+  int c = 0;
+  for (int it = 0; it < FL_ITEMS; it++) {
+    long t;
+    c += __popcll(fl_ballot<dir>(P, n, phase, flag, blockIdx.x, it, &t));
+  }
+  if (lane == 0) s_n[wv] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) bcount[blockIdx.x] = (s_n[0] + s_n[1]) + (s_n[2] + s_n[3]);
+}
+
+/* Exclusive prefix sums of the nb workgroup counts (one workgroup of 1024
+ * threads, each summing a run of consecutive counts) and the total. */
+__global__ void __launch_bounds__(1024)
+face_offsets_kernel(const int *__restrict__ bcount, int nb,
+  int *__restrict__ boff, int *__restrict__ total)
+{
+  __shared__ int s[1024];
+  const int per = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * per;
+  int sum = 0;
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) sum += bcount[b0 + q];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {     /* inclusive scan */
+    int v = threadIdx.x >= (unsigned) off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - sum;
+  for (int q = 0; q < per; q++)
+    if (b0 + q < nb) {
+      boff[b0 + q] = run;
+      run += bcount[b0 + q];
+    }
+  if (threadIdx.x == 1023) *total = s[1023];
+}
+
+template <int dir>
+__global__ void __launch_bounds__(256)
+face_write_kernel(PDom P, long n, const int *__restrict__ phase,
+  const int *__restrict__ flag, const int *__restrict__ boff,
+  int *__restrict__ list)
+{
+  __shared__ int s_n[4];
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x >> 6;
+  int base = boff[blockIdx.x];
+  for (int it = 0; it < FL_ITEMS; it++) {
+    long t;
+    const unsigned long long m =
+      fl_ballot<dir>(P, n, phase, flag, blockIdx.x, it, &t);
+    if (lane == 0) s_n[wv] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wv; w++) before += s_n[w];
+    if (t >= 0)
+      list[base + before + __popcll(m & ((1ull << lane) - 1ull))] = (int) t;
+    base += (s_n[0] + s_n[1]) + (s_n[2] + s_n[3]);
+    __syncthreads();
+  }
 }
 
 /* One wavefront per workgroup: the Lamb series needs more registers than
@@ -554,6 +633,8 @@ __global__ void part_bc_apply_kernel(double *vel, const int *__restrict__ list,
  * until a function that changes the particles or the cages runs. */
 static int *_bc_list[3] = {NULL, NULL, NULL};
 static int *_bc_count = NULL;
+static int *_bc_bcount = NULL, *_bc_boff = NULL;  /* per-workgroup counts,
+                                                   offsets */
 static int bc_n[3] = {0, 0, 0};
 static double *_bc_val[3] = {NULL, NULL, NULL};
 static int bc_vals_ok = 0;
@@ -563,21 +644,40 @@ void bb_particle_face_lists(void)
 {
   PDom P = pdom();
   const long gn[3] = {ddom.fx.n, ddom.fy.n, ddom.fz.n};
+  long nbmax = 0;
+  for (int d = 0; d < 3; d++) {
+    if (gn[d] >= (1L << 31)) {
+      fprintf(stderr, "particle face lists: a face grid of %ld elements "
+        "exceeds the 32-bit face index\n", gn[d]);
+      exit(EXIT_FAILURE);
+    }
+    long nb = (gn[d] + FL_SPAN - 1) / FL_SPAN;
+    nbmax = nb > nbmax ? nb : nbmax;
+  }
   if (!_bc_count) {
     BB_CHECK(hipMalloc((void **) &_bc_count, 3 * sizeof(int)));
+    BB_CHECK(hipMalloc((void **) &_bc_bcount, nbmax * sizeof(int)));
+    BB_CHECK(hipMalloc((void **) &_bc_boff, nbmax * sizeof(int)));
     for (int d = 0; d < 3; d++) {
       BB_CHECK(hipMalloc((void **) &_bc_list[d], gn[d] * sizeof(int)));
       BB_CHECK(hipMalloc((void **) &_bc_val[d], gn[d] * sizeof(double)));
     }
   }
   bc_vals_ok = 0;
-  BB_CHECK(hipMemsetAsync(_bc_count, 0, 3 * sizeof(int), bb_stream));
-  hipLaunchKernelGGL(face_list_kernel<0>, dim3(nblk(gn[0], 256)), dim3(256),
-    0, bb_stream, P, _phase, _flag_u, _bc_count, _bc_list[0]);
-  hipLaunchKernelGGL(face_list_kernel<1>, dim3(nblk(gn[1], 256)), dim3(256),
-    0, bb_stream, P, _phase, _flag_v, _bc_count, _bc_list[1]);
-  hipLaunchKernelGGL(face_list_kernel<2>, dim3(nblk(gn[2], 256)), dim3(256),
-    0, bb_stream, P, _phase, _flag_w, _bc_count, _bc_list[2]);
+  const int *flags[3] = {_flag_u, _flag_v, _flag_w};
+  for (int d = 0; d < 3; d++) {
+    const unsigned nb = (unsigned) ((gn[d] + FL_SPAN - 1) / FL_SPAN);
+#define FL(D) do {                                                          \
+      hipLaunchKernelGGL(face_count_kernel<D>, dim3(nb), dim3(256), 0,      \
+        bb_stream, P, gn[D], _phase, flags[D], _bc_bcount);                 \
+      hipLaunchKernelGGL(face_offsets_kernel, dim3(1), dim3(1024), 0,        \
+        bb_stream, _bc_bcount, (int) nb, _bc_boff, _bc_count + D);          \
+      hipLaunchKernelGGL(face_write_kernel<D>, dim3(nb), dim3(256), 0,      \
+        bb_stream, P, gn[D], _phase, flags[D], _bc_boff, _bc_list[D]);      \
+    } while (0)
+    if (d == 0) FL(0); else if (d == 1) FL(1); else FL(2);
+#undef FL
+  }
   BB_LAUNCH_CHECK();
   BB_CHECK(hipMemcpyAsync(bc_n, _bc_count, 3 * sizeof(int),
     hipMemcpyDeviceToHost, bb_stream));
@@ -596,7 +696,9 @@ static void free_face_lists(void)
   }
   bc_vals_ok = 0;
   BB_CHECK(hipFree(_bc_count));
-  _bc_count = NULL;
+  BB_CHECK(hipFree(_bc_bcount));
+  BB_CHECK(hipFree(_bc_boff));
+  _bc_count = _bc_bcount = _bc_boff = NULL;
 }
 
 static void part_bc(double *u, double *v, double *w)

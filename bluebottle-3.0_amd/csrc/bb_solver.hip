@@ -1824,19 +1824,54 @@ static void sym_eigen(int m, double a[GMAX][GMAX], double v[GMAX][GMAX])
 }
 
 
+/* Coefficients c of the Galerkin system G c = g on the leading m
+ * directions, through the eigen-decomposition of G with directions of
+ * relative eigenvalue below 1e-12 dropped (nearly parallel history
+ * vectors); returns the energy c . g they capture, 0 for no guess (c = 0,
+ * also when a coefficient is not finite). */
+static double galerkin_solve(int m, const double G0[GMAX][GMAX],
+  const double g[GMAX], double c[GMAX])
+{
+  double G[GMAX][GMAX], E[GMAX][GMAX];
+  for (int p = 0; p < GMAX; p++) {
+    c[p] = 0.;
+    for (int q = 0; q < GMAX; q++) G[p][q] = G0[p][q];
+  }
+  sym_eigen(m, G, E);
+  double lmax = 0.;
+  for (int q = 0; q < m; q++) lmax = fmax(lmax, G[q][q]);
+  if (!(lmax > 0.) || !isfinite(lmax)) return 0.;
+  for (int q = 0; q < m; q++) {
+    double lam = G[q][q];
+    if (!(lam > 1e-12 * lmax)) continue;
+    double proj = 0.;
+    for (int k = 0; k < m; k++) proj += E[k][q] * g[k];
+    for (int k = 0; k < m; k++) c[k] += E[k][q] * proj / lam;
+  }
+  double energy = 0.;
+  for (int q = 0; q < m; q++) energy += c[q] * g[q];
+  for (int q = 0; q < m; q++)
+    if (!isfinite(c[q])) {
+      for (int k = 0; k < GMAX; k++) c[k] = 0.;
+      return 0.;
+    }
+  return energy > 0. ? energy : 0.;
+}
+
 /* The initial guess with the least A-norm error in span{v_0 .. v_{m-1}}
  * (m <= GMAX, each zero in particle cells) into _phi: the Gram system
- * G c = g, G_ab = (v_a, -A v_b), g_a = (b, v_a), solved through its
- * eigen-decomposition with directions of relative eigenvalue below 1e-12
- * dropped (nearly parallel history vectors).  Returns the energy
- * 2 (b, phi0) - (phi0, -A phi0) = c . g the guess captures (0: no guess,
- * phi = 0), and in *e_first the same for the Galerkin guess on v_0 alone
- * when first_alone is set. */
-double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
-  double *e_first)
+ * G c = g, G_ab = (v_a, -A v_b), g_a = (b, v_a), from one pass over the
+ * vectors.  Returns the energy 2 (b, phi0) - (phi0, -A phi0) = c . g the
+ * guess captures (0: no guess, phi = 0).  From the same Gram matrix,
+ * *e_first gets the energy of the Galerkin guess on v_0 alone and
+ * *e_lead that on v_0 .. v_{m-2} (without the last direction), when the
+ * pointers are given. */
+double bb_pp_galerkin_many(const double *const *vecs, int m, double *e_first,
+  double *e_lead)
 {
   cg_buffers();
-  *e_first = 0.;
+  if (e_first) *e_first = 0.;
+  if (e_lead) *e_lead = 0.;
   if (m <= 0) return 0.;
   if (m > GMAX) m = GMAX;
   const long nt = n_tiles(ddom);
@@ -1867,28 +1902,16 @@ double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
   BB_CHECK(hipMemcpyAsync(gram_host, _gram_out, GTERMS * sizeof(double),
     hipMemcpyDeviceToHost, bb_stream));
   BB_CHECK(hipStreamSynchronize(bb_stream));
-  double g[GMAX], G[GMAX][GMAX], E[GMAX][GMAX];
+  double g[GMAX], G[GMAX][GMAX];
   int t = 0;
   for (int q = 0; q < GMAX; q++, t++) g[q] = gram_host[t];
   for (int p = 0; p < GMAX; p++)
     for (int q = p; q < GMAX; q++, t++) G[p][q] = G[q][p] = gram_host[t];
-  if (first_alone && G[0][0] > 0.) *e_first = g[0] * g[0] / G[0][0];
-  sym_eigen(m, G, E);
-  double lmax = 0.;
-  for (int q = 0; q < m; q++) lmax = fmax(lmax, G[q][q]);
-  double c[GMAX] = {0., 0., 0., 0., 0., 0.};
-  if (!(lmax > 0.) || !isfinite(lmax)) return 0.;
-  for (int q = 0; q < m; q++) {
-    double lam = G[q][q];
-    if (!(lam > 1e-12 * lmax)) continue;
-    double proj = 0.;
-    for (int k = 0; k < m; k++) proj += E[k][q] * g[k];
-    for (int k = 0; k < m; k++) c[k] += E[k][q] * proj / lam;
-  }
-  double energy = 0.;
-  for (int q = 0; q < m; q++) energy += c[q] * g[q];
-  for (int q = 0; q < m; q++)
-    if (!isfinite(c[q])) return 0.;
+  double c[GMAX];
+  if (e_first && G[0][0] > 0.) *e_first = g[0] * g[0] / G[0][0];
+  if (e_lead && m > 1) *e_lead = galerkin_solve(m - 1, G, g, c);
+  const double energy = galerkin_solve(m, G, g, c);
+  if (energy <= 0.) return 0.;
   double *ch = gram_host + GTERMS;
   for (int q = 0; q < m; q++) ch[q] = c[q];
   BB_CHECK(hipMemcpyAsync(_gram_cf, ch, m * sizeof(double),
@@ -1896,7 +1919,7 @@ double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
   hipLaunchKernelGGL(combine_many_kernel, TILES_LAUNCH, ddom, V, m, _gram_cf,
     fluid_mask(), _phi);
   BB_LAUNCH_CHECK();
-  return energy > 0. ? energy : 0.;
+  return energy;
 }
 
 /* nsweeps damped Jacobi sweeps (rounded up to even) on the initial guess

@@ -12,17 +12,22 @@
  * phi0 = 0 (the reference's start) or than phi_1 alone (the fallback).
  * Successive solves of a timestep (the Lamb passes) change the right-hand
  * side by amounts that shrink as the particle boundary values converge, so
- * the earlier solutions span most of the next correction.  A guess that
- * came from the network is then smoothed by bb_warm_sweeps damped Jacobi
- * sweeps (its BF16 output carries high-frequency error that the PCG's
- * residual test sees first).  Every solve still ends at the same test
- * (r, M^-1 r) <= pp_residual^2 (b, b).
+ * the earlier solutions span most of the next correction; the first solve
+ * of a timestep is where they span little, and where the network runs.
+ * By default (bb_warm_coarse) the network works on the grid coarsened 2x
+ * per direction: its input is the defect averaged over 2x2x2 cells, and
+ * its output is interpolated back trilinearly, which leaves psi smooth
+ * (the part of the correction the PCG reduces slowest) at an eighth of
+ * the full grid's cost.  A guess that came from the network can then be
+ * smoothed by bb_warm_sweeps damped Jacobi sweeps.  Every solve still ends
+ * at the same test (r, M^-1 r) <= pp_residual^2 (b, b).
  *
  * The network is trained online: the first bb_warm_train_solves solves
  * each contribute one r0 -> (phi - phi_1) pair (both scaled to unit
- * RMS), on which bb_warm_steps Adam steps are taken over random cubic
- * crops of bb_warm_crop cells; afterwards the network only infers, by
- * default for the first solve of each timestep.
+ * RMS, averaged like the input on the coarse grid), on which
+ * bb_warm_steps Adam steps are taken over random cubic crops of
+ * bb_warm_crop voxels; afterwards the network only infers, by default for
+ * the first solve of each timestep.
  *
  * No counterpart in the reference, which always starts from phi = 0
  * (src/solver_kernel.cu:258-283).
@@ -40,7 +45,7 @@ extern "C" void bb_unet_adam(float lr);
 extern "C" void bb_unet_destroy(void);
 
 int bb_warm_train_solves = 64;         /* solves that feed the training */
-int bb_warm_steps = 4;                 /* Adam steps per training pair */
+int bb_warm_steps = 32;                /* Adam steps per training pair */
 int bb_warm_crop = 64;                 /* training crop edge (cells) */
 float bb_warm_lr = 1e-3f;
 unsigned long long bb_warm_seed = 20260104ULL;
@@ -63,8 +68,12 @@ int bb_warm_net_all_passes = 0;        /* 0: the network runs for the first
 int bb_warm_use_last = 1;              /* 0: span{psi} only */
 int bb_warm_history = 4;               /* earlier solutions in the span
                                           (1: the last one only) */
-int bb_warm_sweeps = 4;                /* damped Jacobi sweeps on a guess
-                                          that came from the network */
+int bb_warm_sweeps = 0;                /* damped Jacobi sweeps on a guess
+                                          that came from the network (the
+                                          coarse network's interpolated
+                                          output needs none: 2 sweeps saved
+                                          3 iterations per step at 256^3
+                                          and cost more than that) */
 double bb_warm_omega = 0.8;            /* their damping factor */
 /* What the network gained on the last solve that used it: A-norm errors
  * squared of the guess from span{phi_last} alone and of the guess used,
@@ -83,8 +92,18 @@ static double *_psi = NULL, *_wres = NULL;
 static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
 static float *_cin = NULL, *_ctgt = NULL;
 static double *_wpart = NULL, *wpart_host = NULL;
-static long wnv = 0;
+static long wnv = 0;                   /* interior cells of the grid */
 static int crop_edge = 0;
+int bb_warm_coarse = 1;                /* 1: the network works on the grid
+                                          coarsened 2x in each direction
+                                          (inputs and targets averaged over
+                                          2x2x2 cells, its output
+                                          interpolated back trilinearly);
+                                          0: on the grid itself */
+/* The network's volume: the grid's interior divided by nf = 1 or 2. */
+struct NetVol { int nf, nx, ny, nz; };
+static NetVol nvol = {1, 0, 0, 0};
+static long nnv = 0;                   /* voxels of the network's volume */
 
 /* Interior cell v = x + nx*(y + ny*z) of the Gcc grid. */
 __device__ inline long cell_of_v(const DevDom &D, long v)
@@ -135,36 +154,61 @@ static void moments(const double *b, const double *p, double w, double m[4])
     for (int q = 0; q < 4; q++) m[q] += wpart_host[i * 4 + q];
 }
 
-/* Network input: [(r0 - rm) * ir, fluid] per interior cell (fluid = 1 in
- * fluid cells, 0 in particle cells, where r0 is 0 too). */
-__global__ void warm_input_kernel(DevDom D, const double *__restrict__ r0,
+/* Network input per voxel of the network's volume: [(r0 - rm) * ir,
+ * fluid] averaged over the voxel's nf^3 cells (nf = 1: the cell itself;
+ * fluid = 1 in fluid cells, 0 in particle cells, where r0 is 0 too). */
+__device__ inline long cell_of_child(const DevDom &D, const NetVol &N, long v,
+  int q)
+{
+  int x = (int) (v % N.nx);
+  long r = v / N.nx;
+  int y = (int) (r % N.ny), z = (int) (r / N.ny);
+  x = x * N.nf + (q % N.nf);
+  y = y * N.nf + (q / N.nf) % N.nf;
+  z = z * N.nf + q / (N.nf * N.nf);
+  return gidx(D.cc, x + 1, y + 1, z + 1);
+}
+
+__global__ void warm_input_kernel(DevDom D, NetVol N, const double *__restrict__ r0,
   const uint8_t *__restrict__ fluid, double rm, double ir,
   float *__restrict__ in)
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  long nv = (long) D.nx * D.ny * D.nz;
+  long nv = (long) N.nx * N.ny * N.nz;
   if (v >= nv) return;
-  long c = cell_of_v(D, v);
-  bool f = fluid == NULL || fluid[c];
-  in[2 * v] = f ? (float) ((r0[c] - rm) * ir) : 0.f;
-  in[2 * v + 1] = f ? 1.f : 0.f;
+  const int nc = N.nf * N.nf * N.nf;
+  double a = 0., fr = 0.;
+  for (int q = 0; q < nc; q++) {
+    long c = cell_of_child(D, N, v, q);
+    bool f = fluid == NULL || fluid[c];
+    a += f ? (r0[c] - rm) * ir : 0.;
+    fr += f ? 1. : 0.;
+  }
+  in[2 * v] = (float) (a / nc);
+  in[2 * v + 1] = (float) (fr / nc);
 }
 
-/* Training target ((phi - phi_last) - m) * s (interior cells). */
-__global__ void warm_target_kernel(DevDom D, const double *__restrict__ phi,
-  const double *__restrict__ last, const uint8_t *__restrict__ fluid,
-  double m, double s, float *__restrict__ t)
+/* Training target ((phi - phi_last) - m) * s, averaged over the nf^3
+ * cells of a network voxel (0 in particle cells). */
+__global__ void warm_target_kernel(DevDom D, NetVol N,
+  const double *__restrict__ phi, const double *__restrict__ last,
+  const uint8_t *__restrict__ fluid, double m, double s, float *__restrict__ t)
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
-  long nv = (long) D.nx * D.ny * D.nz;
+  long nv = (long) N.nx * N.ny * N.nz;
   if (v >= nv) return;
-  long c = cell_of_v(D, v);
-  bool f = fluid == NULL || fluid[c];
-  t[v] = f ? (float) (((phi[c] - last[c]) - m) * s) : 0.f;
+  const int nc = N.nf * N.nf * N.nf;
+  double a = 0.;
+  for (int q = 0; q < nc; q++) {
+    long c = cell_of_child(D, N, v, q);
+    bool f = fluid == NULL || fluid[c];
+    a += f ? ((phi[c] - last[c]) - m) * s : 0.;
+  }
+  t[v] = (float) (a / nc);
 }
 
 /* Random cubic crop (edge e, origin ox, oy, oz) of the input and target. */
-__global__ void crop_kernel(DevDom D, const float *__restrict__ in,
+__global__ void crop_kernel(NetVol N, const float *__restrict__ in,
   const float *__restrict__ t, int e, int ox, int oy, int oz,
   float *__restrict__ cin, float *__restrict__ ct)
 {
@@ -172,15 +216,29 @@ __global__ void crop_kernel(DevDom D, const float *__restrict__ in,
   long n = (long) e * e * e;
   if (v >= n) return;
   int x = (int) (v % e), y = (int) ((v / e) % e), z = (int) (v / ((long) e * e));
-  long src = (long) (x + ox) + (long) D.nx * ((y + oy) + (long) D.ny * (z + oz));
+  long src = (long) (x + ox) + (long) N.nx * ((y + oy) + (long) N.ny * (z + oz));
   cin[2 * v] = in[2 * src];
   cin[2 * v + 1] = in[2 * src + 1];
   ct[v] = t[src];
 }
 
+/* The two network voxels along one direction that interpolate fine cell
+ * x (nf = 2): its own coarse voxel with weight 3/4 and the neighbour on
+ * the side of x with 1/4 (cell-centred trilinear weights), wrapped in a
+ * periodic direction and clamped to the voxel itself at a wall. */
+__device__ inline void prolong_1d(int x, int n, int per, int *i0, int *i1)
+{
+  int I = x >> 1, J = (x & 1) ? I + 1 : I - 1;
+  if (J < 0) J = per ? n - 1 : I;
+  if (J >= n) J = per ? 0 : I;
+  *i0 = I;
+  *i1 = J;
+}
+
 /* psi (FP32 network output) into the Gcc FP64 array phi, zero in particle
- * cells and ghosts untouched. */
-__global__ void warm_place_kernel(DevDom D, const float *__restrict__ y,
+ * cells and ghosts untouched: the voxel's value (nf = 1) or the trilinear
+ * interpolation of the coarse voxels around the cell (nf = 2). */
+__global__ void warm_place_kernel(DevDom D, NetVol N, const float *__restrict__ y,
   const uint8_t *__restrict__ fluid, double *__restrict__ phi)
 {
   long v = (long) blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,7 +246,23 @@ __global__ void warm_place_kernel(DevDom D, const float *__restrict__ y,
   if (v >= nv) return;
   long c = cell_of_v(D, v);
   bool f = fluid == NULL || fluid[c];
-  phi[c] = f ? (double) y[v] : 0.;
+  if (!f) { phi[c] = 0.; return; }
+  if (N.nf == 1) { phi[c] = (double) y[v]; return; }
+  int x = (int) (v % D.nx);
+  long r = v / D.nx;
+  int yy = (int) (r % D.ny), z = (int) (r / D.ny);
+  int ix[2], iy[2], iz[2];
+  prolong_1d(x, N.nx, D.perx, &ix[0], &ix[1]);
+  prolong_1d(yy, N.ny, D.pery, &iy[0], &iy[1]);
+  prolong_1d(z, N.nz, D.perz, &iz[0], &iz[1]);
+  const double w[2] = {0.75, 0.25};
+  double a = 0.;
+  for (int cz = 0; cz < 2; cz++)
+    for (int cy = 0; cy < 2; cy++)
+      for (int cx = 0; cx < 2; cx++)
+        a += w[cx] * w[cy] * w[cz] *
+             (double) y[ix[cx] + (long) N.nx * (iy[cy] + (long) N.ny * iz[cz])];
+  phi[c] = a;
 }
 
 /* Zero in the particle cells of the current mask (interior cells).  A
@@ -207,27 +281,32 @@ __global__ void mask_solid_kernel(DevDom D, const uint8_t *__restrict__ fluid,
 
 static inline unsigned nb(long n) { return (unsigned) ((n + 255) / 256); }
 
-static bool dims_ok(void)
+static bool dims_ok(int nf)
 {
-  return ddom.nx % 4 == 0 && ddom.ny % 4 == 0 && ddom.nz % 4 == 0;
+  const int m = 4 * nf;
+  return ddom.nx % m == 0 && ddom.ny % m == 0 && ddom.nz % m == 0;
 }
 
 extern "C" void bb_warm_setup(void)
 {
   if (pp_warmstart != PP_WARM_LEARNED) return;
-  if (!dims_ok()) {
+  const int nf = bb_warm_coarse ? 2 : 1;
+  if (!dims_ok(nf)) {
     fprintf(stderr, "pp_warmstart LEARNED needs grid dimensions divisible by "
-      "4 (got %d %d %d)\n", ddom.nx, ddom.ny, ddom.nz);
+      "%d (got %d %d %d)\n", 4 * nf, ddom.nx, ddom.ny, ddom.nz);
     exit(EXIT_FAILURE);
   }
   bb_unet_create(bb_warm_seed);
   wnv = (long) ddom.nx * ddom.ny * ddom.nz;
-  BB_CHECK(hipMalloc((void **) &_win, 2 * wnv * sizeof(float)));
-  BB_CHECK(hipMalloc((void **) &_wtgt, wnv * sizeof(float)));
-  BB_CHECK(hipMalloc((void **) &_wout, wnv * sizeof(float)));
+  nvol.nf = nf;
+  nvol.nx = ddom.nx / nf; nvol.ny = ddom.ny / nf; nvol.nz = ddom.nz / nf;
+  nnv = (long) nvol.nx * nvol.ny * nvol.nz;
+  BB_CHECK(hipMalloc((void **) &_win, 2 * nnv * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_wtgt, nnv * sizeof(float)));
+  BB_CHECK(hipMalloc((void **) &_wout, nnv * sizeof(float)));
   int e = bb_warm_crop;
-  int m = ddom.nx < ddom.ny ? ddom.nx : ddom.ny;
-  m = m < ddom.nz ? m : ddom.nz;
+  int m = nvol.nx < nvol.ny ? nvol.nx : nvol.ny;
+  m = m < nvol.nz ? m : nvol.nz;
   if (e > m) e = m;
   e -= e % 4;
   crop_edge = e;
@@ -296,13 +375,13 @@ static void build_input(const uint8_t *fluid)
   moments(_wres, _wres, 0., m);
   double rm;
   double ir = rms_scale(m[0], m[1], wnv, &rm);
-  hipLaunchKernelGGL(warm_input_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _wres, fluid, rm, ir, _win);
+  hipLaunchKernelGGL(warm_input_kernel, dim3(nb(nnv)), dim3(256), 0,
+    bb_stream, ddom, nvol, _wres, fluid, rm, ir, _win);
 }
 
 /* Galerkin guess on several directions and (b, x), bb_solver.hip. */
-double bb_pp_galerkin_many(const double *const *vecs, int m, int first_alone,
-  double *e_first);
+double bb_pp_galerkin_many(const double *const *vecs, int m, double *e_first,
+  double *e_lead);
 double bb_pp_energy(const double *x);
 void bb_pp_smooth(int nsweeps, double omega);
 static double e_guess = 0., e_last = 0., e_hist = 0.;
@@ -334,26 +413,22 @@ extern "C" int cuda_PP_warmstart_infer(void)
   }
   if (net_ready) {
     build_input(fluid);
-    bb_unet_volume(ddom.nx, ddom.ny, ddom.nz);
+    bb_unet_volume(nvol.nx, nvol.ny, nvol.nz);
     bb_unet_forward(_win, _wout);
     hipLaunchKernelGGL(warm_place_kernel, dim3(nb(wnv)), dim3(256), 0,
-      bb_stream, ddom, _wout, fluid, _psi);
+      bb_stream, ddom, nvol, _wout, fluid, _psi);
     BB_LAUNCH_CHECK();
   }
   int m = nh;
   if (net_ready) vecs[m++] = _psi;
   e_guess = e_last = e_hist = 0.;
   if (m == 0) return 0;
-  if (net_ready && nh > 0) {
-    /* the same span without psi, for the record of what psi adds */
-    e_hist = bb_pp_galerkin_many(vecs, nh, 1, &e_last);
-  }
-  double ef = 0.;
-  e_guess = bb_pp_galerkin_many(vecs, m, 1, &ef);
-  if (!(net_ready && nh > 0)) {
-    e_last = nh > 0 ? ef : 0.;
-    e_hist = e_guess;
-  }
+  /* one Gram pass; the energies of the span without psi and of phi_1
+   * alone come from the same matrix, for the record of what psi adds */
+  double ef = 0., el = 0.;
+  e_guess = bb_pp_galerkin_many(vecs, m, &ef, &el);
+  e_last = nh > 0 ? ef : 0.;
+  e_hist = (net_ready && nh > 0) ? el : e_guess;
   bb_warm_used = e_guess > 0.;
   net_this_solve = net_ready && nh > 0 && bb_warm_used;
   /* smoothing sweeps on the network's guess */
@@ -379,15 +454,15 @@ static void train_pair(void)
   double em;
   double is = rms_scale(m[0], m[1], wnv, &em);
   if (is == 0.) return;
-  hipLaunchKernelGGL(warm_target_kernel, dim3(nb(wnv)), dim3(256), 0,
-    bb_stream, ddom, _phi, phi_last(), fluid, em, is, _wtgt);
+  hipLaunchKernelGGL(warm_target_kernel, dim3(nb(nnv)), dim3(256), 0,
+    bb_stream, ddom, nvol, _phi, phi_last(), fluid, em, is, _wtgt);
   int e = crop_edge;
   bb_unet_volume(e, e, e);
   for (int s = 0; s < bb_warm_steps; s++) {
-    int ox = crand(ddom.nx - e + 1), oy = crand(ddom.ny - e + 1),
-        oz = crand(ddom.nz - e + 1);
+    int ox = crand(nvol.nx - e + 1), oy = crand(nvol.ny - e + 1),
+        oz = crand(nvol.nz - e + 1);
     hipLaunchKernelGGL(crop_kernel, dim3(nb((long) e * e * e)), dim3(256), 0,
-      bb_stream, ddom, _win, _wtgt, e, ox, oy, oz, _cin, _ctgt);
+      bb_stream, nvol, _win, _wtgt, e, ox, oy, oz, _cin, _ctgt);
     bb_unet_forward(_cin, _wout);
     bb_warm_loss = bb_unet_backward(_ctgt);
     if (bb_warm_loss_n < LOSS_LOG) bb_warm_loss_log[bb_warm_loss_n++] = bb_warm_loss;
