@@ -611,6 +611,148 @@ __device__ inline void run_store_nt(V *__restrict__ a, unsigned i,
   }
 }
 
+/* Lane moves that do not go through the LDS crossbar (ds_bpermute):
+ * lane_prev gives lane l the value of lane (l - 1) mod 64 (DPP wave
+ * rotate right by one), lane_next that of lane (l + 1) mod 64 (wave rotate
+ * left by one), lane_get that of a wavefront-uniform lane (v_readlane).
+ * 8-byte values move as two 32-bit halves. */
+template <int CTRL>
+__device__ inline unsigned dpp32(unsigned x)
+{
+  return (unsigned) __builtin_amdgcn_mov_dpp((int) x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL, typename V>
+__device__ inline V dpp_move(V x)
+{
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, dpp32<CTRL>(__builtin_bit_cast(unsigned, x)));
+  } else {
+    typedef unsigned U2 __attribute__((ext_vector_type(2)));
+    U2 u = __builtin_bit_cast(U2, x);
+    u[0] = dpp32<CTRL>(u[0]);
+    u[1] = dpp32<CTRL>(u[1]);
+    return __builtin_bit_cast(V, u);
+  }
+}
+
+#define DPP_WAVE_ROL1 0x134
+#define DPP_WAVE_ROR1 0x13C
+#define DPP_ROW_SHL(d) (0x100 + (d))
+
+/* The value of lane l + d for d < 16 within l's row of 16 lanes (DPP
+ * row shift left); lanes whose source lies past the row keep their own
+ * value, as __shfl_down keeps it past the wavefront.  A tree sum whose
+ * steps stay inside 16-lane groups reads only valid sources at the
+ * group's first lane, so it gives the same result there as with
+ * __shfl_down. */
+template <int D, typename V>
+__device__ inline V lane_down_row(V x)
+{
+  static_assert(D >= 1 && D < 16, "a DPP row shift moves 1 to 15 lanes");
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, (unsigned) __builtin_amdgcn_update_dpp(
+      __builtin_bit_cast(int, x), __builtin_bit_cast(int, x),
+      DPP_ROW_SHL(D), 0xF, 0xF, false));
+  } else {
+    typedef int I2 __attribute__((ext_vector_type(2)));
+    I2 u = __builtin_bit_cast(I2, x);
+    u[0] = __builtin_amdgcn_update_dpp(u[0], u[0], DPP_ROW_SHL(D), 0xF, 0xF,
+                                       false);
+    u[1] = __builtin_amdgcn_update_dpp(u[1], u[1], DPP_ROW_SHL(D), 0xF, 0xF,
+                                       false);
+    return __builtin_bit_cast(V, u);
+  }
+}
+
+/* One step of the tile tree sum: acc += the value D lanes down (a DPP row
+ * shift within 16-lane groups, else a bpermute across them). */
+template <int D>
+__device__ inline double tree_step(double a)
+{
+  if constexpr (D < 16) return a + lane_down_row<D>(a);
+  else return a + __shfl_down(a, D, WAVE);
+}
+
+template <typename V> __device__ inline V lane_prev(V x)
+{
+  return dpp_move<DPP_WAVE_ROR1>(x);
+}
+
+template <typename V> __device__ inline V lane_next(V x)
+{
+  return dpp_move<DPP_WAVE_ROL1>(x);
+}
+
+template <typename V> __device__ inline V lane_get(V x, int src)
+{
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_readlane(
+      __builtin_bit_cast(int, x), src));
+  } else {
+    typedef int I2 __attribute__((ext_vector_type(2)));
+    I2 u = __builtin_bit_cast(I2, x);
+    u[0] = __builtin_amdgcn_readlane(u[0], src);
+    u[1] = __builtin_amdgcn_readlane(u[1], src);
+    return __builtin_bit_cast(V, u);
+  }
+}
+
+/* A row of WAVE runs of N elements in LDS, laid out as lane-consecutive
+ * chunks of at most 16 bytes: chunk c of lane l holds elements
+ * c*H .. c*H + H - 1 of the lane's run at (c*WAVE + l)*H.  Each chunk
+ * access is then one ds_read_b128 / ds_write_b128 (or narrower) over
+ * consecutive addresses, free of bank conflicts; a run stored whole (16
+ * bytes of FP32, 32 of FP64) at a run-sized lane stride was split by the
+ * compiler into 8-byte accesses that conflict. */
+template <typename V, int N> struct LdsRow {
+  static constexpr int H = (int) (N * sizeof(V) <= 16 ? N : 16 / sizeof(V));
+  static constexpr int NH = N / H;
+  static_assert(NH * H == N, "run length must divide into chunks");
+  __device__ static void put(V *__restrict__ row, int lane, const Run<V, N> &o)
+  {
+#pragma unroll
+    for (int c = 0; c < NH; c++) {
+      /* every row starts on a 16-byte boundary of the aligned LDS array */
+      V *q = (V *) __builtin_assume_aligned(row + (c * WAVE + lane) * H,
+                                            H * sizeof(V));
+      if constexpr (H == 1) {
+        *q = o.v[c];
+      } else {
+        typedef V VH __attribute__((ext_vector_type(H)));
+        VH t;
+#pragma unroll
+        for (int e = 0; e < H; e++) t[e] = o.v[c * H + e];
+        *(VH *) q = t;
+      }
+    }
+  }
+  __device__ static Run<V, N> get(const V *__restrict__ row, int lane)
+  {
+    Run<V, N> o;
+#pragma unroll
+    for (int c = 0; c < NH; c++) {
+      const V *q = (const V *) __builtin_assume_aligned(
+        row + (c * WAVE + lane) * H, H * sizeof(V));
+      if constexpr (H == 1) {
+        o.v[c] = *q;
+      } else {
+        typedef V VH __attribute__((ext_vector_type(H)));
+        VH t = *(const VH *) q;
+#pragma unroll
+        for (int e = 0; e < H; e++) o.v[c * H + e] = t[e];
+      }
+    }
+    return o;
+  }
+  /* position of element q (0-based x - 1) of the row */
+  __device__ static int pos(int q)
+  {
+    const int l = q / N, e = q - l * N;
+    return ((e / H) * WAVE + l) * H + (e % H);
+  }
+};
+
 /* Source row of LDS row jj (0 and ny + 1 are the halo rows): the row
  * itself, the wrapped row, or -1 for a row that holds zeros (a wall ghost,
  * or a row past the halo in the domain's last workgroup). */
@@ -633,7 +775,9 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   /* FP64 p_old is read with the non-temporal hint, FP32 p_old ordinarily
    * (as in the z-column kernel) */
   constexpr bool NTP = sizeof(V) == 8;
-  __shared__ R_t lds[3 * (RY + 2) * WAVE];
+  typedef LdsRow<V, CPR> LR;
+  constexpr int ROWV = WAVE * CPR;        /* elements of one LDS row */
+  __shared__ __attribute__((aligned(16))) V lds[3 * (RY + 2) * ROWV];
   __shared__ V s_invm[27];
   __shared__ double s_tp[ROW_ZCMAX / TILE_Z][CPR][RY];
   if (st->done | st->inner_done) return;
@@ -711,7 +855,7 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   }
   /* p_new of a loaded plane into LDS buffer `buf` (own run and halo cell,
    * the halo cell at its x in the row); returns the own run */
-  auto put = [&](const Raw &R, R_t *buf, int kk) {
+  auto put = [&](const Raw &R, V *buf, int kk) {
     int ks = plane_src(D, kk);
     /* a ghost plane (ks = 0) holds zeros whatever the class */
     const int zc2 = wcount(ks, D.nz, D.perz) == 2;
@@ -722,15 +866,15 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       V v = FIRST ? z : z + beta*R.p.v[e];
       o.v[e] = (ks && oko) ? v : (V) 0;
     }
-    buf[(wv + 1) * WAVE + lane] = o;
+    LR::put(buf + (wv + 1) * ROWV, lane, o);
     V zh = R.hr*(zc2 ? mh[1] : mh[0]);
     V vh = FIRST ? zh : zh + beta*R.hp;
     vh = (ks && okh) ? vh : (V) 0;
-    if (hlane) ((V *) buf)[hs * WAVE * CPR + xh - 1] = vh;
+    if (hlane) buf[hs * ROWV + LR::pos(xh - 1)] = vh;
     return o;
   };
-  R_t *bprev = lds, *bcur = lds + (RY + 2) * WAVE,
-      *bnext = lds + 2 * (RY + 2) * WAVE;
+  V *bprev = lds, *bcur = lds + (RY + 2) * ROWV,
+    *bnext = lds + 2 * (RY + 2) * ROWV;
   R_t prev = put(Rm, bprev, k0 - 1);
   R_t cur = put(Rc, bcur, k0);
   __syncthreads();
@@ -766,13 +910,14 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     R_t nxt = put(Rs, bnext, k + 1);
     Rs = load(min(k + 1 + DEPTH, kend + 1));
     __syncthreads();
-    const R_t S = bcur[wv * WAVE + lane], N = bcur[(wv + 2) * WAVE + lane];
+    const R_t S = LR::get(bcur + wv * ROWV, lane),
+              N = LR::get(bcur + (wv + 2) * ROWV, lane);
     /* run neighbours: the last cell of lane l - 1, the first of l + 1, and
      * the wrapped cells x = nx and x = 1 (0: a wall ghost) */
-    V wl = __shfl(cur.v[CPR - 1], (lane + WAVE - 1) & (WAVE - 1));
-    V er = __shfl(cur.v[0], (lane + 1) & (WAVE - 1));
-    V xn = __shfl(cur.v[CPR - 1], lend);
-    V x1 = __shfl(cur.v[0], 0);
+    V wl = lane_prev(cur.v[CPR - 1]);
+    V er = lane_next(cur.v[0]);
+    V xn = lane_get(cur.v[CPR - 1], lend);
+    V x1 = lane_get(cur.v[0], 0);
     const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
     /* wavefront-uniform: interior rows of interior planes skip the y and
      * z weights (a weight of 1 multiplies exactly) */
@@ -791,7 +936,7 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       acc[e] += colok ? (double) cur.v[e] * (double) a.v[e] : 0.0;
     prev = cur;
     cur = nxt;
-    R_t *t3 = bprev;
+    V *t3 = bprev;
     bprev = bcur;
     bcur = bnext;
     bnext = t3;
@@ -799,11 +944,16 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       /* wave_tree_sum over each 64-cell tile of the row: steps of CPR cells
        * or more are lane moves within the tile's 64 / CPR lanes, the last
        * log2(CPR) steps add within the lane */
+      /* lane distances 32 / CPR, 16 / CPR, ..., 1 (the tile's lanes) */
 #pragma unroll
-      for (int off = 32; off >= CPR; off >>= 1)
-#pragma unroll
-        for (int e = 0; e < CPR; e++)
-          acc[e] += __shfl_down(acc[e], off / CPR, WAVE);
+      for (int e = 0; e < CPR; e++) {
+        if constexpr (CPR == 1) acc[e] = tree_step<32>(acc[e]);
+        if constexpr (CPR <= 2) acc[e] = tree_step<16>(acc[e]);
+        acc[e] = tree_step<8>(acc[e]);
+        acc[e] = tree_step<4>(acc[e]);
+        acc[e] = tree_step<2>(acc[e]);
+        acc[e] = tree_step<1>(acc[e]);
+      }
 #pragma unroll
       for (int off = CPR / 2; off >= 1; off >>= 1)
 #pragma unroll
