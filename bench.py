@@ -84,7 +84,7 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 37.29,
+    "spmv_search_kernel": 37.44,
     "update_kernel": 49.16,
     "update_kernel_skip": 25.15,
     "update_kernel_pair": 57.17,
