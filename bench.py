@@ -12,9 +12,9 @@ in -z, Lamb order 4).  It runs twice, once per form:
   plain          FP64 PCG from phi = 0 (the reference's solver);
   mixed+last     FP32 PCG inside FP64 defect correction, initial guess from
                  the previous solution alone (no network);
-  mixed+history  the same, initial guess from the span of the last four
+  mixed+history  the same, initial guess from the span of the last six
                  solutions (no network);
-  mixed+learned  the same, initial guess from the span of the last four
+  mixed+learned  the same, initial guess from the span of the last six
                  solutions and the U-Net's guess, the network trained
                  online on the solver's own solves (its training solves
                  happen in untimed warm-up steps).
@@ -109,7 +109,7 @@ def parse():
                     help="which forms to time (mixed: FP32-PCG from phi = 0; "
                          "mixed-last: FP32-PCG with the guess from the "
                          "previous solution only; mixed-history: from the "
-                         "last four solutions; no network in either)")
+                         "last six solutions; no network in either)")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -120,7 +120,7 @@ def parse():
                     help="solves that feed the network's online training")
     ap.add_argument("--history", type=int, default=None,
                     help="earlier solutions in the span of the history and "
-                         "learned forms (default 4)")
+                         "learned forms (default 6)")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU oracle baseline")
     ap.add_argument("--spmv-variant", type=int, default=None,
@@ -470,13 +470,13 @@ def main():
     # (key, precision, warm start, network, earlier solutions in the span)
     forms = {"both": [("plain", "PLAIN", "OFF", 1, 0),
                       ("mixed_last", "MIXED", "LEARNED", 0, 1),
-                      ("mixed_history", "MIXED", "LEARNED", 0, 4),
-                      ("mixed_learned", "MIXED", "LEARNED", 1, 4)],
+                      ("mixed_history", "MIXED", "LEARNED", 0, 6),
+                      ("mixed_learned", "MIXED", "LEARNED", 1, 6)],
              "plain": [("plain", "PLAIN", "OFF", 1, 0)],
-             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED", 1, 4)],
+             "mixed-learned": [("mixed_learned", "MIXED", "LEARNED", 1, 6)],
              "mixed": [("mixed", "MIXED", "OFF", 1, 0)],
              "mixed-last": [("mixed_last", "MIXED", "LEARNED", 0, 1)],
-             "mixed-history": [("mixed_history", "MIXED", "LEARNED", 0, 4)]
+             "mixed-history": [("mixed_history", "MIXED", "LEARNED", 0, 6)]
              }[a.form]
     res = {}
     # the learned form first: its warm-up (the network's training solves)

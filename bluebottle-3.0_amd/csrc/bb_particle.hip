@@ -32,6 +32,8 @@
 
 part_struct *_parts = NULL;
 uint8_t *_bb_fluid = NULL;             /* 1: fluid cell, 0: particle cell */
+long bb_mask_gen = 0;                  /* bumped whenever the cages (and the
+                                          mask) are rebuilt */
 static double *_perr = NULL;           /* per-particle Lamb error */
 static int *_pnum = NULL;
 static double *_err_out = NULL;        /* {err, num} of the last error */
@@ -453,6 +455,7 @@ void bb_build_particle_cages(void)
   hipLaunchKernelGGL(fluid_mask_kernel, dim3(nblk(n, 256)), dim3(256), 0,
     bb_stream, n, _phase, _bb_fluid);
   BB_LAUNCH_CHECK();
+  bb_mask_gen++;
 }
 
 /* ------------------------ particle velocity BCs -------------------------- */

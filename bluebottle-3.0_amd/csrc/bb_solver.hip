@@ -595,6 +595,26 @@ __device__ inline Run<V, N> run_load(const V *__restrict__ a, unsigned i)
   return o;
 }
 
+/* N consecutive elements stored from element i by one vector store; NT:
+ * with the non-temporal hint. */
+template <typename V, int N, bool NT>
+__device__ inline void run_store(V *__restrict__ a, unsigned i,
+  const Run<V, N> &o)
+{
+  char *p = (char *) a + i * (unsigned) sizeof(V);
+  if constexpr (N == 1) {
+    if (NT) __builtin_nontemporal_store(o.v[0], (V *) p);
+    else *(V *) p = o.v[0];
+  } else {
+    typedef V VN __attribute__((ext_vector_type(N)));
+    VN t;
+#pragma unroll
+    for (int e = 0; e < N; e++) t[e] = o.v[e];
+    if (NT) __builtin_nontemporal_store(t, (VN *) p);
+    else *(VN *) p = t;
+  }
+}
+
 template <typename V, int N>
 __device__ inline void run_store_nt(V *__restrict__ a, unsigned i,
   const Run<V, N> &o)
@@ -769,7 +789,8 @@ __global__ void __launch_bounds__(RY * WAVE)
 search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles, int zc,
   const V *__restrict__ r, const V *__restrict__ pold,
-  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart,
+  int nts)
 {
   typedef Run<V, CPR> R_t;
   /* FP64 p_old is read with the non-temporal hint, FP32 p_old ordinarily
@@ -929,8 +950,13 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     /* cells outside the domain store to the never-read edge ghost
      * (x = 1, j = 0, k = 0), aligned like every run */
     unsigned c = colok ? co + (unsigned) k * (unsigned) C.sz : 1u;
-    run_store_nt<V, CPR>(pnew, c, cur);
-    run_store_nt<V, CPR>(Ap, c, a);
+    if (nts) {
+      run_store<V, CPR, true>(pnew, c, cur);
+      run_store<V, CPR, true>(Ap, c, a);
+    } else {
+      run_store<V, CPR, false>(pnew, c, cur);
+      run_store<V, CPR, false>(Ap, c, a);
+    }
 #pragma unroll
     for (int e = 0; e < CPR; e++)
       acc[e] += colok ? (double) cur.v[e] * (double) a.v[e] : 0.0;
@@ -1476,7 +1502,7 @@ jacobi_sweep_kernel(DevDom D, Coef K, InvMTab T, double omega,
  * in the order [(b, v_0) .. (b, v_{m-1}), (v_0, A v_0), (v_0, A v_1) ..,
  * (v_1, A v_1) ..].  These are plain FP64 sums in a fixed order; the
  * guess they give is not part of the bit-exact plain solve. */
-#define GMAX 6
+#define GMAX 9
 #define GTERMS (GMAX + GMAX * (GMAX + 1) / 2)
 struct GVecs { const double *v[GMAX]; };
 
@@ -1542,6 +1568,63 @@ gram_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid, GVecs V,
       const int q = p + r;
       row = GMAX + p * GMAX - p * (p - 1) / 2 + (q - p);
     }
+    double v = 0.;
+    for (int w = 0; w < TILE_Y; w++) v += s[w][t];
+    part[(long) row * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+/* The same sums for one new direction v_n among M (n = -1: none): per
+ * workgroup, (b, v_a) for every a into rows 0..M-1 and (v_a, -A v_n) for
+ * every a into rows GMAX..GMAX+M-1.  Used when the other entries of the
+ * Gram matrix are still valid (same directions, same operator). */
+template <int M>
+__global__ void __launch_bounds__(TILE_THREADS)
+gram_new_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
+  GVecs V, int n, const double *__restrict__ rhs, double *__restrict__ part)
+{
+  TILE_PROLOGUE
+  double a[2 * M];
+#pragma unroll
+  for (int t = 0; t < 2 * M; t++) a[t] = 0.;
+  if (colok) {
+    const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+    const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+    const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+    const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+    const double *fn = V.v[n >= 0 ? n : 0];
+#pragma unroll 1
+    for (int k = k0; k <= k1; k++) {
+      long c = gidx(C, x, j, k);
+      if (solid(fluid, c)) continue;
+      double Avn = 0.;
+      if (n >= 0) {
+        const int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+        const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+        Avn = neg_lap7(K, fn[c], fn[gidx(C, xm, j, k)], fn[gidx(C, xp, j, k)],
+                       fn[gidx(C, x, jm, k)], fn[gidx(C, x, jp, k)],
+                       fn[gidx(C, x, j, km)], fn[gidx(C, x, j, kp)],
+                       wW, wE, wS, wN, wB, wT);
+      }
+      const double b = rhs[c];
+#pragma unroll
+      for (int q = 0; q < M; q++) {
+        const double vq = V.v[q][c];
+        a[q] += b * vq;
+        a[M + q] += vq * Avn;
+      }
+    }
+  }
+  __shared__ double s[TILE_Y][2 * M];
+#pragma unroll
+  for (int t = 0; t < 2 * M; t++) {
+    double v = wave_tree_sum(a[t]);
+    if (lane == 0) s[wv][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * M) {
+    const int t = threadIdx.x;
+    const int row = t < M ? t : GMAX + (t - M);
     double v = 0.;
     for (int w = 0; w < TILE_Y; w++) v += s[w][t];
     part[(long) row * gridDim.x + blockIdx.x] = v;
@@ -1738,6 +1821,17 @@ static const uint8_t *fluid_mask(void)
 
 static CGState *_cg_state = NULL;
 static CGState *cg_host = NULL;
+/* The Gram matrix of the last call, kept for its directions (pointer and
+ * generation) under the operator of one particle-mask generation: a later
+ * call whose directions are those plus at most one new one computes only
+ * the new column and the right-hand side terms. */
+static const double *gc_ptr[GMAX];
+static long gc_gen[GMAX];
+static int gc_m = 0;
+static long gc_mask = -1;
+static double gc_G[GMAX][GMAX];
+extern long bb_mask_gen;               /* bb_particle.hip */
+
 /* Gram-system buffers of the multi-direction guess (bb_pp_galerkin_many). */
 static double *_gram_part = NULL, *_gram_out = NULL, *_gram_cf = NULL;
 static double *gram_host = NULL;
@@ -1900,6 +1994,7 @@ void bb_solver_free(void)
   if (gram_host) BB_CHECK(hipHostFree(gram_host));
   _gram_part = _gram_out = _gram_cf = gram_host = NULL;
   gram_blocks = 0;
+  gc_m = 0;
 }
 
 static void fetch_state(void)
@@ -2016,8 +2111,8 @@ static double galerkin_solve(int m, const double G0[GMAX][GMAX],
  * *e_first gets the energy of the Galerkin guess on v_0 alone and
  * *e_lead that on v_0 .. v_{m-2} (without the last direction), when the
  * pointers are given. */
-double bb_pp_galerkin_many(const double *const *vecs, int m, double *e_first,
-  double *e_lead)
+double bb_pp_galerkin_many(const double *const *vecs, const long *gens, int m,
+  double *e_first, double *e_lead)
 {
   cg_buffers();
   if (e_first) *e_first = 0.;
@@ -2038,14 +2133,33 @@ double bb_pp_galerkin_many(const double *const *vecs, int m, double *e_first,
   }
   GVecs V;
   for (int q = 0; q < GMAX; q++) V.v[q] = q < m ? vecs[q] : vecs[0];
+  /* which directions the kept Gram matrix already holds */
+  int idx[GMAX], nnew = 0, inew = -1;
+  for (int q = 0; q < m; q++) {
+    idx[q] = -1;
+    if (gens && gc_mask == bb_mask_gen)
+      for (int r = 0; r < gc_m; r++)
+        if (gc_ptr[r] == vecs[q] && gc_gen[r] == gens[q]) idx[q] = r;
+    if (idx[q] < 0) { nnew++; inew = q; }
+  }
+  const bool incr = nnew <= 1;
   /* rows of the terms a smaller m leaves out are summed too: zero them */
   hipLaunchKernelGGL(zero_rows_kernel, dim3(nblk_zero(GTERMS * nt)),
     dim3(256), 0, bb_stream, _gram_part, (long) GTERMS * nt);
-  switch (m) {
+  if (incr) {
+    switch (m) {
+#define GN(M) case M: hipLaunchKernelGGL(gram_new_kernel<M>, TILES_LAUNCH, \
+    ddom, coef(), fluid_mask(), V, inew, _rhs_p, _gram_part); break;
+      GN(1) GN(2) GN(3) GN(4) GN(5) GN(6) GN(7) GN(8) GN(9)
+#undef GN
+    }
+  } else {
+    switch (m) {
 #define GK(M) case M: hipLaunchKernelGGL(gram_kernel<M>, TILES_LAUNCH, ddom, \
     coef(), fluid_mask(), V, _rhs_p, _gram_part); break;
-    GK(1) GK(2) GK(3) GK(4) GK(5) GK(6)
+      GK(1) GK(2) GK(3) GK(4) GK(5) GK(6) GK(7) GK(8) GK(9)
 #undef GK
+    }
   }
   hipLaunchKernelGGL(gram_sum_kernel, dim3(GTERMS), dim3(256), 0, bb_stream,
     _gram_part, nt, _gram_out);
@@ -2053,14 +2167,48 @@ double bb_pp_galerkin_many(const double *const *vecs, int m, double *e_first,
     hipMemcpyDeviceToHost, bb_stream));
   BB_CHECK(hipStreamSynchronize(bb_stream));
   double g[GMAX], G[GMAX][GMAX];
-  int t = 0;
-  for (int q = 0; q < GMAX; q++, t++) g[q] = gram_host[t];
-  for (int p = 0; p < GMAX; p++)
-    for (int q = p; q < GMAX; q++, t++) G[p][q] = G[q][p] = gram_host[t];
-  double c[GMAX];
-  if (e_first && G[0][0] > 0.) *e_first = g[0] * g[0] / G[0][0];
-  if (e_lead && m > 1) *e_lead = galerkin_solve(m - 1, G, g, c);
-  const double energy = galerkin_solve(m, G, g, c);
+  for (int q = 0; q < GMAX; q++) g[q] = gram_host[q];
+  if (incr) {
+    for (int p = 0; p < m; p++)
+      for (int q = 0; q < m; q++)
+        if (p != inew && q != inew) G[p][q] = gc_G[idx[p]][idx[q]];
+    if (inew >= 0)
+      for (int a = 0; a < m; a++)
+        G[a][inew] = G[inew][a] = gram_host[GMAX + a];
+  } else {
+    int t = GMAX;
+    for (int p = 0; p < GMAX; p++)
+      for (int q = p; q < GMAX; q++, t++) G[p][q] = G[q][p] = gram_host[t];
+  }
+  /* keep this call's matrix for the next */
+  for (int p = 0; p < m; p++) {
+    gc_ptr[p] = vecs[p];
+    gc_gen[p] = gens ? gens[p] : -1;
+    for (int q = 0; q < m; q++) gc_G[p][q] = G[p][q];
+  }
+  gc_m = gens ? m : 0;
+  gc_mask = bb_mask_gen;
+  /* The nested Galerkin solutions on v_0 alone, on v_0 .. v_{m-2} and on
+   * all m directions.  In exact arithmetic each captures at least the
+   * energy of the one before; the eigen-solve's dropping of near-null
+   * directions of nearly parallel vectors can break that by rounding, so
+   * the guess takes whichever captures the most, and is never worse in
+   * the A-norm than the smaller spans it reports. */
+  double c[GMAX], cl[GMAX];
+  const double ef = G[0][0] > 0. ? g[0] * g[0] / G[0][0] : 0.;
+  const double el = m > 1 ? galerkin_solve(m - 1, G, g, cl) : 0.;
+  double energy = galerkin_solve(m, G, g, c);
+  if (e_first) *e_first = ef;
+  if (e_lead) *e_lead = el;
+  if (m > 1 && el > energy) {
+    for (int q = 0; q < GMAX; q++) c[q] = q < m - 1 ? cl[q] : 0.;
+    energy = el;
+  }
+  if (ef > energy) {
+    for (int q = 0; q < GMAX; q++) c[q] = 0.;
+    c[0] = g[0] / G[0][0];
+    energy = ef;
+  }
   if (energy <= 0.) return 0.;
   double *ch = gram_host + GTERMS;
   for (int q = 0; q < m; q++) ch[q] = c[q];
@@ -2168,9 +2316,10 @@ static void launch_row_t(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
 {
   const long nyb = (ddom.ny + RY - 1) / RY, nzc = (ddom.nz + zc - 1) / zc;
+  const int nts = (sizeof(V) == 8 ? bb_spmv_nts : bb_spmv_nts32) != 0;
   BB_TIMED_LAUNCH((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
     dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K, T,
-    _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);
+    _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p, nts);
 }
 
 template <bool FIRST, typename V, bool MIXED, int RY, int DEPTH>

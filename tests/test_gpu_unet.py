@@ -135,18 +135,19 @@ def test_adam_reduces_loss(net):
 def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
     """PCG iterations per timestep (all Lamb passes) with a given start:
     'zero' (the reference's phi0 = 0), 'last' (span{previous solution}),
-    'hist' (span of the last four solutions), 'learned' (span of the last
-    four solutions and the network's guess)."""
+    'hist' (span of the last six solutions), 'learned' (span of the last
+    six solutions and the network's guess)."""
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})
     ext["pp_warmstart"] = "OFF" if mode == "zero" else "LEARNED"
     kw["extensions"] = ext
     bbconfig.write_case(case_dir, **kw)
     bbconfig.write_parts(case_dir, parts)
+    hist0 = ctypes.c_int.in_dll(L, "bb_warm_history").value
     ctypes.c_int.in_dll(L, "bb_warm_use_net").value = \
         1 if mode == "learned" else 0
     ctypes.c_int.in_dll(L, "bb_warm_history").value = \
-        1 if mode == "last" else 4
+        1 if mode == "last" else 6
     tot = ctypes.c_long.in_dll(L, "bb_pcg_iters_total")
     per_step, resid = [], []
     with bbhip.Simulation(case_dir, kw["n"]) as s:
@@ -160,7 +161,7 @@ def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
                   ctypes.c_double.in_dll(L, "bb_warm_err_guess_sum").value,
                   ctypes.c_int.in_dll(L, "bb_warm_net_solves").value)
     ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1
-    ctypes.c_int.in_dll(L, "bb_warm_history").value = 4
+    ctypes.c_int.in_dll(L, "bb_warm_history").value = hist0
     return per_step, resid, energy
 
 
@@ -172,7 +173,7 @@ def test_warm_start_in_solver(case_dir):
     network's direction (both against the converged solution), and the PCG
     iterations of steps 3-4 with the network must be fewer than without
     it (the last solution alone, the fallback start).  The span of the
-    last four solutions without the network is reported beside them."""
+    last six solutions without the network is reported beside them."""
     kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=7)
     L = lib()
     L.bb_unet_destroy()             # the solver builds its own network
