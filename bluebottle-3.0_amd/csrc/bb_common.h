@@ -84,6 +84,20 @@ __host__ __device__ inline long gidx(const Grid &G, int i, int j, int k)
   return (long) i + (long) j * G.sy + (long) k * G.sz;
 }
 
+/* Element `i` of a grid array by a 32-bit byte offset from the (uniform)
+ * base, so the load takes a scalar base and one vector offset register. */
+template <typename V>
+__device__ inline V ld32(const V *__restrict__ a, unsigned i)
+{
+  return *(const V *) ((const char *) a + i * (unsigned) sizeof(V));
+}
+
+template <typename V>
+__device__ inline void st32(V *__restrict__ a, unsigned i, V v)
+{
+  *(V *) ((char *) a + i * (unsigned) sizeof(V)) = v;
+}
+
 /* Everything a kernel needs to know about the domain, passed by value. */
 struct DevDom {
   Grid cc, fx, fy, fz;

@@ -187,6 +187,19 @@ extern "C" void cuda_dom_malloc_dev(void)
   ddom.perz = (bc.pB == PERIODIC);
 
   const long cc = ddom.cc.n, fx = ddom.fx.n, fy = ddom.fy.n, fz = ddom.fz.n;
+  /* the stencil and PCG kernels address a grid array by a 32-bit byte
+   * offset from its base: every FP64 grid array must stay below 4 GiB
+   * (about 810^3 cells) */
+  {
+    long big = cc > fx ? cc : fx;
+    big = big > fy ? big : fy;
+    big = big > fz ? big : fz;
+    if ((unsigned long) (big + GRID_ORIGIN) * sizeof(real) >= (1UL << 32)) {
+      fprintf(stderr, "grid arrays of %ld elements exceed the 4 GiB that "
+        "32-bit byte offsets address\n", big);
+      exit(EXIT_FAILURE);
+    }
+  }
 #define GA(n) ((real *) bb_grid_alloc(n, sizeof(real)))
   _p = GA(cc); _p0 = GA(cc);
   _phi = GA(cc); _invM = GA(cc);

@@ -328,20 +328,6 @@ __device__ inline int plane_src(const DevDom &D, int kk)
   return kk;
 }
 
-/* Element `i` of a Gcc array by a 32-bit byte offset from the (uniform)
- * base, so the load takes a scalar base and one vector offset register. */
-template <typename V>
-__device__ inline V ld32(const V *__restrict__ a, unsigned i)
-{
-  return *(const V *) ((const char *) a + i * (unsigned) sizeof(V));
-}
-
-template <typename V>
-__device__ inline void st32(V *__restrict__ a, unsigned i, V v)
-{
-  *(V *) ((char *) a + i * (unsigned) sizeof(V)) = v;
-}
-
 /* Loads with the non-temporal hint: p_old in the FP64 search kernel and Ap
  * in the update kernel are read once per iteration and not again before
  * they are rewritten (measured at 256^3: FP64 search 150 -> 146 us, update
@@ -789,12 +775,15 @@ __global__ void __launch_bounds__(RY * WAVE)
 search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles, int zc,
   const V *__restrict__ r, const V *__restrict__ pold,
-  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart,
-  int nts)
+  V *__restrict__ pnew, V *__restrict__ Ap, double *__restrict__ gpart)
 {
   typedef Run<V, CPR> R_t;
   /* FP64 p_old is read with the non-temporal hint, FP32 p_old ordinarily
-   * (as in the z-column kernel) */
+   * (as in the z-column kernel); p_new and Ap are always stored with the
+   * hint.  (A run-time switch between hinted and plain stores let the
+   * compiler merge the two store paths and drop the hint; in that build the
+   * FP64 update that follows took 193 us per launch at 256^3 instead of
+   * 165 us: it paid the search's write-back.) */
   constexpr bool NTP = sizeof(V) == 8;
   typedef LdsRow<V, CPR> LR;
   constexpr int ROWV = WAVE * CPR;        /* elements of one LDS row */
@@ -950,13 +939,8 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     /* cells outside the domain store to the never-read edge ghost
      * (x = 1, j = 0, k = 0), aligned like every run */
     unsigned c = colok ? co + (unsigned) k * (unsigned) C.sz : 1u;
-    if (nts) {
-      run_store<V, CPR, true>(pnew, c, cur);
-      run_store<V, CPR, true>(Ap, c, a);
-    } else {
-      run_store<V, CPR, false>(pnew, c, cur);
-      run_store<V, CPR, false>(Ap, c, a);
-    }
+    run_store<V, CPR, true>(pnew, c, cur);
+    run_store<V, CPR, true>(Ap, c, a);
 #pragma unroll
     for (int e = 0; e < CPR; e++)
       acc[e] += colok ? (double) cur.v[e] * (double) a.v[e] : 0.0;
@@ -1403,6 +1387,64 @@ __device__ inline double neg_lap_at(const DevDom &D, const Coef &K,
                   wall_hi(k, D.nz, D.perz));
 }
 
+/* b - (-A) phi down one thread's column of a whole tile (planes k0 ..
+ * k0 + 7): phi of the column's ten planes is loaded first, then each half
+ * of four planes loads its x and y neighbours and b before any of it is
+ * formed (32-bit offsets, no access behind a branch).  emit(i, c, rr) gets
+ * plane k0 + i in order, with the value neg_lap_at gives bit for bit (the
+ * same neighbours and wall masks). */
+template <bool MASK, typename EMIT>
+__device__ inline void defect_column(const DevDom &D, const Coef &K,
+  const double *__restrict__ phi, const double *__restrict__ rhs,
+  const uint8_t *__restrict__ fluid, int x, int j, int k0, EMIT emit)
+{
+  const Grid &C = D.cc;
+  const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+  const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+  const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+  const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+  const unsigned sz = (unsigned) C.sz;
+  const unsigned c0 = (unsigned) gidx(C, x, j, k0);
+  const unsigned oW = (unsigned) gidx(C, xm, j, k0),
+    oE = (unsigned) gidx(C, xp, j, k0), oS = (unsigned) gidx(C, x, jm, k0),
+    oN = (unsigned) gidx(C, x, jp, k0);
+  const int kb = wrap_lo(k0, D.nz, D.perz);
+  const int kt = wrap_hi(k0 + TILE_Z - 1, D.nz, D.perz);
+  double pc[TILE_Z + 2];
+  pc[0] = ld32(phi, (unsigned) gidx(C, x, j, kb));
+#pragma unroll
+  for (int i = 0; i < TILE_Z; i++) pc[i + 1] = ld32(phi, c0 + i * sz);
+  pc[TILE_Z + 1] = ld32(phi, (unsigned) gidx(C, x, j, kt));
+#pragma unroll
+  for (int h = 0; h < TILE_Z; h += 4) {
+    double w[4], e[4], so[4], no[4], b[4];
+    uint32_t f[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const unsigned o = (unsigned) (h + i) * sz;
+      w[i] = ld32(phi, oW + o);
+      e[i] = ld32(phi, oE + o);
+      so[i] = ld32(phi, oS + o);
+      no[i] = ld32(phi, oN + o);
+      b[i] = ld32(rhs, c0 + o);
+      f[i] = MASK ? (uint32_t) fluid[c0 + o] : 1u;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = k0 + h + i;
+      const double a = neg_lap7(K, pc[h + i + 1], w[i], e[i], so[i], no[i],
+        pc[h + i], pc[h + i + 2], wW, wE, wS, wN, wall_lo(k, D.perz),
+        wall_hi(k, D.nz, D.perz));
+      emit(h + i, c0 + (unsigned) (h + i) * sz, b[i] - a, f[i] != 0u);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+/* The tile is whole (the fast paths above apply). */
+#define WHOLE_TILE \
+  (xb + TILE_X <= D.nx && jb + TILE_Y <= D.ny && k0 + TILE_Z - 1 <= D.nz)
+
 /* ... then r = b - (-A) phi, z = invM r + tile partials of (r, z). */
 __global__ void __launch_bounds__(TILE_THREADS)
 residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
@@ -1413,7 +1455,16 @@ residual_kernel(DevDom D, Coef K, InvMTab T, const CGState *st,
   if (st->done) return;
   TILE_PROLOGUE
   double acc = 0.0;
-  if (colok) {
+  if (WHOLE_TILE) {
+    const int wxy = wcount(x, D.nx, D.perx) + 3 * wcount(j, D.ny, D.pery);
+    auto emit = [&](int i, unsigned c, double rr, bool fl) {
+      st32(r, c, fl ? rr : 0.);
+      double z = rr*(fl ? T.v[wxy + 9 * wcount(k0 + i, D.nz, D.perz)] : 0.);
+      acc += rr * z;
+    };
+    if (fluid) defect_column<true>(D, K, phi, rhs, fluid, x, j, k0, emit);
+    else defect_column<false>(D, K, phi, rhs, fluid, x, j, k0, emit);
+  } else if (colok) {
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
@@ -1680,7 +1731,13 @@ true_residual_kernel(DevDom D, Coef K, const double *__restrict__ phi,
 {
   TILE_PROLOGUE
   double acc = 0.0;
-  if (colok) {
+  if (WHOLE_TILE) {
+    auto emit = [&](int, unsigned, double rr, bool fl) {
+      acc = fl ? acc + rr * rr : acc;
+    };
+    if (fluid) defect_column<true>(D, K, phi, rhs, fluid, x, j, k0, emit);
+    else defect_column<false>(D, K, phi, rhs, fluid, x, j, k0, emit);
+  } else if (colok) {
     for (int k = k0; k <= k1; k++) {
       if (solid(fluid, gidx(C, x, j, k))) continue;
       double rr = rhs[gidx(C, x, j, k)] - neg_lap_at(D, K, phi, x, j, k);
@@ -1769,7 +1826,15 @@ mx_residual_kernel(DevDom D, Coef K, InvMTab T, const double *__restrict__ phi,
 {
   TILE_PROLOGUE
   double acc = 0.0;
-  if (colok) {
+  if (WHOLE_TILE) {
+    const int wxy = wcount(x, D.nx, D.perx) + 3 * wcount(j, D.ny, D.pery);
+    auto emit = [&](int i, unsigned c, double rr, bool fl) {
+      st32(r, c, fl ? (float) rr : 0.f);
+      acc += rr * (rr*(fl ? T.v[wxy + 9 * wcount(k0 + i, D.nz, D.perz)] : 0.));
+    };
+    if (fluid) defect_column<true>(D, K, phi, rhs, fluid, x, j, k0, emit);
+    else defect_column<false>(D, K, phi, rhs, fluid, x, j, k0, emit);
+  } else if (colok) {
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       double rr = rhs[c] - neg_lap_at(D, K, phi, x, j, k);
@@ -1890,8 +1955,9 @@ int bb_spmv_tzb = 0;                   /* tiles per z-column (0: automatic) */
 int bb_spmv_ntp = 1;                   /* 1: FP64 search loads p_old with the
                                           non-temporal hint (0: ordinary
                                           loads, measured slower) */
-int bb_spmv_nts = 1;                   /* 1: the FP64 search stores p_new and
-                                          Ap with the non-temporal hint */
+int bb_spmv_nts = 1;                   /* 1: the FP64 z-column search stores
+                                          p_new and Ap with the non-temporal
+                                          hint (the row kernels always do) */
 int bb_spmv_nts32 = 1;                 /* the same for the FP32 search (with
                                           8-tile columns ordinary stores take
                                           the same time per step, but the
@@ -2316,10 +2382,9 @@ static void launch_row_t(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
 {
   const long nyb = (ddom.ny + RY - 1) / RY, nzc = (ddom.nz + zc - 1) / zc;
-  const int nts = (sizeof(V) == 8 ? bb_spmv_nts : bb_spmv_nts32) != 0;
   BB_TIMED_LAUNCH((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
     dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K, T,
-    _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p, nts);
+    _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);
 }
 
 template <bool FIRST, typename V, bool MIXED, int RY, int DEPTH>
