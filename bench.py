@@ -158,8 +158,9 @@ def parse():
                     help="1: FP64 search kernel loads p_old with the "
                          "non-temporal hint (default), 0: ordinary loads")
     ap.add_argument("--spmv-nts", type=int, default=None,
-                    help="1: the FP64 search stores p_new and Ap with the "
-                         "non-temporal hint (default), 0: ordinary stores")
+                    help="1: the FP64 z-column search stores p_new and Ap "
+                         "with the non-temporal hint (default), 0: ordinary "
+                         "stores (the row kernels always use the hint)")
     ap.add_argument("--spmv-nts32", type=int, default=None,
                     help="the same for the FP32 search (default 1)")
     ap.add_argument("--tzb32", type=int, default=None,
