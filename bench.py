@@ -84,10 +84,10 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 37.44,
+    "spmv_search_kernel": 37.51,
     "update_kernel": 49.16,
     "update_kernel_skip": 25.15,
-    "update_kernel_pair": 57.17,
+    "update_kernel_pair": 57.16,
     "spmv_search_kernel_f32": 16.53,
     "update_kernel_f32_skip": 13.20,
     "update_kernel_f32_pair": 29.34,
