@@ -84,7 +84,7 @@ MASK_BYTES = 1
 # and counter values are in profiles/sediment256_500sph_hbm_traffic.md.
 # They apply to that workload (the mask byte included).
 TRAFFIC_PER_CELL = {
-    "spmv_search_kernel": 37.51,
+    "spmv_search_kernel": 34.50,
     "update_kernel": 49.16,
     "update_kernel_skip": 25.15,
     "update_kernel_pair": 57.16,
@@ -136,6 +136,10 @@ def parse():
     ap.add_argument("--row-depth", type=int, default=None,
                     help="planes of loads in flight in the FP32 row kernel "
                          "(2, 3 or 4; default 2)")
+    ap.add_argument("--row-pairs", type=int, default=None,
+                    help="1: the FP64 row search loads and stores pairs of "
+                         "cells, 1 KB per instruction (default); 0: runs of "
+                         "four cells per lane")
     ap.add_argument("--update-variant", type=int, default=None,
                     help="update kernel: 1 one wavefront per x row where "
                          "the grid allows it (default), 0 one tile per "
@@ -459,6 +463,7 @@ def main():
                       ("bb_row_ry", a.row_ry),
                       ("bb_row_zc", a.row_zc),
                       ("bb_row_depth", a.row_depth),
+                      ("bb_row_pairs", a.row_pairs),
                       ("bb_row_fp64", a.row_fp64),
                       ("bb_update_variant", a.update_variant),
                       ("bb_urow_zc", a.urow_zc),

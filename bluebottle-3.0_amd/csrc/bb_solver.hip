@@ -770,7 +770,8 @@ __device__ inline int row_src(const DevDom &D, int jj)
   return -1;
 }
 
-template <bool FIRST, typename V, bool MIXED, int CPR, int RY, int DEPTH>
+template <bool FIRST, typename V, bool MIXED, int CPR, int RY, int DEPTH,
+          bool PR = false>
 __global__ void __launch_bounds__(RY * WAVE)
 search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const double *__restrict__ rpart, long ntiles, int zc,
@@ -790,6 +791,18 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   __shared__ __attribute__((aligned(16))) V lds[3 * (RY + 2) * ROWV];
   __shared__ V s_invm[27];
   __shared__ double s_tp[ROW_ZCMAX / TILE_Z][CPR][RY];
+  /* PR (pairs; FP64, four cells per lane): lane m loads r and p_old of
+   * cells 2m, 2m + 1 and 128 + 2m, 128 + 2m + 1, forms their p_new and
+   * writes it to the LDS row; each lane reads its own run of four back for
+   * the stencil; p_new and Ap leave through LDS in the same pairs (s_ap:
+   * the wavefront's row of Ap).  Every load and store instruction then
+   * moves 1 KB of consecutive cells instead of 16 bytes at a 32-byte lane
+   * stride, which wrote each line in halves: at 256^3 the kernel wrote 16 %
+   * more than its two arrays and took 124 us per launch; with pairs it
+   * writes exactly its two arrays and takes 103 us.  Arithmetic and tile
+   * partials are unchanged (same bits). */
+  static_assert(!PR || (sizeof(V) == 8 && CPR == 4), "pairs: FP64, 4 cells");
+  __shared__ __attribute__((aligned(16))) V s_ap[PR ? RY * ROWV : 2];
   if (st->done | st->inner_done) return;
   if (threadIdx.x < 27) s_invm[threadIdx.x] = T.v[threadIdx.x];
   const Grid &C = D.cc;
@@ -804,6 +817,10 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const int j = jb + wv + 1;
   const int x0 = CPR * lane + 1;
   const bool xok = x0 <= D.nx;            /* nx % CPR == 0: the whole run */
+  /* pairs: x - 1 of the lane's two pairs, and whether each lies in the row
+   * (nx % 4 == 0: a pair lies in it whole or not at all) */
+  const int qp0 = 2 * lane, qp1 = 2 * WAVE + 2 * lane;
+  const bool pok0 = qp0 < D.nx, pok1 = qp1 < D.nx;
   const bool colok = xok && j <= D.ny;
   /* the wavefront's own row is LDS row wv + 1.  The two halo rows (LDS
    * rows 0 and RY + 1) are shared out evenly, so that every wavefront
@@ -824,7 +841,9 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   const bool oko = xok && so >= 0, okh = sh >= 0;
   int wx[CPR];
 #pragma unroll
-  for (int e = 0; e < CPR; e++) wx[e] = wcount(x0 + e, D.nx, D.perx);
+  for (int e = 0; e < CPR; e++)
+    wx[e] = wcount(PR ? (e < 2 ? qp0 : qp1 - 2) + e + 1 : x0 + e, D.nx,
+                   D.perx);
   const int wyo = 3 * wcount(so, D.ny, D.pery);
   const int wxyh = wcount(xh, D.nx, D.perx) + 3 * wcount(sh, D.ny, D.pery);
 
@@ -833,14 +852,32 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     Raw R;
     int ks = plane_src(D, kk);
     unsigned off = (unsigned) (ks ? ks : 1) * (unsigned) C.sz;
-    R.r = run_load<V, CPR, false>(r, co + off);
+    if constexpr (PR) {
+      /* the lane's two pairs (x = 1 for a pair past the row) */
+      const unsigned rb = 1u + (unsigned) (so >= 0 ? so : 1) * (unsigned) C.sy
+                        + off;
+      const unsigned a0 = rb + (pok0 ? (unsigned) qp0 : 0u),
+                     a1 = rb + (pok1 ? (unsigned) qp1 : 0u);
+      const Run<V, 2> r0 = run_load<V, 2, false>(r, a0),
+                      r1 = run_load<V, 2, false>(r, a1);
+      R.r.v[0] = r0.v[0]; R.r.v[1] = r0.v[1];
+      R.r.v[2] = r1.v[0]; R.r.v[3] = r1.v[1];
+      if (!FIRST) {
+        const Run<V, 2> p0 = run_load<V, 2, NTP>(pold, a0),
+                        p1 = run_load<V, 2, NTP>(pold, a1);
+        R.p.v[0] = p0.v[0]; R.p.v[1] = p0.v[1];
+        R.p.v[2] = p1.v[0]; R.p.v[3] = p1.v[1];
+      }
+    } else {
+      R.r = run_load<V, CPR, false>(r, co + off);
+      if (!FIRST) R.p = run_load<V, CPR, NTP>(pold, co + off);
+    }
     R.hr = ld32(r, ch + off);
     if (FIRST) {
 #pragma unroll
       for (int e = 0; e < CPR; e++) R.p.v[e] = (V) 0;
       R.hp = (V) 0;
     } else {
-      R.p = run_load<V, CPR, NTP>(pold, co + off);
       R.hp = NTP ? ld32_nt(pold, ch + off) : ld32(pold, ch + off);
     }
     return R;
@@ -874,9 +911,24 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     for (int e = 0; e < CPR; e++) {
       V z = R.r.v[e]*(zc2 ? mo[1][e] : mo[0][e]);
       V v = FIRST ? z : z + beta*R.p.v[e];
-      o.v[e] = (ks && oko) ? v : (V) 0;
+      const bool ok = PR ? so >= 0 && (e < 2 ? pok0 : pok1) : oko;
+      o.v[e] = (ks && ok) ? v : (V) 0;
     }
-    LR::put(buf + (wv + 1) * ROWV, lane, o);
+    if constexpr (PR) {
+      /* the pairs at LdsRow<V, 4>::pos(qp0) and pos(qp1) */
+      typedef V V2 __attribute__((ext_vector_type(2)));
+      V *own = buf + (wv + 1) * ROWV;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int ps = ((lane & 1) * WAVE + h * (WAVE / 2) + (lane >> 1)) * 2;
+        V2 t;
+        t[0] = o.v[2 * h];
+        t[1] = o.v[2 * h + 1];
+        *(V2 *) __builtin_assume_aligned(own + ps, 16) = t;
+      }
+    } else {
+      LR::put(buf + (wv + 1) * ROWV, lane, o);
+    }
     V zh = R.hr*(zc2 ? mh[1] : mh[0]);
     V vh = FIRST ? zh : zh + beta*R.hp;
     vh = (ks && okh) ? vh : (V) 0;
@@ -888,6 +940,10 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
   R_t prev = put(Rm, bprev, k0 - 1);
   R_t cur = put(Rc, bcur, k0);
   __syncthreads();
+  if constexpr (PR) {
+    prev = LR::get(bprev + (wv + 1) * ROWV, lane);
+    cur = LR::get(bcur + (wv + 1) * ROWV, lane);
+  }
   const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
   /* x = 1 and x = nx: the first cell of lane 0 and the last of lane
    * nx / CPR - 1, the only cells that can have an x wall */
@@ -920,6 +976,7 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
     R_t nxt = put(Rs, bnext, k + 1);
     Rs = load(min(k + 1 + DEPTH, kend + 1));
     __syncthreads();
+    if constexpr (PR) nxt = LR::get(bnext + (wv + 1) * ROWV, lane);
     const R_t S = LR::get(bcur + wv * ROWV, lane),
               N = LR::get(bcur + (wv + 2) * ROWV, lane);
     /* run neighbours: the last cell of lane l - 1, the first of l + 1, and
@@ -938,9 +995,36 @@ search_row_kernel(DevDom D, CoefT<V> K, InvMT<V> T, CGState *st, int q,
       a = lap(std::false_type(), S, N, nxt, wl, er, xn, x1, wB, wT);
     /* cells outside the domain store to the never-read edge ghost
      * (x = 1, j = 0, k = 0), aligned like every run */
-    unsigned c = colok ? co + (unsigned) k * (unsigned) C.sz : 1u;
-    run_store<V, CPR, true>(pnew, c, cur);
-    run_store<V, CPR, true>(Ap, c, a);
+    if constexpr (PR) {
+      V *sa = s_ap + wv * ROWV;
+      LR::put(sa, lane, a);
+      /* the reads below take other lanes' values: LDS operations of one
+       * wavefront complete in order; keep the compiler from moving them */
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const V *own = bcur + (wv + 1) * ROWV;
+      /* element 1 (x = 1) of this row and plane */
+      const unsigned rb = 1u + (unsigned) (so >= 0 ? so : 1) * (unsigned) C.sy
+                        + (unsigned) k * (unsigned) C.sz;
+      typedef V V2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int qx = h * (2 * WAVE) + 2 * lane;     /* x - 1 of the pair */
+        /* LdsRow<V, 4>::pos(qx), a 16-byte slot */
+        const int ps = ((lane & 1) * WAVE + h * (WAVE / 2) + (lane >> 1)) * 2;
+        const V2 pv = *(const V2 *) __builtin_assume_aligned(own + ps, 16);
+        const V2 av = *(const V2 *) __builtin_assume_aligned(sa + ps, 16);
+        if (j <= D.ny && qx < D.nx) {
+          __builtin_nontemporal_store(pv, (V2 *) (pnew + rb + qx));
+          __builtin_nontemporal_store(av, (V2 *) (Ap + rb + qx));
+        }
+      }
+    } else {
+      unsigned c = colok ? co + (unsigned) k * (unsigned) C.sz : 1u;
+      run_store<V, CPR, true>(pnew, c, cur);
+      run_store<V, CPR, true>(Ap, c, a);
+    }
 #pragma unroll
     for (int e = 0; e < CPR; e++)
       acc[e] += colok ? (double) cur.v[e] * (double) a.v[e] : 0.0;
@@ -1931,6 +2015,11 @@ int bb_row_last[4] = {0, 0, 0, 0};     /* the row kernel's last cells per
 int bb_row_depth = 0;                  /* planes of loads in flight in the
                                           FP32 row kernel: 3 or 4, else 2
                                           (FP64: 2) */
+int bb_row_pairs = 1;                  /* 1: the FP64 row search with four
+                                          cells per lane loads and stores
+                                          pairs, 1 KB per instruction (0:
+                                          runs of four, 16 B per lane at a
+                                          32 B stride) */
 int bb_update_variant = 1;             /* 1: update kernel with one
                                           wavefront per x row where the grid
                                           allows it (FP32; FP64 with
@@ -2382,6 +2471,15 @@ static void launch_row_t(const CoefT<V> &K, const InvMT<V> &T, int q,
   long nrp, int zc, const V *r, const V *pold, V *pnew, V *Ap)
 {
   const long nyb = (ddom.ny + RY - 1) / RY, nzc = (ddom.nz + zc - 1) / zc;
+  if constexpr (sizeof(V) == 8 && CPR == 4) {
+    if (bb_row_pairs) {
+      BB_TIMED_LAUNCH((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH,
+                                         true>),
+        dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K,
+        T, _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);
+      return;
+    }
+  }
   BB_TIMED_LAUNCH((search_row_kernel<FIRST, V, MIXED, CPR, RY, DEPTH>),
     dim3((unsigned) (nyb * nzc)), dim3(RY * WAVE), 0, bb_stream, ddom, K, T,
     _cg_state, q, _part_r, nrp, zc, r, pold, pnew, Ap, _part_p);

@@ -28,7 +28,7 @@ FIELDS = ("u", "v", "w", "p", "phi")
 
 
 def knobs(variant=3, ry=0, zc=0, fp64=1, depth=0, uvar=1, ufp64=0, uzc=0,
-          udepth=0):
+          udepth=0, pairs=None):
     """fp64 = 0: the plain form's FP64 search takes the z-column kernel
     (the row kernel by default); ufp64 = 1: its FP64 update takes the row
     kernel too (by default only the mixed form's FP32 one does); uvar 0:
@@ -45,12 +45,16 @@ def knobs(variant=3, ry=0, zc=0, fp64=1, depth=0, uvar=1, ufp64=0, uzc=0,
     ctypes.c_int.in_dll(lib, "bb_urow_depth").value = udepth
     ctypes.c_int.in_dll(lib, "bb_defer_phi").value = 1
     ctypes.c_int.in_dll(lib, "bb_pcg_batch").value = 16
+    if pairs is not None:
+        ctypes.c_int.in_dll(lib, "bb_row_pairs").value = pairs
 
 
 @pytest.fixture(autouse=True)
 def default_knobs():
+    lib = bbhip.load()
+    pairs0 = ctypes.c_int.in_dll(lib, "bb_row_pairs").value
     yield
-    knobs()
+    knobs(pairs=pairs0)
 
 
 def last_kernel():
@@ -130,6 +134,29 @@ def test_row_kernel_plain_bitwise(case_dir, nx, per_x, per_y, zc, cpr):
     assert out["update"] == 1
     assert z == (zc if zc else 8)
     print("iters gpu", out["iters"], "oracle", ref["iters"])
+    assert out["iters"] == ref["iters"]
+    for f in FIELDS:
+        assert bitwise_equal(out["f"][f], ref["f"][f]), \
+            describe(f, out["f"][f], ref["f"][f])
+
+
+@pytest.mark.parametrize("pairs", [0, 1])
+@pytest.mark.parametrize("nx,per_x,per_y,zc", [
+    (132, 1, 1, 0), (256, 0, 0, 16), (256, 1, 1, 8)])
+def test_row_kernel_fp64_layouts_bitwise(case_dir, nx, per_x, per_y, zc,
+                                         pairs):
+    """The FP64 row search with four cells per lane, in its two layouts:
+    each lane loading and storing its own run of four (pairs 0), or lane m
+    loading r and p_old of cells 2m, 2m + 1 and 128 + 2m, 128 + 2m + 1,
+    passing p_new through LDS and storing p_new and Ap in the same pairs
+    (pairs 1): the oracle's bits either way.  nx = 132 leaves lanes and
+    pairs past the row, ny = 11 a second group of rows past the domain."""
+    kw = box((nx, 11, 13), per_x, per_y)
+    ref = run_oracle(case_dir, kw, 2)
+    knobs(variant=3, zc=zc, fp64=1, pairs=pairs)
+    out = run_gpu(case_dir, kw, 2)
+    kern, (c, _, _, _) = out["kernel"]
+    assert kern == 3 and c == 4, out["kernel"]
     assert out["iters"] == ref["iters"]
     for f in FIELDS:
         assert bitwise_equal(out["f"][f], ref["f"][f]), \
