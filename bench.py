@@ -136,6 +136,11 @@ def parse():
     ap.add_argument("--row-depth", type=int, default=None,
                     help="planes of loads in flight in the FP32 row kernel "
                          "(2, 3 or 4; default 2)")
+    ap.add_argument("--pass-span", type=int, default=None,
+                    help="previous timestep's solutions in the Galerkin "
+                         "span of the history and learned forms: 1 of the "
+                         "same Lamb pass, 2 also of the pass before, 3 also "
+                         "of the pass after (library default 3)")
     ap.add_argument("--row-pairs", type=int, default=None,
                     help="1: the FP64 row search loads and stores pairs of "
                          "cells, 1 KB per instruction (default); 0: runs of "
@@ -214,7 +219,8 @@ def log(a, *msg):
         print(*msg, file=sys.stderr, flush=True)
 
 
-def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None):
+def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None,
+             pass_span=0):
     """Warm-up (and, for the learned guess, the network's training solves),
     then exactly a.steps timed timesteps.  nwarm fixes the warm-up length
     (so that every form times the same physical steps)."""
@@ -223,6 +229,7 @@ def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None):
     write_case(a, case, precision, warm)
     ctypes.c_int.in_dll(lib, "bb_warm_use_net").value = use_net
     ctypes.c_int.in_dll(lib, "bb_warm_history").value = history
+    ctypes.c_int.in_dll(lib, "bb_warm_pass_span").value = pass_span
     tot = ctypes.c_long.in_dll(lib, "bb_pcg_iters_total")
     solves = ctypes.c_long.in_dll(lib, "bb_pcg_solves_total")
     wsolves = ctypes.c_int.in_dll(lib, "bb_warm_solves")
@@ -302,6 +309,7 @@ def run_form(lib, a, precision, warm, use_net=1, history=4, nwarm=None):
         "last_true_resid": stats["true_resid"],
         "warmup_steps": nw,
         "history": history if warm_mode == "LEARNED" else 0,
+        "pass_span": pass_span if warm_mode == "LEARNED" else 0,
         "warmup_s": tw1 - tw0,
         "learned_guess": wstat,
         "ktimes": ktimes,
@@ -453,6 +461,8 @@ def main():
                           "stream4_gbs": stream4_gbs,
                           "stream4_elements": cells}))
         return
+    # the library's default span of earlier timesteps' solutions
+    ps_default = ctypes.c_int.in_dll(lib, "bb_warm_pass_span").value
     for name, val in (("bb_spmv_variant", a.spmv_variant),
                       ("bb_spmv_tzb", a.tzb),
                       ("bb_spmv_ntp", a.spmv_ntp),
@@ -493,7 +503,11 @@ def main():
     for key, prec, warm, use_net, hist in order:
         if a.history is not None and hist > 1:
             hist = a.history
-        res[key] = run_form(lib, a, prec, warm, use_net, hist, nwarm)
+        # the last-solution start (history 1) is that alone; the other
+        # learned-mode forms add the previous timestep's solutions
+        ps = 0 if hist <= 1 else (a.pass_span if a.pass_span is not None
+                                  else ps_default)
+        res[key] = run_form(lib, a, prec, warm, use_net, hist, nwarm, ps)
         if nwarm is None:
             nwarm = res[key]["warmup_steps"]
     mask = a.workload == "sediment" and a.nparts > 0

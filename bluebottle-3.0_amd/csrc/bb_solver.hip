@@ -1709,57 +1709,64 @@ gram_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid, GVecs V,
   }
 }
 
-/* The same sums for one new direction v_n among M (n = -1: none): per
- * workgroup, (b, v_a) for every a into rows 0..M-1 and (v_a, -A v_n) for
- * every a into rows GMAX..GMAX+M-1.  Used when the other entries of the
- * Gram matrix are still valid (same directions, same operator). */
-template <int M>
+/* The same sums for up to two new directions v_n1, v_n2 among M (n1 = -1:
+ * none; NN = 2: n2 as well): per workgroup, (b, v_a) for every a into rows
+ * 0..M-1, (v_a, -A v_n1) into rows GMAX..GMAX+M-1 and (v_a, -A v_n2) into
+ * rows 2 GMAX..2 GMAX+M-1.  Used when the other entries of the Gram matrix
+ * are still valid (same directions, same operator). */
+template <int M, int NN>
 __global__ void __launch_bounds__(TILE_THREADS)
 gram_new_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
-  GVecs V, int n, const double *__restrict__ rhs, double *__restrict__ part)
+  GVecs V, int n1, int n2, const double *__restrict__ rhs,
+  double *__restrict__ part)
 {
   TILE_PROLOGUE
-  double a[2 * M];
+  constexpr int T = (1 + NN) * M;
+  double a[T];
 #pragma unroll
-  for (int t = 0; t < 2 * M; t++) a[t] = 0.;
+  for (int t = 0; t < T; t++) a[t] = 0.;
   if (colok) {
     const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
     const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
     const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
     const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
-    const double *fn = V.v[n >= 0 ? n : 0];
+    const double *f1 = V.v[n1 >= 0 ? n1 : 0];
+    const double *f2 = V.v[NN == 2 && n2 >= 0 ? n2 : 0];
 #pragma unroll 1
     for (int k = k0; k <= k1; k++) {
       long c = gidx(C, x, j, k);
       if (solid(fluid, c)) continue;
-      double Avn = 0.;
-      if (n >= 0) {
-        const int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
-        const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
-        Avn = neg_lap7(K, fn[c], fn[gidx(C, xm, j, k)], fn[gidx(C, xp, j, k)],
-                       fn[gidx(C, x, jm, k)], fn[gidx(C, x, jp, k)],
-                       fn[gidx(C, x, j, km)], fn[gidx(C, x, j, kp)],
-                       wW, wE, wS, wN, wB, wT);
-      }
+      const int km = wrap_lo(k, D.nz, D.perz), kp = wrap_hi(k, D.nz, D.perz);
+      const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+      auto nl = [&](const double *f) {
+        return neg_lap7(K, f[c], f[gidx(C, xm, j, k)], f[gidx(C, xp, j, k)],
+                        f[gidx(C, x, jm, k)], f[gidx(C, x, jp, k)],
+                        f[gidx(C, x, j, km)], f[gidx(C, x, j, kp)],
+                        wW, wE, wS, wN, wB, wT);
+      };
+      const double A1 = n1 >= 0 ? nl(f1) : 0.;
+      const double A2 = NN == 2 ? nl(f2) : 0.;
       const double b = rhs[c];
 #pragma unroll
       for (int q = 0; q < M; q++) {
         const double vq = V.v[q][c];
         a[q] += b * vq;
-        a[M + q] += vq * Avn;
+        a[M + q] += vq * A1;
+        if (NN == 2) a[2 * M + q] += vq * A2;
       }
     }
   }
-  __shared__ double s[TILE_Y][2 * M];
+  __shared__ double s[TILE_Y][T];
 #pragma unroll
-  for (int t = 0; t < 2 * M; t++) {
+  for (int t = 0; t < T; t++) {
     double v = wave_tree_sum(a[t]);
     if (lane == 0) s[wv][t] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 2 * M) {
+  if (threadIdx.x < T) {
     const int t = threadIdx.x;
-    const int row = t < M ? t : GMAX + (t - M);
+    const int row = t < M ? t : (t < 2 * M ? GMAX + (t - M)
+                                           : 2 * GMAX + (t - 2 * M));
     double v = 0.;
     for (int w = 0; w < TILE_Y; w++) v += s[w][t];
     part[(long) row * gridDim.x + blockIdx.x] = v;
@@ -2289,23 +2296,34 @@ double bb_pp_galerkin_many(const double *const *vecs, const long *gens, int m,
   GVecs V;
   for (int q = 0; q < GMAX; q++) V.v[q] = q < m ? vecs[q] : vecs[0];
   /* which directions the kept Gram matrix already holds */
-  int idx[GMAX], nnew = 0, inew = -1;
+  int idx[GMAX], nnew = 0, inew[2] = {-1, -1};
   for (int q = 0; q < m; q++) {
     idx[q] = -1;
     if (gens && gc_mask == bb_mask_gen)
       for (int r = 0; r < gc_m; r++)
         if (gc_ptr[r] == vecs[q] && gc_gen[r] == gens[q]) idx[q] = r;
-    if (idx[q] < 0) { nnew++; inew = q; }
+    if (idx[q] < 0) {
+      if (nnew < 2) inew[nnew] = q;
+      nnew++;
+    }
   }
-  const bool incr = nnew <= 1;
+  const bool incr = nnew <= 2;
   /* rows of the terms a smaller m leaves out are summed too: zero them */
   hipLaunchKernelGGL(zero_rows_kernel, dim3(nblk_zero(GTERMS * nt)),
     dim3(256), 0, bb_stream, _gram_part, (long) GTERMS * nt);
-  if (incr) {
+  if (incr && nnew <= 1) {
     switch (m) {
-#define GN(M) case M: hipLaunchKernelGGL(gram_new_kernel<M>, TILES_LAUNCH, \
-    ddom, coef(), fluid_mask(), V, inew, _rhs_p, _gram_part); break;
+#define GN(M) case M: hipLaunchKernelGGL((gram_new_kernel<M, 1>), TILES_LAUNCH,\
+    ddom, coef(), fluid_mask(), V, inew[0], -1, _rhs_p, _gram_part); break;
       GN(1) GN(2) GN(3) GN(4) GN(5) GN(6) GN(7) GN(8) GN(9)
+#undef GN
+    }
+  } else if (incr) {
+    switch (m) {
+#define GN(M) case M: hipLaunchKernelGGL((gram_new_kernel<M, 2>), TILES_LAUNCH,\
+    ddom, coef(), fluid_mask(), V, inew[0], inew[1], _rhs_p, _gram_part); \
+    break;
+      GN(2) GN(3) GN(4) GN(5) GN(6) GN(7) GN(8) GN(9)
 #undef GN
     }
   } else {
@@ -2326,10 +2344,11 @@ double bb_pp_galerkin_many(const double *const *vecs, const long *gens, int m,
   if (incr) {
     for (int p = 0; p < m; p++)
       for (int q = 0; q < m; q++)
-        if (p != inew && q != inew) G[p][q] = gc_G[idx[p]][idx[q]];
-    if (inew >= 0)
-      for (int a = 0; a < m; a++)
-        G[a][inew] = G[inew][a] = gram_host[GMAX + a];
+        if (idx[p] >= 0 && idx[q] >= 0) G[p][q] = gc_G[idx[p]][idx[q]];
+    for (int d = 0; d < 2; d++)
+      if (inew[d] >= 0)
+        for (int a = 0; a < m; a++)
+          G[a][inew[d]] = G[inew[d]][a] = gram_host[(1 + d) * GMAX + a];
   } else {
     int t = GMAX;
     for (int p = 0; p < GMAX; p++)

@@ -100,6 +100,19 @@ static long psi_gen = 0;
  * cells of (a solution is, for the mask it was solved with). */
 extern long bb_mask_gen;               /* bb_particle.hip */
 static long hist_mask[HIST_MAX];
+/* Solutions by Lamb pass: set cur holds this timestep's, 1 - cur the
+ * previous timestep's (pass j of a step tends to differ from pass j - 1 as
+ * it did one step earlier).  Allocated on first use. */
+#define PASS_MAX 16
+#define SPAN_MAX 9                     /* directions of the Galerkin system */
+int bb_warm_pass_span = 3;             /* the previous timestep's solutions
+                                          in the span: 1 of this Lamb pass,
+                                          2 also of the pass before, 3 also
+                                          of the pass after */
+static double *_pass[2][PASS_MAX];
+static long pass_gen[2][PASS_MAX], pass_mask[2][PASS_MAX];
+static int pass_n[2] = {0, 0}, pass_cur = 0, pass_idx = 0;
+static long pass_step = -1;
 static double *_psi = NULL, *_wres = NULL;
 static float *_win = NULL, *_wtgt = NULL, *_wout = NULL;
 static float *_cin = NULL, *_ctgt = NULL;
@@ -332,6 +345,9 @@ extern "C" void bb_warm_setup(void)
   _wres = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
   hist_n = hist_head = 0;
   for (int h = 0; h < HIST_MAX; h++) hist_mask[h] = -1;
+  pass_n[0] = pass_n[1] = 0;
+  pass_cur = pass_idx = 0;
+  pass_step = -1;
   net_step = -1;
   net_this_solve = 0;
   bb_warm_err_last = bb_warm_err_guess = bb_warm_err_hist = 0.;
@@ -356,6 +372,11 @@ extern "C" void bb_warm_teardown(void)
     _hist[h] = NULL;
   }
   bb_grid_free(_wres);
+  for (int c = 0; c < 2; c++)
+    for (int j = 0; j < PASS_MAX; j++) {
+      if (_pass[c][j]) bb_grid_free(_pass[c][j]);
+      _pass[c][j] = NULL;
+    }
   _psi = _wres = NULL;
   _win = _wtgt = _wout = _cin = _ctgt = NULL;
   _wpart = NULL;
@@ -413,11 +434,28 @@ extern "C" int cuda_PP_warmstart_infer(void)
   int hmax = bb_warm_history < 1 ? 1 : (bb_warm_history > HIST_MAX
                                           ? HIST_MAX : bb_warm_history);
   int nh = bb_warm_use_last ? (hist_n < hmax ? hist_n : hmax) : 0;
+  if (stepnum != pass_step) {            /* a new timestep: pass 0 */
+    pass_cur = 1 - pass_cur;
+    pass_n[pass_cur] = 0;
+    pass_idx = 0;
+    pass_step = stepnum;
+  }
+  /* the previous timestep's solutions of this pass and the one before
+   * join the span; the span holds at most SPAN_MAX directions, psi
+   * included (the solver's Gram system) */
+  const int np = bb_warm_pass_span < 0 ? 0 : (bb_warm_pass_span > 3 ? 3
+                                              : bb_warm_pass_span);
+  int npass = 0;
+  for (int d = 0; d < np; d++) {
+    const int jj = pass_idx + (d == 2 ? 1 : -d);
+    npass += jj >= 0 && jj < pass_n[1 - pass_cur];
+  }
+  if (nh + npass + net_ready > SPAN_MAX) nh = SPAN_MAX - npass - net_ready;
   /* the earlier solutions, most recent first; particles may have moved
    * since they were solved, and a value kept in a cell a particle has
    * entered would act as boundary data of the fluid rows */
-  const double *vecs[HIST_MAX + 1];
-  long gens[HIST_MAX + 1];
+  const double *vecs[SPAN_MAX];
+  long gens[SPAN_MAX];
   for (int h = 0; h < nh; h++) {
     const int slot = (hist_head + HIST_MAX - 1 - h) % HIST_MAX;
     double *v = _hist[slot];
@@ -431,6 +469,21 @@ extern "C" int cuda_PP_warmstart_infer(void)
     }
     vecs[h] = v;
     gens[h] = hist_gen[slot];
+  }
+  {
+    const int pv = 1 - pass_cur;
+    for (int d = 0; d < np; d++) {
+      const int jj = pass_idx + (d == 2 ? 1 : -d);
+      if (jj < 0 || jj >= pass_n[pv]) continue;
+      double *v = _pass[pv][jj];
+      if (fluid && pass_mask[pv][jj] != bb_mask_gen) {
+        hipLaunchKernelGGL(mask_solid_kernel, dim3(nb(wnv)), dim3(256), 0,
+          bb_stream, ddom, fluid, v);
+        pass_mask[pv][jj] = bb_mask_gen;
+      }
+      vecs[nh] = v;
+      gens[nh++] = pass_gen[pv][jj];
+    }
   }
   if (net_ready) {
     build_input(fluid);
@@ -523,4 +576,14 @@ extern "C" void cuda_PP_warmstart_train(void)
   hist_mask[hist_head] = bb_mask_gen;
   hist_head = (hist_head + 1) % HIST_MAX;
   if (hist_n < HIST_MAX) hist_n++;
+  if (bb_warm_pass_span && pass_step == stepnum && pass_idx < PASS_MAX) {
+    double *&v = _pass[pass_cur][pass_idx];
+    if (!v) v = (double *) bb_grid_alloc(ddom.cc.n, sizeof(double));
+    BB_CHECK(hipMemcpyAsync(v, _phi, ddom.cc.n * sizeof(double),
+      hipMemcpyDeviceToDevice, bb_stream));
+    pass_gen[pass_cur][pass_idx] = ++gen_counter;
+    pass_mask[pass_cur][pass_idx] = bb_mask_gen;
+    pass_n[pass_cur] = pass_idx + 1;
+  }
+  pass_idx++;
 }

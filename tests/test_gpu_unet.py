@@ -135,8 +135,9 @@ def test_adam_reduces_loss(net):
 def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
     """PCG iterations per timestep (all Lamb passes) with a given start:
     'zero' (the reference's phi0 = 0), 'last' (span{previous solution}),
-    'hist' (span of the last six solutions), 'learned' (span of the last
-    six solutions and the network's guess)."""
+    'hist' (span of the last six solutions), 'pass' (those and the previous
+    timestep's solutions of the same Lamb pass, the pass before and the
+    pass after), 'learned' (that span and the network's guess)."""
     kw = dict(kw)
     ext = dict(kw.get("extensions") or {})
     ext["pp_warmstart"] = "OFF" if mode == "zero" else "LEARNED"
@@ -144,10 +145,13 @@ def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
     bbconfig.write_case(case_dir, **kw)
     bbconfig.write_parts(case_dir, parts)
     hist0 = ctypes.c_int.in_dll(L, "bb_warm_history").value
+    pass0 = ctypes.c_int.in_dll(L, "bb_warm_pass_span").value
     ctypes.c_int.in_dll(L, "bb_warm_use_net").value = \
         1 if mode == "learned" else 0
     ctypes.c_int.in_dll(L, "bb_warm_history").value = \
         1 if mode == "last" else 6
+    ctypes.c_int.in_dll(L, "bb_warm_pass_span").value = \
+        3 if mode in ("pass", "learned") else 0
     tot = ctypes.c_long.in_dll(L, "bb_pcg_iters_total")
     per_step, resid = [], []
     with bbhip.Simulation(case_dir, kw["n"]) as s:
@@ -162,6 +166,7 @@ def solve_iterations(case_dir, kw, parts, nsteps, mode, L):
                   ctypes.c_int.in_dll(L, "bb_warm_net_solves").value)
     ctypes.c_int.in_dll(L, "bb_warm_use_net").value = 1
     ctypes.c_int.in_dll(L, "bb_warm_history").value = hist0
+    ctypes.c_int.in_dll(L, "bb_warm_pass_span").value = pass0
     return per_step, resid, energy
 
 
@@ -172,8 +177,10 @@ def test_warm_start_in_solver(case_dir):
     of its guess is compared with that of the same span without the
     network's direction (both against the converged solution), and the PCG
     iterations of steps 3-4 with the network must be fewer than without
-    it (the last solution alone, the fallback start).  The span of the
-    last six solutions without the network is reported beside them."""
+    it (the last solution alone, the fallback start).  Adding the previous
+    timestep's solutions of the same Lamb pass, the pass before and the
+    pass after to the span of the last six solutions must take fewer
+    iterations than that span alone (measured 1919 against 2079)."""
     kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=7)
     L = lib()
     L.bb_unet_destroy()             # the solver builds its own network
@@ -184,17 +191,18 @@ def test_warm_start_in_solver(case_dir):
         ctypes.c_int.in_dll(L, k).value = v
     try:
         res = {m: solve_iterations(case_dir, kw, parts, 4, m, L)
-               for m in ("zero", "last", "hist", "learned")}
+               for m in ("zero", "last", "hist", "pass", "learned")}
     finally:
         for k, v in saved.items():
             ctypes.c_int.in_dll(L, k).value = v
     for m, (it, rs, _) in res.items():
         print(f"{m:8s} PCG iterations per step {it}  final resid {rs}")
         assert all(r <= 1e-8 for r in rs)
-    z, last, hist, lrn = (sum(res[m][0][2:]) for m in
-                          ("zero", "last", "hist", "learned"))
+    z, last, hist, pas, lrn = (sum(res[m][0][2:]) for m in
+                               ("zero", "last", "hist", "pass", "learned"))
     el, eh, eg, nsolves = res["learned"][2]
-    print("steps 3-4: zero", z, "last", last, "hist", hist, "learned", lrn)
+    print("steps 3-4: zero", z, "last", last, "hist", hist, "pass", pas,
+          "learned", lrn)
     print(f"solves with the network: {nsolves}; A-norm error^2 summed over "
           f"them: last solution {el:.4e}, earlier solutions {eh:.4e}, "
           f"with the network {eg:.4e}")
@@ -202,3 +210,4 @@ def test_warm_start_in_solver(case_dir):
     assert nsolves > 0
     assert eg <= eh * (1 + 1e-6)
     assert lrn < last
+    assert pas < hist
