@@ -1714,7 +1714,7 @@ gram_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid, GVecs V,
  * 0..M-1, (v_a, -A v_n1) into rows GMAX..GMAX+M-1 and (v_a, -A v_n2) into
  * rows 2 GMAX..2 GMAX+M-1.  Used when the other entries of the Gram matrix
  * are still valid (same directions, same operator). */
-template <int M, int NN>
+template <int M, int NN, bool MASK>
 __global__ void __launch_bounds__(TILE_THREADS)
 gram_new_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
   GVecs V, int n1, int n2, const double *__restrict__ rhs,
@@ -1725,7 +1725,55 @@ gram_new_kernel(DevDom D, Coef K, const uint8_t *__restrict__ fluid,
   double a[T];
 #pragma unroll
   for (int t = 0; t < T; t++) a[t] = 0.;
-  if (colok) {
+  if (WHOLE_TILE && n1 >= 0) {
+    /* whole tile: the new directions' columns are carried in z, every
+     * load by a 32-bit offset and none behind a branch (particle cells add
+     * nothing: the mask selects) */
+    const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
+    const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
+    const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
+    const bool wS = wall_lo(j, D.pery), wN = wall_hi(j, D.ny, D.pery);
+    const unsigned sz = (unsigned) C.sz, c0 = (unsigned) gidx(C, x, j, k0);
+    const unsigned oW = (unsigned) gidx(C, xm, j, k0),
+      oE = (unsigned) gidx(C, xp, j, k0), oS = (unsigned) gidx(C, x, jm, k0),
+      oN = (unsigned) gidx(C, x, jp, k0);
+    const unsigned cb = (unsigned) gidx(C, x, j, wrap_lo(k0, D.nz, D.perz)),
+      ct = (unsigned) gidx(C, x, j, wrap_hi(k0 + TILE_Z - 1, D.nz, D.perz));
+    const double *f[2] = {V.v[n1], V.v[NN == 2 && n2 >= 0 ? n2 : n1]};
+    double pm[2], pc[2];
+#pragma unroll
+    for (int d = 0; d < NN; d++) {
+      pm[d] = ld32(f[d], cb);
+      pc[d] = ld32(f[d], c0);
+    }
+#pragma unroll
+    for (int i = 0; i < TILE_Z; i++) {
+      const unsigned o = (unsigned) i * sz, c = c0 + o;
+      const int k = k0 + i;
+      const bool wB = wall_lo(k, D.perz), wT = wall_hi(k, D.nz, D.perz);
+      double An[2];
+#pragma unroll
+      for (int d = 0; d < NN; d++) {
+        const double pt = ld32(f[d], i == TILE_Z - 1 ? ct : c + sz);
+        An[d] = neg_lap7(K, pc[d], ld32(f[d], oW + o), ld32(f[d], oE + o),
+                         ld32(f[d], oS + o), ld32(f[d], oN + o), pm[d], pt,
+                         wW, wE, wS, wN, wB, wT);
+        pm[d] = pc[d];
+        pc[d] = pt;
+      }
+      const bool fl = MASK ? fluid[c] != 0 : true;
+      const double bl = ld32(rhs, c);
+      const double b = fl ? bl : 0.;
+#pragma unroll
+      for (int q = 0; q < M; q++) {
+        const double vl = ld32(V.v[q], c);
+        const double vq = fl ? vl : 0.;
+        a[q] += b * vq;
+        a[M + q] += vq * An[0];
+        if (NN == 2) a[2 * M + q] += vq * An[1];
+      }
+    }
+  } else if (colok) {
     const int xm = wrap_lo(x, D.nx, D.perx), xp = wrap_hi(x, D.nx, D.perx);
     const int jm = wrap_lo(j, D.ny, D.pery), jp = wrap_hi(j, D.ny, D.pery);
     const bool wW = wall_lo(x, D.perx), wE = wall_hi(x, D.nx, D.perx);
@@ -2308,21 +2356,26 @@ double bb_pp_galerkin_many(const double *const *vecs, const long *gens, int m,
     }
   }
   const bool incr = nnew <= 2;
+  const uint8_t *fl = fluid_mask();
   /* rows of the terms a smaller m leaves out are summed too: zero them */
   hipLaunchKernelGGL(zero_rows_kernel, dim3(nblk_zero(GTERMS * nt)),
     dim3(256), 0, bb_stream, _gram_part, (long) GTERMS * nt);
   if (incr && nnew <= 1) {
     switch (m) {
-#define GN(M) case M: hipLaunchKernelGGL((gram_new_kernel<M, 1>), TILES_LAUNCH,\
-    ddom, coef(), fluid_mask(), V, inew[0], -1, _rhs_p, _gram_part); break;
+#define GN(M) case M: if (fl) hipLaunchKernelGGL((gram_new_kernel<M, 1, true>), \
+    TILES_LAUNCH, ddom, coef(), fl, V, inew[0], -1, _rhs_p, _gram_part);     \
+    else hipLaunchKernelGGL((gram_new_kernel<M, 1, false>), TILES_LAUNCH,     \
+    ddom, coef(), fl, V, inew[0], -1, _rhs_p, _gram_part); break;
       GN(1) GN(2) GN(3) GN(4) GN(5) GN(6) GN(7) GN(8) GN(9)
 #undef GN
     }
   } else if (incr) {
     switch (m) {
-#define GN(M) case M: hipLaunchKernelGGL((gram_new_kernel<M, 2>), TILES_LAUNCH,\
-    ddom, coef(), fluid_mask(), V, inew[0], inew[1], _rhs_p, _gram_part); \
-    break;
+#define GN(M) case M: if (fl) hipLaunchKernelGGL((gram_new_kernel<M, 2, true>), \
+    TILES_LAUNCH, ddom, coef(), fl, V, inew[0], inew[1], _rhs_p,            \
+    _gram_part); else hipLaunchKernelGGL((gram_new_kernel<M, 2, false>),   \
+    TILES_LAUNCH, ddom, coef(), fl, V, inew[0], inew[1], _rhs_p,            \
+    _gram_part); break;
       GN(2) GN(3) GN(4) GN(5) GN(6) GN(7) GN(8) GN(9)
 #undef GN
     }
