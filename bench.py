@@ -136,6 +136,9 @@ def parse():
     ap.add_argument("--row-depth", type=int, default=None,
                     help="planes of loads in flight in the FP32 row kernel "
                          "(2, 3 or 4; default 2)")
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="mixed form: relative fall of the FP32 residual "
+                         "norm per inner sweep (pp_inner_tol, default 1e-3)")
     ap.add_argument("--pass-span", type=int, default=None,
                     help="previous timestep's solutions in the Galerkin "
                          "span of the history and learned forms: 1 of the "
@@ -202,6 +205,8 @@ def parse():
 
 def write_case(a, case, precision, warm):
     ext = {"pp_precision": precision, "pp_warmstart": warm}
+    if a.inner_tol is not None:
+        ext["pp_inner_tol"] = a.inner_tol
     if a.workload == "channel":
         kw = bbconfig.channel(n=a.n, precision=precision)
         kw["extensions"] = dict(kw.get("extensions") or {}, **ext)
