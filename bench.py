@@ -183,6 +183,10 @@ def parse():
                          "phi terms of two iterations at once")
     ap.add_argument("--batch", type=int, default=None,
                     help="PCG iterations enqueued per host check")
+    ap.add_argument("--predict", type=int, default=None,
+                    help="1: the first host check of a sweep after 0.8 of "
+                         "its length one timestep earlier (default); 0: "
+                         "every batch")
     ap.add_argument("--time-every", type=int, default=8,
                     help="HIP-event timing on one launch in this many of the "
                          "timed kernels (0: no event timing)")
@@ -485,6 +489,7 @@ def main():
                       ("bb_urow_depth", a.urow_depth),
                       ("bb_urow_fp64", a.urow_fp64),
                       ("bb_pcg_batch", a.batch),
+                      ("bb_pcg_predict", a.predict),
                       ("bb_warm_train_solves", a.train_solves)):
         if val is not None:
             ctypes.c_int.in_dll(lib, name).value = val

@@ -2046,6 +2046,35 @@ static double *_part_r = NULL;         /* tile partials of (r, z) */
 static double *_part_w1 = NULL, *_part_w2 = NULL;  /* learned-guess scale */
 static long buf_n = 0, part_n = 0;
 int bb_pcg_batch = 16;                 /* iterations enqueued per host check */
+int bb_pcg_predict = 1;                /* 1: the first check of a sweep comes
+                                          after 0.8 of the length the same
+                                          sweep (Lamb pass, correction) had
+                                          one timestep earlier */
+extern "C" int bb_lamb_pass;           /* host/driver.c */
+/* Length of each inner sweep by Lamb pass and correction (the plain form:
+ * correction 0), from the latest timestep that ran it. */
+#define SWEEP_PASSES 32
+#define SWEEP_OUTER 8
+static int sweep_len[2][SWEEP_PASSES][SWEEP_OUTER];
+
+/* Iterations to enqueue before the first check of a sweep. */
+static int first_batch(int form, int outer)
+{
+  const int p = bb_lamb_pass;
+  int n = bb_pcg_batch;
+  if (bb_pcg_predict && p >= 0 && p < SWEEP_PASSES && outer < SWEEP_OUTER) {
+    const int m = (int) (0.8 * sweep_len[form][p][outer]);
+    if (m > n) n = m;
+  }
+  return n;
+}
+
+static void record_sweep(int form, int outer, int len)
+{
+  const int p = bb_lamb_pass;
+  if (p >= 0 && p < SWEEP_PASSES && outer < SWEEP_OUTER)
+    sweep_len[form][p][outer] = len;
+}
 int bb_spmv_variant = 3;               /* 3: search kernel with one
                                           wavefront per x row where the grid
                                           allows it (else 2); 2: over
@@ -2836,7 +2865,7 @@ static void pcg_plain(double ts, int warm)
   }
   int q = 0;
   while (q <= pp_max_iter) {
-    int qend = q + bb_pcg_batch;
+    int qend = q + (q == 0 ? first_batch(0, 0) : bb_pcg_batch);
     if (qend > pp_max_iter + 1) qend = pp_max_iter + 1;
     for (int qq = q + 1; qq <= qend; qq++) enqueue_iteration(qq, K, T);
     hipLaunchKernelGGL(cg_check_kernel<false>, dim3(1), dim3(TILE_THREADS),
@@ -2845,6 +2874,7 @@ static void pcg_plain(double ts, int warm)
     fetch_state();
     if (cg_host->done) break;
   }
+  if (cg_host->done) record_sweep(0, 0, cg_host->q);
   /* a solve that stops right after a PHI_SKIP update: add its term */
   if (cg_host->done && cg_host->q >= 1 && phi_skips(cg_host->q)) {
     hipLaunchKernelGGL(phi_finish_kernel<double>, TILES_LAUNCH, ddom,
@@ -3103,8 +3133,9 @@ static void pcg_mixed(double ts, int warm)
   int q = 0, first_q = 1, nouter = 1;
   pair_base32 = 1;
   int done = cg_host->rz1 <= pp_residual*pp_residual*sp_rhs;
+  int sweep0 = 0;                       /* q at the start of the sweep */
   while (!done && q <= pp_max_iter && nouter <= pp_max_iter) {
-    int qend = q + bb_pcg_batch;
+    int qend = q + (q == sweep0 ? first_batch(1, nouter - 1) : bb_pcg_batch);
     if (qend > pp_max_iter + 1) qend = pp_max_iter + 1;
     for (int qq = q + 1; qq <= qend; qq++)
       enqueue_iteration32(qq, qq == first_q, K32, T32);
@@ -3116,6 +3147,8 @@ static void pcg_mixed(double ts, int warm)
       continue;
     }
     q = cg_host->q;
+    record_sweep(1, nouter - 1, q - sweep0);
+    sweep0 = q;
     /* e must be whole before the fold: add a skipped update's term */
     if (q >= pair_base32 && phi_skips32(q))
       hipLaunchKernelGGL(phi_finish_kernel<float>, TILES_LAUNCH, ddom,
