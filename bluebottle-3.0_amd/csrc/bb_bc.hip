@@ -145,6 +145,68 @@ extern "C" void cuda_exchange(real *array, int grid)
   bb_wrap(array, grid);
 }
 
+/* The periodic planes of u, v and w in one launch: blockIdx.z picks the
+ * (component, direction) pair, each pair copying as wrap_dirs_kernel does
+ * (the three arrays are disjoint). */
+struct WrapVel {
+  int n;
+  double *a[9];
+  Grid G[9];
+  int dir[9], stag[9];
+};
+
+__global__ void wrap_vel_kernel(WrapVel W)
+{
+  const int z = blockIdx.z;
+  double *a = W.a[z];
+  const Grid &G = W.G[z];
+  const int dir = W.dir[z], stag = W.stag[z];
+  int n1 = (dir == 0) ? G.nj : G.ni;
+  int n2 = (dir == 2) ? G.nj : G.nk;
+  int t1 = blockIdx.x * blockDim.x + threadIdx.x;
+  int t2 = blockIdx.y * blockDim.y + threadIdx.y;
+  if (t1 >= n1 || t2 >= n2) return;
+  long b = plane_base(G, dir, t1 + 1, t2 + 1);
+  long S = (dir == 0) ? 1 : (dir == 1 ? G.sy : G.sz);
+  int e = (dir == 0) ? G.ni : (dir == 1 ? G.nj : G.nk);
+  if (stag) {
+    a[b] = a[b + (long) (e - 1) * S];
+    a[b + (long) (e + 1) * S] = a[b + 2 * S];
+  } else {
+    a[b] = a[b + (long) e * S];
+    a[b + (long) (e + 1) * S] = a[b + S];
+  }
+}
+
+/* cuda_exchange of the three velocity components (on GFX, GFY, GFZ). */
+extern "C" void cuda_exchange_vel(real *u, real *v, real *w)
+{
+  int per[3] = {ddom.perx, ddom.pery, ddom.perz};
+  double *arr[3] = {u, v, w};
+  const int grids[3] = {GFX, GFY, GFZ};
+  WrapVel W;
+  W.n = 0;
+  unsigned gx = 0, gy = 0;
+  for (int c = 0; c < 3; c++)
+    for (int dir = 0; dir < 3; dir++) {
+      if (!per[dir]) continue;
+      const Grid &G = grid_of(grids[c]);
+      W.a[W.n] = arr[c];
+      W.G[W.n] = G;
+      W.dir[W.n] = dir;
+      W.stag[W.n] = c == dir;
+      W.n++;
+      dim3 gr, bl;
+      launch_plane(G, dir, gr, bl);
+      gx = gr.x > gx ? gr.x : gx;
+      gy = gr.y > gy ? gr.y : gy;
+    }
+  if (W.n == 0) return;
+  hipLaunchKernelGGL(wrap_vel_kernel, dim3(gx, gy, (unsigned) W.n),
+    dim3(64, 4), 0, bb_stream, W);
+  BB_LAUNCH_CHECK();
+}
+
 /* Kind of BC kernel for a velocity component on a face. */
 static int vel_kind(int type, int normal)
 {

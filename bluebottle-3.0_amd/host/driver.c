@@ -47,11 +47,11 @@ static double wall_seconds(void)
   return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 
+/* cuda_exchange of u, v, w (GFX, GFY, GFZ), in one launch */
+void cuda_exchange_vel(real *a, real *b, real *c);
 static void exchange_vel(real *a, real *b, real *c)
 {
-  cuda_exchange(a, GFX);
-  cuda_exchange(b, GFY);
-  cuda_exchange(c, GFZ);
+  cuda_exchange_vel(a, b, c);
 }
 
 static void pressure_solve(void)
