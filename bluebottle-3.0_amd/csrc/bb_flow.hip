@@ -395,8 +395,9 @@ __global__ void face_gather_kernel(const double *__restrict__ a, Grid G,
 }
 
 __global__ void face_add_kernel(double *a, Grid G, int dir, int loc,
-  double val, int subtract)
+  const double *__restrict__ valp, int subtract)
 {
+  const double val = *valp;
   int n1 = (dir == 0) ? G.nj : G.ni;
   int n2 = (dir == 2) ? G.nj : G.nk;
   int t1 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,14 +421,54 @@ static void face_sum(const double *a, const Grid &G, int dir, int loc,
   bb_reduce_rows(_bb_rows, (long) n1 * n2, _bb_scalars + slot);
 }
 
-static void face_add(double *a, const Grid &G, int dir, int loc, double val,
+/* a += (or -=) the value in device slot `slot` of _bb_scalars on one
+ * boundary face plane */
+static void face_add(double *a, const Grid &G, int dir, int loc, int slot,
   int subtract)
 {
   int n1 = (dir == 0) ? G.nj : G.ni;
   int n2 = (dir == 2) ? G.nj : G.nk;
   hipLaunchKernelGGL(face_add_kernel, dim3((n1 + 63) / 64, (n2 + 3) / 4),
-    dim3(64, 4), 0, bb_stream, a, G, dir, loc, val, subtract);
+    dim3(64, 4), 0, bb_stream, a, G, dir, loc,
+    (const double *) _bb_scalars + slot, subtract);
   BB_LAUNCH_CHECK();
+}
+
+/* The net boundary flux of the six face sums in slots 0-5 and the
+ * correction(s) it calls for, into slots 20 (one out-plane) or 20-22
+ * (HOMOGENEOUS: x, y, z), with the host's operations in the host's order
+ * (src/cuda_bluebottle.cu:2313-2492). */
+__global__ void solv_eps_kernel(double *s, double dx, double dy, double dz,
+  double xl, double yl, double zl, int plane)
+{
+  double eps_xs = s[0], eps_xe = s[1], eps_ys = s[2], eps_ye = s[3];
+  double eps_zs = s[4], eps_ze = s[5];
+  eps_xs *= dy * dz;
+  eps_xe *= dy * dz;
+  eps_ys *= dz * dx;
+  eps_ye *= dz * dx;
+  eps_zs *= dx * dy;
+  eps_ze *= dx * dy;
+  double eps[3];
+  eps[0] = eps_xe - eps_xs;
+  eps[1] = eps_ye - eps_ys;
+  eps[2] = eps_ze - eps_zs;
+  switch (plane) {
+    case WEST: case EAST:
+      s[20] = (eps[0] + eps[1] + eps[2])/(yl * zl);
+      break;
+    case SOUTH: case NORTH:
+      s[20] = (eps[0] + eps[1] + eps[2])/(zl * xl);
+      break;
+    case BOTTOM: case TOP:
+      s[20] = (eps[0] + eps[1] + eps[2])/(xl * yl);
+      break;
+    case HOMOGENEOUS:
+      s[20] = 0.5*eps[0]/(yl * zl);
+      s[21] = 0.5*eps[1]/(zl * xl);
+      s[22] = 0.5*eps[2]/(xl * yl);
+      break;
+  }
 }
 
 extern "C" void cuda_solvability(void)
@@ -441,59 +482,38 @@ extern "C" void cuda_solvability(void)
   face_sum(_v_star, ddom.fy, 1, ddom.fy.nj, 3);
   face_sum(_w_star, ddom.fz, 2, 1, 4);
   face_sum(_w_star, ddom.fz, 2, ddom.fz.nk, 5);
-  bb_fetch_scalars(6);
-  real eps_xs = bb_host_scalars[0], eps_xe = bb_host_scalars[1];
-  real eps_ys = bb_host_scalars[2], eps_ye = bb_host_scalars[3];
-  real eps_zs = bb_host_scalars[4], eps_ze = bb_host_scalars[5];
-  eps_xs *= d->dy * d->dz;
-  eps_xe *= d->dy * d->dz;
-  eps_ys *= d->dz * d->dx;
-  eps_ye *= d->dz * d->dx;
-  eps_zs *= d->dx * d->dy;
-  eps_ze *= d->dx * d->dy;
-  real eps[3];
-  eps[0] = eps_xe - eps_xs;
-  eps[1] = eps_ye - eps_ys;
-  eps[2] = eps_ze - eps_zs;
-  real sum;
+  /* the correction stays on the device: no host read */
+  hipLaunchKernelGGL(solv_eps_kernel, dim3(1), dim3(1), 0, bb_stream,
+    _bb_scalars, d->dx, d->dy, d->dz, DOM.xl, DOM.yl, DOM.zl, out_plane);
   switch (out_plane) {
     case WEST:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.yl * DOM.zl);
-      face_add(_u_star, ddom.fx, 0, 1, sum, 0);
+      face_add(_u_star, ddom.fx, 0, 1, 20, 0);
       break;
     case EAST:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.yl * DOM.zl);
-      face_add(_u_star, ddom.fx, 0, ddom.fx.ni, sum, 1);
+      face_add(_u_star, ddom.fx, 0, ddom.fx.ni, 20, 1);
       break;
     case SOUTH:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.zl * DOM.xl);
-      face_add(_v_star, ddom.fy, 1, 1, sum, 0);
+      face_add(_v_star, ddom.fy, 1, 1, 20, 0);
       break;
     case NORTH:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.zl * DOM.xl);
-      face_add(_v_star, ddom.fy, 1, ddom.fy.nj, sum, 1);
+      face_add(_v_star, ddom.fy, 1, ddom.fy.nj, 20, 1);
       break;
     case BOTTOM:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.xl * DOM.yl);
-      face_add(_w_star, ddom.fz, 2, 1, sum, 0);
+      face_add(_w_star, ddom.fz, 2, 1, 20, 0);
       break;
     case TOP:
-      sum = (eps[0] + eps[1] + eps[2])/(DOM.xl * DOM.yl);
-      face_add(_w_star, ddom.fz, 2, ddom.fz.nk, sum, 1);
+      face_add(_w_star, ddom.fz, 2, ddom.fz.nk, 20, 1);
       break;
-    case HOMOGENEOUS: {
-      real sum_x = 0.5*eps[0]/(DOM.yl * DOM.zl);
-      real sum_y = 0.5*eps[1]/(DOM.zl * DOM.xl);
-      real sum_z = 0.5*eps[2]/(DOM.xl * DOM.yl);
-      face_add(_u_star, ddom.fx, 0, 1, sum_x, 0);
-      face_add(_u_star, ddom.fx, 0, ddom.fx.ni, sum_x, 1);
-      face_add(_v_star, ddom.fy, 1, 1, sum_y, 0);
-      face_add(_v_star, ddom.fy, 1, ddom.fy.nj, sum_y, 1);
-      face_add(_w_star, ddom.fz, 2, 1, sum_z, 0);
-      face_add(_w_star, ddom.fz, 2, ddom.fz.nk, sum_z, 1);
+    case HOMOGENEOUS:
+      face_add(_u_star, ddom.fx, 0, 1, 20, 0);
+      face_add(_u_star, ddom.fx, 0, ddom.fx.ni, 20, 1);
+      face_add(_v_star, ddom.fy, 1, 1, 21, 0);
+      face_add(_v_star, ddom.fy, 1, ddom.fy.nj, 21, 1);
+      face_add(_w_star, ddom.fz, 2, 1, 22, 0);
+      face_add(_w_star, ddom.fz, 2, ddom.fz.nk, 22, 1);
       break;
-    }
   }
+  BB_LAUNCH_CHECK();
 }
 
 /* ----------------------------- projection ------------------------------- */
@@ -565,16 +585,21 @@ update_p_kernel(DevDom D, const double *__restrict__ p0,
   tile_partial(acc, gpart, tile);
 }
 
+/* p -= (sum / n) on interior cells, sum read from device memory (the mean
+ * formed as the host would form it: one division, then the subtraction
+ * as the addition of its negation). */
 __global__ void __launch_bounds__(256)
-add_const_kernel(DevDom D, double *__restrict__ p, double val)
+sub_mean_kernel(DevDom D, double *__restrict__ p,
+  const double *__restrict__ sum, double n)
 {
   const Grid &C = D.cc;
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   int k = blockIdx.z + 1;
   if (i > C.ni || j > C.nj) return;
+  const double pmean = *sum / n;
   long c = gidx(C, i, j, k);
-  p[c] += val;
+  p[c] += -pmean;
 }
 
 extern "C" void cuda_update_p(void)
@@ -586,10 +611,10 @@ extern "C" void cuda_update_p(void)
     bb_stream, D, _p0, _phi, _phase, _p, _bb_rows);
   BB_LAUNCH_CHECK();
   bb_reduce_rows(_bb_rows, n_tiles(D), _bb_scalars + 0);
-  real pmean = bb_fetch_scalar(0);
-  pmean /= (real) DOM.Gcc.s3;
-  hipLaunchKernelGGL(add_const_kernel, dim3((D.cc.ni + 63) / 64,
-    (D.cc.nj + 3) / 4, D.cc.nk), dim3(64, 4, 1), 0, bb_stream, D, _p, -pmean);
+  /* the mean stays on the device: no host read between the two kernels */
+  hipLaunchKernelGGL(sub_mean_kernel, dim3((D.cc.ni + 63) / 64,
+    (D.cc.nj + 3) / 4, D.cc.nk), dim3(64, 4, 1), 0, bb_stream, D, _p,
+    (const double *) _bb_scalars, (double) DOM.Gcc.s3);
   BB_LAUNCH_CHECK();
 }
 
