@@ -2050,7 +2050,20 @@ int bb_pcg_predict = 1;                /* 1: the first check of a sweep comes
                                           after 0.8 of the length the same
                                           sweep (Lamb pass, correction) had
                                           one timestep earlier */
-extern "C" int bb_lamb_pass;           /* host/driver.c */
+/* Index of the pressure solve within its timestep (the Lamb pass), counted
+ * here from the reference's step counter so that it holds under any host
+ * loop: set by solve_begin() at the start of every solve. */
+static int solve_pass = 0;
+static int solve_step = -1;
+static void solve_begin(void)
+{
+  if (stepnum != solve_step) {
+    solve_step = stepnum;
+    solve_pass = 0;
+  } else {
+    solve_pass++;
+  }
+}
 /* Length of each inner sweep by Lamb pass and correction (the plain form:
  * correction 0), from the latest timestep that ran it. */
 #define SWEEP_PASSES 32
@@ -2060,7 +2073,7 @@ static int sweep_len[2][SWEEP_PASSES][SWEEP_OUTER];
 /* Iterations to enqueue before the first check of a sweep. */
 static int first_batch(int form, int outer)
 {
-  const int p = bb_lamb_pass;
+  const int p = solve_pass;
   int n = bb_pcg_batch;
   if (bb_pcg_predict && p >= 0 && p < SWEEP_PASSES && outer < SWEEP_OUTER) {
     const int m = (int) (0.8 * sweep_len[form][p][outer]);
@@ -2071,7 +2084,7 @@ static int first_batch(int form, int outer)
 
 static void record_sweep(int form, int outer, int len)
 {
-  const int p = bb_lamb_pass;
+  const int p = solve_pass;
   if (p >= 0 && p < SWEEP_PASSES && outer < SWEEP_OUTER)
     sweep_len[form][p][outer] = len;
 }
@@ -2831,6 +2844,7 @@ static void enqueue_iteration(int q, const Coef &K, const InvMTab &T)
  * in _rhs_p with the tile partials of (b, b) in _part_p. */
 static void pcg_plain(double ts, int warm)
 {
+  solve_begin();
   Coef K = coef();
   InvMTab T = invm_table(K);
   const char *rname = "solver_expd.rec";
@@ -3089,6 +3103,7 @@ static void enqueue_iteration32(int q, bool first, const CoefT<float> &K,
  * reference; it replaces src/cuda_solver.cu:574-761 when selected. */
 static void pcg_mixed(double ts, int warm)
 {
+  solve_begin();
   Coef K = coef();
   InvMTab T = invm_table(K);
   CoefT<float> K32 = coef32(K);

@@ -62,7 +62,6 @@ static void pressure_solve(void)
 }
 
 int bb_lamb_iters = 0;
-int bb_lamb_pass = 0;                  /* Lamb pass of the solve in progress */
 real bb_lamb_err = 0.;
 int bb_scalar_iters = 0;               /* scalar iterations of the last step */
 real bb_scalar_err = 0.;
@@ -158,7 +157,6 @@ int bluebottle_step(void)
     cuda_dom_BC_star();
     exchange_vel(_u_star, _v_star, _w_star);
 
-    bb_lamb_pass = iter;
     pressure_solve();
     cuda_exchange(_phi, GCC);
     cuda_dom_BC_p(_phi);
