@@ -180,3 +180,40 @@ def test_mixed_search_settings_same_bits(case_dir):
             [st["niter"] for st in out[0]["stats"]]
         for f in FIELDS:
             assert np.array_equal(res["gpu"][f], out[0]["gpu"][f]), f
+
+
+@pytest.mark.parametrize("precision", ["PLAIN", "MIXED"])
+def test_predicted_checks_same_bits(case_dir, precision):
+    """The host's first check of a PCG sweep after 0.8 of the length the
+    same sweep had one timestep earlier (bb_pcg_predict) only moves the
+    points where the host reads the CG state: sedimentation 32^3 with two
+    spheres over three timesteps (several Lamb passes each) gives the same
+    iterations and the same bits with and without it."""
+    lib = bbhip.load()
+    kw, parts = bbconfig.sediment(n=(32, 32, 32), nparts=2, a=4.0, seed=11)
+    kw["extensions"] = dict(kw.get("extensions") or {},
+                            pp_precision=precision)
+    knob = ctypes.c_int.in_dll(lib, "bb_pcg_predict")
+    saved = knob.value
+    out = []
+    try:
+        for pred in (0, 1):
+            knob.value = pred
+            bbconfig.write_case(case_dir, **kw)
+            bbconfig.write_parts(case_dir, parts)
+            res = {"iters": [], "f": {}}
+            with bbhip.Simulation(case_dir, kw["n"]) as s:
+                for _ in range(3):
+                    s.step()
+                    res["iters"].append(s.pp_stats()["niter"])
+                s.pull()
+                for f in FIELDS:
+                    res["f"][f] = np.array(s.field(f))
+            out.append(res)
+    finally:
+        knob.value = saved
+    print("iterations (last solve of each step):", out[0]["iters"],
+          out[1]["iters"])
+    assert out[0]["iters"] == out[1]["iters"]
+    for f in FIELDS:
+        assert np.array_equal(out[0]["f"][f], out[1]["f"][f]), f
